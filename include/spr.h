@@ -1,0 +1,168 @@
+/* spr.h -- C ABI of the MI355X-native scanline path rasteriser (libspr.so).
+ *
+ * This is the drop-in boundary for the hot path of VkScanlinePR.  Every entry point below names
+ * the reference interface it replaces (paths relative to the reference root, src/ =
+ * VkScanlinePR/src/).  All functions return 0 on success and a negative spr_status otherwise;
+ * spr_last_error() gives the message.  No C++ exception crosses this boundary.
+ *
+ * Threading: one handle is used by one host thread at a time; different handles are
+ * independent and each owns its own HIP stream and device memory.
+ * Ownership: input arrays stay owned by the caller and are copied during the call; the handle
+ * owns every device buffer.  Caller data is never modified (the reference's loadVG mutates the
+ * container's colours, src/core/scanline/scanline_rasterizer.cpp:107-118; this ABI takes the
+ * already-packed RGBA8 fill_info instead).
+ */
+#ifndef SPR_H_
+#define SPR_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spr_handle spr_handle;
+typedef struct spr_scene spr_scene;
+
+typedef enum spr_status {
+    SPR_OK = 0,
+    SPR_ERR_INVALID = -1,  /* bad argument or malformed input */
+    SPR_ERR_DEVICE = -2,   /* HIP runtime error */
+    SPR_ERR_STATE = -3,    /* call out of order (e.g. render before load) */
+    SPR_ERR_IO = -4,       /* file could not be read or parsed */
+    SPR_ERR_CAPACITY = -5  /* a size exceeds a fixed limit of this build */
+} spr_status;
+
+/* Curve types and fill rules, src/core/vg/vg_container.h:9-19. */
+enum { SPR_CURVE_LINE = 2, SPR_CURVE_CUBIC = 4 };
+enum { SPR_FILL_NONZERO = 0, SPR_FILL_EVEN_ODD = 1 };
+
+/* Composite modes.  OVERWRITE is the reference's (blending disabled,
+ * src/core/scanline/scanline_rasterizer.cpp:893-895); SRC_OVER is the back-to-front alpha
+ * composite d = s*a + d*(1-a) in path order (new behaviour, no reference counterpart). */
+enum { SPR_COMPOSE_OVERWRITE = 0, SPR_COMPOSE_SRC_OVER = 1 };
+
+/* Allocation modes.  PLAIN is the reference's count-then-write form (count pass, host readback
+ * of the intersection total, write pass; scanline_rasterizer.cpp:317-395).  LEARNED replaces the
+ * count pass by per-path capacities predicted by the built-in MLP; paths whose prediction
+ * overflows are redone by the exact count pass, so the output is identical. */
+enum { SPR_MODE_PLAIN = 0, SPR_MODE_LEARNED = 1 };
+
+typedef struct spr_opts {
+    uint32_t compose;   /* SPR_COMPOSE_* */
+    uint32_t mode;      /* SPR_MODE_* */
+    int32_t device;     /* HIP device ordinal, -1 = current */
+    uint32_t debug;     /* 1 = keep every intermediate for spr_debug_export (slower) */
+    uint32_t reserved[4];
+} spr_opts;
+
+/* Per-frame counters and stage times, filled by spr_get_stats. */
+typedef struct spr_stats {
+    int64_t n_points, n_curves, n_paths;
+    int64_t n_intersections;   /* N: the reference's n_fragments */
+    int64_t n_cells;           /* merged cell records */
+    int64_t n_spans;           /* span records */
+    int64_t n_overflow_paths;  /* LEARNED: paths that took the exact fallback */
+    int64_t alloc_intersections; /* LEARNED: predicted capacity summed over paths */
+    double frame_ms;           /* device time of the last frame (HIP events) */
+    double stage_ms[16];       /* per-stage device time when timing is enabled */
+} spr_stats;
+
+/* Debug stage identifiers for spr_debug_export / spr_debug_inject. */
+enum {
+    SPR_STAGE_TPOS = 0,        /* float2 x P: transformed positions */
+    SPR_STAGE_VISIBLE = 1,     /* int32 x Np: per-path outcode mask */
+    SPR_STAGE_CUTS = 2,        /* float x 5C: cut cache (4 cuts + n_cuts bits) */
+    SPR_STAGE_COUNTS = 3,      /* int32 x (C+1): exclusive-scanned per-curve counts */
+    SPR_STAGE_INTER = 4,       /* int2 x N: (curve, t bits) intersections */
+    SPR_STAGE_KEY = 5,         /* int32 x N: cell keys in intersection order */
+    SPR_STAGE_WN = 6,          /* int32 x N: winding change in intersection order */
+    SPR_STAGE_SEG = 7,         /* int32 x (Np+1): per-path segment starts */
+    SPR_STAGE_SKEY = 8,        /* int32 x N: keys sorted inside each path segment */
+    SPR_STAGE_SIDX = 9,        /* int32 x N: sorted intersection indices */
+    SPR_STAGE_WINDING = 10,    /* int32 x N: exclusive-scanned winding numbers, sorted order */
+    SPR_STAGE_FLAGS = 11,      /* int32 x 2N: cell flags, then span flags */
+    SPR_STAGE_RECORDS = 12,    /* int4 x R: (yx, width, fill, frag_index) output records */
+    SPR_STAGE_IMAGE = 13,      /* uint8 x 4WH: RGBA8 image, top row first */
+    SPR_STAGE_COUNT_ = 14
+};
+
+/* --- lifecycle: VGRasterizer::initialize(void* window, uint32_t w, uint32_t h),
+ *     src/core/rasterizer.h:11, implemented at src/core/scanline/scanline_rasterizer.cpp:42-55.
+ *     No window: the target is an RGBA8 image of w x h in device memory. */
+int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle** out);
+/* ~ScanlineVGRasterizer, src/core/scanline/scanline_rasterizer.h:45-47 */
+int spr_destroy(spr_handle* h);
+const char* spr_last_error(const spr_handle* h);
+/* Message of the last failure of a call that had no handle (spr_create, spr_scene_*). */
+const char* spr_global_error(void);
+
+/* --- input: VGRasterizer::loadVG(shared_ptr<VGContainer>), src/core/rasterizer.h:15,
+ *     implemented at src/core/scanline/scanline_rasterizer.cpp:67-189.  Takes exactly the
+ *     flattened SoA that loadVG builds (src/core/scanline/vk_vg_data.h:14-35):
+ *     per point pos_xy[2P] and pos_path_idx[P]; per curve curve_pos_map[C] (first point),
+ *     curve_type[C] (2 = line, 4 = cubic) and curve_path_idx[C] (non-decreasing); per path
+ *     fill_rule[Np] and fill_info[Np] (RGBA8, R in the low byte). */
+int spr_load_paths(spr_handle* h, const float* pos_xy, const uint32_t* pos_path_idx,
+                   uint32_t n_points, const uint32_t* curve_pos_map, const uint32_t* curve_type,
+                   const uint32_t* curve_path_idx, uint32_t n_curves, const uint32_t* fill_rule,
+                   const uint32_t* fill_info, uint32_t n_paths);
+/* RVG::load + loadVG in one call: src/core/vg/rvg.cpp:9-26, src/app/vg_app.cpp:172-192. */
+int spr_load_rvg(spr_handle* h, const char* path);
+/* Same, from a scene object (parsed or generated by the spr_scene_* calls below). */
+int spr_load_scene(spr_handle* h, const spr_scene* s);
+
+/* --- VGRasterizer::setMVP(const glm::mat4&), src/core/rasterizer.h:17, stored as the
+ *     TransPosIn rows m0..m3 (src/core/scanline/compute_ubo.h:9-13,
+ *     scanline_rasterizer.cpp:191-197): m_rows[4*r + c], x' = dot((x,y,0,1), row 0). */
+int spr_set_mvp(spr_handle* h, const float m_rows[16]);
+
+/* --- VGRasterizer::render(), src/core/rasterizer.h:13 -> drawFrame,
+ *     src/core/scanline/scanline_rasterizer.cpp:282-696.  Renders one frame into the handle's
+ *     device image; copies it to rgba_out (4*w*h bytes, top row first) when not NULL. */
+int spr_render_frame(spr_handle* h, uint8_t* rgba_out_or_null);
+/* Device pointer of the handle's RGBA8 image (valid until the next resize or destroy). */
+int spr_image_device_ptr(spr_handle* h, void** dptr);
+/* Blocks until the handle's stream is idle. */
+int spr_synchronize(spr_handle* h);
+/* Enables per-stage HIP-event timing (adds event records between stages). */
+int spr_set_stage_timing(spr_handle* h, int enable);
+int spr_get_stats(const spr_handle* h, spr_stats* out);
+
+/* --- debug access to intermediate buffers (the reference's drawDebug dumps,
+ *     src/core/scanline/scanline_rasterizer.cpp:698-802).  export copies the last frame's stage
+ *     into dst (cap bytes) and stores the full size in *nbytes; with dst == NULL only the size
+ *     is returned.  inject(SPR_STAGE_INTER) makes the following frames use the given
+ *     intersections (int2 x N, in curve order) instead of generating them, so the integer
+ *     stages can be compared on identical intersection inputs; inject with size 0 clears it. */
+int spr_debug_export(spr_handle* h, int stage, void* dst, size_t cap, size_t* nbytes);
+int spr_debug_inject(spr_handle* h, int stage, const void* src, size_t nbytes);
+
+/* --- scenes: the VGContainer of src/core/vg/vg_container.h:21-87 plus the flattening of
+ *     scanline_rasterizer.cpp:67-134, and the synthetic workload generators. */
+typedef struct spr_scene_view {
+    const float* pos_xy; const uint32_t* pos_path_idx; uint32_t n_points;
+    const uint32_t* curve_pos_map; const uint32_t* curve_type; const uint32_t* curve_path_idx;
+    uint32_t n_curves;
+    const uint32_t* fill_rule; const uint32_t* fill_info; uint32_t n_paths;
+    uint32_t width, height;  /* suggested canvas (the RVG viewport, or the generator's) */
+} spr_scene_view;
+
+/* RVG::load, src/core/vg/rvg.cpp:9-255 (M/L/C/Z, solid rgb/rgba paint, nzfill/ofill). */
+int spr_scene_load_rvg(const char* path, spr_scene** out);
+/* Synthetic workloads.  kind: 0 = random cubic scribbles (64 paths, 1024^2 by default),
+ * 1 = glyph page (quadratic glyphs elevated to cubics, even-odd), 2 = illustration (mixed
+ * rules, alpha, per-path affine).  n_paths / canvas = 0 select the defaults. */
+int spr_scene_generate(int kind, uint64_t seed, uint32_t n_paths, uint32_t canvas, spr_scene** out);
+/* Paths [first, first+count) of a scene as a new scene (path indices renumbered). */
+int spr_scene_subset(const spr_scene* s, uint32_t first, uint32_t count, spr_scene** out);
+int spr_scene_view_get(const spr_scene* s, spr_scene_view* out);
+int spr_scene_write_rvg(const spr_scene* s, const char* path);
+void spr_scene_free(spr_scene* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPR_H_ */

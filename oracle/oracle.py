@@ -1,0 +1,107 @@
+"""Python access to the CPU oracle (oracle/spr_oracle.cpp -> oracle/build/libspr_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, always as the checker; never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "build" / "libspr_oracle.so"
+STAGES = {
+    "tpos": 0, "visible": 1, "cuts": 2, "counts": 3, "inter": 4, "key": 5, "wn": 6, "seg": 7,
+    "skey": 8, "sidx": 9, "winding": 10, "flags": 11, "records": 12, "image": 13, "path": 14,
+}
+IDENTITY = np.eye(4, dtype=np.float32)
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} missing; run `python -m vkscanlinepr_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        vp, u32p, f32p, i32p = C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int32)
+        L.oracle_run.argtypes = [f32p, u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, u32p, u32p,
+                                 C.c_uint32, f32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int64]
+        L.oracle_run.restype = vp
+        L.oracle_get.argtypes = [vp, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
+        L.oracle_free.argtypes = [vp]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_max_threads.restype = C.c_int
+        L.oracle_solve_quad.argtypes = [C.c_float, C.c_float, C.c_float, f32p, f32p]
+        L.oracle_curve_cuts.argtypes = [C.c_uint32, f32p, f32p, C.c_int, f32p]
+        L.oracle_curve_cuts.restype = C.c_int
+        L.oracle_eval.argtypes = [C.c_uint32, C.c_float, f32p]
+        L.oracle_eval.restype = C.c_float
+        L.oracle_solve_crossing.argtypes = [C.c_uint32, f32p, f32p, C.c_int, C.c_float, C.c_float, C.c_float]
+        L.oracle_solve_crossing.restype = C.c_float
+        L.oracle_path_visible.argtypes = [C.c_int32]
+        L.oracle_float2int_rd.argtypes = [C.c_float]
+        L.oracle_float2int_rd.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class Frame:
+    """One oracle frame: every stage's result as numpy arrays."""
+
+    def __init__(self, arrays: dict, width: int, height: int, mvp=None, compose: int = 0,
+                 inter_override=None, threads: int = 1):
+        L = lib()
+        L.oracle_set_threads(int(threads))
+        self._keep = [np.ascontiguousarray(arrays["pos"], dtype=np.float32).reshape(-1),
+                      _u32(arrays["pos_path"]), _u32(arrays["curve_pos"]),
+                      _u32(arrays["curve_type"]), _u32(arrays["curve_path"]),
+                      _u32(arrays["rule"]), _u32(arrays["fill"]),
+                      np.ascontiguousarray(IDENTITY if mvp is None else mvp, dtype=np.float32).reshape(16)]
+        pos, pp, cpos, ctype, cpath, rule, fill, m = self._keep
+        P, Cn, Np = len(pp), len(ctype), len(rule)
+        ov = None
+        n_ov = 0
+        if inter_override is not None:
+            ov = np.ascontiguousarray(inter_override, dtype=np.int32).reshape(-1)
+            self._keep.append(ov)
+            n_ov = len(ov) // 2
+        f = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        self._ctx = L.oracle_run(f(pos, C.c_float), f(pp, C.c_uint32), P, f(cpos, C.c_uint32),
+                                 f(ctype, C.c_uint32), f(cpath, C.c_uint32), Cn, f(rule, C.c_uint32),
+                                 f(fill, C.c_uint32), Np, f(m, C.c_float), int(width), int(height),
+                                 int(compose), f(ov, C.c_int32) if ov is not None else None, n_ov)
+        self.width, self.height = int(width), int(height)
+
+    def get(self, stage: str) -> np.ndarray:
+        p = C.c_void_p()
+        n = C.c_int64()
+        rc = lib().oracle_get(self._ctx, STAGES[stage], C.byref(p), C.byref(n))
+        if rc != 0:
+            raise KeyError(stage)
+        dt = np.uint8 if stage == "image" else (np.float32 if stage in ("tpos", "cuts") else np.int32)
+        cnt = n.value // np.dtype(dt).itemsize
+        if cnt == 0:
+            return np.zeros(0, dtype=dt)
+        buf = (C.c_char * n.value).from_address(p.value)
+        return np.frombuffer(buf, dtype=dt, count=cnt).copy()
+
+    def image(self) -> np.ndarray:
+        return self.get("image").reshape(self.height, self.width, 4)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_ctx", None):
+                lib().oracle_free(self._ctx)
+                self._ctx = None
+        except Exception:
+            pass
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())

@@ -1,0 +1,277 @@
+"""vkscanlinepr_amd -- MI355X-native scanline path rasteriser.
+
+Python mirror of the C ABI in include/spr.h (libspr.so, hand-written gfx950 HIP kernels).  The
+classes follow the reference's plugin interface VGRasterizer (VkScanlinePR/src/core/rasterizer.h:
+9-20): ``Rasterizer(w, h)`` ~ initialize, ``load_scene``/``load_paths``/``load_rvg`` ~ loadVG,
+``set_mvp`` ~ setMVP, ``render`` ~ render.  Errors raise ``SprError`` (the reference throws
+std::runtime_error).  There is no CPU fallback: if libspr.so is missing, loading fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+__all__ = ["SprError", "Scene", "Rasterizer", "lib", "STAGES", "COMPOSE_OVERWRITE",
+           "COMPOSE_SRC_OVER", "MODE_PLAIN", "MODE_LEARNED"]
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libspr.so"
+
+COMPOSE_OVERWRITE, COMPOSE_SRC_OVER = 0, 1
+MODE_PLAIN, MODE_LEARNED = 0, 1
+STAGES = {
+    "tpos": 0, "visible": 1, "cuts": 2, "counts": 3, "inter": 4, "key": 5, "wn": 6, "seg": 7,
+    "skey": 8, "sidx": 9, "winding": 10, "flags": 11, "records": 12, "image": 13,
+}
+_STAGE_DTYPE = {
+    "tpos": np.float32, "visible": np.int32, "cuts": np.float32, "counts": np.int32,
+    "inter": np.int32, "key": np.int32, "wn": np.int32, "seg": np.int32, "skey": np.int32,
+    "sidx": np.int32, "winding": np.int32, "flags": np.int32, "records": np.int32,
+    "image": np.uint8,
+}
+
+
+class SprError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"spr error {code}: {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [("compose", C.c_uint32), ("mode", C.c_uint32), ("device", C.c_int32),
+                ("debug", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_points", C.c_int64), ("n_curves", C.c_int64), ("n_paths", C.c_int64),
+                ("n_intersections", C.c_int64), ("n_cells", C.c_int64), ("n_spans", C.c_int64),
+                ("n_overflow_paths", C.c_int64), ("alloc_intersections", C.c_int64),
+                ("frame_ms", C.c_double), ("stage_ms", C.c_double * 16)]
+
+
+class SceneView(C.Structure):
+    _fields_ = [("pos_xy", C.POINTER(C.c_float)), ("pos_path_idx", C.POINTER(C.c_uint32)),
+                ("n_points", C.c_uint32),
+                ("curve_pos_map", C.POINTER(C.c_uint32)), ("curve_type", C.POINTER(C.c_uint32)),
+                ("curve_path_idx", C.POINTER(C.c_uint32)), ("n_curves", C.c_uint32),
+                ("fill_rule", C.POINTER(C.c_uint32)), ("fill_info", C.POINTER(C.c_uint32)),
+                ("n_paths", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+_lib = None
+EXPORTS = [
+    "spr_create", "spr_destroy", "spr_last_error", "spr_global_error", "spr_load_paths",
+    "spr_load_rvg", "spr_load_scene", "spr_set_mvp", "spr_render_frame", "spr_image_device_ptr",
+    "spr_synchronize", "spr_set_stage_timing", "spr_get_stats", "spr_debug_export",
+    "spr_debug_inject", "spr_scene_load_rvg", "spr_scene_generate", "spr_scene_subset",
+    "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free",
+]
+
+
+def lib() -> C.CDLL:
+    """Loads libspr.so (raises if it was not built: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise FileNotFoundError(f"{LIB_PATH} is missing; run `python -m vkscanlinepr_amd.build`")
+    L = C.CDLL(str(LIB_PATH))
+    vp, u32p, f32p = C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float)
+    L.spr_create.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(Opts), C.POINTER(vp)]
+    L.spr_destroy.argtypes = [vp]
+    L.spr_last_error.argtypes = [vp]
+    L.spr_last_error.restype = C.c_char_p
+    L.spr_global_error.restype = C.c_char_p
+    L.spr_load_paths.argtypes = [vp, f32p, u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, u32p,
+                                 u32p, C.c_uint32]
+    L.spr_load_rvg.argtypes = [vp, C.c_char_p]
+    L.spr_load_scene.argtypes = [vp, vp]
+    L.spr_set_mvp.argtypes = [vp, f32p]
+    L.spr_render_frame.argtypes = [vp, C.c_void_p]
+    L.spr_image_device_ptr.argtypes = [vp, C.POINTER(vp)]
+    L.spr_synchronize.argtypes = [vp]
+    L.spr_set_stage_timing.argtypes = [vp, C.c_int]
+    L.spr_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.spr_debug_export.argtypes = [vp, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.spr_debug_inject.argtypes = [vp, C.c_int, C.c_void_p, C.c_size_t]
+    L.spr_scene_load_rvg.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.spr_scene_generate.argtypes = [C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(vp)]
+    L.spr_scene_subset.argtypes = [vp, C.c_uint32, C.c_uint32, C.POINTER(vp)]
+    L.spr_scene_view_get.argtypes = [vp, C.POINTER(SceneView)]
+    L.spr_scene_write_rvg.argtypes = [vp, C.c_char_p]
+    L.spr_scene_free.argtypes = [vp]
+    L.spr_scene_free.restype = None
+    _lib = L
+    return L
+
+
+def _check(rc: int, handle=None) -> None:
+    if rc != 0:
+        L = lib()
+        msg = (L.spr_last_error(handle) if handle else L.spr_global_error()) or b""
+        raise SprError(rc, msg.decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Scene:
+    """A flattened vector scene (the SoA that loadVG uploads) held by the library."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        self._arrays = None
+
+    @classmethod
+    def generate(cls, kind: int, seed: int, n_paths: int = 0, canvas: int = 0) -> "Scene":
+        out = C.c_void_p()
+        _check(lib().spr_scene_generate(kind, seed, n_paths, canvas, C.byref(out)))
+        return cls(out.value)
+
+    @classmethod
+    def load_rvg(cls, path: str | Path) -> "Scene":
+        out = C.c_void_p()
+        _check(lib().spr_scene_load_rvg(str(path).encode(), C.byref(out)))
+        return cls(out.value)
+
+    def subset(self, first: int, count: int) -> "Scene":
+        out = C.c_void_p()
+        _check(lib().spr_scene_subset(self._h, first, count, C.byref(out)))
+        return Scene(out.value)
+
+    def write_rvg(self, path: str | Path) -> None:
+        _check(lib().spr_scene_write_rvg(self._h, str(path).encode()))
+
+    def arrays(self) -> dict:
+        """Copies of the SoA arrays (numpy), plus width/height."""
+        if self._arrays is None:
+            v = SceneView()
+            _check(lib().spr_scene_view_get(self._h, C.byref(v)))
+
+            def cp(p, n, dt):
+                if n == 0:
+                    return np.zeros(0, dtype=dt)
+                return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+
+            self._arrays = dict(
+                pos=cp(v.pos_xy, 2 * v.n_points, np.float32).reshape(-1, 2),
+                pos_path=cp(v.pos_path_idx, v.n_points, np.uint32),
+                curve_pos=cp(v.curve_pos_map, v.n_curves, np.uint32),
+                curve_type=cp(v.curve_type, v.n_curves, np.uint32),
+                curve_path=cp(v.curve_path_idx, v.n_curves, np.uint32),
+                rule=cp(v.fill_rule, v.n_paths, np.uint32),
+                fill=cp(v.fill_info, v.n_paths, np.uint32),
+                width=int(v.width), height=int(v.height),
+            )
+        return self._arrays
+
+    @property
+    def n_paths(self) -> int:
+        return len(self.arrays()["rule"])
+
+    @property
+    def n_curves(self) -> int:
+        return len(self.arrays()["curve_type"])
+
+    @property
+    def n_points(self) -> int:
+        return len(self.arrays()["pos_path"])
+
+    def __del__(self):
+        try:
+            if self._h and self._h.value:
+                lib().spr_scene_free(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Rasterizer:
+    """One rasteriser handle: a w x h RGBA8 target on one GPU, with its own HIP stream."""
+
+    def __init__(self, width: int, height: int, compose: int = COMPOSE_OVERWRITE,
+                 mode: int = MODE_PLAIN, device: int = -1, debug: bool = False):
+        self.width, self.height = int(width), int(height)
+        o = Opts(compose=compose, mode=mode, device=device, debug=1 if debug else 0)
+        out = C.c_void_p()
+        _check(lib().spr_create(self.width, self.height, C.byref(o), C.byref(out)))
+        self._h = out
+
+    def _c(self, rc):
+        _check(rc, self._h)
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().spr_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_scene(self, scene: Scene) -> None:
+        self._c(lib().spr_load_scene(self._h, scene._h))
+
+    def load_rvg(self, path: str | Path) -> None:
+        self._c(lib().spr_load_rvg(self._h, str(path).encode()))
+
+    def load_paths(self, pos, pos_path, curve_pos, curve_type, curve_path, rule, fill) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+        arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in
+                (pos_path, curve_pos, curve_type, curve_path, rule, fill)]
+        pp, cp_, ct, cpa, ru, fi = arrs
+        self._c(lib().spr_load_paths(self._h, _ptr(pos, C.c_float), _ptr(pp, C.c_uint32), len(pp),
+                                     _ptr(cp_, C.c_uint32), _ptr(ct, C.c_uint32),
+                                     _ptr(cpa, C.c_uint32), len(ct), _ptr(ru, C.c_uint32),
+                                     _ptr(fi, C.c_uint32), len(ru)))
+
+    def set_mvp(self, m_rows) -> None:
+        m = np.ascontiguousarray(m_rows, dtype=np.float32).reshape(16)
+        self._c(lib().spr_set_mvp(self._h, _ptr(m, C.c_float)))
+
+    def render(self, readback: bool = True):
+        if readback:
+            img = np.empty((self.height, self.width, 4), dtype=np.uint8)
+            self._c(lib().spr_render_frame(self._h, img.ctypes.data_as(C.c_void_p)))
+            return img
+        self._c(lib().spr_render_frame(self._h, None))
+        return None
+
+    def synchronize(self) -> None:
+        self._c(lib().spr_synchronize(self._h))
+
+    def set_stage_timing(self, on: bool) -> None:
+        self._c(lib().spr_set_stage_timing(self._h, 1 if on else 0))
+
+    def stats(self) -> Stats:
+        s = Stats()
+        self._c(lib().spr_get_stats(self._h, C.byref(s)))
+        return s
+
+    def image_device_ptr(self) -> int:
+        p = C.c_void_p()
+        self._c(lib().spr_image_device_ptr(self._h, C.byref(p)))
+        return p.value
+
+    def export(self, stage: str) -> np.ndarray:
+        sid = STAGES[stage]
+        n = C.c_size_t()
+        self._c(lib().spr_debug_export(self._h, sid, None, 0, C.byref(n)))
+        dt = _STAGE_DTYPE[stage]
+        out = np.empty(n.value // np.dtype(dt).itemsize, dtype=dt)
+        self._c(lib().spr_debug_export(self._h, sid, out.ctypes.data_as(C.c_void_p), n.value,
+                                       C.byref(n)))
+        return out
+
+    def inject_intersections(self, inter) -> None:
+        if inter is None:
+            self._c(lib().spr_debug_inject(self._h, STAGES["inter"], None, 0))
+            return
+        a = np.ascontiguousarray(inter, dtype=np.int32).reshape(-1)
+        self._c(lib().spr_debug_inject(self._h, STAGES["inter"], a.ctypes.data_as(C.c_void_p),
+                                       a.nbytes))

@@ -1,0 +1,220 @@
+// device_common.hpp -- arithmetic shared by the gfx950 kernels.
+//
+// Every floating-point expression here keeps the operation order of the reference shaders
+// (the library is built with -ffp-contract=off, IEEE division and square root), so that the
+// integer results derived from it (counts, cells, winding) match the reference algorithm bit
+// for bit.  GLSL min/max follow the GLSL definition (max(x,y) = x<y ? y : x), conversions to
+// int saturate, and integer products wrap.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spr {
+
+constexpr int kFrag = 2;           // FRAG_SIZE (make_intersection_0.comp:3)
+constexpr int kCubicIters = 24;    // CUBIC_ITERATION_NUMBER (make_intersection_1.comp:7)
+constexpr uint32_t kLine = 2u, kCubic = 4u;
+constexpr uint32_t kSentinelKey = 0xFFFEFFFEu;  // off-canvas cell key (gen_fragment.comp:111)
+constexpr uint32_t kTwoBits = 0x40000000u;      // bit pattern of 2.0f: an exhausted chain
+
+__host__ __device__ inline uint32_t fbits(float f) { return __builtin_bit_cast(uint32_t, f); }
+__host__ __device__ inline float bitsf(uint32_t u) { return __builtin_bit_cast(float, u); }
+__host__ __device__ inline float glmax(float x, float y) { return x < y ? y : x; }
+__host__ __device__ inline float glmin(float x, float y) { return y < x ? y : x; }
+__host__ __device__ inline int32_t sat_f2i(float x) {
+    if (!(x == x)) return 0;
+    if (x >= 2147483648.0f) return 0x7FFFFFFF;
+    if (x <= -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)x;
+}
+__host__ __device__ inline int32_t wmul(int32_t a, int32_t b) { return (int32_t)((uint32_t)a * (uint32_t)b); }
+__host__ __device__ inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+// float2int_rd (make_intersection_1.comp:76-79)
+__host__ __device__ inline int32_t f2i_rd(float x) { return x >= 0.0f ? sat_f2i(x) : sat_f2i(x - 1.0f); }
+__host__ __device__ inline float lerp(float a, float b, float t) { return a + t * (b - a); }
+
+// PATH_VISIBLE (make_intersection_0.comp:14-20): a path is culled when all its points lie
+// beyond one canvas edge.
+__host__ __device__ inline bool visible_mask(int32_t m) {
+    return !(((m & 0x11111000) == 0) || ((m & 0x01101011) == 0) || ((m & 0x00011111) == 0) ||
+             ((m & 0x11010110) == 0));
+}
+
+// Outcode bit of one transformed point (transform_pos.comp:57-82).
+__host__ __device__ inline int32_t outcode_bits(float x, float y, float fw, float fh) {
+    int xf = x < 0 ? 0 : (x < fw ? 1 : 2);
+    int yf = y < 0 ? 0 : (y < fh ? 1 : 2);
+    switch ((yf << 4) | xf) {
+        case 0x00: return 0x10000000;
+        case 0x01: return 0x01000000;
+        case 0x02: return 0x00100000;
+        case 0x10: return 0x00010000;
+        case 0x11: return 0x10000001;
+        case 0x12: return 0x00001000;
+        case 0x20: return 0x00000100;
+        case 0x21: return 0x00000010;
+        case 0x22: return 0x00000001;
+        default: return 0;
+    }
+}
+
+// One axis of a line or cubic at t (interpolateGeneralCurve, make_intersection_1.comp:117-148).
+__host__ __device__ inline float eval_axis(bool cubic, float t, float c0, float c1, float c2, float c3) {
+    if (!cubic) return lerp(c0, c1, t);
+    float q0 = lerp(c0, c1, t), q1 = lerp(c1, c2, t), q2 = lerp(c2, c3, t);
+    float l0 = lerp(q0, q1, t), l1 = lerp(q1, q2, t);
+    return lerp(l0, l1, t);
+}
+
+struct CurvePts {
+    float x[4], y[4];
+    bool cubic;
+    __host__ __device__ inline float ex(float t) const { return eval_axis(cubic, t, x[0], x[1], x[2], x[3]); }
+    __host__ __device__ inline float ey(float t) const { return eval_axis(cubic, t, y[0], y[1], y[2], y[3]); }
+};
+
+__device__ inline CurvePts load_curve(const float2* __restrict__ tpos, uint32_t base, uint32_t type) {
+    CurvePts c;
+    c.cubic = (type == kCubic);
+    float2 p0 = tpos[base], p1 = tpos[base + 1];
+    c.x[0] = p0.x; c.y[0] = p0.y; c.x[1] = p1.x; c.y[1] = p1.y;
+    if (c.cubic) {
+        float2 p2 = tpos[base + 2], p3 = tpos[base + 3];
+        c.x[2] = p2.x; c.y[2] = p2.y; c.x[3] = p3.x; c.y[3] = p3.y;
+    } else {
+        c.x[2] = c.x[3] = c.y[2] = c.y[3] = 0.0f;
+    }
+    return c;
+}
+
+// Roots of a*t^2 + b*t + c in citardauq form (solveQuadEquation, make_intersection_0.comp:76-129).
+__host__ __device__ inline void solve_quad(float a, float b, float c, float& r0, float& r1) {
+    if (a == 0.0f) {
+        float x = -c / b;
+        r0 = x; r1 = x;
+        return;
+    }
+    float B = b * 0.5f;
+    float R = B * B - a * c;
+    if (R > 0.0f) {
+        float SR = sqrtf(R);
+        if (B > 0.0f) {
+            float TB = B + SR;
+            r0 = -c / TB; r1 = -TB / a;
+        } else {
+            float TB = -B + SR;
+            r0 = TB / a; r1 = c / TB;
+        }
+    } else {
+        r0 = 0.0f; r1 = 0.0f;
+    }
+}
+
+// Interior monotone cuts of a curve, ordered as the reference orders them (including its
+// 4-cut insertion that never moves the fourth cut, make_intersection_0.comp:338-341).  Slots
+// at or beyond the returned count are 0.
+__host__ __device__ inline uint32_t monotone_cuts(const CurvePts& cv, bool visible, float q[4]) {
+    q[0] = q[1] = q[2] = q[3] = 0.0f;
+    if (!visible || !cv.cubic) return 0;
+    uint32_t n = 0;
+    float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int axis = 0; axis < 2; ++axis) {
+        const float* p = axis == 0 ? cv.x : cv.y;
+        float r0, r1;
+        solve_quad(3.0f * (p[1] - p[2]) + (p[3] - p[0]), 2.0f * ((p[0] - p[1]) + (p[2] - p[1])), p[1] - p[0], r0, r1);
+        if (r0 > 0.0f && r0 < 1.0f) t[n++] = r0;
+        if (r1 > 0.0f && r1 < 1.0f && r1 != r0) t[n++] = r1;
+    }
+    float q0 = t[0], q1 = t[1], q2 = t[2], q3 = t[3];
+    if (n >= 2 && q1 < q0) { float s = q0; q0 = q1; q1 = s; }
+    if (n >= 3) {
+        float t2 = q2, t1 = q1;
+        if (t2 < t1) {
+            q2 = t1;
+            float t0 = q0;
+            if (t2 < t0) { q1 = t0; q0 = t2; } else { q1 = t2; }
+        }
+    }
+    q[0] = q0; q[1] = q1; q[2] = q2; q[3] = q3;
+    return n;
+}
+
+// udpate_cut_range (make_intersection_0.comp:186-221): clamps the grid range of one axis to the
+// canvas (+1 cell) and returns how many grid lines it crosses.
+__host__ __device__ inline int clamp_axis(int lim, int& b, int& e) {
+    const int mx = (lim & ~1) + kFrag;
+    if ((b < 0 && e < 0) || (b > mx && e > mx) || (b > e)) return 0;
+    b = b < 0 ? 0 : (b > mx ? mx : b);
+    e = e < 0 ? 0 : (e > mx ? mx : e);
+    int n = (e - b) / kFrag + 1;
+    return n > 0 ? n : 0;
+}
+
+// Grid-line count of one monotone piece (count pass, floor based: make_intersection_0.comp:167-183).
+__host__ __device__ inline int piece_count(float p0x, float p0y, float p1x, float p1y, int w, int h) {
+    auto g = [](float v) { return sat_f2i(floorf(v / (float)kFrag) * (float)kFrag); };
+    int xb, xe, yb, ye;
+    if (p0x <= p1x) { xb = wadd(g(p0x), kFrag); xe = g(p1x); } else { xb = wadd(g(p1x), kFrag); xe = g(p0x); }
+    if (p0y <= p1y) { yb = wadd(g(p0y), kFrag); ye = g(p1y); } else { yb = wadd(g(p1y), kFrag); ye = g(p0y); }
+    return 1 + clamp_axis(w, xb, xe) + clamp_axis(h, yb, ye);
+}
+
+// Geometry of one monotone piece for the write pass (get_xy_begin_end_delta + udpate_cut_range,
+// make_intersection_1.comp:150-213, 278-302).
+struct PieceGrid {
+    int nx, ny;
+    float x0, y0;  // first grid line crossed on each axis
+    float dx, dy;  // +-2: direction of travel
+};
+
+__host__ __device__ inline PieceGrid piece_grid(float p0x, float p0y, float p1x, float p1y, int w, int h) {
+    auto g = [](float v) { return wmul(f2i_rd(v / (float)kFrag), kFrag); };
+    PieceGrid G;
+    int xb, xe, yb, ye;
+    if (p0x <= p1x) { xb = wadd(g(p0x), kFrag); xe = g(p1x); G.dx = (float)kFrag; }
+    else { xb = wadd(g(p1x), kFrag); xe = g(p0x); G.dx = -(float)kFrag; }
+    if (p0y <= p1y) { yb = wadd(g(p0y), kFrag); ye = g(p1y); G.dy = (float)kFrag; }
+    else { yb = wadd(g(p1y), kFrag); ye = g(p0y); G.dy = -(float)kFrag; }
+    G.nx = clamp_axis(w, xb, xe);
+    G.ny = clamp_axis(h, yb, ye);
+    G.x0 = (float)(G.dx < 0 ? xe : xb);
+    G.y0 = (float)(G.dy < 0 ? ye : yb);
+    return G;
+}
+
+// End-of-piece parameter with its side flag (make_intersection_1.comp:271-276): low bits 2 when
+// the end point's x is integral, 3 otherwise.
+__host__ __device__ inline float flag_piece_end(float t_raw, float p1x) {
+    uint32_t b = fbits(t_raw) & 0xFFFFFFFCu;
+    return (floorf(p1x) == p1x) ? bitsf(b | 2u) : bitsf(b | 3u);
+}
+
+// One grid-crossing solve (make_intersection_1.comp:377-437): a line is solved by its
+// reciprocal slope, a cubic by 24 bisection steps on [t_min, t1] keyed on the sign of B(t)-c.
+__device__ inline float solve_crossing(const CurvePts& cv, int side, float c, float t_min, float t1) {
+    const float* p = side == 0 ? cv.x : cv.y;
+    if (!cv.cubic) {
+        float a = p[1] - p[0];
+        a = (a != 0.0f ? (1.0f / a) : 0.0f);
+        return glmin(glmax((c - p[0]) * a, t_min), t1);
+    }
+    const float c0 = p[0], c1 = p[1], c2 = p[2], c3 = p[3];
+    float t0 = t_min;
+    float vt0 = eval_axis(true, t0, c0, c1, c2, c3);
+    if (vt0 == c) return t0;
+    // The sign of vt0-c never changes (vt0 is only replaced by a value of equal sign), so the
+    // comparison is against the initial sign bit.
+    const uint32_t s0 = fbits(vt0 - c) & 0x80000000u;
+    float tm = t0, vtm = vt0;
+#pragma unroll 4
+    for (int j = 0; j < kCubicIters; ++j) {
+        tm = (t0 + t1) * 0.5f;
+        vtm = eval_axis(true, tm, c0, c1, c2, c3);
+        if ((fbits(vtm - c) & 0x80000000u) == s0) t0 = tm; else t1 = tm;
+    }
+    return fabsf(vtm - c) > 1.0f ? t_min : tm;
+}
+
+}  // namespace spr

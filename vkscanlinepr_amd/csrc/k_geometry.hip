@@ -1,0 +1,348 @@
+// k_geometry.hip -- transform + cull, monotonisation + count, the chain-parallel intersection
+// write, and fragment-key generation (gfx950).
+//
+// The intersection write (reference: make_intersection_1.comp:217-447, one thread walking a
+// whole curve) is split along its true dependency structure.  Inside one monotone piece the
+// reference interleaves x-grid and y-grid crossings, but every bisection for the next crossing
+// on one axis starts from the previous crossing of the SAME axis (the t_min it starts from is
+// that axis' pending value, :338-359, :402).  So each (piece, axis) is an independent chain:
+// one thread per chain computes the exact same t values, and a second pass merges the two
+// chains of a piece in the reference's order (ties to x, :338) with its equal-t OR-merge
+// (:361-375).  Results are bit-identical while the parallelism grows from one thread per curve
+// to one thread per (curve, piece, axis).
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace spr {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+    int64_t g = (n + block - 1) / block;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// transform_pos.comp:33-85.  The per-path outcode OR is reduced inside the wave first (points of
+// one path are contiguous), then one atomicOr per (wave, path).
+__global__ __launch_bounds__(kBlock) void k_transform(const float2* __restrict__ pos,
+                                                      const uint32_t* __restrict__ pos_path,
+                                                      uint32_t P, Mvp m, float fw, float fh,
+                                                      float2* __restrict__ tpos,
+                                                      int32_t* __restrict__ visible) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < P;
+    int32_t bits = 0;
+    uint32_t path = 0xFFFFFFFFu;
+    if (valid) {
+        float2 p = pos[i];
+        float o[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            o[r] = ((p.x * m.m[4 * r] + p.y * m.m[4 * r + 1]) + 0.0f * m.m[4 * r + 2]) + 1.0f * m.m[4 * r + 3];
+        float ox = o[0] / o[3], oy = o[1] / o[3];
+        tpos[i] = make_float2(ox, oy);
+        bits = outcode_bits(ox, oy, fw, fh);
+        path = pos_path[i];
+    }
+    unsigned long long todo = __ballot(valid && bits != 0);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const uint32_t lp = __shfl(path, leader);
+        const bool match = valid && bits != 0 && path == lp;
+        todo &= ~__ballot(match);
+        int32_t v = match ? bits : 0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o);
+        if ((int)(threadIdx.x & 63) == leader) atomicOr(&visible[lp], v);
+    }
+}
+
+// make_intersection_0.comp:226-411: monotone cuts, cut cache, per-curve intersection count.
+__global__ __launch_bounds__(kBlock) void k_count(const float2* __restrict__ tpos,
+                                                  const uint32_t* __restrict__ cpos,
+                                                  const uint32_t* __restrict__ ctype,
+                                                  const uint32_t* __restrict__ cpath,
+                                                  const int32_t* __restrict__ visible, uint32_t C,
+                                                  int w, int h, float4* __restrict__ cuts,
+                                                  uint32_t* __restrict__ ncuts,
+                                                  int32_t* __restrict__ counts) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= C) return;
+    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
+    const bool vis = visible_mask(visible[cpath[c]]);
+    float q[4];
+    const uint32_t n = monotone_cuts(cv, vis, q);
+    cuts[c] = make_float4(q[0], q[1], q[2], q[3]);
+    ncuts[c] = n;
+    int32_t cnt = 0;
+    if (vis) {
+        float p0x = cv.x[0], p0y = cv.y[0];
+#pragma unroll
+        for (uint32_t k = 0; k < 5; ++k) {
+            if (k > n) break;
+            const float t = k < n ? q[k] : 1.0f;
+            const float p1x = cv.ex(t), p1y = cv.ey(t);
+            cnt += piece_count(p0x, p0y, p1x, p1y, w, h);
+            p0x = p1x; p0y = p1y;
+        }
+    }
+    counts[c] = cnt;
+}
+
+// Per-path segment starts: seg[p] = entries of all curves of paths < p (the sort segments of
+// gen_fragment.comp:227-244, derived from the curve offsets instead of entry boundaries).
+__global__ void k_path_segments(const uint32_t* __restrict__ cpath, uint32_t C,
+                                const int32_t* __restrict__ offsets, uint32_t Np,
+                                int32_t* __restrict__ seg) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t p = cpath[c];
+    const uint32_t lo = c == 0 ? 0u : cpath[c - 1] + 1u;
+    const int32_t off = offsets[c];
+    for (uint32_t j = lo; j <= p; ++j) seg[j] = off;
+    if (c == C - 1) {
+        const int32_t total = offsets[C];
+        for (uint32_t j = p + 1; j <= Np; ++j) seg[j] = total;
+    }
+}
+
+__global__ void k_segment_stats(const int32_t* __restrict__ seg, uint32_t Np, int32_t* __restrict__ max_len) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    int32_t len = p < Np ? seg[p + 1] - seg[p] : 0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) len = max(len, __shfl_xor(len, o));
+    if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(max_len, len);
+}
+
+// Walks the monotone pieces of a visible curve up to `piece` and returns that piece's context:
+// its flagged end parameters, grid, and first entry offset inside the curve's output range.
+struct PieceCtx {
+    float t0, t1;
+    PieceGrid G;
+    int off;
+};
+
+__device__ inline bool setup_piece(const CurvePts& cv, const float q[4], uint32_t n, int piece,
+                                   int w, int h, PieceCtx& pc) {
+    if ((uint32_t)piece > n) return false;
+    float t0 = 0.0f, p0x = cv.x[0], p0y = cv.y[0];
+    int off = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const float traw = (uint32_t)k < n ? q[k] : 1.0f;
+        const float p1x = cv.ex(traw), p1y = cv.ey(traw);
+        const float t1 = flag_piece_end(traw, p1x);
+        const PieceGrid G = piece_grid(p0x, p0y, p1x, p1y, w, h);
+        if (k == piece) {
+            pc.t0 = t0; pc.t1 = t1; pc.G = G; pc.off = off;
+            return true;
+        }
+        off += 1 + G.nx + G.ny;
+        t0 = t1; p0x = p1x; p0y = p1y;
+    }
+    return false;
+}
+
+// One thread per (curve, piece, axis) chain: the crossings of one axis in travel order, each
+// bisection bracketed from the previous crossing of the same axis.  Stores the masked t with
+// the axis in bit 0 at entry slots 1..nx (x) and nx+1..nx+ny (y) of the piece.
+__global__ __launch_bounds__(kBlock) void k_chain_solve(
+    const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
+    const uint32_t* __restrict__ ctype, const uint32_t* __restrict__ cpath,
+    const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
+    const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
+    int h, float* __restrict__ chain_t) {
+    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t c = gid / 10u, slot = gid % 10u;
+    if (c >= C) return;
+    if (!visible_mask(visible[cpath[c]])) return;
+    const int piece = (int)(slot >> 1), side = (int)(slot & 1u);
+    const uint32_t n = ncuts[c];
+    if ((uint32_t)piece > n) return;
+    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
+    const float4 qq = cuts[c];
+    const float q[4] = {qq.x, qq.y, qq.z, qq.w};
+    PieceCtx pc;
+    if (!setup_piece(cv, q, n, piece, w, h, pc)) return;
+    const int cnt = side ? pc.G.ny : pc.G.nx;
+    if (cnt == 0) return;
+    const float start = side ? pc.G.y0 : pc.G.x0;
+    const float d = side ? pc.G.dy : pc.G.dx;
+    const int first = offsets[c] + pc.off + 1 + (side ? pc.G.nx : 0);
+    const int lim = offsets[c + 1];  // never write past this curve's range
+    float* out = chain_t + first;
+    float tprev = pc.t0;
+    float line = start;
+    for (int k = 0; k < cnt; ++k) {
+        const float ts = solve_crossing(cv, side, line, tprev, pc.t1);
+        const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | (uint32_t)side);
+        if (first + k < lim) out[k] = v;
+        tprev = v;
+        line += d;
+    }
+}
+
+// One thread per (curve, piece): merges the piece start and the two chains in the reference's
+// order and applies its equal-t OR-merge (make_intersection_1.comp:301-375).
+__global__ __launch_bounds__(kBlock) void k_piece_merge(
+    const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
+    const uint32_t* __restrict__ ctype, const uint32_t* __restrict__ cpath,
+    const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
+    const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
+    int h, const float* __restrict__ chain_t, int2* __restrict__ inter) {
+    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t c = gid / 5u;
+    const int piece = (int)(gid % 5u);
+    if (c >= C) return;
+    if (!visible_mask(visible[cpath[c]])) return;
+    const uint32_t n = ncuts[c];
+    if ((uint32_t)piece > n) return;
+    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
+    const float4 qq = cuts[c];
+    const float q[4] = {qq.x, qq.y, qq.z, qq.w};
+    PieceCtx pc;
+    if (!setup_piece(cv, q, n, piece, w, h, pc)) return;
+    const int nx = pc.G.nx, ny = pc.G.ny;
+    const int base = offsets[c] + pc.off;
+    if (base + nx + ny + 1 > offsets[c + 1]) return;  // count and write passes disagree: skip
+    const float* X = chain_t + base + 1;
+    const float* Y = X + nx;
+    const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
+    float tx = pc.t0, ty = pc.t0;
+    int xi = 0, yi = 0;
+    if (nx == 0) tx = kX2;
+    else if (ny == 0) ty = kY2;
+    else { tx = X[0]; xi = 1; }
+    const int n_loop = nx + ny + 1;
+    uint32_t last = fbits(-1.0f);
+    int2* out = inter + base;
+    for (int i = 0; i < n_loop; ++i) {
+        float tmin;
+        if (tx <= ty) { tmin = tx; tx = xi < nx ? X[xi++] : kX2; }
+        else { tmin = ty; ty = yi < ny ? Y[yi++] : kY2; }
+        uint32_t o = fbits(tmin);
+        if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
+            o |= last;
+            out[i - 1] = make_int2((int)c, (int)o);
+        }
+        out[i] = make_int2((int)c, (int)o);
+        last = o;
+    }
+}
+
+// gen_fragment.comp:90-224: cell key of the segment between consecutive intersections of one
+// curve, and its winding change across the cell's centre row.
+__global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict__ inter, int64_t N,
+                                                         const uint32_t* __restrict__ cpos,
+                                                         const uint32_t* __restrict__ ctype,
+                                                         const uint32_t* __restrict__ cpath,
+                                                         const float2* __restrict__ tpos, int w,
+                                                         int h, int32_t* __restrict__ key,
+                                                         int8_t* __restrict__ wn,
+                                                         uint32_t* __restrict__ epath) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= N) return;
+    const int2 a = inter[i];
+    const int2 b = (i + 1 != N) ? inter[i + 1] : make_int2(-1, 0x3f800000);
+    float t0 = glmax(bitsf((uint32_t)a.y & 0xFFFFFFFCu), 0.0f);
+    float t1 = glmax(bitsf((uint32_t)b.y & 0xFFFFFFFCu), 0.0f);
+    const uint32_t c = (uint32_t)a.x;
+    const uint32_t path = cpath[c];
+    uint32_t yx = kSentinelKey;
+    if (a.x != b.x) t1 = 1.0f;
+    int swn = 0;
+    if (t0 < t1) {
+        const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
+        const float pfx = cv.ex(t0), pfy = cv.ey(t0), plx = cv.ex(t1), ply = cv.ey(t1);
+        const int raw_x = wmul(f2i_rd(((pfx + plx) * 0.5f) / (float)kFrag), kFrag);
+        const int raw_y = wmul(f2i_rd(((pfy + ply) * 0.5f) / (float)kFrag), kFrag);
+        const int pos_x = min(max(raw_x, -kFrag), (w & ~1) + kFrag);
+        const int pos_y = min(max(raw_y, -kFrag), (h & ~1) + kFrag);
+        const uint32_t y_shift = (uint32_t)(pos_y + 0x7FFF) << 16;
+        const uint32_t x_shift = (uint32_t)(pos_x + 0x7FFF) & 0xFFFFu;
+        if ((uint32_t)raw_y < (uint32_t)h) yx = y_shift | x_shift;
+        const float fwy = (float)(pos_y + 1);
+        if (pfy == ply) swn = 0;
+        else if (pfy < fwy && fwy <= ply) swn = -1;
+        else if (ply < fwy && fwy <= pfy) swn = 1;
+    }
+    key[i] = (int32_t)yx;
+    wn[i] = (int8_t)swn;
+    epath[i] = path;
+}
+
+}  // namespace
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n + kBlock - 1) / kBlock;
+    if (g > 4096) g = 4096;
+    k_fill_u32<<<(unsigned)g, kBlock, 0, s>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
+                            int w, int h, float2* tpos, int32_t* visible, hipStream_t s) {
+    if (P == 0) return hipSuccess;
+    k_transform<<<grid_for(P), kBlock, 0, s>>>(pos, pos_path, P, m, (float)w, (float)h, tpos, visible);
+    return hipGetLastError();
+}
+
+hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                        const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
+                        float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    k_count<<<grid_for(C), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, C, w, h, cuts, ncuts, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_path_segments(const uint32_t* cpath, uint32_t C, const int32_t* offsets,
+                                uint32_t Np, int32_t* seg, hipStream_t s) {
+    if (C == 0) return launch_fill_u32((uint32_t*)seg, (int64_t)Np + 1, 0u, s);
+    k_path_segments<<<grid_for(C), kBlock, 0, s>>>(cpath, C, offsets, Np, seg);
+    return hipGetLastError();
+}
+
+hipError_t launch_segment_stats(const int32_t* seg, uint32_t Np, int32_t* max_len, hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_segment_stats<<<grid_for(Np), kBlock, 0, s>>>(seg, Np, max_len);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
+                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w,
+                              int h, float* chain_t, hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    k_chain_solve<<<grid_for((int64_t)C * 10), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts,
+                                                               ncuts, offsets, C, w, h, chain_t);
+    return hipGetLastError();
+}
+
+hipError_t launch_piece_merge(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
+                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w,
+                              int h, const float* chain_t, int2* inter, hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    k_piece_merge<<<grid_for((int64_t)C * 5), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts,
+                                                              ncuts, offsets, C, w, h, chain_t, inter);
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
+                               const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
+                               int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
+                               hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    k_gen_fragment<<<grid_for(N), kBlock, 0, s>>>(inter, N, cpos, ctype, cpath, tpos, w, h, key, wn, epath);
+    return hipGetLastError();
+}
+
+}  // namespace spr

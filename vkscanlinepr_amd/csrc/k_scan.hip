@@ -1,0 +1,199 @@
+// k_scan.hip -- single-pass device-wide scans (decoupled look-back) for gfx950.
+//
+// k_scan_i32 replaces the reference's single-workgroup naive_scan (naive_scan.comp:27-73): an
+// exclusive prefix sum with the total at out[n].  Each 256-thread block owns a 4096-element
+// tile read and written once (16 contiguous int32 per thread, 4 x 16-byte accesses), so the
+// kernel moves 8 bytes per element and is bounded by HBM bandwidth.
+//
+// k_mark_emit fuses mark_merged_fragment_and_span.comp:48-120, the 2N-flag scan of
+// scanline_rasterizer.cpp:544-573 and gen_merged_fragment_and_span.comp:152-222 into one pass:
+// the cell/span flags of a tile are counted, the (cells, spans) pair is scanned with one
+// look-back, and the records are written at their final positions.  R is left on the device,
+// which removes the reference's third host synchronisation.
+#include "kernels.hpp"
+#include "scan_common.hpp"
+
+namespace spr {
+
+namespace {
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, int32_t* out, int64_t n,
+                                                           unsigned long long* __restrict__ status,
+                                                           uint32_t* __restrict__ counter) {
+    __shared__ int32_t s_tile;
+    __shared__ int32_t s_w[kWaves];
+    __shared__ int32_t s_prefix;
+    const int64_t tile = ordered_tile_id(counter, &s_tile);
+    const int64_t base = tile * (int64_t)kScanTile;
+    if (base >= n && tile != 0) return;
+    const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    const bool vec = (i0 + kScanItems <= n) && ((((uintptr_t)(in + i0)) & 15u) == 0) &&
+                     ((((uintptr_t)(out + i0)) & 15u) == 0);
+    if (vec) {
+        const int4* p = reinterpret_cast<const int4*>(in + i0);
+#pragma unroll
+        for (int k = 0; k < kScanItems / 4; ++k) {
+            int4 q = p[k];
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) v[j] = (i0 + j < n) ? in[i0 + j] : 0;
+    }
+    int32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) { int32_t x = v[j]; v[j] = run; run = (int32_t)((uint32_t)run + (uint32_t)x); }
+    int32_t total;
+    const int32_t texcl = block_exclusive_add(run, s_w, total);
+    if (threadIdx.x < 64) {
+        const int32_t pre = lookback<OpI32>(status, tile, total);
+        if (threadIdx.x == 0) s_prefix = pre;
+    }
+    __syncthreads();
+    const int32_t off = (int32_t)((uint32_t)s_prefix + (uint32_t)texcl);
+    if (vec) {
+        int4* p = reinterpret_cast<int4*>(out + i0);
+#pragma unroll
+        for (int k = 0; k < kScanItems / 4; ++k)
+            p[k] = make_int4(off + v[4 * k], off + v[4 * k + 1], off + v[4 * k + 2], off + v[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j)
+            if (i0 + j < n) out[i0 + j] = (int32_t)((uint32_t)off + (uint32_t)v[j]);
+    }
+    const bool last_tile = (n == 0) ? (tile == 0) : (base <= n - 1 && n - 1 < base + kScanTile);
+    if (last_tile && threadIdx.x == 0) out[n] = (int32_t)((uint32_t)s_prefix + (uint32_t)total);
+}
+
+// (cells, spans) packed as two 31-bit counts under a 2-bit state.
+struct OpPair {
+    using T = unsigned long long;  // cells in bits 0..31, spans in bits 32..63
+    __device__ static T zero() { return 0ull; }
+    __device__ static T add(T a, T b) { return a + b; }
+    __device__ static uint64_t encode(uint64_t st, T v) {
+        return (st << 62) | ((v >> 32) << 31) | (v & 0x7FFFFFFFull);
+    }
+    __device__ static uint64_t state(uint64_t w) { return w >> 62; }
+    __device__ static T value(uint64_t w) { return (w & 0x7FFFFFFFull) | (((w >> 31) & 0x7FFFFFFFull) << 32); }
+};
+
+__device__ inline int key_x(int32_t k) { return (k & 0xFFFF) - 0x7FFF; }
+__device__ inline int key_y(int32_t k) { return ((k >> 16) & 0xFFFF) - 0x7FFF; }
+
+__global__ __launch_bounds__(kScanThreads) void k_mark_emit(
+    const int32_t* __restrict__ skey, const int32_t* __restrict__ winding,
+    const uint32_t* __restrict__ epath, const uint32_t* __restrict__ rule,
+    const uint32_t* __restrict__ fill, int64_t N, int w, int h,
+    unsigned long long* __restrict__ status, uint32_t* __restrict__ counter,
+    int4* __restrict__ records, uint32_t* __restrict__ rec_path, int32_t* __restrict__ flags,
+    int32_t* __restrict__ totals) {
+    __shared__ int32_t s_tile;
+    __shared__ unsigned long long s_w[kWaves];
+    __shared__ unsigned long long s_prefix;
+    const int64_t tile = ordered_tile_id(counter, &s_tile);
+    const int64_t base = tile * (int64_t)kScanTile;
+    if (base >= N) return;
+    const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+    int32_t kprev = 0;
+    uint32_t pprev = 0;
+    if (i0 > 0 && i0 <= N) { kprev = skey[i0 - 1]; pprev = epath[i0 - 1] & 0x3FFFFFFFu; }
+    int32_t keys[kScanItems];
+    uint32_t paths[kScanItems];
+    uint32_t cellbits = 0, spanbits = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const int64_t f = i0 + j;
+        keys[j] = 0; paths[j] = 0;
+        if (f >= N) continue;
+        const int32_t k1 = skey[f];
+        const uint32_t p1 = epath[f] & 0x3FFFFFFFu;
+        keys[j] = k1; paths[j] = p1;
+        const int x1 = key_x(k1), y1 = key_y(k1);
+        const bool in_canvas = !(x1 < 0 || y1 < 0 || x1 >= w || y1 >= h);
+        bool cell, span = false;
+        if (f == 0) {
+            cell = in_canvas;
+        } else {
+            const uint32_t r = rule[p1];
+            const int x0 = key_x(kprev), y0 = key_y(kprev);
+            cell = in_canvas && (pprev != p1 || kprev != k1);
+            const int32_t wn = winding[f];
+            const bool wf = (r == 0u && wn != 0) || (r == 1u && (wn & 1) != 0);
+            span = (y0 == y1) && (x0 + 2 < x1) && (pprev == p1) && wf;
+        }
+        cellbits |= (uint32_t)cell << j;
+        spanbits |= (uint32_t)span << j;
+        kprev = k1; pprev = p1;
+    }
+    const unsigned long long mine = (unsigned long long)__popc(cellbits) | ((unsigned long long)__popc(spanbits) << 32);
+    unsigned long long total;
+    const unsigned long long texcl = block_exclusive_add(mine, s_w, total);
+    if (threadIdx.x < 64) {
+        const unsigned long long pre = lookback<OpPair>(status, tile, total);
+        if (threadIdx.x == 0) s_prefix = pre;
+    }
+    __syncthreads();
+    const unsigned long long off = s_prefix + texcl;
+    int32_t cells_before = (int32_t)(off & 0xFFFFFFFFull);
+    int32_t spans_before = (int32_t)(off >> 32);
+    if (i0 < N) {
+        int32_t kp = (i0 > 0) ? skey[i0 - 1] : 0;
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int64_t f = i0 + j;
+            if (f >= N) break;
+            const int cell = (cellbits >> j) & 1, span = (spanbits >> j) & 1;
+            const int32_t k1 = keys[j];
+            const uint32_t p1 = paths[j];
+            if (flags) { flags[f] = cell; flags[N + f] = span; }
+            if (cell | span) {
+                const int32_t colour = (int32_t)fill[p1];
+                const int32_t o = cells_before + spans_before;
+                if (cell) {
+                    const int x1 = key_x(k1), y1 = key_y(k1);
+                    records[o] = make_int4((int32_t)((uint32_t)y1 << 16) | x1, 2, colour, cells_before + 1);
+                    rec_path[o] = p1;
+                }
+                if (span) {
+                    const int y0 = key_y(kp);
+                    int x0 = key_x(kp) + 2;
+                    const int x1 = key_x(k1);
+                    x0 = max(0, x0);
+                    records[o + cell] = make_int4((int32_t)((uint32_t)y0 << 16) | x0, x1 - x0, colour, 0);
+                    rec_path[o + cell] = p1;
+                }
+            }
+            cells_before += cell;
+            spans_before += span;
+            kp = k1;
+        }
+    }
+    if (threadIdx.x == 0 && base <= N - 1 && N - 1 < base + kScanTile) {
+        const unsigned long long t = s_prefix + total;
+        totals[0] = (int32_t)(t & 0xFFFFFFFFull);
+        totals[1] = (int32_t)(t >> 32);
+        totals[2] = (int32_t)((t & 0xFFFFFFFFull) + (t >> 32));
+    }
+}
+
+}  // namespace
+
+hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s) {
+    const int64_t tiles = n > 0 ? scan_tiles(n) : 1;
+    k_scan_i32<<<(unsigned)tiles, kScanThreads, 0, s>>>(in, out, n, ctl.status, ctl.counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,
+                            const uint32_t* rule, const uint32_t* fill, int64_t N, int w, int h,
+                            ScanCtl ctl, int4* records, uint32_t* rec_path, int32_t* flags,
+                            int32_t* totals, hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    k_mark_emit<<<(unsigned)scan_tiles(N), kScanThreads, 0, s>>>(skey, winding, epath, rule, fill, N, w, h,
+                                                                 ctl.status, ctl.counter, records, rec_path,
+                                                                 flags, totals);
+    return hipGetLastError();
+}
+
+}  // namespace spr

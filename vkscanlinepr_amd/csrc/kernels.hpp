@@ -1,0 +1,86 @@
+// kernels.hpp -- host-side launch wrappers of the gfx950 kernels.  Every wrapper enqueues on
+// the given stream and returns hipGetLastError() of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spr {
+
+struct Mvp { float m[16]; };
+
+// Scan control: a tile-status array tagged with an epoch so it never needs clearing, and an
+// atomic counter that hands out tile ids in dispatch order.
+struct ScanCtl {
+    unsigned long long* status;  // >= tiles entries
+    uint32_t* counter;           // one u32, zero before the first use of the frame
+    uint32_t epoch;              // distinct per scan invocation inside a frame sequence
+};
+constexpr int kScanTile = 4096;  // elements per scan tile (256 threads x 16)
+inline int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
+
+// ---- control
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+
+// ---- geometry (a3, a4, a7, a8)
+hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
+                            int w, int h, float2* tpos, int32_t* visible, hipStream_t s);
+hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                        const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
+                        float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s);
+hipError_t launch_path_segments(const uint32_t* cpath, uint32_t C, const int32_t* offsets,
+                                uint32_t Np, int32_t* seg, hipStream_t s);
+hipError_t launch_segment_stats(const int32_t* seg, uint32_t Np, int32_t* max_len, hipStream_t s);
+hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
+                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w,
+                              int h, float* chain_t, hipStream_t s);
+hipError_t launch_piece_merge(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
+                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w,
+                              int h, const float* chain_t, int2* inter, hipStream_t s);
+hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
+                               const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
+                               int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
+                               hipStream_t s);
+
+// ---- scans
+hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s);
+
+// ---- segmented sort (a9 + a10): per path segment, ascending (key ^ 0x80000000, index); the
+// winding change rides along in the low bits of the 64-bit sort word.
+constexpr int kSortTile = 4096;
+hipError_t launch_tile_counts(const int32_t* seg, uint32_t Np, int32_t* tile_counts, hipStream_t s);
+hipError_t launch_sort_tiles(const int32_t* key, const int8_t* wn, const int32_t* seg,
+                             const int32_t* tile_prefix, uint32_t Np, int64_t max_tiles,
+                             unsigned long long* out_final, unsigned long long* out_big,
+                             hipStream_t s);
+hipError_t launch_merge_pass(const int32_t* seg, const int32_t* tile_prefix, uint32_t Np,
+                             int64_t max_tiles, int pass, const unsigned long long* src,
+                             unsigned long long* dst, unsigned long long* out_final,
+                             hipStream_t s);
+hipError_t launch_unpack_sorted(const unsigned long long* sorted, int64_t N, int32_t* skey,
+                                int32_t* sidx, int32_t* swn, hipStream_t s);
+
+// ---- winding / mark / emit (a10 scan, a11)
+hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,
+                            const uint32_t* rule, const uint32_t* fill, int64_t N, int w, int h,
+                            ScanCtl ctl, int4* records, uint32_t* rec_path, int32_t* flags,
+                            int32_t* totals, hipStream_t s);
+
+// ---- composite (a12)
+hipError_t launch_run_flags(const int4* records, const uint32_t* rec_path, const int32_t* totals,
+                            int64_t max_records, int32_t* run_flag, hipStream_t s);
+hipError_t launch_run_emit(const int4* records, const int32_t* run_scan, const int32_t* totals,
+                           int64_t max_records, int2* runs, hipStream_t s);
+hipError_t launch_row_count(const int2* runs, const int32_t* run_total, int64_t max_runs,
+                            int32_t* row_count, hipStream_t s);
+hipError_t launch_row_scatter(const int2* runs, const int32_t* run_total, int64_t max_runs,
+                              const int32_t* row_start, int32_t* row_fill, int32_t* row_runs,
+                              hipStream_t s);
+hipError_t launch_composite(const int4* records, const int2* runs, const int32_t* run_total,
+                            const int32_t* totals, const int32_t* row_start,
+                            const int32_t* row_runs, int w, int h, int compose, uint8_t* image,
+                            hipStream_t s);
+
+}  // namespace spr

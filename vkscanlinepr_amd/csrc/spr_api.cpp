@@ -1,0 +1,692 @@
+// spr_api.cpp -- the C ABI of include/spr.h: handle lifecycle, device buffers, and the per-frame
+// orchestration of the gfx950 kernels on one HIP stream.
+//
+// Frame (plain two-pass form, the drop-in for drawFrame, scanline_rasterizer.cpp:282-696):
+//   transform+cull -> monotonise+count -> scan(counts) -> path segments
+//   -> HOST SYNC: read the intersection total N (the reference's readback, :355-356) and grow
+//      buffers -> chain solve + piece merge (intersections) -> fragment keys -> tile sort +
+//      merge passes -> winding scan -> fused mark/scan/emit (R stays on the device; the
+//      reference's second and third host syncs are gone) -> runs by cell row -> composite.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/spr.h"
+#include "kernels.hpp"
+#include "learned.hpp"
+#include "scene.hpp"
+
+struct spr_scene {
+    spr::Scene s;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+#define SPR_HIP(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            throw Fail{SPR_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)};      \
+    } while (0)
+
+// Grow-only device allocation.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+    void ensure(size_t n) {
+        if (n <= cap && p) return;
+        size_t want = std::max<size_t>(n, 1);
+        if (cap) want = std::max(want, cap + cap / 4);
+        if (p) {
+            // the old buffer may still be read by queued work
+            SPR_HIP(hipDeviceSynchronize());
+            (void)hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        void* q = nullptr;
+        SPR_HIP(hipMalloc(&q, want * sizeof(T)));
+        p = (T*)q;
+        cap = want;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+};
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+struct spr_handle {
+    uint32_t w = 0, h = 0;
+    spr_opts opts{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    std::vector<hipEvent_t> ev_stage;
+    bool stage_timing = false;
+    std::string err;
+
+    // inputs
+    bool loaded = false;
+    uint32_t P = 0, C = 0, Np = 0;
+    DBuf<float2> pos;
+    DBuf<uint32_t> pos_path, cpos, ctype, cpath, rule, fill;
+    std::vector<uint32_t> h_cpath;
+    spr::Mvp mvp{};
+
+    // frame buffers
+    DBuf<float2> tpos;
+    DBuf<int32_t> visible;
+    DBuf<float4> cuts;
+    DBuf<uint32_t> ncuts;
+    DBuf<int32_t> counts, seg, tile_counts;
+    DBuf<int2> inter;
+    DBuf<float> chain_t;
+    DBuf<int32_t> key;
+    DBuf<int8_t> wn;
+    DBuf<uint32_t> epath;
+    DBuf<unsigned long long> sorted, sbig, stmp;
+    DBuf<int32_t> skey, sidx, swn, winding, flags;
+    DBuf<int4> records;
+    DBuf<uint32_t> rec_path;
+    DBuf<int32_t> run_flag, row_start, row_fill, row_runs;
+    DBuf<int2> runs;
+    DBuf<uint8_t> image;
+    DBuf<unsigned long long> zero;  // per-frame zeroed control region
+    int32_t* h_pinned = nullptr;    // 16 ints of pinned host memory
+
+    // injection
+    std::vector<int32_t> inj_inter;
+    bool inj = false;
+
+    // learned allocation
+    spr::LearnedState learned;
+
+    // last frame
+    int64_t N = 0;
+    int32_t max_len = 0;
+    spr_stats stats{};
+    bool frame_done = false;
+};
+
+namespace {
+
+// Zero region layout (in 64-bit words): [counters 8][count-scan status][... phase B ...]
+struct ZeroLayout {
+    size_t a_words = 0, total = 0;
+    size_t ctr = 0, st_count = 0;
+    size_t st_tiles = 0, st_wind = 0, st_mark = 0, st_run = 0, st_rows = 0, row_count = 0, row_fill = 0, totals = 0, maxlen = 0;
+};
+
+// Phase A (zeroed at frame start): count-scan counter, max segment length, count-scan status.
+// Phase B (zeroed after the host sync, when N is known): counters of the later scans, totals,
+// the later scans' status arrays and the per-row counters of the composite.
+ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, uint32_t rows) {
+    ZeroLayout z;
+    size_t o = 0;
+    z.maxlen = o; o += 1;              // int32 max segment length + the count-scan counter
+    z.st_count = o; o += spr::scan_tiles(std::max<int64_t>(C, 1));
+    z.a_words = o;
+    z.ctr = o; o += 8;                 // 16 u32 counters of the phase-B scans
+    z.totals = o; o += 2;              // 4 int32: cells, spans, R, spare
+    z.st_tiles = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
+    z.st_wind = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
+    z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
+    z.st_run = o; o += spr::scan_tiles(std::max<int64_t>(2 * N, 1));
+    z.st_rows = o; o += spr::scan_tiles(std::max<uint32_t>(rows, 1));
+    z.row_count = o; o += (rows + 2 + 1) / 2;
+    z.row_fill = o; o += (rows + 1) / 2 + 1;
+    z.total = o;
+    return z;
+}
+
+spr::ScanCtl ctl_at(spr_handle* H, const ZeroLayout& z, size_t st, int counter) {
+    spr::ScanCtl c;
+    c.status = H->zero.p + st;
+    c.counter = counter == 0 ? reinterpret_cast<uint32_t*>(H->zero.p + z.maxlen) + 1
+                             : reinterpret_cast<uint32_t*>(H->zero.p + z.ctr) + counter;
+    c.epoch = 0;
+    return c;
+}
+
+void stage_mark(spr_handle* H, int idx) {
+    if (!H->stage_timing) return;
+    if ((int)H->ev_stage.size() <= idx) return;
+    SPR_HIP(hipEventRecord(H->ev_stage[idx], H->stream));
+}
+
+void run_frame(spr_handle* H) {
+    if (!H->loaded) throw Fail{SPR_ERR_STATE, "render before load"};
+    hipStream_t s = H->stream;
+    const uint32_t P = H->P, C = H->C, Np = H->Np;
+    const int w = (int)H->w, h = (int)H->h;
+    const uint32_t rows = (H->h + 1) / 2;
+    const bool debug = H->opts.debug != 0;
+
+    SPR_HIP(hipEventRecord(H->ev_begin, s));
+    stage_mark(H, 0);
+    // ---- phase A: zero the visibility mask and the count-scan control
+    ZeroLayout za = zero_layout(C, Np, std::max<int64_t>(H->N, 0), rows);
+    H->zero.ensure(std::max(za.total, za.a_words));
+    H->visible.ensure(std::max<uint32_t>(Np, 1));
+    H->tpos.ensure(std::max<uint32_t>(P, 1));
+    H->cuts.ensure(std::max<uint32_t>(C, 1));
+    H->ncuts.ensure(std::max<uint32_t>(C, 1));
+    H->counts.ensure((size_t)C + 1);
+    H->seg.ensure((size_t)Np + 1);
+    H->tile_counts.ensure((size_t)Np + 1);
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)H->zero.p, 2 * (int64_t)za.a_words, 0u, s));
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)H->visible.p, Np, 0u, s));
+    SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p, s));
+    stage_mark(H, 1);
+    SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, C, w, h,
+                              H->cuts.p, H->ncuts.p, H->counts.p, s));
+    stage_mark(H, 2);
+    SPR_HIP(spr::launch_scan_i32(H->counts.p, H->counts.p, C, ctl_at(H, za, za.st_count, 0), s));
+    int32_t* d_maxlen = reinterpret_cast<int32_t*>(H->zero.p + za.maxlen);
+    int64_t N = 0;
+    int32_t max_len = 0;
+    if (!H->inj) {
+        SPR_HIP(spr::launch_path_segments(H->cpath.p, C, H->counts.p, Np, H->seg.p, s));
+        SPR_HIP(spr::launch_segment_stats(H->seg.p, Np, d_maxlen, s));
+        // ---- HOST SYNC: the count-then-write readback
+        SPR_HIP(hipMemcpyAsync(H->h_pinned, H->counts.p + C, 4, hipMemcpyDeviceToHost, s));
+        SPR_HIP(hipMemcpyAsync(H->h_pinned + 1, d_maxlen, 4, hipMemcpyDeviceToHost, s));
+        SPR_HIP(hipStreamSynchronize(s));
+        N = H->h_pinned[0];
+        max_len = H->h_pinned[1];
+    } else {
+        N = (int64_t)H->inj_inter.size() / 2;
+        std::vector<int32_t> hseg((size_t)Np + 1, 0);
+        for (int64_t i = 0; i < N; ++i) {
+            int32_t c0 = H->inj_inter[2 * i];
+            int32_t c1 = i + 1 != N ? H->inj_inter[2 * i + 2] : -1;
+            uint32_t p0 = H->h_cpath[c0];
+            if (c0 != c1) {
+                uint32_t p1 = c1 >= 0 ? H->h_cpath[c1] : Np;
+                for (uint32_t j = p0 + 1; j <= p1; ++j) hseg[j] = (int32_t)(i + 1);
+            }
+            if (i == 0) for (uint32_t j = 0; j <= p0; ++j) hseg[j] = 0;
+        }
+        for (uint32_t p = 0; p < Np; ++p) max_len = std::max(max_len, hseg[p + 1] - hseg[p]);
+        SPR_HIP(hipMemcpyAsync(H->seg.p, hseg.data(), hseg.size() * 4, hipMemcpyHostToDevice, s));
+    }
+    if (N < 0 || N > (int64_t)(1 << 30)) throw Fail{SPR_ERR_CAPACITY, "intersection count exceeds 2^30"};
+    H->N = N;
+    H->max_len = max_len;
+    stage_mark(H, 3);
+
+    // ---- phase B: size buffers for N
+    ZeroLayout z = zero_layout(C, Np, N, rows);
+    H->zero.ensure(z.total);
+    const size_t n1 = (size_t)std::max<int64_t>(N, 1);
+    H->inter.ensure(n1);
+    H->chain_t.ensure(n1);
+    H->key.ensure(n1);
+    H->wn.ensure(n1);
+    H->epath.ensure(n1);
+    H->sorted.ensure(n1);
+    H->sbig.ensure(n1);
+    H->stmp.ensure(n1);
+    H->skey.ensure(n1);
+    H->swn.ensure(n1);
+    H->winding.ensure(n1 + 1);
+    if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); }
+    H->records.ensure(2 * n1);
+    H->rec_path.ensure(2 * n1);
+    H->run_flag.ensure(2 * n1 + 1);
+    H->runs.ensure(2 * n1);
+    H->row_runs.ensure(2 * n1);
+    H->row_start.ensure((size_t)rows + 1);
+    H->image.ensure((size_t)4 * H->w * H->h);
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)(H->zero.p + z.a_words), 2 * (int64_t)(z.total - z.a_words), 0u, s));
+    int32_t* d_totals = reinterpret_cast<int32_t*>(H->zero.p + z.totals);
+
+    if (H->inj) {
+        SPR_HIP(hipMemcpyAsync(H->inter.p, H->inj_inter.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
+    } else if (N > 0) {
+        SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
+                                        H->ncuts.p, H->counts.p, C, w, h, H->chain_t.p, s));
+        stage_mark(H, 4);
+        SPR_HIP(spr::launch_piece_merge(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
+                                        H->ncuts.p, H->counts.p, C, w, h, H->chain_t.p, H->inter.p, s));
+    }
+    stage_mark(H, 5);
+    SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
+                                     H->key.p, H->wn.p, H->epath.p, s));
+    stage_mark(H, 6);
+    // ---- segmented sort
+    if (N > 0) {
+        SPR_HIP(spr::launch_tile_counts(H->seg.p, Np, H->tile_counts.p, s));
+        SPR_HIP(spr::launch_scan_i32(H->tile_counts.p, H->tile_counts.p, Np, ctl_at(H, z, z.st_tiles, 1), s));
+        const int64_t max_tiles = (int64_t)Np + ceil_div(N, spr::kSortTile) + 1;
+        SPR_HIP(spr::launch_sort_tiles(H->key.p, H->wn.p, H->seg.p, H->tile_counts.p, Np, max_tiles, H->sorted.p,
+                                       H->sbig.p, s));
+        int passes = 0;
+        while (((int64_t)spr::kSortTile << passes) < (int64_t)max_len) ++passes;
+        for (int j = 0; j < passes; ++j) {
+            const unsigned long long* src = (j & 1) ? H->stmp.p : H->sbig.p;
+            unsigned long long* dst = (j & 1) ? H->sbig.p : H->stmp.p;
+            SPR_HIP(spr::launch_merge_pass(H->seg.p, H->tile_counts.p, Np, max_tiles, j, src, dst, H->sorted.p, s));
+        }
+        SPR_HIP(spr::launch_unpack_sorted(H->sorted.p, N, H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s));
+    }
+    stage_mark(H, 7);
+    // ---- winding numbers (global exclusive scan, scanline_rasterizer.cpp:478-506)
+    SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, N, ctl_at(H, z, z.st_wind, 2), s));
+    stage_mark(H, 8);
+    SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
+                                  ctl_at(H, z, z.st_mark, 3), H->records.p, H->rec_path.p,
+                                  debug ? H->flags.p : nullptr, d_totals, s));
+    stage_mark(H, 9);
+    // ---- composite
+    const int64_t max_rec = 2 * (int64_t)n1;
+    SPR_HIP(spr::launch_run_flags(H->records.p, H->rec_path.p, d_totals, max_rec, H->run_flag.p, s));
+    SPR_HIP(spr::launch_scan_i32(H->run_flag.p, H->run_flag.p, max_rec, ctl_at(H, z, z.st_run, 4), s));
+    SPR_HIP(spr::launch_run_emit(H->records.p, H->run_flag.p, d_totals, max_rec, H->runs.p, s));
+    const int32_t* d_nruns = H->run_flag.p + max_rec;
+    int32_t* d_row_count = reinterpret_cast<int32_t*>(H->zero.p + z.row_count);
+    int32_t* d_row_fill = reinterpret_cast<int32_t*>(H->zero.p + z.row_fill);
+    SPR_HIP(spr::launch_row_count(H->runs.p, d_nruns, max_rec, d_row_count, s));
+    SPR_HIP(spr::launch_scan_i32(d_row_count, H->row_start.p, rows, ctl_at(H, z, z.st_rows, 5), s));
+    SPR_HIP(spr::launch_row_scatter(H->runs.p, d_nruns, max_rec, H->row_start.p, d_row_fill, H->row_runs.p, s));
+    stage_mark(H, 10);
+    SPR_HIP(spr::launch_composite(H->records.p, H->runs.p, d_nruns, d_totals, H->row_start.p, H->row_runs.p, w, h,
+                                  (int)H->opts.compose, H->image.p, s));
+    stage_mark(H, 11);
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
+    SPR_HIP(hipEventRecord(H->ev_end, s));
+    H->frame_done = true;
+}
+
+template <class F>
+int guarded(spr_handle* H, F&& f) {
+    try {
+        f();
+        if (H) H->err.clear();
+        return SPR_OK;
+    } catch (const Fail& e) {
+        if (H) H->err = e.msg; else g_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (H) H->err = "host out of memory"; else g_error = "host out of memory";
+        return SPR_ERR_CAPACITY;
+    } catch (const std::exception& e) {
+        if (H) H->err = e.what(); else g_error = e.what();
+        return SPR_ERR_INVALID;
+    }
+}
+
+void sync_stats(spr_handle* H) {
+    if (!H->frame_done) return;
+    SPR_HIP(hipStreamSynchronize(H->stream));
+    H->stats.n_points = H->P;
+    H->stats.n_curves = H->C;
+    H->stats.n_paths = H->Np;
+    H->stats.n_intersections = H->N;
+    H->stats.n_cells = H->h_pinned[4];
+    H->stats.n_spans = H->h_pinned[5];
+    float ms = 0.0f;
+    SPR_HIP(hipEventElapsedTime(&ms, H->ev_begin, H->ev_end));
+    H->stats.frame_ms = ms;
+    if (H->stage_timing) {
+        for (size_t k = 0; k + 1 < H->ev_stage.size() && k < 16; ++k) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, H->ev_stage[k], H->ev_stage[k + 1]) == hipSuccess) H->stats.stage_ms[k] = t;
+        }
+    }
+}
+
+void load_arrays(spr_handle* H, const float* pos, const uint32_t* pos_path, uint32_t P, const uint32_t* cpos,
+                 const uint32_t* ctype, const uint32_t* cpath, uint32_t C, const uint32_t* rule,
+                 const uint32_t* fill, uint32_t Np) {
+    std::string bad = spr::validate(pos, pos_path, P, cpos, ctype, cpath, C, rule, Np);
+    if (!bad.empty()) throw Fail{SPR_ERR_INVALID, bad};
+    if (Np && !fill) throw Fail{SPR_ERR_INVALID, "null fill_info"};
+    if (Np > (1u << 30) - 1) throw Fail{SPR_ERR_CAPACITY, "too many paths"};
+    SPR_HIP(hipSetDevice(H->device));
+    H->pos.ensure(std::max<uint32_t>(P, 1));
+    H->pos_path.ensure(std::max<uint32_t>(P, 1));
+    H->cpos.ensure(std::max<uint32_t>(C, 1));
+    H->ctype.ensure(std::max<uint32_t>(C, 1));
+    H->cpath.ensure(std::max<uint32_t>(C, 1));
+    H->rule.ensure(std::max<uint32_t>(Np, 1));
+    H->fill.ensure(std::max<uint32_t>(Np, 1));
+    hipStream_t s = H->stream;
+    if (P) {
+        SPR_HIP(hipMemcpyAsync(H->pos.p, pos, (size_t)P * 8, hipMemcpyHostToDevice, s));
+        SPR_HIP(hipMemcpyAsync(H->pos_path.p, pos_path, (size_t)P * 4, hipMemcpyHostToDevice, s));
+    }
+    if (C) {
+        SPR_HIP(hipMemcpyAsync(H->cpos.p, cpos, (size_t)C * 4, hipMemcpyHostToDevice, s));
+        SPR_HIP(hipMemcpyAsync(H->ctype.p, ctype, (size_t)C * 4, hipMemcpyHostToDevice, s));
+        SPR_HIP(hipMemcpyAsync(H->cpath.p, cpath, (size_t)C * 4, hipMemcpyHostToDevice, s));
+    }
+    if (Np) {
+        SPR_HIP(hipMemcpyAsync(H->rule.p, rule, (size_t)Np * 4, hipMemcpyHostToDevice, s));
+        SPR_HIP(hipMemcpyAsync(H->fill.p, fill, (size_t)Np * 4, hipMemcpyHostToDevice, s));
+    }
+    SPR_HIP(hipStreamSynchronize(s));
+    H->h_cpath.assign(cpath, cpath + C);
+    H->P = P; H->C = C; H->Np = Np;
+    H->loaded = true;
+    H->frame_done = false;
+    H->inj = false;
+    H->inj_inter.clear();
+    spr::learned_on_load(H->learned);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spr_global_error(void) { return g_error.c_str(); }
+
+const char* spr_last_error(const spr_handle* h) { return h ? h->err.c_str() : g_error.c_str(); }
+
+int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle** out) {
+    if (!out) { g_error = "null out"; return SPR_ERR_INVALID; }
+    *out = nullptr;
+    if (width == 0 || height == 0 || width > 8192 || height > 16384) {
+        g_error = "canvas must be 1..8192 x 1..16384";
+        return SPR_ERR_CAPACITY;
+    }
+    spr_handle* H = new (std::nothrow) spr_handle();
+    if (!H) { g_error = "host out of memory"; return SPR_ERR_CAPACITY; }
+    int rc = guarded(nullptr, [&] {
+        H->w = width; H->h = height;
+        if (opts) H->opts = *opts;
+        if (H->opts.compose > 1 || H->opts.mode > 1) throw Fail{SPR_ERR_INVALID, "bad compose or mode"};
+        int dev = 0;
+        if (H->opts.device >= 0) dev = H->opts.device; else SPR_HIP(hipGetDevice(&dev));
+        H->device = dev;
+        SPR_HIP(hipSetDevice(dev));
+        SPR_HIP(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
+        SPR_HIP(hipEventCreate(&H->ev_begin));
+        SPR_HIP(hipEventCreate(&H->ev_end));
+        H->ev_stage.resize(13);
+        for (auto& e : H->ev_stage) SPR_HIP(hipEventCreate(&e));
+        SPR_HIP(hipHostMalloc((void**)&H->h_pinned, 64, hipHostMallocDefault));
+        std::memset(H->h_pinned, 0, 64);
+        const float id[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+        std::memcpy(H->mvp.m, id, sizeof(id));
+        H->image.ensure((size_t)4 * width * height);
+    });
+    if (rc != SPR_OK) { spr_destroy(H); return rc; }
+    *out = H;
+    return SPR_OK;
+}
+
+int spr_destroy(spr_handle* H) {
+    if (!H) return SPR_OK;
+    if (H->stream) (void)hipStreamSynchronize(H->stream);
+    DBuf<int>* unused = nullptr; (void)unused;
+    H->pos.release(); H->pos_path.release(); H->cpos.release(); H->ctype.release(); H->cpath.release();
+    H->rule.release(); H->fill.release(); H->tpos.release(); H->visible.release(); H->cuts.release();
+    H->ncuts.release(); H->counts.release(); H->seg.release(); H->tile_counts.release(); H->inter.release();
+    H->chain_t.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
+    H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release(); H->swn.release();
+    H->winding.release(); H->flags.release(); H->records.release(); H->rec_path.release();
+    H->run_flag.release(); H->row_start.release(); H->row_fill.release(); H->row_runs.release();
+    H->runs.release(); H->image.release(); H->zero.release();
+    spr::learned_release(H->learned);
+    if (H->h_pinned) (void)hipHostFree(H->h_pinned);
+    for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
+    if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
+    if (H->ev_end) (void)hipEventDestroy(H->ev_end);
+    if (H->stream) (void)hipStreamDestroy(H->stream);
+    delete H;
+    return SPR_OK;
+}
+
+int spr_load_paths(spr_handle* H, const float* pos_xy, const uint32_t* pos_path_idx, uint32_t n_points,
+                   const uint32_t* curve_pos_map, const uint32_t* curve_type, const uint32_t* curve_path_idx,
+                   uint32_t n_curves, const uint32_t* fill_rule, const uint32_t* fill_info, uint32_t n_paths) {
+    if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        load_arrays(H, pos_xy, pos_path_idx, n_points, curve_pos_map, curve_type, curve_path_idx, n_curves,
+                    fill_rule, fill_info, n_paths);
+    });
+}
+
+int spr_load_scene(spr_handle* H, const spr_scene* sc) {
+    if (!H || !sc) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    const spr::Scene& s = sc->s;
+    return guarded(H, [&] {
+        load_arrays(H, s.pos.data(), s.pos_path.data(), s.n_points(), s.curve_pos.data(), s.curve_type.data(),
+                    s.curve_path.data(), s.n_curves(), s.rule.data(), s.fill.data(), s.n_paths());
+    });
+}
+
+int spr_load_rvg(spr_handle* H, const char* path) {
+    if (!H || !path) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    spr::Scene s;
+    int rc = guarded(H, [&] {
+        try { s = spr::load_rvg(path); } catch (const std::exception& e) { throw Fail{SPR_ERR_IO, e.what()}; }
+    });
+    if (rc != SPR_OK) return rc;
+    return guarded(H, [&] {
+        load_arrays(H, s.pos.data(), s.pos_path.data(), s.n_points(), s.curve_pos.data(), s.curve_type.data(),
+                    s.curve_path.data(), s.n_curves(), s.rule.data(), s.fill.data(), s.n_paths());
+    });
+}
+
+int spr_set_mvp(spr_handle* H, const float m_rows[16]) {
+    if (!H || !m_rows) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    std::memcpy(H->mvp.m, m_rows, sizeof(H->mvp.m));
+    return SPR_OK;
+}
+
+int spr_render_frame(spr_handle* H, uint8_t* rgba_out) {
+    if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        SPR_HIP(hipSetDevice(H->device));
+        if (H->opts.mode == SPR_MODE_LEARNED && !H->inj)
+            spr::learned_frame(H, H->learned);
+        else
+            run_frame(H);
+        if (rgba_out) {
+            SPR_HIP(hipMemcpyAsync(rgba_out, H->image.p, (size_t)4 * H->w * H->h, hipMemcpyDeviceToHost, H->stream));
+            SPR_HIP(hipStreamSynchronize(H->stream));
+        }
+    });
+}
+
+int spr_image_device_ptr(spr_handle* H, void** dptr) {
+    if (!H || !dptr) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    *dptr = H->image.p;
+    return SPR_OK;
+}
+
+int spr_synchronize(spr_handle* H) {
+    if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] { SPR_HIP(hipStreamSynchronize(H->stream)); });
+}
+
+int spr_set_stage_timing(spr_handle* H, int enable) {
+    if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
+    H->stage_timing = enable != 0;
+    return SPR_OK;
+}
+
+int spr_get_stats(const spr_handle* Hc, spr_stats* out) {
+    spr_handle* H = const_cast<spr_handle*>(Hc);
+    if (!H || !out) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        sync_stats(H);
+        *out = H->stats;
+    });
+}
+
+int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nbytes) {
+    if (!H || !nbytes) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (!H->frame_done) throw Fail{SPR_ERR_STATE, "no frame rendered"};
+        SPR_HIP(hipStreamSynchronize(H->stream));
+        const bool debug = H->opts.debug != 0;
+        const int64_t N = H->N;
+        const void* src = nullptr;
+        size_t n = 0;
+        std::vector<uint8_t> tmp;
+        auto from_dev = [&](const void* p, size_t bytes) {
+            tmp.resize(bytes);
+            if (bytes) SPR_HIP(hipMemcpy(tmp.data(), p, bytes, hipMemcpyDeviceToHost));
+            src = tmp.data(); n = bytes;
+        };
+        switch (stage) {
+            case SPR_STAGE_TPOS: from_dev(H->tpos.p, (size_t)H->P * 8); break;
+            case SPR_STAGE_VISIBLE: from_dev(H->visible.p, (size_t)H->Np * 4); break;
+            case SPR_STAGE_CUTS: {
+                std::vector<float4> q(H->C);
+                std::vector<uint32_t> nc(H->C);
+                if (H->C) {
+                    SPR_HIP(hipMemcpy(q.data(), H->cuts.p, (size_t)H->C * 16, hipMemcpyDeviceToHost));
+                    SPR_HIP(hipMemcpy(nc.data(), H->ncuts.p, (size_t)H->C * 4, hipMemcpyDeviceToHost));
+                }
+                tmp.resize((size_t)H->C * 20);
+                float* o = (float*)tmp.data();
+                for (uint32_t c = 0; c < H->C; ++c) {
+                    o[5 * c] = q[c].x; o[5 * c + 1] = q[c].y; o[5 * c + 2] = q[c].z; o[5 * c + 3] = q[c].w;
+                    std::memcpy(&o[5 * c + 4], &nc[c], 4);
+                }
+                src = tmp.data(); n = tmp.size();
+                break;
+            }
+            case SPR_STAGE_COUNTS: from_dev(H->counts.p, ((size_t)H->C + 1) * 4); break;
+            case SPR_STAGE_INTER: from_dev(H->inter.p, (size_t)N * 8); break;
+            case SPR_STAGE_KEY: from_dev(H->key.p, (size_t)N * 4); break;
+            case SPR_STAGE_WN: {
+                std::vector<int8_t> v((size_t)N);
+                if (N) SPR_HIP(hipMemcpy(v.data(), H->wn.p, (size_t)N, hipMemcpyDeviceToHost));
+                tmp.resize((size_t)N * 4);
+                for (int64_t i = 0; i < N; ++i) ((int32_t*)tmp.data())[i] = v[i];
+                src = tmp.data(); n = tmp.size();
+                break;
+            }
+            case SPR_STAGE_SEG: from_dev(H->seg.p, ((size_t)H->Np + 1) * 4); break;
+            case SPR_STAGE_SKEY: from_dev(H->skey.p, (size_t)N * 4); break;
+            case SPR_STAGE_SIDX:
+                if (!debug) throw Fail{SPR_ERR_STATE, "sorted indices are kept only with the debug option"};
+                from_dev(H->sidx.p, (size_t)N * 4);
+                break;
+            case SPR_STAGE_WINDING: from_dev(H->winding.p, ((size_t)N + 1) * 4); break;
+            case SPR_STAGE_FLAGS:
+                if (!debug) throw Fail{SPR_ERR_STATE, "flags are kept only with the debug option"};
+                from_dev(H->flags.p, (size_t)N * 8);
+                break;
+            case SPR_STAGE_RECORDS: {
+                int64_t R = (int64_t)H->h_pinned[4] + H->h_pinned[5];
+                from_dev(H->records.p, (size_t)R * 16);
+                break;
+            }
+            case SPR_STAGE_IMAGE: from_dev(H->image.p, (size_t)4 * H->w * H->h); break;
+            default: throw Fail{SPR_ERR_INVALID, "unknown stage"};
+        }
+        *nbytes = n;
+        if (dst) {
+            if (cap < n) throw Fail{SPR_ERR_CAPACITY, "destination too small"};
+            if (n) std::memcpy(dst, src, n);
+        }
+    });
+}
+
+int spr_debug_inject(spr_handle* H, int stage, const void* src, size_t nbytes) {
+    if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (stage != SPR_STAGE_INTER) throw Fail{SPR_ERR_INVALID, "only SPR_STAGE_INTER can be injected"};
+        if (nbytes == 0) { H->inj = false; H->inj_inter.clear(); return; }
+        if (!src || nbytes % 8) throw Fail{SPR_ERR_INVALID, "intersections are int2 records"};
+        if (!H->loaded) throw Fail{SPR_ERR_STATE, "inject before load"};
+        const int32_t* v = (const int32_t*)src;
+        const size_t n = nbytes / 8;
+        for (size_t i = 0; i < n; ++i) {
+            if (v[2 * i] < 0 || (uint32_t)v[2 * i] >= H->C) throw Fail{SPR_ERR_INVALID, "curve index out of range"};
+            if (i && v[2 * i] < v[2 * i - 2]) throw Fail{SPR_ERR_INVALID, "intersections must be in curve order"};
+        }
+        H->inj_inter.assign(v, v + 2 * n);
+        H->inj = true;
+    });
+}
+
+// ---- scenes
+int spr_scene_load_rvg(const char* path, spr_scene** out) {
+    if (!path || !out) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    *out = nullptr;
+    try {
+        spr_scene* s = new spr_scene();
+        try { s->s = spr::load_rvg(path); } catch (...) { delete s; throw; }
+        *out = s;
+        return SPR_OK;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return SPR_ERR_IO;
+    }
+}
+
+int spr_scene_generate(int kind, uint64_t seed, uint32_t n_paths, uint32_t canvas, spr_scene** out) {
+    if (!out) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    *out = nullptr;
+    try {
+        spr_scene* s = new spr_scene();
+        try { s->s = spr::generate(kind, seed, n_paths, canvas); } catch (...) { delete s; throw; }
+        *out = s;
+        return SPR_OK;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return SPR_ERR_INVALID;
+    }
+}
+
+int spr_scene_subset(const spr_scene* s, uint32_t first, uint32_t count, spr_scene** out) {
+    if (!s || !out) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    *out = nullptr;
+    try {
+        spr_scene* o = new spr_scene();
+        try { o->s = spr::subset(s->s, first, count); } catch (...) { delete o; throw; }
+        *out = o;
+        return SPR_OK;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return SPR_ERR_INVALID;
+    }
+}
+
+int spr_scene_view_get(const spr_scene* sc, spr_scene_view* v) {
+    if (!sc || !v) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    const spr::Scene& s = sc->s;
+    v->pos_xy = s.pos.data(); v->pos_path_idx = s.pos_path.data(); v->n_points = s.n_points();
+    v->curve_pos_map = s.curve_pos.data(); v->curve_type = s.curve_type.data();
+    v->curve_path_idx = s.curve_path.data(); v->n_curves = s.n_curves();
+    v->fill_rule = s.rule.data(); v->fill_info = s.fill.data(); v->n_paths = s.n_paths();
+    v->width = s.width; v->height = s.height;
+    return SPR_OK;
+}
+
+int spr_scene_write_rvg(const spr_scene* s, const char* path) {
+    if (!s || !path) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    try { spr::write_rvg(s->s, path); return SPR_OK; }
+    catch (const std::exception& e) { g_error = e.what(); return SPR_ERR_IO; }
+}
+
+void spr_scene_free(spr_scene* s) { delete s; }
+
+}  // extern "C"
+
+// Entry used by the learned-allocation module to run the plain frame.
+namespace spr {
+void run_plain_frame(spr_handle* H) { run_frame(H); }
+}
