@@ -1,0 +1,152 @@
+"""Benchmark: one frame of the scanline path-rasterisation pipeline on one MI355X.
+
+A step is one frame: device-resident SoA input (after load) -> final RGBA8 image in HBM, with
+every count, allocation and host synchronisation of the chosen allocation mode inside the timed
+region (image readback excluded).  Default workload: the illustration scene (5,000 paths of
+4-400 cubic segments, log-normal sizes, per-path affine, mixed fill rules, alpha 0.3-1.0,
+src-over composite into 4096x4096 RGBA8), synthetic from a seeded generator.
+
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+import vkscanlinepr_amd as V  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Stage boundaries recorded by the library (stage_ms[k] = events k -> k+1).
+STAGE_NAMES = ["transform", "count", "scan_counts+sync", "chain_solve", "piece_merge",
+               "gen_fragment", "sort", "winding_scan", "mark_emit", "runs+rows", "composite"]
+
+WORKLOADS = {
+    # name: (generator kind, seed, n_paths, canvas, compose)
+    "illustration_4096": (2, 3, 5000, 4096, V.COMPOSE_SRC_OVER),
+    "glyphs_2048": (1, 2, 20000, 2048, V.COMPOSE_OVERWRITE),
+    "scribbles_1024": (0, 1, 64, 1024, V.COMPOSE_OVERWRITE),
+}
+
+
+def stage_bytes(st, P, C, Np, N, R, w, h):
+    """Algorithmic HBM bytes of each stage: every input read once, every output written once."""
+    return {
+        "transform": 20 * P + 4 * Np,
+        "count": 72 * C,
+        "chain_solve": 48 * C + 4 * N,
+        "piece_merge": 48 * C + 4 * N + 8 * N,
+        "gen_fragment": 8 * N + 48 * C + 9 * N,
+        "sort": 16 * N,
+        "winding_scan": 8 * N,
+        "mark_emit": 16 * N + 16 * R,
+        "composite": 16 * R + 4 * w * h,
+    }
+
+
+def cpu_baseline(kind, seed, n_paths, canvas, compose, budget_s=20.0):
+    """The CPU oracle (C++ restatement of the reference shaders) on a bounded prefix of the
+    same scene, all host threads (capped at 16)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    threads = min(16, os.cpu_count() or 1)
+    scene = V.Scene.generate(kind, seed, n_paths, canvas)
+    k = max(1, min(n_paths, 50))
+    best = None
+    while True:
+        sub = scene.subset(0, k)
+        a = sub.arrays()
+        t0 = time.perf_counter()
+        O.Frame(a, canvas, canvas, compose=compose, threads=threads)
+        dt = time.perf_counter() - t0
+        best = (k, dt)
+        if dt > budget_s / 4 or k >= n_paths:
+            break
+        k = min(n_paths, int(k * max(2.0, budget_s / 4 / max(dt, 1e-3))))
+    k, dt = best
+    return {"value": k / dt / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": f"first {k} of {n_paths} paths of the same scene, {canvas}x{canvas}, "
+                      f"one frame in {dt:.2f} s"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="illustration_4096", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="plain", choices=["plain", "learned"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stages", action="store_true", help="print per-stage times to stderr")
+    a = ap.parse_args(argv)
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build renders on one GPU; multi-GPU runs are not part of it "
+              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        return 2
+
+    kind, seed, n_paths, canvas, compose = WORKLOADS[a.workload]
+    scene = V.Scene.generate(kind, seed, n_paths, canvas)
+    mode = V.MODE_LEARNED if a.mode == "learned" else V.MODE_PLAIN
+    r = V.Rasterizer(canvas, canvas, compose=compose, mode=mode, device=0)
+    r.load_scene(scene)
+    r.set_stage_timing(True)
+    for _ in range(a.warmup):
+        r.render(readback=False)
+    r.synchronize()
+
+    frame_ms, stage_acc = [], np.zeros(16)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        r.render(readback=False)
+        s = r.stats()  # synchronises: per-frame device times from HIP events on the stream
+        frame_ms.append(s.frame_ms)
+        stage_acc += np.array(s.stage_ms[:16])
+    r.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / a.steps * 1e3
+    st = r.stats()
+    P, C, Np = st.n_points, st.n_curves, st.n_paths
+    N, R = st.n_intersections, st.n_cells + st.n_spans
+    stages = dict(zip(STAGE_NAMES, stage_acc[: len(STAGE_NAMES)] / a.steps))
+    algo = stage_bytes(st, P, C, Np, N, R, canvas, canvas)
+    if a.stages:
+        for k, v in stages.items():
+            gbs = algo.get(k, 0) / (v * 1e-3) / 1e9 if v > 0 and k in algo else 0.0
+            print(f"  {k:18s} {v:8.3f} ms  {gbs:8.1f} GB/s", file=sys.stderr)
+    dom = max((k for k in algo if k in stages), key=lambda k: stages[k])
+    dom_ms = stages[dom]
+    achieved = algo[dom] / (dom_ms * 1e-3) / 1e9
+    out = {
+        "metric": "ms/frame & Mpaths/s at 4096^2 (learned 1-pass vs plain 2-pass); HBM GB/s",
+        "value": Np / (ms_per_step * 1e-3) / 1e6,
+        "unit": "Mpaths/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+int32", "data": "synthetic (seeded generator)",
+        "config": {"workload": a.workload, "mode": a.mode, "paths": Np, "curves": C,
+                   "points": P, "intersections": N, "records": R, "canvas": canvas,
+                   "compose": "src_over" if compose else "overwrite",
+                   "device_ms_per_frame": float(np.mean(frame_ms))},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "stage_ms": {k: round(v, 4) for k, v in stages.items()}},
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(kind, seed, n_paths, canvas, compose)
+    print(json.dumps(out))
+    r.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -63,7 +63,11 @@ struct DBuf {
         p = (T*)q;
         cap = want;
     }
-    void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+    void release() {
+        if (p) { (void)hipFree(p); (void)hipGetLastError(); }
+        p = nullptr;
+        cap = 0;
+    }
 };
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -78,6 +82,7 @@ struct spr_handle {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::vector<hipEvent_t> ev_stage;
     bool stage_timing = false;
+    int n_marks = 0;  // stage events recorded in the last frame
     std::string err;
 
     // inputs
@@ -167,6 +172,7 @@ void stage_mark(spr_handle* H, int idx) {
     if (!H->stage_timing) return;
     if ((int)H->ev_stage.size() <= idx) return;
     SPR_HIP(hipEventRecord(H->ev_stage[idx], H->stream));
+    H->n_marks = idx + 1;
 }
 
 void run_frame(spr_handle* H) {
@@ -178,6 +184,7 @@ void run_frame(spr_handle* H) {
     const bool debug = H->opts.debug != 0;
 
     SPR_HIP(hipEventRecord(H->ev_begin, s));
+    H->n_marks = 0;
     stage_mark(H, 0);
     // ---- phase A: zero the visibility mask and the count-scan control
     ZeroLayout za = zero_layout(C, Np, std::max<int64_t>(H->N, 0), rows);
@@ -256,12 +263,15 @@ void run_frame(spr_handle* H) {
     SPR_HIP(spr::launch_fill_u32((uint32_t*)(H->zero.p + z.a_words), 2 * (int64_t)(z.total - z.a_words), 0u, s));
     int32_t* d_totals = reinterpret_cast<int32_t*>(H->zero.p + z.totals);
 
+    const bool gen = !H->inj && N > 0;
     if (H->inj) {
         SPR_HIP(hipMemcpyAsync(H->inter.p, H->inj_inter.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
-    } else if (N > 0) {
+    } else if (gen) {
         SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
                                         H->ncuts.p, H->counts.p, C, w, h, H->chain_t.p, s));
-        stage_mark(H, 4);
+    }
+    stage_mark(H, 4);
+    if (gen) {
         SPR_HIP(spr::launch_piece_merge(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
                                         H->ncuts.p, H->counts.p, C, w, h, H->chain_t.p, H->inter.p, s));
     }
@@ -343,10 +353,12 @@ void sync_stats(spr_handle* H) {
     float ms = 0.0f;
     SPR_HIP(hipEventElapsedTime(&ms, H->ev_begin, H->ev_end));
     H->stats.frame_ms = ms;
+    for (int k = 0; k < 16; ++k) H->stats.stage_ms[k] = 0.0;
     if (H->stage_timing) {
-        for (size_t k = 0; k + 1 < H->ev_stage.size() && k < 16; ++k) {
+        for (int k = 0; k + 1 < H->n_marks && k < 16; ++k) {
             float t = 0.0f;
-            if (hipEventElapsedTime(&t, H->ev_stage[k], H->ev_stage[k + 1]) == hipSuccess) H->stats.stage_ms[k] = t;
+            SPR_HIP(hipEventElapsedTime(&t, H->ev_stage[k], H->ev_stage[k + 1]));
+            H->stats.stage_ms[k] = t;
         }
     }
 }
@@ -449,6 +461,7 @@ int spr_destroy(spr_handle* H) {
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
     if (H->ev_end) (void)hipEventDestroy(H->ev_end);
     if (H->stream) (void)hipStreamDestroy(H->stream);
+    (void)hipGetLastError();  // nothing of a destroyed handle may leak into later calls
     delete H;
     return SPR_OK;
 }
