@@ -26,8 +26,8 @@ import vkscanlinepr_amd as V  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Stage boundaries recorded by the library (stage_ms[k] = events k -> k+1).
-STAGE_NAMES = ["transform", "count", "scan_counts+sync", "chain_solve", "piece_merge",
-               "gen_fragment", "sort", "winding_scan", "mark_emit", "runs+rows", "composite"]
+STAGE_NAMES = ["transform", "count", "scans+sync", "chain_solve", "piece_merge",
+               "gen_fragment", "sort", "winding_scan", "mark_emit", "(unused)", "composite"]
 
 WORKLOADS = {
     # name: (generator kind, seed, n_paths, canvas, compose)

@@ -217,4 +217,60 @@ __device__ inline float solve_crossing(const CurvePts& cv, int side, float c, fl
     return fabsf(vtm - c) > 1.0f ? t_min : tm;
 }
 
+// Bucket of an entry inside its path's (path, cell row) bucket range.  pr = (first row >= 1,
+// number of rows): rows first..first+nrows-1 come first, then the off-canvas bucket, then row 0,
+// which is the order of the reference's signed key comparison.  A row outside the path's bound
+// (impossible for finite input: every curve point lies inside its control points' hull) raises
+// *err and is clamped so that no write goes out of range.
+__device__ inline uint32_t bucket_local(int32_t key, int2 pr, int32_t* err) {
+    if ((uint32_t)key == kSentinelKey) return (uint32_t)pr.y;
+    const int y = ((key >> 16) & 0xFFFF) - 0x7FFF;
+    const int r = y >> 1;
+    if (r == 0) return (uint32_t)pr.y + 1u;
+    int l = r - pr.x;
+    if (l < 0 || l >= pr.y) {
+        atomicOr(err, 1);
+        if (pr.y == 0) return 0u;
+        l = l < 0 ? 0 : pr.y - 1;
+    }
+    return (uint32_t)l;
+}
+
+// Wave-aggregated atomicAdd on keyed counters: lanes sharing a key take consecutive slots.  The
+// groups are found with ballots first, then every group leader issues its one atomic at the same
+// time, so a wave waits for a single atomic round trip.  Every lane of the wave must call it.
+__device__ inline int32_t wave_slot(int32_t* __restrict__ cnt, uint32_t key, bool valid) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(valid);
+    int leader = lane, count = 0, rank = 0;
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        const uint32_t k = __shfl(key, l);
+        const unsigned long long m = __ballot(valid && key == k);
+        todo &= ~m;
+        if (valid && key == k) { leader = l; count = (int)__popcll(m); rank = (int)__popcll(m & lt); }
+    }
+    int32_t base = 0;
+    if (valid && leader == lane) base = atomicAdd(&cnt[key], count);
+    base = __shfl(base, leader);
+    return base + rank;
+}
+
+// Same grouping for counters whose result is not needed: every group leader adds its group's
+// count in one atomic instruction for the whole wave.
+__device__ inline void wave_count(int32_t* __restrict__ cnt, uint32_t key, bool valid) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(valid);
+    int count = 0;
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        const uint32_t k = __shfl(key, l);
+        const unsigned long long m = __ballot(valid && key == k);
+        todo &= ~m;
+        if (lane == l) count = (int)__popcll(m);
+    }
+    if (count) atomicAdd(&cnt[key], count);
+}
+
 }  // namespace spr

@@ -1,0 +1,152 @@
+// k_bucket.hip -- (path, cell row) buckets: the MSD step of the segmented sort (gfx950).
+//
+// A path's sort keys are (row, x) cell keys, so its entries split into one bucket per cell row,
+// followed by an off-canvas bucket and the row-0 bucket (the reference compares keys as signed
+// int32: rows 2..h-2, then the off-canvas sentinel, then row 0; gen_fragment.comp:180-187,
+// naive_seg_sort_pairs.comp:69).  The transform pass bounds each path's rows from its control
+// points (a Bezier curve stays inside their hull), so each path owns a small, exactly sized
+// bucket range.  Entries are counted per bucket while their keys are generated, scattered once
+// into bucket order, and the buckets are packed into bands of at most 4096 entries that one
+// workgroup sorts in LDS.  The per-bucket ranges also give the composite each path's records of
+// each cell row without any further sorting.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace spr {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n) {
+    int64_t g = (n + kBlock - 1) / kBlock;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// Per-path bucket table from the path's cell-row bounds [lo, hi] (clamped to the canvas):
+// in-canvas rows r >= 1 get buckets first..first+nrows-1, then the off-canvas bucket, then row 0.
+__global__ void k_bucket_counts(const int32_t* __restrict__ row_lo, const int32_t* __restrict__ row_hi,
+                                uint32_t Np, int rmax, int32_t* __restrict__ nb, int2* __restrict__ prow) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= Np) return;
+    int lo = row_lo[p], hi = row_hi[p];
+    int first = lo < 1 ? 1 : lo;
+    int last = hi > rmax ? rmax : hi;
+    int nrows = last >= first ? last - first + 1 : 0;
+    if (lo > hi) { first = 1; nrows = 0; }  // no points
+    prow[p] = make_int2(first, nrows);
+    nb[p] = nrows + 2;
+}
+
+// Scatter into bucket order.  The sort word orders exactly like the reference's signed
+// (key, index) pair and carries the winding change in its two low bits.
+__global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __restrict__ key,
+                                                           const int8_t* __restrict__ wn,
+                                                           const uint32_t* __restrict__ epath, int64_t N,
+                                                           const int32_t* __restrict__ bucket_base,
+                                                           const int2* __restrict__ prow,
+                                                           const int32_t* __restrict__ boff,
+                                                           int32_t* __restrict__ fill,
+                                                           unsigned long long* __restrict__ out,
+                                                           int32_t* __restrict__ err) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < N;
+    uint32_t b = 0;
+    int32_t k = 0;
+    if (valid) {
+        const uint32_t p = epath[i];
+        k = key[i];
+        b = (uint32_t)bucket_base[p] + bucket_local(k, prow[p], err);
+    }
+    const int32_t slot = wave_slot(fill, b, valid);
+    if (valid) {
+        const unsigned long long w = ((unsigned long long)((uint32_t)k ^ 0x80000000u) << 32) |
+                                     ((unsigned long long)i << 2) | (unsigned long long)(uint32_t)(wn[i] + 1);
+        out[boff[b] + slot] = w;
+    }
+}
+
+// Band starts: a band is a run of consecutive buckets of one path whose entries start in the
+// same 2048-entry window; a bucket of more than 2048 entries is a band of its own.  Every band
+// therefore holds at most 4096 entries unless it is a single larger bucket.
+// One wave per path: the most recent non-empty bucket before each bucket comes from an inclusive
+// max-scan of (start << 32 | size) over the non-empty buckets (starts only grow along a path).
+__global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict__ bucket_base, uint32_t Np,
+                                                       const int32_t* __restrict__ boff,
+                                                       int32_t* __restrict__ flag) {
+    const uint32_t p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= Np) return;
+    const int32_t b0 = bucket_base[p], b1 = bucket_base[p + 1];
+    unsigned long long carry = 0ull;  // (start + 1) << 32 | size of the last non-empty bucket; 0 = none
+    for (int32_t c = b0; c < b1; c += 64) {
+        const int32_t b = c + lane;
+        int32_t o = 0, sz = 0;
+        if (b < b1) { o = boff[b]; sz = boff[b + 1] - o; }
+        const unsigned long long mine = sz > 0 ? (((unsigned long long)(uint32_t)(o + 1) << 32) | (uint32_t)sz) : 0ull;
+        unsigned long long inc = mine;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned long long u = __shfl_up(inc, d);
+            if (lane >= d && u > inc) inc = u;
+        }
+        unsigned long long prev = __shfl_up(inc, 1);
+        if (lane == 0) prev = 0ull;
+        if (carry > prev) prev = carry;
+        if (b < b1) {
+            int32_t f = 0;
+            if (sz > 0) {
+                if (prev == 0ull) {
+                    f = 1;
+                } else {
+                    const int32_t ps = (int32_t)(prev >> 32) - 1, pz = (int32_t)(prev & 0xFFFFFFFFull);
+                    f = (o >> 11) != (ps >> 11) || pz > 2048 || sz > 2048;
+                }
+            }
+            flag[b] = f;
+        }
+        const unsigned long long last = __shfl(inc, 63);
+        if (last > carry) carry = last;
+    }
+}
+
+__global__ void k_band_emit(const int32_t* __restrict__ flag_scan, const int32_t* __restrict__ boff,
+                            int32_t NB, int64_t N, int32_t* __restrict__ band_start) {
+    const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (b >= NB) return;
+    if (flag_scan[b + 1] != flag_scan[b]) band_start[flag_scan[b]] = boff[b];
+    if (b == NB - 1) band_start[flag_scan[NB]] = (int32_t)N;
+}
+
+}  // namespace
+
+hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, uint32_t Np, int rmax,
+                                int32_t* nb, int2* prow, hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_bucket_counts<<<grid_for(Np), kBlock, 0, s>>>(row_lo, row_hi, Np, rmax, nb, prow);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
+                                 const int32_t* bucket_base, const int2* prow, const int32_t* boff,
+                                 int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    k_bucket_scatter<<<grid_for(N), kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, boff, fill, out, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
+                             hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_band_flags<<<(Np + 3) / 4, kBlock, 0, s>>>(bucket_base, Np, boff, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,
+                            int32_t* band_start, hipStream_t s) {
+    if (NB == 0) return hipSuccess;
+    k_band_emit<<<grid_for(NB), kBlock, 0, s>>>(flag_scan, boff, NB, N, band_start);
+    return hipGetLastError();
+}
+
+}  // namespace spr

@@ -3,14 +3,16 @@
 // Replaces the reference's line-list draw (scanlinepr.vert:18-50, scanlinepr.frag:7-14, pipeline
 // state scanline_rasterizer.cpp:881-944): every record (yx, width, fill) covers canvas rows y and
 // y+1 over columns [x, x+width), records are applied in order (blending disabled: the later one
-// wins), and the canvas row y lands on image row h-1-y.  All records sit on even rows, so the
-// image is a stack of independent two-pixel "cell rows".
+// wins), and canvas row y lands on image row h-1-y.  All records sit on even rows, so the image
+// is a stack of independent two-pixel "cell rows".
 //
-// Records are grouped into runs (one path, one cell row: contiguous and x-sorted, never
-// overlapping).  Runs are bucketed by cell row; one workgroup per cell row sorts its run ids
-// (= path order), and each thread owns a fixed group of columns in registers, finds the records
-// of each run that touch its columns by binary search, and applies them in path order.  The
-// image is written exactly once.
+// One workgroup owns one cell row, in column chunks of 4096; each thread keeps 16 pixels of the
+// chunk in registers (columns t, t+256, ...).  The (path, row) bucket table gives each path's
+// records on this row as one contiguous range of non-overlapping records with one colour.  Paths
+// are taken in order, 64 at a time: every record ORs its path's bit into a 64-bit coverage mask
+// per pixel in LDS (order does not matter for an OR), then each thread folds the set bits of its
+// pixels in path order (overwrite: the last one wins; src-over: d = s*a + d*(1-a)).  That is two
+// barriers per 64 paths instead of one per path, and each image row is written exactly once.
 #include "kernels.hpp"
 
 namespace spr {
@@ -18,145 +20,124 @@ namespace spr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxLdsRuns = 8192;
+constexpr int kChunk = 4096;               // columns per pass
+constexpr int kPix = kChunk / kBlock;      // pixels per thread
+constexpr int kBatch = 64;                 // paths per coverage mask
+constexpr int kUnroll = 4;                 // records loaded per thread before applying
 
-__device__ inline int rec_row(const int4& r) { return (r.x >> 16) >> 1; }
-
-__global__ void k_run_flags(const int4* __restrict__ records, const uint32_t* __restrict__ rec_path,
-                            const int32_t* __restrict__ totals, int64_t max_records,
-                            int32_t* __restrict__ run_flag) {
-    const int64_t R = totals[2];
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < max_records; k += (int64_t)gridDim.x * blockDim.x) {
-        int32_t f = 0;
-        if (k < R) {
-            f = k == 0 || rec_row(records[k]) != rec_row(records[k - 1]) || rec_path[k] != rec_path[k - 1];
-        }
-        run_flag[k] = f;
-    }
-}
-
-__global__ void k_run_emit(const int4* __restrict__ records, const int32_t* __restrict__ run_scan,
-                           const int32_t* __restrict__ totals, int64_t max_records, int2* __restrict__ runs) {
-    const int64_t R = totals[2];
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < R; k += (int64_t)gridDim.x * blockDim.x) {
-        if (run_scan[k + 1] != run_scan[k]) runs[run_scan[k]] = make_int2(rec_row(records[k]), (int32_t)k);
-    }
-    (void)max_records;
-}
-
-__global__ void k_row_count(const int2* __restrict__ runs, const int32_t* __restrict__ run_total,
-                            int32_t* __restrict__ row_count) {
-    const int64_t NR = *run_total;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < NR; i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&row_count[runs[i].x], 1);
-}
-
-__global__ void k_row_scatter(const int2* __restrict__ runs, const int32_t* __restrict__ run_total,
-                              const int32_t* __restrict__ row_start, int32_t* __restrict__ row_fill,
-                              int32_t* __restrict__ row_runs) {
-    const int64_t NR = *run_total;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < NR; i += (int64_t)gridDim.x * blockDim.x) {
-        const int row = runs[i].x;
-        const int slot = atomicAdd(&row_fill[row], 1);
-        row_runs[row_start[row] + slot] = (int32_t)i;
-    }
-}
-
-// Ascending bitonic sort of n ints at `a` (LDS or global) by one workgroup.  Indices >= n are
-// treated as +infinity; with every comparator moving the minimum down, such virtual elements
-// only ever move upward, so nothing at or beyond n is read or written.
-template <class P>
-__device__ inline void block_sort_ints(P a, int n) {
-    int n2 = 1;
-    while (n2 < n) n2 <<= 1;
-    for (int k = 2; k <= n2; k <<= 1) {
-        const int half = k >> 1;
-        for (int q = threadIdx.x; q < n2 / 2; q += blockDim.x) {
-            const int i = (q / half) * k + (q % half);
-            const int l = (q / half) * k + k - 1 - (q % half);
-            if (l < n) { int x = a[i], y = a[l]; if (y < x) { a[i] = y; a[l] = x; } }
-        }
-        __syncthreads();
-        for (int j = k >> 2; j > 0; j >>= 1) {
-            for (int q = threadIdx.x; q < n2 / 2; q += blockDim.x) {
-                const int i = (q / j) * 2 * j + (q % j);
-                const int l = i + j;
-                if (l < n) { int x = a[i], y = a[l]; if (y < x) { a[i] = y; a[l] = x; } }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-template <int CW, int COMPOSE>
+template <int COMPOSE>
 __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ records,
-                                                      const int2* __restrict__ runs,
-                                                      const int32_t* __restrict__ run_total,
-                                                      const int32_t* __restrict__ totals,
-                                                      const int32_t* __restrict__ row_start,
-                                                      int32_t* __restrict__ row_runs, int w, int h,
-                                                      uint8_t* __restrict__ image) {
-    __shared__ int32_t ids[kMaxLdsRuns];
+                                                      const int2* __restrict__ brec,
+                                                      const int32_t* __restrict__ bucket_base,
+                                                      const int2* __restrict__ prow,
+                                                      const uint32_t* __restrict__ fill, uint32_t Np,
+                                                      int w, int h, uint8_t* __restrict__ image) {
+    __shared__ unsigned long long mask[kChunk];
+    __shared__ int2 list[kBlock];
+    __shared__ uint32_t lcol[kBlock];
+    __shared__ int32_t wcount[kBlock / 64];
+    __shared__ int32_t bpre[kBatch + 1];
     const int r = blockIdx.x;
     const int y = 2 * r;
-    const int s = row_start[r];
-    const int cnt = row_start[r + 1] - s;
-    const int NR = *run_total;
-    const int R = totals[2];
-    const bool lds = cnt <= kMaxLdsRuns;
-    if (lds) {
-        for (int i = threadIdx.x; i < cnt; i += kBlock) ids[i] = row_runs[s + i];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int c0 = 0; c0 < w; c0 += kChunk) {
+        const int cw = w - c0 < kChunk ? w - c0 : kChunk;
+        uint32_t pix[kPix];
+        float acc[COMPOSE ? kPix : 1][4];
+#pragma unroll
+        for (int j = 0; j < kPix; ++j) {
+            pix[j] = 0xFFFFFFFFu;
+            if constexpr (COMPOSE != 0) { acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 1.0f; }
+            mask[j * kBlock + tid] = 0ull;
+        }
         __syncthreads();
-        block_sort_ints(ids, cnt);
-    } else {
-        block_sort_ints(row_runs + s, cnt);
-        __threadfence_block();
-    }
-    __syncthreads();
-    const int c0 = threadIdx.x * CW;
-    uint32_t pix[CW];
-    float acc[COMPOSE ? CW : 1][4];
-#pragma unroll
-    for (int j = 0; j < CW; ++j) {
-        pix[j] = 0xFFFFFFFFu;
-        if (COMPOSE) { acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 1.0f; }
-    }
-    if (c0 < w) {
-        for (int q = 0; q < cnt; ++q) {
-            const int run = lds ? ids[q] : row_runs[s + q];
-            const int rs = runs[run].y;
-            const int re = run + 1 < NR ? runs[run + 1].y : R;
-            // Inside a run, element f emits its cell [x_f, x_f+2) and then its span
-            // [x_{f-1}+2, x_f): both have end + (span ? 2 : 0) == x_f + 2, which therefore never
-            // decreases along the run.  Find the first record that can reach column c0.
-            int lo = rs, hi = re;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                const int4 m = records[mid];
-                if ((m.x & 0xFFFF) + m.y + (m.w == 0 ? 2 : 0) <= c0) lo = mid + 1; else hi = mid;
+        for (uint32_t p0 = 0; p0 < Np; p0 += kBlock) {
+            // ordered list of the paths of this chunk that have records on this cell row
+            const uint32_t p = p0 + tid;
+            int2 rng = make_int2(0, 0);
+            if (p < Np) {
+                const int2 pr = prow[p];
+                const int loc = r == 0 ? pr.y + 1 : ((r >= pr.x && r < pr.x + pr.y) ? r - pr.x : -1);
+                if (loc >= 0) rng = brec[bucket_base[p] + loc];
             }
-            for (int k = lo; k < re; ++k) {
-                const int4 rec = records[k];
-                const int x = rec.x & 0xFFFF;
-                if (x >= c0 + CW) {
-                    // Later elements start right of this one; only the span of this same
-                    // element (the next record, if it is a span) can still reach back.
-                    if (rec.w == 0 || k + 1 >= re || records[k + 1].w != 0) break;
-                    continue;
+            const bool keep = rng.y > rng.x;
+            const unsigned long long m = __ballot(keep);
+            if (lane == 0) wcount[wave] = (int32_t)__popcll(m);
+            __syncthreads();
+            int off = 0, n = 0;
+#pragma unroll
+            for (int k = 0; k < kBlock / 64; ++k) {
+                if (k < wave) off += wcount[k];
+                n += wcount[k];
+            }
+            if (keep) {
+                const int slot = off + (int)__popcll(m & ((1ull << lane) - 1ull));
+                list[slot] = rng;
+                lcol[slot] = fill[p];
+            }
+            __syncthreads();
+            for (int q0 = 0; q0 < n; q0 += kBatch) {
+                const int qn = n - q0 < kBatch ? n - q0 : kBatch;
+                // the batch's records as one flat index space spread over all threads
+                if (wave == 0) {
+                    const int2 rg = lane < qn ? list[q0 + lane] : make_int2(0, 0);
+                    int inc = rg.y - rg.x;
+#pragma unroll
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const int u = __shfl_up(inc, d);
+                        if (lane >= d) inc += u;
+                    }
+                    bpre[lane + 1] = inc;
+                    if (lane == 0) bpre[0] = 0;
                 }
-                const int xe = x + rec.y;
-                const uint32_t col = (uint32_t)rec.z;
-                if (COMPOSE == 0) {
+                __syncthreads();
+                const int total = bpre[qn];
+                for (int f0 = tid; f0 < total; f0 += kBlock * kUnroll) {
+                    int4 rec[kUnroll];
+                    int qi[kUnroll];
 #pragma unroll
-                    for (int j = 0; j < CW; ++j)
-                        if (c0 + j >= x && c0 + j < xe) pix[j] = col;
-                } else {
-                    const float s0 = (float)(col & 0xFFu) / 255.0f, s1 = (float)((col >> 8) & 0xFFu) / 255.0f;
-                    const float s2 = (float)((col >> 16) & 0xFFu) / 255.0f, a = (float)((col >> 24) & 0xFFu) / 255.0f;
-                    const float ia = 1.0f - a;
+                    for (int u = 0; u < kUnroll; ++u) {
+                        const int f = f0 + u * kBlock;
+                        qi[u] = -1;
+                        if (f < total) {
+                            int lo = 0, hi = qn;
+                            while (hi - lo > 1) {
+                                const int mid = (lo + hi) >> 1;
+                                if (bpre[mid] <= f) lo = mid; else hi = mid;
+                            }
+                            qi[u] = lo;
+                            rec[u] = records[list[q0 + lo].x + (f - bpre[lo])];
+                        }
+                    }
 #pragma unroll
-                    for (int j = 0; j < CW; ++j) {
-                        if (c0 + j >= x && c0 + j < xe) {
+                    for (int u = 0; u < kUnroll; ++u) {
+                        if (qi[u] < 0) continue;
+                        const unsigned long long bit = 1ull << qi[u];
+                        int xs = (rec[u].x & 0xFFFF) - c0;
+                        int xe = xs + rec[u].y;
+                        xs = xs < 0 ? 0 : xs;
+                        xe = xe > cw ? cw : xe;
+                        for (int x = xs; x < xe; ++x) atomicOr(&mask[x], bit);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < kPix; ++j) {
+                    const int x = j * kBlock + tid;
+                    unsigned long long mm = mask[x];
+                    if (mm == 0ull) continue;
+                    mask[x] = 0ull;
+                    if constexpr (COMPOSE == 0) {
+                        pix[j] = lcol[q0 + 63 - __clzll((long long)mm)];
+                    } else {
+                        while (mm) {
+                            const int q = __ffsll((long long)mm) - 1;
+                            mm &= mm - 1ull;
+                            const uint32_t col = lcol[q0 + q];
+                            const float s0 = (float)(col & 0xFFu) / 255.0f, s1 = (float)((col >> 8) & 0xFFu) / 255.0f;
+                            const float s2 = (float)((col >> 16) & 0xFFu) / 255.0f;
+                            const float a = (float)((col >> 24) & 0xFFu) / 255.0f;
+                            const float ia = 1.0f - a;
                             acc[j][0] = s0 * a + acc[j][0] * ia;
                             acc[j][1] = s1 * a + acc[j][1] * ia;
                             acc[j][2] = s2 * a + acc[j][2] * ia;
@@ -164,85 +145,43 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
                         }
                     }
                 }
+                __syncthreads();
             }
         }
-        if (COMPOSE) {
+        if constexpr (COMPOSE != 0) {
 #pragma unroll
-            for (int j = 0; j < CW; ++j) {
+            for (int j = 0; j < kPix; ++j) {
                 uint32_t o = 0;
 #pragma unroll
                 for (int ch = 0; ch < 4; ++ch) {
-                    float v = acc[j][ch] * 255.0f + 0.5f;
-                    v = v < 0.0f ? 0.0f : (v > 255.0f ? 255.0f : v);
-                    o |= (uint32_t)v << (8 * ch);
+                    float q = acc[j][ch] * 255.0f + 0.5f;
+                    q = q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q);
+                    o |= (uint32_t)q << (8 * ch);
                 }
                 pix[j] = o;
             }
         }
         for (int yy = y; yy < y + 2 && yy < h; ++yy) {
-            uint32_t* row = reinterpret_cast<uint32_t*>(image) + (int64_t)(h - 1 - yy) * w;
+            uint32_t* row = reinterpret_cast<uint32_t*>(image) + (int64_t)(h - 1 - yy) * w + c0;
 #pragma unroll
-            for (int j = 0; j < CW; ++j)
-                if (c0 + j < w) row[c0 + j] = pix[j];
+            for (int j = 0; j < kPix; ++j) {
+                const int x = j * kBlock + tid;
+                if (x < cw) row[x] = pix[j];
+            }
         }
+        __syncthreads();
     }
-}
-
-inline unsigned stride_grid(int64_t n) {
-    int64_t g = (n + kBlock - 1) / kBlock;
-    if (g > 8192) g = 8192;
-    return (unsigned)(g < 1 ? 1 : g);
 }
 
 }  // namespace
 
-hipError_t launch_run_flags(const int4* records, const uint32_t* rec_path, const int32_t* totals,
-                            int64_t max_records, int32_t* run_flag, hipStream_t s) {
-    k_run_flags<<<stride_grid(max_records), kBlock, 0, s>>>(records, rec_path, totals, max_records, run_flag);
-    return hipGetLastError();
-}
-
-hipError_t launch_run_emit(const int4* records, const int32_t* run_scan, const int32_t* totals,
-                           int64_t max_records, int2* runs, hipStream_t s) {
-    k_run_emit<<<stride_grid(max_records), kBlock, 0, s>>>(records, run_scan, totals, max_records, runs);
-    return hipGetLastError();
-}
-
-hipError_t launch_row_count(const int2* runs, const int32_t* run_total, int64_t max_runs,
-                            int32_t* row_count, hipStream_t s) {
-    k_row_count<<<stride_grid(max_runs), kBlock, 0, s>>>(runs, run_total, row_count);
-    return hipGetLastError();
-}
-
-hipError_t launch_row_scatter(const int2* runs, const int32_t* run_total, int64_t max_runs,
-                              const int32_t* row_start, int32_t* row_fill, int32_t* row_runs,
-                              hipStream_t s) {
-    k_row_scatter<<<stride_grid(max_runs), kBlock, 0, s>>>(runs, run_total, row_start, row_fill, row_runs);
-    return hipGetLastError();
-}
-
-hipError_t launch_composite(const int4* records, const int2* runs, const int32_t* run_total,
-                            const int32_t* totals, const int32_t* row_start,
-                            const int32_t* row_runs, int w, int h, int compose, uint8_t* image,
-                            hipStream_t s) {
+hipError_t launch_composite(const int4* records, const int2* brec, const int32_t* bucket_base,
+                            const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
+                            uint8_t* image, hipStream_t s) {
     const int rows = (h + 1) / 2;
     if (rows == 0 || w == 0) return hipSuccess;
-    int32_t* rr = const_cast<int32_t*>(row_runs);
-    if (w <= 256 * 4) {
-        if (compose) k_composite<4, 1><<<rows, kBlock, 0, s>>>(records, runs, run_total, totals, row_start, rr, w, h, image);
-        else k_composite<4, 0><<<rows, kBlock, 0, s>>>(records, runs, run_total, totals, row_start, rr, w, h, image);
-    } else if (w <= 256 * 8) {
-        if (compose) k_composite<8, 1><<<rows, kBlock, 0, s>>>(records, runs, run_total, totals, row_start, rr, w, h, image);
-        else k_composite<8, 0><<<rows, kBlock, 0, s>>>(records, runs, run_total, totals, row_start, rr, w, h, image);
-    } else if (w <= 256 * 16) {
-        if (compose) k_composite<16, 1><<<rows, kBlock, 0, s>>>(records, runs, run_total, totals, row_start, rr, w, h, image);
-        else k_composite<16, 0><<<rows, kBlock, 0, s>>>(records, runs, run_total, totals, row_start, rr, w, h, image);
-    } else if (w <= 256 * 32) {
-        if (compose) k_composite<32, 1><<<rows, kBlock, 0, s>>>(records, runs, run_total, totals, row_start, rr, w, h, image);
-        else k_composite<32, 0><<<rows, kBlock, 0, s>>>(records, runs, run_total, totals, row_start, rr, w, h, image);
-    } else {
-        return hipErrorInvalidValue;
-    }
+    if (compose) k_composite<1><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, image);
+    else k_composite<0><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, image);
     return hipGetLastError();
 }
 

@@ -31,14 +31,19 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
 
 // transform_pos.comp:33-85.  The per-path outcode OR is reduced inside the wave first (points of
 // one path are contiguous), then one atomicOr per (wave, path).
+// It also bounds each path's cell rows from its transformed control points (the curves stay
+// inside their hull; the margin of 1 px plus 2^-20 of the magnitude covers the rounding of the
+// curve evaluation), which sizes the path's (path, row) buckets of the sort.
 __global__ __launch_bounds__(kBlock) void k_transform(const float2* __restrict__ pos,
                                                       const uint32_t* __restrict__ pos_path,
-                                                      uint32_t P, Mvp m, float fw, float fh,
+                                                      uint32_t P, Mvp m, float fw, float fh, int rmax,
                                                       float2* __restrict__ tpos,
-                                                      int32_t* __restrict__ visible) {
+                                                      int32_t* __restrict__ visible,
+                                                      int32_t* __restrict__ row_lo,
+                                                      int32_t* __restrict__ row_hi) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < P;
-    int32_t bits = 0;
+    int32_t bits = 0, lo = 0, hi = 0;
     uint32_t path = 0xFFFFFFFFu;
     if (valid) {
         float2 p = pos[i];
@@ -50,6 +55,12 @@ __global__ __launch_bounds__(kBlock) void k_transform(const float2* __restrict__
         tpos[i] = make_float2(ox, oy);
         bits = outcode_bits(ox, oy, fw, fh);
         path = pos_path[i];
+        const float mg = 1.0f + fabsf(oy) * 0x1p-20f;
+        lo = sat_f2i(floorf((oy - mg) * 0.5f));
+        hi = sat_f2i(floorf((oy + mg) * 0.5f));
+        lo = lo < -1 ? -1 : (lo > rmax + 1 ? rmax + 1 : lo);
+        hi = hi < -1 ? -1 : (hi > rmax + 1 ? rmax + 1 : hi);
+        if (!(oy == oy)) { lo = -1; hi = rmax + 1; }
     }
     unsigned long long todo = __ballot(valid && bits != 0);
     while (todo) {
@@ -58,9 +69,18 @@ __global__ __launch_bounds__(kBlock) void k_transform(const float2* __restrict__
         const bool match = valid && bits != 0 && path == lp;
         todo &= ~__ballot(match);
         int32_t v = match ? bits : 0;
+        int32_t vlo = match ? lo : 0x7FFFFFFF, vhi = match ? hi : (int32_t)0x80000000;
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o);
-        if ((int)(threadIdx.x & 63) == leader) atomicOr(&visible[lp], v);
+        for (int o = 32; o >= 1; o >>= 1) {
+            v |= __shfl_xor(v, o);
+            vlo = min(vlo, __shfl_xor(vlo, o));
+            vhi = max(vhi, __shfl_xor(vhi, o));
+        }
+        if ((int)(threadIdx.x & 63) == leader) {
+            atomicOr(&visible[lp], v);
+            atomicMin(&row_lo[lp], vlo);
+            atomicMax(&row_hi[lp], vhi);
+        }
     }
 }
 
@@ -238,17 +258,13 @@ __global__ __launch_bounds__(kBlock) void k_piece_merge(
 }
 
 // gen_fragment.comp:90-224: cell key of the segment between consecutive intersections of one
-// curve, and its winding change across the cell's centre row.
-__global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict__ inter, int64_t N,
-                                                         const uint32_t* __restrict__ cpos,
-                                                         const uint32_t* __restrict__ ctype,
-                                                         const uint32_t* __restrict__ cpath,
-                                                         const float2* __restrict__ tpos, int w,
-                                                         int h, int32_t* __restrict__ key,
-                                                         int8_t* __restrict__ wn,
-                                                         uint32_t* __restrict__ epath) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= N) return;
+// curve, and its winding change across the cell's centre row.  Returns the entry's bucket.
+__device__ inline uint32_t gen_one(const int2* __restrict__ inter, int64_t N, int64_t i,
+                                   const uint32_t* __restrict__ cpos, const uint32_t* __restrict__ ctype,
+                                   const uint32_t* __restrict__ cpath, const float2* __restrict__ tpos, int w,
+                                   int h, int32_t* __restrict__ key, int8_t* __restrict__ wn,
+                                   uint32_t* __restrict__ epath, const int32_t* __restrict__ bucket_base,
+                                   const int2* __restrict__ prow, int32_t* __restrict__ err) {
     const int2 a = inter[i];
     const int2 b = (i + 1 != N) ? inter[i + 1] : make_int2(-1, 0x3f800000);
     float t0 = glmax(bitsf((uint32_t)a.y & 0xFFFFFFFCu), 0.0f);
@@ -276,6 +292,28 @@ __global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict_
     key[i] = (int32_t)yx;
     wn[i] = (int8_t)swn;
     epath[i] = path;
+    return (uint32_t)bucket_base[path] + bucket_local((int32_t)yx, prow[path], err);
+}
+
+
+// Keys of all entries, counted per (path, row) bucket while they are produced.
+__global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict__ inter, int64_t N,
+                                                         const uint32_t* __restrict__ cpos,
+                                                         const uint32_t* __restrict__ ctype,
+                                                         const uint32_t* __restrict__ cpath,
+                                                         const float2* __restrict__ tpos, int w,
+                                                         int h, int32_t* __restrict__ key,
+                                                         int8_t* __restrict__ wn,
+                                                         uint32_t* __restrict__ epath,
+                                                         const int32_t* __restrict__ bucket_base,
+                                                         const int2* __restrict__ prow,
+                                                         int32_t* __restrict__ hist,
+                                                         int32_t* __restrict__ err) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < N;
+    uint32_t bucket = 0;
+    if (valid) bucket = gen_one(inter, N, i, cpos, ctype, cpath, tpos, w, h, key, wn, epath, bucket_base, prow, err);
+    wave_count(hist, bucket, valid);
 }
 
 }  // namespace
@@ -289,9 +327,11 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
 }
 
 hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
-                            int w, int h, float2* tpos, int32_t* visible, hipStream_t s) {
+                            int w, int h, float2* tpos, int32_t* visible, int32_t* row_lo, int32_t* row_hi,
+                            hipStream_t s) {
     if (P == 0) return hipSuccess;
-    k_transform<<<grid_for(P), kBlock, 0, s>>>(pos, pos_path, P, m, (float)w, (float)h, tpos, visible);
+    k_transform<<<grid_for(P), kBlock, 0, s>>>(pos, pos_path, P, m, (float)w, (float)h, (h - 1) / 2, tpos,
+                                               visible, row_lo, row_hi);
     return hipGetLastError();
 }
 
@@ -339,9 +379,11 @@ hipError_t launch_piece_merge(const float2* tpos, const uint32_t* cpos, const ui
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
+                               const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
                                hipStream_t s) {
     if (N == 0) return hipSuccess;
-    k_gen_fragment<<<grid_for(N), kBlock, 0, s>>>(inter, N, cpos, ctype, cpath, tpos, w, h, key, wn, epath);
+    k_gen_fragment<<<grid_for(N), kBlock, 0, s>>>(inter, N, cpos, ctype, cpath, tpos, w, h, key, wn, epath,
+                                                  bucket_base, prow, hist, err);
     return hipGetLastError();
 }
 

@@ -86,8 +86,8 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     const uint32_t* __restrict__ epath, const uint32_t* __restrict__ rule,
     const uint32_t* __restrict__ fill, int64_t N, int w, int h,
     unsigned long long* __restrict__ status, uint32_t* __restrict__ counter,
-    int4* __restrict__ records, uint32_t* __restrict__ rec_path, int32_t* __restrict__ flags,
-    int32_t* __restrict__ totals) {
+    int4* __restrict__ records, int32_t* __restrict__ flags, int32_t* __restrict__ totals,
+    const int32_t* __restrict__ bucket_base, const int2* __restrict__ prow, int2* __restrict__ brec) {
     __shared__ int32_t s_tile;
     __shared__ unsigned long long s_w[kWaves];
     __shared__ unsigned long long s_prefix;
@@ -139,6 +139,10 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     int32_t spans_before = (int32_t)(off >> 32);
     if (i0 < N) {
         int32_t kp = (i0 > 0) ? skey[i0 - 1] : 0;
+        uint32_t pp = (i0 > 0) ? (epath[i0 - 1] & 0x3FFFFFFFu) : 0xFFFFFFFFu;
+        const int64_t inext = i0 + kScanItems;
+        const int32_t knext_out = inext < N ? skey[inext] : 0;
+        const uint32_t pnext_out = inext < N ? (epath[inext] & 0x3FFFFFFFu) : 0xFFFFFFFFu;
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t f = i0 + j;
@@ -153,7 +157,6 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
                 if (cell) {
                     const int x1 = key_x(k1), y1 = key_y(k1);
                     records[o] = make_int4((int32_t)((uint32_t)y1 << 16) | x1, 2, colour, cells_before + 1);
-                    rec_path[o] = p1;
                 }
                 if (span) {
                     const int y0 = key_y(kp);
@@ -161,12 +164,30 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
                     const int x1 = key_x(k1);
                     x0 = max(0, x0);
                     records[o + cell] = make_int4((int32_t)((uint32_t)y0 << 16) | x0, x1 - x0, colour, 0);
-                    rec_path[o + cell] = p1;
+                }
+            }
+            // record range of this element's (path, cell row) bucket, for the composite
+            const int y1 = key_y(k1);
+            if ((uint32_t)k1 != 0xFFFEFFFEu && y1 >= 0 && y1 < h) {
+                const int32_t kn = j + 1 < kScanItems ? keys[j + 1 < kScanItems ? j + 1 : j] : knext_out;
+                const uint32_t pn = j + 1 < kScanItems ? paths[j + 1 < kScanItems ? j + 1 : j] : pnext_out;
+                const bool first = (f == 0) || pp != p1 || (kp >> 16) != (k1 >> 16);
+                const bool last = (f == N - 1) || pn != p1 || (kn >> 16) != (k1 >> 16);
+                if (first || last) {
+                    const int2 pr = prow[p1];
+                    const int r = y1 >> 1;
+                    const int loc = r == 0 ? pr.y + 1 : r - pr.x;
+                    if (loc >= 0 && loc < pr.y + 2) {  // out of range only after a flagged bound error
+                        int2* br = &brec[bucket_base[p1] + loc];
+                        if (first) br->x = cells_before + spans_before;
+                        if (last) br->y = cells_before + spans_before + cell + span;
+                    }
                 }
             }
             cells_before += cell;
             spans_before += span;
             kp = k1;
+            pp = p1;
         }
     }
     if (threadIdx.x == 0 && base <= N - 1 && N - 1 < base + kScanTile) {
@@ -187,12 +208,12 @@ hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl c
 
 hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,
                             const uint32_t* rule, const uint32_t* fill, int64_t N, int w, int h,
-                            ScanCtl ctl, int4* records, uint32_t* rec_path, int32_t* flags,
-                            int32_t* totals, hipStream_t s) {
+                            ScanCtl ctl, int4* records, int32_t* flags, int32_t* totals,
+                            const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s) {
     if (N == 0) return hipSuccess;
     k_mark_emit<<<(unsigned)scan_tiles(N), kScanThreads, 0, s>>>(skey, winding, epath, rule, fill, N, w, h,
-                                                                 ctl.status, ctl.counter, records, rec_path,
-                                                                 flags, totals);
+                                                                 ctl.status, ctl.counter, records, flags,
+                                                                 totals, bucket_base, prow, brec);
     return hipGetLastError();
 }
 

@@ -1,16 +1,17 @@
-// k_sort.hip -- per-path segmented sort of the cell keys (gfx950).
+// k_sort.hip -- sort of the cell keys inside each (path, row) band (gfx950).
 //
 // Replaces naive_seg_sort_pairs.comp:26-97 (one workgroup per path running an O(n^2) odd-even
 // transposition network) and shuffle_fragment.comp:17-26.  The reference compares (key, index)
-// as signed int32 inside each path segment; here the key is biased (key ^ 0x80000000) so the
-// unsigned order equals the signed one, and the sort word is
-//     (biased key << 32) | (index << 2) | (winding change + 1)
-// which is unique, orders exactly like (key, index), and carries the winding change so the
-// reference's separate gather pass disappears.
+// as signed int32 inside each path segment; the sort word here is
+//     ((key ^ 0x80000000) << 32) | (index << 2) | (winding change + 1)
+// whose unsigned order equals the reference's (key, index) order, is unique, and carries the
+// winding change, so the reference's separate gather pass disappears.
 //
-// Segments are cut into 4096-entry tiles.  A tile is loaded once from HBM, sorted in LDS (a
-// 16-element register network per thread, then merge-path rounds in LDS) and stored once.
-// Segments longer than one tile are finished by global merge-path passes.
+// After the bucket scatter every band (<= 4096 entries, a run of whole buckets of one path) only
+// needs sorting in place: one 256-thread workgroup loads it once, sorts it in LDS (16-element
+// register network per thread, then merge-path rounds) and stores it once.  A band above 4096
+// entries is a single oversized (path, row) bucket; such bands are queued and each is sorted by
+// one workgroup with LDS chunk sorts followed by merge-path passes through global memory.
 #include "kernels.hpp"
 
 namespace spr {
@@ -22,10 +23,6 @@ constexpr int kItems = 16;
 static_assert(kThreads * kItems == kSortTile, "tile shape");
 typedef unsigned long long u64;
 constexpr u64 kPad = ~0ull;
-
-__device__ inline u64 sort_word(int32_t key, int64_t idx, int8_t wn) {
-    return ((u64)((uint32_t)key ^ 0x80000000u) << 32) | ((u64)idx << 2) | (u64)(uint32_t)(wn + 1);
-}
 
 __device__ inline void cswap(u64& a, u64& b) {
     u64 lo = a < b ? a : b, hi = a < b ? b : a;
@@ -55,64 +52,23 @@ __device__ inline void sort16(u64 (&v)[kItems]) {
 // Merge-path split: number of elements taken from A among the first `diag` outputs of the
 // stable merge of A (length na) and B (length nb).
 template <class PA, class PB>
-__device__ inline int merge_split(PA A, int na, PB B, int nb, int diag) {
-    int lo = diag > nb ? diag - nb : 0;
-    int hi = diag < na ? diag : na;
+__device__ inline int64_t merge_split(PA A, int64_t na, PB B, int64_t nb, int64_t diag) {
+    int64_t lo = diag > nb ? diag - nb : 0;
+    int64_t hi = diag < na ? diag : na;
     while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
+        const int64_t mid = (lo + hi) >> 1;
         if (A[mid] <= B[diag - 1 - mid]) lo = mid + 1; else hi = mid;
     }
     return lo;
 }
 
-// Maps a block to (path, tile of that path) through the exclusive tile prefix.
-__device__ inline bool block_tile(const int32_t* tile_prefix, uint32_t Np, int64_t b, uint32_t& p, int& t) {
-    __shared__ int32_t s_p;
-    if (threadIdx.x == 0) {
-        const int32_t total = tile_prefix[Np];
-        int32_t res = -1;
-        if (b < total) {
-            uint32_t lo = 0, hi = Np;  // largest p with tile_prefix[p] <= b
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (tile_prefix[mid] <= b) lo = mid; else hi = mid;
-            }
-            while (lo + 1 <= Np && tile_prefix[lo + 1] <= b) ++lo;
-            res = (int32_t)lo;
-        }
-        s_p = res;
-    }
-    __syncthreads();
-    const int32_t r = s_p;
-    __syncthreads();
-    if (r < 0) return false;
-    p = (uint32_t)r;
-    t = (int)(b - tile_prefix[p]);
-    return true;
-}
+// LDS slot of element i: one pad word per 16 elements, so the 16-element runs owned by
+// consecutive lanes start on different banks.
+constexpr int kPadded = kSortTile + kSortTile / 16;
+__device__ inline int pad(int i) { return i + (i >> 4); }
 
-__global__ void k_tile_counts(const int32_t* __restrict__ seg, uint32_t Np, int32_t* __restrict__ tile_counts) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= Np) return;
-    const int32_t n = seg[p + 1] - seg[p];
-    tile_counts[p] = (n + kSortTile - 1) / kSortTile;
-}
-
-__global__ __launch_bounds__(kThreads) void k_sort_tiles(const int32_t* __restrict__ key,
-                                                         const int8_t* __restrict__ wn,
-                                                         const int32_t* __restrict__ seg,
-                                                         const int32_t* __restrict__ tile_prefix,
-                                                         uint32_t Np, u64* __restrict__ out_final,
-                                                         u64* __restrict__ out_big) {
-    __shared__ u64 buf[2][kSortTile];
-    uint32_t p;
-    int t;
-    if (!block_tile(tile_prefix, Np, blockIdx.x, p, t)) return;
-    const int64_t s = seg[p], e = seg[p + 1];
-    const int64_t len = e - s;
-    const int64_t b0 = s + (int64_t)t * kSortTile;
-    const int n = (int)(len - (int64_t)t * kSortTile < kSortTile ? len - (int64_t)t * kSortTile : kSortTile);
-    u64* out = len > kSortTile ? out_big : out_final;
+// Sorts n <= 4096 words from src into dst with one workgroup (buf: 2 x kPadded words of LDS).
+__device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ dst, int n, u64 (*buf)[kPadded]) {
     int P2 = kItems;
     while (P2 < n) P2 <<= 1;
     const int active = P2 / kItems;
@@ -122,72 +78,113 @@ __global__ __launch_bounds__(kThreads) void k_sort_tiles(const int32_t* __restri
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const int i = tid * kItems + j;
-            v[j] = i < n ? sort_word(key[b0 + i], b0 + i, wn[b0 + i]) : kPad;
+            v[j] = i < n ? src[i] : kPad;
         }
         sort16(v);
 #pragma unroll
-        for (int j = 0; j < kItems; ++j) buf[0][tid * kItems + j] = v[j];
+        for (int j = 0; j < kItems; ++j) buf[0][pad(tid * kItems + j)] = v[j];
     }
     __syncthreads();
-    int src = 0;
+    int cur = 0;
     for (int run = kItems; run < P2; run <<= 1) {
         if (tid < active) {
             const int o = tid * kItems;
             const int pb = (o / (2 * run)) * (2 * run);
-            const u64* A = &buf[src][pb];
-            const u64* B = A + run;
+            const u64* S = buf[cur];
+            auto A = [&](int i) { return S[pad(pb + i)]; };
+            auto B = [&](int j) { return S[pad(pb + run + j)]; };
             const int diag = o - pb;
-            int i = merge_split(A, run, B, run, diag);
-            int j = diag - i;
-            u64* D = &buf[src ^ 1][o];
+            int lo = diag > run ? diag - run : 0, hi = diag < run ? diag : run;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (A(mid) <= B(diag - 1 - mid)) lo = mid + 1; else hi = mid;
+            }
+            int i = lo, j = diag - lo;
+            u64* D = buf[cur ^ 1];
+            u64 a = i < run ? A(i) : kPad, b = j < run ? B(j) : kPad;
 #pragma unroll
             for (int k = 0; k < kItems; ++k) {
-                const bool takeA = j >= run || (i < run && A[i] <= B[j]);
-                D[k] = takeA ? A[i] : B[j];
-                i += takeA; j += !takeA;
+                const bool takeA = j >= run || (i < run && a <= b);
+                D[pad(o + k)] = takeA ? a : b;
+                if (takeA) { ++i; a = i < run ? A(i) : kPad; } else { ++j; b = j < run ? B(j) : kPad; }
             }
         }
         __syncthreads();
-        src ^= 1;
+        cur ^= 1;
     }
-    for (int i = tid; i < n; i += kThreads) out[b0 + i] = buf[src][i];
+    for (int i = tid; i < n; i += kThreads) dst[i] = buf[cur][pad(i)];
+    __syncthreads();
 }
 
-// One merge pass over every segment longer than (kSortTile << pass): runs of length
-// r = kSortTile << pass are merged pairwise from src into dst, or into out_final on the
-// segment's last pass.  Blocks map to 4096-output slices of the segments.
-__global__ __launch_bounds__(kThreads) void k_merge_pass(const int32_t* __restrict__ seg,
-                                                         const int32_t* __restrict__ tile_prefix,
-                                                         uint32_t Np, int pass,
-                                                         const u64* __restrict__ src,
-                                                         u64* __restrict__ dst,
-                                                         u64* __restrict__ out_final) {
-    uint32_t p;
-    int t;
-    if (!block_tile(tile_prefix, Np, blockIdx.x, p, t)) return;
-    const int64_t s = seg[p], len = seg[p + 1] - s;
-    const int64_t r = (int64_t)kSortTile << pass;
-    if (len <= r) return;
-    u64* out = (2 * r >= len) ? out_final : dst;
-    const int64_t o = (int64_t)t * kSortTile + (int64_t)threadIdx.x * kItems;
-    if (o >= len) return;
-    const int64_t pb = (o / (2 * r)) * (2 * r);
-    const int64_t na = (pb + r < len ? r : len - pb);
-    const int64_t nb = pb + r < len ? ((pb + 2 * r < len ? pb + 2 * r : len) - (pb + r)) : 0;
-    const u64* A = src + s + pb;
-    const u64* B = A + na;
-    const int cnt = (int)(len - o < kItems ? len - o : kItems);
-    if (nb == 0) {
-        for (int k = 0; k < cnt; ++k) out[s + o + k] = src[s + o + k];
+__global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restrict__ band_start,
+                                                         const int32_t* __restrict__ n_bands,
+                                                         const u64* __restrict__ src, u64* __restrict__ dst,
+                                                         int32_t* __restrict__ big_list,
+                                                         int32_t* __restrict__ big_count) {
+    __shared__ u64 buf[2][kPadded];
+    const int32_t b = blockIdx.x;
+    if (b >= *n_bands) return;
+    const int64_t s = band_start[b], e = band_start[b + 1];
+    const int64_t n = e - s;
+    if (n <= 0) return;
+    if (n > kSortTile) {
+        if (threadIdx.x == 0) big_list[atomicAdd(big_count, 1)] = b;
         return;
     }
-    const int diag = (int)(o - pb);
-    int i = merge_split(A, (int)na, B, (int)nb, diag);
-    int j = diag - i;
-    for (int k = 0; k < cnt; ++k) {
-        const bool takeA = j >= nb || (i < na && A[i] <= B[j]);
-        out[s + o + k] = takeA ? A[i] : B[j];
-        i += takeA; j += !takeA;
+    if (n == 1) {
+        if (threadIdx.x == 0) dst[s] = src[s];
+        return;
+    }
+    tile_sort(src + s, dst + s, (int)n, buf);
+}
+
+// One workgroup per oversized band: LDS chunk sorts into `a`, then merge-path passes between
+// `a` and `b` (the scatter buffer, already consumed), the last pass writing `dst`.
+__global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __restrict__ band_start,
+                                                             const int32_t* __restrict__ big_list,
+                                                             const int32_t* __restrict__ big_count,
+                                                             u64* __restrict__ scatter, u64* __restrict__ a,
+                                                             u64* __restrict__ dst) {
+    __shared__ u64 buf[2][kPadded];
+    const int32_t nbig = *big_count;
+    for (int32_t q = blockIdx.x; q < nbig; q += gridDim.x) {
+        const int32_t band = big_list[q];
+        const int64_t s = band_start[band], n = band_start[band + 1] - s;
+        for (int64_t c = 0; c < n; c += kSortTile) {
+            const int m = (int)(n - c < kSortTile ? n - c : kSortTile);
+            tile_sort(scatter + s + c, a + s + c, m, buf);
+        }
+        __threadfence_block();
+        __syncthreads();
+        u64* cur = a + s;
+        u64* nxt = scatter + s;
+        for (int64_t r = kSortTile; r < n; r <<= 1) {
+            u64* out = (2 * r >= n) ? dst + s : nxt;
+            for (int64_t o = (int64_t)threadIdx.x * kItems; o < n; o += (int64_t)kThreads * kItems) {
+                const int64_t pb = (o / (2 * r)) * (2 * r);
+                const int64_t na = pb + r < n ? r : n - pb;
+                const int64_t nb = pb + r < n ? ((pb + 2 * r < n ? pb + 2 * r : n) - (pb + r)) : 0;
+                const u64* A = cur + pb;
+                const u64* B = A + na;
+                const int cnt = (int)(n - o < kItems ? n - o : kItems);
+                const int64_t diag = o - pb;
+                int64_t i = nb ? merge_split(A, na, B, nb, diag) : diag;
+                int64_t j = diag - i;
+                for (int k = 0; k < cnt; ++k) {
+                    const bool takeA = j >= nb || (i < na && A[i] <= B[j]);
+                    out[o + k] = takeA ? A[i] : B[j];
+                    i += takeA; j += !takeA;
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+            if (2 * r >= n) { cur = nullptr; break; }
+            u64* t = cur; cur = nxt; nxt = t;
+        }
+        if (cur) {  // a single chunk: no merge pass ran
+            for (int64_t i = threadIdx.x; i < n; i += kThreads) dst[s + i] = cur[i];
+            __syncthreads();
+        }
     }
 }
 
@@ -203,27 +200,18 @@ __global__ void k_unpack_sorted(const u64* __restrict__ sorted, int64_t N, int32
 
 }  // namespace
 
-hipError_t launch_tile_counts(const int32_t* seg, uint32_t Np, int32_t* tile_counts, hipStream_t s) {
-    if (Np == 0) return hipSuccess;
-    k_tile_counts<<<(Np + 255) / 256, 256, 0, s>>>(seg, Np, tile_counts);
+hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
+                             const unsigned long long* src, unsigned long long* dst, int32_t* big_list,
+                             int32_t* big_count, hipStream_t s) {
+    if (max_bands <= 0) return hipSuccess;
+    k_sort_bands<<<(unsigned)max_bands, kThreads, 0, s>>>(band_start, n_bands, src, dst, big_list, big_count);
     return hipGetLastError();
 }
 
-hipError_t launch_sort_tiles(const int32_t* key, const int8_t* wn, const int32_t* seg,
-                             const int32_t* tile_prefix, uint32_t Np, int64_t max_tiles,
-                             unsigned long long* out_final, unsigned long long* out_big,
-                             hipStream_t s) {
-    if (max_tiles <= 0) return hipSuccess;
-    k_sort_tiles<<<(unsigned)max_tiles, kThreads, 0, s>>>(key, wn, seg, tile_prefix, Np, out_final, out_big);
-    return hipGetLastError();
-}
-
-hipError_t launch_merge_pass(const int32_t* seg, const int32_t* tile_prefix, uint32_t Np,
-                             int64_t max_tiles, int pass, const unsigned long long* src,
-                             unsigned long long* dst, unsigned long long* out_final,
-                             hipStream_t s) {
-    if (max_tiles <= 0) return hipSuccess;
-    k_merge_pass<<<(unsigned)max_tiles, kThreads, 0, s>>>(seg, tile_prefix, Np, pass, src, dst, out_final);
+hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
+                                 unsigned long long* scatter, unsigned long long* tmp, unsigned long long* dst,
+                                 hipStream_t s) {
+    k_sort_big_bands<<<64, kThreads, 0, s>>>(band_start, big_list, big_count, scatter, tmp, dst);
     return hipGetLastError();
 }
 

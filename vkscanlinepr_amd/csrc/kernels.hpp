@@ -24,7 +24,8 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 
 // ---- geometry (a3, a4, a7, a8)
 hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
-                            int w, int h, float2* tpos, int32_t* visible, hipStream_t s);
+                            int w, int h, float2* tpos, int32_t* visible, int32_t* row_lo, int32_t* row_hi,
+                            hipStream_t s);
 hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                         const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
                         float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s);
@@ -42,45 +43,42 @@ hipError_t launch_piece_merge(const float2* tpos, const uint32_t* cpos, const ui
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
+                               const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
                                hipStream_t s);
 
 // ---- scans
 hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s);
 
-// ---- segmented sort (a9 + a10): per path segment, ascending (key ^ 0x80000000, index); the
-// winding change rides along in the low bits of the 64-bit sort word.
+// ---- (path, row) buckets and the band sort (a9 + a10): per path segment, ascending
+// (key ^ 0x80000000, index); the winding change rides along in the low bits of the sort word.
 constexpr int kSortTile = 4096;
-hipError_t launch_tile_counts(const int32_t* seg, uint32_t Np, int32_t* tile_counts, hipStream_t s);
-hipError_t launch_sort_tiles(const int32_t* key, const int8_t* wn, const int32_t* seg,
-                             const int32_t* tile_prefix, uint32_t Np, int64_t max_tiles,
-                             unsigned long long* out_final, unsigned long long* out_big,
+hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, uint32_t Np, int rmax,
+                                int32_t* nb, int2* prow, hipStream_t s);
+hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
+                                 const int32_t* bucket_base, const int2* prow, const int32_t* boff,
+                                 int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s);
+hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
                              hipStream_t s);
-hipError_t launch_merge_pass(const int32_t* seg, const int32_t* tile_prefix, uint32_t Np,
-                             int64_t max_tiles, int pass, const unsigned long long* src,
-                             unsigned long long* dst, unsigned long long* out_final,
-                             hipStream_t s);
+hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,
+                            int32_t* band_start, hipStream_t s);
+hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
+                             const unsigned long long* src, unsigned long long* dst, int32_t* big_list,
+                             int32_t* big_count, hipStream_t s);
+hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
+                                 unsigned long long* scatter, unsigned long long* tmp, unsigned long long* dst,
+                                 hipStream_t s);
 hipError_t launch_unpack_sorted(const unsigned long long* sorted, int64_t N, int32_t* skey,
                                 int32_t* sidx, int32_t* swn, hipStream_t s);
 
-// ---- winding / mark / emit (a10 scan, a11)
+// ---- winding / mark / emit (a10 scan, a11); also writes each (path, row) bucket's record range
 hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,
                             const uint32_t* rule, const uint32_t* fill, int64_t N, int w, int h,
-                            ScanCtl ctl, int4* records, uint32_t* rec_path, int32_t* flags,
-                            int32_t* totals, hipStream_t s);
+                            ScanCtl ctl, int4* records, int32_t* flags, int32_t* totals,
+                            const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s);
 
 // ---- composite (a12)
-hipError_t launch_run_flags(const int4* records, const uint32_t* rec_path, const int32_t* totals,
-                            int64_t max_records, int32_t* run_flag, hipStream_t s);
-hipError_t launch_run_emit(const int4* records, const int32_t* run_scan, const int32_t* totals,
-                           int64_t max_records, int2* runs, hipStream_t s);
-hipError_t launch_row_count(const int2* runs, const int32_t* run_total, int64_t max_runs,
-                            int32_t* row_count, hipStream_t s);
-hipError_t launch_row_scatter(const int2* runs, const int32_t* run_total, int64_t max_runs,
-                              const int32_t* row_start, int32_t* row_fill, int32_t* row_runs,
-                              hipStream_t s);
-hipError_t launch_composite(const int4* records, const int2* runs, const int32_t* run_total,
-                            const int32_t* totals, const int32_t* row_start,
-                            const int32_t* row_runs, int w, int h, int compose, uint8_t* image,
-                            hipStream_t s);
+hipError_t launch_composite(const int4* records, const int2* brec, const int32_t* bucket_base,
+                            const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
+                            uint8_t* image, hipStream_t s);
 
 }  // namespace spr

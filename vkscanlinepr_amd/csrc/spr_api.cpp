@@ -98,7 +98,7 @@ struct spr_handle {
     DBuf<int32_t> visible;
     DBuf<float4> cuts;
     DBuf<uint32_t> ncuts;
-    DBuf<int32_t> counts, seg, tile_counts;
+    DBuf<int32_t> counts, seg;
     DBuf<int2> inter;
     DBuf<float> chain_t;
     DBuf<int32_t> key;
@@ -107,9 +107,8 @@ struct spr_handle {
     DBuf<unsigned long long> sorted, sbig, stmp;
     DBuf<int32_t> skey, sidx, swn, winding, flags;
     DBuf<int4> records;
-    DBuf<uint32_t> rec_path;
-    DBuf<int32_t> run_flag, row_start, row_fill, row_runs;
-    DBuf<int2> runs;
+    DBuf<int32_t> row_lo, row_hi, bucket_base, bflag, band_start, big_list;
+    DBuf<int2> prow;
     DBuf<uint8_t> image;
     DBuf<unsigned long long> zero;  // per-frame zeroed control region
     int32_t* h_pinned = nullptr;    // 16 ints of pinned host memory
@@ -123,47 +122,46 @@ struct spr_handle {
 
     // last frame
     int64_t N = 0;
-    int32_t max_len = 0;
     spr_stats stats{};
     bool frame_done = false;
 };
 
 namespace {
 
-// Zero region layout (in 64-bit words): [counters 8][count-scan status][... phase B ...]
+// Per-frame zeroed control region, in 64-bit words.  Phase A is cleared at frame start (before
+// the host sync), phase B once N and the bucket count NB are known.
 struct ZeroLayout {
     size_t a_words = 0, total = 0;
-    size_t ctr = 0, st_count = 0;
-    size_t st_tiles = 0, st_wind = 0, st_mark = 0, st_run = 0, st_rows = 0, row_count = 0, row_fill = 0, totals = 0, maxlen = 0;
+    size_t ctr_a = 0, st_count = 0, st_pbase = 0;
+    size_t ctr = 0, totals = 0, misc = 0, st_hist = 0, st_band = 0, st_wind = 0, st_mark = 0;
+    size_t hist = 0, fill = 0, brec = 0;
 };
 
-// Phase A (zeroed at frame start): count-scan counter, max segment length, count-scan status.
-// Phase B (zeroed after the host sync, when N is known): counters of the later scans, totals,
-// the later scans' status arrays and the per-row counters of the composite.
-ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, uint32_t rows) {
+ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
     ZeroLayout z;
     size_t o = 0;
-    z.maxlen = o; o += 1;              // int32 max segment length + the count-scan counter
+    z.ctr_a = o; o += 1;               // two u32 counters of the phase-A scans
     z.st_count = o; o += spr::scan_tiles(std::max<int64_t>(C, 1));
+    z.st_pbase = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
     z.a_words = o;
-    z.ctr = o; o += 8;                 // 16 u32 counters of the phase-B scans
-    z.totals = o; o += 2;              // 4 int32: cells, spans, R, spare
-    z.st_tiles = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
+    z.ctr = o; o += 4;                 // eight u32 counters of the phase-B scans
+    z.totals = o; o += 2;              // int32 cells, spans, R, spare
+    z.misc = o; o += 1;                // int32 error flag, big-band count
+    z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
+    z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
     z.st_wind = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
     z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
-    z.st_run = o; o += spr::scan_tiles(std::max<int64_t>(2 * N, 1));
-    z.st_rows = o; o += spr::scan_tiles(std::max<uint32_t>(rows, 1));
-    z.row_count = o; o += (rows + 2 + 1) / 2;
-    z.row_fill = o; o += (rows + 1) / 2 + 1;
+    z.hist = o; o += (size_t)(NB + 2 + 1) / 2;  // NB+1 int32 (scanned in place)
+    z.fill = o; o += (size_t)(NB + 1) / 2;      // NB int32
+    z.brec = o; o += (size_t)NB;                // NB int2
     z.total = o;
     return z;
 }
 
-spr::ScanCtl ctl_at(spr_handle* H, const ZeroLayout& z, size_t st, int counter) {
+spr::ScanCtl ctl_at(spr_handle* H, size_t st, size_t ctr_word, int counter) {
     spr::ScanCtl c;
     c.status = H->zero.p + st;
-    c.counter = counter == 0 ? reinterpret_cast<uint32_t*>(H->zero.p + z.maxlen) + 1
-                             : reinterpret_cast<uint32_t*>(H->zero.p + z.ctr) + counter;
+    c.counter = reinterpret_cast<uint32_t*>(H->zero.p + ctr_word) + counter;
     c.epoch = 0;
     return c;
 }
@@ -180,42 +178,41 @@ void run_frame(spr_handle* H) {
     hipStream_t s = H->stream;
     const uint32_t P = H->P, C = H->C, Np = H->Np;
     const int w = (int)H->w, h = (int)H->h;
-    const uint32_t rows = (H->h + 1) / 2;
     const bool debug = H->opts.debug != 0;
 
     SPR_HIP(hipEventRecord(H->ev_begin, s));
     H->n_marks = 0;
     stage_mark(H, 0);
-    // ---- phase A: zero the visibility mask and the count-scan control
-    ZeroLayout za = zero_layout(C, Np, std::max<int64_t>(H->N, 0), rows);
-    H->zero.ensure(std::max(za.total, za.a_words));
+    // ---- phase A: transform + cull + row bounds, count pass, bucket tables
+    const ZeroLayout za = zero_layout(C, Np, 0, 0);
+    H->zero.ensure(za.a_words);
     H->visible.ensure(std::max<uint32_t>(Np, 1));
+    H->row_lo.ensure(std::max<uint32_t>(Np, 1));
+    H->row_hi.ensure(std::max<uint32_t>(Np, 1));
+    H->prow.ensure(std::max<uint32_t>(Np, 1));
+    H->bucket_base.ensure((size_t)Np + 1);
     H->tpos.ensure(std::max<uint32_t>(P, 1));
     H->cuts.ensure(std::max<uint32_t>(C, 1));
     H->ncuts.ensure(std::max<uint32_t>(C, 1));
     H->counts.ensure((size_t)C + 1);
     H->seg.ensure((size_t)Np + 1);
-    H->tile_counts.ensure((size_t)Np + 1);
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->zero.p, 2 * (int64_t)za.a_words, 0u, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->visible.p, Np, 0u, s));
-    SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p, s));
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_lo.p, Np, 0x7FFFFFFFu, s));
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_hi.p, Np, 0x80000000u, s));
+    SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p,
+                                  H->row_lo.p, H->row_hi.p, s));
     stage_mark(H, 1);
     SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, C, w, h,
                               H->cuts.p, H->ncuts.p, H->counts.p, s));
     stage_mark(H, 2);
-    SPR_HIP(spr::launch_scan_i32(H->counts.p, H->counts.p, C, ctl_at(H, za, za.st_count, 0), s));
-    int32_t* d_maxlen = reinterpret_cast<int32_t*>(H->zero.p + za.maxlen);
+    SPR_HIP(spr::launch_scan_i32(H->counts.p, H->counts.p, C, ctl_at(H, za.st_count, za.ctr_a, 0), s));
+    SPR_HIP(spr::launch_bucket_counts(H->row_lo.p, H->row_hi.p, Np, (h - 1) / 2, H->bucket_base.p, H->prow.p, s));
+    SPR_HIP(spr::launch_scan_i32(H->bucket_base.p, H->bucket_base.p, Np, ctl_at(H, za.st_pbase, za.ctr_a, 1), s));
     int64_t N = 0;
-    int32_t max_len = 0;
     if (!H->inj) {
         SPR_HIP(spr::launch_path_segments(H->cpath.p, C, H->counts.p, Np, H->seg.p, s));
-        SPR_HIP(spr::launch_segment_stats(H->seg.p, Np, d_maxlen, s));
-        // ---- HOST SYNC: the count-then-write readback
         SPR_HIP(hipMemcpyAsync(H->h_pinned, H->counts.p + C, 4, hipMemcpyDeviceToHost, s));
-        SPR_HIP(hipMemcpyAsync(H->h_pinned + 1, d_maxlen, 4, hipMemcpyDeviceToHost, s));
-        SPR_HIP(hipStreamSynchronize(s));
-        N = H->h_pinned[0];
-        max_len = H->h_pinned[1];
     } else {
         N = (int64_t)H->inj_inter.size() / 2;
         std::vector<int32_t> hseg((size_t)Np + 1, 0);
@@ -229,18 +226,23 @@ void run_frame(spr_handle* H) {
             }
             if (i == 0) for (uint32_t j = 0; j <= p0; ++j) hseg[j] = 0;
         }
-        for (uint32_t p = 0; p < Np; ++p) max_len = std::max(max_len, hseg[p + 1] - hseg[p]);
         SPR_HIP(hipMemcpyAsync(H->seg.p, hseg.data(), hseg.size() * 4, hipMemcpyHostToDevice, s));
     }
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 1, H->bucket_base.p + Np, 4, hipMemcpyDeviceToHost, s));
+    // ---- HOST SYNC: the count-then-write readback (scanline_rasterizer.cpp:355-356)
+    SPR_HIP(hipStreamSynchronize(s));
+    if (!H->inj) N = H->h_pinned[0];
+    const int64_t NB = H->h_pinned[1];
     if (N < 0 || N > (int64_t)(1 << 30)) throw Fail{SPR_ERR_CAPACITY, "intersection count exceeds 2^30"};
+    if (NB < 0 || NB > (int64_t)(1 << 30)) throw Fail{SPR_ERR_CAPACITY, "bucket count exceeds 2^30"};
     H->N = N;
-    H->max_len = max_len;
     stage_mark(H, 3);
 
-    // ---- phase B: size buffers for N
-    ZeroLayout z = zero_layout(C, Np, N, rows);
+    // ---- phase B: size buffers for N and NB
+    const ZeroLayout z = zero_layout(C, Np, N, NB);
     H->zero.ensure(z.total);
-    const size_t n1 = (size_t)std::max<int64_t>(N, 1);
+    const size_t n1 = (size_t)std::max<int64_t>(N, 1), nb1 = (size_t)std::max<int64_t>(NB, 1);
+    const int64_t max_bands = (int64_t)Np + 3 * (N / 2048) + 2;
     H->inter.ensure(n1);
     H->chain_t.ensure(n1);
     H->key.ensure(n1);
@@ -252,16 +254,19 @@ void run_frame(spr_handle* H) {
     H->skey.ensure(n1);
     H->swn.ensure(n1);
     H->winding.ensure(n1 + 1);
+    H->bflag.ensure(nb1 + 1);
+    H->band_start.ensure((size_t)max_bands + 1);
+    H->big_list.ensure((size_t)max_bands);
     if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); }
     H->records.ensure(2 * n1);
-    H->rec_path.ensure(2 * n1);
-    H->run_flag.ensure(2 * n1 + 1);
-    H->runs.ensure(2 * n1);
-    H->row_runs.ensure(2 * n1);
-    H->row_start.ensure((size_t)rows + 1);
     H->image.ensure((size_t)4 * H->w * H->h);
     SPR_HIP(spr::launch_fill_u32((uint32_t*)(H->zero.p + z.a_words), 2 * (int64_t)(z.total - z.a_words), 0u, s));
     int32_t* d_totals = reinterpret_cast<int32_t*>(H->zero.p + z.totals);
+    int32_t* d_err = reinterpret_cast<int32_t*>(H->zero.p + z.misc);
+    int32_t* d_bigcount = d_err + 1;
+    int32_t* d_hist = reinterpret_cast<int32_t*>(H->zero.p + z.hist);
+    int32_t* d_fill = reinterpret_cast<int32_t*>(H->zero.p + z.fill);
+    int2* d_brec = reinterpret_cast<int2*>(H->zero.p + z.brec);
 
     const bool gen = !H->inj && N > 0;
     if (H->inj) {
@@ -277,48 +282,37 @@ void run_frame(spr_handle* H) {
     }
     stage_mark(H, 5);
     SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
-                                     H->key.p, H->wn.p, H->epath.p, s));
+                                     H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err, s));
     stage_mark(H, 6);
-    // ---- segmented sort
+    // ---- sort: bucket offsets, scatter into bucket order, band sort in LDS
     if (N > 0) {
-        SPR_HIP(spr::launch_tile_counts(H->seg.p, Np, H->tile_counts.p, s));
-        SPR_HIP(spr::launch_scan_i32(H->tile_counts.p, H->tile_counts.p, Np, ctl_at(H, z, z.st_tiles, 1), s));
-        const int64_t max_tiles = (int64_t)Np + ceil_div(N, spr::kSortTile) + 1;
-        SPR_HIP(spr::launch_sort_tiles(H->key.p, H->wn.p, H->seg.p, H->tile_counts.p, Np, max_tiles, H->sorted.p,
-                                       H->sbig.p, s));
-        int passes = 0;
-        while (((int64_t)spr::kSortTile << passes) < (int64_t)max_len) ++passes;
-        for (int j = 0; j < passes; ++j) {
-            const unsigned long long* src = (j & 1) ? H->stmp.p : H->sbig.p;
-            unsigned long long* dst = (j & 1) ? H->sbig.p : H->stmp.p;
-            SPR_HIP(spr::launch_merge_pass(H->seg.p, H->tile_counts.p, Np, max_tiles, j, src, dst, H->sorted.p, s));
-        }
+        SPR_HIP(spr::launch_scan_i32(d_hist, d_hist, NB, ctl_at(H, z.st_hist, z.ctr, 0), s));
+        SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, N, H->bucket_base.p, H->prow.p, d_hist,
+                                           d_fill, H->sbig.p, d_err, s));
+        SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, d_hist, H->bflag.p, s));
+        SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, NB, ctl_at(H, z.st_band, z.ctr, 1), s));
+        SPR_HIP(spr::launch_band_emit(H->bflag.p, d_hist, (int32_t)NB, N, H->band_start.p, s));
+        SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->sorted.p,
+                                       H->big_list.p, d_bigcount, s));
+        SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->sbig.p, H->stmp.p,
+                                           H->sorted.p, s));
         SPR_HIP(spr::launch_unpack_sorted(H->sorted.p, N, H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s));
     }
     stage_mark(H, 7);
     // ---- winding numbers (global exclusive scan, scanline_rasterizer.cpp:478-506)
-    SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, N, ctl_at(H, z, z.st_wind, 2), s));
+    SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, N, ctl_at(H, z.st_wind, z.ctr, 2), s));
     stage_mark(H, 8);
     SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
-                                  ctl_at(H, z, z.st_mark, 3), H->records.p, H->rec_path.p,
-                                  debug ? H->flags.p : nullptr, d_totals, s));
+                                  ctl_at(H, z.st_mark, z.ctr, 3), H->records.p, debug ? H->flags.p : nullptr,
+                                  d_totals, H->bucket_base.p, H->prow.p, d_brec, s));
     stage_mark(H, 9);
-    // ---- composite
-    const int64_t max_rec = 2 * (int64_t)n1;
-    SPR_HIP(spr::launch_run_flags(H->records.p, H->rec_path.p, d_totals, max_rec, H->run_flag.p, s));
-    SPR_HIP(spr::launch_scan_i32(H->run_flag.p, H->run_flag.p, max_rec, ctl_at(H, z, z.st_run, 4), s));
-    SPR_HIP(spr::launch_run_emit(H->records.p, H->run_flag.p, d_totals, max_rec, H->runs.p, s));
-    const int32_t* d_nruns = H->run_flag.p + max_rec;
-    int32_t* d_row_count = reinterpret_cast<int32_t*>(H->zero.p + z.row_count);
-    int32_t* d_row_fill = reinterpret_cast<int32_t*>(H->zero.p + z.row_fill);
-    SPR_HIP(spr::launch_row_count(H->runs.p, d_nruns, max_rec, d_row_count, s));
-    SPR_HIP(spr::launch_scan_i32(d_row_count, H->row_start.p, rows, ctl_at(H, z, z.st_rows, 5), s));
-    SPR_HIP(spr::launch_row_scatter(H->runs.p, d_nruns, max_rec, H->row_start.p, d_row_fill, H->row_runs.p, s));
     stage_mark(H, 10);
-    SPR_HIP(spr::launch_composite(H->records.p, H->runs.p, d_nruns, d_totals, H->row_start.p, H->row_runs.p, w, h,
+    // ---- composite
+    SPR_HIP(spr::launch_composite(H->records.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
                                   (int)H->opts.compose, H->image.p, s));
     stage_mark(H, 11);
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 8, d_err, 4, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipEventRecord(H->ev_end, s));
     H->frame_done = true;
 }
@@ -348,6 +342,10 @@ void sync_stats(spr_handle* H) {
     H->stats.n_curves = H->C;
     H->stats.n_paths = H->Np;
     H->stats.n_intersections = H->N;
+    if (H->h_pinned[8] != 0) {
+        H->h_pinned[8] = 0;
+        throw Fail{SPR_ERR_DEVICE, "internal: an entry fell outside its path's row bound"};
+    }
     H->stats.n_cells = H->h_pinned[4];
     H->stats.n_spans = H->h_pinned[5];
     float ms = 0.0f;
@@ -449,12 +447,12 @@ int spr_destroy(spr_handle* H) {
     DBuf<int>* unused = nullptr; (void)unused;
     H->pos.release(); H->pos_path.release(); H->cpos.release(); H->ctype.release(); H->cpath.release();
     H->rule.release(); H->fill.release(); H->tpos.release(); H->visible.release(); H->cuts.release();
-    H->ncuts.release(); H->counts.release(); H->seg.release(); H->tile_counts.release(); H->inter.release();
+    H->ncuts.release(); H->counts.release(); H->seg.release(); H->inter.release();
     H->chain_t.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release(); H->swn.release();
-    H->winding.release(); H->flags.release(); H->records.release(); H->rec_path.release();
-    H->run_flag.release(); H->row_start.release(); H->row_fill.release(); H->row_runs.release();
-    H->runs.release(); H->image.release(); H->zero.release();
+    H->winding.release(); H->flags.release(); H->records.release();
+    H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
+    H->band_start.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
     spr::learned_release(H->learned);
     if (H->h_pinned) (void)hipHostFree(H->h_pinned);
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
