@@ -25,9 +25,12 @@ sys.path.insert(0, str(ROOT))
 import vkscanlinepr_amd as V  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# Stage boundaries recorded by the library (stage_ms[k] = events k -> k+1).
-STAGE_NAMES = ["transform", "count", "scans+sync", "chain_solve", "piece_merge",
-               "gen_fragment", "sort", "winding_scan", "mark_emit", "(unused)", "composite"]
+# Stage boundaries recorded by the library on its stream (stage_ms[k] = events k -> k+1).  The
+# hot kernels (chain_solve, band_sort, winding_scan, mark_emit, composite, ...) sit alone between
+# two events, so these are their device times.
+STAGE_NAMES = ["transform", "count", "scans+sync", "chain_prep", "chain_solve", "piece_merge",
+               "gen_fragment", "bucket_scatter", "band_sort", "winding_scan", "mark_emit",
+               "composite"]
 
 WORKLOADS = {
     # name: (generator kind, seed, n_paths, canvas, compose)
@@ -37,17 +40,22 @@ WORKLOADS = {
 }
 
 
-def stage_bytes(st, P, C, Np, N, R, w, h):
-    """Algorithmic HBM bytes of each stage: every input read once, every output written once."""
+def stage_bytes(P, C, Np, N, R, w, h):
+    """Algorithmic HBM bytes per stage: every input read once and every output written once, in
+    the reference's data types (transform 20 B/point + 4 B/path; count 72 B/curve; intersection
+    write 72 B/curve + 8 B/intersection, split into the solve (curve data in, one t out) and the
+    merge (t in, (curve, t) out); fragment keys 24 B/intersection + 48 B/curve; sort 16 B/entry
+    (key and value in and out); scan 8 B/entry; mark + compaction 44 B/entry + 16 B/record;
+    composite 16 B/record + 4 B/pixel)."""
     return {
         "transform": 20 * P + 4 * Np,
         "count": 72 * C,
         "chain_solve": 48 * C + 4 * N,
-        "piece_merge": 48 * C + 4 * N + 8 * N,
-        "gen_fragment": 8 * N + 48 * C + 9 * N,
-        "sort": 16 * N,
+        "piece_merge": 24 * C + 4 * N + 8 * N,
+        "gen_fragment": 24 * N + 48 * C,
+        "band_sort": 16 * N,
         "winding_scan": 8 * N,
-        "mark_emit": 16 * N + 16 * R,
+        "mark_emit": 44 * N + 16 * R,
         "composite": 16 * R + 4 * w * h,
     }
 
@@ -117,14 +125,16 @@ def main(argv=None):
     P, C, Np = st.n_points, st.n_curves, st.n_paths
     N, R = st.n_intersections, st.n_cells + st.n_spans
     stages = dict(zip(STAGE_NAMES, stage_acc[: len(STAGE_NAMES)] / a.steps))
-    algo = stage_bytes(st, P, C, Np, N, R, canvas, canvas)
-    if a.stages:
-        for k, v in stages.items():
-            gbs = algo.get(k, 0) / (v * 1e-3) / 1e9 if v > 0 and k in algo else 0.0
-            print(f"  {k:18s} {v:8.3f} ms  {gbs:8.1f} GB/s", file=sys.stderr)
+    algo = stage_bytes(P, C, Np, N, R, canvas, canvas)
+    table = {}
+    for k, v in stages.items():
+        gbs = algo[k] / (v * 1e-3) / 1e9 if v > 0 and k in algo else None
+        table[k] = {"ms": round(v, 4), "GB/s": None if gbs is None else round(gbs, 1),
+                    "frac": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4)}
+        if a.stages:
+            print(f"  {k:16s} {v:8.3f} ms  {gbs or 0:8.1f} GB/s", file=sys.stderr)
     dom = max((k for k in algo if k in stages), key=lambda k: stages[k])
-    dom_ms = stages[dom]
-    achieved = algo[dom] / (dom_ms * 1e-3) / 1e9
+    achieved = algo[dom] / (stages[dom] * 1e-3) / 1e9
     out = {
         "metric": "ms/frame & Mpaths/s at 4096^2 (learned 1-pass vs plain 2-pass); HBM GB/s",
         "value": Np / (ms_per_step * 1e-3) / 1e6,
@@ -139,7 +149,8 @@ def main(argv=None):
                    "device_ms_per_frame": float(np.mean(frame_ms))},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "stage_ms": {k: round(v, 4) for k, v in stages.items()}},
+                     "algorithmic_bytes": algo[dom], "kernel_ms": stages[dom]},
+        "stages": table,
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kind, seed, n_paths, canvas, compose)

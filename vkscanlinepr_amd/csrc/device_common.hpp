@@ -191,11 +191,16 @@ __host__ __device__ inline float flag_piece_end(float t_raw, float p1x) {
     return (floorf(p1x) == p1x) ? bitsf(b | 2u) : bitsf(b | 3u);
 }
 
+// Control values of one axis of a curve.
+struct CurveAxis {
+    float c[4];
+};
+
 // One grid-crossing solve (make_intersection_1.comp:377-437): a line is solved by its
 // reciprocal slope, a cubic by 24 bisection steps on [t_min, t1] keyed on the sign of B(t)-c.
-__device__ inline float solve_crossing(const CurvePts& cv, int side, float c, float t_min, float t1) {
-    const float* p = side == 0 ? cv.x : cv.y;
-    if (!cv.cubic) {
+__device__ inline float solve_axis(bool cubic, const CurveAxis& ax, float c, float t_min, float t1) {
+    const float* p = ax.c;
+    if (!cubic) {
         float a = p[1] - p[0];
         a = (a != 0.0f ? (1.0f / a) : 0.0f);
         return glmin(glmax((c - p[0]) * a, t_min), t1);

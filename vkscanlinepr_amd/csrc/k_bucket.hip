@@ -6,7 +6,7 @@
 // naive_seg_sort_pairs.comp:69).  The transform pass bounds each path's rows from its control
 // points (a Bezier curve stays inside their hull), so each path owns a small, exactly sized
 // bucket range.  Entries are counted per bucket while their keys are generated, scattered once
-// into bucket order, and the buckets are packed into bands of at most 4096 entries that one
+// into bucket order, and the buckets are packed into bands of at most 2048 entries that one
 // workgroup sorts in LDS.  The per-bucket ranges also give the composite each path's records of
 // each cell row without any further sorting.
 #include "device_common.hpp"
@@ -67,8 +67,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
 }
 
 // Band starts: a band is a run of consecutive buckets of one path whose entries start in the
-// same 2048-entry window; a bucket of more than 2048 entries is a band of its own.  Every band
-// therefore holds at most 4096 entries unless it is a single larger bucket.
+// same 1024-entry window; a bucket of more than 1024 entries is a band of its own.  Every band
+// therefore holds at most 2048 entries (one sort tile) unless it is a single larger bucket.
 // One wave per path: the most recent non-empty bucket before each bucket comes from an inclusive
 // max-scan of (start << 32 | size) over the non-empty buckets (starts only grow along a path).
 __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict__ bucket_base, uint32_t Np,
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict
                     f = 1;
                 } else {
                     const int32_t ps = (int32_t)(prev >> 32) - 1, pz = (int32_t)(prev & 0xFFFFFFFFull);
-                    f = (o >> 11) != (ps >> 11) || pz > 2048 || sz > 2048;
+                    f = (o >> 10) != (ps >> 10) || pz > 1024 || sz > 1024;
                 }
             }
             flag[b] = f;

@@ -1,0 +1,521 @@
+// k_chain.hip -- intersection write (make_intersection_1.comp:217-447) for gfx950.
+//
+// Inside one monotone piece the reference interleaves x-grid and y-grid crossings, but every
+// bisection for the next crossing on one axis starts from the previous crossing of the SAME axis
+// (the t_min it starts from is that axis' pending value, :338-359, :402).  So each (piece, axis)
+// is an independent chain of dependent solves, and the reference's output order is a merge of the
+// two chains (ties to x, :338) followed by its equal-t OR-merge (:361-375).  This file keeps
+// every float operation of the reference and only changes who computes what:
+//
+//  1. k_chain_prep: one thread per (curve, piece) walks the piece geometry and writes one
+//     descriptor per non-empty chain and per piece; chains are counted in length classes.
+//  2. k_chain_order: chains are laid out longest class first (longest-processing-time order).
+//  3. k_chain_solve: a persistent kernel.  Waves first take the long cubic chains, one chain per
+//     wave: the 64 lanes evaluate the 63 midpoints of the next 6 bisection levels at once (lane l
+//     is node l of the decision tree and recomputes exactly the midpoints on its path), a ballot
+//     resolves the path, and 4 rounds give the reference's 24 iterations.  Then waves take the
+//     remaining chains 64 at a time, one chain per lane, in descending length.
+//  4. k_piece_merge: one thread per piece merges its start and two chains in the reference's
+//     order; long pieces whose chains are non-decreasing are merged by one wave with merge-path
+//     splits instead (k_piece_merge_wave).  Both write raw merged values.
+//  5. k_or_merge: one thread per entry applies the reference's equal-t OR-merge inside pieces.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace spr {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kLongClass = 10;     // cubic chains of 2^10 = 1024+ crossings get a whole wave
+constexpr int kLongPiece = 256;    // pieces at least this long are merged by a wave
+constexpr int kStage = 4096;       // piece values staged in LDS by the merging wave
+constexpr int kClasses = 32;       // 0..15: cubic chains by log2 length, 16..31: line chains
+
+inline unsigned grid_for(int64_t n) {
+    int64_t g = (n + kBlock - 1) / kBlock;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+__device__ inline int log2_class(int n) { int l = 31 - __clz(n); return l > 15 ? 15 : l; }
+
+// Walks the monotone pieces of a visible curve up to `piece`: flagged end parameters, grid and
+// first entry offset inside the curve's output range.
+struct PieceCtx {
+    float t0, t1;
+    PieceGrid G;
+    int off;
+};
+
+__device__ inline bool setup_piece(const CurvePts& cv, const float q[4], uint32_t n, int piece,
+                                   int w, int h, PieceCtx& pc) {
+    if ((uint32_t)piece > n) return false;
+    float t0 = 0.0f, p0x = cv.x[0], p0y = cv.y[0];
+    int off = 0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const float traw = (uint32_t)k < n ? q[k] : 1.0f;
+        const float p1x = cv.ex(traw), p1y = cv.ey(traw);
+        const float t1 = flag_piece_end(traw, p1x);
+        const PieceGrid G = piece_grid(p0x, p0y, p1x, p1y, w, h);
+        if (k == piece) {
+            pc.t0 = t0; pc.t1 = t1; pc.G = G; pc.off = off;
+            return true;
+        }
+        off += 1 + G.nx + G.ny;
+        t0 = t1; p0x = p1x; p0y = p1y;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kBlock) void k_chain_prep(
+    const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
+    const uint32_t* __restrict__ ctype, const uint32_t* __restrict__ cpath,
+    const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
+    const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
+    int h, ChainDesc* __restrict__ chains, PieceDesc* __restrict__ pieces,
+    int32_t* __restrict__ class_count) {
+    __shared__ int32_t hist[kClasses];
+    if (threadIdx.x < kClasses) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t c = gid / 5u;
+    const int piece = (int)(gid % 5u);
+    PieceDesc pd;
+    pd.base = -1;
+    ChainDesc cd[2];
+    cd[0].n = cd[1].n = 0;
+    if (c < C && visible_mask(visible[cpath[c]]) && (uint32_t)piece <= ncuts[c]) {
+        const uint32_t type = ctype[c];
+        const CurvePts cv = load_curve(tpos, cpos[c], type);
+        const float4 qq = cuts[c];
+        const float q[4] = {qq.x, qq.y, qq.z, qq.w};
+        PieceCtx pc;
+        if (setup_piece(cv, q, ncuts[c], piece, w, h, pc)) {
+            const int base = offsets[c] + pc.off;
+            if (base + pc.G.nx + pc.G.ny + 1 <= offsets[c + 1]) {  // count and write passes agree
+                pd.base = base; pd.nx = pc.G.nx; pd.ny = pc.G.ny; pd.curve = (int32_t)c; pd.t0 = pc.t0;
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    const int n = side ? pc.G.ny : pc.G.nx;
+                    ChainDesc& d = cd[side];
+                    d.n = n;
+                    if (n == 0) continue;
+                    d.curve = (int32_t)c;
+                    d.side = side;
+                    d.out = base + 1 + (side ? pc.G.nx : 0);
+                    d.line0 = side ? pc.G.y0 : pc.G.x0;
+                    d.step = side ? pc.G.dy : pc.G.dx;
+                    d.t0 = pc.t0;
+                    d.t1 = pc.t1;
+                    d.cubic = type == kCubic;
+                    atomicAdd(&hist[(d.cubic ? 0 : 16) + log2_class(n)], 1);
+                }
+            }
+        }
+    }
+    if (gid < 5u * C) {
+        pieces[gid] = pd;
+        chains[2 * gid] = cd[0];
+        chains[2 * gid + 1] = cd[1];
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses && hist[threadIdx.x]) atomicAdd(&class_count[threadIdx.x], hist[threadIdx.x]);
+}
+
+// Class order: by estimated cost, longest first (a line crossing costs about 2^-6 of a cubic
+// one), cubic before line on ties.  The cubic classes >= kLongClass come first: the wave-solved
+// chains.
+__global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t* __restrict__ class_off,
+                                int32_t* __restrict__ n_long) {
+    if (threadIdx.x != 0) return;
+    int32_t cls[kClasses], key[kClasses];
+    for (int c = 0; c < kClasses; ++c) {
+        cls[c] = c;
+        key[c] = c < 16 ? 2 * (c + 6) + 1 : 2 * (c - 16);
+    }
+    for (int i = 1; i < kClasses; ++i)  // insertion sort, descending key
+        for (int j = i; j > 0 && key[j] > key[j - 1]; --j) {
+            int32_t t = key[j]; key[j] = key[j - 1]; key[j - 1] = t;
+            t = cls[j]; cls[j] = cls[j - 1]; cls[j - 1] = t;
+        }
+    int32_t off = 0, nl = 0;
+    for (int r = 0; r < kClasses; ++r) {
+        const int c = cls[r];
+        class_off[c] = off;
+        off += class_count[c];
+        if (c < 16 && c >= kLongClass) nl = off;
+    }
+    class_off[kClasses] = off;
+    *n_long = nl;
+}
+
+__global__ __launch_bounds__(kBlock) void k_chain_order(const ChainDesc* __restrict__ chains, int64_t n_slots,
+                                                        int32_t* __restrict__ class_fill,
+                                                        const int32_t* __restrict__ class_off,
+                                                        int32_t* __restrict__ order) {
+    __shared__ int32_t cnt[kClasses], base[kClasses];
+    if (threadIdx.x < kClasses) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = i < n_slots && chains[i].n > 0;
+    int cls = 0, rank = 0;
+    if (valid) {
+        cls = (chains[i].cubic ? 0 : 16) + log2_class(chains[i].n);
+        rank = atomicAdd(&cnt[cls], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kClasses && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&class_fill[threadIdx.x], cnt[threadIdx.x]);
+    __syncthreads();
+    if (valid) order[class_off[cls] + base[cls] + rank] = (int32_t)i;
+}
+
+__device__ inline CurveAxis load_axis(const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
+                                      const ChainDesc& d) {
+    const float2* p = tpos + cpos[d.curve];
+    CurveAxis a;
+    const int npts = d.cubic ? 4 : 2;
+    a.c[0] = d.side ? p[0].y : p[0].x;
+    a.c[1] = d.side ? p[1].y : p[1].x;
+    a.c[2] = npts == 4 ? (d.side ? p[2].y : p[2].x) : 0.0f;
+    a.c[3] = npts == 4 ? (d.side ? p[3].y : p[3].x) : 0.0f;
+    return a;
+}
+
+// Sequential chain (one lane): the reference's solve per crossing.
+__device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
+                                        uint8_t* __restrict__ mono) {
+    float tprev = d.t0, line = d.line0;
+    bool up = true, down = true;
+    const uint32_t side = (uint32_t)d.side;
+    for (int k = 0; k < d.n; ++k) {
+        const float ts = solve_axis(d.cubic, a, line, tprev, d.t1);
+        const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | side);
+        out[k] = v;
+        // the y chain is preceded by the piece start in the merge, the x chain is not
+        if (k > 0 || side) { up = up && v >= tprev; down = down && v <= tprev; }
+        tprev = v;
+        line += d.step;
+    }
+    *mono = (up ? 1 : 0) | (down ? 2 : 0);
+}
+
+// Cubic chain solved by one wave: 4 rounds of a 6-level speculative bisection per crossing.
+__device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
+                                        uint8_t* __restrict__ mono) {
+    const int lane = threadIdx.x & 63;
+    const int depth = lane ? 31 - __clz(lane) : 0;
+    const uint32_t side = (uint32_t)d.side;
+    float tprev = d.t0, line = d.line0;
+    bool up = true, down = true;
+    for (int k = 0; k < d.n; ++k) {
+        const float c = line;
+        const float t_min = tprev;
+        const float vt0 = eval_axis(true, t_min, a.c[0], a.c[1], a.c[2], a.c[3]);
+        float ts = t_min;
+        if (vt0 != c) {
+            const uint32_t s0 = fbits(vt0 - c) & 0x80000000u;
+            float lo = t_min, hi = d.t1;
+            float tm = t_min, vtm = vt0;
+            for (int round = 0; round < 4; ++round) {
+                // lane l = node l of a 63-node decision tree below (lo, hi)
+                float llo = lo, lhi = hi;
+                for (int lev = depth - 1; lev >= 0; --lev) {
+                    const float mid = (llo + lhi) * 0.5f;
+                    if ((lane >> lev) & 1) llo = mid; else lhi = mid;
+                }
+                const float mym = (llo + lhi) * 0.5f;
+                const float myv = eval_axis(true, mym, a.c[0], a.c[1], a.c[2], a.c[3]);
+                const bool right = lane != 0 && (fbits(myv - c) & 0x80000000u) == s0;
+                const unsigned long long D = __ballot(right);
+                int node = 1;
+#pragma unroll
+                for (int lev = 0; lev < 5; ++lev) node = 2 * node + (int)((D >> node) & 1ull);
+                // node: the depth-5 node on the path, the last of this round's 6 midpoints
+                const float nlo = __shfl(llo, node), nhi = __shfl(lhi, node);
+                tm = __shfl(mym, node);
+                vtm = __shfl(myv, node);
+                if ((D >> node) & 1ull) { lo = tm; hi = nhi; } else { lo = nlo; hi = tm; }
+            }
+            ts = fabsf(vtm - c) > 1.0f ? t_min : tm;
+        }
+        const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | side);
+        if (lane == 0) out[k] = v;
+        if (k > 0 || side) { up = up && v >= tprev; down = down && v <= tprev; }
+        tprev = v;
+        line += d.step;
+    }
+    if (lane == 0) *mono = (up ? 1 : 0) | (down ? 2 : 0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict__ tpos,
+                                                        const uint32_t* __restrict__ cpos,
+                                                        const ChainDesc* __restrict__ chains,
+                                                        const int32_t* __restrict__ order,
+                                                        const int32_t* __restrict__ class_off,
+                                                        const int32_t* __restrict__ n_long,
+                                                        int32_t* __restrict__ work,
+                                                        float* __restrict__ chain_t,
+                                                        uint8_t* __restrict__ mono) {
+    const int lane = threadIdx.x & 63;
+    const int32_t total = class_off[kClasses];
+    const int32_t nl = *n_long;
+    const int32_t items = nl + (total - nl + 63) / 64;
+    while (true) {
+        int32_t item = 0;
+        if (lane == 0) item = atomicAdd(work, 1);
+        item = __shfl(item, 0);
+        if (item >= items) break;
+        if (item < nl) {
+            __builtin_amdgcn_s_setprio(3);
+            const int32_t ci = order[item];
+            const ChainDesc d = chains[ci];
+            const CurveAxis a = load_axis(tpos, cpos, d);
+            solve_chain_wave(d, a, chain_t + d.out, mono + ci);
+            __builtin_amdgcn_s_setprio(0);
+        } else {
+            const int32_t k0 = nl + (item - nl) * 64;
+            const int32_t k = k0 + lane;
+            // issue priority follows the length of the wave's chains (the longest run first
+            // and must not be starved by the short ones)
+            const int cls = log2_class(chains[order[k0]].n) + (chains[order[k0]].cubic ? 6 : 0);
+            if (cls >= 13) __builtin_amdgcn_s_setprio(2);
+            else if (cls >= 11) __builtin_amdgcn_s_setprio(1);
+            if (k < total) {
+                const int32_t ci = order[k];
+                const ChainDesc d = chains[ci];
+                const CurveAxis a = load_axis(tpos, cpos, d);
+                solve_chain_lane(d, a, chain_t + d.out, mono + ci);
+            }
+            __builtin_amdgcn_s_setprio(0);
+        }
+    }
+}
+
+// Reference merge of one piece (one lane), raw values: the piece start and both chains in the
+// order of make_intersection_1.comp:310-359.
+__device__ inline void merge_piece_lane(const PieceDesc& pd, const float* __restrict__ chain_t,
+                                        int32_t* __restrict__ raw, int8_t* __restrict__ start_mark,
+                                        int2* __restrict__ inter) {
+    const int nx = pd.nx, ny = pd.ny;
+    const float* X = chain_t + pd.base + 1;
+    const float* Y = X + nx;
+    const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
+    float tx = pd.t0, ty = pd.t0;
+    int xi = 0, yi = 0;
+    if (nx == 0) tx = kX2;
+    else if (ny == 0) ty = kY2;
+    else { tx = X[0]; xi = 1; }
+    const int n = nx + ny + 1;
+    for (int i = 0; i < n; ++i) {
+        float tmin;
+        if (tx <= ty) { tmin = tx; tx = xi < nx ? X[xi++] : kX2; }
+        else { tmin = ty; ty = yi < ny ? Y[yi++] : kY2; }
+        raw[pd.base + i] = (int32_t)fbits(tmin);
+        start_mark[pd.base + i] = i == 0;
+        inter[pd.base + i].x = pd.curve;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_piece_merge(const PieceDesc* __restrict__ pieces, int64_t n_slots,
+                                                        const ChainDesc* __restrict__ chains,
+                                                        const uint8_t* __restrict__ mono,
+                                                        const float* __restrict__ chain_t,
+                                                        int32_t* __restrict__ raw, int8_t* __restrict__ start_mark,
+                                                        int2* __restrict__ inter, int32_t* __restrict__ long_list,
+                                                        int32_t* __restrict__ long_count,
+                                                        uint8_t* __restrict__ pieces_seq) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool wave_it = false;
+    if (i < n_slots) {
+        const PieceDesc pd = pieces[i];
+        if (pd.base >= 0) {
+            const int n = pd.nx + pd.ny + 1;
+            if (n >= kLongPiece) {
+                // Both merged sequences non-decreasing: merge-path.  Both non-increasing: the
+                // reference merge takes the sequence with the smaller head to its end, then the
+                // other one (a concatenation).  Anything else: the sequential merge.
+                // Sequences: nx>0 && ny>0: A = X, B = [t0, Y]; ny == 0: A = [t0, X]; nx == 0: B = [t0, Y].
+                uint32_t fa = 3u, fb = 3u;
+                if (pd.nx > 0) fa = mono[2 * i];
+                if (pd.ny > 0) fb = mono[2 * i + 1];
+                if (pd.ny == 0 && pd.nx > 0) {
+                    const float x1 = chain_t[pd.base + 1];
+                    if (!(x1 >= pd.t0)) fa &= ~1u;
+                    if (!(x1 <= pd.t0)) fa &= ~2u;
+                }
+                uint8_t mode = 2;
+                if ((fa & fb & 1u) != 0) mode = 0;
+                else if ((fa & fb & 2u) != 0) mode = 1;
+                pieces_seq[i] = mode;
+                wave_it = true;
+            } else {
+                merge_piece_lane(pd, chain_t, raw, start_mark, inter);
+            }
+        }
+    }
+    const int32_t slot = wave_slot(long_count, 0u, wave_it);
+    if (wave_it) long_list[slot] = (int32_t)i;
+}
+
+// One wave per long piece.  The piece start and both chains are staged in LDS (coalesced), the
+// merged sequence is built in LDS, and then written out coalesced.  Mode 0: each lane merges a
+// contiguous slice after a merge-path split; mode 1: concatenation; mode 2: lane 0 runs the
+// reference's sequential merge.  A = the first sequence, B = the second, ties to A.
+__global__ __launch_bounds__(64) void k_piece_merge_wave(const PieceDesc* __restrict__ pieces,
+                                                         const int32_t* __restrict__ long_list,
+                                                         const int32_t* __restrict__ long_count,
+                                                         const uint8_t* __restrict__ piece_mode,
+                                                         const float* __restrict__ chain_t,
+                                                         int32_t* __restrict__ raw,
+                                                         int8_t* __restrict__ start_mark,
+                                                         int2* __restrict__ inter) {
+    __shared__ float st_in[kStage];
+    __shared__ float st_out[kStage + 1];
+    const int lane = threadIdx.x;
+    const int32_t cnt = *long_count;
+    for (int32_t q = blockIdx.x; q < cnt; q += gridDim.x) {
+        const int32_t pi = long_list[q];
+        const PieceDesc pd = pieces[pi];
+        const int nx = pd.nx, ny = pd.ny;
+        const int nv = nx + ny;
+        const bool lds = nv <= kStage;
+        const float* X = chain_t + pd.base + 1;
+        if (lds) {
+            for (int i = lane; i < nv; i += 64) st_in[i] = X[i];
+            __syncthreads();
+            X = st_in;
+        }
+        const float* Y = X + nx;
+        const bool head_in_a = ny == 0;
+        const int na = nx + (head_in_a ? 1 : 0), nb = head_in_a ? 0 : ny + 1;
+        auto Aat = [&](int i) { return head_in_a ? (i == 0 ? pd.t0 : X[i - 1]) : X[i]; };
+        auto Bat = [&](int j) { return j == 0 ? pd.t0 : Y[j - 1]; };
+        const int n = na + nb;
+        float* outv = lds ? st_out : nullptr;
+        auto emit = [&](int o, float v) {
+            if (lds) outv[o] = v;
+            else {
+                raw[pd.base + o] = (int32_t)fbits(v);
+                start_mark[pd.base + o] = o == 0;
+                inter[pd.base + o].x = pd.curve;
+            }
+        };
+        const uint8_t mode = piece_mode[pi];
+        if (mode == 2) {
+            if (lane == 0) {
+                const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
+                float tx = pd.t0, ty = pd.t0;
+                int xi = 0, yi = 0;
+                if (nx == 0) tx = kX2;
+                else if (ny == 0) ty = kY2;
+                else { tx = X[0]; xi = 1; }
+                for (int i = 0; i < n; ++i) {
+                    float tmin;
+                    if (tx <= ty) { tmin = tx; tx = xi < nx ? X[xi++] : kX2; }
+                    else { tmin = ty; ty = yi < ny ? Y[yi++] : kY2; }
+                    emit(i, tmin);
+                }
+            }
+        } else if (mode == 1) {
+            const bool a_first = nb == 0 || (na > 0 && Aat(0) <= Bat(0));
+            for (int o = lane; o < n; o += 64) {
+                float v;
+                if (a_first) v = o < na ? Aat(o) : Bat(o - na);
+                else v = o < nb ? Bat(o) : Aat(o - nb);
+                emit(o, v);
+            }
+        } else {
+            const int per = (n + 63) / 64;
+            const int o0 = lane * per;
+            const int o1 = o0 + per < n ? o0 + per : n;
+            if (o0 < n) {
+                int lo = o0 > nb ? o0 - nb : 0, hi = o0 < na ? o0 : na;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (Aat(mid) <= Bat(o0 - 1 - mid)) lo = mid + 1; else hi = mid;
+                }
+                int i = lo, j = o0 - lo;
+                for (int o = o0; o < o1; ++o) {
+                    const bool takeA = j >= nb || (i < na && Aat(i) <= Bat(j));
+                    const float v = takeA ? Aat(i) : Bat(j);
+                    i += takeA; j += !takeA;
+                    emit(o, v);
+                }
+            }
+        }
+        if (lds) {
+            __syncthreads();
+            for (int o = lane; o < n; o += 64) {
+                raw[pd.base + o] = (int32_t)fbits(st_out[o]);
+                start_mark[pd.base + o] = o == 0;
+                inter[pd.base + o].x = pd.curve;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// The reference's OR-merge of equal t inside a piece (make_intersection_1.comp:361-375): within a
+// run [s, e) of equal masked t, entry j ends up as the OR of raw values s..min(j+1, e-1).
+__global__ __launch_bounds__(kBlock) void k_or_merge(const int32_t* __restrict__ raw,
+                                                     const int8_t* __restrict__ start_mark, int64_t N,
+                                                     int2* __restrict__ inter) {
+    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= N) return;
+    const uint32_t v = (uint32_t)raw[j], m = v & 0xFFFFFFFCu;
+    uint32_t acc = v;
+    for (int64_t s = j; s > 0 && !start_mark[s] && ((uint32_t)raw[s - 1] & 0xFFFFFFFCu) == m; --s)
+        acc |= (uint32_t)raw[s - 1];
+    if (j + 1 < N && !start_mark[j + 1] && ((uint32_t)raw[j + 1] & 0xFFFFFFFCu) == m) acc |= (uint32_t)raw[j + 1];
+    inter[j].y = (int32_t)acc;
+}
+
+}  // namespace
+
+hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                             const uint32_t* cpath, const int32_t* visible, const float4* cuts,
+                             const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
+                             ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    k_chain_prep<<<grid_for((int64_t)C * 5), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts, ncuts,
+                                                             offsets, C, w, h, chains, pieces, class_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_order(const ChainDesc* chains, uint32_t C, const int32_t* class_count,
+                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order,
+                              hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    k_chain_offsets<<<1, 64, 0, s>>>(class_count, class_off, n_long);
+    k_chain_order<<<grid_for((int64_t)C * 10), kBlock, 0, s>>>(chains, (int64_t)C * 10, class_fill, class_off, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
+                              const int32_t* order, const int32_t* class_off, const int32_t* n_long,
+                              int32_t* work, float* chain_t, uint8_t* mono, int n_cu, hipStream_t s) {
+    k_chain_solve<<<(unsigned)(n_cu * 8), kBlock, 0, s>>>(tpos, cpos, chains, order, class_off, n_long, work,
+                                                          chain_t, mono);
+    return hipGetLastError();
+}
+
+hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const ChainDesc* chains, const uint8_t* mono,
+                              const float* chain_t, int32_t* raw, int8_t* start_mark, int2* inter,
+                              int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, int n_cu,
+                              hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    k_piece_merge<<<grid_for((int64_t)C * 5), kBlock, 0, s>>>(pieces, (int64_t)C * 5, chains, mono, chain_t, raw,
+                                                              start_mark, inter, long_list, long_count, pieces_seq);
+    k_piece_merge_wave<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, long_list, long_count, pieces_seq, chain_t, raw,
+                                                           start_mark, inter);
+    return hipGetLastError();
+}
+
+hipError_t launch_or_merge(const int32_t* raw, const int8_t* start_mark, int64_t N, int2* inter, hipStream_t s) {
+    if (N == 0) return hipSuccess;
+    k_or_merge<<<grid_for(N), kBlock, 0, s>>>(raw, start_mark, N, inter);
+    return hipGetLastError();
+}
+
+}  // namespace spr

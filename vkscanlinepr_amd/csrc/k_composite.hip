@@ -35,6 +35,8 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
     __shared__ unsigned long long mask[kChunk];
     __shared__ int2 list[kBlock];
     __shared__ uint32_t lcol[kBlock];
+    __shared__ float4 lsa[kBlock];   // (r*a, g*a, b*a, a) of each listed path, as the blend uses them
+    __shared__ float lia[kBlock];    // 1 - a
     __shared__ int32_t wcount[kBlock / 64];
     __shared__ int32_t bpre[kBatch + 1];
     const int r = blockIdx.x;
@@ -73,7 +75,14 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
             if (keep) {
                 const int slot = off + (int)__popcll(m & ((1ull << lane) - 1ull));
                 list[slot] = rng;
-                lcol[slot] = fill[p];
+                const uint32_t col = fill[p];
+                lcol[slot] = col;
+                if constexpr (COMPOSE != 0) {
+                    const float a = (float)((col >> 24) & 0xFFu) / 255.0f;
+                    lsa[slot] = make_float4(((float)(col & 0xFFu) / 255.0f) * a, ((float)((col >> 8) & 0xFFu) / 255.0f) * a,
+                                            ((float)((col >> 16) & 0xFFu) / 255.0f) * a, a);
+                    lia[slot] = 1.0f - a;
+                }
             }
             __syncthreads();
             for (int q0 = 0; q0 < n; q0 += kBatch) {
@@ -133,15 +142,12 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
                         while (mm) {
                             const int q = __ffsll((long long)mm) - 1;
                             mm &= mm - 1ull;
-                            const uint32_t col = lcol[q0 + q];
-                            const float s0 = (float)(col & 0xFFu) / 255.0f, s1 = (float)((col >> 8) & 0xFFu) / 255.0f;
-                            const float s2 = (float)((col >> 16) & 0xFFu) / 255.0f;
-                            const float a = (float)((col >> 24) & 0xFFu) / 255.0f;
-                            const float ia = 1.0f - a;
-                            acc[j][0] = s0 * a + acc[j][0] * ia;
-                            acc[j][1] = s1 * a + acc[j][1] * ia;
-                            acc[j][2] = s2 * a + acc[j][2] * ia;
-                            acc[j][3] = a + acc[j][3] * ia;
+                            const float4 sa = lsa[q0 + q];
+                            const float ia = lia[q0 + q];
+                            acc[j][0] = sa.x + acc[j][0] * ia;
+                            acc[j][1] = sa.y + acc[j][1] * ia;
+                            acc[j][2] = sa.z + acc[j][2] * ia;
+                            acc[j][3] = sa.w + acc[j][3] * ia;
                         }
                     }
                 }

@@ -1,15 +1,5 @@
-// k_geometry.hip -- transform + cull, monotonisation + count, the chain-parallel intersection
-// write, and fragment-key generation (gfx950).
-//
-// The intersection write (reference: make_intersection_1.comp:217-447, one thread walking a
-// whole curve) is split along its true dependency structure.  Inside one monotone piece the
-// reference interleaves x-grid and y-grid crossings, but every bisection for the next crossing
-// on one axis starts from the previous crossing of the SAME axis (the t_min it starts from is
-// that axis' pending value, :338-359, :402).  So each (piece, axis) is an independent chain:
-// one thread per chain computes the exact same t values, and a second pass merges the two
-// chains of a piece in the reference's order (ties to x, :338) with its equal-t OR-merge
-// (:361-375).  Results are bit-identical while the parallelism grows from one thread per curve
-// to one thread per (curve, piece, axis).
+// k_geometry.hip -- transform + cull, monotonisation + count, and fragment-key generation
+// (gfx950).  The intersection write between count and keys lives in k_chain.hip.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -141,122 +131,6 @@ __global__ void k_segment_stats(const int32_t* __restrict__ seg, uint32_t Np, in
     if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(max_len, len);
 }
 
-// Walks the monotone pieces of a visible curve up to `piece` and returns that piece's context:
-// its flagged end parameters, grid, and first entry offset inside the curve's output range.
-struct PieceCtx {
-    float t0, t1;
-    PieceGrid G;
-    int off;
-};
-
-__device__ inline bool setup_piece(const CurvePts& cv, const float q[4], uint32_t n, int piece,
-                                   int w, int h, PieceCtx& pc) {
-    if ((uint32_t)piece > n) return false;
-    float t0 = 0.0f, p0x = cv.x[0], p0y = cv.y[0];
-    int off = 0;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const float traw = (uint32_t)k < n ? q[k] : 1.0f;
-        const float p1x = cv.ex(traw), p1y = cv.ey(traw);
-        const float t1 = flag_piece_end(traw, p1x);
-        const PieceGrid G = piece_grid(p0x, p0y, p1x, p1y, w, h);
-        if (k == piece) {
-            pc.t0 = t0; pc.t1 = t1; pc.G = G; pc.off = off;
-            return true;
-        }
-        off += 1 + G.nx + G.ny;
-        t0 = t1; p0x = p1x; p0y = p1y;
-    }
-    return false;
-}
-
-// One thread per (curve, piece, axis) chain: the crossings of one axis in travel order, each
-// bisection bracketed from the previous crossing of the same axis.  Stores the masked t with
-// the axis in bit 0 at entry slots 1..nx (x) and nx+1..nx+ny (y) of the piece.
-__global__ __launch_bounds__(kBlock) void k_chain_solve(
-    const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
-    const uint32_t* __restrict__ ctype, const uint32_t* __restrict__ cpath,
-    const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
-    const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
-    int h, float* __restrict__ chain_t) {
-    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t c = gid / 10u, slot = gid % 10u;
-    if (c >= C) return;
-    if (!visible_mask(visible[cpath[c]])) return;
-    const int piece = (int)(slot >> 1), side = (int)(slot & 1u);
-    const uint32_t n = ncuts[c];
-    if ((uint32_t)piece > n) return;
-    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
-    const float4 qq = cuts[c];
-    const float q[4] = {qq.x, qq.y, qq.z, qq.w};
-    PieceCtx pc;
-    if (!setup_piece(cv, q, n, piece, w, h, pc)) return;
-    const int cnt = side ? pc.G.ny : pc.G.nx;
-    if (cnt == 0) return;
-    const float start = side ? pc.G.y0 : pc.G.x0;
-    const float d = side ? pc.G.dy : pc.G.dx;
-    const int first = offsets[c] + pc.off + 1 + (side ? pc.G.nx : 0);
-    const int lim = offsets[c + 1];  // never write past this curve's range
-    float* out = chain_t + first;
-    float tprev = pc.t0;
-    float line = start;
-    for (int k = 0; k < cnt; ++k) {
-        const float ts = solve_crossing(cv, side, line, tprev, pc.t1);
-        const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | (uint32_t)side);
-        if (first + k < lim) out[k] = v;
-        tprev = v;
-        line += d;
-    }
-}
-
-// One thread per (curve, piece): merges the piece start and the two chains in the reference's
-// order and applies its equal-t OR-merge (make_intersection_1.comp:301-375).
-__global__ __launch_bounds__(kBlock) void k_piece_merge(
-    const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
-    const uint32_t* __restrict__ ctype, const uint32_t* __restrict__ cpath,
-    const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
-    const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
-    int h, const float* __restrict__ chain_t, int2* __restrict__ inter) {
-    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t c = gid / 5u;
-    const int piece = (int)(gid % 5u);
-    if (c >= C) return;
-    if (!visible_mask(visible[cpath[c]])) return;
-    const uint32_t n = ncuts[c];
-    if ((uint32_t)piece > n) return;
-    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
-    const float4 qq = cuts[c];
-    const float q[4] = {qq.x, qq.y, qq.z, qq.w};
-    PieceCtx pc;
-    if (!setup_piece(cv, q, n, piece, w, h, pc)) return;
-    const int nx = pc.G.nx, ny = pc.G.ny;
-    const int base = offsets[c] + pc.off;
-    if (base + nx + ny + 1 > offsets[c + 1]) return;  // count and write passes disagree: skip
-    const float* X = chain_t + base + 1;
-    const float* Y = X + nx;
-    const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
-    float tx = pc.t0, ty = pc.t0;
-    int xi = 0, yi = 0;
-    if (nx == 0) tx = kX2;
-    else if (ny == 0) ty = kY2;
-    else { tx = X[0]; xi = 1; }
-    const int n_loop = nx + ny + 1;
-    uint32_t last = fbits(-1.0f);
-    int2* out = inter + base;
-    for (int i = 0; i < n_loop; ++i) {
-        float tmin;
-        if (tx <= ty) { tmin = tx; tx = xi < nx ? X[xi++] : kX2; }
-        else { tmin = ty; ty = yi < ny ? Y[yi++] : kY2; }
-        uint32_t o = fbits(tmin);
-        if ((o & 0xFFFFFFFCu) == (last & 0xFFFFFFFCu)) {
-            o |= last;
-            out[i - 1] = make_int2((int)c, (int)o);
-        }
-        out[i] = make_int2((int)c, (int)o);
-        last = o;
-    }
-}
-
 // gen_fragment.comp:90-224: cell key of the segment between consecutive intersections of one
 // curve, and its winding change across the cell's centre row.  Returns the entry's bucket.
 __device__ inline uint32_t gen_one(const int2* __restrict__ inter, int64_t N, int64_t i,
@@ -353,26 +227,6 @@ hipError_t launch_path_segments(const uint32_t* cpath, uint32_t C, const int32_t
 hipError_t launch_segment_stats(const int32_t* seg, uint32_t Np, int32_t* max_len, hipStream_t s) {
     if (Np == 0) return hipSuccess;
     k_segment_stats<<<grid_for(Np), kBlock, 0, s>>>(seg, Np, max_len);
-    return hipGetLastError();
-}
-
-hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
-                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
-                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w,
-                              int h, float* chain_t, hipStream_t s) {
-    if (C == 0) return hipSuccess;
-    k_chain_solve<<<grid_for((int64_t)C * 10), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts,
-                                                               ncuts, offsets, C, w, h, chain_t);
-    return hipGetLastError();
-}
-
-hipError_t launch_piece_merge(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
-                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
-                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w,
-                              int h, const float* chain_t, int2* inter, hipStream_t s) {
-    if (C == 0) return hipSuccess;
-    k_piece_merge<<<grid_for((int64_t)C * 5), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts,
-                                                              ncuts, offsets, C, w, h, chain_t, inter);
     return hipGetLastError();
 }
 

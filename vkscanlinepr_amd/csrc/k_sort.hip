@@ -7,9 +7,9 @@
 // whose unsigned order equals the reference's (key, index) order, is unique, and carries the
 // winding change, so the reference's separate gather pass disappears.
 //
-// After the bucket scatter every band (<= 4096 entries, a run of whole buckets of one path) only
-// needs sorting in place: one 256-thread workgroup loads it once, sorts it in LDS (16-element
-// register network per thread, then merge-path rounds) and stores it once.  A band above 4096
+// After the bucket scatter every band (<= 2048 entries, a run of whole buckets of one path) only
+// needs sorting in place: one 256-thread workgroup loads it once, sorts it in LDS (8-element
+// register network per thread, then merge-path rounds) and stores it once.  A band above 2048
 // entries is a single oversized (path, row) bucket; such bands are queued and each is sorted by
 // one workgroup with LDS chunk sorts followed by merge-path passes through global memory.
 #include "kernels.hpp"
@@ -19,7 +19,7 @@ namespace spr {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kItems = 16;
+constexpr int kItems = 8;
 static_assert(kThreads * kItems == kSortTile, "tile shape");
 typedef unsigned long long u64;
 constexpr u64 kPad = ~0ull;
@@ -29,7 +29,7 @@ __device__ inline void cswap(u64& a, u64& b) {
     a = lo; b = hi;
 }
 
-// Ascending bitonic network over 16 registers (indices are compile-time constants).
+// Ascending bitonic network over kItems registers (indices are compile-time constants).
 __device__ inline void sort16(u64 (&v)[kItems]) {
 #pragma unroll
     for (int k = 2; k <= kItems; k <<= 1) {
@@ -62,12 +62,12 @@ __device__ inline int64_t merge_split(PA A, int64_t na, PB B, int64_t nb, int64_
     return lo;
 }
 
-// LDS slot of element i: one pad word per 16 elements, so the 16-element runs owned by
+// LDS slot of element i: one pad word per 8 elements, so the 8-element runs owned by
 // consecutive lanes start on different banks.
-constexpr int kPadded = kSortTile + kSortTile / 16;
-__device__ inline int pad(int i) { return i + (i >> 4); }
+constexpr int kPadded = kSortTile + kSortTile / 8;
+__device__ inline int pad(int i) { return i + (i >> 3); }
 
-// Sorts n <= 4096 words from src into dst with one workgroup (buf: 2 x kPadded words of LDS).
+// Sorts n <= kSortTile words from src into dst with one workgroup (buf: 2 x kPadded words of LDS).
 __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ dst, int n, u64 (*buf)[kPadded]) {
     int P2 = kItems;
     while (P2 < n) P2 <<= 1;

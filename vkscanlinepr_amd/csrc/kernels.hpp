@@ -32,14 +32,33 @@ hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t
 hipError_t launch_path_segments(const uint32_t* cpath, uint32_t C, const int32_t* offsets,
                                 uint32_t Np, int32_t* seg, hipStream_t s);
 hipError_t launch_segment_stats(const int32_t* seg, uint32_t Np, int32_t* max_len, hipStream_t s);
-hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
-                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
-                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w,
-                              int h, float* chain_t, hipStream_t s);
-hipError_t launch_piece_merge(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
-                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
-                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w,
-                              int h, const float* chain_t, int2* inter, hipStream_t s);
+// One (curve, piece, axis) chain of dependent crossing solves, and one monotone piece.
+struct ChainDesc {
+    int32_t curve, n, out;  // curve, number of crossings, first output slot
+    int16_t side, cubic;
+    float line0, step;      // first grid line and its step (+-2)
+    float t0, t1;           // flagged piece start and end parameters
+};
+struct PieceDesc {
+    int32_t base, nx, ny, curve;  // base = -1: no piece in this slot
+    float t0;
+};
+constexpr int kChainClasses = 32;
+hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                             const uint32_t* cpath, const int32_t* visible, const float4* cuts,
+                             const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
+                             ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s);
+hipError_t launch_chain_order(const ChainDesc* chains, uint32_t C, const int32_t* class_count,
+                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order,
+                              hipStream_t s);
+hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
+                              const int32_t* order, const int32_t* class_off, const int32_t* n_long,
+                              int32_t* work, float* chain_t, uint8_t* mono, int n_cu, hipStream_t s);
+hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const ChainDesc* chains, const uint8_t* mono,
+                              const float* chain_t, int32_t* raw, int8_t* start_mark, int2* inter,
+                              int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, int n_cu,
+                              hipStream_t s);
+hipError_t launch_or_merge(const int32_t* raw, const int8_t* start_mark, int64_t N, int2* inter, hipStream_t s);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
@@ -51,7 +70,7 @@ hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl c
 
 // ---- (path, row) buckets and the band sort (a9 + a10): per path segment, ascending
 // (key ^ 0x80000000, index); the winding change rides along in the low bits of the sort word.
-constexpr int kSortTile = 4096;
+constexpr int kSortTile = 2048;
 hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, uint32_t Np, int rmax,
                                 int32_t* nb, int2* prow, hipStream_t s);
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,

@@ -101,6 +101,11 @@ struct spr_handle {
     DBuf<int32_t> counts, seg;
     DBuf<int2> inter;
     DBuf<float> chain_t;
+    DBuf<spr::ChainDesc> chains;
+    DBuf<spr::PieceDesc> pieces;
+    DBuf<int32_t> chain_order, long_list;
+    DBuf<uint8_t> mono, pieces_seq;
+    int n_cu = 256;
     DBuf<int32_t> key;
     DBuf<int8_t> wn;
     DBuf<uint32_t> epath;
@@ -133,7 +138,7 @@ namespace {
 struct ZeroLayout {
     size_t a_words = 0, total = 0;
     size_t ctr_a = 0, st_count = 0, st_pbase = 0;
-    size_t ctr = 0, totals = 0, misc = 0, st_hist = 0, st_band = 0, st_wind = 0, st_mark = 0;
+    size_t ctr = 0, totals = 0, misc = 0, chainctl = 0, st_hist = 0, st_band = 0, st_wind = 0, st_mark = 0;
     size_t hist = 0, fill = 0, brec = 0;
 };
 
@@ -147,6 +152,7 @@ ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
     z.ctr = o; o += 4;                 // eight u32 counters of the phase-B scans
     z.totals = o; o += 2;              // int32 cells, spans, R, spare
     z.misc = o; o += 1;                // int32 error flag, big-band count
+    z.chainctl = o; o += 64;           // chain classes: counts[32], fill[32], offsets[33], n_long, work, long pieces
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
     z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
     z.st_wind = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
@@ -242,7 +248,7 @@ void run_frame(spr_handle* H) {
     const ZeroLayout z = zero_layout(C, Np, N, NB);
     H->zero.ensure(z.total);
     const size_t n1 = (size_t)std::max<int64_t>(N, 1), nb1 = (size_t)std::max<int64_t>(NB, 1);
-    const int64_t max_bands = (int64_t)Np + 3 * (N / 2048) + 2;
+    const int64_t max_bands = (int64_t)Np + 3 * (N / 1024) + 2;
     H->inter.ensure(n1);
     H->chain_t.ensure(n1);
     H->key.ensure(n1);
@@ -269,21 +275,43 @@ void run_frame(spr_handle* H) {
     int2* d_brec = reinterpret_cast<int2*>(H->zero.p + z.brec);
 
     const bool gen = !H->inj && N > 0;
+    int32_t* d_cc = reinterpret_cast<int32_t*>(H->zero.p + z.chainctl);  // class counts [32]
+    int32_t* d_cfill = d_cc + 32;                                         // class fill [32]
+    int32_t* d_coff = d_cc + 64;                                          // class offsets [33]
+    int32_t* d_nlong = d_cc + 97;
+    int32_t* d_work = d_cc + 98;
+    int32_t* d_nlpieces = d_cc + 99;
     if (H->inj) {
         SPR_HIP(hipMemcpyAsync(H->inter.p, H->inj_inter.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
     } else if (gen) {
-        SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
-                                        H->ncuts.p, H->counts.p, C, w, h, H->chain_t.p, s));
+        H->chains.ensure((size_t)C * 10);
+        H->pieces.ensure((size_t)C * 5);
+        H->chain_order.ensure((size_t)C * 10);
+        H->long_list.ensure((size_t)C * 5);
+        H->mono.ensure((size_t)C * 10);
+        H->pieces_seq.ensure((size_t)C * 5 + 4);
+        SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
+                                       H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, d_cc, s));
+        SPR_HIP(spr::launch_chain_order(H->chains.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, s));
     }
     stage_mark(H, 4);
     if (gen) {
-        SPR_HIP(spr::launch_piece_merge(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
-                                        H->ncuts.p, H->counts.p, C, w, h, H->chain_t.p, H->inter.p, s));
+        SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong,
+                                        d_work, H->chain_t.p, H->mono.p, H->n_cu, s));
     }
     stage_mark(H, 5);
+    if (gen) {
+        // raw merged values and piece-start marks borrow the key and winding buffers, which
+        // gen_fragment fills afterwards
+        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
+        SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->chains.p, H->mono.p, H->chain_t.p, H->key.p, H->wn.p,
+                                        H->inter.p, H->long_list.p, d_nlpieces, H->pieces_seq.p, H->n_cu, s));
+        SPR_HIP(spr::launch_or_merge(H->key.p, H->wn.p, N, H->inter.p, s));
+    }
+    stage_mark(H, 6);
     SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
                                      H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err, s));
-    stage_mark(H, 6);
+    stage_mark(H, 7);
     // ---- sort: bucket offsets, scatter into bucket order, band sort in LDS
     if (N > 0) {
         SPR_HIP(spr::launch_scan_i32(d_hist, d_hist, NB, ctl_at(H, z.st_hist, z.ctr, 0), s));
@@ -292,25 +320,27 @@ void run_frame(spr_handle* H) {
         SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, d_hist, H->bflag.p, s));
         SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, NB, ctl_at(H, z.st_band, z.ctr, 1), s));
         SPR_HIP(spr::launch_band_emit(H->bflag.p, d_hist, (int32_t)NB, N, H->band_start.p, s));
+    }
+    stage_mark(H, 8);
+    if (N > 0) {
         SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->sorted.p,
                                        H->big_list.p, d_bigcount, s));
         SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->sbig.p, H->stmp.p,
                                            H->sorted.p, s));
         SPR_HIP(spr::launch_unpack_sorted(H->sorted.p, N, H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s));
     }
-    stage_mark(H, 7);
+    stage_mark(H, 9);
     // ---- winding numbers (global exclusive scan, scanline_rasterizer.cpp:478-506)
     SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, N, ctl_at(H, z.st_wind, z.ctr, 2), s));
-    stage_mark(H, 8);
+    stage_mark(H, 10);
     SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
                                   ctl_at(H, z.st_mark, z.ctr, 3), H->records.p, debug ? H->flags.p : nullptr,
                                   d_totals, H->bucket_base.p, H->prow.p, d_brec, s));
-    stage_mark(H, 9);
-    stage_mark(H, 10);
+    stage_mark(H, 11);
     // ---- composite
     SPR_HIP(spr::launch_composite(H->records.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
                                   (int)H->opts.compose, H->image.p, s));
-    stage_mark(H, 11);
+    stage_mark(H, 12);
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 8, d_err, 4, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipEventRecord(H->ev_end, s));
@@ -425,10 +455,14 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
         if (H->opts.device >= 0) dev = H->opts.device; else SPR_HIP(hipGetDevice(&dev));
         H->device = dev;
         SPR_HIP(hipSetDevice(dev));
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+            H->n_cu = ncu;
+        (void)hipGetLastError();
         SPR_HIP(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
         SPR_HIP(hipEventCreate(&H->ev_begin));
         SPR_HIP(hipEventCreate(&H->ev_end));
-        H->ev_stage.resize(13);
+        H->ev_stage.resize(16);
         for (auto& e : H->ev_stage) SPR_HIP(hipEventCreate(&e));
         SPR_HIP(hipHostMalloc((void**)&H->h_pinned, 64, hipHostMallocDefault));
         std::memset(H->h_pinned, 0, 64);
@@ -448,7 +482,8 @@ int spr_destroy(spr_handle* H) {
     H->pos.release(); H->pos_path.release(); H->cpos.release(); H->ctype.release(); H->cpath.release();
     H->rule.release(); H->fill.release(); H->tpos.release(); H->visible.release(); H->cuts.release();
     H->ncuts.release(); H->counts.release(); H->seg.release(); H->inter.release();
-    H->chain_t.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
+    H->chain_t.release(); H->chains.release(); H->pieces.release(); H->chain_order.release();
+    H->long_list.release(); H->mono.release(); H->pieces_seq.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release(); H->swn.release();
     H->winding.release(); H->flags.release(); H->records.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
