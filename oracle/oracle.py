@@ -6,11 +6,14 @@ cpu_baseline leg, always as the checker; never by the product package.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "build" / "libspr_oracle.so"
+# SPR_ORACLE_LIB selects another build of the same source (used to check that the tests catch
+# mistakes planted in a copy of the oracle).
+LIB_PATH = Path(os.environ.get("SPR_ORACLE_LIB", Path(__file__).resolve().parent / "build" / "libspr_oracle.so"))
 STAGES = {
     "tpos": 0, "visible": 1, "cuts": 2, "counts": 3, "inter": 4, "key": 5, "wn": 6, "seg": 7,
     "skey": 8, "sidx": 9, "winding": 10, "flags": 11, "records": 12, "image": 13, "path": 14,
@@ -43,6 +46,8 @@ def lib() -> C.CDLL:
         L.oracle_path_visible.argtypes = [C.c_int32]
         L.oracle_float2int_rd.argtypes = [C.c_float]
         L.oracle_float2int_rd.restype = C.c_int32
+        L.oracle_draw.argtypes = [i32p, C.c_int32, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_uint8)]
+        L.oracle_draw.restype = None
         _lib = L
     return _lib
 
@@ -105,3 +110,32 @@ class Frame:
 
 def max_threads() -> int:
     return int(lib().oracle_max_threads())
+
+
+def draw(records, width: int, height: int, compose: int = 0) -> np.ndarray:
+    """The draw stage alone: (R, 4) int32 records to an RGBA8 image (top row first)."""
+    r = np.ascontiguousarray(records, dtype=np.int32).reshape(-1)
+    out = np.empty((height, width, 4), dtype=np.uint8)
+    lib().oracle_draw(r.ctypes.data_as(C.POINTER(C.c_int32)), len(r) // 4, int(width), int(height),
+                      int(compose), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out
+
+
+def curve_cuts(type_: int, xs, ys, visible: bool = True):
+    """Monotone cuts of one curve (before the t=1 append): (n, [q0..q3])."""
+    q = np.zeros(4, dtype=np.float32)
+    x = np.ascontiguousarray(xs, dtype=np.float32)
+    y = np.ascontiguousarray(ys, dtype=np.float32)
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    n = lib().oracle_curve_cuts(int(type_), f(x), f(y), 1 if visible else 0, f(q))
+    return n, q
+
+
+def solve_quad(a: float, b: float, c: float):
+    r0, r1 = C.c_float(), C.c_float()
+    lib().oracle_solve_quad(a, b, c, C.byref(r0), C.byref(r1))
+    return np.float32(r0.value), np.float32(r1.value)
+
+
+def path_visible(mask: int) -> bool:
+    return bool(lib().oracle_path_visible(int(np.int32(np.uint32(mask & 0xFFFFFFFF)))))

@@ -713,6 +713,16 @@ float oracle_solve_crossing(uint32_t type, const float* xs, const float* ys, int
 
 int oracle_path_visible(int32_t mask) { return path_visible(mask) ? 1 : 0; }
 
+// Draw stage alone (scanlinepr.vert/.frag + pipeline state): records (4 ints each) to RGBA8.
+void oracle_draw(const int32_t* records, int32_t R, int w, int h, int compose, uint8_t* out) {
+    Frame F;
+    F.w = w; F.h = h; F.compose = compose;
+    F.records.assign(records, records + 4 * (size_t)R);
+    F.n_cells = R; F.n_spans = 0;
+    stage_draw(F);
+    std::memcpy(out, F.image.data(), F.image.size());
+}
+
 int32_t oracle_float2int_rd(float x) { return float2int_rd(x); }
 
 }  // extern "C"
