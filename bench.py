@@ -86,6 +86,17 @@ def cpu_baseline(kind, seed, n_paths, canvas, compose, budget_s=20.0):
                       f"one frame in {dt:.2f} s"}
 
 
+def measured_traffic(workload, mode, stage):
+    """HBM bytes per launch of a stage's kernel(s), from the committed rocprofv3 FETCH_SIZE +
+    WRITE_SIZE summary of this workload (already scaled by the copy-kernel calibration), or
+    None when no such measurement is committed."""
+    f = ROOT / "profiles" / f"traffic_{workload}_{mode}.json"
+    if not f.exists():
+        return None
+    d = json.loads(f.read_text()).get("stages", {}).get(stage)
+    return None if d is None else d.get("bytes_per_launch")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,7 +106,19 @@ def main(argv=None):
     ap.add_argument("--mode", default="plain", choices=["plain", "learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stages", action="store_true", help="print per-stage times to stderr")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device-to-device copies of 1 GiB (the reference kernel that "
+                         "scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)")
     a = ap.parse_args(argv)
+    if a.copy_calibration:
+        import torch
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0").fill_(1)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel(), "copies": 5}))
+        return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build renders on one GPU; multi-GPU runs are not part of it "
               "(see DESIGN.md for how the path would shard).", file=sys.stderr)
@@ -148,7 +171,8 @@ def main(argv=None):
                    "compose": "src_over" if compose else "overwrite",
                    "device_ms_per_frame": float(np.mean(frame_ms))},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(a.workload, a.mode, dom),
                      "algorithmic_bytes": algo[dom], "kernel_ms": stages[dom]},
         "stages": table,
     }

@@ -140,3 +140,24 @@ def test_empty_and_offcanvas(spr, orc):
     r.load_paths(np.zeros((0, 2)), [], [], [], [], [], [])
     img = r.render()
     assert (img == 255).all()
+
+
+def test_golden_vectors(spr):
+    """The HIP path against the committed golden vectors (oracle outputs on fixed scenes)."""
+    import make_golden
+    g = np.load(make_golden.HERE / "oracle_golden.npz")
+    for name, arrays, w, h, mvp, compose in make_golden.cases(spr):
+        print(f"[golden] case={name}", flush=True)
+        r = spr.Rasterizer(w, h, compose=compose, debug=True)
+        r.load_paths(arrays["pos"], arrays["pos_path"], arrays["curve_pos"], arrays["curve_type"],
+                     arrays["curve_path"], arrays["rule"], arrays["fill"])
+        if mvp is not None:
+            r.set_mvp(mvp)
+        img = r.render()
+        for stage in make_golden.STAGES:
+            k = f"{name}/{stage}"
+            if k in g:
+                got = r.export(stage)[: len(g[k])]
+                assert np.array_equal(got, g[k]), k
+        assert make_golden.digest(img) == str(g[f"{name}/image_sha256"]), name
+        r.close()

@@ -165,3 +165,22 @@ def test_write_rvg_round_trip(spr, tmp_path, kind):
         k = a["curve_type"][c]
         assert np.array_equal(a["pos"][a["curve_pos"][c]:a["curve_pos"][c] + k],
                               b["pos"][b["curve_pos"][c]:b["curve_pos"][c] + k]), c
+
+
+def test_cpp_adapter_header_compiles_and_links(spr, tmp_path):
+    """include/spr_rasterizer.hpp (the C++ face with the reference's method set) compiles with
+    strict warnings and links against libspr.so; no GPU call is made."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "spr_rasterizer.hpp"\n'
+                   "int main() { spr::Rasterizer r; spr_stats s{}; (void)s;\n"
+                   "  try { r.render(); } catch (const std::runtime_error&) { return 0; }\n"
+                   "  return 1; }\n")
+    lib_dir = spr.LIB_PATH.parent
+    exe = tmp_path / "t"
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}", str(src),
+                    f"-L{lib_dir}", "-lspr", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0

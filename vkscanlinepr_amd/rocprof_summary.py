@@ -1,0 +1,108 @@
+"""Turns rocprofv3 CSV output into the short summaries kept under profiles/.
+
+    python -m vkscanlinepr_amd.rocprof_summary kernels STATS_CSV TITLE COMMAND > profiles/X.md
+    python -m vkscanlinepr_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR COPY_FETCH_DIR \
+        COPY_WRITE_DIR WORKLOAD MODE COMMAND > profiles/traffic_WORKLOAD_MODE.json
+
+`traffic` averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel, scales each by the
+reading of a plain 1 GiB device-to-device copy (bytes moved / counter reading), and sums the
+kernels of each pipeline stage into HBM bytes per launch of that stage.
+"""
+from __future__ import annotations
+
+import collections
+import csv
+import json
+import re
+import sys
+from pathlib import Path
+
+# pipeline stage -> kernels (names without namespace) whose traffic it sums
+STAGE_KERNELS = {
+    "transform": ["k_transform"],
+    "count": ["k_count"],
+    "chain_prep": ["k_chain_prep", "k_chain_offsets", "k_chain_order"],
+    "chain_solve": ["k_chain_solve"],
+    "piece_merge": ["k_piece_merge", "k_piece_merge_wave", "k_or_merge"],
+    "gen_fragment": ["k_gen_fragment"],
+    "bucket_scatter": ["k_bucket_scatter", "k_band_flags", "k_band_emit"],
+    "band_sort": ["k_sort_bands", "k_sort_big_bands", "k_unpack_sorted"],
+    "mark_emit": ["k_mark_emit"],
+    "composite": ["k_composite"],
+}
+
+
+def kernel_name(full: str) -> str:
+    s = re.sub(r"\(anonymous namespace\)::", "", full)
+    s = re.sub(r"^void ", "", s)
+    s = s.split("(")[0]
+    s = s.split("<")[0]
+    return s.split("::")[-1]
+
+
+def _per_kernel(dirpath: Path, counter: str):
+    rows = list(csv.DictReader(open(next(Path(dirpath).glob("*counter_collection.csv")))))
+    agg = collections.defaultdict(list)
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        agg[(kernel_name(r["Kernel_Name"]), int(r["Grid_Size"]))].append(float(r["Counter_Value"]))
+    return agg
+
+
+def traffic(fetch_dir, write_dir, cfetch_dir, cwrite_dir, workload, mode, command):
+    one_gib_kib = float(1 << 20)
+    cf = _per_kernel(cfetch_dir, "FETCH_SIZE")
+    cw = _per_kernel(cwrite_dir, "WRITE_SIZE")
+    copy_f = [v for (k, _), vs in cf.items() if "copyBuffer" in k for v in vs]
+    copy_w = [v for (k, _), vs in cw.items() if "copyBuffer" in k for v in vs]
+    f_scale = one_gib_kib / (sum(copy_f) / len(copy_f))
+    w_scale = one_gib_kib / (sum(copy_w) / len(copy_w))
+    fe = _per_kernel(fetch_dir, "FETCH_SIZE")
+    wr = _per_kernel(write_dir, "WRITE_SIZE")
+    # the scan kernel runs several times per frame at different sizes: the winding scan is the
+    # launch with the largest grid
+    kern = collections.defaultdict(lambda: [0.0, 0.0])
+    scans = [g for (k, g) in fe if k == "k_scan_i32"]
+    for (k, g), vs in fe.items():
+        name = "k_scan_i32@winding" if k == "k_scan_i32" and scans and g == max(scans) else k
+        kern[name][0] += sum(vs) / len(vs) * f_scale * 1024
+    for (k, g), vs in wr.items():
+        name = "k_scan_i32@winding" if k == "k_scan_i32" and scans and g == max(scans) else k
+        kern[name][1] += sum(vs) / len(vs) * w_scale * 1024
+    stages = {}
+    for st, ks in list(STAGE_KERNELS.items()) + [("winding_scan", ["k_scan_i32@winding"])]:
+        have = [k for k in ks if k in kern]
+        if not have:
+            continue
+        fb = sum(kern[k][0] for k in have)
+        wb = sum(kern[k][1] for k in have)
+        stages[st] = {"kernels": have, "fetch_bytes": round(fb), "write_bytes": round(wb),
+                      "bytes_per_launch": round(fb + wb)}
+    return {"workload": workload, "mode": mode, "command": command,
+            "calibration": {"copy_bytes": 1 << 30, "fetch_scale": f_scale, "write_scale": w_scale,
+                            "note": "bytes moved by a 1 GiB device-to-device copy / counter reading"},
+            "stages": stages}
+
+
+def kernels(stats_csv, title, command):
+    rows = list(csv.DictReader(open(stats_csv)))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = [f"# {title}", "", "Command (run from the repository root on one MI355X, after "
+           "`python -m vkscanlinepr_amd.build`):", "", f"    {command}", "",
+           "rocprofv3 kernel-trace summary (per-kernel device time):", "",
+           "| kernel | calls | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        out.append(f"| {kernel_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+                   f"{100 * float(r['TotalDurationNs']) / total:.1f} |")
+    return "\n".join(out) + "\n"
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "traffic":
+        print(json.dumps(traffic(*sys.argv[2:9]), indent=1))
+    elif sys.argv[1] == "kernels":
+        sys.stdout.write(kernels(*sys.argv[2:5]))
+    else:
+        raise SystemExit(__doc__)
