@@ -128,6 +128,13 @@ def main(argv=None):
     scene = V.Scene.generate(kind, seed, n_paths, canvas)
     mode = V.MODE_LEARNED if a.mode == "learned" else V.MODE_PLAIN
     r = V.Rasterizer(canvas, canvas, compose=compose, mode=mode, device=0)
+    learned = None
+    if mode == V.MODE_LEARNED:
+        # the predictor is trained here, on the GPU, on 50,000 paths of the same generator
+        # (another seed) counted by the plain method; not part of the timed region
+        train = V.Scene.generate(kind, 4, 50000, canvas)
+        learned = {"train": r.learned_train(train, chunk_paths=5000)}
+        del train
     r.load_scene(scene)
     r.set_stage_timing(True)
     for _ in range(a.warmup):
@@ -176,6 +183,31 @@ def main(argv=None):
                      "algorithmic_bytes": algo[dom], "kernel_ms": stages[dom]},
         "stages": table,
     }
+    if learned is not None:
+        # the plain two-pass form on the same scene, same process, same step count
+        p = V.Rasterizer(canvas, canvas, compose=compose, mode=V.MODE_PLAIN, device=0)
+        p.load_scene(scene)
+        for _ in range(a.warmup):
+            p.render(readback=False)
+        p.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(a.steps):
+            p.render(readback=False)
+            p.stats()
+        p.synchronize()
+        plain_ms = (time.perf_counter() - t1) / a.steps * 1e3
+        same = bool(np.array_equal(p.render(), r.render()))
+        visible = int(np.sum(r.export("features").reshape(-1, 16)[:, 12] > 0))
+        learned.update({
+            "plain_ms_per_step": plain_ms, "speedup_vs_plain": plain_ms / ms_per_step,
+            "overflow_paths": int(st.n_overflow_paths), "visible_paths": visible,
+            "overflow_rate": st.n_overflow_paths / max(1, visible),
+            "alloc_intersections": int(st.alloc_intersections),
+            "over_allocation": st.alloc_intersections / max(1, N) - 1.0,
+            "fallback_frames": int(r.stats().learned_fallbacks), "image_identical_to_plain": same,
+        })
+        out["learned"] = learned
+        p.close()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kind, seed, n_paths, canvas, compose)
     print(json.dumps(out))

@@ -67,6 +67,8 @@ typedef struct spr_stats {
     int64_t alloc_intersections; /* LEARNED: predicted capacity summed over paths */
     double frame_ms;           /* device time of the last frame (HIP events) */
     double stage_ms[16];       /* per-stage device time when timing is enabled */
+    int64_t learned_fallbacks; /* LEARNED: frames that read their budget bound back to the host
+                                  (the first frame after a load, MVP or model change) */
 } spr_stats;
 
 /* Debug stage identifiers for spr_debug_export / spr_debug_inject. */
@@ -85,7 +87,10 @@ enum {
     SPR_STAGE_FLAGS = 11,      /* int32 x 2N: cell flags, then span flags */
     SPR_STAGE_RECORDS = 12,    /* int4 x R: (yx, width, fill, frag_index) output records */
     SPR_STAGE_IMAGE = 13,      /* uint8 x 4WH: RGBA8 image, top row first */
-    SPR_STAGE_COUNT_ = 14
+    SPR_STAGE_FEATURES = 14,   /* float x 16Np: per-path predictor features (last learned frame) */
+    SPR_STAGE_PRED = 15,       /* float x 2Np: predicted log2(1 + intersections), log2(1 + records) */
+    SPR_STAGE_PATH_COUNTS = 16,/* int32 x 2Np: exact (intersections, records) per path, last frame */
+    SPR_STAGE_COUNT_ = 17
 };
 
 /* --- lifecycle: VGRasterizer::initialize(void* window, uint32_t w, uint32_t h),
@@ -160,6 +165,43 @@ int spr_scene_subset(const spr_scene* s, uint32_t first, uint32_t count, spr_sce
 int spr_scene_view_get(const spr_scene* s, spr_scene_view* out);
 int spr_scene_write_rvg(const spr_scene* s, const char* path);
 void spr_scene_free(spr_scene* s);
+
+/* --- learned single-pass allocation.  No reference counterpart: north of the reference's
+ *     count pass (make_intersection_0.comp) this predicts each path's intersection and record
+ *     counts with a 16-64-64-2 MLP (BF16 matrix-core inference, FP32 training kernels).  The
+ *     network is fitted on the exact per-path counts of a training scene, which is rendered on
+ *     this handle with the plain method chunk by chunk; afterwards no scene is loaded. */
+typedef struct spr_train_opts {
+    uint32_t epochs;          /* 0 = 60 */
+    uint32_t batch;           /* 0 = 512; epochs are added until at least 4000 steps ran */
+    float lr;                 /* Adam step, 0 = 3e-3 */
+    uint32_t chunk_paths;     /* training paths rendered per frame, 0 = 5000 */
+    uint64_t seed;            /* initialisation and shuffling */
+    float overflow_quantile;  /* capacity margin keeps this fraction of paths in their arena, 0 = 0.95 */
+    uint32_t reserved[3];
+} spr_train_opts;
+typedef struct spr_train_report {
+    int64_t samples, steps;
+    double first_loss, train_loss, val_loss; /* mean squared error of log2(1 + count) */
+    double log_margin;                       /* chosen capacity margin (log2) */
+    double collect_ms, train_ms;
+} spr_train_report;
+int spr_learned_train(spr_handle* h, const spr_scene* training_scene, const spr_train_opts* opts,
+                      spr_train_report* out);
+int spr_learned_save(spr_handle* h, const char* path);
+int spr_learned_load(spr_handle* h, const char* path);
+/* capacity = 2^(prediction + log2_margin) - 1 + slack per path */
+int spr_learned_set_margin(spr_handle* h, float log2_margin, int32_t slack);
+/* The model as floats: 5378 parameters (W1[16][64], b1, W2[64][64], b2, W3[64][2], b3), then
+ * the feature mean[16] and 1/std[16] (5410 floats). */
+int spr_mlp_get_params(spr_handle* h, float* dst, size_t cap, size_t* n);
+int spr_mlp_set_params(spr_handle* h, const float* src, size_t n);
+/* BF16 matrix-core inference on n raw feature rows (16 floats each) -> 2 floats per row. */
+int spr_mlp_predict(spr_handle* h, const float* features, uint32_t n, float* out);
+/* FP32 forward + backward of the training kernels on one batch: gradients of the mean squared
+ * error in parameter order (5378 floats) and the loss. */
+int spr_mlp_gradients(spr_handle* h, const float* features, const float* targets, uint32_t n, float* grads,
+                      float* loss);
 
 #ifdef __cplusplus
 }

@@ -23,14 +23,17 @@ COMPOSE_OVERWRITE, COMPOSE_SRC_OVER = 0, 1
 MODE_PLAIN, MODE_LEARNED = 0, 1
 STAGES = {
     "tpos": 0, "visible": 1, "cuts": 2, "counts": 3, "inter": 4, "key": 5, "wn": 6, "seg": 7,
-    "skey": 8, "sidx": 9, "winding": 10, "flags": 11, "records": 12, "image": 13,
+    "skey": 8, "sidx": 9, "winding": 10, "flags": 11, "records": 12, "image": 13, "features": 14,
+    "pred": 15, "path_counts": 16,
 }
 _STAGE_DTYPE = {
     "tpos": np.float32, "visible": np.int32, "cuts": np.float32, "counts": np.int32,
     "inter": np.int32, "key": np.int32, "wn": np.int32, "seg": np.int32, "skey": np.int32,
     "sidx": np.int32, "winding": np.int32, "flags": np.int32, "records": np.int32,
-    "image": np.uint8,
+    "image": np.uint8, "features": np.float32, "pred": np.float32, "path_counts": np.int32,
 }
+MLP_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2  # W1, b1, W2, b2, W3, b3
+MLP_FLOATS = MLP_PARAMS + 32                          # + feature mean[16], 1/std[16]
 
 
 class SprError(RuntimeError):
@@ -48,7 +51,20 @@ class Stats(C.Structure):
     _fields_ = [("n_points", C.c_int64), ("n_curves", C.c_int64), ("n_paths", C.c_int64),
                 ("n_intersections", C.c_int64), ("n_cells", C.c_int64), ("n_spans", C.c_int64),
                 ("n_overflow_paths", C.c_int64), ("alloc_intersections", C.c_int64),
-                ("frame_ms", C.c_double), ("stage_ms", C.c_double * 16)]
+                ("frame_ms", C.c_double), ("stage_ms", C.c_double * 16),
+                ("learned_fallbacks", C.c_int64)]
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float),
+                ("chunk_paths", C.c_uint32), ("seed", C.c_uint64), ("overflow_quantile", C.c_float),
+                ("reserved", C.c_uint32 * 3)]
+
+
+class TrainReport(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("steps", C.c_int64), ("first_loss", C.c_double),
+                ("train_loss", C.c_double), ("val_loss", C.c_double), ("log_margin", C.c_double),
+                ("collect_ms", C.c_double), ("train_ms", C.c_double)]
 
 
 class SceneView(C.Structure):
@@ -66,7 +82,9 @@ EXPORTS = [
     "spr_load_rvg", "spr_load_scene", "spr_set_mvp", "spr_render_frame", "spr_image_device_ptr",
     "spr_synchronize", "spr_set_stage_timing", "spr_get_stats", "spr_debug_export",
     "spr_debug_inject", "spr_scene_load_rvg", "spr_scene_generate", "spr_scene_subset",
-    "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free",
+    "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free", "spr_learned_train",
+    "spr_learned_save", "spr_learned_load", "spr_learned_set_margin", "spr_mlp_get_params",
+    "spr_mlp_set_params", "spr_mlp_predict", "spr_mlp_gradients",
 ]
 
 
@@ -103,6 +121,14 @@ def lib() -> C.CDLL:
     L.spr_scene_write_rvg.argtypes = [vp, C.c_char_p]
     L.spr_scene_free.argtypes = [vp]
     L.spr_scene_free.restype = None
+    L.spr_learned_train.argtypes = [vp, vp, C.POINTER(TrainOpts), C.POINTER(TrainReport)]
+    L.spr_learned_save.argtypes = [vp, C.c_char_p]
+    L.spr_learned_load.argtypes = [vp, C.c_char_p]
+    L.spr_learned_set_margin.argtypes = [vp, C.c_float, C.c_int32]
+    L.spr_mlp_get_params.argtypes = [vp, f32p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.spr_mlp_set_params.argtypes = [vp, f32p, C.c_size_t]
+    L.spr_mlp_predict.argtypes = [vp, f32p, C.c_uint32, f32p]
+    L.spr_mlp_gradients.argtypes = [vp, f32p, f32p, C.c_uint32, f32p, f32p]
     _lib = L
     return L
 
@@ -275,3 +301,56 @@ class Rasterizer:
         a = np.ascontiguousarray(inter, dtype=np.int32).reshape(-1)
         self._c(lib().spr_debug_inject(self._h, STAGES["inter"], a.ctypes.data_as(C.c_void_p),
                                        a.nbytes))
+
+
+def _rasterizer_learned_methods():
+    """Learned single-pass allocation: training, model I/O and the predictor's kernels."""
+
+    def learned_train(self, scene: Scene, epochs: int = 0, batch: int = 0, lr: float = 0.0,
+                      chunk_paths: int = 0, seed: int = 4, overflow_quantile: float = 0.0) -> dict:
+        o = TrainOpts(epochs=epochs, batch=batch, lr=lr, chunk_paths=chunk_paths, seed=seed,
+                      overflow_quantile=overflow_quantile)
+        r = TrainReport()
+        self._c(lib().spr_learned_train(self._h, scene._h, C.byref(o), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TrainReport._fields_}
+
+    def learned_save(self, path) -> None:
+        self._c(lib().spr_learned_save(self._h, str(path).encode()))
+
+    def learned_load(self, path) -> None:
+        self._c(lib().spr_learned_load(self._h, str(path).encode()))
+
+    def learned_set_margin(self, log2_margin: float, slack: int) -> None:
+        self._c(lib().spr_learned_set_margin(self._h, float(log2_margin), int(slack)))
+
+    def mlp_params(self) -> np.ndarray:
+        out = np.empty(MLP_FLOATS, dtype=np.float32)
+        n = C.c_size_t()
+        self._c(lib().spr_mlp_get_params(self._h, _ptr(out, C.c_float), out.size, C.byref(n)))
+        return out
+
+    def mlp_set_params(self, params) -> None:
+        a = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        self._c(lib().spr_mlp_set_params(self._h, _ptr(a, C.c_float), a.size))
+
+    def mlp_predict(self, features) -> np.ndarray:
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 16)
+        out = np.empty((len(x), 2), dtype=np.float32)
+        self._c(lib().spr_mlp_predict(self._h, _ptr(x, C.c_float), len(x), _ptr(out, C.c_float)))
+        return out
+
+    def mlp_gradients(self, features, targets):
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 16)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1, 2)
+        g = np.empty(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._c(lib().spr_mlp_gradients(self._h, _ptr(x, C.c_float), _ptr(t, C.c_float), len(x),
+                                        _ptr(g, C.c_float), C.byref(loss)))
+        return g, float(loss.value)
+
+    for f in (learned_train, learned_save, learned_load, learned_set_margin, mlp_params,
+              mlp_set_params, mlp_predict, mlp_gradients):
+        setattr(Rasterizer, f.__name__, f)
+
+
+_rasterizer_learned_methods()

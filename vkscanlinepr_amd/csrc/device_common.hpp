@@ -16,6 +16,7 @@ constexpr int kFrag = 2;           // FRAG_SIZE (make_intersection_0.comp:3)
 constexpr int kCubicIters = 24;    // CUBIC_ITERATION_NUMBER (make_intersection_1.comp:7)
 constexpr uint32_t kLine = 2u, kCubic = 4u;
 constexpr uint32_t kSentinelKey = 0xFFFEFFFEu;  // off-canvas cell key (gen_fragment.comp:111)
+constexpr uint32_t kNoPath = 0xFFFFFFFFu;      // entry path of a gap in a predicted-allocation layout
 constexpr uint32_t kTwoBits = 0x40000000u;      // bit pattern of 2.0f: an exhausted chain
 
 __host__ __device__ inline uint32_t fbits(float f) { return __builtin_bit_cast(uint32_t, f); }

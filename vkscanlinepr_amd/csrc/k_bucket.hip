@@ -48,15 +48,21 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
                                                            const int32_t* __restrict__ boff,
                                                            int32_t* __restrict__ fill,
                                                            unsigned long long* __restrict__ out,
-                                                           int32_t* __restrict__ err) {
+                                                           int32_t* __restrict__ err,
+                                                           const int32_t* __restrict__ n_dev) {
+    const int64_t n = n_dev ? min((int64_t)*n_dev, N) : N;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = i < N;
+    bool valid = i < n;
     uint32_t b = 0;
     int32_t k = 0;
     if (valid) {
         const uint32_t p = epath[i];
-        k = key[i];
-        b = (uint32_t)bucket_base[p] + bucket_local(k, prow[p], err);
+        if (p == kNoPath) {
+            valid = false;  // gap of a predicted-allocation layout
+        } else {
+            k = key[i];
+            b = (uint32_t)bucket_base[p] + bucket_local(k, prow[p], err);
+        }
     }
     const int32_t slot = wave_slot(fill, b, valid);
     if (valid) {
@@ -111,7 +117,10 @@ __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict
 }
 
 __global__ void k_band_emit(const int32_t* __restrict__ flag_scan, const int32_t* __restrict__ boff,
-                            int32_t NB, int64_t N, int32_t* __restrict__ band_start) {
+                            int32_t NB_max, int64_t N_max, int32_t* __restrict__ band_start,
+                            const int32_t* __restrict__ nb_dev, const int32_t* __restrict__ n_dev) {
+    const int32_t NB = nb_dev ? min(*nb_dev, NB_max) : NB_max;
+    const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
     const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (b >= NB) return;
     if (flag_scan[b + 1] != flag_scan[b]) band_start[flag_scan[b]] = boff[b];
@@ -129,9 +138,11 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
 
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
                                  const int32_t* bucket_base, const int2* prow, const int32_t* boff,
-                                 int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s) {
+                                 int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s,
+                                 const int32_t* n_dev) {
     if (N == 0) return hipSuccess;
-    k_bucket_scatter<<<grid_for(N), kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, boff, fill, out, err);
+    k_bucket_scatter<<<grid_for(N), kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, boff, fill, out, err,
+                                                    n_dev);
     return hipGetLastError();
 }
 
@@ -143,9 +154,9 @@ hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int3
 }
 
 hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,
-                            int32_t* band_start, hipStream_t s) {
+                            int32_t* band_start, hipStream_t s, const int32_t* nb_dev, const int32_t* n_dev) {
     if (NB == 0) return hipSuccess;
-    k_band_emit<<<grid_for(NB), kBlock, 0, s>>>(flag_scan, boff, NB, N, band_start);
+    k_band_emit<<<grid_for(NB), kBlock, 0, s>>>(flag_scan, boff, NB, N, band_start, nb_dev, n_dev);
     return hipGetLastError();
 }
 

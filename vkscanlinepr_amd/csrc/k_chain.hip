@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
     const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
     const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
     int h, ChainDesc* __restrict__ chains, PieceDesc* __restrict__ pieces,
-    int32_t* __restrict__ class_count) {
+    int32_t* __restrict__ class_count, const int32_t* __restrict__ ends) {
     __shared__ int32_t hist[kClasses];
     if (threadIdx.x < kClasses) hist[threadIdx.x] = 0;
     __syncthreads();
@@ -93,7 +93,10 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
         PieceCtx pc;
         if (setup_piece(cv, q, ncuts[c], piece, w, h, pc)) {
             const int base = offsets[c] + pc.off;
-            if (base + pc.G.nx + pc.G.ny + 1 <= offsets[c + 1]) {  // count and write passes agree
+            // the curve's range ends at the next curve's offset, or at ends[c] when curves are
+            // placed in per-path arenas
+            const int limit = ends ? ends[c] : offsets[c + 1];
+            if (base + pc.G.nx + pc.G.ny + 1 <= limit) {  // count and write passes agree
                 pd.base = base; pd.nx = pc.G.nx; pd.ny = pc.G.ny; pd.curve = (int32_t)c; pd.t0 = pc.t0;
 #pragma unroll
                 for (int side = 0; side < 2; ++side) {
@@ -458,9 +461,12 @@ __global__ __launch_bounds__(64) void k_piece_merge_wave(const PieceDesc* __rest
 
 // The reference's OR-merge of equal t inside a piece (make_intersection_1.comp:361-375): within a
 // run [s, e) of equal masked t, entry j ends up as the OR of raw values s..min(j+1, e-1).
+// Gaps of a predicted-allocation layout carry a start mark, so no run crosses into them.
 __global__ __launch_bounds__(kBlock) void k_or_merge(const int32_t* __restrict__ raw,
-                                                     const int8_t* __restrict__ start_mark, int64_t N,
+                                                     const int8_t* __restrict__ start_mark, int64_t N_max,
+                                                     const int32_t* __restrict__ n_dev,
                                                      int2* __restrict__ inter) {
+    const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
     const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (j >= N) return;
     const uint32_t v = (uint32_t)raw[j], m = v & 0xFFFFFFFCu;
@@ -476,10 +482,11 @@ __global__ __launch_bounds__(kBlock) void k_or_merge(const int32_t* __restrict__
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
-                             ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s) {
+                             ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s,
+                             const int32_t* ends) {
     if (C == 0) return hipSuccess;
     k_chain_prep<<<grid_for((int64_t)C * 5), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts, ncuts,
-                                                             offsets, C, w, h, chains, pieces, class_count);
+                                                             offsets, C, w, h, chains, pieces, class_count, ends);
     return hipGetLastError();
 }
 
@@ -512,9 +519,10 @@ hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const ChainDe
     return hipGetLastError();
 }
 
-hipError_t launch_or_merge(const int32_t* raw, const int8_t* start_mark, int64_t N, int2* inter, hipStream_t s) {
+hipError_t launch_or_merge(const int32_t* raw, const int8_t* start_mark, int64_t N, int2* inter, hipStream_t s,
+                           const int32_t* n_dev) {
     if (N == 0) return hipSuccess;
-    k_or_merge<<<grid_for(N), kBlock, 0, s>>>(raw, start_mark, N, inter);
+    k_or_merge<<<grid_for(N), kBlock, 0, s>>>(raw, start_mark, N, n_dev, inter);
     return hipGetLastError();
 }
 

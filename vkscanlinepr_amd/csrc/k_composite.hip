@@ -31,7 +31,8 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
                                                       const int32_t* __restrict__ bucket_base,
                                                       const int2* __restrict__ prow,
                                                       const uint32_t* __restrict__ fill, uint32_t Np,
-                                                      int w, int h, uint8_t* __restrict__ image) {
+                                                      int w, int h, int64_t rec_cap,
+                                                      uint8_t* __restrict__ image) {
     __shared__ unsigned long long mask[kChunk];
     __shared__ int2 list[kBlock];
     __shared__ uint32_t lcol[kBlock];
@@ -61,6 +62,10 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
                 const int2 pr = prow[p];
                 const int loc = r == 0 ? pr.y + 1 : ((r >= pr.x && r < pr.x + pr.y) ? r - pr.x : -1);
                 if (loc >= 0) rng = brec[bucket_base[p] + loc];
+                // ranges never reach past the records actually stored (a frame whose record
+                // budget overflowed is redone by the host; it must still read in bounds)
+                rng.y = (int64_t)rng.y < rec_cap ? rng.y : (int32_t)rec_cap;
+                rng.x = rng.x < rng.y ? rng.x : rng.y;
             }
             const bool keep = rng.y > rng.x;
             const unsigned long long m = __ballot(keep);
@@ -183,11 +188,11 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
 
 hipError_t launch_composite(const int4* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
-                            uint8_t* image, hipStream_t s) {
+                            uint8_t* image, hipStream_t s, int64_t rec_cap) {
     const int rows = (h + 1) / 2;
     if (rows == 0 || w == 0) return hipSuccess;
-    if (compose) k_composite<1><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, image);
-    else k_composite<0><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, image);
+    if (compose) k_composite<1><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image);
+    else k_composite<0><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image);
     return hipGetLastError();
 }
 

@@ -19,6 +19,14 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
         p[i] = v;
 }
 
+// Fill of min(n_max, mult * *n_dev + add) words (a count known only on the device).
+__global__ void k_fill_u32_dev(uint32_t* __restrict__ p, int64_t n_max, const int32_t* __restrict__ n_dev,
+                               int64_t mult, int64_t add, uint32_t v) {
+    const int64_t n = min(n_max, mult * (int64_t)*n_dev + add);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // transform_pos.comp:33-85.  The per-path outcode OR is reduced inside the wave first (points of
 // one path are contiguous), then one atomicOr per (wave, path).
 // It also bounds each path's cell rows from its transformed control points (the curves stay
@@ -182,15 +190,31 @@ __global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict_
                                                          const int32_t* __restrict__ bucket_base,
                                                          const int2* __restrict__ prow,
                                                          int32_t* __restrict__ hist,
-                                                         int32_t* __restrict__ err) {
+                                                         int32_t* __restrict__ err,
+                                                         const int32_t* __restrict__ n_dev) {
+    const int64_t n = n_dev ? min((int64_t)*n_dev, N) : N;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = i < N;
+    bool valid = i < n;
     uint32_t bucket = 0;
-    if (valid) bucket = gen_one(inter, N, i, cpos, ctype, cpath, tpos, w, h, key, wn, epath, bucket_base, prow, err);
+    if (valid && inter[i].x < 0) {
+        // a gap of a predicted-allocation layout: no fragment, no bucket
+        epath[i] = kNoPath;
+        valid = false;
+    }
+    if (valid) bucket = gen_one(inter, n, i, cpos, ctype, cpath, tpos, w, h, key, wn, epath, bucket_base, prow, err);
     wave_count(hist, bucket, valid);
 }
 
 }  // namespace
+
+hipError_t launch_fill_u32_dev(uint32_t* p, int64_t n_max, const int32_t* n_dev, int64_t mult, int64_t add,
+                               uint32_t v, hipStream_t s) {
+    if (n_max <= 0) return hipSuccess;
+    int64_t g = (n_max + kBlock - 1) / kBlock;
+    if (g > 4096) g = 4096;
+    k_fill_u32_dev<<<(unsigned)g, kBlock, 0, s>>>(p, n_max, n_dev, mult, add, v);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -234,10 +258,10 @@ hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpo
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
                                const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
-                               hipStream_t s) {
+                               hipStream_t s, const int32_t* n_dev) {
     if (N == 0) return hipSuccess;
     k_gen_fragment<<<grid_for(N), kBlock, 0, s>>>(inter, N, cpos, ctype, cpath, tpos, w, h, key, wn, epath,
-                                                  bucket_base, prow, hist, err);
+                                                  bucket_base, prow, hist, err, n_dev);
     return hipGetLastError();
 }
 

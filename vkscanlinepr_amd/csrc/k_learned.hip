@@ -1,0 +1,482 @@
+// k_learned.hip -- learned single-pass allocation (gfx950).
+//
+// The plain frame counts every curve's intersections, scans the counts and waits for the host
+// to read the total before anything is written (make_intersection_0.comp -> naive_scan.comp ->
+// host readback, scanline_rasterizer.cpp:317-395).  The learned frame instead:
+//   1. k_path_features: 16 cheap features per path from its transformed control points;
+//   2. k_mlp_infer: a 16-64-64-2 MLP (BF16 weights, FP32 accumulation on the matrix cores)
+//      predicts log2(1 + intersections) and log2(1 + records) per path;
+//   3. k_caps: predicted capacities (with a safety margin) become per-path arenas after one
+//      device scan;
+//   4. k_learned_count: one workgroup per path counts its curves exactly, places them in the
+//      arena when they fit, or takes an exact-size block of the overflow pool when they do not
+//      (the exact fallback), and marks the unused tail of the arena as a gap.
+// No step waits for the host.  A path that fits neither its arena nor the pool's remaining
+// budget is dropped and flagged; the host then redoes the frame with the plain method, so the
+// output never changes.
+//
+// The training kernels (k_dense_*, k_loss_grad, k_adam) are plain FP32 forward/backward passes
+// of the same network, used to fit it on counts produced by the plain method.
+#include "device_common.hpp"
+#include "kernels.hpp"
+#include "scan_common.hpp"
+
+namespace spr {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kF = kMlpIn, kH = kMlpHidden;
+
+__device__ inline short f2bf(float f) {  // round to nearest even (finite inputs)
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+__device__ inline float wave_sum_f(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline float wave_min_f(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline float wave_max_f(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// One wave per path; curves of path p are [pcurve[p], pcurve[p+1]).
+__global__ __launch_bounds__(256) void k_path_features(const float2* __restrict__ tpos,
+                                                       const uint32_t* __restrict__ cpos,
+                                                       const uint32_t* __restrict__ ctype,
+                                                       const uint32_t* __restrict__ pcurve,
+                                                       const int32_t* __restrict__ visible,
+                                                       const uint32_t* __restrict__ rule, uint32_t Np,
+                                                       int w, int h, float* __restrict__ feat,
+                                                       unsigned long long* __restrict__ bound_total) {
+    const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= Np) return;
+    const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
+    float n_cub = 0, xmin = 3.4e38f, xmax = -3.4e38f, ymin = 3.4e38f, ymax = -3.4e38f;
+    float plen = 0, sdx = 0, sdy = 0, cdx = 0, cdy = 0, turn = 0, inside = 0, npts = 0;
+    for (uint32_t c = c0 + lane; c < c1; c += 64) {
+        const uint32_t k = ctype[c] == 4u ? 4u : 2u;
+        const uint32_t b = cpos[c];
+        float2 q[4];
+        for (uint32_t i = 0; i < k; ++i) q[i] = tpos[b + i];
+        n_cub += k == 4u ? 1.0f : 0.0f;
+        float pdx = 0, pdy = 0;
+        for (uint32_t i = 0; i < k; ++i) {
+            xmin = fminf(xmin, q[i].x); xmax = fmaxf(xmax, q[i].x);
+            ymin = fminf(ymin, q[i].y); ymax = fmaxf(ymax, q[i].y);
+            inside += (q[i].x >= 0 && q[i].x < (float)w && q[i].y >= 0 && q[i].y < (float)h) ? 1.0f : 0.0f;
+            npts += 1.0f;
+            if (i > 0) {
+                const float dx = q[i].x - q[i - 1].x, dy = q[i].y - q[i - 1].y;
+                plen += sqrtf(dx * dx + dy * dy);
+                sdx += fabsf(dx); sdy += fabsf(dy);
+                if (i > 1) turn += fabsf(atan2f(pdx * dy - pdy * dx, pdx * dx + pdy * dy));
+                pdx = dx; pdy = dy;
+            }
+        }
+        cdx += fabsf(q[k - 1].x - q[0].x);
+        cdy += fabsf(q[k - 1].y - q[0].y);
+    }
+    n_cub = wave_sum_f(n_cub); plen = wave_sum_f(plen); sdx = wave_sum_f(sdx); sdy = wave_sum_f(sdy);
+    cdx = wave_sum_f(cdx); cdy = wave_sum_f(cdy); turn = wave_sum_f(turn);
+    inside = wave_sum_f(inside); npts = wave_sum_f(npts);
+    xmin = wave_min_f(xmin); xmax = wave_max_f(xmax); ymin = wave_min_f(ymin); ymax = wave_max_f(ymax);
+    if (lane == 0) {
+        const float bw = npts > 0 ? xmax - xmin : 0.0f, bh = npts > 0 ? ymax - ymin : 0.0f;
+        const float cw = npts > 0 ? fmaxf(0.0f, fminf(xmax, (float)w) - fmaxf(xmin, 0.0f)) : 0.0f;
+        const float ch = npts > 0 ? fmaxf(0.0f, fminf(ymax, (float)h) - fmaxf(ymin, 0.0f)) : 0.0f;
+        float* f = feat + (size_t)p * kF;
+        f[0] = log2f(1.0f + (float)(c1 - c0));
+        f[1] = log2f(1.0f + n_cub);
+        f[2] = log2f(1.0f + bw);
+        f[3] = log2f(1.0f + bh);
+        f[4] = log2f(1.0f + plen);
+        f[5] = log2f(1.0f + 0.5f * sdx);
+        f[6] = log2f(1.0f + 0.5f * sdy);
+        f[7] = log2f(1.0f + 0.5f * cdx);
+        f[8] = log2f(1.0f + 0.5f * cdy);
+        f[9] = log2f(1.0f + turn * 0.15915494f);
+        f[10] = log2f(1.0f + cw);
+        f[11] = log2f(1.0f + ch);
+        f[12] = visible_mask(visible[p]) ? 1.0f : 0.0f;
+        f[13] = npts > 0 ? inside / npts : 0.0f;
+        f[14] = (float)rule[p];
+        f[15] = log2f(1.0f + 0.25f * cw * ch);
+        if (bound_total && visible_mask(visible[p])) {
+            // Upper bound of the path's intersections.  A Bezier curve varies no more than its
+            // control polygon; its <= 5 pieces cover [0, 1] at most three times over (the
+            // 4-cut order quirk runs one piece backwards); a piece from a to b gives at most
+            // 1 + (|dx|/2 + 1) + (|dy|/2 + 1) entries.  The float terms carry a 0.1% margin.
+            const double b = 1.5 * ((double)sdx + (double)sdy) * 1.001 + 20.0 * (double)(c1 - c0) + 2.0;
+            atomicAdd(bound_total, (unsigned long long)b);
+        }
+    }
+}
+
+// MLP inference on the matrix cores: one wave per 16 paths, v_mfma_f32_16x16x16_bf16 tiles.
+// Operand layout of the 16x16x16 BF16 MFMA: lane l holds A[l%16][4*(l/16)+q], B[4*(l/16)+q][l%16]
+// (q = 0..3) and receives D[4*(l/16)+q][l%16].
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ feat, uint32_t Np, MlpDev m,
+                                                   float* __restrict__ pred) {
+    __shared__ float act[4][16][kH + 1];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+    const int li = lane & 15, lq = lane >> 4;  // row/col inside a tile, k group
+    // layer 1: 16 inputs -> 64
+    bf16x4 a;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = 4 * lq + q;
+        const int64_t r = r0 + li;
+        const float x = r < Np ? (feat[r * kF + k] - m.mean[k]) * m.inv_std[k] : 0.0f;
+        a[q] = f2bf(x);
+    }
+#pragma unroll
+    for (int t = 0; t < kH / 16; ++t) {
+        bf16x4 b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) b[q] = (short)m.w1[(4 * lq + q) * kH + 16 * t + li];
+        f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+        d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) act[wave][4 * lq + q][16 * t + li] = fmaxf(d[q] + m.b1[16 * t + li], 0.0f);
+    }
+    __syncthreads();
+    // layer 2: 64 -> 64
+    f32x4 d2[kH / 16];
+#pragma unroll
+    for (int t = 0; t < kH / 16; ++t) d2[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < kH / 16; ++ks) {
+        bf16x4 a2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a2[q] = f2bf(act[wave][li][16 * ks + 4 * lq + q]);
+#pragma unroll
+        for (int t = 0; t < kH / 16; ++t) {
+            bf16x4 b;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) b[q] = (short)m.w2[(16 * ks + 4 * lq + q) * kH + 16 * t + li];
+            d2[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b, d2[t], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kH / 16; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) act[wave][4 * lq + q][16 * t + li] = fmaxf(d2[t][q] + m.b2[16 * t + li], 0.0f);
+    __syncthreads();
+    // layer 3: 64 -> 2 (the 16-wide tile is padded with zero columns)
+    f32x4 d3 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < kH / 16; ++ks) {
+        bf16x4 a3, b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a3[q] = f2bf(act[wave][li][16 * ks + 4 * lq + q]);
+            b[q] = (short)m.w3[(16 * ks + 4 * lq + q) * 16 + li];
+        }
+        d3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, b, d3, 0, 0, 0);
+    }
+    if (li < kMlpOut) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t r = r0 + 4 * lq + q;
+            if (r < Np) pred[r * kMlpOut + li] = d3[q] + m.b3[li];
+        }
+    }
+}
+
+// Predicted capacities: cap = 2^(pred + margin) - 1 + slack (0 for culled paths).  capN goes
+// to `arena` (scanned afterwards into arena bases); the record capacities are summed.
+__global__ void k_caps(const float* __restrict__ pred, const int32_t* __restrict__ visible, uint32_t Np,
+                       float log_margin, int32_t slack, int32_t* __restrict__ arena, int32_t* __restrict__ cap_n,
+                       int32_t* __restrict__ cap_r_total) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t cr = 0;
+    if (p < Np) {
+        int32_t cn = 0;
+        if (visible_mask(visible[p])) {
+            const float en = fminf(pred[2 * p] + log_margin, 28.0f);
+            const float er = fminf(pred[2 * p + 1] + log_margin, 28.0f);
+            cn = (int32_t)ceilf(exp2f(en)) - 1 + slack;
+            cr = (int32_t)ceilf(exp2f(er)) - 1 + slack;
+            cn = cn < 0 ? 0 : cn;
+            cr = cr < 0 ? 0 : cr;
+        }
+        arena[p] = cn;
+        cap_n[p] = cn;
+    }
+    for (int o = 32; o >= 1; o >>= 1) cr += __shfl_xor(cr, o);
+    if ((threadIdx.x & 63) == 0 && cr) atomicAdd(cap_r_total, cr);
+}
+
+// One workgroup per path: exact per-curve counts (make_intersection_0.comp:226-411, as
+// k_count), intra-path offsets, placement in the arena or the overflow pool, gap marking.
+// ctl[0] = pool entries taken, ctl[1] = overflowing paths, ctl[2] = live positions.
+__global__ __launch_bounds__(256) void k_learned_count(
+    const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos, const uint32_t* __restrict__ ctype,
+    int32_t* __restrict__ visible, const uint32_t* __restrict__ pcurve, uint32_t Np, int w, int h,
+    const int32_t* __restrict__ arena, const int32_t* __restrict__ cap_n, int64_t budget,
+    float4* __restrict__ cuts, uint32_t* __restrict__ ncuts, int32_t* __restrict__ counts,
+    int32_t* __restrict__ ends, int32_t* __restrict__ ctl, int32_t* __restrict__ n_exact, int32_t* __restrict__ pbase,
+    int2* __restrict__ inter, int8_t* __restrict__ mark, int32_t* __restrict__ err) {
+    __shared__ int32_t s_w[kWaves];
+    __shared__ int64_t s_base;
+    __shared__ int32_t s_mode;  // 0 arena, 1 pool, 2 dropped
+    const uint32_t p = blockIdx.x;
+    const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
+    const bool vis = visible_mask(visible[p]);
+    int32_t carry = 0;
+    for (uint32_t cs = c0; cs < c1; cs += 256) {
+        const uint32_t c = cs + threadIdx.x;
+        int32_t cnt = 0;
+        if (c < c1) {
+            const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
+            float q[4];
+            const uint32_t n = monotone_cuts(cv, vis, q);
+            cuts[c] = make_float4(q[0], q[1], q[2], q[3]);
+            ncuts[c] = n;
+            if (vis) {
+                float p0x = cv.x[0], p0y = cv.y[0];
+#pragma unroll
+                for (uint32_t k = 0; k < 5; ++k) {
+                    if (k > n) break;
+                    const float t = k < n ? q[k] : 1.0f;
+                    const float p1x = cv.ex(t), p1y = cv.ey(t);
+                    cnt += piece_count(p0x, p0y, p1x, p1y, w, h);
+                    p0x = p1x; p0y = p1y;
+                }
+            }
+        }
+        int32_t tot;
+        const int32_t ex = block_exclusive_add(cnt, s_w, tot);
+        if (c < c1) { counts[c] = carry + ex; ends[c] = carry + ex + cnt; }
+        carry += tot;
+        __syncthreads();
+    }
+    const int32_t n = carry;
+    const int64_t A = arena[p], cap = cap_n[p], arena_total = arena[Np];
+    if (threadIdx.x == 0) {
+        int mode = 0;
+        int64_t base = A;
+        if (n > cap || A + cap > budget) {
+            mode = 1;
+            const int32_t off = n > 0 ? atomicAdd(&ctl[0], n) : 0;
+            base = arena_total + off;
+            atomicAdd(&ctl[1], 1);
+            if (base + n > budget) { mode = 2; atomicOr(err, 4); }
+        }
+        s_base = base;
+        s_mode = mode;
+        n_exact[p] = mode == 2 ? 0 : n;
+        pbase[p] = (int32_t)base;
+        if (mode == 2) visible[p] = 0;  // no chains: nothing is written for a dropped path
+    }
+    __syncthreads();
+    const int64_t base = s_base;
+    const int mode = s_mode;
+    if (mode != 2)
+        for (uint32_t c = c0 + threadIdx.x; c < c1; c += 256) {
+            counts[c] = (int32_t)(counts[c] + base);
+            ends[c] = (int32_t)(ends[c] + base);
+        }
+    // gaps: the unused arena tail (the whole arena after an overflow), and the part of a
+    // dropped path's pool block that lies inside the budget
+    const int64_t g0 = mode == 0 ? A + n : A, g1 = min(A + cap, budget);
+    for (int64_t i = g0 + threadIdx.x; i < g1; i += 256) { inter[i] = make_int2(-1, 0); mark[i] = 1; }
+    if (mode == 2) {
+        const int64_t e = min(base + n, budget);
+        for (int64_t i = base + threadIdx.x; i < e; i += 256) { inter[i] = make_int2(-1, 0); mark[i] = 1; }
+    }
+}
+
+__global__ void k_learned_finish(const int32_t* __restrict__ arena, uint32_t Np, int64_t budget,
+                                 int32_t* __restrict__ ctl) {
+    const int64_t live = (int64_t)arena[Np] + ctl[0];
+    ctl[2] = (int32_t)(live < budget ? live : budget);
+}
+
+// sstart[p] = first sorted element of path p = bucket offset of the path's first bucket.
+__global__ void k_sorted_starts(const int32_t* __restrict__ boff, const int32_t* __restrict__ bucket_base,
+                                uint32_t Np, int32_t* __restrict__ sstart) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p <= Np) sstart[p] = boff[bucket_base[p]];
+}
+
+// Per-path labels for training: log2(1 + intersections), log2(1 + records).
+__global__ void k_path_targets(const int32_t* __restrict__ n_per_path_or_seg, int seg_form,
+                               const int2* __restrict__ brec, const int32_t* __restrict__ bucket_base,
+                               uint32_t Np, float* __restrict__ targets, int32_t* __restrict__ exact) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Np) return;
+    const int32_t n = seg_form ? n_per_path_or_seg[p + 1] - n_per_path_or_seg[p] : n_per_path_or_seg[p];
+    int64_t r = 0;
+    for (int32_t b = bucket_base[p]; b < bucket_base[p + 1]; ++b) {
+        const int2 q = brec[b];
+        r += q.y > q.x ? q.y - q.x : 0;
+    }
+    targets[2 * p] = log2f(1.0f + (float)n);
+    targets[2 * p + 1] = log2f(1.0f + (float)r);
+    if (exact) { exact[2 * p] = n; exact[2 * p + 1] = (int32_t)r; }
+}
+
+// ---- FP32 training kernels (row-major activations, W[k][n])
+__global__ void k_dense_fwd(const float* __restrict__ X, int B, int K, const float* __restrict__ W,
+                            const float* __restrict__ bias, int N, int relu, float* __restrict__ Y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * N) return;
+    const int r = (int)(i / N), n = (int)(i % N);
+    float acc = bias[n];
+    for (int k = 0; k < K; ++k) acc += X[(int64_t)r * K + k] * W[k * N + n];
+    Y[i] = relu ? fmaxf(acc, 0.0f) : acc;
+}
+
+__global__ void k_loss_grad(const float* __restrict__ Y, const float* __restrict__ T, int B, int N,
+                            float* __restrict__ dY, float* __restrict__ sq) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * N) return;
+    const float e = Y[i] - T[i];
+    dY[i] = 2.0f * e / (float)((int64_t)B * N);
+    sq[i] = e * e;
+}
+
+// dA = (A > 0) * dY W^T  (A is the layer's ReLU output that fed this layer)
+__global__ void k_dense_bwd_data(const float* __restrict__ dY, int B, int N, const float* __restrict__ W, int K,
+                                 const float* __restrict__ A, float* __restrict__ dA) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * K) return;
+    const int r = (int)(i / K), k = (int)(i % K);
+    float acc = 0.0f;
+    for (int n = 0; n < N; ++n) acc += dY[(int64_t)r * N + n] * W[k * N + n];
+    dA[i] = A[i] > 0.0f ? acc : 0.0f;
+}
+
+// dW[k][n] = sum_r X[r][k] dY[r][n] for k < K; row K of the grid gives db[n] = sum_r dY[r][n].
+__global__ void k_dense_bwd_weight(const float* __restrict__ X, const float* __restrict__ dY, int B, int K,
+                                   int N, float* __restrict__ dW, float* __restrict__ db) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (K + 1) * N) return;
+    const int k = i / N, n = i % N;
+    float acc = 0.0f;
+    if (k < K) {
+        for (int r = 0; r < B; ++r) acc += X[(int64_t)r * K + k] * dY[(int64_t)r * N + n];
+        dW[k * N + n] = acc;
+    } else {
+        for (int r = 0; r < B; ++r) acc += dY[(int64_t)r * N + n];
+        db[n] = acc;
+    }
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float bc1,
+                       float bc2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+__global__ void k_gather_rows(const float* __restrict__ src, const int32_t* __restrict__ idx, int B, int K,
+                              float* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)B * K) return;
+    const int r = (int)(i / K), k = (int)(i % K);
+    dst[i] = src[(int64_t)idx[r] * K + k];
+}
+
+inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b < 1 ? 1 : (n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_path_features(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                                const uint32_t* pcurve, const int32_t* visible, const uint32_t* rule,
+                                uint32_t Np, int w, int h, float* feat, hipStream_t s,
+                                unsigned long long* bound_total) {
+    if (Np == 0) return hipSuccess;
+    k_path_features<<<(Np + 3) / 4, 256, 0, s>>>(tpos, cpos, ctype, pcurve, visible, rule, Np, w, h, feat,
+                                                 bound_total);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, float* pred, hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_mlp_infer<<<(Np + 63) / 64, 256, 0, s>>>(feat, Np, m, pred);
+    return hipGetLastError();
+}
+
+hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, float log_margin, int32_t slack,
+                       int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_caps<<<blocks(Np), 256, 0, s>>>(pred, visible, Np, log_margin, slack, arena, cap_n, cap_r_total);
+    return hipGetLastError();
+}
+
+hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, int32_t* visible,
+                                const uint32_t* pcurve, uint32_t Np, int w, int h, const int32_t* arena,
+                                const int32_t* cap_n, int64_t budget, float4* cuts, uint32_t* ncuts,
+                                int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact, int32_t* pbase,
+                                int2* inter, int8_t* mark, int32_t* err, hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_learned_count<<<Np, 256, 0, s>>>(tpos, cpos, ctype, visible, pcurve, Np, w, h, arena, cap_n, budget, cuts,
+                                       ncuts, counts, ends, ctl, n_exact, pbase, inter, mark, err);
+    k_learned_finish<<<1, 1, 0, s>>>(arena, Np, budget, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_sorted_starts(const int32_t* boff, const int32_t* bucket_base, uint32_t Np, int32_t* sstart,
+                                hipStream_t s) {
+    k_sorted_starts<<<blocks((int64_t)Np + 1), 256, 0, s>>>(boff, bucket_base, Np, sstart);
+    return hipGetLastError();
+}
+
+hipError_t launch_path_targets(const int32_t* n_or_seg, int seg_form, const int2* brec, const int32_t* bucket_base,
+                               uint32_t Np, float* targets, int32_t* exact, hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_path_targets<<<blocks(Np), 256, 0, s>>>(n_or_seg, seg_form, brec, bucket_base, Np, targets, exact);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_fwd(const float* X, int B, int K, const float* W, const float* b, int N, int relu, float* Y,
+                            hipStream_t s) {
+    k_dense_fwd<<<blocks((int64_t)B * N), 256, 0, s>>>(X, B, K, W, b, N, relu, Y);
+    return hipGetLastError();
+}
+hipError_t launch_loss_grad(const float* Y, const float* T, int B, int N, float* dY, float* sq, hipStream_t s) {
+    k_loss_grad<<<blocks((int64_t)B * N), 256, 0, s>>>(Y, T, B, N, dY, sq);
+    return hipGetLastError();
+}
+hipError_t launch_dense_bwd_data(const float* dY, int B, int N, const float* W, int K, const float* A, float* dA,
+                                 hipStream_t s) {
+    k_dense_bwd_data<<<blocks((int64_t)B * K), 256, 0, s>>>(dY, B, N, W, K, A, dA);
+    return hipGetLastError();
+}
+hipError_t launch_dense_bwd_weight(const float* X, const float* dY, int B, int K, int N, float* dW, float* db,
+                                   hipStream_t s) {
+    k_dense_bwd_weight<<<blocks((int64_t)(K + 1) * N, 64), 64, 0, s>>>(X, dY, B, K, N, dW, db);
+    return hipGetLastError();
+}
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2, hipStream_t s) {
+    k_adam<<<blocks(n), 256, 0, s>>>(p, g, m, v, n, lr, b1, b2, eps, bc1, bc2);
+    return hipGetLastError();
+}
+hipError_t launch_gather_rows(const float* src, const int32_t* idx, int B, int K, float* dst, hipStream_t s) {
+    k_gather_rows<<<blocks((int64_t)B * K), 256, 0, s>>>(src, idx, B, K, dst);
+    return hipGetLastError();
+}
+
+}  // namespace spr

@@ -17,12 +17,17 @@ namespace spr {
 
 namespace {
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, int32_t* out, int64_t n,
+// n_dev (optional) holds the element count on the device (the launch covers n_max); total_out
+// (optional) also receives the total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, int32_t* out, int64_t n_max,
+                                                           const int32_t* __restrict__ n_dev,
+                                                           int32_t* __restrict__ total_out,
                                                            unsigned long long* __restrict__ status,
                                                            uint32_t* __restrict__ counter) {
     __shared__ int32_t s_tile;
     __shared__ int32_t s_w[kWaves];
     __shared__ int32_t s_prefix;
+    const int64_t n = n_dev ? min((int64_t)*n_dev, n_max) : n_max;
     const int64_t tile = ordered_tile_id(counter, &s_tile);
     const int64_t base = tile * (int64_t)kScanTile;
     if (base >= n && tile != 0) return;
@@ -63,7 +68,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, in
             if (i0 + j < n) out[i0 + j] = (int32_t)((uint32_t)off + (uint32_t)v[j]);
     }
     const bool last_tile = (n == 0) ? (tile == 0) : (base <= n - 1 && n - 1 < base + kScanTile);
-    if (last_tile && threadIdx.x == 0) out[n] = (int32_t)((uint32_t)s_prefix + (uint32_t)total);
+    if (last_tile && threadIdx.x == 0) {
+        out[n] = (int32_t)((uint32_t)s_prefix + (uint32_t)total);
+        if (total_out) *total_out = out[n];
+    }
 }
 
 // (cells, spans) packed as two 31-bit counts under a 2-bit state.
@@ -78,26 +86,66 @@ struct OpPair {
     __device__ static T value(uint64_t w) { return (w & 0x7FFFFFFFull) | (((w >> 31) & 0x7FFFFFFFull) << 32); }
 };
 
+// Path lookup of sorted elements.  With sstart == nullptr the path is epath[f]; otherwise f's
+// path is the p with sstart[p] <= f < sstart[p+1] (binary search once, then a forward walk).
+struct PathCursor {
+    const uint32_t* epath;
+    const int32_t* sstart;
+    uint32_t Np, p;
+    __device__ PathCursor(const uint32_t* e, const int32_t* s, uint32_t n, int64_t f0)
+        : epath(e), sstart(s), Np(n), p(0) {
+        if (sstart) {
+            uint32_t lo = 0, hi = Np;  // largest p with sstart[p] <= f0
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((int64_t)sstart[mid] <= f0) lo = mid; else hi = mid;
+            }
+            p = lo;
+        }
+    }
+    __device__ uint32_t at(int64_t f) {
+        if (!sstart) return epath[f] & 0x3FFFFFFFu;
+        while (p + 1 < Np && (int64_t)sstart[p + 1] <= f) ++p;
+        return p;
+    }
+};
+
+// Path of element i0 - 1 for the emit loop (same rule as PathCursor, without moving it).
+__device__ inline uint32_t pprev_of(const uint32_t* epath, const int32_t* sstart, uint32_t Np, int64_t i0,
+                                    uint32_t p_i0, int32_t) {
+    if (!sstart) return epath[i0 - 1] & 0x3FFFFFFFu;
+    uint32_t p = p_i0;
+    while (p > 0 && (int64_t)sstart[p] > i0 - 1) --p;
+    (void)Np;
+    return p;
+}
+
 __device__ inline int key_x(int32_t k) { return (k & 0xFFFF) - 0x7FFF; }
 __device__ inline int key_y(int32_t k) { return ((k >> 16) & 0xFFFF) - 0x7FFF; }
 
 __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     const int32_t* __restrict__ skey, const int32_t* __restrict__ winding,
     const uint32_t* __restrict__ epath, const uint32_t* __restrict__ rule,
-    const uint32_t* __restrict__ fill, int64_t N, int w, int h,
+    const uint32_t* __restrict__ fill, int64_t N_max, const int32_t* __restrict__ n_dev,
+    const int32_t* __restrict__ sstart, uint32_t Np, int64_t rec_cap, int w, int h,
     unsigned long long* __restrict__ status, uint32_t* __restrict__ counter,
     int4* __restrict__ records, int32_t* __restrict__ flags, int32_t* __restrict__ totals,
-    const int32_t* __restrict__ bucket_base, const int2* __restrict__ prow, int2* __restrict__ brec) {
+    const int32_t* __restrict__ bucket_base, const int2* __restrict__ prow, int2* __restrict__ brec,
+    int32_t* __restrict__ err) {
     __shared__ int32_t s_tile;
     __shared__ unsigned long long s_w[kWaves];
     __shared__ unsigned long long s_prefix;
+    const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
     const int64_t tile = ordered_tile_id(counter, &s_tile);
     const int64_t base = tile * (int64_t)kScanTile;
     if (base >= N) return;
     const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
+    // Path of sorted element f: epath[f] when sorted segments coincide with the intersection
+    // segments (plain layout); otherwise from the sorted-space path starts sstart[0..Np].
+    PathCursor cur(epath, sstart, Np, i0 > 0 ? i0 - 1 : 0);
     int32_t kprev = 0;
     uint32_t pprev = 0;
-    if (i0 > 0 && i0 <= N) { kprev = skey[i0 - 1]; pprev = epath[i0 - 1] & 0x3FFFFFFFu; }
+    if (i0 > 0 && i0 <= N) { kprev = skey[i0 - 1]; pprev = cur.at(i0 - 1); }
     int32_t keys[kScanItems];
     uint32_t paths[kScanItems];
     uint32_t cellbits = 0, spanbits = 0;
@@ -107,7 +155,7 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
         keys[j] = 0; paths[j] = 0;
         if (f >= N) continue;
         const int32_t k1 = skey[f];
-        const uint32_t p1 = epath[f] & 0x3FFFFFFFu;
+        const uint32_t p1 = cur.at(f);
         keys[j] = k1; paths[j] = p1;
         const int x1 = key_x(k1), y1 = key_y(k1);
         const bool in_canvas = !(x1 < 0 || y1 < 0 || x1 >= w || y1 >= h);
@@ -139,10 +187,10 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     int32_t spans_before = (int32_t)(off >> 32);
     if (i0 < N) {
         int32_t kp = (i0 > 0) ? skey[i0 - 1] : 0;
-        uint32_t pp = (i0 > 0) ? (epath[i0 - 1] & 0x3FFFFFFFu) : 0xFFFFFFFFu;
+        uint32_t pp = (i0 > 0) ? pprev_of(epath, sstart, Np, i0, paths[0], keys[0]) : 0xFFFFFFFFu;
         const int64_t inext = i0 + kScanItems;
         const int32_t knext_out = inext < N ? skey[inext] : 0;
-        const uint32_t pnext_out = inext < N ? (epath[inext] & 0x3FFFFFFFu) : 0xFFFFFFFFu;
+        const uint32_t pnext_out = inext < N ? cur.at(inext) : 0xFFFFFFFFu;
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t f = i0 + j;
@@ -151,7 +199,9 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
             const int32_t k1 = keys[j];
             const uint32_t p1 = paths[j];
             if (flags) { flags[f] = cell; flags[N + f] = span; }
-            if (cell | span) {
+            if ((cell | span) && (int64_t)(cells_before + spans_before + cell + span) > rec_cap) {
+                atomicOr(err, 2);  // record capacity exceeded: the frame is redone by the host
+            } else if (cell | span) {
                 const int32_t colour = (int32_t)fill[p1];
                 const int32_t o = cells_before + spans_before;
                 if (cell) {
@@ -179,8 +229,9 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
                     const int loc = r == 0 ? pr.y + 1 : r - pr.x;
                     if (loc >= 0 && loc < pr.y + 2) {  // out of range only after a flagged bound error
                         int2* br = &brec[bucket_base[p1] + loc];
-                        if (first) br->x = cells_before + spans_before;
-                        if (last) br->y = cells_before + spans_before + cell + span;
+                        const int64_t e = min((int64_t)(cells_before + spans_before + cell + span), rec_cap);
+                        if (first) br->x = (int32_t)min((int64_t)(cells_before + spans_before), e);
+                        if (last) br->y = (int32_t)e;
                     }
                 }
             }
@@ -200,20 +251,24 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
 
 }  // namespace
 
-hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s) {
+hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s,
+                           const int32_t* n_dev, int32_t* total_out) {
     const int64_t tiles = n > 0 ? scan_tiles(n) : 1;
-    k_scan_i32<<<(unsigned)tiles, kScanThreads, 0, s>>>(in, out, n, ctl.status, ctl.counter);
+    k_scan_i32<<<(unsigned)tiles, kScanThreads, 0, s>>>(in, out, n, n_dev, total_out, ctl.status, ctl.counter);
     return hipGetLastError();
 }
 
 hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,
                             const uint32_t* rule, const uint32_t* fill, int64_t N, int w, int h,
                             ScanCtl ctl, int4* records, int32_t* flags, int32_t* totals,
-                            const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s) {
+                            const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s,
+                            const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
+                            int32_t* err) {
     if (N == 0) return hipSuccess;
-    k_mark_emit<<<(unsigned)scan_tiles(N), kScanThreads, 0, s>>>(skey, winding, epath, rule, fill, N, w, h,
-                                                                 ctl.status, ctl.counter, records, flags,
-                                                                 totals, bucket_base, prow, brec);
+    k_mark_emit<<<(unsigned)scan_tiles(N), kScanThreads, 0, s>>>(skey, winding, epath, rule, fill, N, n_dev,
+                                                                 sstart, Np, rec_cap, w, h, ctl.status,
+                                                                 ctl.counter, records, flags, totals,
+                                                                 bucket_base, prow, brec, err);
     return hipGetLastError();
 }
 

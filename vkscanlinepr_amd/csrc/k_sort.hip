@@ -188,8 +188,10 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
     }
 }
 
-__global__ void k_unpack_sorted(const u64* __restrict__ sorted, int64_t N, int32_t* __restrict__ skey,
-                                int32_t* __restrict__ sidx, int32_t* __restrict__ swn) {
+__global__ void k_unpack_sorted(const u64* __restrict__ sorted, int64_t N_max, int32_t* __restrict__ skey,
+                                int32_t* __restrict__ sidx, int32_t* __restrict__ swn,
+                                const int32_t* __restrict__ n_dev) {
+    const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const u64 w = sorted[i];
@@ -216,9 +218,9 @@ hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_l
 }
 
 hipError_t launch_unpack_sorted(const unsigned long long* sorted, int64_t N, int32_t* skey,
-                                int32_t* sidx, int32_t* swn, hipStream_t s) {
+                                int32_t* sidx, int32_t* swn, hipStream_t s, const int32_t* n_dev) {
     if (N == 0) return hipSuccess;
-    k_unpack_sorted<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(sorted, N, skey, sidx, swn);
+    k_unpack_sorted<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(sorted, N, skey, sidx, swn, n_dev);
     return hipGetLastError();
 }
 

@@ -21,6 +21,8 @@ inline int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
 // ---- control
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_fill_u32_dev(uint32_t* p, int64_t n_max, const int32_t* n_dev, int64_t mult, int64_t add,
+                               uint32_t v, hipStream_t s);
 
 // ---- geometry (a3, a4, a7, a8)
 hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
@@ -47,7 +49,8 @@ constexpr int kChainClasses = 32;
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
-                             ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s);
+                             ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s,
+                             const int32_t* ends = nullptr);
 hipError_t launch_chain_order(const ChainDesc* chains, uint32_t C, const int32_t* class_count,
                               int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order,
                               hipStream_t s);
@@ -58,15 +61,19 @@ hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const ChainDe
                               const float* chain_t, int32_t* raw, int8_t* start_mark, int2* inter,
                               int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, int n_cu,
                               hipStream_t s);
-hipError_t launch_or_merge(const int32_t* raw, const int8_t* start_mark, int64_t N, int2* inter, hipStream_t s);
+// Launchers taking `n_dev` size the grid for the host bound and read the live count from the
+// device when n_dev is not null (a frame that never waits for the host).
+hipError_t launch_or_merge(const int32_t* raw, const int8_t* start_mark, int64_t N, int2* inter, hipStream_t s,
+                           const int32_t* n_dev = nullptr);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
                                const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
-                               hipStream_t s);
+                               hipStream_t s, const int32_t* n_dev = nullptr);
 
 // ---- scans
-hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s);
+hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s,
+                           const int32_t* n_dev = nullptr, int32_t* total_out = nullptr);
 
 // ---- (path, row) buckets and the band sort (a9 + a10): per path segment, ascending
 // (key ^ 0x80000000, index); the winding change rides along in the low bits of the sort word.
@@ -75,11 +82,13 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
                                 int32_t* nb, int2* prow, hipStream_t s);
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
                                  const int32_t* bucket_base, const int2* prow, const int32_t* boff,
-                                 int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s);
+                                 int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s,
+                                 const int32_t* n_dev = nullptr);
 hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
                              hipStream_t s);
 hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,
-                            int32_t* band_start, hipStream_t s);
+                            int32_t* band_start, hipStream_t s, const int32_t* nb_dev = nullptr,
+                            const int32_t* n_dev = nullptr);
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                              const unsigned long long* src, unsigned long long* dst, int32_t* big_list,
                              int32_t* big_count, hipStream_t s);
@@ -87,17 +96,61 @@ hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_l
                                  unsigned long long* scatter, unsigned long long* tmp, unsigned long long* dst,
                                  hipStream_t s);
 hipError_t launch_unpack_sorted(const unsigned long long* sorted, int64_t N, int32_t* skey,
-                                int32_t* sidx, int32_t* swn, hipStream_t s);
+                                int32_t* sidx, int32_t* swn, hipStream_t s, const int32_t* n_dev = nullptr);
 
 // ---- winding / mark / emit (a10 scan, a11); also writes each (path, row) bucket's record range
 hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,
                             const uint32_t* rule, const uint32_t* fill, int64_t N, int w, int h,
                             ScanCtl ctl, int4* records, int32_t* flags, int32_t* totals,
-                            const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s);
+                            const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s,
+                            const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
+                            int32_t* err);
+
+// ---- learned single-pass allocation (k_learned.hip)
+constexpr int kMlpIn = 16, kMlpHidden = 64, kMlpOut = 2;
+// flat FP32 parameter vector: W1[16][64], b1[64], W2[64][64], b2[64], W3[64][2], b3[2]
+constexpr int kMlpParams = kMlpIn * kMlpHidden + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden +
+                           kMlpHidden * kMlpOut + kMlpOut;
+struct MlpDev {
+    const uint16_t* w1;     // BF16 [16][64]
+    const uint16_t* w2;     // BF16 [64][64]
+    const uint16_t* w3;     // BF16 [64][16], columns >= 2 are zero
+    const float* b1;        // [64]
+    const float* b2;        // [64]
+    const float* b3;        // [2]
+    const float* mean;      // [16] feature normalisation
+    const float* inv_std;   // [16]
+};
+hipError_t launch_path_features(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
+                                const uint32_t* pcurve, const int32_t* visible, const uint32_t* rule,
+                                uint32_t Np, int w, int h, float* feat, hipStream_t s,
+                                unsigned long long* bound_total = nullptr);
+hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, float* pred, hipStream_t s);
+hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, float log_margin, int32_t slack,
+                       int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s);
+hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, int32_t* visible,
+                                const uint32_t* pcurve, uint32_t Np, int w, int h, const int32_t* arena,
+                                const int32_t* cap_n, int64_t budget, float4* cuts, uint32_t* ncuts,
+                                int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact, int32_t* pbase,
+                                int2* inter, int8_t* mark, int32_t* err, hipStream_t s);
+hipError_t launch_sorted_starts(const int32_t* boff, const int32_t* bucket_base, uint32_t Np, int32_t* sstart,
+                                hipStream_t s);
+hipError_t launch_path_targets(const int32_t* n_or_seg, int seg_form, const int2* brec, const int32_t* bucket_base,
+                               uint32_t Np, float* targets, int32_t* exact, hipStream_t s);
+hipError_t launch_dense_fwd(const float* X, int B, int K, const float* W, const float* b, int N, int relu, float* Y,
+                            hipStream_t s);
+hipError_t launch_loss_grad(const float* Y, const float* T, int B, int N, float* dY, float* sq, hipStream_t s);
+hipError_t launch_dense_bwd_data(const float* dY, int B, int N, const float* W, int K, const float* A, float* dA,
+                                 hipStream_t s);
+hipError_t launch_dense_bwd_weight(const float* X, const float* dY, int B, int K, int N, float* dW, float* db,
+                                   hipStream_t s);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,
+                       float bc1, float bc2, hipStream_t s);
+hipError_t launch_gather_rows(const float* src, const int32_t* idx, int B, int K, float* dst, hipStream_t s);
 
 // ---- composite (a12)
 hipError_t launch_composite(const int4* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
-                            uint8_t* image, hipStream_t s);
+                            uint8_t* image, hipStream_t s, int64_t rec_cap);
 
 }  // namespace spr

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -123,7 +124,18 @@ struct spr_handle {
     bool inj = false;
 
     // learned allocation
-    spr::LearnedState learned;
+    spr::MlpModel model;
+    spr::MlpDevice mlpdev;
+    DBuf<uint32_t> pcurve;                          // first curve of each path, Np+1
+    DBuf<float> feat, pred, train_feat, train_tgt;
+    DBuf<int32_t> arena, capn, nexact, pbase, sstart, hist_l, fill_l, cend;
+    DBuf<int2> brec_l;
+    int64_t lb_n = 0, lb_r = 0;                     // learned frame budgets
+    bool lb_dirty = true;                           // scene, MVP or model changed since they were set
+    int64_t n_live = 0;                             // positions of the last learned layout
+    int64_t learned_fallbacks = 0;                  // learned frames that read their budgets back
+    bool last_learned = false;
+    int2* last_brec = nullptr;                      // per-bucket record ranges of the last frame
 
     // last frame
     int64_t N = 0;
@@ -179,14 +191,14 @@ void stage_mark(spr_handle* H, int idx) {
     H->n_marks = idx + 1;
 }
 
-void run_frame(spr_handle* H) {
+void run_frame(spr_handle* H, bool record_begin = true) {
     if (!H->loaded) throw Fail{SPR_ERR_STATE, "render before load"};
     hipStream_t s = H->stream;
     const uint32_t P = H->P, C = H->C, Np = H->Np;
     const int w = (int)H->w, h = (int)H->h;
     const bool debug = H->opts.debug != 0;
 
-    SPR_HIP(hipEventRecord(H->ev_begin, s));
+    if (record_begin) SPR_HIP(hipEventRecord(H->ev_begin, s));
     H->n_marks = 0;
     stage_mark(H, 0);
     // ---- phase A: transform + cull + row bounds, count pass, bucket tables
@@ -335,16 +347,260 @@ void run_frame(spr_handle* H) {
     stage_mark(H, 10);
     SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
                                   ctl_at(H, z.st_mark, z.ctr, 3), H->records.p, debug ? H->flags.p : nullptr,
-                                  d_totals, H->bucket_base.p, H->prow.p, d_brec, s));
+                                  d_totals, H->bucket_base.p, H->prow.p, d_brec, s, nullptr, nullptr, Np,
+                                  (int64_t)H->records.cap, d_err));
+    H->last_brec = d_brec;
+    H->last_learned = false;
     stage_mark(H, 11);
     // ---- composite
     SPR_HIP(spr::launch_composite(H->records.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
-                                  (int)H->opts.compose, H->image.p, s));
+                                  (int)H->opts.compose, H->image.p, s, (int64_t)H->records.cap));
     stage_mark(H, 12);
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 8, d_err, 4, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipEventRecord(H->ev_end, s));
     H->frame_done = true;
+}
+
+// ---- learned single-pass frame
+//
+// Same stages as run_frame, but no host synchronisation between them: per-path capacities
+// predicted by the MLP become arenas, one workgroup per path counts its curves exactly and
+// places them (arena, or an exact block of the overflow pool), and every later kernel reads its
+// element counts from device memory.  Buffers are sized by budgets kept from earlier frames
+// (the first learned frame after a load reads the predicted total once).  If a budget turns out
+// too small the device flags it, and the frame is redone with the plain method: the output is
+// always the plain method's.
+struct LearnedLayout {
+    size_t total = 0, ctl = 0, misc = 0, capr = 0, counters = 0, totals = 0, slots = 0, bound = 0;
+    size_t st_arena = 0, st_pb = 0, st_hist = 0, st_band = 0, st_wind = 0, st_mark = 0, chainctl = 0;
+};
+
+LearnedLayout learned_layout(uint32_t Np, int64_t nb_budget, int64_t n_budget) {
+    LearnedLayout z;
+    size_t o = 0;
+    z.counters = o; o += 4;     // eight u32 scan tile counters
+    z.ctl = o; o += 2;          // int32 pool entries, overflow paths, live positions, spare
+    z.misc = o; o += 1;         // int32 error flags, big-band count
+    z.capr = o; o += 1;         // int32 predicted record total, spare
+    z.totals = o; o += 2;       // int32 cells, spans, R, spare
+    z.slots = o; o += 2;        // int32 sorted entries, bands, spare x2
+    z.bound = o; o += 1;        // u64 upper bound of the intersections
+    z.chainctl = o; o += 64;
+    z.st_arena = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
+    z.st_pb = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
+    z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
+    z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
+    z.st_wind = o; o += spr::scan_tiles(std::max<int64_t>(n_budget, 1));
+    z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(n_budget, 1));
+    z.total = o;
+    return z;
+}
+
+void run_learned_frame(spr_handle* H) {
+    if (!H->loaded) throw Fail{SPR_ERR_STATE, "render before load"};
+    if (!H->model.ready)
+        throw Fail{SPR_ERR_STATE, "learned mode needs a trained or loaded model (spr_learned_train or spr_learned_load)"};
+    hipStream_t s = H->stream;
+    if (!H->mlpdev.valid()) H->mlpdev.upload(H->model, s);
+    const uint32_t P = H->P, C = H->C, Np = H->Np;
+    const int w = (int)H->w, h = (int)H->h;
+    const bool debug = H->opts.debug != 0;
+    const int rmax = (h - 1) / 2;
+    const int64_t nb_budget = (int64_t)Np * (rmax + 2);
+    if (nb_budget > (int64_t)1 << 30) throw Fail{SPR_ERR_CAPACITY, "bucket bound exceeds 2^30"};
+
+    SPR_HIP(hipEventRecord(H->ev_begin, s));
+    H->n_marks = 0;
+    stage_mark(H, 0);
+    H->visible.ensure(std::max<uint32_t>(Np, 1));
+    H->row_lo.ensure(std::max<uint32_t>(Np, 1));
+    H->row_hi.ensure(std::max<uint32_t>(Np, 1));
+    H->prow.ensure(std::max<uint32_t>(Np, 1));
+    H->bucket_base.ensure((size_t)Np + 1);
+    H->tpos.ensure(std::max<uint32_t>(P, 1));
+    H->cuts.ensure(std::max<uint32_t>(C, 1));
+    H->ncuts.ensure(std::max<uint32_t>(C, 1));
+    H->counts.ensure((size_t)C + 1);
+    H->cend.ensure((size_t)C + 1);
+    H->feat.ensure((size_t)std::max<uint32_t>(Np, 1) * spr::kMlpIn);
+    H->pred.ensure((size_t)std::max<uint32_t>(Np, 1) * spr::kMlpOut);
+    H->arena.ensure((size_t)Np + 1);
+    H->capn.ensure((size_t)Np + 1);
+    H->nexact.ensure((size_t)Np + 1);
+    H->pbase.ensure((size_t)Np + 1);
+    H->sstart.ensure((size_t)Np + 1);
+
+    // phase 1: transform, features, prediction, arenas (no dependency on any count)
+    LearnedLayout z = learned_layout(Np, nb_budget, std::max<int64_t>(H->lb_n, 1));
+    H->zero.ensure(z.total);
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)H->zero.p, 2 * (int64_t)z.total, 0u, s));
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)H->visible.p, Np, 0u, s));
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_lo.p, Np, 0x7FFFFFFFu, s));
+    SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_hi.p, Np, 0x80000000u, s));
+    SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p,
+                                  H->row_lo.p, H->row_hi.p, s));
+    stage_mark(H, 1);
+    int32_t* d_capr = reinterpret_cast<int32_t*>(H->zero.p + z.capr);
+    SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p, H->rule.p, Np,
+                                      w, h, H->feat.p, s, H->zero.p + z.bound));
+    SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s));
+    SPR_HIP(spr::launch_caps(H->pred.p, H->visible.p, Np, H->model.log_margin, H->model.slack, H->arena.p,
+                             H->capn.p, d_capr, s));
+    SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, 0), s));
+    SPR_HIP(spr::launch_bucket_counts(H->row_lo.p, H->row_hi.p, Np, rmax, H->bucket_base.p, H->prow.p, s));
+    SPR_HIP(spr::launch_scan_i32(H->bucket_base.p, H->bucket_base.p, Np, ctl_at(H, z.st_pb, z.counters, 1), s));
+    stage_mark(H, 2);
+    if (H->lb_dirty) {
+        // first learned frame after a change of scene, MVP or model: size the budgets from the
+        // arena total plus a proven upper bound of all intersections (the overflow pool can
+        // then never run out); records are at most two per sorted entry
+        SPR_HIP(hipMemcpyAsync(H->h_pinned + 12, H->arena.p + Np, 4, hipMemcpyDeviceToHost, s));
+        SPR_HIP(hipMemcpyAsync(H->h_pinned + 14, H->zero.p + z.bound, 8, hipMemcpyDeviceToHost, s));
+        SPR_HIP(hipStreamSynchronize(s));
+        const int64_t caps = std::max<int32_t>(H->h_pinned[12], 0);
+        unsigned long long bound = 0;
+        std::memcpy(&bound, H->h_pinned + 14, 8);
+        H->lb_n = caps + (int64_t)std::min<unsigned long long>(bound, 1ull << 40) + 1024;
+        H->lb_r = 2 * H->lb_n;
+        ++H->learned_fallbacks;
+        // the scan status arrays of phase 2 are sized by the budget: re-lay the control region
+        // (phase 1 is complete: its counters are no longer needed)
+        z = learned_layout(Np, nb_budget, H->lb_n);
+        H->zero.ensure(z.total);
+        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->zero.p, 2 * (int64_t)z.total, 0u, s));
+    }
+    int32_t* d_ctl = reinterpret_cast<int32_t*>(H->zero.p + z.ctl);
+    int32_t* d_err = reinterpret_cast<int32_t*>(H->zero.p + z.misc);
+    int32_t* d_bigcount = d_err + 1;
+    d_capr = reinterpret_cast<int32_t*>(H->zero.p + z.capr);
+    int32_t* d_totals = reinterpret_cast<int32_t*>(H->zero.p + z.totals);
+    int32_t* d_nsorted = reinterpret_cast<int32_t*>(H->zero.p + z.slots);
+    int32_t* d_nbands = d_nsorted + 1;
+    if (H->lb_n > (int64_t)1 << 30 || H->lb_r > (int64_t)1 << 30)
+        throw Fail{SPR_ERR_CAPACITY, "intersection or record budget exceeds 2^30"};
+    const int64_t nl = std::max<int64_t>(H->lb_n, 1), nr = std::max<int64_t>(H->lb_r, 1);
+    const int64_t max_bands = (int64_t)Np + 3 * (nl / 1024) + 2;
+    H->inter.ensure(nl); H->chain_t.ensure(nl); H->key.ensure(nl); H->wn.ensure(nl); H->epath.ensure(nl);
+    H->sorted.ensure(nl); H->sbig.ensure(nl); H->stmp.ensure(nl); H->skey.ensure(nl); H->swn.ensure(nl);
+    H->winding.ensure(nl + 1);
+    H->bflag.ensure((size_t)nb_budget + 1);
+    H->band_start.ensure((size_t)max_bands + 1);
+    H->big_list.ensure((size_t)max_bands);
+    if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); }
+    H->records.ensure(nr);
+    H->image.ensure((size_t)4 * H->w * H->h);
+    H->hist_l.ensure((size_t)nb_budget + 1);
+    H->fill_l.ensure((size_t)nb_budget + 1);
+    H->brec_l.ensure((size_t)nb_budget + 1);
+    // per-bucket tables: cleared up to the live bucket count (bucket_base[Np])
+    const int32_t* d_nb = H->bucket_base.p + Np;
+    SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->hist_l.p, nb_budget + 1, d_nb, 1, 1, 0u, s));
+    SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->fill_l.p, nb_budget, d_nb, 1, 0, 0u, s));
+    SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->brec_l.p, 2 * nb_budget, d_nb, 2, 0, 0u, s));
+
+    // phase 2: exact counts placed in the arenas, then the plain stages on device counts
+    SPR_HIP(spr::launch_learned_count(H->tpos.p, H->cpos.p, H->ctype.p, H->visible.p, H->pcurve.p, Np, w, h,
+                                      H->arena.p, H->capn.p, H->lb_n, H->cuts.p, H->ncuts.p, H->counts.p,
+                                      H->cend.p, d_ctl, H->nexact.p, H->pbase.p, H->inter.p, H->wn.p, d_err, s));
+    stage_mark(H, 3);
+    int32_t* d_cc = reinterpret_cast<int32_t*>(H->zero.p + z.chainctl);
+    int32_t* d_cfill = d_cc + 32;
+    int32_t* d_coff = d_cc + 64;
+    int32_t* d_nlong = d_cc + 97;
+    int32_t* d_work = d_cc + 98;
+    int32_t* d_nlpieces = d_cc + 99;
+    const int32_t* d_live = d_ctl + 2;
+    if (C) {
+        H->chains.ensure((size_t)C * 10);
+        H->pieces.ensure((size_t)C * 5);
+        H->chain_order.ensure((size_t)C * 10);
+        H->long_list.ensure((size_t)C * 5);
+        H->mono.ensure((size_t)C * 10);
+        H->pieces_seq.ensure((size_t)C * 5 + 4);
+        SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
+                                       H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, d_cc, s,
+                                       H->cend.p));
+        SPR_HIP(spr::launch_chain_order(H->chains.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, s));
+    }
+    stage_mark(H, 4);
+    if (C) {
+        SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong,
+                                        d_work, H->chain_t.p, H->mono.p, H->n_cu, s));
+    }
+    stage_mark(H, 5);
+    if (C) {
+        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
+        SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->chains.p, H->mono.p, H->chain_t.p, H->key.p, H->wn.p,
+                                        H->inter.p, H->long_list.p, d_nlpieces, H->pieces_seq.p, H->n_cu, s));
+        SPR_HIP(spr::launch_or_merge(H->key.p, H->wn.p, nl, H->inter.p, s, d_live));
+    }
+    stage_mark(H, 6);
+    SPR_HIP(spr::launch_gen_fragment(H->inter.p, nl, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
+                                     H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, H->hist_l.p, d_err,
+                                     s, d_live));
+    stage_mark(H, 7);
+    SPR_HIP(spr::launch_scan_i32(H->hist_l.p, H->hist_l.p, nb_budget, ctl_at(H, z.st_hist, z.counters, 2), s,
+                                 d_nb, d_nsorted));
+    SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, nl, H->bucket_base.p, H->prow.p,
+                                       H->hist_l.p, H->fill_l.p, H->sbig.p, d_err, s, d_live));
+    SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, H->hist_l.p, H->bflag.p, s));
+    SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, nb_budget, ctl_at(H, z.st_band, z.counters, 3), s,
+                                 d_nb, d_nbands));
+    SPR_HIP(spr::launch_band_emit(H->bflag.p, H->hist_l.p, (int32_t)nb_budget, nl, H->band_start.p, s, d_nb,
+                                  d_nsorted));
+    stage_mark(H, 8);
+    SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->sorted.p, H->big_list.p,
+                                   d_bigcount, s));
+    SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->sbig.p, H->stmp.p,
+                                       H->sorted.p, s));
+    SPR_HIP(spr::launch_unpack_sorted(H->sorted.p, nl, H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s,
+                                      d_nsorted));
+    SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
+    stage_mark(H, 9);
+    SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, nl, ctl_at(H, z.st_wind, z.counters, 4), s, d_nsorted));
+    stage_mark(H, 10);
+    SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, nl, w, h,
+                                  ctl_at(H, z.st_mark, z.counters, 5), H->records.p, debug ? H->flags.p : nullptr,
+                                  d_totals, H->bucket_base.p, H->prow.p, H->brec_l.p, s, d_nsorted, H->sstart.p,
+                                  Np, nr, d_err));
+    stage_mark(H, 11);
+    SPR_HIP(spr::launch_composite(H->records.p, H->brec_l.p, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
+                                  (int)H->opts.compose, H->image.p, s, nr));
+    stage_mark(H, 12);
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 8, d_err, 4, hipMemcpyDeviceToHost, s));
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 9, d_ctl, 12, hipMemcpyDeviceToHost, s));
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 12, H->arena.p + Np, 4, hipMemcpyDeviceToHost, s));
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 13, d_nsorted, 4, hipMemcpyDeviceToHost, s));
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 14, d_capr, 4, hipMemcpyDeviceToHost, s));
+    SPR_HIP(hipEventRecord(H->ev_end, s));
+    // the end of a learned frame is where a budget overflow is seen
+    SPR_HIP(hipStreamSynchronize(s));
+    const int32_t err = H->h_pinned[8];
+    const int64_t caps = std::max<int32_t>(H->h_pinned[12], 0), pool = std::max<int32_t>(H->h_pinned[9], 0);
+    const int64_t r_need = (int64_t)H->h_pinned[4] + H->h_pinned[5];
+    H->stats.n_overflow_paths = H->h_pinned[10];
+    H->stats.alloc_intersections = caps;
+    H->frame_done = true;
+    H->last_learned = true;
+    H->last_brec = H->brec_l.p;
+    if (err & 1) {
+        H->h_pinned[8] = 0;
+        throw Fail{SPR_ERR_DEVICE, "internal: an entry fell outside its path's row bound"};
+    }
+    if (err & 6) {
+        // cannot happen with bounded budgets; reported instead of rendered
+        H->h_pinned[8] = 0;
+        H->lb_dirty = true;
+        throw Fail{SPR_ERR_DEVICE, "internal: learned frame exceeded its bounded budget"};
+    }
+    H->N = H->h_pinned[13];
+    H->n_live = H->h_pinned[11];
+    // the same scene, MVP and model give the same counts: later frames need exactly this much
+    H->lb_n = caps + pool + std::max<int64_t>((caps + pool) / 64, 1024);
+    H->lb_r = r_need + std::max<int64_t>(r_need / 64, 1024);
+    H->lb_dirty = false;
 }
 
 template <class F>
@@ -378,6 +634,7 @@ void sync_stats(spr_handle* H) {
     }
     H->stats.n_cells = H->h_pinned[4];
     H->stats.n_spans = H->h_pinned[5];
+    H->stats.learned_fallbacks = H->learned_fallbacks;
     float ms = 0.0f;
     SPR_HIP(hipEventElapsedTime(&ms, H->ev_begin, H->ev_end));
     H->stats.frame_ms = ms;
@@ -420,6 +677,13 @@ void load_arrays(spr_handle* H, const float* pos, const uint32_t* pos_path, uint
         SPR_HIP(hipMemcpyAsync(H->rule.p, rule, (size_t)Np * 4, hipMemcpyHostToDevice, s));
         SPR_HIP(hipMemcpyAsync(H->fill.p, fill, (size_t)Np * 4, hipMemcpyHostToDevice, s));
     }
+    std::vector<uint32_t> pc((size_t)Np + 1, 0);
+    for (uint32_t c = 0, p = 0; p <= Np; ++p) {
+        while (c < C && cpath[c] < p) ++c;
+        pc[p] = c;
+    }
+    H->pcurve.ensure((size_t)Np + 1);
+    SPR_HIP(hipMemcpyAsync(H->pcurve.p, pc.data(), pc.size() * 4, hipMemcpyHostToDevice, s));
     SPR_HIP(hipStreamSynchronize(s));
     H->h_cpath.assign(cpath, cpath + C);
     H->P = P; H->C = C; H->Np = Np;
@@ -427,7 +691,7 @@ void load_arrays(spr_handle* H, const float* pos, const uint32_t* pos_path, uint
     H->frame_done = false;
     H->inj = false;
     H->inj_inter.clear();
-    spr::learned_on_load(H->learned);
+    H->lb_dirty = true;
 }
 
 }  // namespace
@@ -488,7 +752,10 @@ int spr_destroy(spr_handle* H) {
     H->winding.release(); H->flags.release(); H->records.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
     H->band_start.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
-    spr::learned_release(H->learned);
+    H->mlpdev.release();
+    H->pcurve.release(); H->feat.release(); H->pred.release(); H->train_feat.release(); H->train_tgt.release();
+    H->arena.release(); H->capn.release(); H->nexact.release(); H->pbase.release(); H->sstart.release();
+    H->hist_l.release(); H->fill_l.release(); H->brec_l.release(); H->cend.release();
     if (H->h_pinned) (void)hipHostFree(H->h_pinned);
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
@@ -533,6 +800,7 @@ int spr_load_rvg(spr_handle* H, const char* path) {
 
 int spr_set_mvp(spr_handle* H, const float m_rows[16]) {
     if (!H || !m_rows) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    if (std::memcmp(H->mvp.m, m_rows, sizeof(H->mvp.m)) != 0) H->lb_dirty = true;
     std::memcpy(H->mvp.m, m_rows, sizeof(H->mvp.m));
     return SPR_OK;
 }
@@ -542,7 +810,7 @@ int spr_render_frame(spr_handle* H, uint8_t* rgba_out) {
     return guarded(H, [&] {
         SPR_HIP(hipSetDevice(H->device));
         if (H->opts.mode == SPR_MODE_LEARNED && !H->inj)
-            spr::learned_frame(H, H->learned);
+            run_learned_frame(H);
         else
             run_frame(H);
         if (rgba_out) {
@@ -585,6 +853,9 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
         SPR_HIP(hipStreamSynchronize(H->stream));
         const bool debug = H->opts.debug != 0;
         const int64_t N = H->N;
+        // learned frames keep intersections in per-path arenas: position-indexed stages span
+        // the live positions (gaps have curve -1), sorted stages span the N entries
+        const int64_t NP = H->last_learned ? H->n_live : N;
         const void* src = nullptr;
         size_t n = 0;
         std::vector<uint8_t> tmp;
@@ -613,13 +884,13 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 break;
             }
             case SPR_STAGE_COUNTS: from_dev(H->counts.p, ((size_t)H->C + 1) * 4); break;
-            case SPR_STAGE_INTER: from_dev(H->inter.p, (size_t)N * 8); break;
-            case SPR_STAGE_KEY: from_dev(H->key.p, (size_t)N * 4); break;
+            case SPR_STAGE_INTER: from_dev(H->inter.p, (size_t)NP * 8); break;
+            case SPR_STAGE_KEY: from_dev(H->key.p, (size_t)NP * 4); break;
             case SPR_STAGE_WN: {
-                std::vector<int8_t> v((size_t)N);
-                if (N) SPR_HIP(hipMemcpy(v.data(), H->wn.p, (size_t)N, hipMemcpyDeviceToHost));
-                tmp.resize((size_t)N * 4);
-                for (int64_t i = 0; i < N; ++i) ((int32_t*)tmp.data())[i] = v[i];
+                std::vector<int8_t> v((size_t)NP);
+                if (NP) SPR_HIP(hipMemcpy(v.data(), H->wn.p, (size_t)NP, hipMemcpyDeviceToHost));
+                tmp.resize((size_t)NP * 4);
+                for (int64_t i = 0; i < NP; ++i) ((int32_t*)tmp.data())[i] = v[i];
                 src = tmp.data(); n = tmp.size();
                 break;
             }
@@ -640,6 +911,28 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 break;
             }
             case SPR_STAGE_IMAGE: from_dev(H->image.p, (size_t)4 * H->w * H->h); break;
+            case SPR_STAGE_FEATURES:
+                if (!H->last_learned) throw Fail{SPR_ERR_STATE, "features are computed by learned frames"};
+                from_dev(H->feat.p, (size_t)H->Np * spr::kMlpIn * 4);
+                break;
+            case SPR_STAGE_PRED:
+                if (!H->last_learned) throw Fail{SPR_ERR_STATE, "predictions are computed by learned frames"};
+                from_dev(H->pred.p, (size_t)H->Np * spr::kMlpOut * 4);
+                break;
+            case SPR_STAGE_PATH_COUNTS: {
+                if (!H->last_brec) throw Fail{SPR_ERR_STATE, "no frame rendered"};
+                if (!H->last_learned && H->inj) throw Fail{SPR_ERR_STATE, "not available with injected intersections"};
+                H->train_tgt.ensure((size_t)std::max<uint32_t>(H->Np, 1) * 2);
+                DBuf<int32_t> exact;
+                exact.ensure((size_t)std::max<uint32_t>(H->Np, 1) * 2);
+                SPR_HIP(spr::launch_path_targets(H->last_learned ? H->nexact.p : H->seg.p, H->last_learned ? 0 : 1,
+                                                 H->last_brec, H->bucket_base.p, H->Np, H->train_tgt.p, exact.p,
+                                                 H->stream));
+                SPR_HIP(hipStreamSynchronize(H->stream));  // the copy below runs on another stream
+                from_dev(exact.p, (size_t)H->Np * 8);
+                exact.release();
+                break;
+            }
             default: throw Fail{SPR_ERR_INVALID, "unknown stage"};
         }
         *nbytes = n;
@@ -665,6 +958,142 @@ int spr_debug_inject(spr_handle* H, int stage, const void* src, size_t nbytes) {
         }
         H->inj_inter.assign(v, v + 2 * n);
         H->inj = true;
+    });
+}
+
+// ---- learned allocation
+int spr_learned_train(spr_handle* H, const spr_scene* sc, const spr_train_opts* o, spr_train_report* out) {
+    if (!H || !sc) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        SPR_HIP(hipSetDevice(H->device));
+        const spr::Scene& S = sc->s;
+        const uint32_t Np = S.n_paths();
+        if (Np < 16) throw Fail{SPR_ERR_INVALID, "training scene needs at least 16 paths"};
+        const uint32_t chunk = o && o->chunk_paths ? o->chunk_paths : 5000u;
+        H->train_feat.ensure((size_t)Np * spr::kMlpIn);
+        H->train_tgt.ensure((size_t)Np * spr::kMlpOut);
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t first = 0; first < Np; first += chunk) {
+            const uint32_t cnt = std::min(chunk, Np - first);
+            const spr::Scene sub = spr::subset(S, first, cnt);
+            load_arrays(H, sub.pos.data(), sub.pos_path.data(), sub.n_points(), sub.curve_pos.data(),
+                        sub.curve_type.data(), sub.curve_path.data(), sub.n_curves(), sub.rule.data(),
+                        sub.fill.data(), sub.n_paths());
+            run_frame(H);
+            SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p,
+                                              H->rule.p, cnt, (int)H->w, (int)H->h,
+                                              H->train_feat.p + (size_t)first * spr::kMlpIn, H->stream));
+            SPR_HIP(spr::launch_path_targets(H->seg.p, 1, H->last_brec, H->bucket_base.p, cnt,
+                                             H->train_tgt.p + (size_t)first * spr::kMlpOut, nullptr, H->stream));
+            SPR_HIP(hipStreamSynchronize(H->stream));
+        }
+        const double collect_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        spr::TrainOptions to;
+        if (o) {
+            if (o->epochs) to.epochs = (int)o->epochs;
+            if (o->batch) to.batch = (int)o->batch;
+            if (o->lr > 0) to.lr = o->lr;
+            to.seed = o->seed;
+            if (o->overflow_quantile > 0) to.overflow_quantile = o->overflow_quantile;
+        }
+        spr::TrainReport rep;
+        spr::mlp_train(H->model, H->train_feat.p, H->train_tgt.p, (int)Np, to, rep, H->stream);
+        H->mlpdev.upload(H->model, H->stream);
+        H->loaded = false;
+        H->frame_done = false;
+        H->last_brec = nullptr;
+        H->lb_dirty = true;
+        if (out) {
+            out->samples = rep.samples;
+            out->steps = rep.steps;
+            out->first_loss = rep.first_loss;
+            out->train_loss = rep.train_loss;
+            out->val_loss = rep.val_loss;
+            out->log_margin = rep.log_margin;
+            out->collect_ms = collect_ms;
+            out->train_ms = rep.ms;
+        }
+    });
+}
+
+int spr_learned_save(spr_handle* H, const char* path) {
+    if (!H || !path) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        try { spr::mlp_save(H->model, path); } catch (const std::exception& e) { throw Fail{SPR_ERR_IO, e.what()}; }
+    });
+}
+
+int spr_learned_load(spr_handle* H, const char* path) {
+    if (!H || !path) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        try { spr::mlp_load(H->model, path); } catch (const std::exception& e) { throw Fail{SPR_ERR_IO, e.what()}; }
+        H->mlpdev.release();
+        H->lb_dirty = true;
+    });
+}
+
+int spr_learned_set_margin(spr_handle* H, float log2_margin, int32_t slack) {
+    if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (!(log2_margin > -30.0f && log2_margin < 30.0f) || slack < 0) throw Fail{SPR_ERR_INVALID, "bad margin"};
+        H->model.log_margin = log2_margin;
+        H->model.slack = slack;
+    });
+}
+
+int spr_mlp_get_params(spr_handle* H, float* dst, size_t cap, size_t* n) {
+    if (!H || !n) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        const size_t total = (size_t)spr::kMlpParams + 2 * spr::kMlpIn;
+        *n = total;
+        if (!dst) return;
+        if (!H->model.ready) throw Fail{SPR_ERR_STATE, "no model"};
+        if (cap < total) throw Fail{SPR_ERR_CAPACITY, "destination too small"};
+        std::memcpy(dst, H->model.params.data(), spr::kMlpParams * 4);
+        std::memcpy(dst + spr::kMlpParams, H->model.mean.data(), spr::kMlpIn * 4);
+        std::memcpy(dst + spr::kMlpParams + spr::kMlpIn, H->model.inv_std.data(), spr::kMlpIn * 4);
+    });
+}
+
+int spr_mlp_set_params(spr_handle* H, const float* src, size_t n) {
+    if (!H || !src) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (n != (size_t)spr::kMlpParams + 2 * spr::kMlpIn) throw Fail{SPR_ERR_INVALID, "expected 5410 floats"};
+        H->model.params.assign(src, src + spr::kMlpParams);
+        H->model.mean.assign(src + spr::kMlpParams, src + spr::kMlpParams + spr::kMlpIn);
+        H->model.inv_std.assign(src + spr::kMlpParams + spr::kMlpIn, src + n);
+        H->model.ready = true;
+        H->mlpdev.release();
+        H->lb_dirty = true;
+    });
+}
+
+int spr_mlp_predict(spr_handle* H, const float* features, uint32_t n, float* out) {
+    if (!H || (!features && n) || (!out && n)) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        SPR_HIP(hipSetDevice(H->device));
+        if (!H->mlpdev.valid()) H->mlpdev.upload(H->model, H->stream);
+        if (!n) return;
+        DBuf<float> f, p;
+        f.ensure((size_t)n * spr::kMlpIn);
+        p.ensure((size_t)n * spr::kMlpOut);
+        SPR_HIP(hipMemcpyAsync(f.p, features, (size_t)n * spr::kMlpIn * 4, hipMemcpyHostToDevice, H->stream));
+        SPR_HIP(spr::launch_mlp_infer(f.p, n, H->mlpdev.dev(), p.p, H->stream));
+        SPR_HIP(hipMemcpyAsync(out, p.p, (size_t)n * spr::kMlpOut * 4, hipMemcpyDeviceToHost, H->stream));
+        SPR_HIP(hipStreamSynchronize(H->stream));
+        f.release();
+        p.release();
+    });
+}
+
+int spr_mlp_gradients(spr_handle* H, const float* features, const float* targets, uint32_t n, float* grads,
+                      float* loss) {
+    if (!H || !features || !targets || !grads || !loss) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        SPR_HIP(hipSetDevice(H->device));
+        if (!H->model.ready) throw Fail{SPR_ERR_STATE, "no model"};
+        spr::mlp_gradients(H->model, features, targets, (int)n, grads, loss, H->stream);
     });
 }
 
@@ -732,7 +1161,3 @@ void spr_scene_free(spr_scene* s) { delete s; }
 
 }  // extern "C"
 
-// Entry used by the learned-allocation module to run the plain frame.
-namespace spr {
-void run_plain_frame(spr_handle* H) { run_frame(H); }
-}
