@@ -203,6 +203,26 @@ int spr_mlp_predict(spr_handle* h, const float* features, uint32_t n, float* out
 int spr_mlp_gradients(spr_handle* h, const float* features, const float* targets, uint32_t n, float* grads,
                       float* loss);
 
+/* --- batches of documents.  No reference counterpart (the reference renders one scene).  Many
+ *     independent documents of different sizes and canvases are loaded into one handle and
+ *     rendered by one frame without host synchronisation: every stage runs once over all
+ *     documents, each document keeps its own canvas and winding numbers, and once the scene is
+ *     unchanged the frame is replayed as one captured HIP graph.  Uses the handle's learned
+ *     model for the per-path arenas when one is loaded; otherwise every path takes an exact block
+ *     of the overflow pool.  spr_render_frame(h, NULL) renders the batch; the images are read
+ *     with spr_batch_image (4*w*h bytes each, top row first). */
+int spr_load_batch(spr_handle* h, const spr_scene* const* docs, const uint32_t* widths, const uint32_t* heights,
+                   uint32_t n_docs);
+int spr_batch_image(spr_handle* h, uint32_t doc, uint8_t* rgba_out, size_t cap);
+/* Project-written autotuner: times the captured frame of the loaded batch (or learned scene)
+ * for each candidate of the persistent-kernel occupancies (chain solve and long-piece merge
+ * blocks per CU, 4/8/16 each, `reps` replays per candidate) and keeps the fastest. */
+typedef struct spr_tune_report {
+    int32_t chain_blocks_per_cu, merge_blocks_per_cu, n_tried, reserved;
+    double default_ms, best_ms;
+} spr_tune_report;
+int spr_autotune(spr_handle* h, uint32_t reps, spr_tune_report* out);
+
 #ifdef __cplusplus
 }
 #endif

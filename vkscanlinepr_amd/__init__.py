@@ -61,6 +61,12 @@ class TrainOpts(C.Structure):
                 ("reserved", C.c_uint32 * 3)]
 
 
+class TuneReport(C.Structure):
+    _fields_ = [("chain_blocks_per_cu", C.c_int32), ("merge_blocks_per_cu", C.c_int32),
+                ("n_tried", C.c_int32), ("reserved", C.c_int32), ("default_ms", C.c_double),
+                ("best_ms", C.c_double)]
+
+
 class TrainReport(C.Structure):
     _fields_ = [("samples", C.c_int64), ("steps", C.c_int64), ("first_loss", C.c_double),
                 ("train_loss", C.c_double), ("val_loss", C.c_double), ("log_margin", C.c_double),
@@ -84,7 +90,8 @@ EXPORTS = [
     "spr_debug_inject", "spr_scene_load_rvg", "spr_scene_generate", "spr_scene_subset",
     "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free", "spr_learned_train",
     "spr_learned_save", "spr_learned_load", "spr_learned_set_margin", "spr_mlp_get_params",
-    "spr_mlp_set_params", "spr_mlp_predict", "spr_mlp_gradients",
+    "spr_mlp_set_params", "spr_mlp_predict", "spr_mlp_gradients", "spr_load_batch", "spr_batch_image",
+    "spr_autotune",
 ]
 
 
@@ -129,6 +136,9 @@ def lib() -> C.CDLL:
     L.spr_mlp_set_params.argtypes = [vp, f32p, C.c_size_t]
     L.spr_mlp_predict.argtypes = [vp, f32p, C.c_uint32, f32p]
     L.spr_mlp_gradients.argtypes = [vp, f32p, f32p, C.c_uint32, f32p, f32p]
+    L.spr_load_batch.argtypes = [vp, C.POINTER(vp), u32p, u32p, C.c_uint32]
+    L.spr_batch_image.argtypes = [vp, C.c_uint32, C.c_void_p, C.c_size_t]
+    L.spr_autotune.argtypes = [vp, C.c_uint32, C.POINTER(TuneReport)]
     _lib = L
     return L
 
@@ -354,3 +364,34 @@ def _rasterizer_learned_methods():
 
 
 _rasterizer_learned_methods()
+
+
+def _rasterizer_batch_methods():
+    """Batches of documents rendered by one frame (one captured graph once unchanged)."""
+
+    def load_batch(self, scenes, canvases) -> None:
+        """scenes: list of Scene; canvases: list of (width, height)."""
+        n = len(scenes)
+        arr = (C.c_void_p * n)(*[sc._h.value for sc in scenes])
+        w = np.ascontiguousarray([c[0] for c in canvases], dtype=np.uint32)
+        h = np.ascontiguousarray([c[1] for c in canvases], dtype=np.uint32)
+        self._c(lib().spr_load_batch(self._h, arr, _ptr(w, C.c_uint32), _ptr(h, C.c_uint32), n))
+        self._batch = [(int(a), int(b)) for a, b in zip(w, h)]
+        self._batch_scenes = list(scenes)  # keep the scene handles alive with the batch
+
+    def batch_image(self, doc: int) -> np.ndarray:
+        w, h = self._batch[doc]
+        out = np.empty((h, w, 4), dtype=np.uint8)
+        self._c(lib().spr_batch_image(self._h, int(doc), out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def autotune(self, reps: int = 3) -> dict:
+        r = TuneReport()
+        self._c(lib().spr_autotune(self._h, int(reps), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneReport._fields_ if k != "reserved"}
+
+    for f in (load_batch, batch_image, autotune):
+        setattr(Rasterizer, f.__name__, f)
+
+
+_rasterizer_batch_methods()

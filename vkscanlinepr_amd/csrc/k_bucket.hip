@@ -26,9 +26,11 @@ inline unsigned grid_for(int64_t n) {
 // Per-path bucket table from the path's cell-row bounds [lo, hi] (clamped to the canvas):
 // in-canvas rows r >= 1 get buckets first..first+nrows-1, then the off-canvas bucket, then row 0.
 __global__ void k_bucket_counts(const int32_t* __restrict__ row_lo, const int32_t* __restrict__ row_hi,
-                                uint32_t Np, int rmax, int32_t* __restrict__ nb, int2* __restrict__ prow) {
+                                uint32_t Np, int rmax, int32_t* __restrict__ nb, int2* __restrict__ prow,
+                                const int2* __restrict__ pcanvas) {
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= Np) return;
+    if (pcanvas) rmax = (pcanvas[p].y - 1) / 2;
     int lo = row_lo[p], hi = row_hi[p];
     int first = lo < 1 ? 1 : lo;
     int last = hi > rmax ? rmax : hi;
@@ -130,9 +132,9 @@ __global__ void k_band_emit(const int32_t* __restrict__ flag_scan, const int32_t
 }  // namespace
 
 hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, uint32_t Np, int rmax,
-                                int32_t* nb, int2* prow, hipStream_t s) {
+                                int32_t* nb, int2* prow, hipStream_t s, const int2* pcanvas) {
     if (Np == 0) return hipSuccess;
-    k_bucket_counts<<<grid_for(Np), kBlock, 0, s>>>(row_lo, row_hi, Np, rmax, nb, prow);
+    k_bucket_counts<<<grid_for(Np), kBlock, 0, s>>>(row_lo, row_hi, Np, rmax, nb, prow, pcanvas);
     return hipGetLastError();
 }
 

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
     const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
     const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
     int h, ChainDesc* __restrict__ chains, PieceDesc* __restrict__ pieces,
-    int32_t* __restrict__ class_count, const int32_t* __restrict__ ends) {
+    int32_t* __restrict__ class_count, const int32_t* __restrict__ ends, const int2* __restrict__ pcanvas) {
     __shared__ int32_t hist[kClasses];
     if (threadIdx.x < kClasses) hist[threadIdx.x] = 0;
     __syncthreads();
@@ -91,6 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
         const float4 qq = cuts[c];
         const float q[4] = {qq.x, qq.y, qq.z, qq.w};
         PieceCtx pc;
+        if (pcanvas) { const int2 cvs = pcanvas[cpath[c]]; w = cvs.x; h = cvs.y; }
         if (setup_piece(cv, q, ncuts[c], piece, w, h, pc)) {
             const int base = offsets[c] + pc.off;
             // the curve's range ends at the next curve's offset, or at ends[c] when curves are
@@ -483,10 +484,11 @@ hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uin
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
                              ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s,
-                             const int32_t* ends) {
+                             const int32_t* ends, const int2* pcanvas) {
     if (C == 0) return hipSuccess;
     k_chain_prep<<<grid_for((int64_t)C * 5), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts, ncuts,
-                                                             offsets, C, w, h, chains, pieces, class_count, ends);
+                                                             offsets, C, w, h, chains, pieces, class_count, ends,
+                                                             pcanvas);
     return hipGetLastError();
 }
 

@@ -32,7 +32,8 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
                                                       const int2* __restrict__ prow,
                                                       const uint32_t* __restrict__ fill, uint32_t Np,
                                                       int w, int h, int64_t rec_cap,
-                                                      uint8_t* __restrict__ image) {
+                                                      uint8_t* __restrict__ image,
+                                                      const DocDev* __restrict__ docs, int n_docs) {
     __shared__ unsigned long long mask[kChunk];
     __shared__ int2 list[kBlock];
     __shared__ uint32_t lcol[kBlock];
@@ -40,7 +41,21 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
     __shared__ float lia[kBlock];    // 1 - a
     __shared__ int32_t wcount[kBlock / 64];
     __shared__ int32_t bpre[kBatch + 1];
-    const int r = blockIdx.x;
+    int r = blockIdx.x;
+    uint32_t pbeg = 0, pend = Np;
+    if (docs) {
+        // batch: find this cell row's document (documents are numbered by first row)
+        int lo = 0, hi = n_docs;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (docs[mid].row0 <= r) lo = mid; else hi = mid;
+        }
+        const DocDev d = docs[lo];
+        r -= d.row0;
+        w = d.w; h = d.h;
+        pbeg = (uint32_t)d.p0; pend = (uint32_t)d.p1;
+        image += d.img_off;
+    }
     const int y = 2 * r;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int c0 = 0; c0 < w; c0 += kChunk) {
@@ -54,11 +69,11 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
             mask[j * kBlock + tid] = 0ull;
         }
         __syncthreads();
-        for (uint32_t p0 = 0; p0 < Np; p0 += kBlock) {
+        for (uint32_t p0 = pbeg; p0 < pend; p0 += kBlock) {
             // ordered list of the paths of this chunk that have records on this cell row
             const uint32_t p = p0 + tid;
             int2 rng = make_int2(0, 0);
-            if (p < Np) {
+            if (p < pend) {
                 const int2 pr = prow[p];
                 const int loc = r == 0 ? pr.y + 1 : ((r >= pr.x && r < pr.x + pr.y) ? r - pr.x : -1);
                 if (loc >= 0) rng = brec[bucket_base[p] + loc];
@@ -188,11 +203,16 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
 
 hipError_t launch_composite(const int4* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
-                            uint8_t* image, hipStream_t s, int64_t rec_cap) {
-    const int rows = (h + 1) / 2;
-    if (rows == 0 || w == 0) return hipSuccess;
-    if (compose) k_composite<1><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image);
-    else k_composite<0><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image);
+                            uint8_t* image, hipStream_t s, int64_t rec_cap, const DocDev* docs, int n_docs,
+                            int total_rows) {
+    const int rows = docs ? total_rows : (h + 1) / 2;
+    if (rows == 0 || (!docs && w == 0)) return hipSuccess;
+    if (compose)
+        k_composite<1><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image,
+                                               docs, n_docs);
+    else
+        k_composite<0><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image,
+                                               docs, n_docs);
     return hipGetLastError();
 }
 

@@ -38,12 +38,17 @@ __global__ __launch_bounds__(kBlock) void k_transform(const float2* __restrict__
                                                       float2* __restrict__ tpos,
                                                       int32_t* __restrict__ visible,
                                                       int32_t* __restrict__ row_lo,
-                                                      int32_t* __restrict__ row_hi) {
+                                                      int32_t* __restrict__ row_hi,
+                                                      const int2* __restrict__ pcanvas) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < P;
     int32_t bits = 0, lo = 0, hi = 0;
     uint32_t path = 0xFFFFFFFFu;
     if (valid) {
+        if (pcanvas) {  // documents of a batch: each path has its own canvas
+            const int2 cv = pcanvas[pos_path[i]];
+            fw = (float)cv.x; fh = (float)cv.y; rmax = (cv.y - 1) / 2;
+        }
         float2 p = pos[i];
         float o[4];
 #pragma unroll
@@ -191,7 +196,8 @@ __global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict_
                                                          const int2* __restrict__ prow,
                                                          int32_t* __restrict__ hist,
                                                          int32_t* __restrict__ err,
-                                                         const int32_t* __restrict__ n_dev) {
+                                                         const int32_t* __restrict__ n_dev,
+                                                         const int2* __restrict__ pcanvas) {
     const int64_t n = n_dev ? min((int64_t)*n_dev, N) : N;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool valid = i < n;
@@ -201,7 +207,11 @@ __global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict_
         epath[i] = kNoPath;
         valid = false;
     }
-    if (valid) bucket = gen_one(inter, n, i, cpos, ctype, cpath, tpos, w, h, key, wn, epath, bucket_base, prow, err);
+    if (valid) {
+        int cw = w, ch = h;
+        if (pcanvas) { const int2 cv = pcanvas[cpath[inter[i].x]]; cw = cv.x; ch = cv.y; }
+        bucket = gen_one(inter, n, i, cpos, ctype, cpath, tpos, cw, ch, key, wn, epath, bucket_base, prow, err);
+    }
     wave_count(hist, bucket, valid);
 }
 
@@ -226,10 +236,10 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
 
 hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
                             int w, int h, float2* tpos, int32_t* visible, int32_t* row_lo, int32_t* row_hi,
-                            hipStream_t s) {
+                            hipStream_t s, const int2* pcanvas) {
     if (P == 0) return hipSuccess;
     k_transform<<<grid_for(P), kBlock, 0, s>>>(pos, pos_path, P, m, (float)w, (float)h, (h - 1) / 2, tpos,
-                                               visible, row_lo, row_hi);
+                                               visible, row_lo, row_hi, pcanvas);
     return hipGetLastError();
 }
 
@@ -258,10 +268,10 @@ hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpo
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
                                const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
-                               hipStream_t s, const int32_t* n_dev) {
+                               hipStream_t s, const int32_t* n_dev, const int2* pcanvas) {
     if (N == 0) return hipSuccess;
     k_gen_fragment<<<grid_for(N), kBlock, 0, s>>>(inter, N, cpos, ctype, cpath, tpos, w, h, key, wn, epath,
-                                                  bucket_base, prow, hist, err, n_dev);
+                                                  bucket_base, prow, hist, err, n_dev, pcanvas);
     return hipGetLastError();
 }
 

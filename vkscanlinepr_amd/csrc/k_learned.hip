@@ -57,10 +57,13 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
                                                        const int32_t* __restrict__ visible,
                                                        const uint32_t* __restrict__ rule, uint32_t Np,
                                                        int w, int h, float* __restrict__ feat,
-                                                       unsigned long long* __restrict__ bound_total) {
+                                                       unsigned long long* __restrict__ bound_total,
+                                                       const int2* __restrict__ pcanvas,
+                                                       int32_t* __restrict__ pbound) {
     const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (p >= Np) return;
+    if (pcanvas) { w = pcanvas[p].x; h = pcanvas[p].y; }
     const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
     float n_cub = 0, xmin = 3.4e38f, xmax = -3.4e38f, ymin = 3.4e38f, ymax = -3.4e38f;
     float plen = 0, sdx = 0, sdy = 0, cdx = 0, cdy = 0, turn = 0, inside = 0, npts = 0;
@@ -112,14 +115,15 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
         f[13] = npts > 0 ? inside / npts : 0.0f;
         f[14] = (float)rule[p];
         f[15] = log2f(1.0f + 0.25f * cw * ch);
-        if (bound_total && visible_mask(visible[p])) {
-            // Upper bound of the path's intersections.  A Bezier curve varies no more than its
-            // control polygon; its <= 5 pieces cover [0, 1] at most three times over (the
-            // 4-cut order quirk runs one piece backwards); a piece from a to b gives at most
-            // 1 + (|dx|/2 + 1) + (|dy|/2 + 1) entries.  The float terms carry a 0.1% margin.
-            const double b = 1.5 * ((double)sdx + (double)sdy) * 1.001 + 20.0 * (double)(c1 - c0) + 2.0;
-            atomicAdd(bound_total, (unsigned long long)b);
-        }
+        // Upper bound of the path's intersections.  A Bezier curve varies no more than its
+        // control polygon; its <= 5 pieces cover [0, 1] at most three times over (the 4-cut
+        // order quirk runs one piece backwards); a piece from a to b gives at most
+        // 1 + (|dx|/2 + 1) + (|dy|/2 + 1) entries.  The float terms carry a 0.1% margin.
+        double b = 0.0;
+        if (visible_mask(visible[p])) b = 1.5 * ((double)sdx + (double)sdy) * 1.001 + 20.0 * (double)(c1 - c0) + 2.0;
+        if (!(b < 1e15)) b = 1e15;  // non-finite coordinates
+        if (bound_total && b > 0.0) atomicAdd(bound_total, (unsigned long long)b);
+        if (pbound) pbound[p] = b < 1073741824.0 ? (int32_t)b : 1073741824;
     }
 }
 
@@ -200,7 +204,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ fea
 // to `arena` (scanned afterwards into arena bases); the record capacities are summed.
 __global__ void k_caps(const float* __restrict__ pred, const int32_t* __restrict__ visible, uint32_t Np,
                        float log_margin, int32_t slack, int32_t* __restrict__ arena, int32_t* __restrict__ cap_n,
-                       int32_t* __restrict__ cap_r_total) {
+                       int32_t* __restrict__ cap_r_total, const int32_t* __restrict__ pbound) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t cr = 0;
     if (p < Np) {
@@ -212,6 +216,9 @@ __global__ void k_caps(const float* __restrict__ pred, const int32_t* __restrict
             cr = (int32_t)ceilf(exp2f(er)) - 1 + slack;
             cn = cn < 0 ? 0 : cn;
             cr = cr < 0 ? 0 : cr;
+            // never more than the path's proven bound (records: two per entry), so the arena
+            // total stays within the bound total
+            if (pbound) { cn = min(cn, pbound[p]); cr = min(cr, 2 * min(pbound[p], 1 << 29)); }
         }
         arena[p] = cn;
         cap_n[p] = cn;
@@ -229,11 +236,13 @@ __global__ __launch_bounds__(256) void k_learned_count(
     const int32_t* __restrict__ arena, const int32_t* __restrict__ cap_n, int64_t budget,
     float4* __restrict__ cuts, uint32_t* __restrict__ ncuts, int32_t* __restrict__ counts,
     int32_t* __restrict__ ends, int32_t* __restrict__ ctl, int32_t* __restrict__ n_exact, int32_t* __restrict__ pbase,
-    int2* __restrict__ inter, int8_t* __restrict__ mark, int32_t* __restrict__ err) {
+    int2* __restrict__ inter, int8_t* __restrict__ mark, int32_t* __restrict__ err,
+    const int2* __restrict__ pcanvas) {
     __shared__ int32_t s_w[kWaves];
     __shared__ int64_t s_base;
     __shared__ int32_t s_mode;  // 0 arena, 1 pool, 2 dropped
     const uint32_t p = blockIdx.x;
+    if (pcanvas) { w = pcanvas[p].x; h = pcanvas[p].y; }
     const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
     const bool vis = visible_mask(visible[p]);
     int32_t carry = 0;
@@ -398,6 +407,13 @@ __global__ void k_gather_rows(const float* __restrict__ src, const int32_t* __re
     dst[i] = src[(int64_t)idx[r] * K + k];
 }
 
+__global__ void k_doc_winding_base(const int32_t* __restrict__ winding, const int32_t* __restrict__ sstart,
+                                   const int32_t* __restrict__ doc_first_path, uint32_t Np,
+                                   int32_t* __restrict__ wbase) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < Np) wbase[p] = winding[sstart[doc_first_path[p]]];
+}
+
 inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b < 1 ? 1 : (n + b - 1) / b); }
 
 }  // namespace
@@ -405,10 +421,10 @@ inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) /
 hipError_t launch_path_features(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                                 const uint32_t* pcurve, const int32_t* visible, const uint32_t* rule,
                                 uint32_t Np, int w, int h, float* feat, hipStream_t s,
-                                unsigned long long* bound_total) {
+                                unsigned long long* bound_total, const int2* pcanvas, int32_t* pbound) {
     if (Np == 0) return hipSuccess;
     k_path_features<<<(Np + 3) / 4, 256, 0, s>>>(tpos, cpos, ctype, pcurve, visible, rule, Np, w, h, feat,
-                                                 bound_total);
+                                                 bound_total, pcanvas, pbound);
     return hipGetLastError();
 }
 
@@ -419,9 +435,9 @@ hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, flo
 }
 
 hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, float log_margin, int32_t slack,
-                       int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s) {
+                       int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s, const int32_t* pbound) {
     if (Np == 0) return hipSuccess;
-    k_caps<<<blocks(Np), 256, 0, s>>>(pred, visible, Np, log_margin, slack, arena, cap_n, cap_r_total);
+    k_caps<<<blocks(Np), 256, 0, s>>>(pred, visible, Np, log_margin, slack, arena, cap_n, cap_r_total, pbound);
     return hipGetLastError();
 }
 
@@ -429,11 +445,18 @@ hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const 
                                 const uint32_t* pcurve, uint32_t Np, int w, int h, const int32_t* arena,
                                 const int32_t* cap_n, int64_t budget, float4* cuts, uint32_t* ncuts,
                                 int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact, int32_t* pbase,
-                                int2* inter, int8_t* mark, int32_t* err, hipStream_t s) {
+                                int2* inter, int8_t* mark, int32_t* err, hipStream_t s, const int2* pcanvas) {
     if (Np == 0) return hipSuccess;
     k_learned_count<<<Np, 256, 0, s>>>(tpos, cpos, ctype, visible, pcurve, Np, w, h, arena, cap_n, budget, cuts,
-                                       ncuts, counts, ends, ctl, n_exact, pbase, inter, mark, err);
+                                       ncuts, counts, ends, ctl, n_exact, pbase, inter, mark, err, pcanvas);
     k_learned_finish<<<1, 1, 0, s>>>(arena, Np, budget, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_doc_winding_base(const int32_t* winding, const int32_t* sstart, const int32_t* doc_first_path,
+                                   uint32_t Np, int32_t* wbase, hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_doc_winding_base<<<blocks(Np), 256, 0, s>>>(winding, sstart, doc_first_path, Np, wbase);
     return hipGetLastError();
 }
 

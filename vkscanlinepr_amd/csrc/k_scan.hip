@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     unsigned long long* __restrict__ status, uint32_t* __restrict__ counter,
     int4* __restrict__ records, int32_t* __restrict__ flags, int32_t* __restrict__ totals,
     const int32_t* __restrict__ bucket_base, const int2* __restrict__ prow, int2* __restrict__ brec,
-    int32_t* __restrict__ err) {
+    int32_t* __restrict__ err, const int2* __restrict__ pcanvas, const int32_t* __restrict__ wbase) {
     __shared__ int32_t s_tile;
     __shared__ unsigned long long s_w[kWaves];
     __shared__ unsigned long long s_prefix;
@@ -158,7 +158,9 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
         const uint32_t p1 = cur.at(f);
         keys[j] = k1; paths[j] = p1;
         const int x1 = key_x(k1), y1 = key_y(k1);
-        const bool in_canvas = !(x1 < 0 || y1 < 0 || x1 >= w || y1 >= h);
+        int cw = w, ch = h;
+        if (pcanvas) { const int2 cv = pcanvas[p1]; cw = cv.x; ch = cv.y; }
+        const bool in_canvas = !(x1 < 0 || y1 < 0 || x1 >= cw || y1 >= ch);
         bool cell, span = false;
         if (f == 0) {
             cell = in_canvas;
@@ -166,7 +168,8 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
             const uint32_t r = rule[p1];
             const int x0 = key_x(kprev), y0 = key_y(kprev);
             cell = in_canvas && (pprev != p1 || kprev != k1);
-            const int32_t wn = winding[f];
+            // documents of a batch each scan their own winding numbers from zero
+            const int32_t wn = (int32_t)((uint32_t)winding[f] - (uint32_t)(wbase ? wbase[p1] : 0));
             const bool wf = (r == 0u && wn != 0) || (r == 1u && (wn & 1) != 0);
             span = (y0 == y1) && (x0 + 2 < x1) && (pprev == p1) && wf;
         }
@@ -218,7 +221,8 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
             }
             // record range of this element's (path, cell row) bucket, for the composite
             const int y1 = key_y(k1);
-            if ((uint32_t)k1 != 0xFFFEFFFEu && y1 >= 0 && y1 < h) {
+            const int chh = pcanvas ? pcanvas[p1].y : h;
+            if ((uint32_t)k1 != 0xFFFEFFFEu && y1 >= 0 && y1 < chh) {
                 const int32_t kn = j + 1 < kScanItems ? keys[j + 1 < kScanItems ? j + 1 : j] : knext_out;
                 const uint32_t pn = j + 1 < kScanItems ? paths[j + 1 < kScanItems ? j + 1 : j] : pnext_out;
                 const bool first = (f == 0) || pp != p1 || (kp >> 16) != (k1 >> 16);
@@ -263,12 +267,12 @@ hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const u
                             ScanCtl ctl, int4* records, int32_t* flags, int32_t* totals,
                             const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s,
                             const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
-                            int32_t* err) {
+                            int32_t* err, const int2* pcanvas, const int32_t* wbase) {
     if (N == 0) return hipSuccess;
     k_mark_emit<<<(unsigned)scan_tiles(N), kScanThreads, 0, s>>>(skey, winding, epath, rule, fill, N, n_dev,
                                                                  sstart, Np, rec_cap, w, h, ctl.status,
                                                                  ctl.counter, records, flags, totals,
-                                                                 bucket_base, prow, brec, err);
+                                                                 bucket_base, prow, brec, err, pcanvas, wbase);
     return hipGetLastError();
 }
 

@@ -27,7 +27,7 @@ hipError_t launch_fill_u32_dev(uint32_t* p, int64_t n_max, const int32_t* n_dev,
 // ---- geometry (a3, a4, a7, a8)
 hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
                             int w, int h, float2* tpos, int32_t* visible, int32_t* row_lo, int32_t* row_hi,
-                            hipStream_t s);
+                            hipStream_t s, const int2* pcanvas = nullptr);
 hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                         const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
                         float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s);
@@ -50,7 +50,7 @@ hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uin
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
                              ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s,
-                             const int32_t* ends = nullptr);
+                             const int32_t* ends = nullptr, const int2* pcanvas = nullptr);
 hipError_t launch_chain_order(const ChainDesc* chains, uint32_t C, const int32_t* class_count,
                               int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order,
                               hipStream_t s);
@@ -69,7 +69,7 @@ hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpo
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
                                const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
-                               hipStream_t s, const int32_t* n_dev = nullptr);
+                               hipStream_t s, const int32_t* n_dev = nullptr, const int2* pcanvas = nullptr);
 
 // ---- scans
 hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s,
@@ -79,7 +79,7 @@ hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl c
 // (key ^ 0x80000000, index); the winding change rides along in the low bits of the sort word.
 constexpr int kSortTile = 2048;
 hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, uint32_t Np, int rmax,
-                                int32_t* nb, int2* prow, hipStream_t s);
+                                int32_t* nb, int2* prow, hipStream_t s, const int2* pcanvas = nullptr);
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
                                  const int32_t* bucket_base, const int2* prow, const int32_t* boff,
                                  int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s,
@@ -104,7 +104,7 @@ hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const u
                             ScanCtl ctl, int4* records, int32_t* flags, int32_t* totals,
                             const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s,
                             const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
-                            int32_t* err);
+                            int32_t* err, const int2* pcanvas = nullptr, const int32_t* wbase = nullptr);
 
 // ---- learned single-pass allocation (k_learned.hip)
 constexpr int kMlpIn = 16, kMlpHidden = 64, kMlpOut = 2;
@@ -124,15 +124,18 @@ struct MlpDev {
 hipError_t launch_path_features(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                                 const uint32_t* pcurve, const int32_t* visible, const uint32_t* rule,
                                 uint32_t Np, int w, int h, float* feat, hipStream_t s,
-                                unsigned long long* bound_total = nullptr);
+                                unsigned long long* bound_total = nullptr, const int2* pcanvas = nullptr,
+                                int32_t* pbound = nullptr);
 hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, float* pred, hipStream_t s);
 hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, float log_margin, int32_t slack,
-                       int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s);
+                       int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s,
+                       const int32_t* pbound = nullptr);
 hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, int32_t* visible,
                                 const uint32_t* pcurve, uint32_t Np, int w, int h, const int32_t* arena,
                                 const int32_t* cap_n, int64_t budget, float4* cuts, uint32_t* ncuts,
                                 int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact, int32_t* pbase,
-                                int2* inter, int8_t* mark, int32_t* err, hipStream_t s);
+                                int2* inter, int8_t* mark, int32_t* err, hipStream_t s,
+                                const int2* pcanvas = nullptr);
 hipError_t launch_sorted_starts(const int32_t* boff, const int32_t* bucket_base, uint32_t Np, int32_t* sstart,
                                 hipStream_t s);
 hipError_t launch_path_targets(const int32_t* n_or_seg, int seg_form, const int2* brec, const int32_t* bucket_base,
@@ -148,9 +151,22 @@ hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, floa
                        float bc1, float bc2, hipStream_t s);
 hipError_t launch_gather_rows(const float* src, const int32_t* idx, int B, int K, float* dst, hipStream_t s);
 
+// ---- batches of documents: per-document paths, canvas, image offset and first cell row
+struct DocDev {
+    int32_t p0, p1;    // paths [p0, p1)
+    int32_t w, h;      // canvas
+    int64_t img_off;   // byte offset of the document's RGBA8 image
+    int32_t row0;      // first cell row of the document in the batch's row numbering
+    int32_t pad;
+};
+// wbase[p] = winding[first sorted entry of p's document] (each document scans from zero)
+hipError_t launch_doc_winding_base(const int32_t* winding, const int32_t* sstart, const int32_t* doc_first_path,
+                                   uint32_t Np, int32_t* wbase, hipStream_t s);
+
 // ---- composite (a12)
 hipError_t launch_composite(const int4* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
-                            uint8_t* image, hipStream_t s, int64_t rec_cap);
+                            uint8_t* image, hipStream_t s, int64_t rec_cap, const DocDev* docs = nullptr,
+                            int n_docs = 0, int total_rows = 0);
 
 }  // namespace spr

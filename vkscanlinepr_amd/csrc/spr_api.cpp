@@ -128,7 +128,7 @@ struct spr_handle {
     spr::MlpDevice mlpdev;
     DBuf<uint32_t> pcurve;                          // first curve of each path, Np+1
     DBuf<float> feat, pred, train_feat, train_tgt;
-    DBuf<int32_t> arena, capn, nexact, pbase, sstart, hist_l, fill_l, cend;
+    DBuf<int32_t> arena, capn, nexact, pbase, sstart, hist_l, fill_l, cend, pbound;
     DBuf<int2> brec_l;
     int64_t lb_n = 0, lb_r = 0;                     // learned frame budgets
     bool lb_dirty = true;                           // scene, MVP or model changed since they were set
@@ -136,6 +136,23 @@ struct spr_handle {
     int64_t learned_fallbacks = 0;                  // learned frames that read their budgets back
     bool last_learned = false;
     int2* last_brec = nullptr;                      // per-bucket record ranges of the last frame
+
+    // batch of documents (spr_load_batch): per-path canvas, per-document tables
+    bool batch = false;
+    uint32_t n_docs = 0;
+    int total_rows = 0, max_h = 0;
+    size_t img_bytes = 0;
+    std::vector<spr::DocDev> h_docs;
+    DBuf<int2> pcanvas;
+    DBuf<spr::DocDev> docs;
+    DBuf<int32_t> doc_first, wbase;
+
+    // captured frame (no-host-sync frames only) and launch knobs picked by spr_autotune
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int64_t graph_lb_n = -1, graph_lb_r = -1;
+    bool capturing = false;
+    int chain_k = 8, merge_k = 8;                   // persistent blocks per CU
 
     // last frame
     int64_t N = 0;
@@ -185,7 +202,7 @@ spr::ScanCtl ctl_at(spr_handle* H, size_t st, size_t ctr_word, int counter) {
 }
 
 void stage_mark(spr_handle* H, int idx) {
-    if (!H->stage_timing) return;
+    if (!H->stage_timing || H->capturing) return;
     if ((int)H->ev_stage.size() <= idx) return;
     SPR_HIP(hipEventRecord(H->ev_stage[idx], H->stream));
     H->n_marks = idx + 1;
@@ -397,20 +414,28 @@ LearnedLayout learned_layout(uint32_t Np, int64_t nb_budget, int64_t n_budget) {
     return z;
 }
 
+void learned_finish(spr_handle* H);
+
+// Enqueues one frame without any host synchronisation unless the budgets must be derived
+// (H->lb_dirty).  With H->capturing the work is being recorded into a graph: no events, no
+// waits, and every buffer already has its size.
 void run_learned_frame(spr_handle* H) {
     if (!H->loaded) throw Fail{SPR_ERR_STATE, "render before load"};
-    if (!H->model.ready)
+    const bool use_model = H->model.ready;
+    if (!use_model && !H->batch)
         throw Fail{SPR_ERR_STATE, "learned mode needs a trained or loaded model (spr_learned_train or spr_learned_load)"};
+    if (H->capturing && H->lb_dirty) throw Fail{SPR_ERR_STATE, "internal: capture of a frame without budgets"};
     hipStream_t s = H->stream;
-    if (!H->mlpdev.valid()) H->mlpdev.upload(H->model, s);
+    if (use_model && !H->mlpdev.valid()) H->mlpdev.upload(H->model, s);
     const uint32_t P = H->P, C = H->C, Np = H->Np;
     const int w = (int)H->w, h = (int)H->h;
     const bool debug = H->opts.debug != 0;
-    const int rmax = (h - 1) / 2;
+    const int rmax = ((H->batch ? H->max_h : h) - 1) / 2;
     const int64_t nb_budget = (int64_t)Np * (rmax + 2);
     if (nb_budget > (int64_t)1 << 30) throw Fail{SPR_ERR_CAPACITY, "bucket bound exceeds 2^30"};
+    const int2* pcv = H->batch ? H->pcanvas.p : nullptr;
 
-    SPR_HIP(hipEventRecord(H->ev_begin, s));
+    if (!H->capturing) SPR_HIP(hipEventRecord(H->ev_begin, s));
     H->n_marks = 0;
     stage_mark(H, 0);
     H->visible.ensure(std::max<uint32_t>(Np, 1));
@@ -430,6 +455,7 @@ void run_learned_frame(spr_handle* H) {
     H->nexact.ensure((size_t)Np + 1);
     H->pbase.ensure((size_t)Np + 1);
     H->sstart.ensure((size_t)Np + 1);
+    H->pbound.ensure((size_t)Np + 1);
 
     // phase 1: transform, features, prediction, arenas (no dependency on any count)
     LearnedLayout z = learned_layout(Np, nb_budget, std::max<int64_t>(H->lb_n, 1));
@@ -439,16 +465,22 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_lo.p, Np, 0x7FFFFFFFu, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_hi.p, Np, 0x80000000u, s));
     SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p,
-                                  H->row_lo.p, H->row_hi.p, s));
+                                  H->row_lo.p, H->row_hi.p, s, pcv));
     stage_mark(H, 1);
     int32_t* d_capr = reinterpret_cast<int32_t*>(H->zero.p + z.capr);
     SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p, H->rule.p, Np,
-                                      w, h, H->feat.p, s, H->zero.p + z.bound));
-    SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s));
-    SPR_HIP(spr::launch_caps(H->pred.p, H->visible.p, Np, H->model.log_margin, H->model.slack, H->arena.p,
-                             H->capn.p, d_capr, s));
+                                      w, h, H->feat.p, s, H->zero.p + z.bound, pcv, H->pbound.p));
+    if (use_model) {
+        SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s));
+        SPR_HIP(spr::launch_caps(H->pred.p, H->visible.p, Np, H->model.log_margin, H->model.slack, H->arena.p,
+                                 H->capn.p, d_capr, s, H->pbound.p));
+    } else {
+        // no model: every path takes an exact block of the pool
+        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->arena.p, (int64_t)Np + 1, 0u, s));
+        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->capn.p, (int64_t)Np + 1, 0u, s));
+    }
     SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, 0), s));
-    SPR_HIP(spr::launch_bucket_counts(H->row_lo.p, H->row_hi.p, Np, rmax, H->bucket_base.p, H->prow.p, s));
+    SPR_HIP(spr::launch_bucket_counts(H->row_lo.p, H->row_hi.p, Np, rmax, H->bucket_base.p, H->prow.p, s, pcv));
     SPR_HIP(spr::launch_scan_i32(H->bucket_base.p, H->bucket_base.p, Np, ctl_at(H, z.st_pb, z.counters, 1), s));
     stage_mark(H, 2);
     if (H->lb_dirty) {
@@ -489,7 +521,7 @@ void run_learned_frame(spr_handle* H) {
     H->big_list.ensure((size_t)max_bands);
     if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); }
     H->records.ensure(nr);
-    H->image.ensure((size_t)4 * H->w * H->h);
+    H->image.ensure(H->batch ? H->img_bytes : (size_t)4 * H->w * H->h);
     H->hist_l.ensure((size_t)nb_budget + 1);
     H->fill_l.ensure((size_t)nb_budget + 1);
     H->brec_l.ensure((size_t)nb_budget + 1);
@@ -502,7 +534,8 @@ void run_learned_frame(spr_handle* H) {
     // phase 2: exact counts placed in the arenas, then the plain stages on device counts
     SPR_HIP(spr::launch_learned_count(H->tpos.p, H->cpos.p, H->ctype.p, H->visible.p, H->pcurve.p, Np, w, h,
                                       H->arena.p, H->capn.p, H->lb_n, H->cuts.p, H->ncuts.p, H->counts.p,
-                                      H->cend.p, d_ctl, H->nexact.p, H->pbase.p, H->inter.p, H->wn.p, d_err, s));
+                                      H->cend.p, d_ctl, H->nexact.p, H->pbase.p, H->inter.p, H->wn.p, d_err, s,
+                                      pcv));
     stage_mark(H, 3);
     int32_t* d_cc = reinterpret_cast<int32_t*>(H->zero.p + z.chainctl);
     int32_t* d_cfill = d_cc + 32;
@@ -520,25 +553,26 @@ void run_learned_frame(spr_handle* H) {
         H->pieces_seq.ensure((size_t)C * 5 + 4);
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
                                        H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, d_cc, s,
-                                       H->cend.p));
+                                       H->cend.p, pcv));
         SPR_HIP(spr::launch_chain_order(H->chains.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, s));
     }
     stage_mark(H, 4);
     if (C) {
         SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong,
-                                        d_work, H->chain_t.p, H->mono.p, H->n_cu, s));
+                                        d_work, H->chain_t.p, H->mono.p, H->n_cu * H->chain_k / 8, s));
     }
     stage_mark(H, 5);
     if (C) {
         SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
         SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->chains.p, H->mono.p, H->chain_t.p, H->key.p, H->wn.p,
-                                        H->inter.p, H->long_list.p, d_nlpieces, H->pieces_seq.p, H->n_cu, s));
+                                        H->inter.p, H->long_list.p, d_nlpieces, H->pieces_seq.p,
+                                        H->n_cu * H->merge_k / 8, s));
         SPR_HIP(spr::launch_or_merge(H->key.p, H->wn.p, nl, H->inter.p, s, d_live));
     }
     stage_mark(H, 6);
     SPR_HIP(spr::launch_gen_fragment(H->inter.p, nl, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
                                      H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, H->hist_l.p, d_err,
-                                     s, d_live));
+                                     s, d_live, pcv));
     stage_mark(H, 7);
     SPR_HIP(spr::launch_scan_i32(H->hist_l.p, H->hist_l.p, nb_budget, ctl_at(H, z.st_hist, z.counters, 2), s,
                                  d_nb, d_nsorted));
@@ -559,14 +593,17 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
     stage_mark(H, 9);
     SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, nl, ctl_at(H, z.st_wind, z.counters, 4), s, d_nsorted));
+    if (H->batch)
+        SPR_HIP(spr::launch_doc_winding_base(H->winding.p, H->sstart.p, H->doc_first.p, Np, H->wbase.p, s));
     stage_mark(H, 10);
     SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, nl, w, h,
                                   ctl_at(H, z.st_mark, z.counters, 5), H->records.p, debug ? H->flags.p : nullptr,
                                   d_totals, H->bucket_base.p, H->prow.p, H->brec_l.p, s, d_nsorted, H->sstart.p,
-                                  Np, nr, d_err));
+                                  Np, nr, d_err, pcv, H->batch ? H->wbase.p : nullptr));
     stage_mark(H, 11);
     SPR_HIP(spr::launch_composite(H->records.p, H->brec_l.p, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
-                                  (int)H->opts.compose, H->image.p, s, nr));
+                                  (int)H->opts.compose, H->image.p, s, nr, H->batch ? H->docs.p : nullptr,
+                                  (int)H->n_docs, H->total_rows));
     stage_mark(H, 12);
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 8, d_err, 4, hipMemcpyDeviceToHost, s));
@@ -574,9 +611,15 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 12, H->arena.p + Np, 4, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 13, d_nsorted, 4, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 14, d_capr, 4, hipMemcpyDeviceToHost, s));
+    if (H->capturing) return;
     SPR_HIP(hipEventRecord(H->ev_end, s));
-    // the end of a learned frame is where a budget overflow is seen
-    SPR_HIP(hipStreamSynchronize(s));
+    learned_finish(H);
+}
+
+// The end of a no-host-sync frame (eager or replayed graph): wait, check the device flags, and
+// keep the exact budgets of this scene for the next frames.
+void learned_finish(spr_handle* H) {
+    SPR_HIP(hipStreamSynchronize(H->stream));
     const int32_t err = H->h_pinned[8];
     const int64_t caps = std::max<int32_t>(H->h_pinned[12], 0), pool = std::max<int32_t>(H->h_pinned[9], 0);
     const int64_t r_need = (int64_t)H->h_pinned[4] + H->h_pinned[5];
@@ -601,6 +644,55 @@ void run_learned_frame(spr_handle* H) {
     H->lb_n = caps + pool + std::max<int64_t>((caps + pool) / 64, 1024);
     H->lb_r = r_need + std::max<int64_t>(r_need / 64, 1024);
     H->lb_dirty = false;
+}
+
+void drop_graph(spr_handle* H) {
+    if (H->gexec) { (void)hipGraphExecDestroy(H->gexec); H->gexec = nullptr; }
+    if (H->graph) { (void)hipGraphDestroy(H->graph); H->graph = nullptr; }
+    H->graph_lb_n = H->graph_lb_r = -1;
+}
+
+// Records the frame's device work (kernels and the final device-to-host copies) into a graph.
+void capture_frame(spr_handle* H) {
+    hipStream_t s = H->stream;
+    drop_graph(H);
+    SPR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    H->capturing = true;
+    try {
+        run_learned_frame(H);
+    } catch (...) {
+        H->capturing = false;
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        throw;
+    }
+    H->capturing = false;
+    SPR_HIP(hipStreamEndCapture(s, &H->graph));
+    SPR_HIP(hipGraphInstantiate(&H->gexec, H->graph, nullptr, nullptr, 0));
+    H->graph_lb_n = H->lb_n;
+    H->graph_lb_r = H->lb_r;
+}
+
+// Frames without a host synchronisation inside (learned mode, batches).  Once the budgets of
+// the current scene, MVP and model are exact and stable, the frame is captured once and later
+// frames replay the graph: one launch per frame.
+void render_nosync(spr_handle* H) {
+    hipStream_t s = H->stream;
+    if (!H->lb_dirty && H->gexec && H->graph_lb_n == H->lb_n && H->graph_lb_r == H->lb_r && !H->stage_timing) {
+        SPR_HIP(hipEventRecord(H->ev_begin, s));
+        SPR_HIP(hipGraphLaunch(H->gexec, s));
+        SPR_HIP(hipEventRecord(H->ev_end, s));
+        H->n_marks = 0;
+        learned_finish(H);
+        return;
+    }
+    drop_graph(H);
+    const bool was_dirty = H->lb_dirty;
+    const int64_t n0 = H->lb_n, r0 = H->lb_r;
+    run_learned_frame(H);
+    if (!was_dirty && !H->lb_dirty && H->lb_n == n0 && H->lb_r == r0 && !H->stage_timing) capture_frame(H);
 }
 
 template <class F>
@@ -648,6 +740,8 @@ void sync_stats(spr_handle* H) {
     }
 }
 
+void drop_graph(spr_handle* H);
+
 void load_arrays(spr_handle* H, const float* pos, const uint32_t* pos_path, uint32_t P, const uint32_t* cpos,
                  const uint32_t* ctype, const uint32_t* cpath, uint32_t C, const uint32_t* rule,
                  const uint32_t* fill, uint32_t Np) {
@@ -692,6 +786,8 @@ void load_arrays(spr_handle* H, const float* pos, const uint32_t* pos_path, uint
     H->inj = false;
     H->inj_inter.clear();
     H->lb_dirty = true;
+    H->batch = false;
+    drop_graph(H);
 }
 
 }  // namespace
@@ -752,10 +848,12 @@ int spr_destroy(spr_handle* H) {
     H->winding.release(); H->flags.release(); H->records.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
     H->band_start.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
+    drop_graph(H);
+    H->pcanvas.release(); H->docs.release(); H->doc_first.release(); H->wbase.release();
     H->mlpdev.release();
     H->pcurve.release(); H->feat.release(); H->pred.release(); H->train_feat.release(); H->train_tgt.release();
     H->arena.release(); H->capn.release(); H->nexact.release(); H->pbase.release(); H->sstart.release();
-    H->hist_l.release(); H->fill_l.release(); H->brec_l.release(); H->cend.release();
+    H->hist_l.release(); H->fill_l.release(); H->brec_l.release(); H->cend.release(); H->pbound.release();
     if (H->h_pinned) (void)hipHostFree(H->h_pinned);
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
@@ -809,8 +907,13 @@ int spr_render_frame(spr_handle* H, uint8_t* rgba_out) {
     if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
         SPR_HIP(hipSetDevice(H->device));
+        if (H->batch) {
+            if (rgba_out) throw Fail{SPR_ERR_INVALID, "a batch has one image per document: use spr_batch_image"};
+            render_nosync(H);
+            return;
+        }
         if (H->opts.mode == SPR_MODE_LEARNED && !H->inj)
-            run_learned_frame(H);
+            render_nosync(H);
         else
             run_frame(H);
         if (rgba_out) {
@@ -958,6 +1061,129 @@ int spr_debug_inject(spr_handle* H, int stage, const void* src, size_t nbytes) {
         }
         H->inj_inter.assign(v, v + 2 * n);
         H->inj = true;
+    });
+}
+
+// ---- batches of documents
+int spr_load_batch(spr_handle* H, const spr_scene* const* docs, const uint32_t* widths, const uint32_t* heights,
+                   uint32_t n_docs) {
+    if (!H || !docs || !widths || !heights) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (n_docs == 0) throw Fail{SPR_ERR_INVALID, "empty batch"};
+        std::vector<float> pos;
+        std::vector<uint32_t> pos_path, cpos, ctype, cpath, rule, fill;
+        std::vector<spr::DocDev> dd(n_docs);
+        std::vector<int2> pcv;
+        std::vector<int32_t> first;
+        size_t img = 0;
+        int rows = 0, max_h = 1;
+        for (uint32_t d = 0; d < n_docs; ++d) {
+            if (!docs[d]) throw Fail{SPR_ERR_INVALID, "null document"};
+            const uint32_t w = widths[d], h = heights[d];
+            if (w == 0 || h == 0 || w > 8192 || h > 16384) throw Fail{SPR_ERR_CAPACITY, "canvas must be 1..8192 x 1..16384"};
+            const spr::Scene& S = docs[d]->s;
+            const uint32_t p_off = (uint32_t)rule.size(), pt_off = (uint32_t)pos_path.size();
+            pos.insert(pos.end(), S.pos.begin(), S.pos.end());
+            for (uint32_t v : S.pos_path) pos_path.push_back(v + p_off);
+            for (uint32_t v : S.curve_pos) cpos.push_back(v + pt_off);
+            ctype.insert(ctype.end(), S.curve_type.begin(), S.curve_type.end());
+            for (uint32_t v : S.curve_path) cpath.push_back(v + p_off);
+            rule.insert(rule.end(), S.rule.begin(), S.rule.end());
+            fill.insert(fill.end(), S.fill.begin(), S.fill.end());
+            dd[d].p0 = (int32_t)p_off;
+            dd[d].p1 = (int32_t)rule.size();
+            dd[d].w = (int32_t)w;
+            dd[d].h = (int32_t)h;
+            dd[d].img_off = (int64_t)img;
+            dd[d].row0 = rows;
+            dd[d].pad = 0;
+            img += (size_t)4 * w * h;
+            rows += (int)(h + 1) / 2;
+            max_h = std::max<int>(max_h, (int)h);
+            for (uint32_t p = p_off; p < rule.size(); ++p) { pcv.push_back(make_int2((int)w, (int)h)); first.push_back((int32_t)p_off); }
+        }
+        if (rule.size() > (1u << 30) || pos_path.size() > (1u << 31)) throw Fail{SPR_ERR_CAPACITY, "batch too large"};
+        const uint32_t Np = (uint32_t)rule.size();
+        load_arrays(H, pos.data(), pos_path.data(), (uint32_t)pos_path.size(), cpos.data(), ctype.data(), cpath.data(),
+                    (uint32_t)ctype.size(), rule.data(), fill.data(), Np);
+        first.push_back(0);
+        pcv.push_back(make_int2(1, 1));
+        H->pcanvas.ensure(pcv.size());
+        H->doc_first.ensure(first.size());
+        H->docs.ensure(n_docs);
+        H->wbase.ensure((size_t)Np + 1);
+        SPR_HIP(hipMemcpyAsync(H->pcanvas.p, pcv.data(), pcv.size() * sizeof(int2), hipMemcpyHostToDevice, H->stream));
+        SPR_HIP(hipMemcpyAsync(H->doc_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, H->stream));
+        SPR_HIP(hipMemcpyAsync(H->docs.p, dd.data(), dd.size() * sizeof(spr::DocDev), hipMemcpyHostToDevice, H->stream));
+        SPR_HIP(hipStreamSynchronize(H->stream));
+        H->batch = true;
+        H->n_docs = n_docs;
+        H->h_docs = dd;
+        H->total_rows = rows;
+        H->max_h = max_h;
+        H->img_bytes = img;
+        H->lb_dirty = true;
+    });
+}
+
+int spr_batch_image(spr_handle* H, uint32_t doc, uint8_t* rgba_out, size_t cap) {
+    if (!H || !rgba_out) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (!H->batch) throw Fail{SPR_ERR_STATE, "no batch loaded"};
+        if (doc >= H->n_docs) throw Fail{SPR_ERR_INVALID, "document index out of range"};
+        if (!H->frame_done) throw Fail{SPR_ERR_STATE, "no frame rendered"};
+        const spr::DocDev& d = H->h_docs[doc];
+        const size_t bytes = (size_t)4 * d.w * d.h;
+        if (cap < bytes) throw Fail{SPR_ERR_CAPACITY, "destination too small"};
+        SPR_HIP(hipMemcpyAsync(rgba_out, H->image.p + d.img_off, bytes, hipMemcpyDeviceToHost, H->stream));
+        SPR_HIP(hipStreamSynchronize(H->stream));
+    });
+}
+
+int spr_autotune(spr_handle* H, uint32_t reps, spr_tune_report* out) {
+    if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (!H->loaded) throw Fail{SPR_ERR_STATE, "autotune before load"};
+        if (!H->batch && !(H->opts.mode == SPR_MODE_LEARNED && H->model.ready))
+            throw Fail{SPR_ERR_STATE, "autotune times captured frames: a batch, or learned mode with a model"};
+        if (reps == 0) reps = 3;
+        const bool timing = H->stage_timing;
+        H->stage_timing = false;
+        const int ks[3] = {4, 8, 16};
+        double best = 1e30, dflt = 0.0;
+        int best_c = H->chain_k, best_m = H->merge_k, tried = 0;
+        for (int ci = 0; ci < 3; ++ci)
+            for (int mi = 0; mi < 3; ++mi) {
+                H->chain_k = ks[ci];
+                H->merge_k = ks[mi];
+                drop_graph(H);
+                render_nosync(H);  // eager; sizes the budgets when needed
+                render_nosync(H);  // eager, then captured
+                if (!H->gexec) render_nosync(H);
+                double acc = 0.0;
+                for (uint32_t r = 0; r < reps; ++r) {
+                    render_nosync(H);
+                    float ms = 0.0f;
+                    SPR_HIP(hipEventElapsedTime(&ms, H->ev_begin, H->ev_end));
+                    acc += ms;
+                }
+                const double avg = acc / reps;
+                if (ks[ci] == 8 && ks[mi] == 8) dflt = avg;
+                if (avg < best) { best = avg; best_c = ks[ci]; best_m = ks[mi]; }
+                ++tried;
+            }
+        H->chain_k = best_c;
+        H->merge_k = best_m;
+        drop_graph(H);
+        H->stage_timing = timing;
+        if (out) {
+            out->chain_blocks_per_cu = best_c;
+            out->merge_blocks_per_cu = best_m;
+            out->n_tried = tried;
+            out->reserved = 0;
+            out->default_ms = dflt;
+            out->best_ms = best;
+        }
     });
 }
 
