@@ -85,7 +85,7 @@ enum {
     SPR_STAGE_SIDX = 9,        /* int32 x N: sorted intersection indices */
     SPR_STAGE_WINDING = 10,    /* int32 x N: exclusive-scanned winding numbers, sorted order */
     SPR_STAGE_FLAGS = 11,      /* int32 x 2N: cell flags, then span flags */
-    SPR_STAGE_RECORDS = 12,    /* int4 x R: (yx, width, fill, frag_index) output records */
+    SPR_STAGE_RECORDS = 12,    /* int4 x R: (yx, width, fill, frag_index) output records (debug) */
     SPR_STAGE_IMAGE = 13,      /* uint8 x 4WH: RGBA8 image, top row first */
     SPR_STAGE_FEATURES = 14,   /* float x 16Np: per-path predictor features (last learned frame) */
     SPR_STAGE_PRED = 15,       /* float x 2Np: predicted log2(1 + intersections), log2(1 + records) */

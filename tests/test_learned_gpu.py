@@ -89,7 +89,6 @@ def test_learned_frame_identical_to_plain(spr, trained):
     scene = spr.Scene.generate(2, 3, 400, 1024)
     p, ip, il = _plain_and_learned(spr, r, scene, 1024, spr.COMPOSE_SRC_OVER)
     assert np.array_equal(ip, il)
-    assert np.array_equal(p.export("records"), r.export("records"))
     sp, sl = p.stats(), r.stats()
     assert sp.n_intersections == sl.n_intersections and sp.n_cells == sl.n_cells and sp.n_spans == sl.n_spans
     assert sl.alloc_intersections > 0
@@ -97,6 +96,17 @@ def test_learned_frame_identical_to_plain(spr, trained):
     il2 = r.render()
     assert np.array_equal(ip, il2)
     p.close()
+    # full records (debug handles): identical too
+    pd = spr.Rasterizer(1024, 1024, compose=spr.COMPOSE_SRC_OVER, debug=True)
+    qd = spr.Rasterizer(1024, 1024, compose=spr.COMPOSE_SRC_OVER, mode=spr.MODE_LEARNED, debug=True)
+    qd.mlp_set_params(r.mlp_params())
+    pd.load_scene(scene)
+    qd.load_scene(scene)
+    pd.render(readback=False)
+    qd.render(readback=False)
+    assert np.array_equal(pd.export("records"), qd.export("records"))
+    pd.close()
+    qd.close()
 
 
 def test_forced_overflow_and_generous_arenas(spr, trained):

@@ -26,7 +26,7 @@ constexpr int kBatch = 64;                 // paths per coverage mask
 constexpr int kUnroll = 4;                 // records loaded per thread before applying
 
 template <int COMPOSE>
-__global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ records,
+__global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict__ records,
                                                       const int2* __restrict__ brec,
                                                       const int32_t* __restrict__ bucket_base,
                                                       const int2* __restrict__ prow,
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
                 __syncthreads();
                 const int total = bpre[qn];
                 for (int f0 = tid; f0 < total; f0 += kBlock * kUnroll) {
-                    int4 rec[kUnroll];
+                    uint32_t rec[kUnroll];
                     int qi[kUnroll];
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u) {
@@ -142,8 +142,8 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
                     for (int u = 0; u < kUnroll; ++u) {
                         if (qi[u] < 0) continue;
                         const unsigned long long bit = 1ull << qi[u];
-                        int xs = (rec[u].x & 0xFFFF) - c0;
-                        int xe = xs + rec[u].y;
+                        int xs = (int)(rec[u] & 0xFFFFu) - c0;   // compact record: x | width << 16
+                        int xe = xs + (int)(rec[u] >> 16);
                         xs = xs < 0 ? 0 : xs;
                         xe = xe > cw ? cw : xe;
                         for (int x = xs; x < xe; ++x) atomicOr(&mask[x], bit);
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const int4* __restrict__ r
 
 }  // namespace
 
-hipError_t launch_composite(const int4* records, const int2* brec, const int32_t* bucket_base,
+hipError_t launch_composite(const uint32_t* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
                             uint8_t* image, hipStream_t s, int64_t rec_cap, const DocDev* docs, int n_docs,
                             int total_rows) {

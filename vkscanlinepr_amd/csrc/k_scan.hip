@@ -18,7 +18,9 @@ namespace spr {
 namespace {
 
 // n_dev (optional) holds the element count on the device (the launch covers n_max); total_out
-// (optional) also receives the total.
+// (optional) also receives the total.  ITEMS consecutive elements per thread: large scans use
+// 32 (8192-element tiles), which halves the tiles that take a ticket from the one counter.
+template <int ITEMS>
 __global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, int32_t* out, int64_t n_max,
                                                            const int32_t* __restrict__ n_dev,
                                                            int32_t* __restrict__ total_out,
@@ -29,26 +31,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, in
     __shared__ int32_t s_prefix;
     const int64_t n = n_dev ? min((int64_t)*n_dev, n_max) : n_max;
     const int64_t tile = ordered_tile_id(counter, &s_tile);
-    const int64_t base = tile * (int64_t)kScanTile;
+    const int64_t base = tile * (int64_t)(ITEMS * kScanThreads);
     if (base >= n && tile != 0) return;
-    const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
-    int32_t v[kScanItems];
-    const bool vec = (i0 + kScanItems <= n) && ((((uintptr_t)(in + i0)) & 15u) == 0) &&
+    const int64_t i0 = base + (int64_t)threadIdx.x * ITEMS;
+    int32_t v[ITEMS];
+    const bool vec = (i0 + ITEMS <= n) && ((((uintptr_t)(in + i0)) & 15u) == 0) &&
                      ((((uintptr_t)(out + i0)) & 15u) == 0);
     if (vec) {
         const int4* p = reinterpret_cast<const int4*>(in + i0);
 #pragma unroll
-        for (int k = 0; k < kScanItems / 4; ++k) {
+        for (int k = 0; k < ITEMS / 4; ++k) {
             int4 q = p[k];
             v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j) v[j] = (i0 + j < n) ? in[i0 + j] : 0;
+        for (int j = 0; j < ITEMS; ++j) v[j] = (i0 + j < n) ? in[i0 + j] : 0;
     }
     int32_t run = 0;
 #pragma unroll
-    for (int j = 0; j < kScanItems; ++j) { int32_t x = v[j]; v[j] = run; run = (int32_t)((uint32_t)run + (uint32_t)x); }
+    for (int j = 0; j < ITEMS; ++j) { int32_t x = v[j]; v[j] = run; run = (int32_t)((uint32_t)run + (uint32_t)x); }
     int32_t total;
     const int32_t texcl = block_exclusive_add(run, s_w, total);
     if (threadIdx.x < 64) {
@@ -60,14 +62,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, in
     if (vec) {
         int4* p = reinterpret_cast<int4*>(out + i0);
 #pragma unroll
-        for (int k = 0; k < kScanItems / 4; ++k)
+        for (int k = 0; k < ITEMS / 4; ++k)
             p[k] = make_int4(off + v[4 * k], off + v[4 * k + 1], off + v[4 * k + 2], off + v[4 * k + 3]);
     } else {
 #pragma unroll
-        for (int j = 0; j < kScanItems; ++j)
+        for (int j = 0; j < ITEMS; ++j)
             if (i0 + j < n) out[i0 + j] = (int32_t)((uint32_t)off + (uint32_t)v[j]);
     }
-    const bool last_tile = (n == 0) ? (tile == 0) : (base <= n - 1 && n - 1 < base + kScanTile);
+    const bool last_tile = (n == 0) ? (tile == 0) : (base <= n - 1 && n - 1 < base + (ITEMS * kScanThreads));
     if (last_tile && threadIdx.x == 0) {
         out[n] = (int32_t)((uint32_t)s_prefix + (uint32_t)total);
         if (total_out) *total_out = out[n];
@@ -129,12 +131,14 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     const uint32_t* __restrict__ fill, int64_t N_max, const int32_t* __restrict__ n_dev,
     const int32_t* __restrict__ sstart, uint32_t Np, int64_t rec_cap, int w, int h,
     unsigned long long* __restrict__ status, uint32_t* __restrict__ counter,
-    int4* __restrict__ records, int32_t* __restrict__ flags, int32_t* __restrict__ totals,
+    int4* __restrict__ records, uint32_t* __restrict__ crec, int32_t* __restrict__ flags,
+    int32_t* __restrict__ totals,
     const int32_t* __restrict__ bucket_base, const int2* __restrict__ prow, int2* __restrict__ brec,
     int32_t* __restrict__ err, const int2* __restrict__ pcanvas, const int32_t* __restrict__ wbase) {
     __shared__ int32_t s_tile;
     __shared__ unsigned long long s_w[kWaves];
     __shared__ unsigned long long s_prefix;
+    __shared__ uint32_t s_rec[2 * kScanTile];  // the tile's compact records, stored coalesced
     const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
     const int64_t tile = ordered_tile_id(counter, &s_tile);
     const int64_t base = tile * (int64_t)kScanTile;
@@ -188,6 +192,8 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     const unsigned long long off = s_prefix + texcl;
     int32_t cells_before = (int32_t)(off & 0xFFFFFFFFull);
     int32_t spans_before = (int32_t)(off >> 32);
+    const int64_t tile_rec0 = (int64_t)(s_prefix & 0xFFFFFFFFull) + (int64_t)(s_prefix >> 32);
+    const int64_t tile_recs = (int64_t)(total & 0xFFFFFFFFull) + (int64_t)(total >> 32);
     if (i0 < N) {
         int32_t kp = (i0 > 0) ? skey[i0 - 1] : 0;
         uint32_t pp = (i0 > 0) ? pprev_of(epath, sstart, Np, i0, paths[0], keys[0]) : 0xFFFFFFFFu;
@@ -203,20 +209,27 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
             const uint32_t p1 = paths[j];
             if (flags) { flags[f] = cell; flags[N + f] = span; }
             if ((cell | span) && (int64_t)(cells_before + spans_before + cell + span) > rec_cap) {
-                atomicOr(err, 2);  // record capacity exceeded: the frame is redone by the host
+                atomicOr(err, 2);  // record capacity exceeded
             } else if (cell | span) {
-                const int32_t colour = (int32_t)fill[p1];
                 const int32_t o = cells_before + spans_before;
+                const int loc = (int)(o - tile_rec0);
+                // compact record for the composite: x | width << 16 (row and colour are implied by
+                // the (path, row) record range); the full int4 record only for debug export
                 if (cell) {
-                    const int x1 = key_x(k1), y1 = key_y(k1);
-                    records[o] = make_int4((int32_t)((uint32_t)y1 << 16) | x1, 2, colour, cells_before + 1);
+                    const int x1 = key_x(k1);
+                    s_rec[loc] = (uint32_t)x1 | (2u << 16);
+                    if (records)
+                        records[o] = make_int4((int32_t)((uint32_t)key_y(k1) << 16) | x1, 2, (int32_t)fill[p1],
+                                               cells_before + 1);
                 }
                 if (span) {
-                    const int y0 = key_y(kp);
                     int x0 = key_x(kp) + 2;
                     const int x1 = key_x(k1);
                     x0 = max(0, x0);
-                    records[o + cell] = make_int4((int32_t)((uint32_t)y0 << 16) | x0, x1 - x0, colour, 0);
+                    s_rec[loc + cell] = (uint32_t)x0 | ((uint32_t)(x1 - x0) << 16);
+                    if (records)
+                        records[o + cell] = make_int4((int32_t)((uint32_t)key_y(kp) << 16) | x0, x1 - x0,
+                                                      (int32_t)fill[p1], 0);
                 }
             }
             // record range of this element's (path, cell row) bucket, for the composite
@@ -245,6 +258,9 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
             pp = p1;
         }
     }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < tile_recs; i += kScanThreads)
+        if (tile_rec0 + i < rec_cap) crec[tile_rec0 + i] = s_rec[i];
     if (threadIdx.x == 0 && base <= N - 1 && N - 1 < base + kScanTile) {
         const unsigned long long t = s_prefix + total;
         totals[0] = (int32_t)(t & 0xFFFFFFFFull);
@@ -257,21 +273,29 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
 
 hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s,
                            const int32_t* n_dev, int32_t* total_out) {
-    const int64_t tiles = n > 0 ? scan_tiles(n) : 1;
-    k_scan_i32<<<(unsigned)tiles, kScanThreads, 0, s>>>(in, out, n, n_dev, total_out, ctl.status, ctl.counter);
+    // the status array holds scan_tiles(n) (4096-element) entries, enough for either tile size
+    if (n >= ((int64_t)1 << 20)) {
+        const int64_t tiles = (n + 32 * kScanThreads - 1) / (32 * kScanThreads);
+        k_scan_i32<32><<<(unsigned)tiles, kScanThreads, 0, s>>>(in, out, n, n_dev, total_out, ctl.status,
+                                                                ctl.counter);
+    } else {
+        const int64_t tiles = n > 0 ? scan_tiles(n) : 1;
+        k_scan_i32<16><<<(unsigned)tiles, kScanThreads, 0, s>>>(in, out, n, n_dev, total_out, ctl.status,
+                                                                ctl.counter);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,
                             const uint32_t* rule, const uint32_t* fill, int64_t N, int w, int h,
-                            ScanCtl ctl, int4* records, int32_t* flags, int32_t* totals,
+                            ScanCtl ctl, int4* records, uint32_t* crec, int32_t* flags, int32_t* totals,
                             const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s,
                             const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
                             int32_t* err, const int2* pcanvas, const int32_t* wbase) {
     if (N == 0) return hipSuccess;
     k_mark_emit<<<(unsigned)scan_tiles(N), kScanThreads, 0, s>>>(skey, winding, epath, rule, fill, N, n_dev,
                                                                  sstart, Np, rec_cap, w, h, ctl.status,
-                                                                 ctl.counter, records, flags, totals,
+                                                                 ctl.counter, records, crec, flags, totals,
                                                                  bucket_base, prow, brec, err, pcanvas, wbase);
     return hipGetLastError();
 }

@@ -62,13 +62,27 @@ __device__ inline int64_t merge_split(PA A, int64_t na, PB B, int64_t nb, int64_
     return lo;
 }
 
+// Sorted output, unpacked from the sort words: key, winding change and (optionally) index.
+struct Unpacked {
+    int32_t* skey;
+    int32_t* sidx;
+    int32_t* swn;
+};
+__device__ inline void put(const Unpacked& o, int64_t i, u64 w) {
+    o.skey[i] = (int32_t)((uint32_t)(w >> 32) ^ 0x80000000u);
+    if (o.sidx) o.sidx[i] = (int32_t)((w >> 2) & 0x3FFFFFFFull);
+    o.swn[i] = (int32_t)(w & 3ull) - 1;
+}
+
 // LDS slot of element i: one pad word per 8 elements, so the 8-element runs owned by
 // consecutive lanes start on different banks.
 constexpr int kPadded = kSortTile + kSortTile / 8;
 __device__ inline int pad(int i) { return i + (i >> 3); }
 
-// Sorts n <= kSortTile words from src into dst with one workgroup (buf: 2 x kPadded words of LDS).
-__device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ dst, int n, u64 (*buf)[kPadded]) {
+// Sorts n <= kSortTile words from src with one workgroup (buf: 2 x kPadded words of LDS) into dst
+// (sort words) or, when dst is null, unpacked into out at offset o0.
+__device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ dst, int n, u64 (*buf)[kPadded],
+                                 const Unpacked& out, int64_t o0) {
     int P2 = kItems;
     while (P2 < n) P2 <<= 1;
     const int active = P2 / kItems;
@@ -112,13 +126,17 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
         __syncthreads();
         cur ^= 1;
     }
-    for (int i = tid; i < n; i += kThreads) dst[i] = buf[cur][pad(i)];
+    if (dst) {
+        for (int i = tid; i < n; i += kThreads) dst[i] = buf[cur][pad(i)];
+    } else {
+        for (int i = tid; i < n; i += kThreads) put(out, o0 + i, buf[cur][pad(i)]);
+    }
     __syncthreads();
 }
 
 __global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restrict__ band_start,
                                                          const int32_t* __restrict__ n_bands,
-                                                         const u64* __restrict__ src, u64* __restrict__ dst,
+                                                         const u64* __restrict__ src, Unpacked out,
                                                          int32_t* __restrict__ big_list,
                                                          int32_t* __restrict__ big_count) {
     __shared__ u64 buf[2][kPadded];
@@ -132,10 +150,10 @@ __global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restri
         return;
     }
     if (n == 1) {
-        if (threadIdx.x == 0) dst[s] = src[s];
+        if (threadIdx.x == 0) put(out, s, src[s]);
         return;
     }
-    tile_sort(src + s, dst + s, (int)n, buf);
+    tile_sort(src + s, nullptr, (int)n, buf, out, s);
 }
 
 // One workgroup per oversized band: LDS chunk sorts into `a`, then merge-path passes between
@@ -144,7 +162,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
                                                              const int32_t* __restrict__ big_list,
                                                              const int32_t* __restrict__ big_count,
                                                              u64* __restrict__ scatter, u64* __restrict__ a,
-                                                             u64* __restrict__ dst) {
+                                                             Unpacked out) {
     __shared__ u64 buf[2][kPadded];
     const int32_t nbig = *big_count;
     for (int32_t q = blockIdx.x; q < nbig; q += gridDim.x) {
@@ -152,14 +170,15 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
         const int64_t s = band_start[band], n = band_start[band + 1] - s;
         for (int64_t c = 0; c < n; c += kSortTile) {
             const int m = (int)(n - c < kSortTile ? n - c : kSortTile);
-            tile_sort(scatter + s + c, a + s + c, m, buf);
+            tile_sort(scatter + s + c, a + s + c, m, buf, out, 0);
         }
         __threadfence_block();
         __syncthreads();
         u64* cur = a + s;
         u64* nxt = scatter + s;
         for (int64_t r = kSortTile; r < n; r <<= 1) {
-            u64* out = (2 * r >= n) ? dst + s : nxt;
+            const bool last_pass = 2 * r >= n;
+            u64* dstw = last_pass ? nullptr : nxt;
             for (int64_t o = (int64_t)threadIdx.x * kItems; o < n; o += (int64_t)kThreads * kItems) {
                 const int64_t pb = (o / (2 * r)) * (2 * r);
                 const int64_t na = pb + r < n ? r : n - pb;
@@ -172,17 +191,18 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
                 int64_t j = diag - i;
                 for (int k = 0; k < cnt; ++k) {
                     const bool takeA = j >= nb || (i < na && A[i] <= B[j]);
-                    out[o + k] = takeA ? A[i] : B[j];
+                    const u64 v = takeA ? A[i] : B[j];
+                    if (dstw) dstw[o + k] = v; else put(out, s + o + k, v);
                     i += takeA; j += !takeA;
                 }
             }
             __threadfence_block();
             __syncthreads();
-            if (2 * r >= n) { cur = nullptr; break; }
+            if (last_pass) { cur = nullptr; break; }
             u64* t = cur; cur = nxt; nxt = t;
         }
         if (cur) {  // a single chunk: no merge pass ran
-            for (int64_t i = threadIdx.x; i < n; i += kThreads) dst[s + i] = cur[i];
+            for (int64_t i = threadIdx.x; i < n; i += kThreads) put(out, s + i, cur[i]);
             __syncthreads();
         }
     }
@@ -203,17 +223,19 @@ __global__ void k_unpack_sorted(const u64* __restrict__ sorted, int64_t N_max, i
 }  // namespace
 
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
-                             const unsigned long long* src, unsigned long long* dst, int32_t* big_list,
-                             int32_t* big_count, hipStream_t s) {
+                             const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* swn,
+                             int32_t* big_list, int32_t* big_count, hipStream_t s) {
     if (max_bands <= 0) return hipSuccess;
-    k_sort_bands<<<(unsigned)max_bands, kThreads, 0, s>>>(band_start, n_bands, src, dst, big_list, big_count);
+    k_sort_bands<<<(unsigned)max_bands, kThreads, 0, s>>>(band_start, n_bands, src, Unpacked{skey, sidx, swn},
+                                                           big_list, big_count);
     return hipGetLastError();
 }
 
 hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
-                                 unsigned long long* scatter, unsigned long long* tmp, unsigned long long* dst,
-                                 hipStream_t s) {
-    k_sort_big_bands<<<64, kThreads, 0, s>>>(band_start, big_list, big_count, scatter, tmp, dst);
+                                 unsigned long long* scatter, unsigned long long* tmp, int32_t* skey, int32_t* sidx,
+                                 int32_t* swn, hipStream_t s) {
+    k_sort_big_bands<<<64, kThreads, 0, s>>>(band_start, big_list, big_count, scatter, tmp,
+                                             Unpacked{skey, sidx, swn});
     return hipGetLastError();
 }
 

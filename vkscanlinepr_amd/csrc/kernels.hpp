@@ -89,19 +89,20 @@ hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int3
 hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,
                             int32_t* band_start, hipStream_t s, const int32_t* nb_dev = nullptr,
                             const int32_t* n_dev = nullptr);
+// The band sorts write the sorted keys, winding changes and (optionally) indices directly.
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
-                             const unsigned long long* src, unsigned long long* dst, int32_t* big_list,
-                             int32_t* big_count, hipStream_t s);
+                             const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* swn,
+                             int32_t* big_list, int32_t* big_count, hipStream_t s);
 hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
-                                 unsigned long long* scatter, unsigned long long* tmp, unsigned long long* dst,
-                                 hipStream_t s);
+                                 unsigned long long* scatter, unsigned long long* tmp, int32_t* skey, int32_t* sidx,
+                                 int32_t* swn, hipStream_t s);
 hipError_t launch_unpack_sorted(const unsigned long long* sorted, int64_t N, int32_t* skey,
                                 int32_t* sidx, int32_t* swn, hipStream_t s, const int32_t* n_dev = nullptr);
 
 // ---- winding / mark / emit (a10 scan, a11); also writes each (path, row) bucket's record range
 hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,
                             const uint32_t* rule, const uint32_t* fill, int64_t N, int w, int h,
-                            ScanCtl ctl, int4* records, int32_t* flags, int32_t* totals,
+                            ScanCtl ctl, int4* records, uint32_t* crec, int32_t* flags, int32_t* totals,
                             const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s,
                             const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
                             int32_t* err, const int2* pcanvas = nullptr, const int32_t* wbase = nullptr);
@@ -164,7 +165,7 @@ hipError_t launch_doc_winding_base(const int32_t* winding, const int32_t* sstart
                                    uint32_t Np, int32_t* wbase, hipStream_t s);
 
 // ---- composite (a12)
-hipError_t launch_composite(const int4* records, const int2* brec, const int32_t* bucket_base,
+hipError_t launch_composite(const uint32_t* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
                             uint8_t* image, hipStream_t s, int64_t rec_cap, const DocDev* docs = nullptr,
                             int n_docs = 0, int total_rows = 0);

@@ -112,7 +112,8 @@ struct spr_handle {
     DBuf<uint32_t> epath;
     DBuf<unsigned long long> sorted, sbig, stmp;
     DBuf<int32_t> skey, sidx, swn, winding, flags;
-    DBuf<int4> records;
+    DBuf<int4> records;     // full records, kept only with the debug option
+    DBuf<uint32_t> crec;    // compact records (x | width << 16) read by the composite
     DBuf<int32_t> row_lo, row_hi, bucket_base, bflag, band_start, big_list;
     DBuf<int2> prow;
     DBuf<uint8_t> image;
@@ -283,7 +284,6 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->key.ensure(n1);
     H->wn.ensure(n1);
     H->epath.ensure(n1);
-    H->sorted.ensure(n1);
     H->sbig.ensure(n1);
     H->stmp.ensure(n1);
     H->skey.ensure(n1);
@@ -292,8 +292,8 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->bflag.ensure(nb1 + 1);
     H->band_start.ensure((size_t)max_bands + 1);
     H->big_list.ensure((size_t)max_bands);
-    if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); }
-    H->records.ensure(2 * n1);
+    if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
+    H->crec.ensure(2 * n1);
     H->image.ensure((size_t)4 * H->w * H->h);
     SPR_HIP(spr::launch_fill_u32((uint32_t*)(H->zero.p + z.a_words), 2 * (int64_t)(z.total - z.a_words), 0u, s));
     int32_t* d_totals = reinterpret_cast<int32_t*>(H->zero.p + z.totals);
@@ -352,26 +352,25 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     }
     stage_mark(H, 8);
     if (N > 0) {
-        SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->sorted.p,
-                                       H->big_list.p, d_bigcount, s));
+        SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
+                                       debug ? H->sidx.p : nullptr, H->swn.p, H->big_list.p, d_bigcount, s));
         SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->sbig.p, H->stmp.p,
-                                           H->sorted.p, s));
-        SPR_HIP(spr::launch_unpack_sorted(H->sorted.p, N, H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s));
+                                           H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s));
     }
     stage_mark(H, 9);
     // ---- winding numbers (global exclusive scan, scanline_rasterizer.cpp:478-506)
     SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, N, ctl_at(H, z.st_wind, z.ctr, 2), s));
     stage_mark(H, 10);
     SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
-                                  ctl_at(H, z.st_mark, z.ctr, 3), H->records.p, debug ? H->flags.p : nullptr,
-                                  d_totals, H->bucket_base.p, H->prow.p, d_brec, s, nullptr, nullptr, Np,
-                                  (int64_t)H->records.cap, d_err));
+                                  ctl_at(H, z.st_mark, z.ctr, 3), debug ? H->records.p : nullptr, H->crec.p,
+                                  debug ? H->flags.p : nullptr, d_totals, H->bucket_base.p, H->prow.p, d_brec, s,
+                                  nullptr, nullptr, Np, (int64_t)(2 * n1), d_err));
     H->last_brec = d_brec;
     H->last_learned = false;
     stage_mark(H, 11);
     // ---- composite
-    SPR_HIP(spr::launch_composite(H->records.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
-                                  (int)H->opts.compose, H->image.p, s, (int64_t)H->records.cap));
+    SPR_HIP(spr::launch_composite(H->crec.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
+                                  (int)H->opts.compose, H->image.p, s, (int64_t)(2 * n1)));
     stage_mark(H, 12);
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 8, d_err, 4, hipMemcpyDeviceToHost, s));
@@ -514,13 +513,13 @@ void run_learned_frame(spr_handle* H) {
     const int64_t nl = std::max<int64_t>(H->lb_n, 1), nr = std::max<int64_t>(H->lb_r, 1);
     const int64_t max_bands = (int64_t)Np + 3 * (nl / 1024) + 2;
     H->inter.ensure(nl); H->chain_t.ensure(nl); H->key.ensure(nl); H->wn.ensure(nl); H->epath.ensure(nl);
-    H->sorted.ensure(nl); H->sbig.ensure(nl); H->stmp.ensure(nl); H->skey.ensure(nl); H->swn.ensure(nl);
+    H->sbig.ensure(nl); H->stmp.ensure(nl); H->skey.ensure(nl); H->swn.ensure(nl);
     H->winding.ensure(nl + 1);
     H->bflag.ensure((size_t)nb_budget + 1);
     H->band_start.ensure((size_t)max_bands + 1);
     H->big_list.ensure((size_t)max_bands);
-    if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); }
-    H->records.ensure(nr);
+    if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); H->records.ensure(nr); }
+    H->crec.ensure(nr);
     H->image.ensure(H->batch ? H->img_bytes : (size_t)4 * H->w * H->h);
     H->hist_l.ensure((size_t)nb_budget + 1);
     H->fill_l.ensure((size_t)nb_budget + 1);
@@ -584,12 +583,10 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_band_emit(H->bflag.p, H->hist_l.p, (int32_t)nb_budget, nl, H->band_start.p, s, d_nb,
                                   d_nsorted));
     stage_mark(H, 8);
-    SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->sorted.p, H->big_list.p,
-                                   d_bigcount, s));
+    SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
+                                   debug ? H->sidx.p : nullptr, H->swn.p, H->big_list.p, d_bigcount, s));
     SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->sbig.p, H->stmp.p,
-                                       H->sorted.p, s));
-    SPR_HIP(spr::launch_unpack_sorted(H->sorted.p, nl, H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s,
-                                      d_nsorted));
+                                       H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s));
     SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
     stage_mark(H, 9);
     SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, nl, ctl_at(H, z.st_wind, z.counters, 4), s, d_nsorted));
@@ -597,11 +594,12 @@ void run_learned_frame(spr_handle* H) {
         SPR_HIP(spr::launch_doc_winding_base(H->winding.p, H->sstart.p, H->doc_first.p, Np, H->wbase.p, s));
     stage_mark(H, 10);
     SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, nl, w, h,
-                                  ctl_at(H, z.st_mark, z.counters, 5), H->records.p, debug ? H->flags.p : nullptr,
-                                  d_totals, H->bucket_base.p, H->prow.p, H->brec_l.p, s, d_nsorted, H->sstart.p,
+                                  ctl_at(H, z.st_mark, z.counters, 5), debug ? H->records.p : nullptr, H->crec.p,
+                                  debug ? H->flags.p : nullptr, d_totals, H->bucket_base.p, H->prow.p, H->brec_l.p,
+                                  s, d_nsorted, H->sstart.p,
                                   Np, nr, d_err, pcv, H->batch ? H->wbase.p : nullptr));
     stage_mark(H, 11);
-    SPR_HIP(spr::launch_composite(H->records.p, H->brec_l.p, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
+    SPR_HIP(spr::launch_composite(H->crec.p, H->brec_l.p, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
                                   (int)H->opts.compose, H->image.p, s, nr, H->batch ? H->docs.p : nullptr,
                                   (int)H->n_docs, H->total_rows));
     stage_mark(H, 12);
@@ -845,7 +843,7 @@ int spr_destroy(spr_handle* H) {
     H->chain_t.release(); H->chains.release(); H->pieces.release(); H->chain_order.release();
     H->long_list.release(); H->mono.release(); H->pieces_seq.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release(); H->swn.release();
-    H->winding.release(); H->flags.release(); H->records.release();
+    H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
     H->band_start.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
     drop_graph(H);
@@ -1009,6 +1007,7 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 from_dev(H->flags.p, (size_t)N * 8);
                 break;
             case SPR_STAGE_RECORDS: {
+                if (!debug) throw Fail{SPR_ERR_STATE, "full records are kept only with the debug option"};
                 int64_t R = (int64_t)H->h_pinned[4] + H->h_pinned[5];
                 from_dev(H->records.p, (size_t)R * 16);
                 break;
