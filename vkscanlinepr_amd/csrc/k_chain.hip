@@ -15,10 +15,11 @@
 //     is node l of the decision tree and recomputes exactly the midpoints on its path), a ballot
 //     resolves the path, and 4 rounds give the reference's 24 iterations.  Then waves take the
 //     remaining chains 64 at a time, one chain per lane, in descending length.
-//  4. k_piece_merge: one thread per piece merges its start and two chains in the reference's
-//     order; long pieces whose chains are non-decreasing are merged by one wave with merge-path
-//     splits instead (k_piece_merge_wave).  Both write raw merged values.
-//  5. k_or_merge: one thread per entry applies the reference's equal-t OR-merge inside pieces.
+//  4. k_piece_merge: one wave per piece stages its two chains in LDS, merges them with its start
+//     in the reference's order (merge-path when both are non-decreasing, a concatenation when
+//     both are non-increasing, the sequential walk otherwise), applies the reference's equal-t
+//     OR-merge there, and writes the piece's (curve, t bits) entries once, coalesced.  Pieces of
+//     more than 1024 entries go to k_piece_merge_wave, which does the same with more LDS.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -28,7 +29,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kLongClass = 10;     // cubic chains of 2^10 = 1024+ crossings get a whole wave
-constexpr int kLongPiece = 256;    // pieces at least this long are merged by a wave
 constexpr int kStage = 4096;       // piece values staged in LDS by the merging wave
 constexpr int kClasses = 32;       // 0..15: cubic chains by log2 length, 16..31: line chains
 
@@ -296,186 +296,168 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
     }
 }
 
-// Reference merge of one piece (one lane), raw values: the piece start and both chains in the
-// order of make_intersection_1.comp:310-359.
-__device__ inline void merge_piece_lane(const PieceDesc& pd, const float* __restrict__ chain_t,
-                                        int32_t* __restrict__ raw, int8_t* __restrict__ start_mark,
-                                        int2* __restrict__ inter) {
-    const int nx = pd.nx, ny = pd.ny;
-    const float* X = chain_t + pd.base + 1;
-    const float* Y = X + nx;
-    const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
-    float tx = pd.t0, ty = pd.t0;
-    int xi = 0, yi = 0;
-    if (nx == 0) tx = kX2;
-    else if (ny == 0) ty = kY2;
-    else { tx = X[0]; xi = 1; }
-    const int n = nx + ny + 1;
-    for (int i = 0; i < n; ++i) {
-        float tmin;
-        if (tx <= ty) { tmin = tx; tx = xi < nx ? X[xi++] : kX2; }
-        else { tmin = ty; ty = yi < ny ? Y[yi++] : kY2; }
-        raw[pd.base + i] = (int32_t)fbits(tmin);
-        start_mark[pd.base + i] = i == 0;
-        inter[pd.base + i].x = pd.curve;
+// Merge mode of a piece: 0 = both merged sequences non-decreasing (merge-path), 1 = both
+// non-increasing (the reference merge takes the sequence with the smaller head to its end, then
+// the other one: a concatenation), 2 = anything else (the sequential merge).  Sequences:
+// nx>0 && ny>0: A = X, B = [t0, Y]; ny == 0: A = [t0, X]; nx == 0: B = [t0, Y].
+__device__ inline uint8_t piece_mode(const PieceDesc& pd, int64_t slot, const uint8_t* __restrict__ mono,
+                                     const float* __restrict__ chain_t) {
+    uint32_t fa = 3u, fb = 3u;
+    if (pd.nx > 0) fa = mono[2 * slot];
+    if (pd.ny > 0) fb = mono[2 * slot + 1];
+    if (pd.ny == 0 && pd.nx > 0) {
+        const float x1 = chain_t[pd.base + 1];
+        if (!(x1 >= pd.t0)) fa &= ~1u;
+        if (!(x1 <= pd.t0)) fa &= ~2u;
     }
+    if ((fa & fb & 1u) != 0) return 0;
+    if ((fa & fb & 2u) != 0) return 1;
+    return 2;
 }
 
-__global__ __launch_bounds__(kBlock) void k_piece_merge(const PieceDesc* __restrict__ pieces, int64_t n_slots,
-                                                        const ChainDesc* __restrict__ chains,
-                                                        const uint8_t* __restrict__ mono,
-                                                        const float* __restrict__ chain_t,
-                                                        int32_t* __restrict__ raw, int8_t* __restrict__ start_mark,
-                                                        int2* __restrict__ inter, int32_t* __restrict__ long_list,
-                                                        int32_t* __restrict__ long_count,
-                                                        uint8_t* __restrict__ pieces_seq) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool wave_it = false;
-    if (i < n_slots) {
-        const PieceDesc pd = pieces[i];
-        if (pd.base >= 0) {
-            const int n = pd.nx + pd.ny + 1;
-            if (n >= kLongPiece) {
-                // Both merged sequences non-decreasing: merge-path.  Both non-increasing: the
-                // reference merge takes the sequence with the smaller head to its end, then the
-                // other one (a concatenation).  Anything else: the sequential merge.
-                // Sequences: nx>0 && ny>0: A = X, B = [t0, Y]; ny == 0: A = [t0, X]; nx == 0: B = [t0, Y].
-                uint32_t fa = 3u, fb = 3u;
-                if (pd.nx > 0) fa = mono[2 * i];
-                if (pd.ny > 0) fb = mono[2 * i + 1];
-                if (pd.ny == 0 && pd.nx > 0) {
-                    const float x1 = chain_t[pd.base + 1];
-                    if (!(x1 >= pd.t0)) fa &= ~1u;
-                    if (!(x1 <= pd.t0)) fa &= ~2u;
-                }
-                uint8_t mode = 2;
-                if ((fa & fb & 1u) != 0) mode = 0;
-                else if ((fa & fb & 2u) != 0) mode = 1;
-                pieces_seq[i] = mode;
-                wave_it = true;
-            } else {
-                merge_piece_lane(pd, chain_t, raw, start_mark, inter);
+// Orders within one wave: LDS written by some lanes and then read by others.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Merges one piece held in LDS (X = xs[0..nx), Y = xs[nx..nx+ny)) into out[0..n) with the 64
+// lanes of a wave, in the order of make_intersection_1.comp:310-359, ties to x.
+__device__ inline void wave_merge(const PieceDesc& pd, uint8_t mode, const float* X, uint32_t* out, int lane) {
+    const int nx = pd.nx, ny = pd.ny;
+    const float* Y = X + nx;
+    const bool head_in_a = ny == 0;
+    const int na = nx + (head_in_a ? 1 : 0), nb = head_in_a ? 0 : ny + 1;
+    auto Aat = [&](int i) { return head_in_a ? (i == 0 ? pd.t0 : X[i - 1]) : X[i]; };
+    auto Bat = [&](int j) { return j == 0 ? pd.t0 : Y[j - 1]; };
+    const int n = na + nb;
+    if (mode == 2) {
+        if (lane == 0) {
+            const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
+            float tx = pd.t0, ty = pd.t0;
+            int xi = 0, yi = 0;
+            if (nx == 0) tx = kX2;
+            else if (ny == 0) ty = kY2;
+            else { tx = X[0]; xi = 1; }
+            for (int i = 0; i < n; ++i) {
+                float tmin;
+                if (tx <= ty) { tmin = tx; tx = xi < nx ? X[xi++] : kX2; }
+                else { tmin = ty; ty = yi < ny ? Y[yi++] : kY2; }
+                out[i] = fbits(tmin);
+            }
+        }
+    } else if (mode == 1) {
+        const bool a_first = nb == 0 || (na > 0 && Aat(0) <= Bat(0));
+        for (int o = lane; o < n; o += 64) {
+            float v;
+            if (a_first) v = o < na ? Aat(o) : Bat(o - na);
+            else v = o < nb ? Bat(o) : Aat(o - nb);
+            out[o] = fbits(v);
+        }
+    } else {
+        const int per = (n + 63) / 64;
+        const int o0 = lane * per;
+        const int o1 = o0 + per < n ? o0 + per : n;
+        if (o0 < n) {
+            int lo = o0 > nb ? o0 - nb : 0, hi = o0 < na ? o0 : na;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (Aat(mid) <= Bat(o0 - 1 - mid)) lo = mid + 1; else hi = mid;
+            }
+            int i = lo, j = o0 - lo;
+            for (int o = o0; o < o1; ++o) {
+                const bool takeA = j >= nb || (i < na && Aat(i) <= Bat(j));
+                const float v = takeA ? Aat(i) : Bat(j);
+                i += takeA; j += !takeA;
+                out[o] = fbits(v);
             }
         }
     }
-    const int32_t slot = wave_slot(long_count, 0u, wave_it);
-    if (wave_it) long_list[slot] = (int32_t)i;
 }
 
-// One wave per long piece.  The piece start and both chains are staged in LDS (coalesced), the
-// merged sequence is built in LDS, and then written out coalesced.  Mode 0: each lane merges a
-// contiguous slice after a merge-path split; mode 1: concatenation; mode 2: lane 0 runs the
-// reference's sequential merge.  A = the first sequence, B = the second, ties to A.
+// The reference's OR of equal t inside a piece (make_intersection_1.comp:361-375): within a run
+// [s, e) of equal masked t, entry o ends up as the OR of raw values s..min(o+1, e-1).  Runs never
+// leave the piece (its first entry starts a run).  `v` is the merged piece in LDS.
+__device__ inline uint32_t or_merged(const uint32_t* v, int n, int o) {
+    const uint32_t m = v[o] & 0xFFFFFFFCu;
+    uint32_t acc = v[o];
+    for (int s = o; s > 0 && (v[s - 1] & 0xFFFFFFFCu) == m; --s) acc |= v[s - 1];
+    if (o + 1 < n && (v[o + 1] & 0xFFFFFFFCu) == m) acc |= v[o + 1];
+    return acc;
+}
+
+constexpr int kWaveStage = 1024;  // pieces up to this many entries are merged by k_piece_merge
+
+// One wave per piece, four waves per workgroup, pieces taken in slot order with a stride (the
+// outputs of neighbouring pieces are neighbours in memory).  The piece's chains are staged in
+// the wave's LDS, merged, OR-merged, and written once as (curve, t bits) entries.  Longer pieces
+// are queued for k_piece_merge_wave.
+__global__ __launch_bounds__(kBlock) void k_piece_merge(const PieceDesc* __restrict__ pieces, int64_t n_slots,
+                                                        const uint8_t* __restrict__ mono,
+                                                        const float* __restrict__ chain_t,
+                                                        int2* __restrict__ inter, int32_t* __restrict__ long_list,
+                                                        int32_t* __restrict__ long_count,
+                                                        uint8_t* __restrict__ pieces_seq) {
+    __shared__ float s_in[kBlock / 64][kWaveStage];
+    __shared__ uint32_t s_out[kBlock / 64][kWaveStage];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* xin = s_in[wave];
+    uint32_t* vout = s_out[wave];
+    const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
+    for (int64_t slot = (int64_t)blockIdx.x * (kBlock / 64) + wave; slot < n_slots; slot += stride) {
+        const PieceDesc pd = pieces[slot];
+        if (pd.base < 0) continue;
+        const int nv = pd.nx + pd.ny, n = nv + 1;
+        const uint8_t mode = piece_mode(pd, slot, mono, chain_t);
+        if (n > kWaveStage) {
+            if (lane == 0) {
+                pieces_seq[slot] = mode;
+                long_list[atomicAdd(long_count, 1)] = (int32_t)slot;
+            }
+            continue;
+        }
+        for (int i = lane; i < nv; i += 64) xin[i] = chain_t[pd.base + 1 + i];
+        wave_sync();
+        wave_merge(pd, mode, xin, vout, lane);
+        wave_sync();
+        for (int o = lane; o < n; o += 64) inter[pd.base + o] = make_int2(pd.curve, (int32_t)or_merged(vout, n, o));
+        wave_sync();
+    }
+}
+
+// One wave per long piece (more than kWaveStage entries).  Pieces up to kStage entries are
+// staged in LDS and finished there like in k_piece_merge; longer ones are merged into the raw
+// value buffer in global memory, then OR-merged from it by the same wave.
 __global__ __launch_bounds__(64) void k_piece_merge_wave(const PieceDesc* __restrict__ pieces,
                                                          const int32_t* __restrict__ long_list,
                                                          const int32_t* __restrict__ long_count,
                                                          const uint8_t* __restrict__ piece_mode,
                                                          const float* __restrict__ chain_t,
-                                                         int32_t* __restrict__ raw,
-                                                         int8_t* __restrict__ start_mark,
+                                                         uint32_t* __restrict__ raw,
                                                          int2* __restrict__ inter) {
     __shared__ float st_in[kStage];
-    __shared__ float st_out[kStage + 1];
+    __shared__ uint32_t st_out[kStage + 1];
     const int lane = threadIdx.x;
     const int32_t cnt = *long_count;
     for (int32_t q = blockIdx.x; q < cnt; q += gridDim.x) {
         const int32_t pi = long_list[q];
         const PieceDesc pd = pieces[pi];
-        const int nx = pd.nx, ny = pd.ny;
-        const int nv = nx + ny;
-        const bool lds = nv <= kStage;
+        const int nv = pd.nx + pd.ny, n = nv + 1;
+        const uint8_t mode = piece_mode[pi];
+        const bool lds = n <= kStage;
         const float* X = chain_t + pd.base + 1;
         if (lds) {
             for (int i = lane; i < nv; i += 64) st_in[i] = X[i];
             __syncthreads();
             X = st_in;
         }
-        const float* Y = X + nx;
-        const bool head_in_a = ny == 0;
-        const int na = nx + (head_in_a ? 1 : 0), nb = head_in_a ? 0 : ny + 1;
-        auto Aat = [&](int i) { return head_in_a ? (i == 0 ? pd.t0 : X[i - 1]) : X[i]; };
-        auto Bat = [&](int j) { return j == 0 ? pd.t0 : Y[j - 1]; };
-        const int n = na + nb;
-        float* outv = lds ? st_out : nullptr;
-        auto emit = [&](int o, float v) {
-            if (lds) outv[o] = v;
-            else {
-                raw[pd.base + o] = (int32_t)fbits(v);
-                start_mark[pd.base + o] = o == 0;
-                inter[pd.base + o].x = pd.curve;
-            }
-        };
-        const uint8_t mode = piece_mode[pi];
-        if (mode == 2) {
-            if (lane == 0) {
-                const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
-                float tx = pd.t0, ty = pd.t0;
-                int xi = 0, yi = 0;
-                if (nx == 0) tx = kX2;
-                else if (ny == 0) ty = kY2;
-                else { tx = X[0]; xi = 1; }
-                for (int i = 0; i < n; ++i) {
-                    float tmin;
-                    if (tx <= ty) { tmin = tx; tx = xi < nx ? X[xi++] : kX2; }
-                    else { tmin = ty; ty = yi < ny ? Y[yi++] : kY2; }
-                    emit(i, tmin);
-                }
-            }
-        } else if (mode == 1) {
-            const bool a_first = nb == 0 || (na > 0 && Aat(0) <= Bat(0));
-            for (int o = lane; o < n; o += 64) {
-                float v;
-                if (a_first) v = o < na ? Aat(o) : Bat(o - na);
-                else v = o < nb ? Bat(o) : Aat(o - nb);
-                emit(o, v);
-            }
-        } else {
-            const int per = (n + 63) / 64;
-            const int o0 = lane * per;
-            const int o1 = o0 + per < n ? o0 + per : n;
-            if (o0 < n) {
-                int lo = o0 > nb ? o0 - nb : 0, hi = o0 < na ? o0 : na;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (Aat(mid) <= Bat(o0 - 1 - mid)) lo = mid + 1; else hi = mid;
-                }
-                int i = lo, j = o0 - lo;
-                for (int o = o0; o < o1; ++o) {
-                    const bool takeA = j >= nb || (i < na && Aat(i) <= Bat(j));
-                    const float v = takeA ? Aat(i) : Bat(j);
-                    i += takeA; j += !takeA;
-                    emit(o, v);
-                }
-            }
-        }
-        if (lds) {
-            __syncthreads();
-            for (int o = lane; o < n; o += 64) {
-                raw[pd.base + o] = (int32_t)fbits(st_out[o]);
-                start_mark[pd.base + o] = o == 0;
-                inter[pd.base + o].x = pd.curve;
-            }
-        }
+        uint32_t* vout = lds ? st_out : raw + pd.base;
+        wave_merge(pd, mode, X, vout, lane);
+        __threadfence_block();
+        __syncthreads();
+        for (int o = lane; o < n; o += 64) inter[pd.base + o] = make_int2(pd.curve, (int32_t)or_merged(vout, n, o));
         __syncthreads();
     }
-}
-
-// The reference's OR-merge of equal t inside a piece (make_intersection_1.comp:361-375): within a
-// run [s, e) of equal masked t, entry j ends up as the OR of raw values s..min(j+1, e-1).
-// Gaps of a predicted-allocation layout carry a start mark, so no run crosses into them.
-__global__ __launch_bounds__(kBlock) void k_or_merge(const int32_t* __restrict__ raw,
-                                                     const int8_t* __restrict__ start_mark, int64_t N_max,
-                                                     const int32_t* __restrict__ n_dev,
-                                                     int2* __restrict__ inter) {
-    const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
-    const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (j >= N) return;
-    const uint32_t v = (uint32_t)raw[j], m = v & 0xFFFFFFFCu;
-    uint32_t acc = v;
-    for (int64_t s = j; s > 0 && !start_mark[s] && ((uint32_t)raw[s - 1] & 0xFFFFFFFCu) == m; --s)
-        acc |= (uint32_t)raw[s - 1];
-    if (j + 1 < N && !start_mark[j + 1] && ((uint32_t)raw[j + 1] & 0xFFFFFFFCu) == m) acc |= (uint32_t)raw[j + 1];
-    inter[j].y = (int32_t)acc;
 }
 
 }  // namespace
@@ -509,22 +491,14 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const Ch
     return hipGetLastError();
 }
 
-hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const ChainDesc* chains, const uint8_t* mono,
-                              const float* chain_t, int32_t* raw, int8_t* start_mark, int2* inter,
-                              int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, int n_cu,
-                              hipStream_t s) {
+hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                              uint32_t* raw, int2* inter, int32_t* long_list, int32_t* long_count,
+                              uint8_t* pieces_seq, int n_cu, hipStream_t s) {
     if (C == 0) return hipSuccess;
-    k_piece_merge<<<grid_for((int64_t)C * 5), kBlock, 0, s>>>(pieces, (int64_t)C * 5, chains, mono, chain_t, raw,
-                                                              start_mark, inter, long_list, long_count, pieces_seq);
+    k_piece_merge<<<(unsigned)(n_cu * 16), kBlock, 0, s>>>(pieces, (int64_t)C * 5, mono, chain_t, inter, long_list,
+                                                           long_count, pieces_seq);
     k_piece_merge_wave<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, long_list, long_count, pieces_seq, chain_t, raw,
-                                                           start_mark, inter);
-    return hipGetLastError();
-}
-
-hipError_t launch_or_merge(const int32_t* raw, const int8_t* start_mark, int64_t N, int2* inter, hipStream_t s,
-                           const int32_t* n_dev) {
-    if (N == 0) return hipSuccess;
-    k_or_merge<<<grid_for(N), kBlock, 0, s>>>(raw, start_mark, N, n_dev, inter);
+                                                           inter);
     return hipGetLastError();
 }
 

@@ -57,14 +57,11 @@ hipError_t launch_chain_order(const ChainDesc* chains, uint32_t C, const int32_t
 hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
                               const int32_t* order, const int32_t* class_off, const int32_t* n_long,
                               int32_t* work, float* chain_t, uint8_t* mono, int n_cu, hipStream_t s);
-hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const ChainDesc* chains, const uint8_t* mono,
-                              const float* chain_t, int32_t* raw, int8_t* start_mark, int2* inter,
-                              int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, int n_cu,
-                              hipStream_t s);
-// Launchers taking `n_dev` size the grid for the host bound and read the live count from the
-// device when n_dev is not null (a frame that never waits for the host).
-hipError_t launch_or_merge(const int32_t* raw, const int8_t* start_mark, int64_t N, int2* inter, hipStream_t s,
-                           const int32_t* n_dev = nullptr);
+// Merges each piece's x and y chains into (curve, t bits) entries, OR-merging equal t (the
+// reference's per-piece walk); raw is scratch for pieces too long for LDS.
+hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                              uint32_t* raw, int2* inter, int32_t* long_list, int32_t* long_count,
+                              uint8_t* pieces_seq, int n_cu, hipStream_t s);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,

@@ -330,12 +330,10 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     }
     stage_mark(H, 5);
     if (gen) {
-        // raw merged values and piece-start marks borrow the key and winding buffers, which
-        // gen_fragment fills afterwards
+        // very long pieces merge through the key buffer, which gen_fragment fills afterwards
         SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
-        SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->chains.p, H->mono.p, H->chain_t.p, H->key.p, H->wn.p,
-                                        H->inter.p, H->long_list.p, d_nlpieces, H->pieces_seq.p, H->n_cu, s));
-        SPR_HIP(spr::launch_or_merge(H->key.p, H->wn.p, N, H->inter.p, s));
+        SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->key.p, H->inter.p,
+                                        H->long_list.p, d_nlpieces, H->pieces_seq.p, H->n_cu, s));
     }
     stage_mark(H, 6);
     SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
@@ -563,10 +561,8 @@ void run_learned_frame(spr_handle* H) {
     stage_mark(H, 5);
     if (C) {
         SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
-        SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->chains.p, H->mono.p, H->chain_t.p, H->key.p, H->wn.p,
-                                        H->inter.p, H->long_list.p, d_nlpieces, H->pieces_seq.p,
-                                        H->n_cu * H->merge_k / 8, s));
-        SPR_HIP(spr::launch_or_merge(H->key.p, H->wn.p, nl, H->inter.p, s, d_live));
+        SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->key.p, H->inter.p,
+                                        H->long_list.p, d_nlpieces, H->pieces_seq.p, H->n_cu * H->merge_k / 8, s));
     }
     stage_mark(H, 6);
     SPR_HIP(spr::launch_gen_fragment(H->inter.p, nl, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
