@@ -6,13 +6,16 @@
 // wins), and canvas row y lands on image row h-1-y.  All records sit on even rows, so the image
 // is a stack of independent two-pixel "cell rows".
 //
-// One workgroup owns one cell row, in column chunks of 4096; each thread keeps 16 pixels of the
-// chunk in registers (columns t, t+256, ...).  The (path, row) bucket table gives each path's
-// records on this row as one contiguous range of non-overlapping records with one colour.  Paths
-// are taken in order, 64 at a time: every record ORs its path's bit into a 64-bit coverage mask
-// per pixel in LDS (order does not matter for an OR), then each thread folds the set bits of its
-// pixels in path order (overwrite: the last one wins; src-over: d = s*a + d*(1-a)).  That is two
-// barriers per 64 paths instead of one per path, and each image row is written exactly once.
+// One workgroup owns one cell row, in column chunks of 4096; each thread keeps 16 consecutive
+// pixels of the chunk in registers.  The (path, row) bucket table gives each path's records on
+// this row as one contiguous range of non-overlapping compact records (x | width << 16) with one
+// colour.  Paths are taken in order, 64 at a time: a record toggles its path's bit at its first
+// column and at its end column in a 64-bit event word per column (LDS), and a prefix XOR over
+// the columns turns the events into each pixel's coverage mask (the records of one path never
+// overlap, so the XOR of the toggles left of a pixel is exactly "inside one of them").  Each
+// thread then folds the set bits of its pixels in path order (overwrite: the last one wins;
+// src-over: d = s*a + d*(1-a)).  A record costs two LDS operations whatever its width, and each
+// image row is written once.
 #include "kernels.hpp"
 
 namespace spr {
@@ -20,10 +23,35 @@ namespace spr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kChunk = 4096;               // columns per pass
-constexpr int kPix = kChunk / kBlock;      // pixels per thread
+constexpr int kPix = 16;                   // consecutive pixels per thread
+constexpr int kChunk = kBlock * kPix;      // columns per pass
 constexpr int kBatch = 64;                 // paths per coverage mask
 constexpr int kUnroll = 4;                 // records loaded per thread before applying
+typedef unsigned long long u64;
+
+__device__ inline u64 shfl_up_u64(u64 v, int d) {
+    const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    return (u64)(uint32_t)__shfl_up((int)lo, d) | ((u64)(uint32_t)__shfl_up((int)hi, d) << 32);
+}
+
+// Exclusive prefix XOR of one value per thread over the workgroup.
+__device__ inline u64 block_exclusive_xor(u64 v, u64* lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    u64 inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const u64 u = shfl_up_u64(inc, d);
+        if (lane >= d) inc ^= u;
+    }
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    u64 off = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k)
+        if (k < wave) off ^= lds[k];
+    __syncthreads();
+    return off ^ inc ^ v;
+}
 
 template <int COMPOSE>
 __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict__ records,
@@ -34,13 +62,14 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                                       int w, int h, int64_t rec_cap,
                                                       uint8_t* __restrict__ image,
                                                       const DocDev* __restrict__ docs, int n_docs) {
-    __shared__ unsigned long long mask[kChunk];
+    __shared__ u64 ev[kChunk + 1];
     __shared__ int2 list[kBlock];
     __shared__ uint32_t lcol[kBlock];
     __shared__ float4 lsa[kBlock];   // (r*a, g*a, b*a, a) of each listed path, as the blend uses them
     __shared__ float lia[kBlock];    // 1 - a
     __shared__ int32_t wcount[kBlock / 64];
     __shared__ int32_t bpre[kBatch + 1];
+    __shared__ u64 xlds[kBlock / 64];
     int r = blockIdx.x;
     uint32_t pbeg = 0, pend = Np;
     if (docs) {
@@ -58,6 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
     }
     const int y = 2 * r;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int xb = tid * kPix;  // first column of this thread inside the chunk
     for (int c0 = 0; c0 < w; c0 += kChunk) {
         const int cw = w - c0 < kChunk ? w - c0 : kChunk;
         uint32_t pix[kPix];
@@ -66,9 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
         for (int j = 0; j < kPix; ++j) {
             pix[j] = 0xFFFFFFFFu;
             if constexpr (COMPOSE != 0) { acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 1.0f; }
-            mask[j * kBlock + tid] = 0ull;
         }
-        __syncthreads();
         for (uint32_t p0 = pbeg; p0 < pend; p0 += kBlock) {
             // ordered list of the paths of this chunk that have records on this cell row
             const uint32_t p = p0 + tid;
@@ -78,12 +106,12 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 const int loc = r == 0 ? pr.y + 1 : ((r >= pr.x && r < pr.x + pr.y) ? r - pr.x : -1);
                 if (loc >= 0) rng = brec[bucket_base[p] + loc];
                 // ranges never reach past the records actually stored (a frame whose record
-                // budget overflowed is redone by the host; it must still read in bounds)
+                // budget overflowed must still read in bounds)
                 rng.y = (int64_t)rng.y < rec_cap ? rng.y : (int32_t)rec_cap;
                 rng.x = rng.x < rng.y ? rng.x : rng.y;
             }
             const bool keep = rng.y > rng.x;
-            const unsigned long long m = __ballot(keep);
+            const u64 m = __ballot(keep);
             if (lane == 0) wcount[wave] = (int32_t)__popcll(m);
             __syncthreads();
             int off = 0, n = 0;
@@ -107,7 +135,10 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
             __syncthreads();
             for (int q0 = 0; q0 < n; q0 += kBatch) {
                 const int qn = n - q0 < kBatch ? n - q0 : kBatch;
-                // the batch's records as one flat index space spread over all threads
+                // clear the event words; the batch's records as one flat index space
+#pragma unroll
+                for (int j = 0; j < kPix; ++j) ev[xb + j] = 0ull;
+                if (tid == 0) ev[kChunk] = 0ull;
                 if (wave == 0) {
                     const int2 rg = lane < qn ? list[q0 + lane] : make_int2(0, 0);
                     int inc = rg.y - rg.x;
@@ -141,21 +172,28 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u) {
                         if (qi[u] < 0) continue;
-                        const unsigned long long bit = 1ull << qi[u];
-                        int xs = (int)(rec[u] & 0xFFFFu) - c0;   // compact record: x | width << 16
+                        const u64 bit = 1ull << qi[u];
+                        int xs = (int)(rec[u] & 0xFFFFu) - c0;
                         int xe = xs + (int)(rec[u] >> 16);
                         xs = xs < 0 ? 0 : xs;
                         xe = xe > cw ? cw : xe;
-                        for (int x = xs; x < xe; ++x) atomicOr(&mask[x], bit);
+                        if (xs < xe) {
+                            atomicXor(&ev[xs], bit);
+                            atomicXor(&ev[xe], bit);  // ev[cw] only collects ends past the chunk
+                        }
                     }
                 }
                 __syncthreads();
+                // coverage = prefix XOR of the events, then the fold in path order
+                u64 run = 0;
+#pragma unroll
+                for (int j = 0; j < kPix; ++j) run ^= ev[xb + j];
+                u64 cov = block_exclusive_xor(run, xlds);
 #pragma unroll
                 for (int j = 0; j < kPix; ++j) {
-                    const int x = j * kBlock + tid;
-                    unsigned long long mm = mask[x];
+                    cov ^= ev[xb + j];
+                    u64 mm = cov;
                     if (mm == 0ull) continue;
-                    mask[x] = 0ull;
                     if constexpr (COMPOSE == 0) {
                         pix[j] = lcol[q0 + 63 - __clzll((long long)mm)];
                     } else {
@@ -189,10 +227,15 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
         }
         for (int yy = y; yy < y + 2 && yy < h; ++yy) {
             uint32_t* row = reinterpret_cast<uint32_t*>(image) + (int64_t)(h - 1 - yy) * w + c0;
+            if (xb + kPix <= cw && (((uintptr_t)(row + xb)) & 15u) == 0) {
+                uint4* q = reinterpret_cast<uint4*>(row + xb);
 #pragma unroll
-            for (int j = 0; j < kPix; ++j) {
-                const int x = j * kBlock + tid;
-                if (x < cw) row[x] = pix[j];
+                for (int k = 0; k < kPix / 4; ++k)
+                    q[k] = make_uint4(pix[4 * k], pix[4 * k + 1], pix[4 * k + 2], pix[4 * k + 3]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < kPix; ++j)
+                    if (xb + j < cw) row[xb + j] = pix[j];
             }
         }
         __syncthreads();
