@@ -8,13 +8,17 @@
 // every float operation of the reference and only changes who computes what:
 //
 //  1. k_chain_prep: one thread per (curve, piece) walks the piece geometry and writes one
-//     descriptor per non-empty chain and per piece; chains are counted in length classes.
-//  2. k_chain_order: chains are laid out longest class first (longest-processing-time order).
+//     descriptor per non-empty chain and per piece; chains are counted in length classes
+//     (quarter octaves for cubic chains, so that the 64 chains of one wave have similar lengths).
+//  2. k_chain_order: chains are laid out costliest class first (longest-processing-time order).
 //  3. k_chain_solve: a persistent kernel.  Waves first take the long cubic chains, one chain per
 //     wave: the 64 lanes evaluate the 63 midpoints of the next 6 bisection levels at once (lane l
 //     is node l of the decision tree and recomputes exactly the midpoints on its path), a ballot
-//     resolves the path, and 4 rounds give the reference's 24 iterations.  Then waves take the
-//     remaining chains 64 at a time, one chain per lane, in descending length.
+//     resolves the path in scalar registers, and 4 rounds give the reference's 24 iterations.
+//     Then waves take the remaining chains 64 at a time, one chain per lane, in descending
+//     length.  The first item of every wave is its global wave index (the first waves of a
+//     launch sit on distinct SIMDs, so the costliest items do not share one), later items come
+//     from an atomic counter; issue priority follows the item's cost.
 //  4. k_piece_merge: one wave per piece stages its two chains in LDS, merges them with its start
 //     in the reference's order (merge-path when both are non-decreasing, a concatenation when
 //     both are non-increasing, the sequential walk otherwise), applies the reference's equal-t
@@ -30,14 +34,27 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kLongClass = 10;     // cubic chains of 2^10 = 1024+ crossings get a whole wave
 constexpr int kStage = 4096;       // piece values staged in LDS by the merging wave
-constexpr int kClasses = 32;       // 0..15: cubic chains by log2 length, 16..31: line chains
+constexpr int kClasses = kChainClasses;  // 0..63: cubic chains (4 per octave), 64..79: line chains
+constexpr int kRounds = 4, kLevels = 6;  // wave solve: 4 rounds of 6 bisection levels
+static_assert(kRounds * kLevels == kCubicIters, "wave solve must run the reference's iterations");
 
 inline unsigned grid_for(int64_t n) {
     int64_t g = (n + kBlock - 1) / kBlock;
     return (unsigned)(g < 1 ? 1 : g);
 }
 
-__device__ inline int log2_class(int n) { int l = 31 - __clz(n); return l > 15 ? 15 : l; }
+// Length class of a chain of n >= 1 crossings: cubic chains by octave and by the two bits after
+// the leading one (quarter octaves), line chains by octave.
+__device__ inline int chain_class(int n, bool cubic) {
+    const int l = 31 - __clz(n);
+    if (!cubic) return 64 + (l > 15 ? 15 : l);
+    if (l > 15) return 63;
+    const int sub = l >= 2 ? (n >> (l - 2)) & 3 : (l == 1 ? (n & 1) << 1 : 0);
+    return 4 * l + sub;
+}
+
+// Cost octave of a class (a line crossing costs about 2^-6 of a cubic one).
+__device__ inline int class_cost_octave(int cls) { return cls < 64 ? cls >> 2 : cls - 64 - 6; }
 
 // Walks the monotone pieces of a visible curve up to `piece`: flagged end parameters, grid and
 // first entry offset inside the curve's output range.
@@ -113,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
                     d.t0 = pc.t0;
                     d.t1 = pc.t1;
                     d.cubic = type == kCubic;
-                    atomicAdd(&hist[(d.cubic ? 0 : 16) + log2_class(n)], 1);
+                    atomicAdd(&hist[chain_class(n, d.cubic)], 1);
                 }
             }
         }
@@ -127,28 +144,22 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
     if (threadIdx.x < kClasses && hist[threadIdx.x]) atomicAdd(&class_count[threadIdx.x], hist[threadIdx.x]);
 }
 
-// Class order: by estimated cost, longest first (a line crossing costs about 2^-6 of a cubic
-// one), cubic before line on ties.  The cubic classes >= kLongClass come first: the wave-solved
-// chains.
+// Class order: by estimated cost in quarter octaves, costliest first (a line crossing costs
+// about 2^-6 of a cubic one), cubic before line on ties.  Cost key: cubic class c -> 2(c+24)+1,
+// line class 64+l -> 8l; the keys are distinct.  The cubic classes of kLongClass octaves and up
+// come first: the wave-solved chains.
 __global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t* __restrict__ class_off,
                                 int32_t* __restrict__ n_long) {
     if (threadIdx.x != 0) return;
-    int32_t cls[kClasses], key[kClasses];
-    for (int c = 0; c < kClasses; ++c) {
-        cls[c] = c;
-        key[c] = c < 16 ? 2 * (c + 6) + 1 : 2 * (c - 16);
-    }
-    for (int i = 1; i < kClasses; ++i)  // insertion sort, descending key
-        for (int j = i; j > 0 && key[j] > key[j - 1]; --j) {
-            int32_t t = key[j]; key[j] = key[j - 1]; key[j - 1] = t;
-            t = cls[j]; cls[j] = cls[j - 1]; cls[j - 1] = t;
-        }
     int32_t off = 0, nl = 0;
-    for (int r = 0; r < kClasses; ++r) {
-        const int c = cls[r];
+    for (int key = 2 * (63 + 24) + 1; key >= 0; --key) {
+        int c = -1;
+        if (key & 1) c = (key - 1) / 2 - 24;
+        else if ((key & 7) == 0 && key / 8 <= 15) c = 64 + key / 8;
+        if (c < 0) continue;
         class_off[c] = off;
         off += class_count[c];
-        if (c < 16 && c >= kLongClass) nl = off;
+        if (c < 64 && c >= 4 * kLongClass) nl = off;
     }
     class_off[kClasses] = off;
     *n_long = nl;
@@ -165,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_order(const ChainDesc* __restr
     const bool valid = i < n_slots && chains[i].n > 0;
     int cls = 0, rank = 0;
     if (valid) {
-        cls = (chains[i].cubic ? 0 : 16) + log2_class(chains[i].n);
+        cls = chain_class(chains[i].n, chains[i].cubic != 0);
         rank = atomicAdd(&cnt[cls], 1);
     }
     __syncthreads();
@@ -205,10 +216,22 @@ __device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, 
 }
 
 // Cubic chain solved by one wave: 4 rounds of a 6-level speculative bisection per crossing.
+// Lane l (1..63) is node l of the round's decision tree: its depth is floor(log2 l) and the bits
+// of l below the leading one are its path (1 = right).  Each lane halves the round's interval
+// along its own path, exactly as the sequential bisection would, and evaluates the midpoint of
+// its node; the ballot of "same sign as at t_min" fixes the path taken through the tree, which
+// is walked in scalar registers, and the chosen depth-5 node's interval, midpoint and value are
+// read from its lane.
 __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
                                         uint8_t* __restrict__ mono) {
     const int lane = threadIdx.x & 63;
     const int depth = lane ? 31 - __clz(lane) : 0;
+    bool on[kLevels - 1], go_right[kLevels - 1];
+#pragma unroll
+    for (int lev = 0; lev < kLevels - 1; ++lev) {
+        on[lev] = lev < depth;
+        go_right[lev] = on[lev] && ((lane >> (depth - 1 - lev)) & 1);
+    }
     const uint32_t side = (uint32_t)d.side;
     float tprev = d.t0, line = d.line0;
     bool up = true, down = true;
@@ -221,24 +244,27 @@ __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, 
             const uint32_t s0 = fbits(vt0 - c) & 0x80000000u;
             float lo = t_min, hi = d.t1;
             float tm = t_min, vtm = vt0;
-            for (int round = 0; round < 4; ++round) {
-                // lane l = node l of a 63-node decision tree below (lo, hi)
+#pragma unroll 1
+            for (int round = 0; round < kRounds; ++round) {
                 float llo = lo, lhi = hi;
-                for (int lev = depth - 1; lev >= 0; --lev) {
+#pragma unroll
+                for (int lev = 0; lev < kLevels - 1; ++lev) {
                     const float mid = (llo + lhi) * 0.5f;
-                    if ((lane >> lev) & 1) llo = mid; else lhi = mid;
+                    llo = go_right[lev] ? mid : llo;
+                    lhi = (on[lev] && !go_right[lev]) ? mid : lhi;
                 }
                 const float mym = (llo + lhi) * 0.5f;
                 const float myv = eval_axis(true, mym, a.c[0], a.c[1], a.c[2], a.c[3]);
                 const bool right = lane != 0 && (fbits(myv - c) & 0x80000000u) == s0;
                 const unsigned long long D = __ballot(right);
-                int node = 1;
+                uint32_t node = 1;
 #pragma unroll
-                for (int lev = 0; lev < 5; ++lev) node = 2 * node + (int)((D >> node) & 1ull);
-                // node: the depth-5 node on the path, the last of this round's 6 midpoints
-                const float nlo = __shfl(llo, node), nhi = __shfl(lhi, node);
-                tm = __shfl(mym, node);
-                vtm = __shfl(myv, node);
+                for (int lev = 0; lev < kLevels - 1; ++lev) node = 2u * node + (uint32_t)((D >> node) & 1ull);
+                node = __builtin_amdgcn_readfirstlane(node);
+                const float nlo = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(llo), (int)node));
+                const float nhi = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(lhi), (int)node));
+                tm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(mym), (int)node));
+                vtm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(myv), (int)node));
                 if ((D >> node) & 1ull) { lo = tm; hi = nhi; } else { lo = nlo; hi = tm; }
             }
             ts = fabsf(vtm - c) > 1.0f ? t_min : tm;
@@ -265,11 +291,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
     const int32_t total = class_off[kClasses];
     const int32_t nl = *n_long;
     const int32_t items = nl + (total - nl + 63) / 64;
-    while (true) {
-        int32_t item = 0;
-        if (lane == 0) item = atomicAdd(work, 1);
-        item = __shfl(item, 0);
-        if (item >= items) break;
+    const int32_t n_waves = (int32_t)gridDim.x * (kBlock / 64);
+    int32_t item = (int32_t)blockIdx.x * (kBlock / 64) + (int32_t)(threadIdx.x >> 6);
+    while (item < items) {
         if (item < nl) {
             __builtin_amdgcn_s_setprio(3);
             const int32_t ci = order[item];
@@ -280,11 +304,12 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
         } else {
             const int32_t k0 = nl + (item - nl) * 64;
             const int32_t k = k0 + lane;
-            // issue priority follows the length of the wave's chains (the longest run first
-            // and must not be starved by the short ones)
-            const int cls = log2_class(chains[order[k0]].n) + (chains[order[k0]].cubic ? 6 : 0);
-            if (cls >= 13) __builtin_amdgcn_s_setprio(2);
-            else if (cls >= 11) __builtin_amdgcn_s_setprio(1);
+            // issue priority follows the cost of the wave's chains (the longest run first and
+            // must not be starved by the short ones)
+            const int oct = class_cost_octave(chain_class(chains[order[k0]].n, chains[order[k0]].cubic != 0));
+            if (oct >= 8) __builtin_amdgcn_s_setprio(3);
+            else if (oct >= 6) __builtin_amdgcn_s_setprio(2);
+            else if (oct >= 4) __builtin_amdgcn_s_setprio(1);
             if (k < total) {
                 const int32_t ci = order[k];
                 const ChainDesc d = chains[ci];
@@ -293,6 +318,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
             }
             __builtin_amdgcn_s_setprio(0);
         }
+        int32_t next = 0;
+        if (lane == 0) next = n_waves + atomicAdd(work, 1);
+        item = __shfl(next, 0);
     }
 }
 

@@ -45,7 +45,13 @@ struct PieceDesc {
     int32_t base, nx, ny, curve;  // base = -1: no piece in this slot
     float t0;
 };
-constexpr int kChainClasses = 32;
+// Chain length classes: cubic chains by quarter octave of their length (16 octaves x 4), then
+// line chains by octave.  The chain control block holds, as int32: class counts, class fill
+// counters, class offsets [kChainClasses + 1], the number of wave-solved chains, the work counter
+// of the persistent solve and the count of long pieces.
+constexpr int kChainClasses = 80;
+constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 * kChainClasses + 1,
+              kCcWork = kCcNLong + 1, kCcLongPieces = kCcWork + 1, kCcWords = kCcLongPieces + 1;
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,

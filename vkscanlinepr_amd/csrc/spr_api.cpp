@@ -182,7 +182,7 @@ ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
     z.ctr = o; o += 4;                 // eight u32 counters of the phase-B scans
     z.totals = o; o += 2;              // int32 cells, spans, R, spare
     z.misc = o; o += 1;                // int32 error flag, big-band count
-    z.chainctl = o; o += 64;           // chain classes: counts[32], fill[32], offsets[33], n_long, work, long pieces
+    z.chainctl = o; o += (spr::kCcWords + 1) / 2;  // chain classes, offsets, counters (int32)
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
     z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
     z.st_wind = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
@@ -304,12 +304,12 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     int2* d_brec = reinterpret_cast<int2*>(H->zero.p + z.brec);
 
     const bool gen = !H->inj && N > 0;
-    int32_t* d_cc = reinterpret_cast<int32_t*>(H->zero.p + z.chainctl);  // class counts [32]
-    int32_t* d_cfill = d_cc + 32;                                         // class fill [32]
-    int32_t* d_coff = d_cc + 64;                                          // class offsets [33]
-    int32_t* d_nlong = d_cc + 97;
-    int32_t* d_work = d_cc + 98;
-    int32_t* d_nlpieces = d_cc + 99;
+    int32_t* d_cc = reinterpret_cast<int32_t*>(H->zero.p + z.chainctl);  // class counts
+    int32_t* d_cfill = d_cc + spr::kCcFill;
+    int32_t* d_coff = d_cc + spr::kCcOff;
+    int32_t* d_nlong = d_cc + spr::kCcNLong;
+    int32_t* d_work = d_cc + spr::kCcWork;
+    int32_t* d_nlpieces = d_cc + spr::kCcLongPieces;
     if (H->inj) {
         SPR_HIP(hipMemcpyAsync(H->inter.p, H->inj_inter.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
     } else if (gen) {
@@ -400,7 +400,7 @@ LearnedLayout learned_layout(uint32_t Np, int64_t nb_budget, int64_t n_budget) {
     z.totals = o; o += 2;       // int32 cells, spans, R, spare
     z.slots = o; o += 2;        // int32 sorted entries, bands, spare x2
     z.bound = o; o += 1;        // u64 upper bound of the intersections
-    z.chainctl = o; o += 64;
+    z.chainctl = o; o += (spr::kCcWords + 1) / 2;
     z.st_arena = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
     z.st_pb = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
@@ -535,11 +535,11 @@ void run_learned_frame(spr_handle* H) {
                                       pcv));
     stage_mark(H, 3);
     int32_t* d_cc = reinterpret_cast<int32_t*>(H->zero.p + z.chainctl);
-    int32_t* d_cfill = d_cc + 32;
-    int32_t* d_coff = d_cc + 64;
-    int32_t* d_nlong = d_cc + 97;
-    int32_t* d_work = d_cc + 98;
-    int32_t* d_nlpieces = d_cc + 99;
+    int32_t* d_cfill = d_cc + spr::kCcFill;
+    int32_t* d_coff = d_cc + spr::kCcOff;
+    int32_t* d_nlong = d_cc + spr::kCcNLong;
+    int32_t* d_work = d_cc + spr::kCcWork;
+    int32_t* d_nlpieces = d_cc + spr::kCcLongPieces;
     const int32_t* d_live = d_ctl + 2;
     if (C) {
         H->chains.ensure((size_t)C * 10);
