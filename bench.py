@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # Stage boundaries recorded by the library on its stream (stage_ms[k] = events k -> k+1).  The
 # hot kernels (chain_solve, band_sort, winding_scan, mark_emit, composite, ...) sit alone between
 # two events, so these are their device times.
-STAGE_NAMES = ["transform", "count", "scans+sync", "chain_prep", "chain_solve", "piece_merge",
+STAGE_NAMES = ["transform", "count", "scans+sync", "chain_prep", "chain_solve", "piece_fragments",
                "gen_fragment", "bucket_scatter", "band_sort", "winding_scan", "mark_emit",
                "composite"]
 
@@ -43,16 +43,16 @@ WORKLOADS = {
 def stage_bytes(P, C, Np, N, R, w, h):
     """Algorithmic HBM bytes per stage: every input read once and every output written once, in
     the reference's data types (transform 20 B/point + 4 B/path; count 72 B/curve; intersection
-    write 72 B/curve + 8 B/intersection, split into the solve (curve data in, one t out) and the
-    merge (t in, (curve, t) out); fragment keys 24 B/intersection + 48 B/curve; sort 16 B/entry
-    (key and value in and out); scan 8 B/entry; mark + compaction 44 B/entry + 16 B/record;
-    composite 16 B/record + 4 B/pixel)."""
+    solve 48 B/curve of curve data in and one 4 B t out per intersection; piece merge fused with
+    the fragment keys: 72 B/curve of piece and curve data, the t in and key, winding change and
+    path out as int32, 16 B/intersection; sort 16 B/entry (key and value in and out); scan
+    8 B/entry; mark + compaction 44 B/entry + 16 B/record; composite 16 B/record + 4 B/pixel).
+    The gen_fragment stage only runs for injected intersections."""
     return {
         "transform": 20 * P + 4 * Np,
         "count": 72 * C,
         "chain_solve": 48 * C + 4 * N,
-        "piece_merge": 24 * C + 4 * N + 8 * N,
-        "gen_fragment": 24 * N + 48 * C,
+        "piece_fragments": 72 * C + 16 * N,
         "band_sort": 16 * N,
         "winding_scan": 8 * N,
         "mark_emit": 44 * N + 16 * R,

@@ -77,7 +77,7 @@ enum {
     SPR_STAGE_VISIBLE = 1,     /* int32 x Np: per-path outcode mask */
     SPR_STAGE_CUTS = 2,        /* float x 5C: cut cache (4 cuts + n_cuts bits) */
     SPR_STAGE_COUNTS = 3,      /* int32 x (C+1): exclusive-scanned per-curve counts */
-    SPR_STAGE_INTER = 4,       /* int2 x N: (curve, t bits) intersections */
+    SPR_STAGE_INTER = 4,       /* int2 x N: (curve, t bits) intersections (debug) */
     SPR_STAGE_KEY = 5,         /* int32 x N: cell keys in intersection order */
     SPR_STAGE_WN = 6,          /* int32 x N: winding change in intersection order */
     SPR_STAGE_SEG = 7,         /* int32 x (Np+1): per-path segment starts */

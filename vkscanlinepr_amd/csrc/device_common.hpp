@@ -242,41 +242,64 @@ __device__ inline uint32_t bucket_local(int32_t key, int2 pr, int32_t* err) {
     return (uint32_t)l;
 }
 
-// Wave-aggregated atomicAdd on keyed counters: lanes sharing a key take consecutive slots.  The
-// groups are found with ballots first, then every group leader issues its one atomic at the same
-// time, so a wave waits for a single atomic round trip.  Every lane of the wave must call it.
-__device__ inline int32_t wave_slot(int32_t* __restrict__ cnt, uint32_t key, bool valid) {
+// Runs of equal keys among consecutive valid lanes of a wave: a lane heads a run when it is
+// valid and the lane below it is invalid or holds another key.  Returns the ballot of heads.
+__device__ inline unsigned long long run_heads(uint32_t key, bool valid, unsigned long long vmask) {
     const int lane = threadIdx.x & 63;
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    unsigned long long todo = __ballot(valid);
-    int leader = lane, count = 0, rank = 0;
-    while (todo) {
-        const int l = __ffsll((long long)todo) - 1;
-        const uint32_t k = __shfl(key, l);
-        const unsigned long long m = __ballot(valid && key == k);
-        todo &= ~m;
-        if (valid && key == k) { leader = l; count = (int)__popcll(m); rank = (int)__popcll(m & lt); }
-    }
-    int32_t base = 0;
-    if (valid && leader == lane) base = atomicAdd(&cnt[key], count);
-    base = __shfl(base, leader);
-    return base + rank;
+    const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+    const bool prev_valid = lane > 0 && ((vmask >> (lane - 1)) & 1ull);
+    return __ballot(valid && (!prev_valid || prev != key));
 }
 
-// Same grouping for counters whose result is not needed: every group leader adds its group's
-// count in one atomic instruction for the whole wave.
+// Length of the run headed by `lane`: up to the next head or the next invalid lane.
+__device__ inline int run_length(int lane, unsigned long long heads, unsigned long long vmask) {
+    const unsigned long long stops = (heads | ~vmask) & ~((2ull << lane) - 1ull);
+    return (stops ? __ffsll((long long)stops) - 1 : 64) - lane;
+}
+
+// Wave-aggregated atomicAdd on keyed counters: every run of equal keys takes consecutive slots
+// with one atomic issued by its head, all heads at once (entries come in runs: consecutive
+// crossings of a curve mostly share their cell row).  Slots of one key are handed out in no
+// particular order.  Every lane of the wave must call it.
+__device__ inline int32_t wave_slot(int32_t* __restrict__ cnt, uint32_t key, bool valid) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long vmask = __ballot(valid);
+    const unsigned long long heads = run_heads(key, valid, vmask);
+    int32_t base = 0;
+    if ((heads >> lane) & 1ull) base = atomicAdd(&cnt[key], run_length(lane, heads, vmask));
+    const unsigned long long mine = heads & ((2ull << lane) - 1ull);
+    const int h = (valid && mine) ? 63 - __clzll((long long)mine) : lane;
+    base = __shfl(base, h);
+    return base + (lane - h);
+}
+
+// Same runs for counters whose result is not needed.
 __device__ inline void wave_count(int32_t* __restrict__ cnt, uint32_t key, bool valid) {
     const int lane = threadIdx.x & 63;
-    unsigned long long todo = __ballot(valid);
-    int count = 0;
-    while (todo) {
-        const int l = __ffsll((long long)todo) - 1;
-        const uint32_t k = __shfl(key, l);
-        const unsigned long long m = __ballot(valid && key == k);
-        todo &= ~m;
-        if (lane == l) count = (int)__popcll(m);
-    }
-    if (count) atomicAdd(&cnt[key], count);
+    const unsigned long long vmask = __ballot(valid);
+    const unsigned long long heads = run_heads(key, valid, vmask);
+    if ((heads >> lane) & 1ull) atomicAdd(&cnt[key], run_length(lane, heads, vmask));
+}
+
+// gen_fragment.comp:90-224: cell key of the curve segment [t0, t1] (the key of its midpoint's
+// cell, or the off-canvas sentinel when t0 >= t1 or the cell row is off the canvas) and its
+// winding change across the cell's centre row.
+__device__ inline void segment_cell(const CurvePts& cv, float t0, float t1, int w, int h, uint32_t& yx, int& swn) {
+    yx = kSentinelKey;
+    swn = 0;
+    if (!(t0 < t1)) return;
+    const float pfx = cv.ex(t0), pfy = cv.ey(t0), plx = cv.ex(t1), ply = cv.ey(t1);
+    const int raw_x = wmul(f2i_rd(((pfx + plx) * 0.5f) / (float)kFrag), kFrag);
+    const int raw_y = wmul(f2i_rd(((pfy + ply) * 0.5f) / (float)kFrag), kFrag);
+    const int pos_x = min(max(raw_x, -kFrag), (w & ~1) + kFrag);
+    const int pos_y = min(max(raw_y, -kFrag), (h & ~1) + kFrag);
+    const uint32_t y_shift = (uint32_t)(pos_y + 0x7FFF) << 16;
+    const uint32_t x_shift = (uint32_t)(pos_x + 0x7FFF) & 0xFFFFu;
+    if ((uint32_t)raw_y < (uint32_t)h) yx = y_shift | x_shift;
+    const float fwy = (float)(pos_y + 1);
+    if (pfy == ply) swn = 0;
+    else if (pfy < fwy && fwy <= ply) swn = -1;
+    else if (ply < fwy && fwy <= pfy) swn = 1;
 }
 
 }  // namespace spr

@@ -19,11 +19,13 @@
 //     length.  The first item of every wave is its global wave index (the first waves of a
 //     launch sit on distinct SIMDs, so the costliest items do not share one), later items come
 //     from an atomic counter; issue priority follows the item's cost.
-//  4. k_piece_merge: one wave per piece stages its two chains in LDS, merges them with its start
-//     in the reference's order (merge-path when both are non-decreasing, a concatenation when
-//     both are non-increasing, the sequential walk otherwise), applies the reference's equal-t
-//     OR-merge there, and writes the piece's (curve, t bits) entries once, coalesced.  Pieces of
-//     more than 1024 entries go to k_piece_merge_wave, which does the same with more LDS.
+//  4. k_piece_fragments: one wave per piece stages its two chains in LDS, merges them with its
+//     start in the reference's order (merge-path when both are non-decreasing, a concatenation
+//     when both are non-increasing, the sequential walk otherwise) and applies the reference's
+//     equal-t OR-merge there.  The same wave then produces the fragment key, winding change and
+//     bucket count of every entry (gen_fragment: the segment from an entry to the next one of
+//     its curve), so the entries themselves are only stored for export.  Pieces of more than
+//     1024 entries go to k_piece_fragments_long, which does the same with more LDS.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -116,6 +118,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
             const int limit = ends ? ends[c] : offsets[c + 1];
             if (base + pc.G.nx + pc.G.ny + 1 <= limit) {  // count and write passes agree
                 pd.base = base; pd.nx = pc.G.nx; pd.ny = pc.G.ny; pd.curve = (int32_t)c; pd.t0 = pc.t0;
+                // the last piece's last segment runs to t = 1; the others run to the next
+                // piece's first entry
+                pd.tend = (uint32_t)piece == ncuts[c] ? 1.0f : -1.0f;
 #pragma unroll
                 for (int side = 0; side < 2; ++side) {
                     const int n = side ? pc.G.ny : pc.G.nx;
@@ -414,23 +419,76 @@ __device__ inline uint32_t or_merged(const uint32_t* v, int n, int o) {
     return acc;
 }
 
-constexpr int kWaveStage = 1024;  // pieces up to this many entries are merged by k_piece_merge
+constexpr int kWaveStage = 1024;  // pieces up to this many entries are merged by k_piece_fragments
+
+// Masked parameter of the entry that follows the last entry of piece `slot`: t = 1 after the
+// curve's last piece, else the first entry of the next piece.  The merges of
+// make_intersection_1.comp:310-359 all start with the smaller of X[0] and the piece start when
+// the piece has crossings on both axes (ties to x), and with the piece start otherwise; a
+// piece that runs backwards in t (unsorted cuts) can therefore start with X[0] < start.
+__device__ inline float piece_end_t(const PieceDesc* __restrict__ pieces, int64_t slot, const PieceDesc& pd,
+                                    const float* __restrict__ chain_t) {
+    if (pd.tend >= 0.0f) return pd.tend;
+    const PieceDesc nd = pieces[slot + 1];
+    if (nd.base < 0) return 1.0f;  // no next piece is written (counts and writes disagree)
+    float first = nd.t0;
+    if (nd.nx > 0 && nd.ny > 0) {
+        const float x0 = chain_t[nd.base + 1];
+        if (x0 <= nd.t0) first = x0;
+    }
+    return glmax(bitsf(fbits(first) & 0xFFFFFFFCu), 0.0f);
+}
+
+// gen_fragment for the entries of one piece, fin[0..n) = their final t bits: entry o's segment
+// runs to entry o+1, the last one's to tend.  Keys, winding changes and paths are written
+// coalesced; bucket counts are aggregated over runs of the wave.  Every lane of the wave calls it.
+__device__ inline void piece_fragments(const PieceDesc& pd, float tend, const uint32_t* fin, int n,
+                                       const FragArgs& A, int lane) {
+    const uint32_t c = (uint32_t)pd.curve;
+    const uint32_t path = A.cpath[c];
+    int w = A.w, h = A.h;
+    if (A.pcanvas) { const int2 cv = A.pcanvas[path]; w = cv.x; h = cv.y; }
+    const CurvePts cv = load_curve(A.tpos, A.cpos[c], A.ctype[c]);
+    const uint32_t bb = (uint32_t)A.bucket_base[path];
+    const int2 pr = A.prow[path];
+    for (int o0 = 0; o0 < n; o0 += 64) {
+        const int o = o0 + lane;
+        const bool valid = o < n;
+        uint32_t bucket = 0;
+        if (valid) {
+            const uint32_t f = fin[o];
+            const float t0 = glmax(bitsf(f & 0xFFFFFFFCu), 0.0f);
+            const float t1 = o + 1 < n ? glmax(bitsf(fin[o + 1] & 0xFFFFFFFCu), 0.0f) : tend;
+            uint32_t yx;
+            int swn;
+            segment_cell(cv, t0, t1, w, h, yx, swn);
+            const int64_t i = (int64_t)pd.base + o;
+            A.key[i] = (int32_t)yx;
+            A.wn[i] = (int8_t)swn;
+            A.epath[i] = path;
+            if (A.inter) A.inter[i] = make_int2(pd.curve, (int32_t)f);
+            bucket = bb + bucket_local((int32_t)yx, pr, A.err);
+        }
+        wave_count(A.hist, bucket, valid);
+    }
+}
 
 // One wave per piece, four waves per workgroup, pieces taken in slot order with a stride (the
 // outputs of neighbouring pieces are neighbours in memory).  The piece's chains are staged in
-// the wave's LDS, merged, OR-merged, and written once as (curve, t bits) entries.  Longer pieces
-// are queued for k_piece_merge_wave.
-__global__ __launch_bounds__(kBlock) void k_piece_merge(const PieceDesc* __restrict__ pieces, int64_t n_slots,
-                                                        const uint8_t* __restrict__ mono,
-                                                        const float* __restrict__ chain_t,
-                                                        int2* __restrict__ inter, int32_t* __restrict__ long_list,
-                                                        int32_t* __restrict__ long_count,
-                                                        uint8_t* __restrict__ pieces_seq) {
+// the wave's LDS, merged, OR-merged, and turned into fragments.  Longer pieces are queued for
+// k_piece_fragments_long.
+__global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __restrict__ pieces, int64_t n_slots,
+                                                            const uint8_t* __restrict__ mono,
+                                                            const float* __restrict__ chain_t,
+                                                            int32_t* __restrict__ long_list,
+                                                            int32_t* __restrict__ long_count,
+                                                            uint8_t* __restrict__ pieces_seq, const FragArgs A) {
     __shared__ float s_in[kBlock / 64][kWaveStage];
     __shared__ uint32_t s_out[kBlock / 64][kWaveStage];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* xin = s_in[wave];
     uint32_t* vout = s_out[wave];
+    uint32_t* fin = reinterpret_cast<uint32_t*>(xin);  // the staged chains are dead after the merge
     const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
     for (int64_t slot = (int64_t)blockIdx.x * (kBlock / 64) + wave; slot < n_slots; slot += stride) {
         const PieceDesc pd = pieces[slot];
@@ -448,21 +506,24 @@ __global__ __launch_bounds__(kBlock) void k_piece_merge(const PieceDesc* __restr
         wave_sync();
         wave_merge(pd, mode, xin, vout, lane);
         wave_sync();
-        for (int o = lane; o < n; o += 64) inter[pd.base + o] = make_int2(pd.curve, (int32_t)or_merged(vout, n, o));
+        const float tend = piece_end_t(pieces, slot, pd, chain_t);
+        for (int o = lane; o < n; o += 64) fin[o] = or_merged(vout, n, o);
+        wave_sync();
+        piece_fragments(pd, tend, fin, n, A, lane);
         wave_sync();
     }
 }
 
 // One wave per long piece (more than kWaveStage entries).  Pieces up to kStage entries are
-// staged in LDS and finished there like in k_piece_merge; longer ones are merged into the raw
-// value buffer in global memory, then OR-merged from it by the same wave.
-__global__ __launch_bounds__(64) void k_piece_merge_wave(const PieceDesc* __restrict__ pieces,
-                                                         const int32_t* __restrict__ long_list,
-                                                         const int32_t* __restrict__ long_count,
-                                                         const uint8_t* __restrict__ piece_mode,
-                                                         const float* __restrict__ chain_t,
-                                                         uint32_t* __restrict__ raw,
-                                                         int2* __restrict__ inter) {
+// staged in LDS and finished there like in k_piece_fragments; longer ones are merged into
+// scratch[base..] in global memory and OR-merged into scratch[n_entries + base..].
+__global__ __launch_bounds__(64) void k_piece_fragments_long(const PieceDesc* __restrict__ pieces,
+                                                             const int32_t* __restrict__ long_list,
+                                                             const int32_t* __restrict__ long_count,
+                                                             const uint8_t* __restrict__ piece_mode,
+                                                             const float* __restrict__ chain_t,
+                                                             uint32_t* __restrict__ scratch, int64_t n_entries,
+                                                             const FragArgs A) {
     __shared__ float st_in[kStage];
     __shared__ uint32_t st_out[kStage + 1];
     const int lane = threadIdx.x;
@@ -479,11 +540,16 @@ __global__ __launch_bounds__(64) void k_piece_merge_wave(const PieceDesc* __rest
             __syncthreads();
             X = st_in;
         }
-        uint32_t* vout = lds ? st_out : raw + pd.base;
+        uint32_t* vout = lds ? st_out : scratch + pd.base;
         wave_merge(pd, mode, X, vout, lane);
         __threadfence_block();
         __syncthreads();
-        for (int o = lane; o < n; o += 64) inter[pd.base + o] = make_int2(pd.curve, (int32_t)or_merged(vout, n, o));
+        uint32_t* fin = lds ? reinterpret_cast<uint32_t*>(st_in) : scratch + n_entries + pd.base;
+        const float tend = piece_end_t(pieces, pi, pd, chain_t);
+        for (int o = lane; o < n; o += 64) fin[o] = or_merged(vout, n, o);
+        __threadfence_block();
+        __syncthreads();
+        piece_fragments(pd, tend, fin, n, A, lane);
         __syncthreads();
     }
 }
@@ -519,14 +585,14 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const Ch
     return hipGetLastError();
 }
 
-hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                              uint32_t* raw, int2* inter, int32_t* long_list, int32_t* long_count,
-                              uint8_t* pieces_seq, int n_cu, hipStream_t s) {
+hipError_t launch_piece_fragments(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                                  uint32_t* scratch, int64_t n_entries, int32_t* long_list, int32_t* long_count,
+                                  uint8_t* pieces_seq, const FragArgs& A, int n_cu, hipStream_t s) {
     if (C == 0) return hipSuccess;
-    k_piece_merge<<<(unsigned)(n_cu * 16), kBlock, 0, s>>>(pieces, (int64_t)C * 5, mono, chain_t, inter, long_list,
-                                                           long_count, pieces_seq);
-    k_piece_merge_wave<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, long_list, long_count, pieces_seq, chain_t, raw,
-                                                           inter);
+    k_piece_fragments<<<(unsigned)(n_cu * 16), kBlock, 0, s>>>(pieces, (int64_t)C * 5, mono, chain_t, long_list,
+                                                               long_count, pieces_seq, A);
+    k_piece_fragments_long<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, long_list, long_count, pieces_seq, chain_t,
+                                                               scratch, n_entries, A);
     return hipGetLastError();
 }
 

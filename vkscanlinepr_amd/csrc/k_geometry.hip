@@ -158,24 +158,11 @@ __device__ inline uint32_t gen_one(const int2* __restrict__ inter, int64_t N, in
     float t1 = glmax(bitsf((uint32_t)b.y & 0xFFFFFFFCu), 0.0f);
     const uint32_t c = (uint32_t)a.x;
     const uint32_t path = cpath[c];
-    uint32_t yx = kSentinelKey;
     if (a.x != b.x) t1 = 1.0f;
-    int swn = 0;
-    if (t0 < t1) {
-        const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
-        const float pfx = cv.ex(t0), pfy = cv.ey(t0), plx = cv.ex(t1), ply = cv.ey(t1);
-        const int raw_x = wmul(f2i_rd(((pfx + plx) * 0.5f) / (float)kFrag), kFrag);
-        const int raw_y = wmul(f2i_rd(((pfy + ply) * 0.5f) / (float)kFrag), kFrag);
-        const int pos_x = min(max(raw_x, -kFrag), (w & ~1) + kFrag);
-        const int pos_y = min(max(raw_y, -kFrag), (h & ~1) + kFrag);
-        const uint32_t y_shift = (uint32_t)(pos_y + 0x7FFF) << 16;
-        const uint32_t x_shift = (uint32_t)(pos_x + 0x7FFF) & 0xFFFFu;
-        if ((uint32_t)raw_y < (uint32_t)h) yx = y_shift | x_shift;
-        const float fwy = (float)(pos_y + 1);
-        if (pfy == ply) swn = 0;
-        else if (pfy < fwy && fwy <= ply) swn = -1;
-        else if (ply < fwy && fwy <= pfy) swn = 1;
-    }
+    uint32_t yx;
+    int swn;
+    if (t0 < t1) segment_cell(load_curve(tpos, cpos[c], ctype[c]), t0, t1, w, h, yx, swn);
+    else { yx = kSentinelKey; swn = 0; }
     key[i] = (int32_t)yx;
     wn[i] = (int8_t)swn;
     epath[i] = path;

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_learned_count(
     const int32_t* __restrict__ arena, const int32_t* __restrict__ cap_n, int64_t budget,
     float4* __restrict__ cuts, uint32_t* __restrict__ ncuts, int32_t* __restrict__ counts,
     int32_t* __restrict__ ends, int32_t* __restrict__ ctl, int32_t* __restrict__ n_exact, int32_t* __restrict__ pbase,
-    int2* __restrict__ inter, int8_t* __restrict__ mark, int32_t* __restrict__ err,
+    int2* __restrict__ inter, int8_t* __restrict__ mark, uint32_t* __restrict__ epath, int32_t* __restrict__ err,
     const int2* __restrict__ pcanvas) {
     __shared__ int32_t s_w[kWaves];
     __shared__ int64_t s_base;
@@ -302,10 +302,10 @@ __global__ __launch_bounds__(256) void k_learned_count(
     // gaps: the unused arena tail (the whole arena after an overflow), and the part of a
     // dropped path's pool block that lies inside the budget
     const int64_t g0 = mode == 0 ? A + n : A, g1 = min(A + cap, budget);
-    for (int64_t i = g0 + threadIdx.x; i < g1; i += 256) { inter[i] = make_int2(-1, 0); mark[i] = 1; }
+    for (int64_t i = g0 + threadIdx.x; i < g1; i += 256) { inter[i] = make_int2(-1, 0); mark[i] = 1; epath[i] = kNoPath; }
     if (mode == 2) {
         const int64_t e = min(base + n, budget);
-        for (int64_t i = base + threadIdx.x; i < e; i += 256) { inter[i] = make_int2(-1, 0); mark[i] = 1; }
+        for (int64_t i = base + threadIdx.x; i < e; i += 256) { inter[i] = make_int2(-1, 0); mark[i] = 1; epath[i] = kNoPath; }
     }
 }
 
@@ -445,10 +445,11 @@ hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const 
                                 const uint32_t* pcurve, uint32_t Np, int w, int h, const int32_t* arena,
                                 const int32_t* cap_n, int64_t budget, float4* cuts, uint32_t* ncuts,
                                 int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact, int32_t* pbase,
-                                int2* inter, int8_t* mark, int32_t* err, hipStream_t s, const int2* pcanvas) {
+                                int2* inter, int8_t* mark, uint32_t* epath, int32_t* err, hipStream_t s,
+                                const int2* pcanvas) {
     if (Np == 0) return hipSuccess;
     k_learned_count<<<Np, 256, 0, s>>>(tpos, cpos, ctype, visible, pcurve, Np, w, h, arena, cap_n, budget, cuts,
-                                       ncuts, counts, ends, ctl, n_exact, pbase, inter, mark, err, pcanvas);
+                                       ncuts, counts, ends, ctl, n_exact, pbase, inter, mark, epath, err, pcanvas);
     k_learned_finish<<<1, 1, 0, s>>>(arena, Np, budget, ctl);
     return hipGetLastError();
 }

@@ -43,7 +43,25 @@ struct ChainDesc {
 };
 struct PieceDesc {
     int32_t base, nx, ny, curve;  // base = -1: no piece in this slot
-    float t0;
+    float t0;                     // flagged start parameter
+    float tend;                   // 1 for the curve's last piece, -1 otherwise (the next piece's first entry)
+};
+// Inputs and outputs of the fragment keys produced from the merged entries of each piece.
+struct FragArgs {
+    const uint32_t* cpos;
+    const uint32_t* ctype;
+    const uint32_t* cpath;
+    const float2* tpos;
+    int w, h;
+    const int2* pcanvas;        // per-path canvas of a batch, or null
+    int32_t* key;
+    int8_t* wn;
+    uint32_t* epath;
+    const int32_t* bucket_base;
+    const int2* prow;
+    int32_t* hist;              // per-bucket entry counts
+    int32_t* err;
+    int2* inter;                // (curve, t bits) entries for export, or null
 };
 // Chain length classes: cubic chains by quarter octave of their length (16 octaves x 4), then
 // line chains by octave.  The chain control block holds, as int32: class counts, class fill
@@ -63,11 +81,13 @@ hipError_t launch_chain_order(const ChainDesc* chains, uint32_t C, const int32_t
 hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
                               const int32_t* order, const int32_t* class_off, const int32_t* n_long,
                               int32_t* work, float* chain_t, uint8_t* mono, int n_cu, hipStream_t s);
-// Merges each piece's x and y chains into (curve, t bits) entries, OR-merging equal t (the
-// reference's per-piece walk); raw is scratch for pieces too long for LDS.
-hipError_t launch_piece_merge(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                              uint32_t* raw, int2* inter, int32_t* long_list, int32_t* long_count,
-                              uint8_t* pieces_seq, int n_cu, hipStream_t s);
+// Merges each piece's x and y chains into its (curve, t bits) entries, OR-merging equal t (the
+// reference's per-piece walk), and turns the entries into fragment keys, winding changes and
+// bucket counts (gen_fragment) without storing the entries unless A.inter is set.  scratch
+// holds 2 * n_entries words for pieces too long for LDS.
+hipError_t launch_piece_fragments(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                                  uint32_t* scratch, int64_t n_entries, int32_t* long_list, int32_t* long_count,
+                                  uint8_t* pieces_seq, const FragArgs& A, int n_cu, hipStream_t s);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
@@ -138,7 +158,7 @@ hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const 
                                 const uint32_t* pcurve, uint32_t Np, int w, int h, const int32_t* arena,
                                 const int32_t* cap_n, int64_t budget, float4* cuts, uint32_t* ncuts,
                                 int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact, int32_t* pbase,
-                                int2* inter, int8_t* mark, int32_t* err, hipStream_t s,
+                                int2* inter, int8_t* mark, uint32_t* epath, int32_t* err, hipStream_t s,
                                 const int2* pcanvas = nullptr);
 hipError_t launch_sorted_starts(const int32_t* boff, const int32_t* bucket_base, uint32_t Np, int32_t* sstart,
                                 hipStream_t s);

@@ -330,14 +330,19 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     }
     stage_mark(H, 5);
     if (gen) {
-        // very long pieces merge through the key buffer, which gen_fragment fills afterwards
+        // piece merge and fragment keys in one pass; very long pieces merge through the sort
+        // buffer, which is free until the scatter
         SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
-        SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->key.p, H->inter.p,
-                                        H->long_list.p, d_nlpieces, H->pieces_seq.p, H->n_cu, s));
+        const spr::FragArgs A{H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h, nullptr,
+                              H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err,
+                              debug ? H->inter.p : nullptr};
+        SPR_HIP(spr::launch_piece_fragments(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p,
+                                            (int64_t)n1, H->long_list.p, d_nlpieces, H->pieces_seq.p, A, H->n_cu, s));
     }
     stage_mark(H, 6);
-    SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
-                                     H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err, s));
+    if (!gen)  // injected entries (or none)
+        SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
+                                         H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err, s));
     stage_mark(H, 7);
     // ---- sort: bucket offsets, scatter into bucket order, band sort in LDS
     if (N > 0) {
@@ -531,8 +536,8 @@ void run_learned_frame(spr_handle* H) {
     // phase 2: exact counts placed in the arenas, then the plain stages on device counts
     SPR_HIP(spr::launch_learned_count(H->tpos.p, H->cpos.p, H->ctype.p, H->visible.p, H->pcurve.p, Np, w, h,
                                       H->arena.p, H->capn.p, H->lb_n, H->cuts.p, H->ncuts.p, H->counts.p,
-                                      H->cend.p, d_ctl, H->nexact.p, H->pbase.p, H->inter.p, H->wn.p, d_err, s,
-                                      pcv));
+                                      H->cend.p, d_ctl, H->nexact.p, H->pbase.p, H->inter.p, H->wn.p, H->epath.p,
+                                      d_err, s, pcv));
     stage_mark(H, 3);
     int32_t* d_cc = reinterpret_cast<int32_t*>(H->zero.p + z.chainctl);
     int32_t* d_cfill = d_cc + spr::kCcFill;
@@ -561,13 +566,14 @@ void run_learned_frame(spr_handle* H) {
     stage_mark(H, 5);
     if (C) {
         SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
-        SPR_HIP(spr::launch_piece_merge(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->key.p, H->inter.p,
-                                        H->long_list.p, d_nlpieces, H->pieces_seq.p, H->n_cu * H->merge_k / 8, s));
+        const spr::FragArgs A{H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h, pcv,
+                              H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, H->hist_l.p, d_err,
+                              debug ? H->inter.p : nullptr};
+        SPR_HIP(spr::launch_piece_fragments(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p, nl,
+                                            H->long_list.p, d_nlpieces, H->pieces_seq.p, A,
+                                            H->n_cu * H->merge_k / 8, s));
     }
     stage_mark(H, 6);
-    SPR_HIP(spr::launch_gen_fragment(H->inter.p, nl, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
-                                     H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, H->hist_l.p, d_err,
-                                     s, d_live, pcv));
     stage_mark(H, 7);
     SPR_HIP(spr::launch_scan_i32(H->hist_l.p, H->hist_l.p, nb_budget, ctl_at(H, z.st_hist, z.counters, 2), s,
                                  d_nb, d_nsorted));
@@ -981,7 +987,11 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 break;
             }
             case SPR_STAGE_COUNTS: from_dev(H->counts.p, ((size_t)H->C + 1) * 4); break;
-            case SPR_STAGE_INTER: from_dev(H->inter.p, (size_t)NP * 8); break;
+            case SPR_STAGE_INTER:
+                // the merged entries go straight into fragment keys unless they are kept
+                if (!debug && !H->inj) throw Fail{SPR_ERR_STATE, "intersections are kept only with the debug option"};
+                from_dev(H->inter.p, (size_t)NP * 8);
+                break;
             case SPR_STAGE_KEY: from_dev(H->key.p, (size_t)NP * 4); break;
             case SPR_STAGE_WN: {
                 std::vector<int8_t> v((size_t)NP);
