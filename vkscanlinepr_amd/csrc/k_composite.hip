@@ -7,7 +7,9 @@
 // is a stack of independent two-pixel "cell rows".
 //
 // One workgroup owns one cell row, in column chunks of 4096; each thread keeps 16 consecutive
-// pixels of the chunk in registers.  The (path, row) bucket table gives each path's records on
+// pixels of the chunk in registers.  The per-column event words live in LDS column-interleaved
+// (column c at (c % 16) * 256 + c / 16), so the 64 lanes of a wave touching "their" j-th column
+// hit consecutive words instead of one bank.  The (path, row) bucket table gives each path's records on
 // this row as one contiguous range of non-overlapping compact records (x | width << 16) with one
 // colour.  Paths are taken in order, 64 at a time: a record toggles its path's bit at its first
 // column and at its end column in a 64-bit event word per column (LDS), and a prefix XOR over
@@ -28,6 +30,10 @@ constexpr int kChunk = kBlock * kPix;      // columns per pass
 constexpr int kBatch = 64;                 // paths per coverage mask
 constexpr int kUnroll = 4;                 // records loaded per thread before applying
 typedef unsigned long long u64;
+
+// LDS slot of chunk column c (0 <= c < kChunk): thread t's column t * kPix + j sits at
+// j * kBlock + t.
+__device__ inline int ev_slot(int c) { return (c % kPix) * kBlock + c / kPix; }
 
 __device__ inline u64 shfl_up_u64(u64 v, int d) {
     const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
@@ -62,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                                       int w, int h, int64_t rec_cap,
                                                       uint8_t* __restrict__ image,
                                                       const DocDev* __restrict__ docs, int n_docs) {
-    __shared__ u64 ev[kChunk + 1];
+    __shared__ u64 ev[kChunk];
     __shared__ int2 list[kBlock];
     __shared__ uint32_t lcol[kBlock];
     __shared__ float4 lsa[kBlock];   // (r*a, g*a, b*a, a) of each listed path, as the blend uses them
@@ -137,8 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 const int qn = n - q0 < kBatch ? n - q0 : kBatch;
                 // clear the event words; the batch's records as one flat index space
 #pragma unroll
-                for (int j = 0; j < kPix; ++j) ev[xb + j] = 0ull;
-                if (tid == 0) ev[kChunk] = 0ull;
+                for (int j = 0; j < kPix; ++j) ev[j * kBlock + tid] = 0ull;
                 if (wave == 0) {
                     const int2 rg = lane < qn ? list[q0 + lane] : make_int2(0, 0);
                     int inc = rg.y - rg.x;
@@ -178,8 +183,8 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                         xs = xs < 0 ? 0 : xs;
                         xe = xe > cw ? cw : xe;
                         if (xs < xe) {
-                            atomicXor(&ev[xs], bit);
-                            atomicXor(&ev[xe], bit);  // ev[cw] only collects ends past the chunk
+                            atomicXor(&ev[ev_slot(xs)], bit);
+                            if (xe < cw) atomicXor(&ev[ev_slot(xe)], bit);  // ends past the chunk change nothing
                         }
                     }
                 }
@@ -187,11 +192,11 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 // coverage = prefix XOR of the events, then the fold in path order
                 u64 run = 0;
 #pragma unroll
-                for (int j = 0; j < kPix; ++j) run ^= ev[xb + j];
+                for (int j = 0; j < kPix; ++j) run ^= ev[j * kBlock + tid];
                 u64 cov = block_exclusive_xor(run, xlds);
 #pragma unroll
                 for (int j = 0; j < kPix; ++j) {
-                    cov ^= ev[xb + j];
+                    cov ^= ev[j * kBlock + tid];
                     u64 mm = cov;
                     if (mm == 0ull) continue;
                     if constexpr (COMPOSE == 0) {

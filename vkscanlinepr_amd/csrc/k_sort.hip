@@ -1,18 +1,25 @@
-// k_sort.hip -- sort of the cell keys inside each (path, row) band (gfx950).
+// k_sort.hip -- sort of the cell keys inside each (path, row) band, fused with the winding
+// scan (gfx950).
 //
 // Replaces naive_seg_sort_pairs.comp:26-97 (one workgroup per path running an O(n^2) odd-even
-// transposition network) and shuffle_fragment.comp:17-26.  The reference compares (key, index)
-// as signed int32 inside each path segment; the sort word here is
+// transposition network), shuffle_fragment.comp:17-26 and the winding-number scan of
+// scanline_rasterizer.cpp:478-506 (naive_scan.comp over the sorted winding changes).  The
+// reference compares (key, index) as signed int32 inside each path segment; the sort word here is
 //     ((key ^ 0x80000000) << 32) | (index << 2) | (winding change + 1)
 // whose unsigned order equals the reference's (key, index) order, is unique, and carries the
 // winding change, so the reference's separate gather pass disappears.
 //
 // After the bucket scatter every band (<= 2048 entries, a run of whole buckets of one path) only
 // needs sorting in place: one 256-thread workgroup loads it once, sorts it in LDS (8-element
-// register network per thread, then merge-path rounds) and stores it once.  A band above 2048
-// entries is a single oversized (path, row) bucket; such bands are queued and each is sorted by
-// one workgroup with LDS chunk sorts followed by merge-path passes through global memory.
+// register network per thread, then merge-path rounds) and stores it once.  Bands are taken in
+// array order (an atomic ticket), so the exclusive scan of the winding changes runs across them
+// with a decoupled look-back on each band's sum (a sum needs no sorting), and each band writes
+// its winding numbers next to its keys.  A band above 2048 entries is a single oversized (path,
+// row) bucket; such bands publish their sum at once and are queued, and each is then sorted by
+// one workgroup with LDS chunk sorts followed by merge-path passes through global memory, the
+// last pass writing keys and winding numbers.
 #include "kernels.hpp"
+#include "scan_common.hpp"
 
 namespace spr {
 
@@ -62,17 +69,19 @@ __device__ inline int64_t merge_split(PA A, int64_t na, PB B, int64_t nb, int64_
     return lo;
 }
 
-// Sorted output, unpacked from the sort words: key, winding change and (optionally) index.
+// Sorted output, unpacked from the sort words: key, (optionally) index, and the winding number
+// (exclusive prefix of the winding changes in sorted order).
 struct Unpacked {
     int32_t* skey;
     int32_t* sidx;
-    int32_t* swn;
+    int32_t* winding;
 };
 __device__ inline void put(const Unpacked& o, int64_t i, u64 w) {
     o.skey[i] = (int32_t)((uint32_t)(w >> 32) ^ 0x80000000u);
     if (o.sidx) o.sidx[i] = (int32_t)((w >> 2) & 0x3FFFFFFFull);
-    o.swn[i] = (int32_t)(w & 3ull) - 1;
 }
+__device__ inline int32_t wn_of(u64 w) { return (int32_t)(w & 3ull) - 1; }
+__device__ inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 
 // LDS slot of element i: one pad word per 8 elements, so the 8-element runs owned by
 // consecutive lanes start on different banks.
@@ -80,20 +89,33 @@ constexpr int kPadded = kSortTile + kSortTile / 8;
 __device__ inline int pad(int i) { return i + (i >> 3); }
 
 // Sorts n <= kSortTile words from src with one workgroup (buf: 2 x kPadded words of LDS) into dst
-// (sort words) or, when dst is null, unpacked into out at offset o0.
+// (sort words) or, when dst is null, unpacked into out at offset o0 with winding numbers from
+// the exclusive prefix `pre` (then `band` must be the band's ticket for the look-back and
+// `status` its status array: the band's sum is published before the sort).
 __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ dst, int n, u64 (*buf)[kPadded],
-                                 const Unpacked& out, int64_t o0) {
+                                 const Unpacked& out, int64_t o0, unsigned long long* status, int64_t band,
+                                 int32_t* s_w, int32_t* s_pre, bool last, int64_t total_at) {
     int P2 = kItems;
     while (P2 < n) P2 <<= 1;
     const int active = P2 / kItems;
     const int tid = threadIdx.x;
+    u64 v[kItems];
+    int32_t part = 0;
     if (tid < active) {
-        u64 v[kItems];
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const int i = tid * kItems + j;
             v[j] = i < n ? src[i] : kPad;
+            if (i < n) part += wn_of(v[j]);
         }
+    }
+    int32_t agg = 0;
+    if (!dst) {
+        // the band's sum is published at once; its exclusive prefix is looked up after the sort
+        block_exclusive_add(part, s_w, agg);
+        if (tid < 64) publish_aggregate<OpI32>(status, band, agg);
+    }
+    if (tid < active) {
         sort16(v);
 #pragma unroll
         for (int j = 0; j < kItems; ++j) buf[0][pad(tid * kItems + j)] = v[j];
@@ -128,121 +150,176 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
     }
     if (dst) {
         for (int i = tid; i < n; i += kThreads) dst[i] = buf[cur][pad(i)];
-    } else {
-        for (int i = tid; i < n; i += kThreads) put(out, o0 + i, buf[cur][pad(i)]);
+        __syncthreads();
+        return;
+    }
+    // exclusive prefix of the band (the predecessors have long published their sums)
+    if (tid < 64) {
+        const int32_t pre = lookback<OpI32>(status, band, agg, true);
+        if (tid == 0) {
+            *s_pre = pre;
+            if (last) out.winding[total_at] = wadd(pre, agg);  // the scan's total
+        }
+    }
+    __syncthreads();
+    // winding numbers: each thread scans its 8 consecutive sorted entries, the block scans the
+    // thread sums, and the results go through the free LDS buffer to coalesced stores
+    int32_t* wl = reinterpret_cast<int32_t*>(buf[cur ^ 1]);
+    int32_t w8[kItems], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = tid * kItems + k;
+        w8[k] = sum;
+        if (i < n) sum += wn_of(buf[cur][pad(i)]);
+    }
+    int32_t tot;
+    const int32_t ex = wadd(block_exclusive_add(sum, s_w, tot), *s_pre);
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int i = tid * kItems + k;
+        if (i < n) wl[i] = wadd(ex, w8[k]);
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += kThreads) {
+        put(out, o0 + i, buf[cur][pad(i)]);
+        out.winding[o0 + i] = wl[i];
     }
     __syncthreads();
 }
 
+// Persistent workgroups take the bands in array order from an atomic ticket (so every band they
+// wait on in the look-back is held by a running workgroup).  A band above one tile publishes its
+// sum (and the scan total when it is the last band) and is queued with its exclusive prefix.
 __global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restrict__ band_start,
                                                          const int32_t* __restrict__ n_bands,
                                                          const u64* __restrict__ src, Unpacked out,
                                                          int32_t* __restrict__ big_list,
-                                                         int32_t* __restrict__ big_count) {
+                                                         int32_t* __restrict__ big_excl,
+                                                         int32_t* __restrict__ big_count,
+                                                         unsigned long long* __restrict__ status,
+                                                         uint32_t* __restrict__ counter) {
     __shared__ u64 buf[2][kPadded];
-    const int32_t b = blockIdx.x;
-    if (b >= *n_bands) return;
-    const int64_t s = band_start[b], e = band_start[b + 1];
-    const int64_t n = e - s;
-    if (n <= 0) return;
-    if (n > kSortTile) {
-        if (threadIdx.x == 0) big_list[atomicAdd(big_count, 1)] = b;
-        return;
+    __shared__ int32_t s_tile, s_w[kThreads / 64], s_pre;
+    const int32_t nb = *n_bands;
+    while (true) {
+        const int64_t t = ordered_tile_id(counter, &s_tile);
+        if (t >= nb) {
+            if (t == 0 && threadIdx.x == 0) out.winding[0] = 0;  // no bands: the total is 0
+            return;
+        }
+        const int64_t s = band_start[t], e = band_start[t + 1];
+        const int64_t n = e - s;
+        const bool last = t == nb - 1;
+        if (n > kSortTile) {
+            int32_t part = 0;
+            for (int64_t i = threadIdx.x; i < n; i += kThreads) part += wn_of(src[s + i]);
+            int32_t agg;
+            block_exclusive_add(part, s_w, agg);
+            if (threadIdx.x < 64) {
+                const int32_t pre = lookback<OpI32>(status, t, agg);
+                if (threadIdx.x == 0) {
+                    const int32_t q = atomicAdd(big_count, 1);
+                    big_list[q] = (int32_t)t;
+                    big_excl[q] = pre;
+                    if (last) out.winding[e] = wadd(pre, agg);
+                }
+            }
+            continue;
+        }
+        tile_sort(src + s, nullptr, (int)n, buf, out, s, status, t, s_w, &s_pre, last, e);
     }
-    if (n == 1) {
-        if (threadIdx.x == 0) put(out, s, src[s]);
-        return;
-    }
-    tile_sort(src + s, nullptr, (int)n, buf, out, s);
 }
 
 // One workgroup per oversized band: LDS chunk sorts into `a`, then merge-path passes between
-// `a` and `b` (the scatter buffer, already consumed), the last pass writing `dst`.
+// `a` and `b` (the scatter buffer, already consumed), the last pass writing keys and winding
+// numbers (the band's exclusive prefix carried across its 2048-entry output blocks).
 __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __restrict__ band_start,
                                                              const int32_t* __restrict__ big_list,
+                                                             const int32_t* __restrict__ big_excl,
                                                              const int32_t* __restrict__ big_count,
                                                              u64* __restrict__ scatter, u64* __restrict__ a,
                                                              Unpacked out) {
     __shared__ u64 buf[2][kPadded];
+    __shared__ int32_t s_w[kThreads / 64];
     const int32_t nbig = *big_count;
     for (int32_t q = blockIdx.x; q < nbig; q += gridDim.x) {
         const int32_t band = big_list[q];
         const int64_t s = band_start[band], n = band_start[band + 1] - s;
         for (int64_t c = 0; c < n; c += kSortTile) {
             const int m = (int)(n - c < kSortTile ? n - c : kSortTile);
-            tile_sort(scatter + s + c, a + s + c, m, buf, out, 0);
+            tile_sort(scatter + s + c, a + s + c, m, buf, out, 0, nullptr, 0, s_w, nullptr, false, 0);
         }
         __threadfence_block();
         __syncthreads();
         u64* cur = a + s;
         u64* nxt = scatter + s;
+        int32_t carry = big_excl[q];
         for (int64_t r = kSortTile; r < n; r <<= 1) {
             const bool last_pass = 2 * r >= n;
-            u64* dstw = last_pass ? nullptr : nxt;
-            for (int64_t o = (int64_t)threadIdx.x * kItems; o < n; o += (int64_t)kThreads * kItems) {
-                const int64_t pb = (o / (2 * r)) * (2 * r);
-                const int64_t na = pb + r < n ? r : n - pb;
-                const int64_t nb = pb + r < n ? ((pb + 2 * r < n ? pb + 2 * r : n) - (pb + r)) : 0;
-                const u64* A = cur + pb;
-                const u64* B = A + na;
-                const int cnt = (int)(n - o < kItems ? n - o : kItems);
-                const int64_t diag = o - pb;
-                int64_t i = nb ? merge_split(A, na, B, nb, diag) : diag;
-                int64_t j = diag - i;
-                for (int k = 0; k < cnt; ++k) {
-                    const bool takeA = j >= nb || (i < na && A[i] <= B[j]);
-                    const u64 v = takeA ? A[i] : B[j];
-                    if (dstw) dstw[o + k] = v; else put(out, s + o + k, v);
-                    i += takeA; j += !takeA;
+            for (int64_t base = 0; base < n; base += (int64_t)kThreads * kItems) {
+                const int64_t o = base + (int64_t)threadIdx.x * kItems;
+                u64 vals[kItems];
+                int cnt = 0;
+                if (o < n) {
+                    const int64_t pb = (o / (2 * r)) * (2 * r);
+                    const int64_t na = pb + r < n ? r : n - pb;
+                    const int64_t nb = pb + r < n ? ((pb + 2 * r < n ? pb + 2 * r : n) - (pb + r)) : 0;
+                    const u64* A = cur + pb;
+                    const u64* B = A + na;
+                    cnt = (int)(n - o < kItems ? n - o : kItems);
+                    const int64_t diag = o - pb;
+                    int64_t i = nb ? merge_split(A, na, B, nb, diag) : diag;
+                    int64_t j = diag - i;
+#pragma unroll
+                    for (int k = 0; k < kItems; ++k) {
+                        if (k >= cnt) break;
+                        const bool takeA = j >= nb || (i < na && A[i] <= B[j]);
+                        vals[k] = takeA ? A[i] : B[j];
+                        i += takeA; j += !takeA;
+                    }
                 }
+                if (!last_pass) {
+                    for (int k = 0; k < cnt; ++k) nxt[o + k] = vals[k];
+                    continue;
+                }
+                int32_t local = 0;
+                for (int k = 0; k < cnt; ++k) local += wn_of(vals[k]);
+                int32_t tot;
+                int32_t run = wadd(carry, block_exclusive_add(local, s_w, tot));
+                for (int k = 0; k < cnt; ++k) {
+                    put(out, s + o + k, vals[k]);
+                    out.winding[s + o + k] = run;
+                    run = wadd(run, wn_of(vals[k]));
+                }
+                carry = wadd(carry, tot);
             }
             __threadfence_block();
             __syncthreads();
-            if (last_pass) { cur = nullptr; break; }
+            if (last_pass) break;
             u64* t = cur; cur = nxt; nxt = t;
         }
-        if (cur) {  // a single chunk: no merge pass ran
-            for (int64_t i = threadIdx.x; i < n; i += kThreads) put(out, s + i, cur[i]);
-            __syncthreads();
-        }
     }
-}
-
-__global__ void k_unpack_sorted(const u64* __restrict__ sorted, int64_t N_max, int32_t* __restrict__ skey,
-                                int32_t* __restrict__ sidx, int32_t* __restrict__ swn,
-                                const int32_t* __restrict__ n_dev) {
-    const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const u64 w = sorted[i];
-    skey[i] = (int32_t)((uint32_t)(w >> 32) ^ 0x80000000u);
-    if (sidx) sidx[i] = (int32_t)((w >> 2) & 0x3FFFFFFFull);
-    swn[i] = (int32_t)(w & 3ull) - 1;
 }
 
 }  // namespace
 
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
-                             const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* swn,
-                             int32_t* big_list, int32_t* big_count, hipStream_t s) {
+                             const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
+                             int32_t* big_list, int32_t* big_excl, int32_t* big_count, ScanCtl ctl, int n_cu,
+                             hipStream_t s) {
     if (max_bands <= 0) return hipSuccess;
-    k_sort_bands<<<(unsigned)max_bands, kThreads, 0, s>>>(band_start, n_bands, src, Unpacked{skey, sidx, swn},
-                                                           big_list, big_count);
+    // four workgroups fit in one CU's LDS
+    const int64_t grid = max_bands < (int64_t)n_cu * 4 ? max_bands : (int64_t)n_cu * 4;
+    k_sort_bands<<<(unsigned)grid, kThreads, 0, s>>>(band_start, n_bands, src, Unpacked{skey, sidx, winding},
+                                                      big_list, big_excl, big_count, ctl.status, ctl.counter);
     return hipGetLastError();
 }
 
-hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
-                                 unsigned long long* scatter, unsigned long long* tmp, int32_t* skey, int32_t* sidx,
-                                 int32_t* swn, hipStream_t s) {
-    k_sort_big_bands<<<64, kThreads, 0, s>>>(band_start, big_list, big_count, scatter, tmp,
-                                             Unpacked{skey, sidx, swn});
-    return hipGetLastError();
-}
-
-hipError_t launch_unpack_sorted(const unsigned long long* sorted, int64_t N, int32_t* skey,
-                                int32_t* sidx, int32_t* swn, hipStream_t s, const int32_t* n_dev) {
-    if (N == 0) return hipSuccess;
-    k_unpack_sorted<<<(unsigned)((N + 255) / 256), 256, 0, s>>>(sorted, N, skey, sidx, swn, n_dev);
+hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_excl,
+                                 const int32_t* big_count, unsigned long long* scatter, unsigned long long* tmp,
+                                 int32_t* skey, int32_t* sidx, int32_t* winding, hipStream_t s) {
+    k_sort_big_bands<<<64, kThreads, 0, s>>>(band_start, big_list, big_excl, big_count, scatter, tmp,
+                                             Unpacked{skey, sidx, winding});
     return hipGetLastError();
 }
 

@@ -113,14 +113,16 @@ hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32
                             int32_t* band_start, hipStream_t s, const int32_t* nb_dev = nullptr,
                             const int32_t* n_dev = nullptr);
 // The band sorts write the sorted keys, winding changes and (optionally) indices directly.
+// Sorts every band in array order and writes keys, (debug) indices and winding numbers (the
+// exclusive scan of the sorted winding changes, total at winding[n]); oversized bands are
+// queued (with their exclusive prefix) for launch_sort_big_bands.  ctl: >= max_bands statuses.
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
-                             const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* swn,
-                             int32_t* big_list, int32_t* big_count, hipStream_t s);
-hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
-                                 unsigned long long* scatter, unsigned long long* tmp, int32_t* skey, int32_t* sidx,
-                                 int32_t* swn, hipStream_t s);
-hipError_t launch_unpack_sorted(const unsigned long long* sorted, int64_t N, int32_t* skey,
-                                int32_t* sidx, int32_t* swn, hipStream_t s, const int32_t* n_dev = nullptr);
+                             const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
+                             int32_t* big_list, int32_t* big_excl, int32_t* big_count, ScanCtl ctl, int n_cu,
+                             hipStream_t s);
+hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_excl,
+                                 const int32_t* big_count, unsigned long long* scatter, unsigned long long* tmp,
+                                 int32_t* skey, int32_t* sidx, int32_t* winding, hipStream_t s);
 
 // ---- winding / mark / emit (a10 scan, a11); also writes each (path, row) bucket's record range
 hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const uint32_t* epath,

@@ -75,14 +75,15 @@ __device__ inline int64_t ordered_tile_id(uint32_t* counter, int32_t* lds_slot) 
 // Publishes the tile's aggregate, walks back over its predecessors 64 at a time, publishes the
 // inclusive prefix and returns the exclusive prefix (valid in every lane of wave 0).
 template <class Op>
-__device__ inline typename Op::T lookback(unsigned long long* status, int64_t tile, typename Op::T agg) {
+__device__ inline typename Op::T lookback(unsigned long long* status, int64_t tile, typename Op::T agg,
+                                         bool published = false) {
     using T = typename Op::T;
     const int lane = threadIdx.x & 63;
     if (tile == 0) {
-        if (lane == 0) store_status(&status[0], Op::encode(kStInclusive, agg));
+        if (lane == 0 && !published) store_status(&status[0], Op::encode(kStInclusive, agg));
         return Op::zero();
     }
-    if (lane == 0) store_status(&status[tile], Op::encode(kStAggregate, agg));
+    if (lane == 0 && !published) store_status(&status[tile], Op::encode(kStAggregate, agg));
     T excl = Op::zero();
     int64_t pred = tile - 1;
     while (true) {
@@ -106,6 +107,15 @@ __device__ inline typename Op::T lookback(unsigned long long* status, int64_t ti
     }
     if (lane == 0) store_status(&status[tile], Op::encode(kStInclusive, Op::add(excl, agg)));
     return excl;
+}
+
+// The first half of lookback, for tiles that have other work before they need their prefix:
+// publishes the tile's aggregate (tile 0: its inclusive prefix) right away.  The later
+// lookback(..., published = true) walks back and publishes the inclusive prefix.
+template <class Op>
+__device__ inline void publish_aggregate(unsigned long long* status, int64_t tile, typename Op::T agg) {
+    if ((threadIdx.x & 63) == 0)
+        store_status(&status[tile], Op::encode(tile == 0 ? kStInclusive : kStAggregate, agg));
 }
 
 struct OpI32 {

@@ -111,7 +111,7 @@ struct spr_handle {
     DBuf<int8_t> wn;
     DBuf<uint32_t> epath;
     DBuf<unsigned long long> sorted, sbig, stmp;
-    DBuf<int32_t> skey, sidx, swn, winding, flags;
+    DBuf<int32_t> skey, sidx, winding, flags;
     DBuf<int4> records;     // full records, kept only with the debug option
     DBuf<uint32_t> crec;    // compact records (x | width << 16) read by the composite
     DBuf<int32_t> row_lo, row_hi, bucket_base, bflag, band_start, big_list;
@@ -185,7 +185,7 @@ ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
     z.chainctl = o; o += (spr::kCcWords + 1) / 2;  // chain classes, offsets, counters (int32)
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
     z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
-    z.st_wind = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
+    z.st_wind = o; o += (size_t)Np + 3 * (size_t)(N / 1024) + 3;  // band statuses of the sort
     z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
     z.hist = o; o += (size_t)(NB + 2 + 1) / 2;  // NB+1 int32 (scanned in place)
     z.fill = o; o += (size_t)(NB + 1) / 2;      // NB int32
@@ -287,11 +287,10 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->sbig.ensure(n1);
     H->stmp.ensure(n1);
     H->skey.ensure(n1);
-    H->swn.ensure(n1);
     H->winding.ensure(n1 + 1);
     H->bflag.ensure(nb1 + 1);
     H->band_start.ensure((size_t)max_bands + 1);
-    H->big_list.ensure((size_t)max_bands);
+    H->big_list.ensure(2 * (size_t)max_bands);  // band ids, then their winding prefixes
     if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
     H->crec.ensure(2 * n1);
     H->image.ensure((size_t)4 * H->w * H->h);
@@ -355,14 +354,17 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     }
     stage_mark(H, 8);
     if (N > 0) {
+        // band sort fused with the winding numbers (global exclusive scan of the sorted winding
+        // changes, scanline_rasterizer.cpp:478-506)
         SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
-                                       debug ? H->sidx.p : nullptr, H->swn.p, H->big_list.p, d_bigcount, s));
-        SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->sbig.p, H->stmp.p,
-                                           H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s));
+                                       debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
+                                       H->big_list.p + max_bands, d_bigcount, ctl_at(H, z.st_wind, z.ctr, 2), H->n_cu,
+                                       s));
+        SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
+                                           H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr,
+                                           H->winding.p, s));
     }
     stage_mark(H, 9);
-    // ---- winding numbers (global exclusive scan, scanline_rasterizer.cpp:478-506)
-    SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, N, ctl_at(H, z.st_wind, z.ctr, 2), s));
     stage_mark(H, 10);
     SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
                                   ctl_at(H, z.st_mark, z.ctr, 3), debug ? H->records.p : nullptr, H->crec.p,
@@ -410,7 +412,7 @@ LearnedLayout learned_layout(uint32_t Np, int64_t nb_budget, int64_t n_budget) {
     z.st_pb = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
     z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
-    z.st_wind = o; o += spr::scan_tiles(std::max<int64_t>(n_budget, 1));
+    z.st_wind = o; o += (size_t)Np + 3 * (size_t)(std::max<int64_t>(n_budget, 1) / 1024) + 3;  // band statuses
     z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(n_budget, 1));
     z.total = o;
     return z;
@@ -516,11 +518,11 @@ void run_learned_frame(spr_handle* H) {
     const int64_t nl = std::max<int64_t>(H->lb_n, 1), nr = std::max<int64_t>(H->lb_r, 1);
     const int64_t max_bands = (int64_t)Np + 3 * (nl / 1024) + 2;
     H->inter.ensure(nl); H->chain_t.ensure(nl); H->key.ensure(nl); H->wn.ensure(nl); H->epath.ensure(nl);
-    H->sbig.ensure(nl); H->stmp.ensure(nl); H->skey.ensure(nl); H->swn.ensure(nl);
+    H->sbig.ensure(nl); H->stmp.ensure(nl); H->skey.ensure(nl);
     H->winding.ensure(nl + 1);
     H->bflag.ensure((size_t)nb_budget + 1);
     H->band_start.ensure((size_t)max_bands + 1);
-    H->big_list.ensure((size_t)max_bands);
+    H->big_list.ensure(2 * (size_t)max_bands);  // band ids, then their winding prefixes
     if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); H->records.ensure(nr); }
     H->crec.ensure(nr);
     H->image.ensure(H->batch ? H->img_bytes : (size_t)4 * H->w * H->h);
@@ -586,12 +588,14 @@ void run_learned_frame(spr_handle* H) {
                                   d_nsorted));
     stage_mark(H, 8);
     SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
-                                   debug ? H->sidx.p : nullptr, H->swn.p, H->big_list.p, d_bigcount, s));
-    SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->sbig.p, H->stmp.p,
-                                       H->skey.p, debug ? H->sidx.p : nullptr, H->swn.p, s));
+                                   debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
+                                   H->big_list.p + max_bands, d_bigcount, ctl_at(H, z.st_wind, z.counters, 4),
+                                   H->n_cu, s));
+    SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
+                                       H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr, H->winding.p,
+                                       s));
     SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
     stage_mark(H, 9);
-    SPR_HIP(spr::launch_scan_i32(H->swn.p, H->winding.p, nl, ctl_at(H, z.st_wind, z.counters, 4), s, d_nsorted));
     if (H->batch)
         SPR_HIP(spr::launch_doc_winding_base(H->winding.p, H->sstart.p, H->doc_first.p, Np, H->wbase.p, s));
     stage_mark(H, 10);
@@ -844,7 +848,7 @@ int spr_destroy(spr_handle* H) {
     H->ncuts.release(); H->counts.release(); H->seg.release(); H->inter.release();
     H->chain_t.release(); H->chains.release(); H->pieces.release(); H->chain_order.release();
     H->long_list.release(); H->mono.release(); H->pieces_seq.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
-    H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release(); H->swn.release();
+    H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
     H->band_start.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
