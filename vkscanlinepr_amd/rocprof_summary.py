@@ -26,7 +26,8 @@ STAGE_KERNELS = {
     "piece_fragments": ["k_piece_fragments", "k_piece_fragments_long"],
     "gen_fragment": ["k_gen_fragment"],
     "bucket_scatter": ["k_bucket_scatter", "k_band_flags", "k_band_emit"],
-    "band_sort": ["k_sort_bands", "k_sort_big_bands", "k_unpack_sorted"],
+    "band_sort": ["k_sort_bands", "k_sort_big_bands"],
+    "sort": ["k_bucket_scatter", "k_band_flags", "k_band_emit", "k_sort_bands", "k_sort_big_bands"],
     "mark_emit": ["k_mark_emit"],
     "composite": ["k_composite"],
 }
@@ -60,18 +61,13 @@ def traffic(fetch_dir, write_dir, cfetch_dir, cwrite_dir, workload, mode, comman
     w_scale = one_gib_kib / (sum(copy_w) / len(copy_w))
     fe = _per_kernel(fetch_dir, "FETCH_SIZE")
     wr = _per_kernel(write_dir, "WRITE_SIZE")
-    # the scan kernel runs several times per frame at different sizes: the winding scan is the
-    # launch with the largest grid
     kern = collections.defaultdict(lambda: [0.0, 0.0])
-    scans = [g for (k, g) in fe if k == "k_scan_i32"]
     for (k, g), vs in fe.items():
-        name = "k_scan_i32@winding" if k == "k_scan_i32" and scans and g == max(scans) else k
-        kern[name][0] += sum(vs) / len(vs) * f_scale * 1024
+        kern[k][0] += sum(vs) / len(vs) * f_scale * 1024
     for (k, g), vs in wr.items():
-        name = "k_scan_i32@winding" if k == "k_scan_i32" and scans and g == max(scans) else k
-        kern[name][1] += sum(vs) / len(vs) * w_scale * 1024
+        kern[k][1] += sum(vs) / len(vs) * w_scale * 1024
     stages = {}
-    for st, ks in list(STAGE_KERNELS.items()) + [("winding_scan", ["k_scan_i32@winding"])]:
+    for st, ks in STAGE_KERNELS.items():
         have = [k for k in ks if k in kern]
         if not have:
             continue
