@@ -302,7 +302,7 @@ def main(argv=None):
         if a.stages:
             print(f"  {k:16s} {v:8.3f} ms  {gbs or 0:8.1f} GB/s", file=sys.stderr)
     # dominant kernel: the longest stage; the solve is rated against the FP32 vector peak
-    timed = [k for k in stages if k != "sort"]
+    timed = [k for k in stages if k in algo and k != "sort"]
     dom = max(timed, key=lambda k: stages[k])
     if dom == "chain_solve":
         tops = SOLVE_FLOP_PER_CROSSING * N / (stages[dom] * 1e-3) / 1e12

@@ -19,13 +19,16 @@
 //     length.  The first item of every wave is its global wave index (the first waves of a
 //     launch sit on distinct SIMDs, so the costliest items do not share one), later items come
 //     from an atomic counter; issue priority follows the item's cost.
-//  4. k_piece_fragments: one wave per piece stages its two chains in LDS, merges them with its
-//     start in the reference's order (merge-path when both are non-decreasing, a concatenation
-//     when both are non-increasing, the sequential walk otherwise) and applies the reference's
-//     equal-t OR-merge there.  The same wave then produces the fragment key, winding change and
-//     bucket count of every entry (gen_fragment: the segment from an entry to the next one of
-//     its curve), so the entries themselves are only stored for export.  Pieces of more than
-//     1024 entries go to k_piece_fragments_long, which does the same with more LDS.
+//  4. k_piece_small: one lane per piece slot.  A piece of at most 32 entries is merged by its
+//     lane with the reference's sequential walk, OR-merged and turned into fragment keys,
+//     winding changes and bucket counts (gen_fragment: the segment from an entry to the next one
+//     of its curve) in one streaming pass; the entries themselves are only stored for export.
+//     Longer pieces are queued.
+//  5. k_piece_fragments: one wave per queued piece stages its two chains in LDS, merges them with
+//     its start in the reference's order (merge-path when both are non-decreasing, a
+//     concatenation when both are non-increasing, the sequential walk otherwise), applies the
+//     equal-t OR-merge there and produces the same fragment outputs.  Pieces of more than 1024
+//     entries go to k_piece_fragments_long, which does the same with more LDS.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -473,35 +476,145 @@ __device__ inline void piece_fragments(const PieceDesc& pd, float tend, const ui
     }
 }
 
-// One wave per piece, four waves per workgroup, pieces taken in slot order with a stride (the
-// outputs of neighbouring pieces are neighbours in memory).  The piece's chains are staged in
-// the wave's LDS, merged, OR-merged, and turned into fragments.  Longer pieces are queued for
-// k_piece_fragments_long.
-__global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __restrict__ pieces, int64_t n_slots,
+constexpr int kLaneMax = 32;  // pieces of at most this many entries are finished by one lane
+
+// Appends v of the lanes with `take` to a queue: one atomic per wave.  Every lane calls it.
+__device__ inline void queue_append(int32_t* __restrict__ list, int32_t* __restrict__ count, int32_t v, bool take) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long m = __ballot(take);
+    if (!m) return;
+    const int leader = __ffsll((long long)m) - 1;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (int32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (take) list[base + (int32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+// One lane per piece slot.  Pieces of at most kLaneMax entries: the lane stages its chains in
+// LDS (entry i of lane l at i * 64 + l), walks the reference's sequential merge
+// (make_intersection_1.comp:310-359, ties to x), OR-merges runs of equal masked t on the fly
+// (:361-375: an entry is the OR of its run up to itself and the next entry when that is in the
+// run) and emits each entry's fragment once the next entry is known.  Longer pieces go to the
+// mid queue (one wave each) or, above kWaveStage entries, to the long queue.
+__global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restrict__ pieces, int64_t n_slots,
+                                                        const uint8_t* __restrict__ mono,
+                                                        const float* __restrict__ chain_t,
+                                                        int32_t* __restrict__ mid_list, int32_t* __restrict__ mid_count,
+                                                        int32_t* __restrict__ long_list,
+                                                        int32_t* __restrict__ long_count,
+                                                        uint8_t* __restrict__ pieces_seq, const FragArgs A) {
+    __shared__ float s_ch[kBlock / 64][kLaneMax - 1][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float (*ch)[64] = s_ch[wave];
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    const int64_t n_iter = (n_slots + stride - 1) / stride;
+    for (int64_t it = 0; it < n_iter; ++it) {  // wave-uniform trip count
+        const int64_t slot = it * stride + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        PieceDesc pd;
+        pd.base = -1; pd.nx = pd.ny = pd.curve = 0; pd.t0 = pd.tend = 0.0f;
+        if (slot < n_slots) pd = pieces[slot];
+        const int nx = pd.nx, ny = pd.ny, nv = nx + ny, n = pd.base >= 0 ? nv + 1 : 0;
+        const bool mine = n > 0 && n <= kLaneMax;
+        const bool to_long = n > kWaveStage, to_mid = n > kLaneMax && !to_long;
+        if (to_long) pieces_seq[slot] = piece_mode(pd, slot, mono, chain_t);
+        queue_append(mid_list, mid_count, (int32_t)slot, to_mid);
+        queue_append(long_list, long_count, (int32_t)slot, to_long);
+        float tend = 1.0f;
+        uint32_t path = 0, bb = 0;
+        int2 pr = make_int2(0, 0);
+        int w = A.w, h = A.h;
+        CurvePts cv;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cv.x[k] = cv.y[k] = 0.0f;
+        cv.cubic = false;
+        if (mine) {
+            for (int i = 0; i < nv; ++i) ch[i][lane] = chain_t[pd.base + 1 + i];
+            tend = piece_end_t(pieces, slot, pd, chain_t);
+            const uint32_t c = (uint32_t)pd.curve;
+            path = A.cpath[c];
+            if (A.pcanvas) { const int2 cw = A.pcanvas[path]; w = cw.x; h = cw.y; }
+            cv = load_curve(A.tpos, A.cpos[c], A.ctype[c]);
+            bb = (uint32_t)A.bucket_base[path];
+            pr = A.prow[path];
+        }
+        __builtin_amdgcn_wave_barrier();
+        // the sequential walk: A = X (or [t0, X] without y crossings), B = [t0, Y]
+        const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
+        float tx = pd.t0, ty = pd.t0;
+        int xi = 0, yi = 0;
+        if (nx == 0) tx = kX2;
+        else if (ny == 0) ty = kY2;
+        else { tx = ch[0][lane]; xi = 1; }
+        auto next = [&]() -> uint32_t {
+            float t;
+            if (tx <= ty) { t = tx; tx = xi < nx ? ch[xi][lane] : kX2; ++xi; }
+            else { t = ty; ty = yi < ny ? ch[nx + yi][lane] : kY2; ++yi; }
+            return fbits(t);
+        };
+        // m0, m1: the current and next merged raw values; acc: OR of the current run up to m0
+        uint32_t m0 = 0, m1 = 0, acc = 0, fin0 = 0;
+        if (mine) {
+            m0 = next();
+            m1 = n > 1 ? next() : 0u;
+            acc = m0;
+            fin0 = acc | ((n > 1 && (m1 & 0xFFFFFFFCu) == (m0 & 0xFFFFFFFCu)) ? m1 : 0u);
+        }
+        int n_max = mine ? n : 0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) n_max = max(n_max, __shfl_xor(n_max, o));
+        for (int o = 0; o < n_max; ++o) {
+            const bool valid = mine && o < n;
+            uint32_t bucket = 0;
+            if (valid) {
+                // the next entry's final value (needs the entry after it)
+                uint32_t fin1 = 0;
+                float t1 = tend;
+                if (o + 1 < n) {
+                    const uint32_t m2 = o + 2 < n ? next() : 0u;
+                    const bool same = (m1 & 0xFFFFFFFCu) == (m0 & 0xFFFFFFFCu);
+                    const uint32_t acc1 = same ? (acc | m1) : m1;
+                    fin1 = acc1 | ((o + 2 < n && (m2 & 0xFFFFFFFCu) == (m1 & 0xFFFFFFFCu)) ? m2 : 0u);
+                    t1 = glmax(bitsf(fin1 & 0xFFFFFFFCu), 0.0f);
+                    acc = acc1; m0 = m1; m1 = m2;
+                }
+                const float t0 = glmax(bitsf(fin0 & 0xFFFFFFFCu), 0.0f);
+                uint32_t yx;
+                int swn;
+                segment_cell(cv, t0, t1, w, h, yx, swn);
+                const int64_t i = (int64_t)pd.base + o;
+                A.key[i] = (int32_t)yx;
+                A.wn[i] = (int8_t)swn;
+                A.epath[i] = path;
+                if (A.inter) A.inter[i] = make_int2(pd.curve, (int32_t)fin0);
+                bucket = bb + bucket_local((int32_t)yx, pr, A.err);
+                fin0 = fin1;
+            }
+            wave_count(A.hist, bucket, valid);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// One wave per queued piece, four waves per workgroup, in queue order with a stride.  The
+// piece's chains are staged in the wave's LDS, merged, OR-merged, and turned into fragments.
+__global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __restrict__ pieces,
+                                                            const int32_t* __restrict__ mid_list,
+                                                            const int32_t* __restrict__ mid_count,
                                                             const uint8_t* __restrict__ mono,
-                                                            const float* __restrict__ chain_t,
-                                                            int32_t* __restrict__ long_list,
-                                                            int32_t* __restrict__ long_count,
-                                                            uint8_t* __restrict__ pieces_seq, const FragArgs A) {
+                                                            const float* __restrict__ chain_t, const FragArgs A) {
     __shared__ float s_in[kBlock / 64][kWaveStage];
     __shared__ uint32_t s_out[kBlock / 64][kWaveStage];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* xin = s_in[wave];
     uint32_t* vout = s_out[wave];
     uint32_t* fin = reinterpret_cast<uint32_t*>(xin);  // the staged chains are dead after the merge
-    const int64_t stride = (int64_t)gridDim.x * (kBlock / 64);
-    for (int64_t slot = (int64_t)blockIdx.x * (kBlock / 64) + wave; slot < n_slots; slot += stride) {
+    const int32_t count = *mid_count;
+    const int32_t stride = (int32_t)gridDim.x * (kBlock / 64);
+    for (int32_t q = (int32_t)blockIdx.x * (kBlock / 64) + wave; q < count; q += stride) {
+        const int64_t slot = mid_list[q];
         const PieceDesc pd = pieces[slot];
-        if (pd.base < 0) continue;
         const int nv = pd.nx + pd.ny, n = nv + 1;
         const uint8_t mode = piece_mode(pd, slot, mono, chain_t);
-        if (n > kWaveStage) {
-            if (lane == 0) {
-                pieces_seq[slot] = mode;
-                long_list[atomicAdd(long_count, 1)] = (int32_t)slot;
-            }
-            continue;
-        }
         for (int i = lane; i < nv; i += 64) xin[i] = chain_t[pd.base + 1 + i];
         wave_sync();
         wave_merge(pd, mode, xin, vout, lane);
@@ -586,11 +699,16 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const Ch
 }
 
 hipError_t launch_piece_fragments(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                                  uint32_t* scratch, int64_t n_entries, int32_t* long_list, int32_t* long_count,
-                                  uint8_t* pieces_seq, const FragArgs& A, int n_cu, hipStream_t s) {
+                                  uint32_t* scratch, int64_t n_entries, int32_t* mid_list, int32_t* mid_count,
+                                  int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, const FragArgs& A,
+                                  int n_cu, hipStream_t s) {
     if (C == 0) return hipSuccess;
-    k_piece_fragments<<<(unsigned)(n_cu * 16), kBlock, 0, s>>>(pieces, (int64_t)C * 5, mono, chain_t, long_list,
-                                                               long_count, pieces_seq, A);
+    const int64_t slots = (int64_t)C * 5;
+    int64_t g = (slots + kBlock - 1) / kBlock;
+    if (g > (int64_t)n_cu * 20) g = (int64_t)n_cu * 20;
+    k_piece_small<<<(unsigned)g, kBlock, 0, s>>>(pieces, slots, mono, chain_t, mid_list, mid_count, long_list,
+                                                 long_count, pieces_seq, A);
+    k_piece_fragments<<<(unsigned)(n_cu * 16), kBlock, 0, s>>>(pieces, mid_list, mid_count, mono, chain_t, A);
     k_piece_fragments_long<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, long_list, long_count, pieces_seq, chain_t,
                                                                scratch, n_entries, A);
     return hipGetLastError();

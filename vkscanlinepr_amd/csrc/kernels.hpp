@@ -69,7 +69,8 @@ struct FragArgs {
 // of the persistent solve and the count of long pieces.
 constexpr int kChainClasses = 80;
 constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 * kChainClasses + 1,
-              kCcWork = kCcNLong + 1, kCcLongPieces = kCcWork + 1, kCcWords = kCcLongPieces + 1;
+              kCcWork = kCcNLong + 1, kCcLongPieces = kCcWork + 1, kCcMidPieces = kCcLongPieces + 1,
+              kCcWords = kCcMidPieces + 1;
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
@@ -83,11 +84,13 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const Ch
                               int32_t* work, float* chain_t, uint8_t* mono, int n_cu, hipStream_t s);
 // Merges each piece's x and y chains into its (curve, t bits) entries, OR-merging equal t (the
 // reference's per-piece walk), and turns the entries into fragment keys, winding changes and
-// bucket counts (gen_fragment) without storing the entries unless A.inter is set.  scratch
-// holds 2 * n_entries words for pieces too long for LDS.
+// bucket counts (gen_fragment) without storing the entries unless A.inter is set.  Pieces are
+// taken by one lane (short), one wave (queued in mid_list) or one wave with more LDS (queued in
+// long_list); scratch holds 2 * n_entries words for pieces too long for LDS.
 hipError_t launch_piece_fragments(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                                  uint32_t* scratch, int64_t n_entries, int32_t* long_list, int32_t* long_count,
-                                  uint8_t* pieces_seq, const FragArgs& A, int n_cu, hipStream_t s);
+                                  uint32_t* scratch, int64_t n_entries, int32_t* mid_list, int32_t* mid_count,
+                                  int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, const FragArgs& A,
+                                  int n_cu, hipStream_t s);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
