@@ -95,12 +95,15 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
     const uint32_t* __restrict__ ctype, const uint32_t* __restrict__ cpath,
     const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
     const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
-    int h, ChainDesc* __restrict__ chains, PieceDesc* __restrict__ pieces,
+    int h, ChainDesc* __restrict__ chains, PieceDesc* __restrict__ pieces, uint8_t* __restrict__ chain_cls,
     int32_t* __restrict__ class_count, const int32_t* __restrict__ ends, const int2* __restrict__ pcanvas) {
     __shared__ int32_t hist[kClasses];
     if (threadIdx.x < kClasses) hist[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t gid = blockIdx.x * kBlock + threadIdx.x;
+    const int w0 = w, h0 = h;
+    // grid-stride over (curve, piece) slots; the class histogram is flushed once per workgroup
+    for (uint32_t gid = blockIdx.x * kBlock + threadIdx.x; gid < 5u * C; gid += gridDim.x * kBlock) {
+    w = w0; h = h0;
     const uint32_t c = gid / 5u;
     const int piece = (int)(gid % 5u);
     PieceDesc pd;
@@ -143,10 +146,13 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
             }
         }
     }
-    if (gid < 5u * C) {
-        pieces[gid] = pd;
-        chains[2 * gid] = cd[0];
-        chains[2 * gid + 1] = cd[1];
+    pieces[gid] = pd;
+    chains[2 * gid] = cd[0];
+    chains[2 * gid + 1] = cd[1];
+    // class of each chain slot (0xFF: no chain) for the ordering pass
+    const uint32_t k0 = cd[0].n ? (uint32_t)chain_class(cd[0].n, cd[0].cubic != 0) : 0xFFu;
+    const uint32_t k1 = cd[1].n ? (uint32_t)chain_class(cd[1].n, cd[1].cubic != 0) : 0xFFu;
+    reinterpret_cast<uint16_t*>(chain_cls)[gid] = (uint16_t)(k0 | (k1 << 8));
     }
     __syncthreads();
     if (threadIdx.x < kClasses && hist[threadIdx.x]) atomicAdd(&class_count[threadIdx.x], hist[threadIdx.x]);
@@ -173,24 +179,32 @@ __global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t
     *n_long = nl;
 }
 
-__global__ __launch_bounds__(kBlock) void k_chain_order(const ChainDesc* __restrict__ chains, int64_t n_slots,
+// Chains in class order: each workgroup owns a contiguous range of chain slots, counts its
+// chains per class (from the one-byte classes written by k_chain_prep), takes one block of
+// places per class with a single atomic, then writes the chain indices.
+__global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restrict__ chain_cls, int64_t n_slots,
                                                         int32_t* __restrict__ class_fill,
                                                         const int32_t* __restrict__ class_off,
                                                         int32_t* __restrict__ order) {
     __shared__ int32_t cnt[kClasses], base[kClasses];
     if (threadIdx.x < kClasses) cnt[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool valid = i < n_slots && chains[i].n > 0;
-    int cls = 0, rank = 0;
-    if (valid) {
-        cls = chain_class(chains[i].n, chains[i].cubic != 0);
-        rank = atomicAdd(&cnt[cls], 1);
+    const int64_t per = ((n_slots + gridDim.x - 1) / gridDim.x + kBlock - 1) / kBlock * kBlock;
+    const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < n_slots ? b0 + per : n_slots;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
+        const uint32_t k = chain_cls[i];
+        if (k != 0xFFu) atomicAdd(&cnt[k], 1);
     }
     __syncthreads();
-    if (threadIdx.x < kClasses && cnt[threadIdx.x]) base[threadIdx.x] = atomicAdd(&class_fill[threadIdx.x], cnt[threadIdx.x]);
+    if (threadIdx.x < kClasses) {
+        base[threadIdx.x] = cnt[threadIdx.x] ? atomicAdd(&class_fill[threadIdx.x], cnt[threadIdx.x]) : 0;
+        cnt[threadIdx.x] = 0;
+    }
     __syncthreads();
-    if (valid) order[class_off[cls] + base[cls] + rank] = (int32_t)i;
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
+        const uint32_t k = chain_cls[i];
+        if (k != 0xFFu) order[class_off[k] + base[k] + atomicAdd(&cnt[k], 1)] = (int32_t)i;
+    }
 }
 
 __device__ inline CurveAxis load_axis(const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
@@ -672,21 +686,24 @@ __global__ __launch_bounds__(64) void k_piece_fragments_long(const PieceDesc* __
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
-                             ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s,
-                             const int32_t* ends, const int2* pcanvas) {
+                             ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
+                             int n_cu, hipStream_t s, const int32_t* ends, const int2* pcanvas) {
     if (C == 0) return hipSuccess;
-    k_chain_prep<<<grid_for((int64_t)C * 5), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts, ncuts,
-                                                             offsets, C, w, h, chains, pieces, class_count, ends,
-                                                             pcanvas);
+    unsigned g = grid_for((int64_t)C * 5);
+    if (g > (unsigned)n_cu * 8) g = (unsigned)n_cu * 8;
+    k_chain_prep<<<g, kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts, ncuts, offsets, C, w, h, chains,
+                                      pieces, chain_cls, class_count, ends, pcanvas);
     return hipGetLastError();
 }
 
-hipError_t launch_chain_order(const ChainDesc* chains, uint32_t C, const int32_t* class_count,
-                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order,
+hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_t* class_count,
+                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order, int n_cu,
                               hipStream_t s) {
     if (C == 0) return hipSuccess;
     k_chain_offsets<<<1, 64, 0, s>>>(class_count, class_off, n_long);
-    k_chain_order<<<grid_for((int64_t)C * 10), kBlock, 0, s>>>(chains, (int64_t)C * 10, class_fill, class_off, order);
+    unsigned g = grid_for((int64_t)C * 10);
+    if (g > (unsigned)n_cu * 4) g = (unsigned)n_cu * 4;
+    k_chain_order<<<g, kBlock, 0, s>>>(chain_cls, (int64_t)C * 10, class_fill, class_off, order);
     return hipGetLastError();
 }
 

@@ -60,9 +60,15 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
                                                        unsigned long long* __restrict__ bound_total,
                                                        const int2* __restrict__ pcanvas,
                                                        int32_t* __restrict__ pbound) {
-    const uint32_t p = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (p >= Np) return;
+    __shared__ unsigned long long s_bound[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int w0 = w, h0 = h;
+    unsigned long long bsum = 0;  // lane 0: this wave's share of the bound total
+    // four paths per wave, one atomic per workgroup for the bound total
+    for (int j = 0; j < 4; ++j) {
+    const uint32_t p = (blockIdx.x * 4 + wave) * 4 + j;
+    if (p >= Np) break;
+    w = w0; h = h0;
     if (pcanvas) { w = pcanvas[p].x; h = pcanvas[p].y; }
     const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
     float n_cub = 0, xmin = 3.4e38f, xmax = -3.4e38f, ymin = 3.4e38f, ymax = -3.4e38f;
@@ -122,8 +128,15 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
         double b = 0.0;
         if (visible_mask(visible[p])) b = 1.5 * ((double)sdx + (double)sdy) * 1.001 + 20.0 * (double)(c1 - c0) + 2.0;
         if (!(b < 1e15)) b = 1e15;  // non-finite coordinates
-        if (bound_total && b > 0.0) atomicAdd(bound_total, (unsigned long long)b);
+        if (b > 0.0) bsum += (unsigned long long)b;
         if (pbound) pbound[p] = b < 1073741824.0 ? (int32_t)b : 1073741824;
+    }
+    }
+    if (lane == 0) s_bound[wave] = bsum;
+    __syncthreads();
+    if (threadIdx.x == 0 && bound_total) {
+        const unsigned long long t = s_bound[0] + s_bound[1] + s_bound[2] + s_bound[3];
+        if (t) atomicAdd(bound_total, t);
     }
 }
 
@@ -227,57 +240,69 @@ __global__ void k_caps(const float* __restrict__ pred, const int32_t* __restrict
     if ((threadIdx.x & 63) == 0 && cr) atomicAdd(cap_r_total, cr);
 }
 
-// One workgroup per path: exact per-curve counts (make_intersection_0.comp:226-411, as
-// k_count), intra-path offsets, placement in the arena or the overflow pool, gap marking.
-// ctl[0] = pool entries taken, ctl[1] = overflowing paths, ctl[2] = live positions.
-__global__ __launch_bounds__(256) void k_learned_count(
+// Exact per-curve counts as k_count (make_intersection_0.comp:226-411), each curve on its own
+// path's canvas.
+__global__ __launch_bounds__(256) void k_learned_curve_counts(
     const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos, const uint32_t* __restrict__ ctype,
-    int32_t* __restrict__ visible, const uint32_t* __restrict__ pcurve, uint32_t Np, int w, int h,
-    const int32_t* __restrict__ arena, const int32_t* __restrict__ cap_n, int64_t budget,
+    const uint32_t* __restrict__ cpath, const int32_t* __restrict__ visible, uint32_t C, int w, int h,
     float4* __restrict__ cuts, uint32_t* __restrict__ ncuts, int32_t* __restrict__ counts,
-    int32_t* __restrict__ ends, int32_t* __restrict__ ctl, int32_t* __restrict__ n_exact, int32_t* __restrict__ pbase,
-    int2* __restrict__ inter, int8_t* __restrict__ mark, uint32_t* __restrict__ epath, int32_t* __restrict__ err,
     const int2* __restrict__ pcanvas) {
-    __shared__ int32_t s_w[kWaves];
-    __shared__ int64_t s_base;
-    __shared__ int32_t s_mode;  // 0 arena, 1 pool, 2 dropped
-    const uint32_t p = blockIdx.x;
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= C) return;
+    const uint32_t p = cpath[c];
     if (pcanvas) { w = pcanvas[p].x; h = pcanvas[p].y; }
-    const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
+    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
     const bool vis = visible_mask(visible[p]);
-    int32_t carry = 0;
-    for (uint32_t cs = c0; cs < c1; cs += 256) {
-        const uint32_t c = cs + threadIdx.x;
-        int32_t cnt = 0;
-        if (c < c1) {
-            const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
-            float q[4];
-            const uint32_t n = monotone_cuts(cv, vis, q);
-            cuts[c] = make_float4(q[0], q[1], q[2], q[3]);
-            ncuts[c] = n;
-            if (vis) {
-                float p0x = cv.x[0], p0y = cv.y[0];
+    float q[4];
+    const uint32_t n = monotone_cuts(cv, vis, q);
+    cuts[c] = make_float4(q[0], q[1], q[2], q[3]);
+    ncuts[c] = n;
+    int32_t cnt = 0;
+    if (vis) {
+        float p0x = cv.x[0], p0y = cv.y[0];
 #pragma unroll
-                for (uint32_t k = 0; k < 5; ++k) {
-                    if (k > n) break;
-                    const float t = k < n ? q[k] : 1.0f;
-                    const float p1x = cv.ex(t), p1y = cv.ey(t);
-                    cnt += piece_count(p0x, p0y, p1x, p1y, w, h);
-                    p0x = p1x; p0y = p1y;
-                }
-            }
+        for (uint32_t k = 0; k < 5; ++k) {
+            if (k > n) break;
+            const float t = k < n ? q[k] : 1.0f;
+            const float p1x = cv.ex(t), p1y = cv.ey(t);
+            cnt += piece_count(p0x, p0y, p1x, p1y, w, h);
+            p0x = p1x; p0y = p1y;
         }
-        int32_t tot;
-        const int32_t ex = block_exclusive_add(cnt, s_w, tot);
-        if (c < c1) { counts[c] = carry + ex; ends[c] = carry + ex + cnt; }
-        carry += tot;
-        __syncthreads();
+    }
+    counts[c] = cnt;
+}
+
+// One wave per path: intra-path offsets of the exact curve counts, placement in the arena or
+// the overflow pool, gap marking.  counts[c] becomes the curve's first position and ends[c] its
+// end.  ctl[0] = pool entries taken, ctl[1] = overflowing paths, ctl[2] = live positions.
+__global__ __launch_bounds__(256) void k_learned_place(
+    int32_t* __restrict__ visible, const uint32_t* __restrict__ pcurve, uint32_t Np,
+    const int32_t* __restrict__ arena, const int32_t* __restrict__ cap_n, int64_t budget,
+    int32_t* __restrict__ counts, int32_t* __restrict__ ends, int32_t* __restrict__ ctl,
+    int32_t* __restrict__ n_exact, int32_t* __restrict__ pbase, int2* __restrict__ inter,
+    int8_t* __restrict__ mark, uint32_t* __restrict__ epath, int32_t* __restrict__ err) {
+    const uint32_t p = blockIdx.x * (256 / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= Np) return;  // wave-uniform
+    const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
+    int32_t carry = 0;
+    for (uint32_t cs = c0; cs < c1; cs += 64) {
+        const uint32_t c = cs + lane;
+        const int32_t cnt = c < c1 ? counts[c] : 0;
+        int32_t inc = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t u = __shfl_up(inc, o);
+            if (lane >= o) inc += u;
+        }
+        if (c < c1) { counts[c] = carry + inc - cnt; ends[c] = carry + inc; }
+        carry += __shfl(inc, 63);
     }
     const int32_t n = carry;
     const int64_t A = arena[p], cap = cap_n[p], arena_total = arena[Np];
-    if (threadIdx.x == 0) {
-        int mode = 0;
-        int64_t base = A;
+    int64_t base = A;
+    int mode = 0;
+    if (lane == 0) {
         if (n > cap || A + cap > budget) {
             mode = 1;
             const int32_t off = n > 0 ? atomicAdd(&ctl[0], n) : 0;
@@ -285,27 +310,24 @@ __global__ __launch_bounds__(256) void k_learned_count(
             atomicAdd(&ctl[1], 1);
             if (base + n > budget) { mode = 2; atomicOr(err, 4); }
         }
-        s_base = base;
-        s_mode = mode;
         n_exact[p] = mode == 2 ? 0 : n;
         pbase[p] = (int32_t)base;
         if (mode == 2) visible[p] = 0;  // no chains: nothing is written for a dropped path
     }
-    __syncthreads();
-    const int64_t base = s_base;
-    const int mode = s_mode;
+    mode = __shfl(mode, 0);
+    base = (int64_t)__shfl((int32_t)base, 0);
     if (mode != 2)
-        for (uint32_t c = c0 + threadIdx.x; c < c1; c += 256) {
+        for (uint32_t c = c0 + lane; c < c1; c += 64) {
             counts[c] = (int32_t)(counts[c] + base);
             ends[c] = (int32_t)(ends[c] + base);
         }
     // gaps: the unused arena tail (the whole arena after an overflow), and the part of a
     // dropped path's pool block that lies inside the budget
     const int64_t g0 = mode == 0 ? A + n : A, g1 = min(A + cap, budget);
-    for (int64_t i = g0 + threadIdx.x; i < g1; i += 256) { inter[i] = make_int2(-1, 0); mark[i] = 1; epath[i] = kNoPath; }
+    for (int64_t i = g0 + lane; i < g1; i += 64) { inter[i] = make_int2(-1, 0); mark[i] = 1; epath[i] = kNoPath; }
     if (mode == 2) {
         const int64_t e = min(base + n, budget);
-        for (int64_t i = base + threadIdx.x; i < e; i += 256) { inter[i] = make_int2(-1, 0); mark[i] = 1; epath[i] = kNoPath; }
+        for (int64_t i = base + lane; i < e; i += 64) { inter[i] = make_int2(-1, 0); mark[i] = 1; epath[i] = kNoPath; }
     }
 }
 
@@ -423,8 +445,8 @@ hipError_t launch_path_features(const float2* tpos, const uint32_t* cpos, const 
                                 uint32_t Np, int w, int h, float* feat, hipStream_t s,
                                 unsigned long long* bound_total, const int2* pcanvas, int32_t* pbound) {
     if (Np == 0) return hipSuccess;
-    k_path_features<<<(Np + 3) / 4, 256, 0, s>>>(tpos, cpos, ctype, pcurve, visible, rule, Np, w, h, feat,
-                                                 bound_total, pcanvas, pbound);
+    k_path_features<<<(Np + 15) / 16, 256, 0, s>>>(tpos, cpos, ctype, pcurve, visible, rule, Np, w, h, feat,
+                                                   bound_total, pcanvas, pbound);
     return hipGetLastError();
 }
 
@@ -441,15 +463,17 @@ hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, f
     return hipGetLastError();
 }
 
-hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, int32_t* visible,
-                                const uint32_t* pcurve, uint32_t Np, int w, int h, const int32_t* arena,
-                                const int32_t* cap_n, int64_t budget, float4* cuts, uint32_t* ncuts,
-                                int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact, int32_t* pbase,
-                                int2* inter, int8_t* mark, uint32_t* epath, int32_t* err, hipStream_t s,
-                                const int2* pcanvas) {
+hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, const uint32_t* cpath,
+                                int32_t* visible, const uint32_t* pcurve, uint32_t C, uint32_t Np, int w, int h,
+                                const int32_t* arena, const int32_t* cap_n, int64_t budget, float4* cuts,
+                                uint32_t* ncuts, int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact,
+                                int32_t* pbase, int2* inter, int8_t* mark, uint32_t* epath, int32_t* err,
+                                hipStream_t s, const int2* pcanvas) {
     if (Np == 0) return hipSuccess;
-    k_learned_count<<<Np, 256, 0, s>>>(tpos, cpos, ctype, visible, pcurve, Np, w, h, arena, cap_n, budget, cuts,
-                                       ncuts, counts, ends, ctl, n_exact, pbase, inter, mark, epath, err, pcanvas);
+    if (C) k_learned_curve_counts<<<blocks(C), 256, 0, s>>>(tpos, cpos, ctype, cpath, visible, C, w, h, cuts, ncuts,
+                                                            counts, pcanvas);
+    k_learned_place<<<(Np + 3) / 4, 256, 0, s>>>(visible, pcurve, Np, arena, cap_n, budget, counts, ends, ctl,
+                                                 n_exact, pbase, inter, mark, epath, err);
     k_learned_finish<<<1, 1, 0, s>>>(arena, Np, budget, ctl);
     return hipGetLastError();
 }

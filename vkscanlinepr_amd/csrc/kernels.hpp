@@ -74,10 +74,10 @@ constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
-                             ChainDesc* chains, PieceDesc* pieces, int32_t* class_count, hipStream_t s,
-                             const int32_t* ends = nullptr, const int2* pcanvas = nullptr);
-hipError_t launch_chain_order(const ChainDesc* chains, uint32_t C, const int32_t* class_count,
-                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order,
+                             ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
+                             int n_cu, hipStream_t s, const int32_t* ends = nullptr, const int2* pcanvas = nullptr);
+hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_t* class_count,
+                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order, int n_cu,
                               hipStream_t s);
 hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
                               const int32_t* order, const int32_t* class_off, const int32_t* n_long,
@@ -159,12 +159,12 @@ hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, flo
 hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, float log_margin, int32_t slack,
                        int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s,
                        const int32_t* pbound = nullptr);
-hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, int32_t* visible,
-                                const uint32_t* pcurve, uint32_t Np, int w, int h, const int32_t* arena,
-                                const int32_t* cap_n, int64_t budget, float4* cuts, uint32_t* ncuts,
-                                int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact, int32_t* pbase,
-                                int2* inter, int8_t* mark, uint32_t* epath, int32_t* err, hipStream_t s,
-                                const int2* pcanvas = nullptr);
+hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, const uint32_t* cpath,
+                                int32_t* visible, const uint32_t* pcurve, uint32_t C, uint32_t Np, int w, int h,
+                                const int32_t* arena, const int32_t* cap_n, int64_t budget, float4* cuts,
+                                uint32_t* ncuts, int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact,
+                                int32_t* pbase, int2* inter, int8_t* mark, uint32_t* epath, int32_t* err,
+                                hipStream_t s, const int2* pcanvas = nullptr);
 hipError_t launch_sorted_starts(const int32_t* boff, const int32_t* bucket_base, uint32_t Np, int32_t* sstart,
                                 hipStream_t s);
 hipError_t launch_path_targets(const int32_t* n_or_seg, int seg_form, const int2* brec, const int32_t* bucket_base,

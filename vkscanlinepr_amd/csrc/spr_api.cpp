@@ -105,7 +105,7 @@ struct spr_handle {
     DBuf<spr::ChainDesc> chains;
     DBuf<spr::PieceDesc> pieces;
     DBuf<int32_t> chain_order, long_list, mid_list;
-    DBuf<uint8_t> mono, pieces_seq;
+    DBuf<uint8_t> mono, pieces_seq, chain_cls;
     int n_cu = 256;
     DBuf<int32_t> key;
     DBuf<int8_t> wn;
@@ -317,11 +317,14 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         H->chain_order.ensure((size_t)C * 10);
         H->long_list.ensure((size_t)C * 5);
         H->mid_list.ensure((size_t)C * 5);
+        H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         H->pieces_seq.ensure((size_t)C * 5 + 4);
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
-                                       H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, d_cc, s));
-        SPR_HIP(spr::launch_chain_order(H->chains.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, s));
+                                       H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
+                                       d_cc, H->n_cu, s));
+        SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, H->n_cu,
+                                        s));
     }
     stage_mark(H, 4);
     if (gen) {
@@ -538,7 +541,8 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->brec_l.p, 2 * nb_budget, d_nb, 2, 0, 0u, s));
 
     // phase 2: exact counts placed in the arenas, then the plain stages on device counts
-    SPR_HIP(spr::launch_learned_count(H->tpos.p, H->cpos.p, H->ctype.p, H->visible.p, H->pcurve.p, Np, w, h,
+    SPR_HIP(spr::launch_learned_count(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->pcurve.p, C,
+                                      Np, w, h,
                                       H->arena.p, H->capn.p, H->lb_n, H->cuts.p, H->ncuts.p, H->counts.p,
                                       H->cend.p, d_ctl, H->nexact.p, H->pbase.p, H->inter.p, H->wn.p, H->epath.p,
                                       d_err, s, pcv));
@@ -556,12 +560,14 @@ void run_learned_frame(spr_handle* H) {
         H->chain_order.ensure((size_t)C * 10);
         H->long_list.ensure((size_t)C * 5);
         H->mid_list.ensure((size_t)C * 5);
+        H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         H->pieces_seq.ensure((size_t)C * 5 + 4);
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
-                                       H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, d_cc, s,
-                                       H->cend.p, pcv));
-        SPR_HIP(spr::launch_chain_order(H->chains.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, s));
+                                       H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
+                                       d_cc, H->n_cu, s, H->cend.p, pcv));
+        SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, H->n_cu,
+                                        s));
     }
     stage_mark(H, 4);
     if (C) {
@@ -850,7 +856,7 @@ int spr_destroy(spr_handle* H) {
     H->rule.release(); H->fill.release(); H->tpos.release(); H->visible.release(); H->cuts.release();
     H->ncuts.release(); H->counts.release(); H->seg.release(); H->inter.release();
     H->chain_t.release(); H->chains.release(); H->pieces.release(); H->chain_order.release();
-    H->long_list.release(); H->mid_list.release(); H->mono.release(); H->pieces_seq.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
+    H->long_list.release(); H->mid_list.release(); H->chain_cls.release(); H->mono.release(); H->pieces_seq.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
