@@ -41,7 +41,11 @@ __global__ void k_bucket_counts(const int32_t* __restrict__ row_lo, const int32_
 }
 
 // Scatter into bucket order.  The sort word orders exactly like the reference's signed
-// (key, index) pair and carries the winding change in its two low bits.
+// (key, index) pair and carries the winding change in its two low bits.  Each wave takes
+// kRounds chunks of 64 consecutive entries and issues every load, then every slot atomic (one per
+// run of equal buckets), of all chunks before it waits on any of them.
+constexpr int kRounds = 4;
+
 __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __restrict__ key,
                                                            const int8_t* __restrict__ wn,
                                                            const uint32_t* __restrict__ epath, int64_t N,
@@ -53,24 +57,46 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
                                                            int32_t* __restrict__ err,
                                                            const int32_t* __restrict__ n_dev) {
     const int64_t n = n_dev ? min((int64_t)*n_dev, N) : N;
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool valid = i < n;
-    uint32_t b = 0;
-    int32_t k = 0;
-    if (valid) {
-        const uint32_t p = epath[i];
-        if (p == kNoPath) {
-            valid = false;  // gap of a predicted-allocation layout
-        } else {
-            k = key[i];
-            b = (uint32_t)bucket_base[p] + bucket_local(k, prow[p], err);
-        }
+    const int lane = threadIdx.x & 63;
+    const int64_t g0 = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64 * kRounds);
+    uint32_t pth[kRounds];
+    int32_t k[kRounds];
+    bool valid[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const int64_t i = g0 + r * 64 + lane;
+        valid[r] = i < n;
+        pth[r] = kNoPath; k[r] = 0;
+        if (valid[r]) { pth[r] = epath[i]; k[r] = key[i]; }
+        if (pth[r] == kNoPath) valid[r] = false;  // gap of a predicted-allocation layout
     }
-    const int32_t slot = wave_slot(fill, b, valid);
-    if (valid) {
-        const unsigned long long w = ((unsigned long long)((uint32_t)k ^ 0x80000000u) << 32) |
-                                     ((unsigned long long)i << 2) | (unsigned long long)(uint32_t)(wn[i] + 1);
-        out[boff[b] + slot] = w;
+    uint32_t b[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        b[r] = 0;
+        if (valid[r]) b[r] = (uint32_t)bucket_base[pth[r]] + bucket_local(k[r], prow[pth[r]], err);
+    }
+    // runs of equal buckets: heads take their run's slots with one atomic each
+    unsigned long long vm[kRounds], hd[kRounds];
+    int32_t base[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        vm[r] = __ballot(valid[r]);
+        hd[r] = run_heads(b[r], valid[r], vm[r]);
+        base[r] = 0;
+        if ((hd[r] >> lane) & 1ull) base[r] = atomicAdd(&fill[b[r]], run_length(lane, hd[r], vm[r]));
+    }
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r) {
+        const unsigned long long mine = hd[r] & ((2ull << lane) - 1ull);
+        const int h = (valid[r] && mine) ? 63 - __clzll((long long)mine) : lane;
+        const int32_t slot = __shfl(base[r], h) + (lane - h);
+        if (valid[r]) {
+            const int64_t i = g0 + r * 64 + lane;
+            const unsigned long long w = ((unsigned long long)((uint32_t)k[r] ^ 0x80000000u) << 32) |
+                                         ((unsigned long long)i << 2) | (unsigned long long)(uint32_t)(wn[i] + 1);
+            out[boff[b[r]] + slot] = w;
+        }
     }
 }
 
@@ -143,8 +169,8 @@ hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uin
                                  int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s,
                                  const int32_t* n_dev) {
     if (N == 0) return hipSuccess;
-    k_bucket_scatter<<<grid_for(N), kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, boff, fill, out, err,
-                                                    n_dev);
+    k_bucket_scatter<<<grid_for((N + kRounds - 1) / kRounds), kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow,
+                                                                              boff, fill, out, err, n_dev);
     return hipGetLastError();
 }
 

@@ -150,17 +150,45 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     int32_t kprev = 0;
     uint32_t pprev = 0;
     if (i0 > 0 && i0 <= N) { kprev = skey[i0 - 1]; pprev = cur.at(i0 - 1); }
-    int32_t keys[kScanItems];
+    int32_t keys[kScanItems], wind[kScanItems];
     uint32_t paths[kScanItems];
+    // the thread's 16 consecutive keys, winding numbers and paths: 16-byte loads when whole
+    if (i0 + kScanItems <= N) {
+        const int4* kq = reinterpret_cast<const int4*>(skey + i0);
+        const int4* wq = reinterpret_cast<const int4*>(winding + i0);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const int4 a = kq[q], b = wq[q];
+            keys[4 * q] = a.x; keys[4 * q + 1] = a.y; keys[4 * q + 2] = a.z; keys[4 * q + 3] = a.w;
+            wind[4 * q] = b.x; wind[4 * q + 1] = b.y; wind[4 * q + 2] = b.z; wind[4 * q + 3] = b.w;
+        }
+        if (!sstart) {
+            const uint4* pq = reinterpret_cast<const uint4*>(epath + i0);
+#pragma unroll
+            for (int q = 0; q < kScanItems / 4; ++q) {
+                const uint4 c = pq[q];
+                paths[4 * q] = c.x & 0x3FFFFFFFu; paths[4 * q + 1] = c.y & 0x3FFFFFFFu;
+                paths[4 * q + 2] = c.z & 0x3FFFFFFFu; paths[4 * q + 3] = c.w & 0x3FFFFFFFu;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanItems; ++j) paths[j] = cur.at(i0 + j);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int64_t f = i0 + j;
+            keys[j] = 0; wind[j] = 0; paths[j] = 0;
+            if (f < N) { keys[j] = skey[f]; wind[j] = winding[f]; paths[j] = cur.at(f); }
+        }
+    }
     uint32_t cellbits = 0, spanbits = 0;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
         const int64_t f = i0 + j;
-        keys[j] = 0; paths[j] = 0;
         if (f >= N) continue;
-        const int32_t k1 = skey[f];
-        const uint32_t p1 = cur.at(f);
-        keys[j] = k1; paths[j] = p1;
+        const int32_t k1 = keys[j];
+        const uint32_t p1 = paths[j];
         const int x1 = key_x(k1), y1 = key_y(k1);
         int cw = w, ch = h;
         if (pcanvas) { const int2 cv = pcanvas[p1]; cw = cv.x; ch = cv.y; }
@@ -173,7 +201,7 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
             const int x0 = key_x(kprev), y0 = key_y(kprev);
             cell = in_canvas && (pprev != p1 || kprev != k1);
             // documents of a batch each scan their own winding numbers from zero
-            const int32_t wn = (int32_t)((uint32_t)winding[f] - (uint32_t)(wbase ? wbase[p1] : 0));
+            const int32_t wn = (int32_t)((uint32_t)wind[j] - (uint32_t)(wbase ? wbase[p1] : 0));
             const bool wf = (r == 0u && wn != 0) || (r == 1u && (wn & 1) != 0);
             span = (y0 == y1) && (x0 + 2 < x1) && (pprev == p1) && wf;
         }
