@@ -61,6 +61,23 @@ def stage_bytes(P, C, Np, N, R, w, h):
     }
 
 
+def stage_io_bytes(P, C, Np, N, R, w, h):
+    """Each kernel's own inputs read once and outputs written once, in the reference's data types
+    (no bytes for the intermediate buffers a fused kernel never materialises): piece merge with
+    keys 72 B/curve + t in, key, winding change and path out (16 B/intersection); sort with the
+    winding gather and scan: key, index, winding change in and key, index, winding number out
+    (24 B/entry); mark and emit: key, winding number, path in (12 B/entry) and 16 B/record out."""
+    return {
+        "transform": 20 * P + 4 * Np,
+        "count": 72 * C,
+        "chain_solve": 48 * C + 4 * N,
+        "piece_fragments": 72 * C + 16 * N,
+        "sort": 24 * N,
+        "mark_emit": 12 * N + 16 * R,
+        "composite": 16 * R + 4 * w * h,
+    }
+
+
 # The intersection solve is bound by FP32 vector issue, not HBM: 24 bisection steps per crossing,
 # each a midpoint (add, mul), a cubic evaluated as six lerps of a sub, a mul and an add, and the
 # sign test (sub): 21 operations, no FMA (bit parity with the reference needs every rounding).
@@ -294,13 +311,16 @@ def main(argv=None):
     # band sort and the winding scan folded into it
     stages["sort"] = stages["bucket_scatter"] + stages["band_sort"] + stages["winding_scan"]
     algo = stage_bytes(P, C, Np, N, R, canvas, canvas)
+    io = stage_io_bytes(P, C, Np, N, R, canvas, canvas)
     table = {}
     for k, v in stages.items():
         gbs = algo[k] / (v * 1e-3) / 1e9 if v > 0 and k in algo else None
+        iob = io[k] / (v * 1e-3) / 1e9 if v > 0 and k in io else None
         table[k] = {"ms": round(v, 4), "GB/s": None if gbs is None else round(gbs, 1),
-                    "frac": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4)}
+                    "frac": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4),
+                    "io_GB/s": None if iob is None else round(iob, 1)}
         if a.stages:
-            print(f"  {k:16s} {v:8.3f} ms  {gbs or 0:8.1f} GB/s", file=sys.stderr)
+            print(f"  {k:16s} {v:8.3f} ms  {gbs or 0:8.1f} GB/s  (own I/O {iob or 0:8.1f} GB/s)", file=sys.stderr)
     # dominant kernel: the longest stage; the solve is rated against the FP32 vector peak
     timed = [k for k in stages if k in algo and k != "sort"]
     dom = max(timed, key=lambda k: stages[k])

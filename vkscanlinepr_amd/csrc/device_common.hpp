@@ -284,11 +284,13 @@ __device__ inline void wave_count(int32_t* __restrict__ cnt, uint32_t key, bool 
 // gen_fragment.comp:90-224: cell key of the curve segment [t0, t1] (the key of its midpoint's
 // cell, or the off-canvas sentinel when t0 >= t1 or the cell row is off the canvas) and its
 // winding change across the cell's centre row.
-__device__ inline void segment_cell(const CurvePts& cv, float t0, float t1, int w, int h, uint32_t& yx, int& swn) {
+// The same from the segment's end points (pf at t0, pl at t1), for callers that share the
+// evaluation of a point between two consecutive segments.
+__device__ inline void segment_cell_pts(bool forward, float pfx, float pfy, float plx, float ply, int w, int h,
+                                        uint32_t& yx, int& swn) {
     yx = kSentinelKey;
     swn = 0;
-    if (!(t0 < t1)) return;
-    const float pfx = cv.ex(t0), pfy = cv.ey(t0), plx = cv.ex(t1), ply = cv.ey(t1);
+    if (!forward) return;
     const int raw_x = wmul(f2i_rd(((pfx + plx) * 0.5f) / (float)kFrag), kFrag);
     const int raw_y = wmul(f2i_rd(((pfy + ply) * 0.5f) / (float)kFrag), kFrag);
     const int pos_x = min(max(raw_x, -kFrag), (w & ~1) + kFrag);
@@ -300,6 +302,11 @@ __device__ inline void segment_cell(const CurvePts& cv, float t0, float t1, int 
     if (pfy == ply) swn = 0;
     else if (pfy < fwy && fwy <= ply) swn = -1;
     else if (ply < fwy && fwy <= pfy) swn = 1;
+}
+
+__device__ inline void segment_cell(const CurvePts& cv, float t0, float t1, int w, int h, uint32_t& yx, int& swn) {
+    if (!(t0 < t1)) { yx = kSentinelKey; swn = 0; return; }
+    segment_cell_pts(true, cv.ex(t0), cv.ey(t0), cv.ex(t1), cv.ey(t1), w, h, yx, swn);
 }
 
 }  // namespace spr

@@ -468,17 +468,22 @@ __device__ inline void piece_fragments(const PieceDesc& pd, float tend, const ui
     const CurvePts cv = load_curve(A.tpos, A.cpos[c], A.ctype[c]);
     const uint32_t bb = (uint32_t)A.bucket_base[path];
     const int2 pr = A.prow[path];
-    for (int o0 = 0; o0 < n; o0 += 64) {
+    // lane l evaluates point o0 + l (point o is entry o's parameter, point n is tend); entry o's
+    // segment runs from point o to point o + 1, so a chunk of 64 points serves 63 entries
+    for (int o0 = 0; o0 < n; o0 += 63) {
         const int o = o0 + lane;
-        const bool valid = o < n;
+        float t = 0.0f;
+        uint32_t f = 0;
+        if (o < n) { f = fin[o]; t = glmax(bitsf(f & 0xFFFFFFFCu), 0.0f); }
+        else if (o == n) t = tend;
+        const float px = cv.ex(t), py = cv.ey(t);
+        const float t1 = __shfl_down(t, 1), qx = __shfl_down(px, 1), qy = __shfl_down(py, 1);
+        const bool valid = lane < 63 && o < n;
         uint32_t bucket = 0;
         if (valid) {
-            const uint32_t f = fin[o];
-            const float t0 = glmax(bitsf(f & 0xFFFFFFFCu), 0.0f);
-            const float t1 = o + 1 < n ? glmax(bitsf(fin[o + 1] & 0xFFFFFFFCu), 0.0f) : tend;
             uint32_t yx;
             int swn;
-            segment_cell(cv, t0, t1, w, h, yx, swn);
+            segment_cell_pts(t < t1, px, py, qx, qy, w, h, yx, swn);
             const int64_t i = (int64_t)pd.base + o;
             A.key[i] = (int32_t)yx;
             A.wn[i] = (int8_t)swn;
@@ -567,11 +572,14 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         };
         // m0, m1: the current and next merged raw values; acc: OR of the current run up to m0
         uint32_t m0 = 0, m1 = 0, acc = 0, fin0 = 0;
+        float t0 = 0.0f, px = 0.0f, py = 0.0f;  // the current entry's parameter and point
         if (mine) {
             m0 = next();
             m1 = n > 1 ? next() : 0u;
             acc = m0;
             fin0 = acc | ((n > 1 && (m1 & 0xFFFFFFFCu) == (m0 & 0xFFFFFFFCu)) ? m1 : 0u);
+            t0 = glmax(bitsf(fin0 & 0xFFFFFFFCu), 0.0f);
+            px = cv.ex(t0); py = cv.ey(t0);
         }
         int n_max = mine ? n : 0;
 #pragma unroll
@@ -591,10 +599,10 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
                     t1 = glmax(bitsf(fin1 & 0xFFFFFFFCu), 0.0f);
                     acc = acc1; m0 = m1; m1 = m2;
                 }
-                const float t0 = glmax(bitsf(fin0 & 0xFFFFFFFCu), 0.0f);
+                const float qx = cv.ex(t1), qy = cv.ey(t1);
                 uint32_t yx;
                 int swn;
-                segment_cell(cv, t0, t1, w, h, yx, swn);
+                segment_cell_pts(t0 < t1, px, py, qx, qy, w, h, yx, swn);
                 const int64_t i = (int64_t)pd.base + o;
                 A.key[i] = (int32_t)yx;
                 A.wn[i] = (int8_t)swn;
@@ -602,6 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
                 if (A.inter) A.inter[i] = make_int2(pd.curve, (int32_t)fin0);
                 bucket = bb + bucket_local((int32_t)yx, pr, A.err);
                 fin0 = fin1;
+                t0 = t1; px = qx; py = qy;
             }
             wave_count(A.hist, bucket, valid);
         }
