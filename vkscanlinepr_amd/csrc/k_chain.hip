@@ -378,8 +378,18 @@ __device__ inline void wave_merge(const PieceDesc& pd, uint8_t mode, const float
     const float* Y = X + nx;
     const bool head_in_a = ny == 0;
     const int na = nx + (head_in_a ? 1 : 0), nb = head_in_a ? 0 : ny + 1;
-    auto Aat = [&](int i) { return head_in_a ? (i == 0 ? pd.t0 : X[i - 1]) : X[i]; };
-    auto Bat = [&](int j) { return j == 0 ? pd.t0 : Y[j - 1]; };
+    const float t0 = pd.t0;
+    // loads always from the staged chains (index clamped), the piece start selected after: the
+    // accessors must not form a pointer to t0, which would turn every access into a flat load
+    auto Aat = [&](int i) -> float {
+        if (!head_in_a) return X[i];
+        const float x = X[i > 0 ? i - 1 : 0];
+        return i == 0 ? t0 : x;
+    };
+    auto Bat = [&](int j) -> float {
+        const float y = Y[j > 0 ? j - 1 : 0];
+        return j == 0 ? t0 : y;
+    };
     const int n = na + nb;
     if (mode == 2) {
         if (lane == 0) {
@@ -564,10 +574,19 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         if (nx == 0) tx = kX2;
         else if (ny == 0) ty = kY2;
         else { tx = ch[0][lane]; xi = 1; }
+        // one step of the walk with value selects only (a branch per side would make the
+        // compiler address tx / ty and keep the walk state in scratch memory)
         auto next = [&]() -> uint32_t {
-            float t;
-            if (tx <= ty) { t = tx; tx = xi < nx ? ch[xi][lane] : kX2; ++xi; }
-            else { t = ty; ty = yi < ny ? ch[nx + yi][lane] : kY2; ++yi; }
+            const bool takex = tx <= ty;
+            const float t = takex ? tx : ty;
+            const bool more = takex ? xi < nx : yi < ny;
+            const int at = takex ? xi : nx + yi;
+            const float v = ch[more ? at : 0][lane];
+            const float repl = more ? v : (takex ? kX2 : kY2);
+            tx = takex ? repl : tx;
+            ty = takex ? ty : repl;
+            xi += takex ? 1 : 0;
+            yi += takex ? 0 : 1;
             return fbits(t);
         };
         // m0, m1: the current and next merged raw values; acc: OR of the current run up to m0
