@@ -688,23 +688,28 @@ __global__ __launch_bounds__(64) void k_piece_fragments_long(const PieceDesc* __
         const PieceDesc pd = pieces[pi];
         const int nv = pd.nx + pd.ny, n = nv + 1;
         const uint8_t mode = piece_mode[pi];
-        const bool lds = n <= kStage;
-        const float* X = chain_t + pd.base + 1;
-        if (lds) {
-            for (int i = lane; i < nv; i += 64) st_in[i] = X[i];
-            __syncthreads();
-            X = st_in;
-        }
-        uint32_t* vout = lds ? st_out : scratch + pd.base;
-        wave_merge(pd, mode, X, vout, lane);
-        __threadfence_block();
-        __syncthreads();
-        uint32_t* fin = lds ? reinterpret_cast<uint32_t*>(st_in) : scratch + n_entries + pd.base;
         const float tend = piece_end_t(pieces, pi, pd, chain_t);
-        for (int o = lane; o < n; o += 64) fin[o] = or_merged(vout, n, o);
-        __threadfence_block();
-        __syncthreads();
-        piece_fragments(pd, tend, fin, n, A, lane);
+        // the two branches keep every pointer in one address space (no flat accesses)
+        if (n <= kStage) {
+            for (int i = lane; i < nv; i += 64) st_in[i] = chain_t[pd.base + 1 + i];
+            __syncthreads();
+            wave_merge(pd, mode, st_in, st_out, lane);
+            __syncthreads();
+            uint32_t* fin = reinterpret_cast<uint32_t*>(st_in);
+            for (int o = lane; o < n; o += 64) fin[o] = or_merged(st_out, n, o);
+            __syncthreads();
+            piece_fragments(pd, tend, fin, n, A, lane);
+        } else {
+            uint32_t* vout = scratch + pd.base;
+            wave_merge(pd, mode, chain_t + pd.base + 1, vout, lane);
+            __threadfence_block();
+            __syncthreads();
+            uint32_t* fin = scratch + n_entries + pd.base;
+            for (int o = lane; o < n; o += 64) fin[o] = or_merged(vout, n, o);
+            __threadfence_block();
+            __syncthreads();
+            piece_fragments(pd, tend, fin, n, A, lane);
+        }
         __syncthreads();
     }
 }

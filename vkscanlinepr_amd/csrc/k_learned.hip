@@ -76,11 +76,14 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
     for (uint32_t c = c0 + lane; c < c1; c += 64) {
         const uint32_t k = ctype[c] == 4u ? 4u : 2u;
         const uint32_t b = cpos[c];
-        float2 q[4];
-        for (uint32_t i = 0; i < k; ++i) q[i] = tpos[b + i];
+        float2 q[4];  // static indices only (the line's points repeat its end point)
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) q[i] = tpos[b + (i < k ? i : k - 1)];
         n_cub += k == 4u ? 1.0f : 0.0f;
         float pdx = 0, pdy = 0;
-        for (uint32_t i = 0; i < k; ++i) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            if (i >= k) break;
             xmin = fminf(xmin, q[i].x); xmax = fmaxf(xmax, q[i].x);
             ymin = fminf(ymin, q[i].y); ymax = fmaxf(ymax, q[i].y);
             inside += (q[i].x >= 0 && q[i].x < (float)w && q[i].y >= 0 && q[i].y < (float)h) ? 1.0f : 0.0f;
@@ -93,8 +96,9 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
                 pdx = dx; pdy = dy;
             }
         }
-        cdx += fabsf(q[k - 1].x - q[0].x);
-        cdy += fabsf(q[k - 1].y - q[0].y);
+        const float2 qe = k == 4u ? q[3] : q[1];
+        cdx += fabsf(qe.x - q[0].x);
+        cdy += fabsf(qe.y - q[0].y);
     }
     n_cub = wave_sum_f(n_cub); plen = wave_sum_f(plen); sdx = wave_sum_f(sdx); sdy = wave_sum_f(sdy);
     cdx = wave_sum_f(cdx); cdy = wave_sum_f(cdy); turn = wave_sum_f(turn);
