@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
                                                         uint8_t* __restrict__ pieces_seq, const FragArgs A) {
     __shared__ float s_ch[kBlock / 64][kLaneMax - 1][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float (*ch)[64] = s_ch[wave];
+    float* ch = &s_ch[wave][0][0];
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     const int64_t n_iter = (n_slots + stride - 1) / stride;
     for (int64_t it = 0; it < n_iter; ++it) {  // wave-uniform trip count
@@ -556,8 +556,20 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
 #pragma unroll
         for (int k = 0; k < 4; ++k) cv.x[k] = cv.y[k] = 0.0f;
         cv.cubic = false;
+        // chain values: when the wave's pieces lie in one window of the LDS size (consecutive
+        // pieces are neighbours), the wave loads the window coalesced and value i of a piece is
+        // at (base + 1 - lo) + i; otherwise each lane loads its own at i * 64 + lane
+        int lo = mine && nv > 0 ? pd.base + 1 : 0x7FFFFFFF, hi = mine && nv > 0 ? pd.base + 1 + nv : 0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) { lo = min(lo, __shfl_xor(lo, o)); hi = max(hi, __shfl_xor(hi, o)); }
+        const bool coop = hi > lo && hi - lo <= 64 * (kLaneMax - 1);
+        const int off0 = coop ? pd.base + 1 - lo : lane, step = coop ? 1 : 64;
+        if (coop) {
+            for (int i = lane; i < hi - lo; i += 64) ch[i] = chain_t[lo + i];
+        } else if (mine) {
+            for (int i = 0; i < nv; ++i) ch[i * 64 + lane] = chain_t[pd.base + 1 + i];
+        }
         if (mine) {
-            for (int i = 0; i < nv; ++i) ch[i][lane] = chain_t[pd.base + 1 + i];
             tend = piece_end_t(pieces, slot, pd, chain_t);
             const uint32_t c = (uint32_t)pd.curve;
             path = A.cpath[c];
@@ -566,14 +578,14 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
             bb = (uint32_t)A.bucket_base[path];
             pr = A.prow[path];
         }
-        __builtin_amdgcn_wave_barrier();
+        wave_sync();  // the window was loaded by all lanes
         // the sequential walk: A = X (or [t0, X] without y crossings), B = [t0, Y]
         const float kX2 = bitsf(kTwoBits), kY2 = bitsf(kTwoBits | 1u);
         float tx = pd.t0, ty = pd.t0;
         int xi = 0, yi = 0;
         if (nx == 0) tx = kX2;
         else if (ny == 0) ty = kY2;
-        else { tx = ch[0][lane]; xi = 1; }
+        else { tx = ch[off0]; xi = 1; }
         // one step of the walk with value selects only (a branch per side would make the
         // compiler address tx / ty and keep the walk state in scratch memory)
         auto next = [&]() -> uint32_t {
@@ -581,7 +593,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
             const float t = takex ? tx : ty;
             const bool more = takex ? xi < nx : yi < ny;
             const int at = takex ? xi : nx + yi;
-            const float v = ch[more ? at : 0][lane];
+            const float v = ch[off0 + (more ? at : 0) * step];
             const float repl = more ? v : (takex ? kX2 : kY2);
             tx = takex ? repl : tx;
             ty = takex ? ty : repl;
