@@ -524,11 +524,12 @@ __device__ inline void queue_append(int32_t* __restrict__ list, int32_t* __restr
 // (make_intersection_1.comp:310-359, ties to x), OR-merges runs of equal masked t on the fly
 // (:361-375: an entry is the OR of its run up to itself and the next entry when that is in the
 // run) and emits each entry's fragment once the next entry is known.  Longer pieces go to the
-// mid queue (one wave each) or, above kWaveStage entries, to the long queue.
+// mid lists (one list of up to 64 slots per group of 64 slots, no atomics; one wave each later)
+// or, above kWaveStage entries, to the long queue.
 __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restrict__ pieces, int64_t n_slots,
                                                         const uint8_t* __restrict__ mono,
                                                         const float* __restrict__ chain_t,
-                                                        int32_t* __restrict__ mid_list, int32_t* __restrict__ mid_count,
+                                                        int32_t* __restrict__ mid_list, int32_t* __restrict__ mid_cnt,
                                                         int32_t* __restrict__ long_list,
                                                         int32_t* __restrict__ long_count,
                                                         uint8_t* __restrict__ pieces_seq, const FragArgs A) {
@@ -546,7 +547,12 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         const bool mine = n > 0 && n <= kLaneMax;
         const bool to_long = n > kWaveStage, to_mid = n > kLaneMax && !to_long;
         if (to_long) pieces_seq[slot] = piece_mode(pd, slot, mono, chain_t);
-        queue_append(mid_list, mid_count, (int32_t)slot, to_mid);
+        {
+            const unsigned long long m = __ballot(to_mid);
+            const int64_t group = (slot - lane) / 64;  // this wave's 64 slots
+            if (to_mid) mid_list[group * 64 + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)slot;
+            if (lane == 0 && group * 64 < n_slots) mid_cnt[group] = (int32_t)__popcll(m);
+        }
         queue_append(long_list, long_count, (int32_t)slot, to_long);
         float tend = 1.0f;
         uint32_t path = 0, bb = 0;
@@ -649,11 +655,12 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
     }
 }
 
-// One wave per queued piece, four waves per workgroup, in queue order with a stride.  The
-// piece's chains are staged in the wave's LDS, merged, OR-merged, and turned into fragments.
+// One workgroup per mid list (the mid pieces of 64 consecutive slots): its four waves take the
+// list's pieces in turn, one wave per piece.  The piece's chains are staged in the wave's LDS,
+// merged, OR-merged, and turned into fragments.
 __global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __restrict__ pieces,
                                                             const int32_t* __restrict__ mid_list,
-                                                            const int32_t* __restrict__ mid_count,
+                                                            const int32_t* __restrict__ mid_cnt, int64_t n_groups,
                                                             const uint8_t* __restrict__ mono,
                                                             const float* __restrict__ chain_t, const FragArgs A) {
     __shared__ float s_in[kBlock / 64][kWaveStage];
@@ -662,10 +669,13 @@ __global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __r
     float* xin = s_in[wave];
     uint32_t* vout = s_out[wave];
     uint32_t* fin = reinterpret_cast<uint32_t*>(xin);  // the staged chains are dead after the merge
-    const int32_t count = *mid_count;
-    const int32_t stride = (int32_t)gridDim.x * (kBlock / 64);
-    for (int32_t q = (int32_t)blockIdx.x * (kBlock / 64) + wave; q < count; q += stride) {
-        const int64_t slot = mid_list[q];
+    const int64_t g = blockIdx.x;
+    if (g >= n_groups) return;
+    {
+    const int32_t cnt = mid_cnt[g];
+    const int32_t my_slot = lane < cnt ? mid_list[g * 64 + lane] : 0;
+    for (int32_t q = wave; q < cnt; q += kBlock / 64) {
+        const int64_t slot = __builtin_amdgcn_readlane(my_slot, q);
         const PieceDesc pd = pieces[slot];
         const int nv = pd.nx + pd.ny, n = nv + 1;
         const uint8_t mode = piece_mode(pd, slot, mono, chain_t);
@@ -678,6 +688,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __r
         wave_sync();
         piece_fragments(pd, tend, fin, n, A, lane);
         wave_sync();
+    }
     }
 }
 
@@ -761,16 +772,17 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const Ch
 }
 
 hipError_t launch_piece_fragments(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                                  uint32_t* scratch, int64_t n_entries, int32_t* mid_list, int32_t* mid_count,
+                                  uint32_t* scratch, int64_t n_entries, int32_t* mid_list, int32_t* mid_cnt,
                                   int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, const FragArgs& A,
                                   int n_cu, hipStream_t s) {
     if (C == 0) return hipSuccess;
     const int64_t slots = (int64_t)C * 5;
     int64_t g = (slots + kBlock - 1) / kBlock;
     if (g > (int64_t)n_cu * 20) g = (int64_t)n_cu * 20;
-    k_piece_small<<<(unsigned)g, kBlock, 0, s>>>(pieces, slots, mono, chain_t, mid_list, mid_count, long_list,
+    k_piece_small<<<(unsigned)g, kBlock, 0, s>>>(pieces, slots, mono, chain_t, mid_list, mid_cnt, long_list,
                                                  long_count, pieces_seq, A);
-    k_piece_fragments<<<(unsigned)(n_cu * 16), kBlock, 0, s>>>(pieces, mid_list, mid_count, mono, chain_t, A);
+    k_piece_fragments<<<(unsigned)((slots + 63) / 64), kBlock, 0, s>>>(pieces, mid_list, mid_cnt, (slots + 63) / 64,
+                                                                       mono, chain_t, A);
     k_piece_fragments_long<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, long_list, long_count, pieces_seq, chain_t,
                                                                scratch, n_entries, A);
     return hipGetLastError();

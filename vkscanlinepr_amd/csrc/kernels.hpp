@@ -85,10 +85,11 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const Ch
 // Merges each piece's x and y chains into its (curve, t bits) entries, OR-merging equal t (the
 // reference's per-piece walk), and turns the entries into fragment keys, winding changes and
 // bucket counts (gen_fragment) without storing the entries unless A.inter is set.  Pieces are
-// taken by one lane (short), one wave (queued in mid_list) or one wave with more LDS (queued in
-// long_list); scratch holds 2 * n_entries words for pieces too long for LDS.
+// taken by one lane (short), one wave (listed in mid_list: up to 64 slots per group of 64 piece
+// slots, mid_cnt[group] of them) or one wave with more LDS (queued in long_list); scratch holds
+// 2 * n_entries words for pieces too long for LDS.
 hipError_t launch_piece_fragments(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                                  uint32_t* scratch, int64_t n_entries, int32_t* mid_list, int32_t* mid_count,
+                                  uint32_t* scratch, int64_t n_entries, int32_t* mid_list, int32_t* mid_cnt,
                                   int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, const FragArgs& A,
                                   int n_cu, hipStream_t s);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,

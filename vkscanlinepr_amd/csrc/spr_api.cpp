@@ -104,7 +104,7 @@ struct spr_handle {
     DBuf<float> chain_t;
     DBuf<spr::ChainDesc> chains;
     DBuf<spr::PieceDesc> pieces;
-    DBuf<int32_t> chain_order, long_list, mid_list;
+    DBuf<int32_t> chain_order, long_list, mid_list, mid_cnt;
     DBuf<uint8_t> mono, pieces_seq, chain_cls;
     int n_cu = 256;
     DBuf<int32_t> key;
@@ -316,7 +316,8 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         H->pieces.ensure((size_t)C * 5);
         H->chain_order.ensure((size_t)C * 10);
         H->long_list.ensure((size_t)C * 5);
-        H->mid_list.ensure((size_t)C * 5);
+        H->mid_list.ensure((size_t)C * 5 + 64);
+        H->mid_cnt.ensure(((size_t)C * 5 + 63) / 64);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         H->pieces_seq.ensure((size_t)C * 5 + 4);
@@ -340,7 +341,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err,
                               debug ? H->inter.p : nullptr};
         SPR_HIP(spr::launch_piece_fragments(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p,
-                                            (int64_t)n1, H->mid_list.p, d_cc + spr::kCcMidPieces, H->long_list.p,
+                                            (int64_t)n1, H->mid_list.p, H->mid_cnt.p, H->long_list.p,
                                             d_nlpieces, H->pieces_seq.p, A, H->n_cu, s));
     }
     stage_mark(H, 6);
@@ -559,7 +560,8 @@ void run_learned_frame(spr_handle* H) {
         H->pieces.ensure((size_t)C * 5);
         H->chain_order.ensure((size_t)C * 10);
         H->long_list.ensure((size_t)C * 5);
-        H->mid_list.ensure((size_t)C * 5);
+        H->mid_list.ensure((size_t)C * 5 + 64);
+        H->mid_cnt.ensure(((size_t)C * 5 + 63) / 64);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         H->pieces_seq.ensure((size_t)C * 5 + 4);
@@ -581,7 +583,7 @@ void run_learned_frame(spr_handle* H) {
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, H->hist_l.p, d_err,
                               debug ? H->inter.p : nullptr};
         SPR_HIP(spr::launch_piece_fragments(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p, nl,
-                                            H->mid_list.p, d_cc + spr::kCcMidPieces, H->long_list.p, d_nlpieces,
+                                            H->mid_list.p, H->mid_cnt.p, H->long_list.p, d_nlpieces,
                                             H->pieces_seq.p, A, H->n_cu * H->merge_k / 8, s));
     }
     stage_mark(H, 6);
@@ -856,7 +858,7 @@ int spr_destroy(spr_handle* H) {
     H->rule.release(); H->fill.release(); H->tpos.release(); H->visible.release(); H->cuts.release();
     H->ncuts.release(); H->counts.release(); H->seg.release(); H->inter.release();
     H->chain_t.release(); H->chains.release(); H->pieces.release(); H->chain_order.release();
-    H->long_list.release(); H->mid_list.release(); H->chain_cls.release(); H->mono.release(); H->pieces_seq.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
+    H->long_list.release(); H->mid_list.release(); H->mid_cnt.release(); H->chain_cls.release(); H->mono.release(); H->pieces_seq.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
