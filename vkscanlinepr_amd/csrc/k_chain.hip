@@ -41,6 +41,9 @@ constexpr int kLongClass = 10;     // cubic chains of 2^10 = 1024+ crossings get
 constexpr int kStage = 4096;       // piece values staged in LDS by the merging wave
 constexpr int kClasses = kChainClasses;  // 0..63: cubic chains (4 per octave), 64..79: line chains
 constexpr int kRounds = 4, kLevels = 6;  // wave solve: 4 rounds of 6 bisection levels
+// first heavy cubic class: a quarter-octave class 4*l + sub holds lengths >= 2^l (l = 9: 512)
+constexpr int kHeavyClass = 4 * 9;
+static_assert(kHeavyCrossings == 1 << 9, "heavy class boundary");
 static_assert(kRounds * kLevels == kCubicIters, "wave solve must run the reference's iterations");
 
 inline unsigned grid_for(int64_t n) {
@@ -163,9 +166,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
 // line class 64+l -> 8l; the keys are distinct.  The cubic classes of kLongClass octaves and up
 // come first: the wave-solved chains.
 __global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t* __restrict__ class_off,
-                                int32_t* __restrict__ n_long) {
+                                int32_t* __restrict__ n_long, int32_t* __restrict__ light_begin) {
     if (threadIdx.x != 0) return;
-    int32_t off = 0, nl = 0;
+    int32_t off = 0, nl = 0, lb = 0;
     for (int key = 2 * (63 + 24) + 1; key >= 0; --key) {
         int c = -1;
         if (key & 1) c = (key - 1) / 2 - 24;
@@ -174,9 +177,11 @@ __global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t
         class_off[c] = off;
         off += class_count[c];
         if (c < 64 && c >= 4 * kLongClass) nl = off;
+        if (c < 64 && c >= kHeavyClass) lb = off;  // heavy classes come first, light ones after
     }
     class_off[kClasses] = off;
     *n_long = nl;
+    *light_begin = lb;
 }
 
 // Chains in class order: each workgroup owns a contiguous range of chain slots, counts its
@@ -306,13 +311,18 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
                                                         const int32_t* __restrict__ order,
                                                         const int32_t* __restrict__ class_off,
                                                         const int32_t* __restrict__ n_long,
+                                                        const int32_t* __restrict__ light_begin, int heavy,
                                                         int32_t* __restrict__ work,
                                                         float* __restrict__ chain_t,
                                                         uint8_t* __restrict__ mono) {
     const int lane = threadIdx.x & 63;
     const int32_t total = class_off[kClasses];
-    const int32_t nl = *n_long;
-    const int32_t items = nl + (total - nl + 63) / 64;
+    const int32_t lb = *light_begin;
+    // heavy launch: wave items on chains [0, nl), lane items on [nl, lb); light: lane items on
+    // [lb, total)
+    const int32_t nl = heavy ? *n_long : 0;
+    const int32_t lane0 = heavy ? nl : lb, lane1 = heavy ? lb : total;
+    const int32_t items = nl + (lane1 - lane0 + 63) / 64;
     const int32_t n_waves = (int32_t)gridDim.x * (kBlock / 64);
     int32_t item = (int32_t)blockIdx.x * (kBlock / 64) + (int32_t)(threadIdx.x >> 6);
     while (item < items) {
@@ -324,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
             solve_chain_wave(d, a, chain_t + d.out, mono + ci);
             __builtin_amdgcn_s_setprio(0);
         } else {
-            const int32_t k0 = nl + (item - nl) * 64;
+            const int32_t k0 = lane0 + (item - nl) * 64;
             const int32_t k = k0 + lane;
             // issue priority follows the cost of the wave's chains (the longest run first and
             // must not be starved by the short ones)
@@ -332,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
             if (oct >= 8) __builtin_amdgcn_s_setprio(3);
             else if (oct >= 6) __builtin_amdgcn_s_setprio(2);
             else if (oct >= 4) __builtin_amdgcn_s_setprio(1);
-            if (k < total) {
+            if (k < lane1) {
                 const int32_t ci = order[k];
                 const ChainDesc d = chains[ci];
                 const CurveAxis a = load_axis(tpos, cpos, d);
@@ -529,10 +539,7 @@ __device__ inline void queue_append(int32_t* __restrict__ list, int32_t* __restr
 __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restrict__ pieces, int64_t n_slots,
                                                         const uint8_t* __restrict__ mono,
                                                         const float* __restrict__ chain_t,
-                                                        int32_t* __restrict__ mid_list, int32_t* __restrict__ mid_cnt,
-                                                        int32_t* __restrict__ long_list,
-                                                        int32_t* __restrict__ long_count,
-                                                        uint8_t* __restrict__ pieces_seq, const FragArgs A) {
+                                                        const PieceLists L, const FragArgs A) {
     __shared__ float s_ch[kBlock / 64][kLaneMax - 1][64];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* ch = &s_ch[wave][0][0];
@@ -544,16 +551,29 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         pd.base = -1; pd.nx = pd.ny = pd.curve = 0; pd.t0 = pd.tend = 0.0f;
         if (slot < n_slots) pd = pieces[slot];
         const int nx = pd.nx, ny = pd.ny, nv = nx + ny, n = pd.base >= 0 ? nv + 1 : 0;
-        const bool mine = n > 0 && n <= kLaneMax;
-        const bool to_long = n > kWaveStage, to_mid = n > kLaneMax && !to_long;
-        if (to_long) pieces_seq[slot] = piece_mode(pd, slot, mono, chain_t);
-        {
-            const unsigned long long m = __ballot(to_mid);
-            const int64_t group = (slot - lane) / 64;  // this wave's 64 slots
-            if (to_mid) mid_list[group * 64 + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)slot;
-            if (lane == 0 && group * 64 < n_slots) mid_cnt[group] = (int32_t)__popcll(m);
+        // light: the piece's chains and the next piece's first x crossing (its last segment
+        // ends there) are all solved by the light launch
+        int n_next = __shfl_down(n, 1);
+        if (lane == 63 && pd.base >= 0 && pd.tend < 0.0f && slot + 1 < n_slots) {
+            const PieceDesc nd = pieces[slot + 1];
+            n_next = nd.base >= 0 ? nd.nx + nd.ny + 1 : 0;
         }
-        queue_append(long_list, long_count, (int32_t)slot, to_long);
+        const bool light = n <= kHeavyCrossings && (pd.tend >= 0.0f || n_next <= kHeavyCrossings);
+        const bool mine = n > 0 && n <= kLaneMax && light;
+        const bool to_long = n > kWaveStage, to_lo = n > kLaneMax && light && !to_long;
+        const bool to_hi = n > 0 && !mine && !to_lo && !to_long;
+        {
+            const int64_t group = (slot - lane) / 64;  // this wave's 64 slots
+            const unsigned long long ml = __ballot(to_lo), mh = __ballot(to_hi);
+            const unsigned long long below = (1ull << lane) - 1ull;
+            if (to_lo) L.lo[group * 64 + __popcll(ml & below)] = (int32_t)slot;
+            if (to_hi) L.hi[group * 64 + __popcll(mh & below)] = (int32_t)slot;
+            if (lane == 0 && group * 64 < n_slots) {
+                L.lo_cnt[group] = (int32_t)__popcll(ml);
+                L.hi_cnt[group] = (int32_t)__popcll(mh);
+            }
+        }
+        queue_append(L.long_list, L.long_count, (int32_t)slot, to_long);
         float tend = 1.0f;
         uint32_t path = 0, bb = 0;
         int2 pr = make_int2(0, 0);
@@ -698,7 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __r
 __global__ __launch_bounds__(64) void k_piece_fragments_long(const PieceDesc* __restrict__ pieces,
                                                              const int32_t* __restrict__ long_list,
                                                              const int32_t* __restrict__ long_count,
-                                                             const uint8_t* __restrict__ piece_mode,
+                                                             const uint8_t* __restrict__ mono,
                                                              const float* __restrict__ chain_t,
                                                              uint32_t* __restrict__ scratch, int64_t n_entries,
                                                              const FragArgs A) {
@@ -710,7 +730,9 @@ __global__ __launch_bounds__(64) void k_piece_fragments_long(const PieceDesc* __
         const int32_t pi = long_list[q];
         const PieceDesc pd = pieces[pi];
         const int nv = pd.nx + pd.ny, n = nv + 1;
-        const uint8_t mode = piece_mode[pi];
+        // the mode is taken here, after every chain is solved (the queueing kernel may run
+        // before the heavy chains are)
+        const uint8_t mode = piece_mode(pd, pi, mono, chain_t);
         const float tend = piece_end_t(pieces, pi, pd, chain_t);
         // the two branches keep every pointer in one address space (no flat accesses)
         if (n <= kStage) {
@@ -753,10 +775,10 @@ hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uin
 }
 
 hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_t* class_count,
-                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order, int n_cu,
-                              hipStream_t s) {
+                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* light_begin,
+                              int32_t* order, int n_cu, hipStream_t s) {
     if (C == 0) return hipSuccess;
-    k_chain_offsets<<<1, 64, 0, s>>>(class_count, class_off, n_long);
+    k_chain_offsets<<<1, 64, 0, s>>>(class_count, class_off, n_long, light_begin);
     unsigned g = grid_for((int64_t)C * 10);
     if (g > (unsigned)n_cu * 4) g = (unsigned)n_cu * 4;
     k_chain_order<<<g, kBlock, 0, s>>>(chain_cls, (int64_t)C * 10, class_fill, class_off, order);
@@ -765,25 +787,37 @@ hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_
 
 hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
                               const int32_t* order, const int32_t* class_off, const int32_t* n_long,
-                              int32_t* work, float* chain_t, uint8_t* mono, int n_cu, hipStream_t s) {
-    k_chain_solve<<<(unsigned)(n_cu * 8), kBlock, 0, s>>>(tpos, cpos, chains, order, class_off, n_long, work,
-                                                          chain_t, mono);
+                              const int32_t* light_begin, int heavy, int32_t* work, float* chain_t, uint8_t* mono,
+                              int n_cu, hipStream_t s) {
+    // the heavy launch holds a few hundred long items: one wave per SIMD at most
+    const unsigned grid = heavy ? (unsigned)n_cu : (unsigned)(n_cu * 8);
+    k_chain_solve<<<grid, kBlock, 0, s>>>(tpos, cpos, chains, order, class_off, n_long, light_begin, heavy, work,
+                                          chain_t, mono);
     return hipGetLastError();
 }
 
-hipError_t launch_piece_fragments(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                                  uint32_t* scratch, int64_t n_entries, int32_t* mid_list, int32_t* mid_cnt,
-                                  int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, const FragArgs& A,
-                                  int n_cu, hipStream_t s) {
+hipError_t launch_piece_small(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                              const PieceLists& L, const FragArgs& A, int n_cu, hipStream_t s) {
     if (C == 0) return hipSuccess;
     const int64_t slots = (int64_t)C * 5;
     int64_t g = (slots + kBlock - 1) / kBlock;
     if (g > (int64_t)n_cu * 20) g = (int64_t)n_cu * 20;
-    k_piece_small<<<(unsigned)g, kBlock, 0, s>>>(pieces, slots, mono, chain_t, mid_list, mid_cnt, long_list,
-                                                 long_count, pieces_seq, A);
-    k_piece_fragments<<<(unsigned)((slots + 63) / 64), kBlock, 0, s>>>(pieces, mid_list, mid_cnt, (slots + 63) / 64,
-                                                                       mono, chain_t, A);
-    k_piece_fragments_long<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, long_list, long_count, pieces_seq, chain_t,
+    k_piece_small<<<(unsigned)g, kBlock, 0, s>>>(pieces, slots, mono, chain_t, L, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_piece_mid(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                            const int32_t* list, const int32_t* cnt, const FragArgs& A, hipStream_t s) {
+    if (C == 0) return hipSuccess;
+    const int64_t groups = ((int64_t)C * 5 + 63) / 64;
+    k_piece_fragments<<<(unsigned)groups, kBlock, 0, s>>>(pieces, list, cnt, groups, mono, chain_t, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_piece_long(const PieceDesc* pieces, const PieceLists& L, const uint8_t* mono, const float* chain_t,
+                             uint32_t* scratch,
+                             int64_t n_entries, const FragArgs& A, int n_cu, hipStream_t s) {
+    k_piece_fragments_long<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, L.long_list, L.long_count, mono, chain_t,
                                                                scratch, n_entries, A);
     return hipGetLastError();
 }

@@ -69,29 +69,49 @@ struct FragArgs {
 // of the persistent solve and the count of long pieces.
 constexpr int kChainClasses = 80;
 constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 * kChainClasses + 1,
-              kCcWork = kCcNLong + 1, kCcLongPieces = kCcWork + 1, kCcMidPieces = kCcLongPieces + 1,
-              kCcWords = kCcMidPieces + 1;
+              kCcWork = kCcNLong + 1, kCcLongPieces = kCcWork + 1, kCcLightBegin = kCcLongPieces + 1,
+              kCcWorkLight = kCcLightBegin + 1, kCcWords = kCcWorkLight + 1;
+// Chains of at least kHeavyCrossings crossings are "heavy": they are solved by their own launch
+// (wave-solved and the longest lane-solved chains), and the pieces of at most kHeavyCrossings
+// entries, which never wait on them, are finished by a concurrent stream meanwhile.
+constexpr int kHeavyCrossings = 512;
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
                              ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
                              int n_cu, hipStream_t s, const int32_t* ends = nullptr, const int2* pcanvas = nullptr);
 hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_t* class_count,
-                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* order, int n_cu,
-                              hipStream_t s);
+                              int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* light_begin,
+                              int32_t* order, int n_cu, hipStream_t s);
+// heavy = 1: the wave-solved chains and the lane-solved chains of kHeavyCrossings and more;
+// heavy = 0: the rest.  The two launches are independent.
 hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
                               const int32_t* order, const int32_t* class_off, const int32_t* n_long,
-                              int32_t* work, float* chain_t, uint8_t* mono, int n_cu, hipStream_t s);
+                              const int32_t* light_begin, int heavy, int32_t* work, float* chain_t, uint8_t* mono,
+                              int n_cu, hipStream_t s);
 // Merges each piece's x and y chains into its (curve, t bits) entries, OR-merging equal t (the
 // reference's per-piece walk), and turns the entries into fragment keys, winding changes and
-// bucket counts (gen_fragment) without storing the entries unless A.inter is set.  Pieces are
-// taken by one lane (short), one wave (listed in mid_list: up to 64 slots per group of 64 piece
-// slots, mid_cnt[group] of them) or one wave with more LDS (queued in long_list); scratch holds
-// 2 * n_entries words for pieces too long for LDS.
-hipError_t launch_piece_fragments(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                                  uint32_t* scratch, int64_t n_entries, int32_t* mid_list, int32_t* mid_cnt,
-                                  int32_t* long_list, int32_t* long_count, uint8_t* pieces_seq, const FragArgs& A,
-                                  int n_cu, hipStream_t s);
+// bucket counts (gen_fragment) without storing the entries unless A.inter is set.
+//  launch_piece_small: one lane per piece of at most 32 entries whose work depends on light
+//    chains only; the other pieces are listed per group of 64 piece slots, in lo (33..512
+//    entries, light chains only) and hi (the rest up to 1024) lists, or queued as long pieces.
+//  launch_piece_mid: one wave per piece of a set of per-group lists.
+//  launch_piece_long: the long queue; scratch holds 2 * n_entries words.
+struct PieceLists {
+    int32_t* lo;       // [groups * 64] slots
+    int32_t* lo_cnt;   // [groups]
+    int32_t* hi;
+    int32_t* hi_cnt;
+    int32_t* long_list;
+    int32_t* long_count;
+};
+hipError_t launch_piece_small(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                              const PieceLists& L, const FragArgs& A, int n_cu, hipStream_t s);
+hipError_t launch_piece_mid(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                            const int32_t* list, const int32_t* cnt, const FragArgs& A, hipStream_t s);
+hipError_t launch_piece_long(const PieceDesc* pieces, const PieceLists& L, const uint8_t* mono, const float* chain_t,
+                             uint32_t* scratch,
+                             int64_t n_entries, const FragArgs& A, int n_cu, hipStream_t s);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,

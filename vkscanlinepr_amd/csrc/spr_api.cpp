@@ -80,6 +80,8 @@ struct spr_handle {
     spr_opts opts{};
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // the light chain solve and the pieces that need only it
+    hipEvent_t ev_fork = nullptr, ev_light = nullptr, ev_small = nullptr, ev_join = nullptr;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::vector<hipEvent_t> ev_stage;
     bool stage_timing = false;
@@ -104,8 +106,8 @@ struct spr_handle {
     DBuf<float> chain_t;
     DBuf<spr::ChainDesc> chains;
     DBuf<spr::PieceDesc> pieces;
-    DBuf<int32_t> chain_order, long_list, mid_list, mid_cnt;
-    DBuf<uint8_t> mono, pieces_seq, chain_cls;
+    DBuf<int32_t> chain_order, long_list, mid_list, mid_cnt, hi_list, hi_cnt;
+    DBuf<uint8_t> mono, chain_cls;
     int n_cu = 256;
     DBuf<int32_t> key;
     DBuf<int8_t> wn;
@@ -207,6 +209,39 @@ void stage_mark(spr_handle* H, int idx) {
     if ((int)H->ev_stage.size() <= idx) return;
     SPR_HIP(hipEventRecord(H->ev_stage[idx], H->stream));
     H->n_marks = idx + 1;
+}
+
+// Chain solve and piece kernels of one frame, on two streams: the heavy solve runs on the main
+// stream while the second stream solves the light chains and finishes the pieces that depend on
+// them only; the main stream then finishes the pieces that wait on heavy chains.  Stage marks 4
+// and 6 are the caller's; mark 5 is set when both solves are done.
+void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragArgs& A, int64_t n_entries,
+                      int n_cu_solve, int n_cu_merge) {
+    hipStream_t s = H->stream, s2 = H->stream2;
+    const int32_t* d_coff = d_cc + spr::kCcOff;
+    const int32_t* d_nlong = d_cc + spr::kCcNLong;
+    const int32_t* d_lb = d_cc + spr::kCcLightBegin;
+    const spr::PieceLists L{H->mid_list.p, H->mid_cnt.p, H->hi_list.p, H->hi_cnt.p,
+                            H->long_list.p, d_cc + spr::kCcLongPieces};
+    SPR_HIP(hipEventRecord(H->ev_fork, s));
+    SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 1,
+                                    d_cc + spr::kCcWork, H->chain_t.p, H->mono.p, n_cu_solve, s));
+    SPR_HIP(hipStreamWaitEvent(s2, H->ev_fork, 0));
+    SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 0,
+                                    d_cc + spr::kCcWorkLight, H->chain_t.p, H->mono.p, n_cu_solve, s2));
+    SPR_HIP(hipEventRecord(H->ev_light, s2));
+    SPR_HIP(spr::launch_piece_small(H->pieces.p, C, H->mono.p, H->chain_t.p, L, A, n_cu_merge, s2));
+    SPR_HIP(hipEventRecord(H->ev_small, s2));
+    SPR_HIP(spr::launch_piece_mid(H->pieces.p, C, H->mono.p, H->chain_t.p, L.lo, L.lo_cnt, A, s2));
+    SPR_HIP(hipEventRecord(H->ev_join, s2));
+    SPR_HIP(hipStreamWaitEvent(s, H->ev_light, 0));
+    stage_mark(H, 5);
+    SPR_HIP(hipStreamWaitEvent(s, H->ev_small, 0));
+    SPR_HIP(spr::launch_piece_mid(H->pieces.p, C, H->mono.p, H->chain_t.p, L.hi, L.hi_cnt, A, s));
+    // very long pieces merge through the sort buffer, which is free until the scatter
+    SPR_HIP(spr::launch_piece_long(H->pieces.p, L, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p, n_entries, A,
+                                   n_cu_merge, s));
+    SPR_HIP(hipStreamWaitEvent(s, H->ev_join, 0));
 }
 
 void run_frame(spr_handle* H, bool record_begin = true) {
@@ -318,31 +353,25 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         H->long_list.ensure((size_t)C * 5);
         H->mid_list.ensure((size_t)C * 5 + 64);
         H->mid_cnt.ensure(((size_t)C * 5 + 63) / 64);
+        H->hi_list.ensure((size_t)C * 5 + 64);
+        H->hi_cnt.ensure(((size_t)C * 5 + 63) / 64);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
-        H->pieces_seq.ensure((size_t)C * 5 + 4);
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
                                        H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
                                        d_cc, H->n_cu, s));
-        SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, H->n_cu,
-                                        s));
+        SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, d_cc + spr::kCcLightBegin,
+                                        H->chain_order.p, H->n_cu, s));
     }
     stage_mark(H, 4);
     if (gen) {
-        SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong,
-                                        d_work, H->chain_t.p, H->mono.p, H->n_cu, s));
-    }
-    stage_mark(H, 5);
-    if (gen) {
-        // piece merge and fragment keys in one pass; very long pieces merge through the sort
-        // buffer, which is free until the scatter
-        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
+        // chain solve, then piece merge and fragment keys in one pass
         const spr::FragArgs A{H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h, nullptr,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err,
                               debug ? H->inter.p : nullptr};
-        SPR_HIP(spr::launch_piece_fragments(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p,
-                                            (int64_t)n1, H->mid_list.p, H->mid_cnt.p, H->long_list.p,
-                                            d_nlpieces, H->pieces_seq.p, A, H->n_cu, s));
+        solve_and_pieces(H, C, d_cc, A, (int64_t)n1, H->n_cu, H->n_cu);
+    } else {
+        stage_mark(H, 5);
     }
     stage_mark(H, 6);
     if (!gen)  // injected entries (or none)
@@ -562,29 +591,24 @@ void run_learned_frame(spr_handle* H) {
         H->long_list.ensure((size_t)C * 5);
         H->mid_list.ensure((size_t)C * 5 + 64);
         H->mid_cnt.ensure(((size_t)C * 5 + 63) / 64);
+        H->hi_list.ensure((size_t)C * 5 + 64);
+        H->hi_cnt.ensure(((size_t)C * 5 + 63) / 64);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
-        H->pieces_seq.ensure((size_t)C * 5 + 4);
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
                                        H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
                                        d_cc, H->n_cu, s, H->cend.p, pcv));
-        SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, H->chain_order.p, H->n_cu,
-                                        s));
+        SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, d_cc + spr::kCcLightBegin,
+                                        H->chain_order.p, H->n_cu, s));
     }
     stage_mark(H, 4);
     if (C) {
-        SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong,
-                                        d_work, H->chain_t.p, H->mono.p, H->n_cu * H->chain_k / 8, s));
-    }
-    stage_mark(H, 5);
-    if (C) {
-        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->pieces_seq.p, ((int64_t)C * 5 + 3) / 4, 0u, s));
         const spr::FragArgs A{H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h, pcv,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, H->hist_l.p, d_err,
                               debug ? H->inter.p : nullptr};
-        SPR_HIP(spr::launch_piece_fragments(H->pieces.p, C, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p, nl,
-                                            H->mid_list.p, H->mid_cnt.p, H->long_list.p, d_nlpieces,
-                                            H->pieces_seq.p, A, H->n_cu * H->merge_k / 8, s));
+        solve_and_pieces(H, C, d_cc, A, nl, H->n_cu * H->chain_k / 8, H->n_cu * H->merge_k / 8);
+    } else {
+        stage_mark(H, 5);
     }
     stage_mark(H, 6);
     stage_mark(H, 7);
@@ -835,6 +859,9 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
             H->n_cu = ncu;
         (void)hipGetLastError();
         SPR_HIP(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
+        SPR_HIP(hipStreamCreateWithFlags(&H->stream2, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&H->ev_fork, &H->ev_light, &H->ev_small, &H->ev_join})
+            SPR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
         SPR_HIP(hipEventCreate(&H->ev_begin));
         SPR_HIP(hipEventCreate(&H->ev_end));
         H->ev_stage.resize(16);
@@ -858,7 +885,7 @@ int spr_destroy(spr_handle* H) {
     H->rule.release(); H->fill.release(); H->tpos.release(); H->visible.release(); H->cuts.release();
     H->ncuts.release(); H->counts.release(); H->seg.release(); H->inter.release();
     H->chain_t.release(); H->chains.release(); H->pieces.release(); H->chain_order.release();
-    H->long_list.release(); H->mid_list.release(); H->mid_cnt.release(); H->chain_cls.release(); H->mono.release(); H->pieces_seq.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
+    H->long_list.release(); H->mid_list.release(); H->mid_cnt.release(); H->hi_list.release(); H->hi_cnt.release(); H->chain_cls.release(); H->mono.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
@@ -873,6 +900,9 @@ int spr_destroy(spr_handle* H) {
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
     if (H->ev_end) (void)hipEventDestroy(H->ev_end);
+    for (hipEvent_t e : {H->ev_fork, H->ev_light, H->ev_small, H->ev_join})
+        if (e) (void)hipEventDestroy(e);
+    if (H->stream2) (void)hipStreamDestroy(H->stream2);
     if (H->stream) (void)hipStreamDestroy(H->stream);
     (void)hipGetLastError();  // nothing of a destroyed handle may leak into later calls
     delete H;
