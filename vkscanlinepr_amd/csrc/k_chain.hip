@@ -552,13 +552,15 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         if (slot < n_slots) pd = pieces[slot];
         const int nx = pd.nx, ny = pd.ny, nv = nx + ny, n = pd.base >= 0 ? nv + 1 : 0;
         // light: the piece's chains and the next piece's first x crossing (its last segment
-        // ends there) are all solved by the light launch
-        int n_next = __shfl_down(n, 1);
+        // ends there when that piece has crossings on both axes) are all solved by the light
+        // launch (chains of fewer than kHeavyCrossings crossings)
+        const bool x_light = pd.base < 0 || pd.ny == 0 || pd.nx < kHeavyCrossings;  // as a next piece
+        bool next_ok = __shfl_down(x_light ? 1 : 0, 1) != 0;
         if (lane == 63 && pd.base >= 0 && pd.tend < 0.0f && slot + 1 < n_slots) {
             const PieceDesc nd = pieces[slot + 1];
-            n_next = nd.base >= 0 ? nd.nx + nd.ny + 1 : 0;
+            next_ok = nd.base < 0 || nd.ny == 0 || nd.nx < kHeavyCrossings;
         }
-        const bool light = n <= kHeavyCrossings && (pd.tend >= 0.0f || n_next <= kHeavyCrossings);
+        const bool light = nx < kHeavyCrossings && ny < kHeavyCrossings && (pd.tend >= 0.0f || next_ok);
         const bool mine = n > 0 && n <= kLaneMax && light;
         const bool to_long = n > kWaveStage, to_lo = n > kLaneMax && light && !to_long;
         const bool to_hi = n > 0 && !mine && !to_lo && !to_long;

@@ -93,7 +93,7 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const Ch
 // reference's per-piece walk), and turns the entries into fragment keys, winding changes and
 // bucket counts (gen_fragment) without storing the entries unless A.inter is set.
 //  launch_piece_small: one lane per piece of at most 32 entries whose work depends on light
-//    chains only; the other pieces are listed per group of 64 piece slots, in lo (33..512
+//    chains only; the other pieces are listed per group of 64 piece slots, in lo (up to 1024
 //    entries, light chains only) and hi (the rest up to 1024) lists, or queued as long pieces.
 //  launch_piece_mid: one wave per piece of a set of per-group lists.
 //  launch_piece_long: the long queue; scratch holds 2 * n_entries words.
