@@ -23,12 +23,13 @@
 //     lane with the reference's sequential walk, OR-merged and turned into fragment keys,
 //     winding changes and bucket counts (gen_fragment: the segment from an entry to the next one
 //     of its curve) in one streaming pass; the entries themselves are only stored for export.
-//     Longer pieces are queued.
-//  5. k_piece_fragments: one wave per queued piece stages its two chains in LDS, merges them with
+//     Longer pieces are listed (lo lists) or queued (the heavy queue).
+//  5. k_piece_fragments: one wave per listed piece stages its two chains in LDS, merges them with
 //     its start in the reference's order (merge-path when both are non-decreasing, a
 //     concatenation when both are non-increasing, the sequential walk otherwise), applies the
-//     equal-t OR-merge there and produces the same fragment outputs.  Pieces of more than 1024
-//     entries go to k_piece_fragments_long, which does the same with more LDS.
+//     equal-t OR-merge there and produces the same fragment outputs.  Pieces that wait on heavy
+//     chains or hold more than 1024 entries go to k_piece_heavy, which does the same with a
+//     whole workgroup per piece and more LDS.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -381,9 +382,11 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// Merges one piece held in LDS (X = xs[0..nx), Y = xs[nx..nx+ny)) into out[0..n) with the 64
-// lanes of a wave, in the order of make_intersection_1.comp:310-359, ties to x.
-__device__ inline void wave_merge(const PieceDesc& pd, uint8_t mode, const float* X, uint32_t* out, int lane) {
+// Merges one piece held in LDS (X = xs[0..nx), Y = xs[nx..nx+ny)) into out[0..n) with nthr
+// threads (a wave or a workgroup; lane = the thread's index among them), in the order of
+// make_intersection_1.comp:310-359, ties to x.
+__device__ inline void wave_merge(const PieceDesc& pd, uint8_t mode, const float* X, uint32_t* out, int lane,
+                                  int nthr = 64) {
     const int nx = pd.nx, ny = pd.ny;
     const float* Y = X + nx;
     const bool head_in_a = ny == 0;
@@ -418,14 +421,14 @@ __device__ inline void wave_merge(const PieceDesc& pd, uint8_t mode, const float
         }
     } else if (mode == 1) {
         const bool a_first = nb == 0 || (na > 0 && Aat(0) <= Bat(0));
-        for (int o = lane; o < n; o += 64) {
+        for (int o = lane; o < n; o += nthr) {
             float v;
             if (a_first) v = o < na ? Aat(o) : Bat(o - na);
             else v = o < nb ? Bat(o) : Aat(o - nb);
             out[o] = fbits(v);
         }
     } else {
-        const int per = (n + 63) / 64;
+        const int per = (n + nthr - 1) / nthr;
         const int o0 = lane * per;
         const int o1 = o0 + per < n ? o0 + per : n;
         if (o0 < n) {
@@ -476,40 +479,80 @@ __device__ inline float piece_end_t(const PieceDesc* __restrict__ pieces, int64_
     return glmax(bitsf(fbits(first) & 0xFFFFFFFCu), 0.0f);
 }
 
+// What the merge and the fragments of one piece need besides its chains.  One lane gathers it
+// (a chain of dependent loads), the wave reads it with meta_of_lane.
+struct PieceMeta {
+    PieceDesc pd;
+    float t_end;        // parameter where the last entry's segment ends
+    uint32_t path, bb;  // path, its first bucket
+    int32_t w, h, mode;
+    int2 pr;            // the path's row range
+    CurvePts cv;
+};
+
+__device__ inline PieceMeta gather_meta(const PieceDesc* __restrict__ pieces, int64_t slot,
+                                        const uint8_t* __restrict__ mono, const float* __restrict__ chain_t,
+                                        const FragArgs& A) {
+    PieceMeta m;
+    m.pd = pieces[slot];
+    m.mode = piece_mode(m.pd, slot, mono, chain_t);
+    m.t_end = piece_end_t(pieces, slot, m.pd, chain_t);
+    const uint32_t c = (uint32_t)m.pd.curve;
+    m.path = A.cpath[c];
+    m.w = A.w; m.h = A.h;
+    if (A.pcanvas) { const int2 cw = A.pcanvas[m.path]; m.w = cw.x; m.h = cw.y; }
+    m.cv = load_curve(A.tpos, A.cpos[c], A.ctype[c]);
+    m.bb = (uint32_t)A.bucket_base[m.path];
+    m.pr = A.prow[m.path];
+    return m;
+}
+
+__device__ inline int32_t rl_i(int32_t v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ inline float rl_f(float v, int k) { return bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(v), k)); }
+
+// Lane k's meta, wave-uniform (k must be uniform).
+__device__ inline PieceMeta meta_of_lane(const PieceMeta& m, int k) {
+    PieceMeta r;
+    r.pd.base = rl_i(m.pd.base, k); r.pd.nx = rl_i(m.pd.nx, k); r.pd.ny = rl_i(m.pd.ny, k);
+    r.pd.curve = rl_i(m.pd.curve, k); r.pd.t0 = rl_f(m.pd.t0, k); r.pd.tend = rl_f(m.pd.tend, k);
+    r.t_end = rl_f(m.t_end, k);
+    r.path = (uint32_t)rl_i((int32_t)m.path, k); r.bb = (uint32_t)rl_i((int32_t)m.bb, k);
+    r.w = rl_i(m.w, k); r.h = rl_i(m.h, k); r.mode = rl_i(m.mode, k);
+    r.pr = make_int2(rl_i(m.pr.x, k), rl_i(m.pr.y, k));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { r.cv.x[i] = rl_f(m.cv.x[i], k); r.cv.y[i] = rl_f(m.cv.y[i], k); }
+    r.cv.cubic = rl_i(m.cv.cubic ? 1 : 0, k) != 0;
+    return r;
+}
+
 // gen_fragment for the entries of one piece, fin[0..n) = their final t bits: entry o's segment
-// runs to entry o+1, the last one's to tend.  Keys, winding changes and paths are written
-// coalesced; bucket counts are aggregated over runs of the wave.  Every lane of the wave calls it.
-__device__ inline void piece_fragments(const PieceDesc& pd, float tend, const uint32_t* fin, int n,
-                                       const FragArgs& A, int lane) {
-    const uint32_t c = (uint32_t)pd.curve;
-    const uint32_t path = A.cpath[c];
-    int w = A.w, h = A.h;
-    if (A.pcanvas) { const int2 cv = A.pcanvas[path]; w = cv.x; h = cv.y; }
-    const CurvePts cv = load_curve(A.tpos, A.cpos[c], A.ctype[c]);
-    const uint32_t bb = (uint32_t)A.bucket_base[path];
-    const int2 pr = A.prow[path];
-    // lane l evaluates point o0 + l (point o is entry o's parameter, point n is tend); entry o's
+// runs to entry o+1, the last one's to m.t_end.  Keys, winding changes and paths are written
+// coalesced; bucket counts are aggregated over runs of the wave.  Every lane of the n_waves
+// waves calls it; wave w takes the chunks w, w + n_waves, ...
+__device__ inline void piece_fragments(const PieceMeta& m, const uint32_t* fin, int n, const FragArgs& A, int lane,
+                                       int wave = 0, int n_waves = 1) {
+    // lane l evaluates point o0 + l (point o is entry o's parameter, point n is t_end); entry o's
     // segment runs from point o to point o + 1, so a chunk of 64 points serves 63 entries
-    for (int o0 = 0; o0 < n; o0 += 63) {
+    for (int o0 = wave * 63; o0 < n; o0 += n_waves * 63) {
         const int o = o0 + lane;
         float t = 0.0f;
         uint32_t f = 0;
         if (o < n) { f = fin[o]; t = glmax(bitsf(f & 0xFFFFFFFCu), 0.0f); }
-        else if (o == n) t = tend;
-        const float px = cv.ex(t), py = cv.ey(t);
+        else if (o == n) t = m.t_end;
+        const float px = m.cv.ex(t), py = m.cv.ey(t);
         const float t1 = __shfl_down(t, 1), qx = __shfl_down(px, 1), qy = __shfl_down(py, 1);
         const bool valid = lane < 63 && o < n;
         uint32_t bucket = 0;
         if (valid) {
             uint32_t yx;
             int swn;
-            segment_cell_pts(t < t1, px, py, qx, qy, w, h, yx, swn);
-            const int64_t i = (int64_t)pd.base + o;
+            segment_cell_pts(t < t1, px, py, qx, qy, m.w, m.h, yx, swn);
+            const int64_t i = (int64_t)m.pd.base + o;
             A.key[i] = (int32_t)yx;
             A.wn[i] = (int8_t)swn;
-            A.epath[i] = path;
-            if (A.inter) A.inter[i] = make_int2(pd.curve, (int32_t)f);
-            bucket = bb + bucket_local((int32_t)yx, pr, A.err);
+            A.epath[i] = m.path;
+            if (A.inter) A.inter[i] = make_int2(m.pd.curve, (int32_t)f);
+            bucket = m.bb + bucket_local((int32_t)yx, m.pr, A.err);
         }
         wave_count(A.hist, bucket, valid);
     }
@@ -534,8 +577,9 @@ __device__ inline void queue_append(int32_t* __restrict__ list, int32_t* __restr
 // (make_intersection_1.comp:310-359, ties to x), OR-merges runs of equal masked t on the fly
 // (:361-375: an entry is the OR of its run up to itself and the next entry when that is in the
 // run) and emits each entry's fragment once the next entry is known.  Longer pieces go to the
-// mid lists (one list of up to 64 slots per group of 64 slots, no atomics; one wave each later)
-// or, above kWaveStage entries, to the long queue.
+// lo lists when their chains are light and they hold at most kWaveStage entries (one list of up
+// to 64 slots per group of 64 slots, no atomics; one wave per piece later), else to the heavy
+// queue (one workgroup per piece once every chain is solved).
 __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restrict__ pieces, int64_t n_slots,
                                                         const uint8_t* __restrict__ mono,
                                                         const float* __restrict__ chain_t,
@@ -562,20 +606,15 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         }
         const bool light = nx < kHeavyCrossings && ny < kHeavyCrossings && (pd.tend >= 0.0f || next_ok);
         const bool mine = n > 0 && n <= kLaneMax && light;
-        const bool to_long = n > kWaveStage, to_lo = n > kLaneMax && light && !to_long;
-        const bool to_hi = n > 0 && !mine && !to_lo && !to_long;
+        const bool to_lo = n > kLaneMax && n <= kWaveStage && light;
+        const bool to_heavy = n > 0 && !mine && !to_lo;
         {
             const int64_t group = (slot - lane) / 64;  // this wave's 64 slots
-            const unsigned long long ml = __ballot(to_lo), mh = __ballot(to_hi);
-            const unsigned long long below = (1ull << lane) - 1ull;
-            if (to_lo) L.lo[group * 64 + __popcll(ml & below)] = (int32_t)slot;
-            if (to_hi) L.hi[group * 64 + __popcll(mh & below)] = (int32_t)slot;
-            if (lane == 0 && group * 64 < n_slots) {
-                L.lo_cnt[group] = (int32_t)__popcll(ml);
-                L.hi_cnt[group] = (int32_t)__popcll(mh);
-            }
+            const unsigned long long ml = __ballot(to_lo);
+            if (to_lo) L.lo[group * 64 + __popcll(ml & ((1ull << lane) - 1ull))] = (int32_t)slot;
+            if (lane == 0 && group * 64 < n_slots) L.lo_cnt[group] = (int32_t)__popcll(ml);
         }
-        queue_append(L.long_list, L.long_count, (int32_t)slot, to_long);
+        queue_append(L.heavy, L.heavy_count, (int32_t)slot, to_heavy);
         float tend = 1.0f;
         uint32_t path = 0, bb = 0;
         int2 pr = make_int2(0, 0);
@@ -677,87 +716,93 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
     }
 }
 
-// One workgroup per mid list (the mid pieces of 64 consecutive slots): its four waves take the
-// list's pieces in turn, one wave per piece.  The piece's chains are staged in the wave's LDS,
-// merged, OR-merged, and turned into fragments.
+// One workgroup per lo list (the lo pieces of 64 consecutive slots): its four waves take the
+// list's pieces in turn, one wave per piece.  Lane k of wave w gathers the meta of the list's
+// piece w + 4k up front, so the wave's pieces pay that chain of loads once.  The piece's chains
+// are staged in the wave's LDS, merged, OR-merged, and turned into fragments.
 __global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __restrict__ pieces,
-                                                            const int32_t* __restrict__ mid_list,
-                                                            const int32_t* __restrict__ mid_cnt, int64_t n_groups,
+                                                            const int32_t* __restrict__ lo_list,
+                                                            const int32_t* __restrict__ lo_cnt, int64_t n_groups,
                                                             const uint8_t* __restrict__ mono,
                                                             const float* __restrict__ chain_t, const FragArgs A) {
     __shared__ float s_in[kBlock / 64][kWaveStage];
     __shared__ uint32_t s_out[kBlock / 64][kWaveStage];
+    constexpr int kWaves = kBlock / 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float* xin = s_in[wave];
     uint32_t* vout = s_out[wave];
     uint32_t* fin = reinterpret_cast<uint32_t*>(xin);  // the staged chains are dead after the merge
     const int64_t g = blockIdx.x;
     if (g >= n_groups) return;
-    {
-    const int32_t cnt = mid_cnt[g];
-    const int32_t my_slot = lane < cnt ? mid_list[g * 64 + lane] : 0;
-    for (int32_t q = wave; q < cnt; q += kBlock / 64) {
-        const int64_t slot = __builtin_amdgcn_readlane(my_slot, q);
-        const PieceDesc pd = pieces[slot];
-        const int nv = pd.nx + pd.ny, n = nv + 1;
-        const uint8_t mode = piece_mode(pd, slot, mono, chain_t);
-        for (int i = lane; i < nv; i += 64) xin[i] = chain_t[pd.base + 1 + i];
+    const int32_t cnt = lo_cnt[g];
+    const int32_t mine = (cnt - wave + kWaves - 1) / kWaves;  // pieces wave, wave + 4, ...
+    if (mine <= 0) return;
+    PieceMeta mm;
+    if (lane < mine) mm = gather_meta(pieces, lo_list[g * 64 + wave + kWaves * lane], mono, chain_t, A);
+    for (int k = 0; k < mine; ++k) {
+        const PieceMeta m = meta_of_lane(mm, k);
+        const int nv = m.pd.nx + m.pd.ny, n = nv + 1;
+        for (int i = lane; i < nv; i += 64) xin[i] = chain_t[m.pd.base + 1 + i];
         wave_sync();
-        wave_merge(pd, mode, xin, vout, lane);
+        wave_merge(m.pd, (uint8_t)m.mode, xin, vout, lane);
         wave_sync();
-        const float tend = piece_end_t(pieces, slot, pd, chain_t);
         for (int o = lane; o < n; o += 64) fin[o] = or_merged(vout, n, o);
         wave_sync();
-        piece_fragments(pd, tend, fin, n, A, lane);
+        piece_fragments(m, fin, n, A, lane);
         wave_sync();
-    }
     }
 }
 
-// One wave per long piece (more than kWaveStage entries).  Pieces up to kStage entries are
-// staged in LDS and finished there like in k_piece_fragments; longer ones are merged into
-// scratch[base..] in global memory and OR-merged into scratch[n_entries + base..].
-__global__ __launch_bounds__(64) void k_piece_fragments_long(const PieceDesc* __restrict__ pieces,
-                                                             const int32_t* __restrict__ long_list,
-                                                             const int32_t* __restrict__ long_count,
-                                                             const uint8_t* __restrict__ mono,
-                                                             const float* __restrict__ chain_t,
-                                                             uint32_t* __restrict__ scratch, int64_t n_entries,
-                                                             const FragArgs A) {
+// One workgroup per piece of the heavy queue (pieces that wait on heavy chains, and pieces of
+// more than kWaveStage entries), its four waves together.  Every wave gathers the meta of the
+// workgroup's pieces (lane k: piece blockIdx.x + k * gridDim.x of the queue).  Pieces up to
+// kStage entries are staged in LDS; longer ones are merged into scratch[base..] in global memory
+// and OR-merged into scratch[n_entries + base..].
+__global__ __launch_bounds__(kBlock) void k_piece_heavy(const PieceDesc* __restrict__ pieces,
+                                                        const int32_t* __restrict__ queue,
+                                                        const int32_t* __restrict__ queue_count,
+                                                        const uint8_t* __restrict__ mono,
+                                                        const float* __restrict__ chain_t,
+                                                        uint32_t* __restrict__ scratch, int64_t n_entries,
+                                                        const FragArgs A) {
     __shared__ float st_in[kStage];
     __shared__ uint32_t st_out[kStage + 1];
-    const int lane = threadIdx.x;
-    const int32_t cnt = *long_count;
-    for (int32_t q = blockIdx.x; q < cnt; q += gridDim.x) {
-        const int32_t pi = long_list[q];
-        const PieceDesc pd = pieces[pi];
-        const int nv = pd.nx + pd.ny, n = nv + 1;
-        // the mode is taken here, after every chain is solved (the queueing kernel may run
+    constexpr int kWaves = kBlock / 64;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int32_t cnt = *queue_count;
+    const int32_t G = (int32_t)gridDim.x;
+    for (int32_t q0 = (int32_t)blockIdx.x; q0 < cnt; q0 += 64 * G) {
+        const int32_t mine = min(64, (cnt - q0 + G - 1) / G);
+        PieceMeta mm;
+        // the meta is gathered here, after every chain is solved (the queueing kernel may run
         // before the heavy chains are)
-        const uint8_t mode = piece_mode(pd, pi, mono, chain_t);
-        const float tend = piece_end_t(pieces, pi, pd, chain_t);
-        // the two branches keep every pointer in one address space (no flat accesses)
-        if (n <= kStage) {
-            for (int i = lane; i < nv; i += 64) st_in[i] = chain_t[pd.base + 1 + i];
+        if (lane < mine) mm = gather_meta(pieces, queue[q0 + lane * G], mono, chain_t, A);
+        for (int k = 0; k < mine; ++k) {
+            const PieceMeta m = meta_of_lane(mm, k);
+            const int nv = m.pd.nx + m.pd.ny, n = nv + 1;
+            // the two branches keep every pointer in one address space (no flat accesses)
+            if (n <= kStage) {
+                for (int i = tid; i < nv; i += kBlock) st_in[i] = chain_t[m.pd.base + 1 + i];
+                __syncthreads();
+                wave_merge(m.pd, (uint8_t)m.mode, st_in, st_out, tid, kBlock);
+                __syncthreads();
+                uint32_t* fin = reinterpret_cast<uint32_t*>(st_in);
+                for (int o = tid; o < n; o += kBlock) fin[o] = or_merged(st_out, n, o);
+                __syncthreads();
+                piece_fragments(m, fin, n, A, lane, wave, kWaves);
+            } else {
+                uint32_t* vout = scratch + m.pd.base;
+                wave_merge(m.pd, (uint8_t)m.mode, chain_t + m.pd.base + 1, vout, tid, kBlock);
+                __threadfence_block();
+                __syncthreads();
+                uint32_t* fin = scratch + n_entries + m.pd.base;
+                for (int o = tid; o < n; o += kBlock) fin[o] = or_merged(vout, n, o);
+                __threadfence_block();
+                __syncthreads();
+                piece_fragments(m, fin, n, A, lane, wave, kWaves);
+            }
             __syncthreads();
-            wave_merge(pd, mode, st_in, st_out, lane);
-            __syncthreads();
-            uint32_t* fin = reinterpret_cast<uint32_t*>(st_in);
-            for (int o = lane; o < n; o += 64) fin[o] = or_merged(st_out, n, o);
-            __syncthreads();
-            piece_fragments(pd, tend, fin, n, A, lane);
-        } else {
-            uint32_t* vout = scratch + pd.base;
-            wave_merge(pd, mode, chain_t + pd.base + 1, vout, lane);
-            __threadfence_block();
-            __syncthreads();
-            uint32_t* fin = scratch + n_entries + pd.base;
-            for (int o = lane; o < n; o += 64) fin[o] = or_merged(vout, n, o);
-            __threadfence_block();
-            __syncthreads();
-            piece_fragments(pd, tend, fin, n, A, lane);
         }
-        __syncthreads();
     }
 }
 
@@ -808,19 +853,19 @@ hipError_t launch_piece_small(const PieceDesc* pieces, uint32_t C, const uint8_t
     return hipGetLastError();
 }
 
-hipError_t launch_piece_mid(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                            const int32_t* list, const int32_t* cnt, const FragArgs& A, hipStream_t s) {
+hipError_t launch_piece_lo(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                           const int32_t* list, const int32_t* cnt, const FragArgs& A, hipStream_t s) {
     if (C == 0) return hipSuccess;
     const int64_t groups = ((int64_t)C * 5 + 63) / 64;
     k_piece_fragments<<<(unsigned)groups, kBlock, 0, s>>>(pieces, list, cnt, groups, mono, chain_t, A);
     return hipGetLastError();
 }
 
-hipError_t launch_piece_long(const PieceDesc* pieces, const PieceLists& L, const uint8_t* mono, const float* chain_t,
-                             uint32_t* scratch,
-                             int64_t n_entries, const FragArgs& A, int n_cu, hipStream_t s) {
-    k_piece_fragments_long<<<(unsigned)(n_cu * 8), 64, 0, s>>>(pieces, L.long_list, L.long_count, mono, chain_t,
-                                                               scratch, n_entries, A);
+hipError_t launch_piece_heavy(const PieceDesc* pieces, const PieceLists& L, const uint8_t* mono,
+                              const float* chain_t, uint32_t* scratch, int64_t n_entries, const FragArgs& A, int n_cu,
+                              hipStream_t s) {
+    k_piece_heavy<<<(unsigned)(n_cu * 4), kBlock, 0, s>>>(pieces, L.heavy, L.heavy_count, mono, chain_t, scratch,
+                                                          n_entries, A);
     return hipGetLastError();
 }
 

@@ -66,10 +66,10 @@ struct FragArgs {
 // Chain length classes: cubic chains by quarter octave of their length (16 octaves x 4), then
 // line chains by octave.  The chain control block holds, as int32: class counts, class fill
 // counters, class offsets [kChainClasses + 1], the number of wave-solved chains, the work counter
-// of the persistent solve and the count of long pieces.
+// of the persistent solve and the count of queued heavy pieces.
 constexpr int kChainClasses = 80;
 constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 * kChainClasses + 1,
-              kCcWork = kCcNLong + 1, kCcLongPieces = kCcWork + 1, kCcLightBegin = kCcLongPieces + 1,
+              kCcWork = kCcNLong + 1, kCcHeavyPieces = kCcWork + 1, kCcLightBegin = kCcHeavyPieces + 1,
               kCcWorkLight = kCcLightBegin + 1, kCcWords = kCcWorkLight + 1;
 // Chains of at least kHeavyCrossings crossings are "heavy": they are solved by their own launch
 // (wave-solved and the longest lane-solved chains), and the pieces of at most kHeavyCrossings
@@ -93,25 +93,24 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const Ch
 // reference's per-piece walk), and turns the entries into fragment keys, winding changes and
 // bucket counts (gen_fragment) without storing the entries unless A.inter is set.
 //  launch_piece_small: one lane per piece of at most 32 entries whose work depends on light
-//    chains only; the other pieces are listed per group of 64 piece slots, in lo (up to 1024
-//    entries, light chains only) and hi (the rest up to 1024) lists, or queued as long pieces.
-//  launch_piece_mid: one wave per piece of a set of per-group lists.
-//  launch_piece_long: the long queue; scratch holds 2 * n_entries words.
+//    chains only; the other pieces are listed per group of 64 piece slots in lo lists (up to
+//    1024 entries, light chains only) or queued as heavy pieces (the rest).
+//  launch_piece_lo: one wave per piece of the lo lists.
+//  launch_piece_heavy: one workgroup per piece of the heavy queue, after every chain is solved;
+//    scratch holds 2 * n_entries words.
 struct PieceLists {
     int32_t* lo;       // [groups * 64] slots
     int32_t* lo_cnt;   // [groups]
-    int32_t* hi;
-    int32_t* hi_cnt;
-    int32_t* long_list;
-    int32_t* long_count;
+    int32_t* heavy;    // [slots]
+    int32_t* heavy_count;
 };
 hipError_t launch_piece_small(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
                               const PieceLists& L, const FragArgs& A, int n_cu, hipStream_t s);
-hipError_t launch_piece_mid(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
-                            const int32_t* list, const int32_t* cnt, const FragArgs& A, hipStream_t s);
-hipError_t launch_piece_long(const PieceDesc* pieces, const PieceLists& L, const uint8_t* mono, const float* chain_t,
-                             uint32_t* scratch,
-                             int64_t n_entries, const FragArgs& A, int n_cu, hipStream_t s);
+hipError_t launch_piece_lo(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
+                           const int32_t* list, const int32_t* cnt, const FragArgs& A, hipStream_t s);
+hipError_t launch_piece_heavy(const PieceDesc* pieces, const PieceLists& L, const uint8_t* mono,
+                              const float* chain_t, uint32_t* scratch, int64_t n_entries, const FragArgs& A, int n_cu,
+                              hipStream_t s);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
                                const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,

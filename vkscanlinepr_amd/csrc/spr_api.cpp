@@ -106,7 +106,7 @@ struct spr_handle {
     DBuf<float> chain_t;
     DBuf<spr::ChainDesc> chains;
     DBuf<spr::PieceDesc> pieces;
-    DBuf<int32_t> chain_order, long_list, mid_list, mid_cnt, hi_list, hi_cnt;
+    DBuf<int32_t> chain_order, heavy_list, lo_list, lo_cnt;
     DBuf<uint8_t> mono, chain_cls;
     int n_cu = 256;
     DBuf<int32_t> key;
@@ -221,8 +221,7 @@ void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragA
     const int32_t* d_coff = d_cc + spr::kCcOff;
     const int32_t* d_nlong = d_cc + spr::kCcNLong;
     const int32_t* d_lb = d_cc + spr::kCcLightBegin;
-    const spr::PieceLists L{H->mid_list.p, H->mid_cnt.p, H->hi_list.p, H->hi_cnt.p,
-                            H->long_list.p, d_cc + spr::kCcLongPieces};
+    const spr::PieceLists L{H->lo_list.p, H->lo_cnt.p, H->heavy_list.p, d_cc + spr::kCcHeavyPieces};
     SPR_HIP(hipEventRecord(H->ev_fork, s));
     SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 1,
                                     d_cc + spr::kCcWork, H->chain_t.p, H->mono.p, n_cu_solve, s));
@@ -232,15 +231,14 @@ void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragA
     SPR_HIP(hipEventRecord(H->ev_light, s2));
     SPR_HIP(spr::launch_piece_small(H->pieces.p, C, H->mono.p, H->chain_t.p, L, A, n_cu_merge, s2));
     SPR_HIP(hipEventRecord(H->ev_small, s2));
-    SPR_HIP(spr::launch_piece_mid(H->pieces.p, C, H->mono.p, H->chain_t.p, L.lo, L.lo_cnt, A, s2));
+    SPR_HIP(spr::launch_piece_lo(H->pieces.p, C, H->mono.p, H->chain_t.p, L.lo, L.lo_cnt, A, s2));
     SPR_HIP(hipEventRecord(H->ev_join, s2));
     SPR_HIP(hipStreamWaitEvent(s, H->ev_light, 0));
     stage_mark(H, 5);
     SPR_HIP(hipStreamWaitEvent(s, H->ev_small, 0));
-    SPR_HIP(spr::launch_piece_mid(H->pieces.p, C, H->mono.p, H->chain_t.p, L.hi, L.hi_cnt, A, s));
-    // very long pieces merge through the sort buffer, which is free until the scatter
-    SPR_HIP(spr::launch_piece_long(H->pieces.p, L, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p, n_entries, A,
-                                   n_cu_merge, s));
+    // pieces longer than the LDS stage merge through the sort buffer, which is free until the scatter
+    SPR_HIP(spr::launch_piece_heavy(H->pieces.p, L, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p, n_entries, A,
+                                    n_cu_merge, s));
     SPR_HIP(hipStreamWaitEvent(s, H->ev_join, 0));
 }
 
@@ -343,18 +341,15 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     int32_t* d_coff = d_cc + spr::kCcOff;
     int32_t* d_nlong = d_cc + spr::kCcNLong;
     int32_t* d_work = d_cc + spr::kCcWork;
-    int32_t* d_nlpieces = d_cc + spr::kCcLongPieces;
     if (H->inj) {
         SPR_HIP(hipMemcpyAsync(H->inter.p, H->inj_inter.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
     } else if (gen) {
         H->chains.ensure((size_t)C * 10);
         H->pieces.ensure((size_t)C * 5);
         H->chain_order.ensure((size_t)C * 10);
-        H->long_list.ensure((size_t)C * 5);
-        H->mid_list.ensure((size_t)C * 5 + 64);
-        H->mid_cnt.ensure(((size_t)C * 5 + 63) / 64);
-        H->hi_list.ensure((size_t)C * 5 + 64);
-        H->hi_cnt.ensure(((size_t)C * 5 + 63) / 64);
+        H->heavy_list.ensure((size_t)C * 5);
+        H->lo_list.ensure((size_t)C * 5 + 64);
+        H->lo_cnt.ensure(((size_t)C * 5 + 63) / 64);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
@@ -582,17 +577,14 @@ void run_learned_frame(spr_handle* H) {
     int32_t* d_coff = d_cc + spr::kCcOff;
     int32_t* d_nlong = d_cc + spr::kCcNLong;
     int32_t* d_work = d_cc + spr::kCcWork;
-    int32_t* d_nlpieces = d_cc + spr::kCcLongPieces;
     const int32_t* d_live = d_ctl + 2;
     if (C) {
         H->chains.ensure((size_t)C * 10);
         H->pieces.ensure((size_t)C * 5);
         H->chain_order.ensure((size_t)C * 10);
-        H->long_list.ensure((size_t)C * 5);
-        H->mid_list.ensure((size_t)C * 5 + 64);
-        H->mid_cnt.ensure(((size_t)C * 5 + 63) / 64);
-        H->hi_list.ensure((size_t)C * 5 + 64);
-        H->hi_cnt.ensure(((size_t)C * 5 + 63) / 64);
+        H->heavy_list.ensure((size_t)C * 5);
+        H->lo_list.ensure((size_t)C * 5 + 64);
+        H->lo_cnt.ensure(((size_t)C * 5 + 63) / 64);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
@@ -885,7 +877,7 @@ int spr_destroy(spr_handle* H) {
     H->rule.release(); H->fill.release(); H->tpos.release(); H->visible.release(); H->cuts.release();
     H->ncuts.release(); H->counts.release(); H->seg.release(); H->inter.release();
     H->chain_t.release(); H->chains.release(); H->pieces.release(); H->chain_order.release();
-    H->long_list.release(); H->mid_list.release(); H->mid_cnt.release(); H->hi_list.release(); H->hi_cnt.release(); H->chain_cls.release(); H->mono.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
+    H->heavy_list.release(); H->lo_list.release(); H->lo_cnt.release(); H->chain_cls.release(); H->mono.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();

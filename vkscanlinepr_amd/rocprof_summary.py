@@ -23,7 +23,7 @@ STAGE_KERNELS = {
     "count": ["k_count"],
     "chain_prep": ["k_chain_prep", "k_chain_offsets", "k_chain_order"],
     "chain_solve": ["k_chain_solve"],
-    "piece_fragments": ["k_piece_small", "k_piece_fragments", "k_piece_fragments_long"],
+    "piece_fragments": ["k_piece_small", "k_piece_fragments", "k_piece_heavy"],
     "gen_fragment": ["k_gen_fragment"],
     "bucket_scatter": ["k_bucket_scatter", "k_band_flags", "k_band_emit"],
     "band_sort": ["k_sort_bands", "k_sort_big_bands"],
