@@ -247,18 +247,29 @@ __device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, 
 // Lane l (1..63) is node l of the round's decision tree: its depth is floor(log2 l) and the bits
 // of l below the leading one are its path (1 = right).  Each lane halves the round's interval
 // along its own path, exactly as the sequential bisection would, and evaluates the midpoint of
-// its node; the ballot of "same sign as at t_min" fixes the path taken through the tree, which
-// is walked in scalar registers, and the chosen depth-5 node's interval, midpoint and value are
-// read from its lane.
+// its node; the ballot D of "same sign as at t_min" fixes the path taken through the tree.  A
+// depth-5 node (lanes 32..63) lies on that path when D holds, at its five ancestors, the turns
+// that lead to it: one masked compare per lane and a second ballot name the node, whose
+// interval, midpoint and value are then read from its lane.
 __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
                                         uint8_t* __restrict__ mono) {
     const int lane = threadIdx.x & 63;
     const int depth = lane ? 31 - __clz(lane) : 0;
     bool on[kLevels - 1], go_right[kLevels - 1];
+    unsigned long long anc_mask = 0ull, anc_turn = 1ull;  // inner nodes never match
 #pragma unroll
     for (int lev = 0; lev < kLevels - 1; ++lev) {
         on[lev] = lev < depth;
         go_right[lev] = on[lev] && ((lane >> (depth - 1 - lev)) & 1);
+    }
+    if (depth == kLevels - 1) {
+        anc_turn = 0ull;
+#pragma unroll
+        for (int lev = 0; lev < kLevels - 1; ++lev) {
+            const int anc = lane >> (kLevels - 1 - lev);  // ancestor at depth lev
+            anc_mask |= 1ull << anc;
+            anc_turn |= (unsigned long long)((lane >> (kLevels - 2 - lev)) & 1) << anc;
+        }
     }
     const uint32_t side = (uint32_t)d.side;
     float tprev = d.t0, line = d.line0;
@@ -272,7 +283,7 @@ __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, 
             const uint32_t s0 = fbits(vt0 - c) & 0x80000000u;
             float lo = t_min, hi = d.t1;
             float tm = t_min, vtm = vt0;
-#pragma unroll 1
+#pragma unroll
             for (int round = 0; round < kRounds; ++round) {
                 float llo = lo, lhi = hi;
 #pragma unroll
@@ -283,17 +294,19 @@ __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, 
                 }
                 const float mym = (llo + lhi) * 0.5f;
                 const float myv = eval_axis(true, mym, a.c[0], a.c[1], a.c[2], a.c[3]);
-                const bool right = lane != 0 && (fbits(myv - c) & 0x80000000u) == s0;
+                // lane 0 is no node: its bit is never read
+                const bool right = (fbits(myv - c) & 0x80000000u) == s0;
+                // the interval this node leaves for the next round
+                const float nlo = right ? mym : llo, nhi = right ? lhi : mym;
                 const unsigned long long D = __ballot(right);
-                uint32_t node = 1;
-#pragma unroll
-                for (int lev = 0; lev < kLevels - 1; ++lev) node = 2u * node + (uint32_t)((D >> node) & 1ull);
-                node = __builtin_amdgcn_readfirstlane(node);
-                const float nlo = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(llo), (int)node));
-                const float nhi = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(lhi), (int)node));
-                tm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(mym), (int)node));
-                vtm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(myv), (int)node));
-                if ((D >> node) & 1ull) { lo = tm; hi = nhi; } else { lo = nlo; hi = tm; }
+                const unsigned long long P = __ballot((D & anc_mask) == anc_turn);
+                const int node = __builtin_amdgcn_readfirstlane(__builtin_ctzll(P));
+                lo = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(nlo), node));
+                hi = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(nhi), node));
+                if (round == kRounds - 1) {
+                    tm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(mym), node));
+                    vtm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(myv), node));
+                }
             }
             ts = fabsf(vtm - c) > 1.0f ? t_min : tm;
         }
@@ -682,9 +695,12 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         int n_max = mine ? n : 0;
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) n_max = max(n_max, __shfl_xor(n_max, o));
+        // bucket counts: each lane counts its own run of equal buckets (consecutive entries of a
+        // piece mostly share their cell row) and adds it when the bucket changes
+        uint32_t run_b = 0;
+        int32_t run_n = 0;
         for (int o = 0; o < n_max; ++o) {
             const bool valid = mine && o < n;
-            uint32_t bucket = 0;
             if (valid) {
                 // the next entry's final value (needs the entry after it)
                 uint32_t fin1 = 0;
@@ -706,43 +722,66 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
                 A.wn[i] = (int8_t)swn;
                 A.epath[i] = path;
                 if (A.inter) A.inter[i] = make_int2(pd.curve, (int32_t)fin0);
-                bucket = bb + bucket_local((int32_t)yx, pr, A.err);
+                const uint32_t bucket = bb + bucket_local((int32_t)yx, pr, A.err);
+                if (run_n > 0 && bucket != run_b) {
+                    atomicAdd(&A.hist[run_b], run_n);
+                    run_n = 0;
+                }
+                run_b = bucket;
+                ++run_n;
                 fin0 = fin1;
                 t0 = t1; px = qx; py = qy;
             }
-            wave_count(A.hist, bucket, valid);
         }
+        if (run_n > 0) atomicAdd(&A.hist[run_b], run_n);
         __builtin_amdgcn_wave_barrier();
     }
 }
 
-// One workgroup per lo list (the lo pieces of 64 consecutive slots): its four waves take the
-// list's pieces in turn, one wave per piece.  Lane k of wave w gathers the meta of the list's
-// piece w + 4k up front, so the wave's pieces pay that chain of loads once.  The piece's chains
-// are staged in the wave's LDS, merged, OR-merged, and turned into fragments.
-__global__ __launch_bounds__(kBlock) void k_piece_fragments(const PieceDesc* __restrict__ pieces,
-                                                            const int32_t* __restrict__ lo_list,
-                                                            const int32_t* __restrict__ lo_cnt, int64_t n_groups,
-                                                            const uint8_t* __restrict__ mono,
-                                                            const float* __restrict__ chain_t, const FragArgs A) {
-    __shared__ float s_in[kBlock / 64][kWaveStage];
-    __shared__ uint32_t s_out[kBlock / 64][kWaveStage];
-    constexpr int kWaves = kBlock / 64;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float* xin = s_in[wave];
-    uint32_t* vout = s_out[wave];
+// One wave (a workgroup of its own) per quarter of a lo list (the lo pieces of 64 consecutive
+// slots): wave w of list g takes the list's pieces w, w + 4, ...; lane k gathers the meta of the
+// wave's k-th piece up front, so the wave pays that chain of loads once.  The chains of the next
+// piece are loaded into registers while the current one is merged, OR-merged and turned into
+// fragments in the wave's LDS.
+constexpr int kLoQuarters = 4;
+__global__ __launch_bounds__(64) void k_piece_fragments(const PieceDesc* __restrict__ pieces,
+                                                        const int32_t* __restrict__ lo_list,
+                                                        const int32_t* __restrict__ lo_cnt, int64_t n_groups,
+                                                        const uint8_t* __restrict__ mono,
+                                                        const float* __restrict__ chain_t, const FragArgs A) {
+    __shared__ float xin[kWaveStage];
+    __shared__ uint32_t vout[kWaveStage];
+    constexpr int kPre = kWaveStage / 64;
     uint32_t* fin = reinterpret_cast<uint32_t*>(xin);  // the staged chains are dead after the merge
-    const int64_t g = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x / kLoQuarters;
+    const int q = (int)(blockIdx.x % kLoQuarters);
     if (g >= n_groups) return;
     const int32_t cnt = lo_cnt[g];
-    const int32_t mine = (cnt - wave + kWaves - 1) / kWaves;  // pieces wave, wave + 4, ...
+    const int32_t mine = (cnt - q + kLoQuarters - 1) / kLoQuarters;
     if (mine <= 0) return;
     PieceMeta mm;
-    if (lane < mine) mm = gather_meta(pieces, lo_list[g * 64 + wave + kWaves * lane], mono, chain_t, A);
+    if (lane < mine) mm = gather_meta(pieces, lo_list[g * 64 + q + kLoQuarters * lane], mono, chain_t, A);
+    // the chains of piece k: values base + 1 .. base + nv, lane + 64 j held in pre[j]
+    float pre[kPre];
+    auto fetch = [&](int k) {
+        const int base = rl_i(mm.pd.base, k), nv = rl_i(mm.pd.nx, k) + rl_i(mm.pd.ny, k);
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int i = lane + 64 * j;
+            pre[j] = i < nv ? chain_t[base + 1 + i] : 0.0f;
+        }
+    };
+    fetch(0);
     for (int k = 0; k < mine; ++k) {
         const PieceMeta m = meta_of_lane(mm, k);
         const int nv = m.pd.nx + m.pd.ny, n = nv + 1;
-        for (int i = lane; i < nv; i += 64) xin[i] = chain_t[m.pd.base + 1 + i];
+#pragma unroll
+        for (int j = 0; j < kPre; ++j) {
+            const int i = lane + 64 * j;
+            if (i < nv) xin[i] = pre[j];
+        }
+        if (k + 1 < mine) fetch(k + 1);
         wave_sync();
         wave_merge(m.pd, (uint8_t)m.mode, xin, vout, lane);
         wave_sync();
@@ -857,7 +896,7 @@ hipError_t launch_piece_lo(const PieceDesc* pieces, uint32_t C, const uint8_t* m
                            const int32_t* list, const int32_t* cnt, const FragArgs& A, hipStream_t s) {
     if (C == 0) return hipSuccess;
     const int64_t groups = ((int64_t)C * 5 + 63) / 64;
-    k_piece_fragments<<<(unsigned)groups, kBlock, 0, s>>>(pieces, list, cnt, groups, mono, chain_t, A);
+    k_piece_fragments<<<(unsigned)(groups * kLoQuarters), 64, 0, s>>>(pieces, list, cnt, groups, mono, chain_t, A);
     return hipGetLastError();
 }
 
