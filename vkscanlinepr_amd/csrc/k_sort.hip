@@ -125,7 +125,7 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
     for (int run = kItems; run < P2; run <<= 1) {
         if (tid < active) {
             const int o = tid * kItems;
-            const int pb = (o / (2 * run)) * (2 * run);
+            const int pb = o & ~(2 * run - 1);  // run is a power of two
             const u64* S = buf[cur];
             auto A = [&](int i) { return S[pad(pb + i)]; };
             auto B = [&](int j) { return S[pad(pb + run + j)]; };
@@ -137,12 +137,23 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
             }
             int i = lo, j = diag - lo;
             u64* D = buf[cur ^ 1];
-            u64 a = i < run ? A(i) : kPad, b = j < run ? B(j) : kPad;
+            // a run's end reads as kPad, which no sort word equals (its low two bits never hold
+            // 3), so the merge takes the smaller head without bounds tests; each step refills the
+            // consumed side with one load at a clamped index
+            u64 a = A(i < run ? i : run - 1), b = B(j < run ? j : run - 1);
+            a = i < run ? a : kPad;
+            b = j < run ? b : kPad;
 #pragma unroll
             for (int k = 0; k < kItems; ++k) {
-                const bool takeA = j >= run || (i < run && a <= b);
+                const bool takeA = a < b;
                 D[pad(o + k)] = takeA ? a : b;
-                if (takeA) { ++i; a = i < run ? A(i) : kPad; } else { ++j; b = j < run ? B(j) : kPad; }
+                i += takeA ? 1 : 0;
+                j += takeA ? 0 : 1;
+                const int nx = takeA ? i : j;
+                const u64 v = S[pad(pb + (takeA ? 0 : run) + (nx < run ? nx : run - 1))];
+                const u64 w = nx < run ? v : kPad;
+                a = takeA ? w : a;
+                b = takeA ? b : w;
             }
         }
         __syncthreads();
