@@ -40,12 +40,17 @@ __global__ void k_bucket_counts(const int32_t* __restrict__ row_lo, const int32_
     nb[p] = nrows + 2;
 }
 
-// Scatter into bucket order.  The sort word orders exactly like the reference's signed
-// (key, index) pair and carries the winding change in its two low bits.  Each wave takes
-// kRounds chunks of 64 consecutive entries and issues every load, then every slot atomic (one per
-// run of equal buckets), of all chunks before it waits on any of them.
+// Scatter into bucket order.  The wide sort word orders exactly like the reference's signed
+// (key, index) pair and carries the winding change in its two low bits.  The narrow word (used
+// when no stage is exported) drops the index: (bucket inside its path) << 17 | (x + 2) << 2 |
+// (winding change + 1); inside a band (whole buckets of one path) it orders like the key, and the
+// order among equal keys changes no record, no span and no pixel (no span starts between equal
+// keys, and the winding number after a run of equal keys is the sum over the whole run).  Each
+// wave takes kRounds chunks of 64 consecutive entries and issues every load, then every slot
+// atomic (one per run of equal buckets), of all chunks before it waits on any of them.
 constexpr int kRounds = 4;
 
+template <bool NARROW>
 __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __restrict__ key,
                                                            const int8_t* __restrict__ wn,
                                                            const uint32_t* __restrict__ epath, int64_t N,
@@ -54,6 +59,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
                                                            const int32_t* __restrict__ boff,
                                                            int32_t* __restrict__ fill,
                                                            unsigned long long* __restrict__ out,
+                                                           uint32_t* __restrict__ out32,
                                                            int32_t* __restrict__ err,
                                                            const int32_t* __restrict__ n_dev) {
     const int64_t n = n_dev ? min((int64_t)*n_dev, N) : N;
@@ -70,11 +76,17 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
         if (valid[r]) { pth[r] = epath[i]; k[r] = key[i]; }
         if (pth[r] == kNoPath) valid[r] = false;  // gap of a predicted-allocation layout
     }
-    uint32_t b[kRounds];
+    uint32_t b[kRounds], loc[kRounds];
+    bool sent[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
-        b[r] = 0;
-        if (valid[r]) b[r] = (uint32_t)bucket_base[pth[r]] + bucket_local(k[r], prow[pth[r]], err);
+        b[r] = 0; loc[r] = 0; sent[r] = false;
+        if (valid[r]) {
+            const int2 pr = prow[pth[r]];
+            loc[r] = bucket_local(k[r], pr, err);
+            sent[r] = loc[r] == (uint32_t)pr.y;
+            b[r] = (uint32_t)bucket_base[pth[r]] + loc[r];
+        }
     }
     // runs of equal buckets: heads take their run's slots with one atomic each
     unsigned long long vm[kRounds], hd[kRounds];
@@ -93,9 +105,15 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
         const int32_t slot = __shfl(base[r], h) + (lane - h);
         if (valid[r]) {
             const int64_t i = g0 + r * 64 + lane;
-            const unsigned long long w = ((unsigned long long)((uint32_t)k[r] ^ 0x80000000u) << 32) |
-                                         ((unsigned long long)i << 2) | (unsigned long long)(uint32_t)(wn[i] + 1);
-            out[boff[b[r]] + slot] = w;
+            if constexpr (NARROW) {
+                // the off-canvas bucket holds one key: its x is left at 0
+                const uint32_t xl = sent[r] ? 0u : ((uint32_t)k[r] & 0xFFFFu) - kNarrowX0;
+                out32[boff[b[r]] + slot] = (loc[r] << 17) | (xl << 2) | (uint32_t)(wn[i] + 1);
+            } else {
+                const unsigned long long w = ((unsigned long long)((uint32_t)k[r] ^ 0x80000000u) << 32) |
+                                             ((unsigned long long)i << 2) | (unsigned long long)(uint32_t)(wn[i] + 1);
+                out[boff[b[r]] + slot] = w;
+            }
         }
     }
 }
@@ -144,14 +162,28 @@ __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict
     }
 }
 
+// Band starts, and each band's path rows (of the path whose bucket range holds the band's first
+// bucket).
 __global__ void k_band_emit(const int32_t* __restrict__ flag_scan, const int32_t* __restrict__ boff,
                             int32_t NB_max, int64_t N_max, int32_t* __restrict__ band_start,
-                            const int32_t* __restrict__ nb_dev, const int32_t* __restrict__ n_dev) {
+                            const int32_t* __restrict__ nb_dev, const int32_t* __restrict__ n_dev,
+                            const int32_t* __restrict__ bucket_base, uint32_t Np, const int2* __restrict__ prow,
+                            int2* __restrict__ band_pr) {
     const int32_t NB = nb_dev ? min(*nb_dev, NB_max) : NB_max;
     const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
     const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (b >= NB) return;
-    if (flag_scan[b + 1] != flag_scan[b]) band_start[flag_scan[b]] = boff[b];
+    if (flag_scan[b + 1] != flag_scan[b]) {
+        band_start[flag_scan[b]] = boff[b];
+        if (band_pr) {
+            uint32_t lo = 0, hi = Np;  // bucket_base[lo] <= b < bucket_base[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (bucket_base[mid] <= b) lo = mid; else hi = mid;
+            }
+            band_pr[flag_scan[b]] = prow[lo];
+        }
+    }
     if (b == NB - 1) band_start[flag_scan[NB]] = (int32_t)N;
 }
 
@@ -167,10 +199,15 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
                                  const int32_t* bucket_base, const int2* prow, const int32_t* boff,
                                  int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s,
-                                 const int32_t* n_dev) {
+                                 const int32_t* n_dev, bool narrow) {
     if (N == 0) return hipSuccess;
-    k_bucket_scatter<<<grid_for((N + kRounds - 1) / kRounds), kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow,
-                                                                              boff, fill, out, err, n_dev);
+    const unsigned g = grid_for((N + kRounds - 1) / kRounds);
+    if (narrow)
+        k_bucket_scatter<true><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, boff, fill, nullptr,
+                                                    reinterpret_cast<uint32_t*>(out), err, n_dev);
+    else
+        k_bucket_scatter<false><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, boff, fill, out, nullptr,
+                                                     err, n_dev);
     return hipGetLastError();
 }
 
@@ -182,9 +219,11 @@ hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int3
 }
 
 hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,
-                            int32_t* band_start, hipStream_t s, const int32_t* nb_dev, const int32_t* n_dev) {
+                            int32_t* band_start, hipStream_t s, const int32_t* nb_dev, const int32_t* n_dev,
+                            const int32_t* bucket_base, uint32_t Np, const int2* prow, int2* band_pr) {
     if (NB == 0) return hipSuccess;
-    k_band_emit<<<grid_for(NB), kBlock, 0, s>>>(flag_scan, boff, NB, N, band_start, nb_dev, n_dev);
+    k_band_emit<<<grid_for(NB), kBlock, 0, s>>>(flag_scan, boff, NB, N, band_start, nb_dev, n_dev, bucket_base, Np,
+                                                prow, band_pr);
     return hipGetLastError();
 }
 

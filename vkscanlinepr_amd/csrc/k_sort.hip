@@ -18,6 +18,11 @@
 // row) bucket; such bands publish their sum at once and are queued, and each is then sorted by
 // one workgroup with LDS chunk sorts followed by merge-path passes through global memory, the
 // last pass writing keys and winding numbers.
+//
+// Frames that export no stage sort the narrow 32-bit words of the scatter instead (bucket inside
+// the path, x, winding change: see k_bucket.hip), with the same kernels; the keys are rebuilt
+// from the band's path rows.
+#include "device_common.hpp"
 #include "kernels.hpp"
 #include "scan_common.hpp"
 
@@ -29,15 +34,16 @@ constexpr int kThreads = 256;
 constexpr int kItems = 8;
 static_assert(kThreads * kItems == kSortTile, "tile shape");
 typedef unsigned long long u64;
-constexpr u64 kPad = ~0ull;
 
-__device__ inline void cswap(u64& a, u64& b) {
-    u64 lo = a < b ? a : b, hi = a < b ? b : a;
+template <class T>
+__device__ inline void cswap(T& a, T& b) {
+    T lo = a < b ? a : b, hi = a < b ? b : a;
     a = lo; b = hi;
 }
 
 // Ascending bitonic network over kItems registers (indices are compile-time constants).
-__device__ inline void sort16(u64 (&v)[kItems]) {
+template <class T>
+__device__ inline void sort16(T (&v)[kItems]) {
 #pragma unroll
     for (int k = 2; k <= kItems; k <<= 1) {
 #pragma unroll
@@ -76,11 +82,33 @@ struct Unpacked {
     int32_t* sidx;
     int32_t* winding;
 };
-__device__ inline void put(const Unpacked& o, int64_t i, u64 w) {
-    o.skey[i] = (int32_t)((uint32_t)(w >> 32) ^ 0x80000000u);
-    if (o.sidx) o.sidx[i] = (int32_t)((w >> 2) & 0x3FFFFFFFull);
-}
-__device__ inline int32_t wn_of(u64 w) { return (int32_t)(w & 3ull) - 1; }
+// Wide words: key ^ 0x80000000, entry index, winding change + 1.
+struct Wide {
+    typedef u64 T;
+    static constexpr T kPad = ~0ull;  // no word has 3 in its low two bits
+    __device__ void band(const int2*, int64_t) {}
+    __device__ void put(const Unpacked& o, int64_t i, T w) const {
+        o.skey[i] = (int32_t)((uint32_t)(w >> 32) ^ 0x80000000u);
+        if (o.sidx) o.sidx[i] = (int32_t)((w >> 2) & 0x3FFFFFFFull);
+    }
+};
+// Narrow words: bucket inside the band's path << 17 | (x + 2) << 2 | winding change + 1; the key
+// is rebuilt from the path's rows pr = (first row, number of rows): buckets 0..pr.y-1 are rows
+// pr.x.., bucket pr.y the off-canvas key, bucket pr.y+1 row 0.
+struct Narrow {
+    typedef uint32_t T;
+    static constexpr T kPad = ~0u;
+    int2 pr;
+    __device__ void band(const int2* band_pr, int64_t b) { pr = band_pr[b]; }
+    __device__ void put(const Unpacked& o, int64_t i, T w) const {
+        const int loc = (int)(w >> 17);
+        const uint32_t x = ((w >> 2) & 0x7FFFu) + kNarrowX0;
+        const int y = loc < pr.y ? 2 * (pr.x + loc) : 0;
+        o.skey[i] = loc == pr.y ? (int32_t)kSentinelKey : (int32_t)(((uint32_t)(y + 0x7FFF) << 16) | x);
+    }
+};
+template <class T>
+__device__ inline int32_t wn_of(T w) { return (int32_t)(w & 3u) - 1; }
 __device__ inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 
 // LDS slot of element i: one pad word per 8 elements, so the 8-element runs owned by
@@ -92,14 +120,18 @@ __device__ inline int pad(int i) { return i + (i >> 3); }
 // (sort words) or, when dst is null, unpacked into out at offset o0 with winding numbers from
 // the exclusive prefix `pre` (then `band` must be the band's ticket for the look-back and
 // `status` its status array: the band's sum is published before the sort).
-__device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ dst, int n, u64 (*buf)[kPadded],
-                                 const Unpacked& out, int64_t o0, unsigned long long* status, int64_t band,
-                                 int32_t* s_w, int32_t* s_pre, bool last, int64_t total_at) {
+template <class W>
+__device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __restrict__ src, typename W::T* __restrict__ dst,
+                                 int n, typename W::T (*buf)[kPadded], const Unpacked& out, int64_t o0,
+                                 unsigned long long* status, int64_t band, int32_t* s_w, int32_t* s_pre, bool last,
+                                 int64_t total_at) {
+    typedef typename W::T T;
+    constexpr T kPad = W::kPad;
     int P2 = kItems;
     while (P2 < n) P2 <<= 1;
     const int active = P2 / kItems;
     const int tid = threadIdx.x;
-    u64 v[kItems];
+    T v[kItems];
     int32_t part = 0;
     if (tid < active) {
 #pragma unroll
@@ -126,7 +158,7 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
         if (tid < active) {
             const int o = tid * kItems;
             const int pb = o & ~(2 * run - 1);  // run is a power of two
-            const u64* S = buf[cur];
+            const T* S = buf[cur];
             auto A = [&](int i) { return S[pad(pb + i)]; };
             auto B = [&](int j) { return S[pad(pb + run + j)]; };
             const int diag = o - pb;
@@ -136,11 +168,12 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
                 if (A(mid) <= B(diag - 1 - mid)) lo = mid + 1; else hi = mid;
             }
             int i = lo, j = diag - lo;
-            u64* D = buf[cur ^ 1];
+            T* D = buf[cur ^ 1];
             // a run's end reads as kPad, which no sort word equals (its low two bits never hold
             // 3), so the merge takes the smaller head without bounds tests; each step refills the
-            // consumed side with one load at a clamped index
-            u64 a = A(i < run ? i : run - 1), b = B(j < run ? j : run - 1);
+            // consumed side with one load at a clamped index (equal narrow words are alike, so
+            // the side taken on a tie does not matter)
+            T a = A(i < run ? i : run - 1), b = B(j < run ? j : run - 1);
             a = i < run ? a : kPad;
             b = j < run ? b : kPad;
 #pragma unroll
@@ -150,8 +183,8 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
                 i += takeA ? 1 : 0;
                 j += takeA ? 0 : 1;
                 const int nx = takeA ? i : j;
-                const u64 v = S[pad(pb + (takeA ? 0 : run) + (nx < run ? nx : run - 1))];
-                const u64 w = nx < run ? v : kPad;
+                const T v = S[pad(pb + (takeA ? 0 : run) + (nx < run ? nx : run - 1))];
+                const T w = nx < run ? v : kPad;
                 a = takeA ? w : a;
                 b = takeA ? b : w;
             }
@@ -192,7 +225,7 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
     }
     __syncthreads();
     for (int i = tid; i < n; i += kThreads) {
-        put(out, o0 + i, buf[cur][pad(i)]);
+        wd.put(out, o0 + i, buf[cur][pad(i)]);
         out.winding[o0 + i] = wl[i];
     }
     __syncthreads();
@@ -201,15 +234,17 @@ __device__ inline void tile_sort(const u64* __restrict__ src, u64* __restrict__ 
 // Persistent workgroups take the bands in array order from an atomic ticket (so every band they
 // wait on in the look-back is held by a running workgroup).  A band above one tile publishes its
 // sum (and the scan total when it is the last band) and is queued with its exclusive prefix.
+template <class W>
 __global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restrict__ band_start,
                                                          const int32_t* __restrict__ n_bands,
-                                                         const u64* __restrict__ src, Unpacked out,
+                                                         const typename W::T* __restrict__ src, Unpacked out,
                                                          int32_t* __restrict__ big_list,
                                                          int32_t* __restrict__ big_excl,
                                                          int32_t* __restrict__ big_count,
                                                          unsigned long long* __restrict__ status,
-                                                         uint32_t* __restrict__ counter) {
-    __shared__ u64 buf[2][kPadded];
+                                                         uint32_t* __restrict__ counter,
+                                                         const int2* __restrict__ band_pr) {
+    __shared__ typename W::T buf[2][kPadded];
     __shared__ int32_t s_tile, s_w[kThreads / 64], s_pre;
     const int32_t nb = *n_bands;
     while (true) {
@@ -219,6 +254,8 @@ __global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restri
             return;
         }
         const int64_t s = band_start[t], e = band_start[t + 1];
+        W wd;
+        wd.band(band_pr, t);  // loaded with the band's bounds
         const int64_t n = e - s;
         const bool last = t == nb - 1;
         if (n > kSortTile) {
@@ -237,46 +274,51 @@ __global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restri
             }
             continue;
         }
-        tile_sort(src + s, nullptr, (int)n, buf, out, s, status, t, s_w, &s_pre, last, e);
+        tile_sort(wd, src + s, (typename W::T*)nullptr, (int)n, buf, out, s, status, t, s_w, &s_pre, last, e);
     }
 }
 
 // One workgroup per oversized band: LDS chunk sorts into `a`, then merge-path passes between
 // `a` and `b` (the scatter buffer, already consumed), the last pass writing keys and winding
 // numbers (the band's exclusive prefix carried across its 2048-entry output blocks).
+template <class W>
 __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __restrict__ band_start,
                                                              const int32_t* __restrict__ big_list,
                                                              const int32_t* __restrict__ big_excl,
                                                              const int32_t* __restrict__ big_count,
-                                                             u64* __restrict__ scatter, u64* __restrict__ a,
-                                                             Unpacked out) {
-    __shared__ u64 buf[2][kPadded];
+                                                             typename W::T* __restrict__ scatter,
+                                                             typename W::T* __restrict__ a, Unpacked out,
+                                                             const int2* __restrict__ band_pr) {
+    typedef typename W::T T;
+    __shared__ T buf[2][kPadded];
     __shared__ int32_t s_w[kThreads / 64];
     const int32_t nbig = *big_count;
     for (int32_t q = blockIdx.x; q < nbig; q += gridDim.x) {
         const int32_t band = big_list[q];
         const int64_t s = band_start[band], n = band_start[band + 1] - s;
+        W wd;
+        wd.band(band_pr, band);
         for (int64_t c = 0; c < n; c += kSortTile) {
             const int m = (int)(n - c < kSortTile ? n - c : kSortTile);
-            tile_sort(scatter + s + c, a + s + c, m, buf, out, 0, nullptr, 0, s_w, nullptr, false, 0);
+            tile_sort(wd, scatter + s + c, a + s + c, m, buf, out, 0, nullptr, 0, s_w, nullptr, false, 0);
         }
         __threadfence_block();
         __syncthreads();
-        u64* cur = a + s;
-        u64* nxt = scatter + s;
+        T* cur = a + s;
+        T* nxt = scatter + s;
         int32_t carry = big_excl[q];
         for (int64_t r = kSortTile; r < n; r <<= 1) {
             const bool last_pass = 2 * r >= n;
             for (int64_t base = 0; base < n; base += (int64_t)kThreads * kItems) {
                 const int64_t o = base + (int64_t)threadIdx.x * kItems;
-                u64 vals[kItems];
+                T vals[kItems];
                 int cnt = 0;
                 if (o < n) {
                     const int64_t pb = (o / (2 * r)) * (2 * r);
                     const int64_t na = pb + r < n ? r : n - pb;
                     const int64_t nb = pb + r < n ? ((pb + 2 * r < n ? pb + 2 * r : n) - (pb + r)) : 0;
-                    const u64* A = cur + pb;
-                    const u64* B = A + na;
+                    const T* A = cur + pb;
+                    const T* B = A + na;
                     cnt = (int)(n - o < kItems ? n - o : kItems);
                     const int64_t diag = o - pb;
                     int64_t i = nb ? merge_split(A, na, B, nb, diag) : diag;
@@ -298,7 +340,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
                 int32_t tot;
                 int32_t run = wadd(carry, block_exclusive_add(local, s_w, tot));
                 for (int k = 0; k < cnt; ++k) {
-                    put(out, s + o + k, vals[k]);
+                    wd.put(out, s + o + k, vals[k]);
                     out.winding[s + o + k] = run;
                     run = wadd(run, wn_of(vals[k]));
                 }
@@ -307,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
             __threadfence_block();
             __syncthreads();
             if (last_pass) break;
-            u64* t = cur; cur = nxt; nxt = t;
+            T* t = cur; cur = nxt; nxt = t;
         }
     }
 }
@@ -317,20 +359,42 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, ScanCtl ctl, int n_cu,
-                             hipStream_t s) {
+                             hipStream_t s, const int2* band_pr, int64_t n_entries) {
     if (max_bands <= 0) return hipSuccess;
+    const Unpacked out{skey, sidx, winding};
+    if (band_pr) {
+        if (sidx) return hipErrorInvalidValue;  // narrow words hold no index
+        // eight workgroups fit in one CU's LDS; with short bands (a few hundred entries on
+        // average) four per CU finish sooner (measured on 98-entry paths vs 9.7k-entry paths)
+        const int per_cu = n_entries >= 256 * max_bands ? 8 : 4;
+        const int64_t grid = max_bands < (int64_t)n_cu * per_cu ? max_bands : (int64_t)n_cu * per_cu;
+        k_sort_bands<Narrow><<<(unsigned)grid, kThreads, 0, s>>>(band_start, n_bands,
+                                                                  reinterpret_cast<const uint32_t*>(src), out,
+                                                                  big_list, big_excl, big_count, ctl.status,
+                                                                  ctl.counter, band_pr);
+        return hipGetLastError();
+    }
     // four workgroups fit in one CU's LDS
     const int64_t grid = max_bands < (int64_t)n_cu * 4 ? max_bands : (int64_t)n_cu * 4;
-    k_sort_bands<<<(unsigned)grid, kThreads, 0, s>>>(band_start, n_bands, src, Unpacked{skey, sidx, winding},
-                                                      big_list, big_excl, big_count, ctl.status, ctl.counter);
+    k_sort_bands<Wide><<<(unsigned)grid, kThreads, 0, s>>>(band_start, n_bands, src, out, big_list, big_excl,
+                                                            big_count, ctl.status, ctl.counter, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_excl,
                                  const int32_t* big_count, unsigned long long* scatter, unsigned long long* tmp,
-                                 int32_t* skey, int32_t* sidx, int32_t* winding, hipStream_t s) {
-    k_sort_big_bands<<<64, kThreads, 0, s>>>(band_start, big_list, big_excl, big_count, scatter, tmp,
-                                             Unpacked{skey, sidx, winding});
+                                 int32_t* skey, int32_t* sidx, int32_t* winding, hipStream_t s,
+                                 const int2* band_pr) {
+    const Unpacked out{skey, sidx, winding};
+    if (band_pr) {
+        if (sidx) return hipErrorInvalidValue;
+        k_sort_big_bands<Narrow><<<64, kThreads, 0, s>>>(band_start, big_list, big_excl, big_count,
+                                                          reinterpret_cast<uint32_t*>(scatter),
+                                                          reinterpret_cast<uint32_t*>(tmp), out, band_pr);
+        return hipGetLastError();
+    }
+    k_sort_big_bands<Wide><<<64, kThreads, 0, s>>>(band_start, big_list, big_excl, big_count, scatter, tmp, out,
+                                                    nullptr);
     return hipGetLastError();
 }
 

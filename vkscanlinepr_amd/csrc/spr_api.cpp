@@ -117,6 +117,7 @@ struct spr_handle {
     DBuf<int4> records;     // full records, kept only with the debug option
     DBuf<uint32_t> crec;    // compact records (x | width << 16) read by the composite
     DBuf<int32_t> row_lo, row_hi, bucket_base, bflag, band_start, big_list;
+    DBuf<int2> band_pr;
     DBuf<int2> prow;
     DBuf<uint8_t> image;
     DBuf<unsigned long long> zero;  // per-frame zeroed control region
@@ -248,6 +249,8 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     const uint32_t P = H->P, C = H->C, Np = H->Np;
     const int w = (int)H->w, h = (int)H->h;
     const bool debug = H->opts.debug != 0;
+    // 32-bit sort words unless the sorted indices are exported
+    const bool narrow = !debug && w <= spr::kNarrowMaxW;
 
     if (record_begin) SPR_HIP(hipEventRecord(H->ev_begin, s));
     H->n_marks = 0;
@@ -323,6 +326,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->winding.ensure(n1 + 1);
     H->bflag.ensure(nb1 + 1);
     H->band_start.ensure((size_t)max_bands + 1);
+    H->band_pr.ensure((size_t)max_bands + 1);
     H->big_list.ensure(2 * (size_t)max_bands);  // band ids, then their winding prefixes
     if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
     H->crec.ensure(2 * n1);
@@ -377,10 +381,11 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     if (N > 0) {
         SPR_HIP(spr::launch_scan_i32(d_hist, d_hist, NB, ctl_at(H, z.st_hist, z.ctr, 0), s));
         SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, N, H->bucket_base.p, H->prow.p, d_hist,
-                                           d_fill, H->sbig.p, d_err, s));
+                                           d_fill, H->sbig.p, d_err, s, nullptr, narrow));
         SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, d_hist, H->bflag.p, s));
         SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, NB, ctl_at(H, z.st_band, z.ctr, 1), s));
-        SPR_HIP(spr::launch_band_emit(H->bflag.p, d_hist, (int32_t)NB, N, H->band_start.p, s));
+        SPR_HIP(spr::launch_band_emit(H->bflag.p, d_hist, (int32_t)NB, N, H->band_start.p, s, nullptr, nullptr,
+                                      H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr));
     }
     stage_mark(H, 8);
     if (N > 0) {
@@ -389,10 +394,10 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
                                        debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
                                        H->big_list.p + max_bands, d_bigcount, ctl_at(H, z.st_wind, z.ctr, 2), H->n_cu,
-                                       s));
+                                       s, narrow ? H->band_pr.p : nullptr, N));
         SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                            H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr,
-                                           H->winding.p, s));
+                                           H->winding.p, s, narrow ? H->band_pr.p : nullptr));
     }
     stage_mark(H, 9);
     stage_mark(H, 10);
@@ -468,6 +473,8 @@ void run_learned_frame(spr_handle* H) {
     const int64_t nb_budget = (int64_t)Np * (rmax + 2);
     if (nb_budget > (int64_t)1 << 30) throw Fail{SPR_ERR_CAPACITY, "bucket bound exceeds 2^30"};
     const int2* pcv = H->batch ? H->pcanvas.p : nullptr;
+    // 32-bit sort words unless the sorted indices are exported (canvases are at most 8192 wide)
+    const bool narrow = !debug && w <= spr::kNarrowMaxW;
 
     if (!H->capturing) SPR_HIP(hipEventRecord(H->ev_begin, s));
     H->n_marks = 0;
@@ -552,6 +559,7 @@ void run_learned_frame(spr_handle* H) {
     H->winding.ensure(nl + 1);
     H->bflag.ensure((size_t)nb_budget + 1);
     H->band_start.ensure((size_t)max_bands + 1);
+    H->band_pr.ensure((size_t)max_bands + 1);
     H->big_list.ensure(2 * (size_t)max_bands);  // band ids, then their winding prefixes
     if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); H->records.ensure(nr); }
     H->crec.ensure(nr);
@@ -607,20 +615,20 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_scan_i32(H->hist_l.p, H->hist_l.p, nb_budget, ctl_at(H, z.st_hist, z.counters, 2), s,
                                  d_nb, d_nsorted));
     SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, nl, H->bucket_base.p, H->prow.p,
-                                       H->hist_l.p, H->fill_l.p, H->sbig.p, d_err, s, d_live));
+                                       H->hist_l.p, H->fill_l.p, H->sbig.p, d_err, s, d_live, narrow));
     SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, H->hist_l.p, H->bflag.p, s));
     SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, nb_budget, ctl_at(H, z.st_band, z.counters, 3), s,
                                  d_nb, d_nbands));
     SPR_HIP(spr::launch_band_emit(H->bflag.p, H->hist_l.p, (int32_t)nb_budget, nl, H->band_start.p, s, d_nb,
-                                  d_nsorted));
+                                  d_nsorted, H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr));
     stage_mark(H, 8);
     SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
                                    debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
                                    H->big_list.p + max_bands, d_bigcount, ctl_at(H, z.st_wind, z.counters, 4),
-                                   H->n_cu, s));
+                                   H->n_cu, s, narrow ? H->band_pr.p : nullptr, nl));
     SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                        H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr, H->winding.p,
-                                       s));
+                                       s, narrow ? H->band_pr.p : nullptr));
     SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
     stage_mark(H, 9);
     if (H->batch)
@@ -881,7 +889,7 @@ int spr_destroy(spr_handle* H) {
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
-    H->band_start.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
+    H->band_start.release(); H->band_pr.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
     drop_graph(H);
     H->pcanvas.release(); H->docs.release(); H->doc_first.release(); H->wbase.release();
     H->mlpdev.release();
