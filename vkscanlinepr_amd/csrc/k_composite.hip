@@ -28,6 +28,7 @@ constexpr int kBlock = 256;
 constexpr int kPix = 16;                   // consecutive pixels per thread
 constexpr int kChunk = kBlock * kPix;      // columns per pass
 constexpr int kBatch = 64;                 // paths per coverage mask
+constexpr int kList = 2 * kBlock;          // paths listed before their batches are folded
 constexpr int kUnroll = 4;                 // records loaded per thread before applying
 typedef unsigned long long u64;
 
@@ -69,10 +70,10 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                                       uint8_t* __restrict__ image,
                                                       const DocDev* __restrict__ docs, int n_docs) {
     __shared__ u64 ev[kChunk];
-    __shared__ int2 list[kBlock];
-    __shared__ uint32_t lcol[kBlock];
-    __shared__ float4 lsa[kBlock];   // (r*a, g*a, b*a, a) of each listed path, as the blend uses them
-    __shared__ float lia[kBlock];    // 1 - a
+    __shared__ int2 list[kList];
+    __shared__ uint32_t lcol[kList];
+    __shared__ float4 lsa[kBatch];   // (r*a, g*a, b*a, a) of each path of the batch, as the blend uses them
+    __shared__ float lia[kBatch];    // 1 - a
     __shared__ int32_t wcount[kBlock / 64];
     __shared__ int32_t bpre[kBatch + 1];
     __shared__ u64 xlds[kBlock / 64];
@@ -103,48 +104,58 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
             pix[j] = 0xFFFFFFFFu;
             if constexpr (COMPOSE != 0) { acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 1.0f; }
         }
-        for (uint32_t p0 = pbeg; p0 < pend; p0 += kBlock) {
-            // ordered list of the paths of this chunk that have records on this cell row
-            const uint32_t p = p0 + tid;
-            int2 rng = make_int2(0, 0);
-            if (p < pend) {
-                const int2 pr = prow[p];
-                const int loc = r == 0 ? pr.y + 1 : ((r >= pr.x && r < pr.x + pr.y) ? r - pr.x : -1);
-                if (loc >= 0) rng = brec[bucket_base[p] + loc];
-                // ranges never reach past the records actually stored (a frame whose record
-                // budget overflowed must still read in bounds)
-                rng.y = (int64_t)rng.y < rec_cap ? rng.y : (int32_t)rec_cap;
-                rng.x = rng.x < rng.y ? rng.x : rng.y;
-            }
-            const bool keep = rng.y > rng.x;
-            const u64 m = __ballot(keep);
-            if (lane == 0) wcount[wave] = (int32_t)__popcll(m);
-            __syncthreads();
-            int off = 0, n = 0;
-#pragma unroll
-            for (int k = 0; k < kBlock / 64; ++k) {
-                if (k < wave) off += wcount[k];
-                n += wcount[k];
-            }
-            if (keep) {
-                const int slot = off + (int)__popcll(m & ((1ull << lane) - 1ull));
-                list[slot] = rng;
-                const uint32_t col = fill[p];
-                lcol[slot] = col;
-                if constexpr (COMPOSE != 0) {
-                    const float a = (float)((col >> 24) & 0xFFu) / 255.0f;
-                    lsa[slot] = make_float4(((float)(col & 0xFFu) / 255.0f) * a, ((float)((col >> 8) & 0xFFu) / 255.0f) * a,
-                                            ((float)((col >> 16) & 0xFFu) / 255.0f) * a, a);
-                    lia[slot] = 1.0f - a;
+        uint32_t p0 = pbeg;
+        while (p0 < pend) {
+            // ordered list of the paths with records on this cell row, 256 candidates at a time
+            // until the list could overflow: batches are then full whatever the path numbering
+            int n = 0;
+            while (p0 < pend && n + kBlock <= kList) {
+                const uint32_t p = p0 + tid;
+                int2 rng = make_int2(0, 0);
+                if (p < pend) {
+                    const int2 pr = prow[p];
+                    const int loc = r == 0 ? pr.y + 1 : ((r >= pr.x && r < pr.x + pr.y) ? r - pr.x : -1);
+                    if (loc >= 0) rng = brec[bucket_base[p] + loc];
+                    // ranges never reach past the records actually stored (a frame whose record
+                    // budget overflowed must still read in bounds)
+                    rng.y = (int64_t)rng.y < rec_cap ? rng.y : (int32_t)rec_cap;
+                    rng.x = rng.x < rng.y ? rng.x : rng.y;
                 }
+                const bool keep = rng.y > rng.x;
+                const u64 m = __ballot(keep);
+                if (lane == 0) wcount[wave] = (int32_t)__popcll(m);
+                __syncthreads();
+                int off = n, add = 0;
+#pragma unroll
+                for (int k = 0; k < kBlock / 64; ++k) {
+                    if (k < wave) off += wcount[k];
+                    add += wcount[k];
+                }
+                if (keep) {
+                    const int slot = off + (int)__popcll(m & ((1ull << lane) - 1ull));
+                    list[slot] = rng;
+                    lcol[slot] = fill[p];
+                }
+                __syncthreads();
+                n += add;
+                p0 += kBlock;
             }
-            __syncthreads();
             for (int q0 = 0; q0 < n; q0 += kBatch) {
                 const int qn = n - q0 < kBatch ? n - q0 : kBatch;
-                // clear the event words; the batch's records as one flat index space
+                // clear the event words; the batch's colours; its records as one flat index space
 #pragma unroll
                 for (int j = 0; j < kPix; ++j) ev[j * kBlock + tid] = 0ull;
                 if (wave == 0) {
+                    if constexpr (COMPOSE != 0) {
+                        if (lane < qn) {
+                            const uint32_t col = lcol[q0 + lane];
+                            const float a = (float)((col >> 24) & 0xFFu) / 255.0f;
+                            lsa[lane] = make_float4(((float)(col & 0xFFu) / 255.0f) * a,
+                                                    ((float)((col >> 8) & 0xFFu) / 255.0f) * a,
+                                                    ((float)((col >> 16) & 0xFFu) / 255.0f) * a, a);
+                            lia[lane] = 1.0f - a;
+                        }
+                    }
                     const int2 rg = lane < qn ? list[q0 + lane] : make_int2(0, 0);
                     int inc = rg.y - rg.x;
 #pragma unroll
@@ -205,8 +216,8 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                         while (mm) {
                             const int q = __ffsll((long long)mm) - 1;
                             mm &= mm - 1ull;
-                            const float4 sa = lsa[q0 + q];
-                            const float ia = lia[q0 + q];
+                            const float4 sa = lsa[q];
+                            const float ia = lia[q];
                             acc[j][0] = sa.x + acc[j][0] * ia;
                             acc[j][1] = sa.y + acc[j][1] * ia;
                             acc[j][2] = sa.z + acc[j][2] * ia;
