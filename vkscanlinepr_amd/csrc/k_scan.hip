@@ -112,16 +112,6 @@ struct PathCursor {
     }
 };
 
-// Path of element i0 - 1 for the emit loop (same rule as PathCursor, without moving it).
-__device__ inline uint32_t pprev_of(const uint32_t* epath, const int32_t* sstart, uint32_t Np, int64_t i0,
-                                    uint32_t p_i0, int32_t) {
-    if (!sstart) return epath[i0 - 1] & 0x3FFFFFFFu;
-    uint32_t p = p_i0;
-    while (p > 0 && (int64_t)sstart[p] > i0 - 1) --p;
-    (void)Np;
-    return p;
-}
-
 __device__ inline int key_x(int32_t k) { return (k & 0xFFFF) - 0x7FFF; }
 __device__ inline int key_y(int32_t k) { return ((k >> 16) & 0xFFFF) - 0x7FFF; }
 
@@ -182,6 +172,17 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
             if (f < N) { keys[j] = skey[f]; wind[j] = winding[f]; paths[j] = cur.at(f); }
         }
     }
+    // the neighbours across the thread's range, and the per-path tables of its first and last
+    // paths (sorted, so the 16 elements almost always have one or two paths), all loaded here
+    const int32_t k_before = kprev;
+    const uint32_t p_before = pprev;
+    const int64_t inext = i0 + kScanItems;
+    const int32_t knext_out = inext < N ? skey[inext] : 0;
+    const uint32_t pnext_out = inext < N ? cur.at(inext) : 0xFFFFFFFFu;
+    const uint32_t pa = paths[0], pz = paths[kScanItems - 1];
+    const uint32_t rule_a = rule[pa], rule_z = rule[pz];
+    const int2 pr_a = prow[pa], pr_z = prow[pz];
+    const int32_t bb_a = bucket_base[pa], bb_z = bucket_base[pz];
     uint32_t cellbits = 0, spanbits = 0;
 #pragma unroll
     for (int j = 0; j < kScanItems; ++j) {
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
         if (f == 0) {
             cell = in_canvas;
         } else {
-            const uint32_t r = rule[p1];
+            const uint32_t r = p1 == pa ? rule_a : (p1 == pz ? rule_z : rule[p1]);
             const int x0 = key_x(kprev), y0 = key_y(kprev);
             cell = in_canvas && (pprev != p1 || kprev != k1);
             // documents of a batch each scan their own winding numbers from zero
@@ -223,11 +224,8 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     const int64_t tile_rec0 = (int64_t)(s_prefix & 0xFFFFFFFFull) + (int64_t)(s_prefix >> 32);
     const int64_t tile_recs = (int64_t)(total & 0xFFFFFFFFull) + (int64_t)(total >> 32);
     if (i0 < N) {
-        int32_t kp = (i0 > 0) ? skey[i0 - 1] : 0;
-        uint32_t pp = (i0 > 0) ? pprev_of(epath, sstart, Np, i0, paths[0], keys[0]) : 0xFFFFFFFFu;
-        const int64_t inext = i0 + kScanItems;
-        const int32_t knext_out = inext < N ? skey[inext] : 0;
-        const uint32_t pnext_out = inext < N ? cur.at(inext) : 0xFFFFFFFFu;
+        int32_t kp = (i0 > 0) ? k_before : 0;
+        uint32_t pp = (i0 > 0) ? p_before : 0xFFFFFFFFu;
 #pragma unroll
         for (int j = 0; j < kScanItems; ++j) {
             const int64_t f = i0 + j;
@@ -269,11 +267,12 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
                 const bool first = (f == 0) || pp != p1 || (kp >> 16) != (k1 >> 16);
                 const bool last = (f == N - 1) || pn != p1 || (kn >> 16) != (k1 >> 16);
                 if (first || last) {
-                    const int2 pr = prow[p1];
+                    const int2 pr = p1 == pa ? pr_a : (p1 == pz ? pr_z : prow[p1]);
                     const int r = y1 >> 1;
                     const int loc = r == 0 ? pr.y + 1 : r - pr.x;
                     if (loc >= 0 && loc < pr.y + 2) {  // out of range only after a flagged bound error
-                        int2* br = &brec[bucket_base[p1] + loc];
+                        const int32_t bb = p1 == pa ? bb_a : (p1 == pz ? bb_z : bucket_base[p1]);
+                        int2* br = &brec[bb + loc];
                         const int64_t e = min((int64_t)(cells_before + spans_before + cell + span), rec_cap);
                         if (first) br->x = (int32_t)min((int64_t)(cells_before + spans_before), e);
                         if (last) br->y = (int32_t)e;
