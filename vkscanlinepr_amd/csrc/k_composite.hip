@@ -73,7 +73,6 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
     __shared__ int2 list[kList];
     __shared__ uint32_t lcol[kList];
     __shared__ float4 lsa[kBatch];   // (r*a, g*a, b*a, a) of each path of the batch, as the blend uses them
-    __shared__ float lia[kBatch];    // 1 - a
     __shared__ int32_t wcount[kBlock / 64];
     __shared__ int32_t bpre[kBatch + 1];
     __shared__ u64 xlds[kBlock / 64];
@@ -153,7 +152,6 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                             lsa[lane] = make_float4(((float)(col & 0xFFu) / 255.0f) * a,
                                                     ((float)((col >> 8) & 0xFFu) / 255.0f) * a,
                                                     ((float)((col >> 16) & 0xFFu) / 255.0f) * a, a);
-                            lia[lane] = 1.0f - a;
                         }
                     }
                     const int2 rg = lane < qn ? list[q0 + lane] : make_int2(0, 0);
@@ -213,11 +211,12 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                     if constexpr (COMPOSE == 0) {
                         pix[j] = lcol[q0 + 63 - __clzll((long long)mm)];
                     } else {
+                        // src-over, d = s*a + d*(1-a) per channel, the paths in order
                         while (mm) {
-                            const int q = __ffsll((long long)mm) - 1;
+                            const int q = __builtin_ctzll(mm);
                             mm &= mm - 1ull;
                             const float4 sa = lsa[q];
-                            const float ia = lia[q];
+                            const float ia = 1.0f - sa.w;
                             acc[j][0] = sa.x + acc[j][0] * ia;
                             acc[j][1] = sa.y + acc[j][1] * ia;
                             acc[j][2] = sa.z + acc[j][2] * ia;
