@@ -5,9 +5,11 @@
 // int32: rows 2..h-2, then the off-canvas sentinel, then row 0; gen_fragment.comp:180-187,
 // naive_seg_sort_pairs.comp:69).  The transform pass bounds each path's rows from its control
 // points (a Bezier curve stays inside their hull), so each path owns a small, exactly sized
-// bucket range.  Entries are counted per bucket while their keys are generated, scattered once
-// into bucket order, and the buckets are packed into bands of at most 2048 entries that one
-// workgroup sorts in LDS.  The per-bucket ranges also give the composite each path's records of
+// bucket range.  Entries are counted per bucket while their keys are generated, and the buckets
+// are packed into bands of at most 2048 entries that one workgroup sorts in LDS.  The scatter
+// only has to bring each entry into its band (the band sort orders by bucket, then x), so the
+// entries of a run that fall into one band take their slots with one atomic and are stored
+// contiguously.  After the sort the per-bucket ranges give the composite each path's records of
 // each cell row without any further sorting.
 #include "device_common.hpp"
 #include "kernels.hpp"
@@ -40,14 +42,16 @@ __global__ void k_bucket_counts(const int32_t* __restrict__ row_lo, const int32_
     nb[p] = nrows + 2;
 }
 
-// Scatter into bucket order.  The wide sort word orders exactly like the reference's signed
+// Scatter into band order.  The wide sort word orders exactly like the reference's signed
 // (key, index) pair and carries the winding change in its two low bits.  The narrow word (used
 // when no stage is exported) drops the index: (bucket inside its path) << 17 | (x + 2) << 2 |
 // (winding change + 1); inside a band (whole buckets of one path) it orders like the key, and the
 // order among equal keys changes no record, no span and no pixel (no span starts between equal
-// keys, and the winding number after a run of equal keys is the sum over the whole run).  Each
-// wave takes kRounds chunks of 64 consecutive entries and issues every load, then every slot
-// atomic (one per run of equal buckets), of all chunks before it waits on any of them.
+// keys, and the winding number after a run of equal keys is the sum over the whole run).  The
+// band of bucket b is flag_scan[b + 1] - 1 (flag_scan: exclusive scan of the band-start flags).
+// Each wave takes kRounds chunks of 64 consecutive entries and issues every load, then every
+// slot atomic (one per run of equal bands, on the band's fill counter), of all chunks before it
+// waits on any of them.
 constexpr int kRounds = 4;
 
 template <bool NARROW>
@@ -56,7 +60,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
                                                            const uint32_t* __restrict__ epath, int64_t N,
                                                            const int32_t* __restrict__ bucket_base,
                                                            const int2* __restrict__ prow,
-                                                           const int32_t* __restrict__ boff,
+                                                           const int32_t* __restrict__ flag_scan,
+                                                           const int32_t* __restrict__ band_start,
                                                            int32_t* __restrict__ fill,
                                                            unsigned long long* __restrict__ out,
                                                            uint32_t* __restrict__ out32,
@@ -88,7 +93,10 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
             b[r] = (uint32_t)bucket_base[pth[r]] + loc[r];
         }
     }
-    // runs of equal buckets: heads take their run's slots with one atomic each
+#pragma unroll
+    for (int r = 0; r < kRounds; ++r)
+        if (valid[r]) b[r] = (uint32_t)(flag_scan[b[r] + 1] - 1);  // bucket -> band
+    // runs of equal bands: heads take their run's slots with one atomic each
     unsigned long long vm[kRounds], hd[kRounds];
     int32_t base[kRounds];
 #pragma unroll
@@ -96,7 +104,8 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
         vm[r] = __ballot(valid[r]);
         hd[r] = run_heads(b[r], valid[r], vm[r]);
         base[r] = 0;
-        if ((hd[r] >> lane) & 1ull) base[r] = atomicAdd(&fill[b[r]], run_length(lane, hd[r], vm[r]));
+        if ((hd[r] >> lane) & 1ull)
+            base[r] = band_start[b[r]] + atomicAdd(&fill[b[r]], run_length(lane, hd[r], vm[r]));
     }
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
@@ -108,11 +117,11 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
             if constexpr (NARROW) {
                 // the off-canvas bucket holds one key: its x is left at 0
                 const uint32_t xl = sent[r] ? 0u : ((uint32_t)k[r] & 0xFFFFu) - kNarrowX0;
-                out32[boff[b[r]] + slot] = (loc[r] << 17) | (xl << 2) | (uint32_t)(wn[i] + 1);
+                out32[slot] = (loc[r] << 17) | (xl << 2) | (uint32_t)(wn[i] + 1);
             } else {
                 const unsigned long long w = ((unsigned long long)((uint32_t)k[r] ^ 0x80000000u) << 32) |
                                              ((unsigned long long)i << 2) | (unsigned long long)(uint32_t)(wn[i] + 1);
-                out[boff[b[r]] + slot] = w;
+                out[slot] = w;
             }
         }
     }
@@ -197,17 +206,17 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
 }
 
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
-                                 const int32_t* bucket_base, const int2* prow, const int32_t* boff,
-                                 int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s,
-                                 const int32_t* n_dev, bool narrow) {
+                                 const int32_t* bucket_base, const int2* prow, const int32_t* flag_scan,
+                                 const int32_t* band_start, int32_t* fill, unsigned long long* out, int32_t* err,
+                                 hipStream_t s, const int32_t* n_dev, bool narrow) {
     if (N == 0) return hipSuccess;
     const unsigned g = grid_for((N + kRounds - 1) / kRounds);
     if (narrow)
-        k_bucket_scatter<true><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, boff, fill, nullptr,
-                                                    reinterpret_cast<uint32_t*>(out), err, n_dev);
+        k_bucket_scatter<true><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan, band_start,
+                                                    fill, nullptr, reinterpret_cast<uint32_t*>(out), err, n_dev);
     else
-        k_bucket_scatter<false><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, boff, fill, out, nullptr,
-                                                     err, n_dev);
+        k_bucket_scatter<false><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan, band_start,
+                                                     fill, out, nullptr, err, n_dev);
     return hipGetLastError();
 }
 

@@ -131,9 +131,9 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
 constexpr uint32_t kNarrowX0 = 0x7FFDu;   // x field of the leftmost key (x = -2)
 constexpr int kNarrowMaxW = (1 << 15) - 4;
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
-                                 const int32_t* bucket_base, const int2* prow, const int32_t* boff,
-                                 int32_t* fill, unsigned long long* out, int32_t* err, hipStream_t s,
-                                 const int32_t* n_dev = nullptr, bool narrow = false);
+                                 const int32_t* bucket_base, const int2* prow, const int32_t* flag_scan,
+                                 const int32_t* band_start, int32_t* fill, unsigned long long* out, int32_t* err,
+                                 hipStream_t s, const int32_t* n_dev = nullptr, bool narrow = false);
 hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
                              hipStream_t s);
 // band_pr (or null): each band's path rows (prow of its path), for the narrow sort.

@@ -377,15 +377,16 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
                                          H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err, s));
     stage_mark(H, 7);
-    // ---- sort: bucket offsets, scatter into bucket order, band sort in LDS
+    // ---- sort: bucket offsets, bands, scatter into band order, band sort in LDS
     if (N > 0) {
         SPR_HIP(spr::launch_scan_i32(d_hist, d_hist, NB, ctl_at(H, z.st_hist, z.ctr, 0), s));
-        SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, N, H->bucket_base.p, H->prow.p, d_hist,
-                                           d_fill, H->sbig.p, d_err, s, nullptr, narrow));
         SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, d_hist, H->bflag.p, s));
         SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, NB, ctl_at(H, z.st_band, z.ctr, 1), s));
         SPR_HIP(spr::launch_band_emit(H->bflag.p, d_hist, (int32_t)NB, N, H->band_start.p, s, nullptr, nullptr,
                                       H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr));
+        SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, N, H->bucket_base.p, H->prow.p,
+                                           H->bflag.p, H->band_start.p, d_fill, H->sbig.p, d_err, s, nullptr,
+                                           narrow));
     }
     stage_mark(H, 8);
     if (N > 0) {
@@ -614,13 +615,14 @@ void run_learned_frame(spr_handle* H) {
     stage_mark(H, 7);
     SPR_HIP(spr::launch_scan_i32(H->hist_l.p, H->hist_l.p, nb_budget, ctl_at(H, z.st_hist, z.counters, 2), s,
                                  d_nb, d_nsorted));
-    SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, nl, H->bucket_base.p, H->prow.p,
-                                       H->hist_l.p, H->fill_l.p, H->sbig.p, d_err, s, d_live, narrow));
     SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, H->hist_l.p, H->bflag.p, s));
     SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, nb_budget, ctl_at(H, z.st_band, z.counters, 3), s,
                                  d_nb, d_nbands));
     SPR_HIP(spr::launch_band_emit(H->bflag.p, H->hist_l.p, (int32_t)nb_budget, nl, H->band_start.p, s, d_nb,
                                   d_nsorted, H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr));
+    SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, nl, H->bucket_base.p, H->prow.p,
+                                       H->bflag.p, H->band_start.p, H->fill_l.p, H->sbig.p, d_err, s, d_live,
+                                       narrow));
     stage_mark(H, 8);
     SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
                                    debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
