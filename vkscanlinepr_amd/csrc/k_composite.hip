@@ -212,15 +212,21 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                         pix[j] = lcol[q0 + 63 - __clzll((long long)mm)];
                     } else {
                         // src-over, d = s*a + d*(1-a) per channel, the paths in order
-                        while (mm) {
-                            const int q = __builtin_ctzll(mm);
-                            mm &= mm - 1ull;
-                            const float4 sa = lsa[q];
-                            const float ia = 1.0f - sa.w;
-                            acc[j][0] = sa.x + acc[j][0] * ia;
-                            acc[j][1] = sa.y + acc[j][1] * ia;
-                            acc[j][2] = sa.z + acc[j][2] * ia;
-                            acc[j][3] = sa.w + acc[j][3] * ia;
+                        // the mask's two words in turn: 32-bit bit scans and clears
+#pragma unroll
+                        for (int hw = 0; hw < 2; ++hw) {
+                            uint32_t m32 = (uint32_t)(mm >> (32 * hw));
+                            const float4* L = lsa + 32 * hw;
+                            while (m32) {
+                                const int q = __builtin_ctz(m32);
+                                m32 &= m32 - 1u;
+                                const float4 sa = L[q];
+                                const float ia = 1.0f - sa.w;
+                                acc[j][0] = sa.x + acc[j][0] * ia;
+                                acc[j][1] = sa.y + acc[j][1] * ia;
+                                acc[j][2] = sa.z + acc[j][2] * ia;
+                                acc[j][3] = sa.w + acc[j][3] * ia;
+                            }
                         }
                     }
                 }
