@@ -51,7 +51,8 @@ __global__ void k_bucket_counts(const int32_t* __restrict__ row_lo, const int32_
 // band of bucket b is flag_scan[b + 1] - 1 (flag_scan: exclusive scan of the band-start flags).
 // Each wave takes kRounds chunks of 64 consecutive entries and issues every load, then every
 // slot atomic (one per run of equal bands, on the band's fill counter), of all chunks before it
-// waits on any of them.
+// waits on any of them.  The head of a run also adds the run's winding changes to the band's sum
+// (bsum), from which one scan gives every band its exclusive prefix before the band sort.
 constexpr int kRounds = 4;
 
 template <bool NARROW>
@@ -63,6 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
                                                            const int32_t* __restrict__ flag_scan,
                                                            const int32_t* __restrict__ band_start,
                                                            int32_t* __restrict__ fill,
+                                                           int32_t* __restrict__ bsum,
                                                            unsigned long long* __restrict__ out,
                                                            uint32_t* __restrict__ out32,
                                                            int32_t* __restrict__ err,
@@ -71,15 +73,15 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
     const int lane = threadIdx.x & 63;
     const int64_t g0 = ((int64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64 * kRounds);
     uint32_t pth[kRounds];
-    int32_t k[kRounds];
+    int32_t k[kRounds], wv[kRounds];
     bool valid[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         const int64_t i = g0 + r * 64 + lane;
         valid[r] = i < n;
-        pth[r] = kNoPath; k[r] = 0;
-        if (valid[r]) { pth[r] = epath[i]; k[r] = key[i]; }
-        if (pth[r] == kNoPath) valid[r] = false;  // gap of a predicted-allocation layout
+        pth[r] = kNoPath; k[r] = 0; wv[r] = 0;
+        if (valid[r]) { pth[r] = epath[i]; k[r] = key[i]; wv[r] = wn[i]; }
+        if (pth[r] == kNoPath) { valid[r] = false; wv[r] = 0; }  // gap of a predicted-allocation layout
     }
     uint32_t b[kRounds], loc[kRounds];
     bool sent[kRounds];
@@ -96,16 +98,29 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
 #pragma unroll
     for (int r = 0; r < kRounds; ++r)
         if (valid[r]) b[r] = (uint32_t)(flag_scan[b[r] + 1] - 1);  // bucket -> band
-    // runs of equal bands: heads take their run's slots with one atomic each
+    // runs of equal bands: heads take their run's slots with one atomic each, and add the run's
+    // winding changes (an inclusive wave scan read at the run's last lane) to the band's sum
     unsigned long long vm[kRounds], hd[kRounds];
     int32_t base[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         vm[r] = __ballot(valid[r]);
         hd[r] = run_heads(b[r], valid[r], vm[r]);
+        int32_t inc = wv[r];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int32_t u = __shfl_up(inc, d);
+            if (lane >= d) inc += u;
+        }
+        const bool head = (hd[r] >> lane) & 1ull;
+        const int len = head ? run_length(lane, hd[r], vm[r]) : 1;
+        const int32_t at_end = __shfl(inc, lane + len - 1);
         base[r] = 0;
-        if ((hd[r] >> lane) & 1ull)
-            base[r] = band_start[b[r]] + atomicAdd(&fill[b[r]], run_length(lane, hd[r], vm[r]));
+        if (head) {
+            base[r] = band_start[b[r]] + atomicAdd(&fill[b[r]], len);
+            const int32_t rs = at_end - inc + wv[r];
+            if (rs != 0) atomicAdd(&bsum[b[r]], rs);
+        }
     }
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
@@ -117,10 +132,10 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
             if constexpr (NARROW) {
                 // the off-canvas bucket holds one key: its x is left at 0
                 const uint32_t xl = sent[r] ? 0u : ((uint32_t)k[r] & 0xFFFFu) - kNarrowX0;
-                out32[slot] = (loc[r] << 17) | (xl << 2) | (uint32_t)(wn[i] + 1);
+                out32[slot] = (loc[r] << 17) | (xl << 2) | (uint32_t)(wv[r] + 1);
             } else {
                 const unsigned long long w = ((unsigned long long)((uint32_t)k[r] ^ 0x80000000u) << 32) |
-                                             ((unsigned long long)i << 2) | (unsigned long long)(uint32_t)(wn[i] + 1);
+                                             ((unsigned long long)i << 2) | (unsigned long long)(uint32_t)(wv[r] + 1);
                 out[slot] = w;
             }
         }
@@ -207,16 +222,17 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
 
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
                                  const int32_t* bucket_base, const int2* prow, const int32_t* flag_scan,
-                                 const int32_t* band_start, int32_t* fill, unsigned long long* out, int32_t* err,
-                                 hipStream_t s, const int32_t* n_dev, bool narrow) {
+                                 const int32_t* band_start, int32_t* fill, int32_t* bsum, unsigned long long* out,
+                                 int32_t* err, hipStream_t s, const int32_t* n_dev, bool narrow) {
     if (N == 0) return hipSuccess;
     const unsigned g = grid_for((N + kRounds - 1) / kRounds);
     if (narrow)
         k_bucket_scatter<true><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan, band_start,
-                                                    fill, nullptr, reinterpret_cast<uint32_t*>(out), err, n_dev);
+                                                    fill, bsum, nullptr, reinterpret_cast<uint32_t*>(out), err,
+                                                    n_dev);
     else
         k_bucket_scatter<false><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan, band_start,
-                                                     fill, out, nullptr, err, n_dev);
+                                                     fill, bsum, out, nullptr, err, n_dev);
     return hipGetLastError();
 }
 

@@ -10,14 +10,19 @@
 // winding change, so the reference's separate gather pass disappears.
 //
 // After the bucket scatter every band (<= 2048 entries, a run of whole buckets of one path) only
-// needs sorting in place: one 256-thread workgroup loads it once, sorts it in LDS (8-element
-// register network per thread, then merge-path rounds) and stores it once.  Bands are taken in
-// array order (an atomic ticket), so the exclusive scan of the winding changes runs across them
-// with a decoupled look-back on each band's sum (a sum needs no sorting), and each band writes
-// its winding numbers next to its keys.  A band above 2048 entries is a single oversized (path,
-// row) bucket; such bands publish their sum at once and are queued, and each is then sorted by
-// one workgroup with LDS chunk sorts followed by merge-path passes through global memory, the
-// last pass writing keys and winding numbers.
+// needs sorting in place: one workgroup loads it once, sorts it in LDS (8-element register
+// network per thread, then merge-path rounds) and stores it once.  A band's sum of winding
+// changes needs no sorting: the scatter adds the sums up per band and one scan turns them into
+// each band's exclusive prefix before any band is sorted, so the bands are independent and each
+// writes its winding numbers next to its keys.  Bands of up to 512 entries are sorted by
+// one-wave workgroups (so that the many short bands of a page of small paths do not leave most
+// of a large workgroup idle), the longer ones, listed by a classifying pass, by 256-thread
+// workgroups that take them in chunks, the first chunk by workgroup id and the rest from an
+// atomic ticket (their cost varies with the band length, so the work is handed out as it frees
+// up, and a frame without long bands takes no ticket at all).  A band above 2048 entries is a single oversized (path, row)
+// bucket; such bands are queued, and each is then sorted by one workgroup with LDS chunk sorts
+// followed by merge-path passes through global memory, the last pass writing keys and winding
+// numbers.
 //
 // Frames that export no stage sort the narrow 32-bit words of the scatter instead (bucket inside
 // the path, x, winding change: see k_bucket.hip), with the same kernels; the keys are rebuilt
@@ -30,10 +35,20 @@ namespace spr {
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kItems = 8;
-static_assert(kThreads * kItems == kSortTile, "tile shape");
+constexpr int kItems = 8;  // sort words per thread
 typedef unsigned long long u64;
+
+// Workgroup shape of a band class: THREADS threads sort up to THREADS * kItems words, in two LDS
+// buffers with one pad word per 8 words (so the 8-word runs of consecutive lanes start on
+// different banks).
+template <int THREADS>
+struct Tile {
+    static constexpr int kWaves = THREADS / 64;
+    static constexpr int kSize = THREADS * kItems;
+    static constexpr int kPadded = kSize + kSize / 8;
+};
+constexpr int kBigThreads = 256;  // oversized bands: chunks of one full tile
+static_assert(Tile<kBigThreads>::kSize == kSortTile, "tile shape");
 
 template <class T>
 __device__ inline void cswap(T& a, T& b) {
@@ -111,20 +126,17 @@ template <class T>
 __device__ inline int32_t wn_of(T w) { return (int32_t)(w & 3u) - 1; }
 __device__ inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 
-// LDS slot of element i: one pad word per 8 elements, so the 8-element runs owned by
-// consecutive lanes start on different banks.
-constexpr int kPadded = kSortTile + kSortTile / 8;
+// LDS slot of element i (one pad word per 8 elements).
 __device__ inline int pad(int i) { return i + (i >> 3); }
 
-// Sorts n <= kSortTile words from src with one workgroup (buf: 2 x kPadded words of LDS) into dst
-// (sort words) or, when dst is null, unpacked into out at offset o0 with winding numbers from
-// the exclusive prefix `pre` (then `band` must be the band's ticket for the look-back and
-// `status` its status array: the band's sum is published before the sort).
-template <class W>
-__device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __restrict__ src, typename W::T* __restrict__ dst,
-                                 int n, typename W::T (*buf)[kPadded], const Unpacked& out, int64_t o0,
-                                 unsigned long long* status, int64_t band, int32_t* s_w, int32_t* s_pre, bool last,
-                                 int64_t total_at) {
+// Sorts n <= THREADS * kItems words from src with one workgroup of THREADS threads (buf: 2 x
+// kPadded words of LDS) into dst (sort words) or, when dst is null, unpacked into out at offset
+// o0 with winding numbers from the band's exclusive prefix `pre`.
+template <class W, int THREADS>
+__device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __restrict__ src,
+                                          typename W::T* __restrict__ dst, int n,
+                                          typename W::T (*buf)[Tile<THREADS>::kPadded], const Unpacked& out,
+                                          int64_t o0, int32_t pre, int32_t* s_w) {
     typedef typename W::T T;
     constexpr T kPad = W::kPad;
     int P2 = kItems;
@@ -132,20 +144,12 @@ __device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __re
     const int active = P2 / kItems;
     const int tid = threadIdx.x;
     T v[kItems];
-    int32_t part = 0;
     if (tid < active) {
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const int i = tid * kItems + j;
             v[j] = i < n ? src[i] : kPad;
-            if (i < n) part += wn_of(v[j]);
         }
-    }
-    int32_t agg = 0;
-    if (!dst) {
-        // the band's sum is published at once; its exclusive prefix is looked up after the sort
-        block_exclusive_add(part, s_w, agg);
-        if (tid < 64) publish_aggregate<OpI32>(status, band, agg);
     }
     if (tid < active) {
         sort16(v);
@@ -193,19 +197,10 @@ __device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __re
         cur ^= 1;
     }
     if (dst) {
-        for (int i = tid; i < n; i += kThreads) dst[i] = buf[cur][pad(i)];
+        for (int i = tid; i < n; i += THREADS) dst[i] = buf[cur][pad(i)];
         __syncthreads();
         return;
     }
-    // exclusive prefix of the band (the predecessors have long published their sums)
-    if (tid < 64) {
-        const int32_t pre = lookback<OpI32>(status, band, agg, true);
-        if (tid == 0) {
-            *s_pre = pre;
-            if (last) out.winding[total_at] = wadd(pre, agg);  // the scan's total
-        }
-    }
-    __syncthreads();
     // winding numbers: each thread scans its 8 consecutive sorted entries, the block scans the
     // thread sums, and the results go through the free LDS buffer to coalesced stores
     int32_t* wl = reinterpret_cast<int32_t*>(buf[cur ^ 1]);
@@ -217,64 +212,87 @@ __device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __re
         if (i < n) sum += wn_of(buf[cur][pad(i)]);
     }
     int32_t tot;
-    const int32_t ex = wadd(block_exclusive_add(sum, s_w, tot), *s_pre);
+    const int32_t ex = wadd(block_exclusive_add<int32_t, Tile<THREADS>::kWaves>(sum, s_w, tot), pre);
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const int i = tid * kItems + k;
         if (i < n) wl[i] = wadd(ex, w8[k]);
     }
     __syncthreads();
-    for (int i = tid; i < n; i += kThreads) {
+    for (int i = tid; i < n; i += THREADS) {
         wd.put(out, o0 + i, buf[cur][pad(i)]);
         out.winding[o0 + i] = wl[i];
     }
     __syncthreads();
 }
 
-// Persistent workgroups take the bands in array order from an atomic ticket (so every band they
-// wait on in the look-back is held by a running workgroup).  A band above one tile publishes its
-// sum (and the scan total when it is the last band) and is queued with its exclusive prefix.
-template <class W>
-__global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restrict__ band_start,
-                                                         const int32_t* __restrict__ n_bands,
-                                                         const typename W::T* __restrict__ src, Unpacked out,
-                                                         int32_t* __restrict__ big_list,
-                                                         int32_t* __restrict__ big_excl,
-                                                         int32_t* __restrict__ big_count,
-                                                         unsigned long long* __restrict__ status,
-                                                         uint32_t* __restrict__ counter,
-                                                         const int2* __restrict__ band_pr) {
-    __shared__ typename W::T buf[2][kPadded];
-    __shared__ int32_t s_tile, s_w[kThreads / 64], s_pre;
+// Long bands (above one 64-thread tile) are listed in `list`, their number in *count, one
+// wave-aggregated atomic per wave; the order of the list does not matter.
+constexpr int kShortMax = 64 * kItems;
+__global__ __launch_bounds__(256) void k_band_classify(const int32_t* __restrict__ band_start,
+                                                       const int32_t* __restrict__ n_bands,
+                                                       int32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int32_t nb = *n_bands;
+    const bool is_long = t < nb && band_start[t + 1] - band_start[t] > kShortMax;
+    const unsigned long long m = __ballot(is_long);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == __ffsll((long long)m) - 1) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, __ffsll((long long)m) - 1);
+    if (is_long) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)t;
+}
+
+// BIG = false: the bands of up to THREADS * kItems entries, the workgroups striding over all
+// bands.  BIG = true: the listed long bands, kChunk at a time (the first chunk by workgroup id,
+// the rest from ticket[0]; ticket[1] holds the list length); this kernel also queues the
+// oversized bands with their exclusive prefixes and writes the scan's total after the last entry.
+constexpr int kChunk = 4;
+
+template <class W, int THREADS, bool BIG>
+__global__ __launch_bounds__(THREADS) void k_sort_bands(const int32_t* __restrict__ band_start,
+                                                        const int32_t* __restrict__ n_bands,
+                                                        const typename W::T* __restrict__ src, Unpacked out,
+                                                        const int32_t* __restrict__ bpre,
+                                                        const int32_t* __restrict__ list,
+                                                        uint32_t* __restrict__ ticket,
+                                                        int32_t* __restrict__ big_list,
+                                                        int32_t* __restrict__ big_excl,
+                                                        int32_t* __restrict__ big_count,
+                                                        const int2* __restrict__ band_pr) {
+    constexpr int kSize = Tile<THREADS>::kSize;
+    __shared__ typename W::T buf[2][Tile<THREADS>::kPadded];
+    __shared__ int32_t s_w[Tile<THREADS>::kWaves], s_t;
+    const int32_t nb = *n_bands;
+    if (BIG && blockIdx.x == 0 && threadIdx.x == 0) {
+        if (nb > 0) out.winding[band_start[nb]] = bpre[nb];  // the scan's total
+        else out.winding[0] = 0;
+    }
+    const int64_t n_items = BIG ? (int64_t)ticket[1] : (int64_t)nb;
+    int64_t c = blockIdx.x;
     while (true) {
-        const int64_t t = ordered_tile_id(counter, &s_tile);
-        if (t >= nb) {
-            if (t == 0 && threadIdx.x == 0) out.winding[0] = 0;  // no bands: the total is 0
-            return;
-        }
-        const int64_t s = band_start[t], e = band_start[t + 1];
-        W wd;
-        wd.band(band_pr, t);  // loaded with the band's bounds
-        const int64_t n = e - s;
-        const bool last = t == nb - 1;
-        if (n > kSortTile) {
-            int32_t part = 0;
-            for (int64_t i = threadIdx.x; i < n; i += kThreads) part += wn_of(src[s + i]);
-            int32_t agg;
-            block_exclusive_add(part, s_w, agg);
-            if (threadIdx.x < 64) {
-                const int32_t pre = lookback<OpI32>(status, t, agg);
-                if (threadIdx.x == 0) {
+        const int64_t q0 = BIG ? c * kChunk : c;
+        if (q0 >= n_items) return;
+        const int64_t q1 = BIG ? (q0 + kChunk < n_items ? q0 + kChunk : n_items) : q0 + 1;
+        for (int64_t q = q0; q < q1; ++q) {
+            const int64_t t = BIG ? list[q] : q;
+            const int64_t s = band_start[t], n = band_start[t + 1] - s;
+            if (!BIG && n > kSize) continue;  // a long band
+            if (n > kSize) {
+                if (BIG && threadIdx.x == 0) {
                     const int32_t q = atomicAdd(big_count, 1);
                     big_list[q] = (int32_t)t;
-                    big_excl[q] = pre;
-                    if (last) out.winding[e] = wadd(pre, agg);
+                    big_excl[q] = bpre[t];
                 }
+                continue;
             }
-            continue;
+            W wd;
+            wd.band(band_pr, t);
+            tile_sort<W, THREADS>(wd, src + s, (typename W::T*)nullptr, (int)n, buf, out, s, bpre[t], s_w);
         }
-        tile_sort(wd, src + s, (typename W::T*)nullptr, (int)n, buf, out, s, status, t, s_w, &s_pre, last, e);
+        // any order: only the hand-out of the chunks matters
+        c = BIG ? (int64_t)gridDim.x + ordered_tile_id(ticket, &s_t) : c + gridDim.x;
     }
 }
 
@@ -282,7 +300,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_bands(const int32_t* __restri
 // `a` and `b` (the scatter buffer, already consumed), the last pass writing keys and winding
 // numbers (the band's exclusive prefix carried across its 2048-entry output blocks).
 template <class W>
-__global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __restrict__ band_start,
+__global__ __launch_bounds__(kBigThreads) void k_sort_big_bands(const int32_t* __restrict__ band_start,
                                                              const int32_t* __restrict__ big_list,
                                                              const int32_t* __restrict__ big_excl,
                                                              const int32_t* __restrict__ big_count,
@@ -290,8 +308,9 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
                                                              typename W::T* __restrict__ a, Unpacked out,
                                                              const int2* __restrict__ band_pr) {
     typedef typename W::T T;
-    __shared__ T buf[2][kPadded];
-    __shared__ int32_t s_w[kThreads / 64];
+    constexpr int kThreads = kBigThreads;
+    __shared__ T buf[2][Tile<kThreads>::kPadded];
+    __shared__ int32_t s_w[Tile<kThreads>::kWaves];
     const int32_t nbig = *big_count;
     for (int32_t q = blockIdx.x; q < nbig; q += gridDim.x) {
         const int32_t band = big_list[q];
@@ -300,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
         wd.band(band_pr, band);
         for (int64_t c = 0; c < n; c += kSortTile) {
             const int m = (int)(n - c < kSortTile ? n - c : kSortTile);
-            tile_sort(wd, scatter + s + c, a + s + c, m, buf, out, 0, nullptr, 0, s_w, nullptr, false, 0);
+            tile_sort<W, kThreads>(wd, scatter + s + c, a + s + c, m, buf, out, 0, 0, s_w);
         }
         __threadfence_block();
         __syncthreads();
@@ -338,7 +357,7 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
                 int32_t local = 0;
                 for (int k = 0; k < cnt; ++k) local += wn_of(vals[k]);
                 int32_t tot;
-                int32_t run = wadd(carry, block_exclusive_add(local, s_w, tot));
+                int32_t run = wadd(carry, block_exclusive_add<int32_t, Tile<kThreads>::kWaves>(local, s_w, tot));
                 for (int k = 0; k < cnt; ++k) {
                     wd.put(out, s + o + k, vals[k]);
                     out.winding[s + o + k] = run;
@@ -356,29 +375,48 @@ __global__ __launch_bounds__(kThreads) void k_sort_big_bands(const int32_t* __re
 
 }  // namespace
 
+namespace {
+
+// Workgroups per CU for a class: as many as the CU's 32 wave slots and 160 KiB of LDS allow.
+template <class T, int THREADS>
+unsigned class_grid(int64_t max_bands, int n_cu) {
+    constexpr int kLds = (int)(2 * Tile<THREADS>::kPadded * sizeof(T)) + 256;
+    constexpr int kByWaves = 32 / Tile<THREADS>::kWaves, kByLds = 160 * 1024 / kLds;
+    constexpr int kPerCu = kByWaves < kByLds ? kByWaves : kByLds;
+    const int64_t g = (int64_t)n_cu * kPerCu;
+    return (unsigned)(max_bands < g ? max_bands : g);
+}
+
+template <class W>
+hipError_t launch_classes(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
+                          const typename W::T* src, const Unpacked& out, const int32_t* bpre, uint32_t* ticket,
+                          int32_t* big_list, int32_t* big_excl, int32_t* big_count, int n_cu, hipStream_t s,
+                          const int2* band_pr) {
+    typedef typename W::T T;
+    int32_t* long_list = big_list + 2 * max_bands;
+    k_band_classify<<<(unsigned)((max_bands + 255) / 256), 256, 0, s>>>(band_start, n_bands, long_list, ticket + 1);
+    k_sort_bands<W, 64, false><<<class_grid<T, 64>(max_bands, n_cu), 64, 0, s>>>(
+        band_start, n_bands, src, out, bpre, nullptr, nullptr, nullptr, nullptr, nullptr, band_pr);
+    k_sort_bands<W, 256, true><<<class_grid<T, 256>((max_bands + kChunk - 1) / kChunk, n_cu), 256, 0, s>>>(
+        band_start, n_bands, src, out, bpre, long_list, ticket, big_list, big_excl, big_count, band_pr);
+    return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
-                             int32_t* big_list, int32_t* big_excl, int32_t* big_count, ScanCtl ctl, int n_cu,
-                             hipStream_t s, const int2* band_pr, int64_t n_entries) {
+                             int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
+                             uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr) {
     if (max_bands <= 0) return hipSuccess;
     const Unpacked out{skey, sidx, winding};
     if (band_pr) {
         if (sidx) return hipErrorInvalidValue;  // narrow words hold no index
-        // eight workgroups fit in one CU's LDS; with short bands (a few hundred entries on
-        // average) four per CU finish sooner (measured on 98-entry paths vs 9.7k-entry paths)
-        const int per_cu = n_entries >= 256 * max_bands ? 8 : 4;
-        const int64_t grid = max_bands < (int64_t)n_cu * per_cu ? max_bands : (int64_t)n_cu * per_cu;
-        k_sort_bands<Narrow><<<(unsigned)grid, kThreads, 0, s>>>(band_start, n_bands,
-                                                                  reinterpret_cast<const uint32_t*>(src), out,
-                                                                  big_list, big_excl, big_count, ctl.status,
-                                                                  ctl.counter, band_pr);
-        return hipGetLastError();
+        return launch_classes<Narrow>(band_start, n_bands, max_bands, reinterpret_cast<const uint32_t*>(src), out,
+                                      bpre, ticket, big_list, big_excl, big_count, n_cu, s, band_pr);
     }
-    // four workgroups fit in one CU's LDS
-    const int64_t grid = max_bands < (int64_t)n_cu * 4 ? max_bands : (int64_t)n_cu * 4;
-    k_sort_bands<Wide><<<(unsigned)grid, kThreads, 0, s>>>(band_start, n_bands, src, out, big_list, big_excl,
-                                                            big_count, ctl.status, ctl.counter, nullptr);
-    return hipGetLastError();
+    return launch_classes<Wide>(band_start, n_bands, max_bands, src, out, bpre, ticket, big_list, big_excl,
+                                big_count, n_cu, s, nullptr);
 }
 
 hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_excl,
@@ -388,12 +426,12 @@ hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_l
     const Unpacked out{skey, sidx, winding};
     if (band_pr) {
         if (sidx) return hipErrorInvalidValue;
-        k_sort_big_bands<Narrow><<<64, kThreads, 0, s>>>(band_start, big_list, big_excl, big_count,
+        k_sort_big_bands<Narrow><<<64, kBigThreads, 0, s>>>(band_start, big_list, big_excl, big_count,
                                                           reinterpret_cast<uint32_t*>(scatter),
                                                           reinterpret_cast<uint32_t*>(tmp), out, band_pr);
         return hipGetLastError();
     }
-    k_sort_big_bands<Wide><<<64, kThreads, 0, s>>>(band_start, big_list, big_excl, big_count, scatter, tmp, out,
+    k_sort_big_bands<Wide><<<64, kBigThreads, 0, s>>>(band_start, big_list, big_excl, big_count, scatter, tmp, out,
                                                     nullptr);
     return hipGetLastError();
 }

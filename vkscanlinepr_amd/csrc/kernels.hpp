@@ -130,10 +130,11 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
 // export no stage and whose canvases are narrower than kNarrowMaxW.
 constexpr uint32_t kNarrowX0 = 0x7FFDu;   // x field of the leftmost key (x = -2)
 constexpr int kNarrowMaxW = (1 << 15) - 4;
+// bsum (zeroed, one int32 per band) receives each band's sum of winding changes.
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
                                  const int32_t* bucket_base, const int2* prow, const int32_t* flag_scan,
-                                 const int32_t* band_start, int32_t* fill, unsigned long long* out, int32_t* err,
-                                 hipStream_t s, const int32_t* n_dev = nullptr, bool narrow = false);
+                                 const int32_t* band_start, int32_t* fill, int32_t* bsum, unsigned long long* out,
+                                 int32_t* err, hipStream_t s, const int32_t* n_dev = nullptr, bool narrow = false);
 hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
                              hipStream_t s);
 // band_pr (or null): each band's path rows (prow of its path), for the narrow sort.
@@ -142,15 +143,16 @@ hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32
                             const int32_t* n_dev = nullptr, const int32_t* bucket_base = nullptr, uint32_t Np = 0,
                             const int2* prow = nullptr, int2* band_pr = nullptr);
 // The band sorts write the sorted keys, winding changes and (optionally) indices directly.
-// Sorts every band in array order and writes keys, (debug) indices and winding numbers (the
-// exclusive scan of the sorted winding changes, total at winding[n]); oversized bands are
-// queued (with their exclusive prefix) for launch_sort_big_bands.  ctl: >= max_bands statuses.
-// narrow (band_pr non-null): 32-bit words from the narrow scatter; sidx must be null.  n_entries
-// sizes the persistent grid (more workgroups when the bands are long on average).
+// Sorts every band and writes keys, (debug) indices and winding numbers (the exclusive scan of
+// the sorted winding changes, total at winding[n]); bpre holds each band's exclusive prefix of
+// the winding changes (total at bpre[n_bands]).  Oversized bands are queued (with their
+// exclusive prefix) for launch_sort_big_bands.  big_list: 3 * max_bands int32.  ticket: two
+// zeroed u32 counters.  narrow (band_pr non-null): 32-bit words from the narrow scatter; sidx
+// must be null.
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
-                             int32_t* big_list, int32_t* big_excl, int32_t* big_count, ScanCtl ctl, int n_cu,
-                             hipStream_t s, const int2* band_pr = nullptr, int64_t n_entries = 0);
+                             int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
+                             uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr = nullptr);
 hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_excl,
                                  const int32_t* big_count, unsigned long long* scatter, unsigned long long* tmp,
                                  int32_t* skey, int32_t* sidx, int32_t* winding, hipStream_t s,

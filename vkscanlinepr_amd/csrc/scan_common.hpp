@@ -40,8 +40,8 @@ __device__ inline T wave_sum(T v) {
 }
 
 // Exclusive block scan of one value per thread; returns the exclusive prefix and the block
-// total.  `lds` holds kWaves values.
-template <class T>
+// total.  `lds` holds NW values (NW: the block's waves).
+template <class T, int NW = kWaves>
 __device__ inline T block_exclusive_add(T v, T* lds, T& total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     T inc = wave_inclusive_add(v);
@@ -49,7 +49,7 @@ __device__ inline T block_exclusive_add(T v, T* lds, T& total) {
     __syncthreads();
     T wave_off = T(0), tot = T(0);
 #pragma unroll
-    for (int k = 0; k < kWaves; ++k) {
+    for (int k = 0; k < NW; ++k) {
         T x = lds[k];
         if (k < wave) wave_off += x;
         tot += x;

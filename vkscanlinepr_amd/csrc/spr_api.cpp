@@ -132,7 +132,7 @@ struct spr_handle {
     spr::MlpDevice mlpdev;
     DBuf<uint32_t> pcurve;                          // first curve of each path, Np+1
     DBuf<float> feat, pred, train_feat, train_tgt;
-    DBuf<int32_t> arena, capn, nexact, pbase, sstart, hist_l, fill_l, cend, pbound;
+    DBuf<int32_t> arena, capn, nexact, pbase, sstart, hist_l, fill_l, bsum_l, cend, pbound;
     DBuf<int2> brec_l;
     int64_t lb_n = 0, lb_r = 0;                     // learned frame budgets
     bool lb_dirty = true;                           // scene, MVP or model changed since they were set
@@ -172,7 +172,7 @@ struct ZeroLayout {
     size_t a_words = 0, total = 0;
     size_t ctr_a = 0, st_count = 0, st_pbase = 0;
     size_t ctr = 0, totals = 0, misc = 0, chainctl = 0, st_hist = 0, st_band = 0, st_wind = 0, st_mark = 0;
-    size_t hist = 0, fill = 0, brec = 0;
+    size_t hist = 0, fill = 0, bsum = 0, brec = 0;
 };
 
 ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
@@ -188,10 +188,11 @@ ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
     z.chainctl = o; o += (spr::kCcWords + 1) / 2;  // chain classes, offsets, counters (int32)
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
     z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
-    z.st_wind = o; o += (size_t)Np + 3 * (size_t)(N / 1024) + 3;  // band statuses of the sort
+    z.st_wind = o; o += (size_t)Np + 3 * (size_t)(N / 1024) + 3;  // statuses of the band-sum scan
     z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
     z.hist = o; o += (size_t)(NB + 2 + 1) / 2;  // NB+1 int32 (scanned in place)
     z.fill = o; o += (size_t)(NB + 1) / 2;      // NB int32
+    z.bsum = o; o += ((size_t)Np + 3 * (size_t)(N / 1024) + 4) / 2 + 1;  // max_bands + 1 int32
     z.brec = o; o += (size_t)NB;                // NB int2
     z.total = o;
     return z;
@@ -327,7 +328,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->bflag.ensure(nb1 + 1);
     H->band_start.ensure((size_t)max_bands + 1);
     H->band_pr.ensure((size_t)max_bands + 1);
-    H->big_list.ensure(2 * (size_t)max_bands);  // band ids, then their winding prefixes
+    H->big_list.ensure(3 * (size_t)max_bands);  // oversized band ids, their winding prefixes, long bands
     if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
     H->crec.ensure(2 * n1);
     H->image.ensure((size_t)4 * H->w * H->h);
@@ -337,6 +338,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     int32_t* d_bigcount = d_err + 1;
     int32_t* d_hist = reinterpret_cast<int32_t*>(H->zero.p + z.hist);
     int32_t* d_fill = reinterpret_cast<int32_t*>(H->zero.p + z.fill);
+    int32_t* d_bsum = reinterpret_cast<int32_t*>(H->zero.p + z.bsum);
     int2* d_brec = reinterpret_cast<int2*>(H->zero.p + z.brec);
 
     const bool gen = !H->inj && N > 0;
@@ -385,8 +387,10 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         SPR_HIP(spr::launch_band_emit(H->bflag.p, d_hist, (int32_t)NB, N, H->band_start.p, s, nullptr, nullptr,
                                       H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr));
         SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, N, H->bucket_base.p, H->prow.p,
-                                           H->bflag.p, H->band_start.p, d_fill, H->sbig.p, d_err, s, nullptr,
+                                           H->bflag.p, H->band_start.p, d_fill, d_bsum, H->sbig.p, d_err, s, nullptr,
                                            narrow));
+        // each band's exclusive prefix of the winding changes (total at [n_bands])
+        SPR_HIP(spr::launch_scan_i32(d_bsum, d_bsum, max_bands, ctl_at(H, z.st_wind, z.ctr, 2), s, H->bflag.p + NB));
     }
     stage_mark(H, 8);
     if (N > 0) {
@@ -394,8 +398,9 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         // changes, scanline_rasterizer.cpp:478-506)
         SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
                                        debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
-                                       H->big_list.p + max_bands, d_bigcount, ctl_at(H, z.st_wind, z.ctr, 2), H->n_cu,
-                                       s, narrow ? H->band_pr.p : nullptr, N));
+                                       H->big_list.p + max_bands, d_bigcount, d_bsum,
+                                       ctl_at(H, z.st_wind, z.ctr, 4).counter, H->n_cu, s,
+                                       narrow ? H->band_pr.p : nullptr));
         SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                            H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr,
                                            H->winding.p, s, narrow ? H->band_pr.p : nullptr));
@@ -448,7 +453,7 @@ LearnedLayout learned_layout(uint32_t Np, int64_t nb_budget, int64_t n_budget) {
     z.st_pb = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
     z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
-    z.st_wind = o; o += (size_t)Np + 3 * (size_t)(std::max<int64_t>(n_budget, 1) / 1024) + 3;  // band statuses
+    z.st_wind = o; o += (size_t)Np + 3 * (size_t)(std::max<int64_t>(n_budget, 1) / 1024) + 3;  // band-sum scan
     z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(n_budget, 1));
     z.total = o;
     return z;
@@ -561,17 +566,19 @@ void run_learned_frame(spr_handle* H) {
     H->bflag.ensure((size_t)nb_budget + 1);
     H->band_start.ensure((size_t)max_bands + 1);
     H->band_pr.ensure((size_t)max_bands + 1);
-    H->big_list.ensure(2 * (size_t)max_bands);  // band ids, then their winding prefixes
+    H->big_list.ensure(3 * (size_t)max_bands);  // oversized band ids, their winding prefixes, long bands
     if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); H->records.ensure(nr); }
     H->crec.ensure(nr);
     H->image.ensure(H->batch ? H->img_bytes : (size_t)4 * H->w * H->h);
     H->hist_l.ensure((size_t)nb_budget + 1);
     H->fill_l.ensure((size_t)nb_budget + 1);
+    H->bsum_l.ensure((size_t)max_bands + 1);
     H->brec_l.ensure((size_t)nb_budget + 1);
     // per-bucket tables: cleared up to the live bucket count (bucket_base[Np])
     const int32_t* d_nb = H->bucket_base.p + Np;
     SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->hist_l.p, nb_budget + 1, d_nb, 1, 1, 0u, s));
     SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->fill_l.p, nb_budget, d_nb, 1, 0, 0u, s));
+    SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->bsum_l.p, max_bands, d_nb, 1, 0, 0u, s));  // bands <= buckets
     SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->brec_l.p, 2 * nb_budget, d_nb, 2, 0, 0u, s));
 
     // phase 2: exact counts placed in the arenas, then the plain stages on device counts
@@ -621,13 +628,16 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_band_emit(H->bflag.p, H->hist_l.p, (int32_t)nb_budget, nl, H->band_start.p, s, d_nb,
                                   d_nsorted, H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr));
     SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, nl, H->bucket_base.p, H->prow.p,
-                                       H->bflag.p, H->band_start.p, H->fill_l.p, H->sbig.p, d_err, s, d_live,
-                                       narrow));
+                                       H->bflag.p, H->band_start.p, H->fill_l.p, H->bsum_l.p, H->sbig.p, d_err, s,
+                                       d_live, narrow));
+    SPR_HIP(spr::launch_scan_i32(H->bsum_l.p, H->bsum_l.p, max_bands, ctl_at(H, z.st_wind, z.counters, 4), s,
+                                 d_nbands));
     stage_mark(H, 8);
     SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
                                    debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
-                                   H->big_list.p + max_bands, d_bigcount, ctl_at(H, z.st_wind, z.counters, 4),
-                                   H->n_cu, s, narrow ? H->band_pr.p : nullptr, nl));
+                                   H->big_list.p + max_bands, d_bigcount, H->bsum_l.p,
+                                   ctl_at(H, z.st_wind, z.counters, 6).counter, H->n_cu, s,
+                                   narrow ? H->band_pr.p : nullptr));
     SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                        H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr, H->winding.p,
                                        s, narrow ? H->band_pr.p : nullptr));
@@ -897,7 +907,7 @@ int spr_destroy(spr_handle* H) {
     H->mlpdev.release();
     H->pcurve.release(); H->feat.release(); H->pred.release(); H->train_feat.release(); H->train_tgt.release();
     H->arena.release(); H->capn.release(); H->nexact.release(); H->pbase.release(); H->sstart.release();
-    H->hist_l.release(); H->fill_l.release(); H->brec_l.release(); H->cend.release(); H->pbound.release();
+    H->hist_l.release(); H->fill_l.release(); H->bsum_l.release(); H->brec_l.release(); H->cend.release(); H->pbound.release();
     if (H->h_pinned) (void)hipHostFree(H->h_pinned);
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
