@@ -151,8 +151,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
         }
     }
     pieces[gid] = pd;
-    chains[2 * gid] = cd[0];
-    chains[2 * gid + 1] = cd[1];
+    // an empty chain slot is never read (the solve takes chains by their class bytes)
+    if (cd[0].n) chains[2 * gid] = cd[0];
+    if (cd[1].n) chains[2 * gid + 1] = cd[1];
     // class of each chain slot (0xFF: no chain) for the ordering pass
     const uint32_t k0 = cd[0].n ? (uint32_t)chain_class(cd[0].n, cd[0].cubic != 0) : 0xFFu;
     const uint32_t k1 = cd[1].n ? (uint32_t)chain_class(cd[1].n, cd[1].cubic != 0) : 0xFFu;
