@@ -7,8 +7,8 @@
 // two chains (ties to x, :338) followed by its equal-t OR-merge (:361-375).  This file keeps
 // every float operation of the reference and only changes who computes what:
 //
-//  1. k_chain_prep: one thread per (curve, piece) walks the piece geometry and writes one
-//     descriptor per non-empty chain and per piece; chains are counted in length classes
+//  1. k_chain_prep: one thread per curve walks its pieces and writes one descriptor per piece
+//     slot and per non-empty chain; chains are counted in length classes
 //     (quarter octaves for cubic chains, so that the 64 chains of one wave have similar lengths).
 //  2. k_chain_order: chains are laid out costliest class first (longest-processing-time order).
 //  3. k_chain_solve: a persistent kernel.  Waves first take the long cubic chains, one chain per
@@ -65,35 +65,11 @@ __device__ inline int chain_class(int n, bool cubic) {
 // Cost octave of a class (a line crossing costs about 2^-6 of a cubic one).
 __device__ inline int class_cost_octave(int cls) { return cls < 64 ? cls >> 2 : cls - 64 - 6; }
 
-// Walks the monotone pieces of a visible curve up to `piece`: flagged end parameters, grid and
-// first entry offset inside the curve's output range.
-struct PieceCtx {
-    float t0, t1;
-    PieceGrid G;
-    int off;
-};
-
-__device__ inline bool setup_piece(const CurvePts& cv, const float q[4], uint32_t n, int piece,
-                                   int w, int h, PieceCtx& pc) {
-    if ((uint32_t)piece > n) return false;
-    float t0 = 0.0f, p0x = cv.x[0], p0y = cv.y[0];
-    int off = 0;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const float traw = (uint32_t)k < n ? q[k] : 1.0f;
-        const float p1x = cv.ex(traw), p1y = cv.ey(traw);
-        const float t1 = flag_piece_end(traw, p1x);
-        const PieceGrid G = piece_grid(p0x, p0y, p1x, p1y, w, h);
-        if (k == piece) {
-            pc.t0 = t0; pc.t1 = t1; pc.G = G; pc.off = off;
-            return true;
-        }
-        off += 1 + G.nx + G.ny;
-        t0 = t1; p0x = p1x; p0y = p1y;
-    }
-    return false;
-}
-
+// One thread per curve walks its monotone pieces once, in order: piece k runs from the end of
+// piece k-1 (flagged end parameter and end point) to its own end, and its entries start after
+// the 1 + nx + ny entries of each earlier piece.  Each of the curve's 5 piece slots gets a
+// descriptor (base = -1: no piece), each non-empty chain a descriptor and every chain slot its
+// class byte (0xFF: no chain).
 __global__ __launch_bounds__(kBlock) void k_chain_prep(
     const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
     const uint32_t* __restrict__ ctype, const uint32_t* __restrict__ cpath,
@@ -104,60 +80,71 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
     __shared__ int32_t hist[kClasses];
     if (threadIdx.x < kClasses) hist[threadIdx.x] = 0;
     __syncthreads();
-    const int w0 = w, h0 = h;
-    // grid-stride over (curve, piece) slots; the class histogram is flushed once per workgroup
-    for (uint32_t gid = blockIdx.x * kBlock + threadIdx.x; gid < 5u * C; gid += gridDim.x * kBlock) {
-    w = w0; h = h0;
-    const uint32_t c = gid / 5u;
-    const int piece = (int)(gid % 5u);
-    PieceDesc pd;
-    pd.base = -1;
-    ChainDesc cd[2];
-    cd[0].n = cd[1].n = 0;
-    if (c < C && visible_mask(visible[cpath[c]]) && (uint32_t)piece <= ncuts[c]) {
+    // grid-stride over curves; the class histogram is flushed once per workgroup
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < C; c += gridDim.x * kBlock) {
+        const uint32_t path = cpath[c];
+        const uint32_t nc = visible_mask(visible[path]) ? ncuts[c] : 0xFFFFFFFFu;  // all slots empty
+        int cw = w, ch = h;
+        if (pcanvas && nc != 0xFFFFFFFFu) { const int2 cvs = pcanvas[path]; cw = cvs.x; ch = cvs.y; }
         const uint32_t type = ctype[c];
-        const CurvePts cv = load_curve(tpos, cpos[c], type);
-        const float4 qq = cuts[c];
-        const float q[4] = {qq.x, qq.y, qq.z, qq.w};
-        PieceCtx pc;
-        if (pcanvas) { const int2 cvs = pcanvas[cpath[c]]; w = cvs.x; h = cvs.y; }
-        if (setup_piece(cv, q, ncuts[c], piece, w, h, pc)) {
-            const int base = offsets[c] + pc.off;
+        CurvePts cv;
+        float q[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+        int base0 = 0, limit = 0;
+        if (nc != 0xFFFFFFFFu) {
+            cv = load_curve(tpos, cpos[c], type);
+            const float4 qq = cuts[c];
+            q[0] = qq.x; q[1] = qq.y; q[2] = qq.z; q[3] = qq.w;
+            base0 = offsets[c];
             // the curve's range ends at the next curve's offset, or at ends[c] when curves are
             // placed in per-path arenas
-            const int limit = ends ? ends[c] : offsets[c + 1];
-            if (base + pc.G.nx + pc.G.ny + 1 <= limit) {  // count and write passes agree
-                pd.base = base; pd.nx = pc.G.nx; pd.ny = pc.G.ny; pd.curve = (int32_t)c; pd.t0 = pc.t0;
-                // the last piece's last segment runs to t = 1; the others run to the next
-                // piece's first entry
-                pd.tend = (uint32_t)piece == ncuts[c] ? 1.0f : -1.0f;
-#pragma unroll
-                for (int side = 0; side < 2; ++side) {
-                    const int n = side ? pc.G.ny : pc.G.nx;
-                    ChainDesc& d = cd[side];
-                    d.n = n;
-                    if (n == 0) continue;
-                    d.curve = (int32_t)c;
-                    d.side = side;
-                    d.out = base + 1 + (side ? pc.G.nx : 0);
-                    d.line0 = side ? pc.G.y0 : pc.G.x0;
-                    d.step = side ? pc.G.dy : pc.G.dx;
-                    d.t0 = pc.t0;
-                    d.t1 = pc.t1;
-                    d.cubic = type == kCubic;
-                    atomicAdd(&hist[chain_class(n, d.cubic)], 1);
-                }
-            }
+            limit = ends ? ends[c] : offsets[c + 1];
         }
-    }
-    pieces[gid] = pd;
-    // an empty chain slot is never read (the solve takes chains by their class bytes)
-    if (cd[0].n) chains[2 * gid] = cd[0];
-    if (cd[1].n) chains[2 * gid + 1] = cd[1];
-    // class of each chain slot (0xFF: no chain) for the ordering pass
-    const uint32_t k0 = cd[0].n ? (uint32_t)chain_class(cd[0].n, cd[0].cubic != 0) : 0xFFu;
-    const uint32_t k1 = cd[1].n ? (uint32_t)chain_class(cd[1].n, cd[1].cubic != 0) : 0xFFu;
-    reinterpret_cast<uint16_t*>(chain_cls)[gid] = (uint16_t)(k0 | (k1 << 8));
+        float t0 = 0.0f, p0x = 0.0f, p0y = 0.0f;
+        if (nc != 0xFFFFFFFFu) { p0x = cv.x[0]; p0y = cv.y[0]; }
+        int off = 0;
+        uint32_t cls = 0;  // the slots' class bytes, two per slot
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            PieceDesc pd;
+            pd.base = -1; pd.nx = pd.ny = pd.curve = 0; pd.t0 = pd.tend = 0.0f;
+            uint32_t k0 = 0xFFu, k1 = 0xFFu;
+            if (nc != 0xFFFFFFFFu && (uint32_t)k <= nc) {
+                const float traw = (uint32_t)k < nc ? q[k] : 1.0f;
+                const float p1x = cv.ex(traw), p1y = cv.ey(traw);
+                const float t1 = flag_piece_end(traw, p1x);
+                const PieceGrid G = piece_grid(p0x, p0y, p1x, p1y, cw, ch);
+                const int base = base0 + off;
+                if (base + G.nx + G.ny + 1 <= limit) {  // count and write passes agree
+                    pd.base = base; pd.nx = G.nx; pd.ny = G.ny; pd.curve = (int32_t)c; pd.t0 = t0;
+                    // the last piece's last segment runs to t = 1; the others run to the next
+                    // piece's first entry
+                    pd.tend = (uint32_t)k == nc ? 1.0f : -1.0f;
+#pragma unroll
+                    for (int side = 0; side < 2; ++side) {
+                        const int n = side ? G.ny : G.nx;
+                        if (n == 0) continue;
+                        ChainDesc d;
+                        d.n = n;
+                        d.curve = (int32_t)c;
+                        d.side = side;
+                        d.out = base + 1 + (side ? G.nx : 0);
+                        d.line0 = side ? G.y0 : G.x0;
+                        d.step = side ? G.dy : G.dx;
+                        d.t0 = t0;
+                        d.t1 = t1;
+                        d.cubic = type == kCubic;
+                        chains[2 * (5 * (int64_t)c + k) + side] = d;
+                        const uint32_t kc = (uint32_t)chain_class(n, d.cubic != 0);
+                        atomicAdd(&hist[kc], 1);
+                        if (side) k1 = kc; else k0 = kc;
+                    }
+                }
+                off += 1 + G.nx + G.ny;
+                t0 = t1; p0x = p1x; p0y = p1y;
+            }
+            pieces[5 * (int64_t)c + k] = pd;
+            reinterpret_cast<uint16_t*>(chain_cls)[5 * (int64_t)c + k] = (uint16_t)(k0 | (k1 << 8));
+        }
     }
     __syncthreads();
     if (threadIdx.x < kClasses && hist[threadIdx.x]) atomicAdd(&class_count[threadIdx.x], hist[threadIdx.x]);
@@ -854,7 +841,7 @@ hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uin
                              ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
                              int n_cu, hipStream_t s, const int32_t* ends, const int2* pcanvas) {
     if (C == 0) return hipSuccess;
-    unsigned g = grid_for((int64_t)C * 5);
+    unsigned g = grid_for((int64_t)C);
     if (g > (unsigned)n_cu * 8) g = (unsigned)n_cu * 8;
     k_chain_prep<<<g, kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts, ncuts, offsets, C, w, h, chains,
                                       pieces, chain_cls, class_count, ends, pcanvas);
