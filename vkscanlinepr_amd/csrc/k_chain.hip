@@ -156,6 +156,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
 // come first: the wave-solved chains.
 __global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t* __restrict__ class_off,
                                 int32_t* __restrict__ n_long, int32_t* __restrict__ light_begin) {
+    __shared__ int32_t cnt[kClasses];
+    for (int c = threadIdx.x; c < kClasses; c += blockDim.x) cnt[c] = class_count[c];  // loads at once
+    __syncthreads();
     if (threadIdx.x != 0) return;
     int32_t off = 0, nl = 0, lb = 0;
     for (int key = 2 * (63 + 24) + 1; key >= 0; --key) {
@@ -164,7 +167,7 @@ __global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t
         else if ((key & 7) == 0 && key / 8 <= 15) c = 64 + key / 8;
         if (c < 0) continue;
         class_off[c] = off;
-        off += class_count[c];
+        off += cnt[c];
         if (c < 64 && c >= 4 * kLongClass) nl = off;
         if (c < 64 && c >= kHeavyClass) lb = off;  // heavy classes come first, light ones after
     }

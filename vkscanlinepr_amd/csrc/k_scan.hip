@@ -115,7 +115,9 @@ struct PathCursor {
 __device__ inline int key_x(int32_t k) { return (k & 0xFFFF) - 0x7FFF; }
 __device__ inline int key_y(int32_t k) { return ((k >> 16) & 0xFFFF) - 0x7FFF; }
 
-__global__ __launch_bounds__(kScanThreads) void k_mark_emit(
+constexpr int kMarkThreads = 512, kMarkTile = kMarkThreads * kScanItems;
+
+__global__ __launch_bounds__(kMarkThreads) void k_mark_emit(
     const int32_t* __restrict__ skey, const int32_t* __restrict__ winding,
     const uint32_t* __restrict__ epath, const uint32_t* __restrict__ rule,
     const uint32_t* __restrict__ fill, int64_t N_max, const int32_t* __restrict__ n_dev,
@@ -126,12 +128,12 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     const int32_t* __restrict__ bucket_base, const int2* __restrict__ prow, int2* __restrict__ brec,
     int32_t* __restrict__ err, const int2* __restrict__ pcanvas, const int32_t* __restrict__ wbase) {
     __shared__ int32_t s_tile;
-    __shared__ unsigned long long s_w[kWaves];
+    __shared__ unsigned long long s_w[kMarkThreads / 64];
     __shared__ unsigned long long s_prefix;
-    __shared__ uint32_t s_rec[2 * kScanTile];  // the tile's compact records, stored coalesced
+    __shared__ uint32_t s_rec[2 * kMarkTile];  // the tile's compact records, stored coalesced
     const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
     const int64_t tile = ordered_tile_id(counter, &s_tile);
-    const int64_t base = tile * (int64_t)kScanTile;
+    const int64_t base = tile * (int64_t)kMarkTile;
     if (base >= N) return;
     const int64_t i0 = base + (int64_t)threadIdx.x * kScanItems;
     // Path of sorted element f: epath[f] when sorted segments coincide with the intersection
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
     }
     const unsigned long long mine = (unsigned long long)__popc(cellbits) | ((unsigned long long)__popc(spanbits) << 32);
     unsigned long long total;
-    const unsigned long long texcl = block_exclusive_add(mine, s_w, total);
+    const unsigned long long texcl = block_exclusive_add<unsigned long long, kMarkThreads / 64>(mine, s_w, total);
     if (threadIdx.x < 64) {
         const unsigned long long pre = lookback<OpPair>(status, tile, total);
         if (threadIdx.x == 0) s_prefix = pre;
@@ -286,9 +288,9 @@ __global__ __launch_bounds__(kScanThreads) void k_mark_emit(
         }
     }
     __syncthreads();
-    for (int64_t i = threadIdx.x; i < tile_recs; i += kScanThreads)
+    for (int64_t i = threadIdx.x; i < tile_recs; i += kMarkThreads)
         if (tile_rec0 + i < rec_cap) crec[tile_rec0 + i] = s_rec[i];
-    if (threadIdx.x == 0 && base <= N - 1 && N - 1 < base + kScanTile) {
+    if (threadIdx.x == 0 && base <= N - 1 && N - 1 < base + kMarkTile) {
         const unsigned long long t = s_prefix + total;
         totals[0] = (int32_t)(t & 0xFFFFFFFFull);
         totals[1] = (int32_t)(t >> 32);
@@ -320,7 +322,7 @@ hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const u
                             const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
                             int32_t* err, const int2* pcanvas, const int32_t* wbase) {
     if (N == 0) return hipSuccess;
-    k_mark_emit<<<(unsigned)scan_tiles(N), kScanThreads, 0, s>>>(skey, winding, epath, rule, fill, N, n_dev,
+    k_mark_emit<<<(unsigned)((N + kMarkTile - 1) / kMarkTile), kMarkThreads, 0, s>>>(skey, winding, epath, rule, fill, N, n_dev,
                                                                  sstart, Np, rec_cap, w, h, ctl.status,
                                                                  ctl.counter, records, crec, flags, totals,
                                                                  bucket_base, prow, brec, err, pcanvas, wbase);
