@@ -115,6 +115,9 @@ struct PathCursor {
 __device__ inline int key_x(int32_t k) { return (k & 0xFFFF) - 0x7FFF; }
 __device__ inline int key_y(int32_t k) { return ((k >> 16) & 0xFFFF) - 0x7FFF; }
 
+// mark + emit tiles: 512 threads x 16 elements.  The tiles' look-back is one chain, so larger
+// tiles (half as many as 4096-element ones) finish sooner; the 64 KiB record staging keeps two
+// blocks (16 waves) per CU.
 constexpr int kMarkThreads = 512, kMarkTile = kMarkThreads * kScanItems;
 
 __global__ __launch_bounds__(kMarkThreads) void k_mark_emit(
