@@ -13,6 +13,7 @@
 // each cell row without any further sorting.
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include "scan_common.hpp"
 
 namespace spr {
 
@@ -106,12 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
     for (int r = 0; r < kRounds; ++r) {
         vm[r] = __ballot(valid[r]);
         hd[r] = run_heads(b[r], valid[r], vm[r]);
-        int32_t inc = wv[r];
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int32_t u = __shfl_up(inc, d);
-            if (lane >= d) inc += u;
-        }
+        const int32_t inc = wave_inclusive_add(wv[r]);
         const bool head = (hd[r] >> lane) & 1ull;
         const int len = head ? run_length(lane, hd[r], vm[r]) : 1;
         const int32_t at_end = __shfl(inc, lane + len - 1);

@@ -32,6 +32,18 @@ __device__ inline T wave_inclusive_add(T v) {
     return v;
 }
 
+// int32: the same scan with DPP row shifts (1, 2, 4, 8 inside each 16-lane row) and the row
+// broadcasts of lanes 15 and 31, all in vector registers.  A lane without a source adds 0.
+__device__ inline int32_t wave_inclusive_add(int32_t v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31 into rows 2, 3
+    return v;
+}
+
 template <class T>
 __device__ inline T wave_sum(T v) {
 #pragma unroll
