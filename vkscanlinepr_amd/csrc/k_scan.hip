@@ -30,6 +30,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, in
     __shared__ int32_t s_w[kWaves];
     __shared__ int32_t s_prefix;
     const int64_t n = n_dev ? min((int64_t)*n_dev, n_max) : n_max;
+    // a launch sized for a budget: the blocks past the live tiles leave without a ticket, so the
+    // live tiles' tickets go to exactly the first blocks (no atomics spent on the surplus)
+    if ((int64_t)blockIdx.x * (ITEMS * kScanThreads) >= (n > 0 ? n : 1)) return;
     const int64_t tile = ordered_tile_id(counter, &s_tile);
     const int64_t base = tile * (int64_t)(ITEMS * kScanThreads);
     if (base >= n && tile != 0) return;
@@ -135,6 +138,7 @@ __global__ __launch_bounds__(kMarkThreads) void k_mark_emit(
     __shared__ unsigned long long s_prefix;
     __shared__ uint32_t s_rec[2 * kMarkTile];  // the tile's compact records, stored coalesced
     const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
+    if ((int64_t)blockIdx.x * kMarkTile >= N) return;  // past the live tiles: no ticket taken
     const int64_t tile = ordered_tile_id(counter, &s_tile);
     const int64_t base = tile * (int64_t)kMarkTile;
     if (base >= N) return;
