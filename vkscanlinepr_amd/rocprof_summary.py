@@ -26,8 +26,9 @@ STAGE_KERNELS = {
     "piece_fragments": ["k_piece_small", "k_piece_fragments", "k_piece_heavy"],
     "gen_fragment": ["k_gen_fragment"],
     "bucket_scatter": ["k_bucket_scatter", "k_band_flags", "k_band_emit"],
-    "band_sort": ["k_sort_bands", "k_sort_big_bands"],
-    "sort": ["k_bucket_scatter", "k_band_flags", "k_band_emit", "k_sort_bands", "k_sort_big_bands"],
+    "band_sort": ["k_band_classify", "k_sort_bands", "k_sort_big_bands"],
+    "sort": ["k_bucket_scatter", "k_band_flags", "k_band_emit", "k_band_classify", "k_sort_bands",
+             "k_sort_big_bands"],
     "mark_emit": ["k_mark_emit"],
     "composite": ["k_composite"],
 }
