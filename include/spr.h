@@ -81,11 +81,15 @@ enum {
     SPR_STAGE_KEY = 5,         /* int32 x N: cell keys in intersection order */
     SPR_STAGE_WN = 6,          /* int32 x N: winding change in intersection order */
     SPR_STAGE_SEG = 7,         /* int32 x (Np+1): per-path segment starts */
-    SPR_STAGE_SKEY = 8,        /* int32 x N: keys sorted inside each path segment */
+    SPR_STAGE_SKEY = 8,        /* int32 x N: keys sorted inside each path segment (debug; without
+                                  it, canvases up to 32764 wide fuse mark + emit into the sort and
+                                  keep no sorted keys) */
     SPR_STAGE_SIDX = 9,        /* int32 x N: sorted intersection indices (debug) */
-    SPR_STAGE_WINDING = 10,    /* int32 x N: exclusive-scanned winding numbers, sorted order;
-                                  without debug, entries of equal key are in no set order, so
-                                  only each run's first number is the reference's */
+    SPR_STAGE_WINDING = 10,    /* int32 x N: exclusive-scanned winding numbers, sorted order
+                                  (debug; as SPR_STAGE_SKEY: a frame without debug on a canvas
+                                  wider than 32764 keeps them, but entries of equal key are then
+                                  in no set order, so only each run's first number is the
+                                  reference's) */
     SPR_STAGE_FLAGS = 11,      /* int32 x 2N: cell flags, then span flags (debug) */
     SPR_STAGE_RECORDS = 12,    /* int4 x R: (yx, width, fill, frag_index) output records (debug) */
     SPR_STAGE_IMAGE = 13,      /* uint8 x 4WH: RGBA8 image, top row first */

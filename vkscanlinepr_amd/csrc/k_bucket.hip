@@ -188,7 +188,7 @@ __global__ void k_band_emit(const int32_t* __restrict__ flag_scan, const int32_t
                             int32_t NB_max, int64_t N_max, int32_t* __restrict__ band_start,
                             const int32_t* __restrict__ nb_dev, const int32_t* __restrict__ n_dev,
                             const int32_t* __restrict__ bucket_base, uint32_t Np, const int2* __restrict__ prow,
-                            int2* __restrict__ band_pr) {
+                            int2* __restrict__ band_pr, int32_t* __restrict__ band_path) {
     const int32_t NB = nb_dev ? min(*nb_dev, NB_max) : NB_max;
     const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
     const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -202,6 +202,7 @@ __global__ void k_band_emit(const int32_t* __restrict__ flag_scan, const int32_t
                 if (bucket_base[mid] <= b) lo = mid; else hi = mid;
             }
             band_pr[flag_scan[b]] = prow[lo];
+            if (band_path) band_path[flag_scan[b]] = (int32_t)lo;
         }
     }
     if (b == NB - 1) band_start[flag_scan[NB]] = (int32_t)N;
@@ -241,10 +242,11 @@ hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int3
 
 hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,
                             int32_t* band_start, hipStream_t s, const int32_t* nb_dev, const int32_t* n_dev,
-                            const int32_t* bucket_base, uint32_t Np, const int2* prow, int2* band_pr) {
+                            const int32_t* bucket_base, uint32_t Np, const int2* prow, int2* band_pr,
+                            int32_t* band_path) {
     if (NB == 0) return hipSuccess;
     k_band_emit<<<grid_for(NB), kBlock, 0, s>>>(flag_scan, boff, NB, N, band_start, nb_dev, n_dev, bucket_base, Np,
-                                                prow, band_pr);
+                                                prow, band_pr, band_path);
     return hipGetLastError();
 }
 

@@ -440,6 +440,23 @@ __global__ void k_doc_winding_base(const int32_t* __restrict__ winding, const in
     if (p < Np) wbase[p] = winding[sstart[doc_first_path[p]]];
 }
 
+// Winding number at each document's first sorted entry from the bands: that position is a band
+// start (a path's first non-empty bucket starts a band) or the end, whose prefix is the total.
+__global__ void k_doc_winding_base_bands(const int32_t* __restrict__ band_start, const int32_t* __restrict__ bpre,
+                                         const int32_t* __restrict__ n_bands, const int32_t* __restrict__ sstart,
+                                         const int32_t* __restrict__ doc_first_path, uint32_t Np,
+                                         int32_t* __restrict__ wbase) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= Np) return;
+    const int32_t pos = sstart[doc_first_path[p]];
+    int32_t lo = 0, hi = *n_bands;  // band_start[lo] <= pos < band_start[hi], band_start[n_bands] = end
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (band_start[mid] <= pos) lo = mid; else hi = mid;
+    }
+    wbase[p] = band_start[lo] == pos ? bpre[lo] : bpre[hi];
+}
+
 inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b < 1 ? 1 : (n + b - 1) / b); }
 
 }  // namespace
@@ -486,6 +503,14 @@ hipError_t launch_doc_winding_base(const int32_t* winding, const int32_t* sstart
                                    uint32_t Np, int32_t* wbase, hipStream_t s) {
     if (Np == 0) return hipSuccess;
     k_doc_winding_base<<<blocks(Np), 256, 0, s>>>(winding, sstart, doc_first_path, Np, wbase);
+    return hipGetLastError();
+}
+
+hipError_t launch_doc_winding_base_bands(const int32_t* band_start, const int32_t* bpre, const int32_t* n_bands,
+                                         const int32_t* sstart, const int32_t* doc_first_path, uint32_t Np,
+                                         int32_t* wbase, hipStream_t s) {
+    if (Np == 0) return hipSuccess;
+    k_doc_winding_base_bands<<<blocks(Np), 256, 0, s>>>(band_start, bpre, n_bands, sstart, doc_first_path, Np, wbase);
     return hipGetLastError();
 }
 

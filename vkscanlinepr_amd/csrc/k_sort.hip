@@ -101,9 +101,11 @@ struct Unpacked {
 struct Wide {
     typedef u64 T;
     static constexpr T kPad = ~0ull;  // no word has 3 in its low two bits
+    int2 pr;
     __device__ void band(const int2*, int64_t) {}
+    __device__ int32_t key(T w) const { return (int32_t)((uint32_t)(w >> 32) ^ 0x80000000u); }
     __device__ void put(const Unpacked& o, int64_t i, T w) const {
-        o.skey[i] = (int32_t)((uint32_t)(w >> 32) ^ 0x80000000u);
+        o.skey[i] = key(w);
         if (o.sidx) o.sidx[i] = (int32_t)((w >> 2) & 0x3FFFFFFFull);
     }
 };
@@ -115,12 +117,13 @@ struct Narrow {
     static constexpr T kPad = ~0u;
     int2 pr;
     __device__ void band(const int2* band_pr, int64_t b) { pr = band_pr[b]; }
-    __device__ void put(const Unpacked& o, int64_t i, T w) const {
+    __device__ int32_t key(T w) const {
         const int loc = (int)(w >> 17);
         const uint32_t x = ((w >> 2) & 0x7FFFu) + kNarrowX0;
         const int y = loc < pr.y ? 2 * (pr.x + loc) : 0;
-        o.skey[i] = loc == pr.y ? (int32_t)kSentinelKey : (int32_t)(((uint32_t)(y + 0x7FFF) << 16) | x);
+        return loc == pr.y ? (int32_t)kSentinelKey : (int32_t)(((uint32_t)(y + 0x7FFF) << 16) | x);
     }
+    __device__ void put(const Unpacked& o, int64_t i, T w) const { o.skey[i] = key(w); }
 };
 template <class T>
 __device__ inline int32_t wn_of(T w) { return (int32_t)(w & 3u) - 1; }
@@ -129,6 +132,105 @@ __device__ inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_
 // LDS slot of element i (one pad word per 8 elements).
 __device__ inline int pad(int i) { return i + (i >> 3); }
 
+__device__ inline int key_x(int32_t k) { return (k & 0xFFFF) - 0x7FFF; }
+__device__ inline int key_y(int32_t k) { return ((k >> 16) & 0xFFFF) - 0x7FFF; }
+
+// What mark + emit needs of a band's path.
+struct BandPath {
+    uint32_t rule;
+    int cw, ch;       // canvas
+    int32_t wb, bb;   // winding base, first bucket
+    int2 pr;          // rows: first, count
+    int64_t r0;       // first record slot: 2 * band start
+};
+
+__device__ inline BandPath band_path_of(const EmitArgs& E, int64_t band, int2 pr, int64_t start) {
+    BandPath b;
+    const int32_t path = E.band_path[band];
+    b.rule = E.rule[path];
+    b.cw = E.w; b.ch = E.h;
+    if (E.pcanvas) { const int2 c = E.pcanvas[path]; b.cw = c.x; b.ch = c.y; }
+    b.wb = E.wbase ? E.wbase[path] : 0;
+    b.bb = E.bucket_base[path];
+    b.pr = pr;
+    b.r0 = 2 * start;
+    return b;
+}
+
+// Mark + emit of band entries [c0, c0 + m) of a band of n entries (mark_merged_fragment_and_
+// span.comp:48-120, gen_merged_fragment_and_span.comp:152-222), THREADS threads with kItems
+// consecutive entries each.  key(j) and wind(j) give entry j's key and winding number (j in
+// [0, n)).  A cell is an in-canvas entry whose key differs from the previous entry's (the
+// band's first entry follows another bucket, so its key differs); a span runs from the previous
+// entry of the same cell row to this one when the winding number fills.  Records go to
+// crec[r0 + carry + ...]; carry (records of earlier chunks) is advanced and the chunk's
+// (cells, spans) are added to cnt (thread 0).
+template <int THREADS, class KEY, class WIND>
+__device__ inline void emit_chunk(const EmitArgs& E, const BandPath& b, KEY key, WIND wind, int64_t c0, int m,
+                                  int64_t n, int32_t& carry, int2& cnt, int32_t* s_w) {
+    const int64_t j0 = c0 + (int64_t)threadIdx.x * kItems;
+    int32_t keys[kItems];
+    uint32_t cbits = 0, sbits = 0;
+    int32_t kprev = j0 > 0 && j0 - 1 < n ? key(j0 - 1) : 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int64_t j = j0 + k;
+        keys[k] = 0;
+        if (j >= c0 + m) continue;
+        const int32_t k1 = key(j);
+        keys[k] = k1;
+        const int x1 = key_x(k1), y1 = key_y(k1);
+        const bool in_canvas = !(x1 < 0 || y1 < 0 || x1 >= b.cw || y1 >= b.ch);
+        const bool cell = in_canvas && (j == 0 || kprev != k1);
+        bool span = false;
+        if (j > 0) {
+            const int32_t wn = (int32_t)((uint32_t)wind(j) - (uint32_t)b.wb);
+            const bool wf = (b.rule == 0u && wn != 0) || (b.rule == 1u && (wn & 1) != 0);
+            span = key_y(kprev) == y1 && key_x(kprev) + 2 < x1 && wf;
+        }
+        cbits |= (uint32_t)cell << k;
+        sbits |= (uint32_t)span << k;
+        kprev = k1;
+    }
+    // chunk scan of (cells | spans << 16): a chunk holds at most 2048 of each
+    int32_t tot;
+    const int32_t ex = block_exclusive_add<int32_t, THREADS / 64>(__popc(cbits) | (__popc(sbits) << 16), s_w, tot);
+    int64_t pos = b.r0 + carry + (ex & 0xFFFF) + (ex >> 16);
+    kprev = j0 > 0 && j0 - 1 < n ? key(j0 - 1) : 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int64_t j = j0 + k;
+        if (j >= c0 + m) continue;
+        const int32_t k1 = keys[k];
+        const int cell = (cbits >> k) & 1, span = (sbits >> k) & 1;
+        const int x1 = key_x(k1), y1 = key_y(k1);
+        if (cell) E.crec[pos] = (uint32_t)x1 | (2u << 16);
+        if (span) {
+            const int x0 = max(0, key_x(kprev) + 2);
+            E.crec[pos + cell] = (uint32_t)x0 | ((uint32_t)(x1 - x0) << 16);
+        }
+        // record range of this entry's (path, cell row) bucket, for the composite
+        if ((uint32_t)k1 != kSentinelKey && y1 >= 0 && y1 < b.ch) {
+            const int32_t kn = j + 1 < n ? (k + 1 < kItems ? keys[k + 1 < kItems ? k + 1 : k] : key(j + 1)) : 0;
+            const bool first = j == 0 || (kprev >> 16) != (k1 >> 16);
+            const bool last = j == n - 1 || (kn >> 16) != (k1 >> 16);
+            if (first || last) {
+                const int r = y1 >> 1;
+                const int loc = r == 0 ? b.pr.y + 1 : r - b.pr.x;
+                if (loc >= 0 && loc < b.pr.y + 2) {  // out of range only after a flagged bound error
+                    int2* br = &E.brec[b.bb + loc];
+                    if (first) br->x = (int32_t)pos;
+                    if (last) br->y = (int32_t)(pos + cell + span);
+                }
+            }
+        }
+        pos += cell + span;
+        kprev = k1;
+    }
+    carry += (tot & 0xFFFF) + (tot >> 16);
+    if (threadIdx.x == 0) { cnt.x += tot & 0xFFFF; cnt.y += tot >> 16; }
+}
+
 // Sorts n <= THREADS * kItems words from src with one workgroup of THREADS threads (buf: 2 x
 // kPadded words of LDS) into dst (sort words) or, when dst is null, unpacked into out at offset
 // o0 with winding numbers from the band's exclusive prefix `pre`.
@@ -136,7 +238,8 @@ template <class W, int THREADS>
 __device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __restrict__ src,
                                           typename W::T* __restrict__ dst, int n,
                                           typename W::T (*buf)[Tile<THREADS>::kPadded], const Unpacked& out,
-                                          int64_t o0, int32_t pre, int32_t* s_w) {
+                                          int64_t o0, int32_t pre, int32_t* s_w, const EmitArgs& E,
+                                          bool emit, int64_t band) {
     typedef typename W::T T;
     constexpr T kPad = W::kPad;
     int P2 = kItems;
@@ -219,6 +322,18 @@ __device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __re
         if (i < n) wl[i] = wadd(ex, w8[k]);
     }
     __syncthreads();
+    if (emit) {
+        // mark + emit from LDS: sorted words and winding numbers stay on chip
+        const BandPath b = band_path_of(E, band, wd.pr, o0);
+        const T* S = buf[cur];
+        int32_t carry = 0;
+        int2 cnt = make_int2(0, 0);
+        emit_chunk<THREADS>(E, b, [&](int64_t j) { return wd.key(S[pad((int)j)]); },
+                            [&](int64_t j) { return wl[j]; }, 0, n, n, carry, cnt, s_w);
+        if (tid == 0) E.band_cnt[band] = cnt;
+        __syncthreads();
+        return;
+    }
     for (int i = tid; i < n; i += THREADS) {
         wd.put(out, o0 + i, buf[cur][pad(i)]);
         out.winding[o0 + i] = wl[i];
@@ -260,12 +375,12 @@ __global__ __launch_bounds__(THREADS) void k_sort_bands(const int32_t* __restric
                                                         int32_t* __restrict__ big_list,
                                                         int32_t* __restrict__ big_excl,
                                                         int32_t* __restrict__ big_count,
-                                                        const int2* __restrict__ band_pr) {
+                                                        const int2* __restrict__ band_pr, EmitArgs E, bool emit) {
     constexpr int kSize = Tile<THREADS>::kSize;
     __shared__ typename W::T buf[2][Tile<THREADS>::kPadded];
     __shared__ int32_t s_w[Tile<THREADS>::kWaves], s_t;
     const int32_t nb = *n_bands;
-    if (BIG && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (BIG && !emit && blockIdx.x == 0 && threadIdx.x == 0) {
         if (nb > 0) out.winding[band_start[nb]] = bpre[nb];  // the scan's total
         else out.winding[0] = 0;
     }
@@ -289,7 +404,8 @@ __global__ __launch_bounds__(THREADS) void k_sort_bands(const int32_t* __restric
             }
             W wd;
             wd.band(band_pr, t);
-            tile_sort<W, THREADS>(wd, src + s, (typename W::T*)nullptr, (int)n, buf, out, s, bpre[t], s_w);
+            tile_sort<W, THREADS>(wd, src + s, (typename W::T*)nullptr, (int)n, buf, out, s, bpre[t], s_w, E,
+                                  emit, t);
         }
         // any order: only the hand-out of the chunks matters
         c = BIG ? (int64_t)gridDim.x + ordered_tile_id(ticket, &s_t) : c + gridDim.x;
@@ -319,7 +435,7 @@ __global__ __launch_bounds__(kBigThreads) void k_sort_big_bands(const int32_t* _
         wd.band(band_pr, band);
         for (int64_t c = 0; c < n; c += kSortTile) {
             const int m = (int)(n - c < kSortTile ? n - c : kSortTile);
-            tile_sort<W, kThreads>(wd, scatter + s + c, a + s + c, m, buf, out, 0, 0, s_w);
+            tile_sort<W, kThreads>(wd, scatter + s + c, a + s + c, m, buf, out, 0, 0, s_w, EmitArgs{}, false, 0);
         }
         __threadfence_block();
         __syncthreads();
@@ -377,6 +493,52 @@ __global__ __launch_bounds__(kBigThreads) void k_sort_big_bands(const int32_t* _
 
 namespace {
 
+// Mark + emit of each oversized band (one workgroup per band) from its sorted keys and winding
+// numbers, in chunks of one tile with the records carried across.
+__global__ __launch_bounds__(kBigThreads) void k_emit_big_bands(const int32_t* __restrict__ band_start,
+                                                                 const int32_t* __restrict__ big_list,
+                                                                 const int32_t* __restrict__ big_count,
+                                                                 const int32_t* __restrict__ skey,
+                                                                 const int32_t* __restrict__ winding,
+                                                                 const int2* __restrict__ band_pr, EmitArgs E) {
+    __shared__ int32_t s_w[Tile<kBigThreads>::kWaves];
+    const int32_t nbig = *big_count;
+    for (int32_t q = blockIdx.x; q < nbig; q += gridDim.x) {
+        const int32_t band = big_list[q];
+        const int64_t s = band_start[band], n = band_start[band + 1] - s;
+        const BandPath b = band_path_of(E, band, band_pr[band], s);
+        int32_t carry = 0;
+        int2 cnt = make_int2(0, 0);
+        for (int64_t c = 0; c < n; c += kSortTile) {
+            const int m = (int)(n - c < kSortTile ? n - c : kSortTile);
+            emit_chunk<kBigThreads>(E, b, [&](int64_t j) { return skey[s + j]; },
+                                    [&](int64_t j) { return winding[s + j]; }, c, m, n, carry, cnt, s_w);
+        }
+        if (threadIdx.x == 0) E.band_cnt[band] = cnt;
+    }
+}
+
+// totals = (cells, spans, records) summed over the bands' counts.
+__global__ __launch_bounds__(1024) void k_band_totals(const int2* __restrict__ band_cnt,
+                                                      const int32_t* __restrict__ n_bands,
+                                                      int32_t* __restrict__ totals) {
+    __shared__ int32_t s_c[16], s_s[16];
+    const int32_t nb = *n_bands;
+    int32_t c = 0, sp = 0;
+    for (int32_t t = threadIdx.x; t < nb; t += 1024) { const int2 v = band_cnt[t]; c += v.x; sp += v.y; }
+    c = wave_sum(c);
+    sp = wave_sum(sp);
+    if ((threadIdx.x & 63) == 0) { s_c[threadIdx.x >> 6] = c; s_s[threadIdx.x >> 6] = sp; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tc = 0, ts = 0;
+        for (int k = 0; k < 16; ++k) { tc += s_c[k]; ts += s_s[k]; }
+        totals[0] = tc;
+        totals[1] = ts;
+        totals[2] = tc + ts;
+    }
+}
+
 // Workgroups per CU for a class: as many as the CU's 32 wave slots and 160 KiB of LDS allow.
 template <class T, int THREADS>
 unsigned class_grid(int64_t max_bands, int n_cu) {
@@ -391,14 +553,16 @@ template <class W>
 hipError_t launch_classes(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                           const typename W::T* src, const Unpacked& out, const int32_t* bpre, uint32_t* ticket,
                           int32_t* big_list, int32_t* big_excl, int32_t* big_count, int n_cu, hipStream_t s,
-                          const int2* band_pr) {
+                          const int2* band_pr, const EmitArgs* emit) {
     typedef typename W::T T;
+    const EmitArgs E = emit ? *emit : EmitArgs{};
     int32_t* long_list = big_list + 2 * max_bands;
     k_band_classify<<<(unsigned)((max_bands + 255) / 256), 256, 0, s>>>(band_start, n_bands, long_list, ticket + 1);
     k_sort_bands<W, 64, false><<<class_grid<T, 64>(max_bands, n_cu), 64, 0, s>>>(
-        band_start, n_bands, src, out, bpre, nullptr, nullptr, nullptr, nullptr, nullptr, band_pr);
+        band_start, n_bands, src, out, bpre, nullptr, nullptr, nullptr, nullptr, nullptr, band_pr, E, emit != nullptr);
     k_sort_bands<W, 256, true><<<class_grid<T, 256>((max_bands + kChunk - 1) / kChunk, n_cu), 256, 0, s>>>(
-        band_start, n_bands, src, out, bpre, long_list, ticket, big_list, big_excl, big_count, band_pr);
+        band_start, n_bands, src, out, bpre, long_list, ticket, big_list, big_excl, big_count, band_pr, E,
+        emit != nullptr);
     return hipGetLastError();
 }
 
@@ -407,16 +571,30 @@ hipError_t launch_classes(const int32_t* band_start, const int32_t* n_bands, int
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
-                             uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr) {
+                             uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr,
+                             const EmitArgs* emit) {
     if (max_bands <= 0) return hipSuccess;
     const Unpacked out{skey, sidx, winding};
     if (band_pr) {
         if (sidx) return hipErrorInvalidValue;  // narrow words hold no index
         return launch_classes<Narrow>(band_start, n_bands, max_bands, reinterpret_cast<const uint32_t*>(src), out,
-                                      bpre, ticket, big_list, big_excl, big_count, n_cu, s, band_pr);
+                                      bpre, ticket, big_list, big_excl, big_count, n_cu, s, band_pr, emit);
     }
+    if (emit) return hipErrorInvalidValue;  // wide words: the debug path keeps the separate mark + emit
     return launch_classes<Wide>(band_start, n_bands, max_bands, src, out, bpre, ticket, big_list, big_excl,
-                                big_count, n_cu, s, nullptr);
+                                big_count, n_cu, s, nullptr, nullptr);
+}
+
+hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
+                                 const int32_t* skey, const int32_t* winding, const int2* band_pr,
+                                 const EmitArgs& emit, hipStream_t s) {
+    k_emit_big_bands<<<64, kBigThreads, 0, s>>>(band_start, big_list, big_count, skey, winding, band_pr, emit);
+    return hipGetLastError();
+}
+
+hipError_t launch_band_totals(const int2* band_cnt, const int32_t* n_bands, int32_t* totals, hipStream_t s) {
+    k_band_totals<<<1, 1024, 0, s>>>(band_cnt, n_bands, totals);
+    return hipGetLastError();
 }
 
 hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_excl,

@@ -138,10 +138,26 @@ hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uin
 hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
                              hipStream_t s);
 // band_pr (or null): each band's path rows (prow of its path), for the narrow sort.
+// band_path (with band_pr): each band's path.
 hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,
                             int32_t* band_start, hipStream_t s, const int32_t* nb_dev = nullptr,
                             const int32_t* n_dev = nullptr, const int32_t* bucket_base = nullptr, uint32_t Np = 0,
-                            const int2* prow = nullptr, int2* band_pr = nullptr);
+                            const int2* prow = nullptr, int2* band_pr = nullptr, int32_t* band_path = nullptr);
+// Mark + emit fused into the narrow band sort (frames that export no stage): each band flags its
+// cells and spans (every flag depends only on the band's own entries: bands split at (path, row)
+// bucket boundaries), writes its compact records at [2 * band_start, ...) of crec, the record
+// range of each of its buckets into brec and its (cells, spans) into band_cnt.
+struct EmitArgs {
+    uint32_t* crec;
+    int2* brec;
+    int2* band_cnt;
+    const int32_t* band_path;
+    const int32_t* bucket_base;
+    const uint32_t* rule;
+    const int2* pcanvas;  // per-path canvas of a batch, or null
+    const int32_t* wbase; // per-path winding base of a batch, or null
+    int w, h;
+};
 // The band sorts write the sorted keys, winding changes and (optionally) indices directly.
 // Sorts every band and writes keys, (debug) indices and winding numbers (the exclusive scan of
 // the sorted winding changes, total at winding[n]); bpre holds each band's exclusive prefix of
@@ -149,10 +165,19 @@ hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32
 // exclusive prefix) for launch_sort_big_bands.  big_list: 3 * max_bands int32.  ticket: two
 // zeroed u32 counters.  narrow (band_pr non-null): 32-bit words from the narrow scatter; sidx
 // must be null.
+// emit (narrow only, or null): mark + emit fused (EmitArgs); the bands then write no keys or
+// winding numbers, except the oversized ones, which launch_emit_big_bands reads back.
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
-                             uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr = nullptr);
+                             uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr = nullptr,
+                             const EmitArgs* emit = nullptr);
+// Mark + emit of the oversized bands from their sorted keys and winding numbers.
+hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
+                                 const int32_t* skey, const int32_t* winding, const int2* band_pr,
+                                 const EmitArgs& emit, hipStream_t s);
+// totals[0..2] = cells, spans, records: the sums of band_cnt over the bands.
+hipError_t launch_band_totals(const int2* band_cnt, const int32_t* n_bands, int32_t* totals, hipStream_t s);
 hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_excl,
                                  const int32_t* big_count, unsigned long long* scatter, unsigned long long* tmp,
                                  int32_t* skey, int32_t* sidx, int32_t* winding, hipStream_t s,
@@ -220,6 +245,11 @@ struct DocDev {
     int32_t pad;
 };
 // wbase[p] = winding[first sorted entry of p's document] (each document scans from zero)
+// The same from the bands' exclusive prefixes (bpre, total at [n_bands]), before the band sort:
+// every sorted path start is a band start or the end.
+hipError_t launch_doc_winding_base_bands(const int32_t* band_start, const int32_t* bpre, const int32_t* n_bands,
+                                         const int32_t* sstart, const int32_t* doc_first_path, uint32_t Np,
+                                         int32_t* wbase, hipStream_t s);
 hipError_t launch_doc_winding_base(const int32_t* winding, const int32_t* sstart, const int32_t* doc_first_path,
                                    uint32_t Np, int32_t* wbase, hipStream_t s);
 

@@ -118,6 +118,8 @@ struct spr_handle {
     DBuf<uint32_t> crec;    // compact records (x | width << 16) read by the composite
     DBuf<int32_t> row_lo, row_hi, bucket_base, bflag, band_start, big_list;
     DBuf<int2> band_pr;
+    DBuf<int2> band_cnt;      // (cells, spans) per band, with mark + emit fused into the sort
+    DBuf<int32_t> band_path;  // path of each band
     DBuf<int2> prow;
     DBuf<uint8_t> image;
     DBuf<unsigned long long> zero;  // per-frame zeroed control region
@@ -140,6 +142,7 @@ struct spr_handle {
     int64_t learned_fallbacks = 0;                  // learned frames that read their budgets back
     bool last_learned = false;
     int2* last_brec = nullptr;                      // per-bucket record ranges of the last frame
+    bool last_fused = false;                        // last frame fused mark + emit into the sort
 
     // batch of documents (spr_load_batch): per-path canvas, per-document tables
     bool batch = false;
@@ -328,6 +331,8 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->bflag.ensure(nb1 + 1);
     H->band_start.ensure((size_t)max_bands + 1);
     H->band_pr.ensure((size_t)max_bands + 1);
+    H->band_cnt.ensure((size_t)max_bands + 1);
+    H->band_path.ensure((size_t)max_bands + 1);
     H->big_list.ensure(3 * (size_t)max_bands);  // oversized band ids, their winding prefixes, long bands
     if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
     H->crec.ensure(2 * n1);
@@ -385,7 +390,8 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, d_hist, H->bflag.p, s));
         SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, NB, ctl_at(H, z.st_band, z.ctr, 1), s));
         SPR_HIP(spr::launch_band_emit(H->bflag.p, d_hist, (int32_t)NB, N, H->band_start.p, s, nullptr, nullptr,
-                                      H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr));
+                                      H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr,
+                                      narrow ? H->band_path.p : nullptr));
         SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, N, H->bucket_base.p, H->prow.p,
                                            H->bflag.p, H->band_start.p, d_fill, d_bsum, H->sbig.p, d_err, s, nullptr,
                                            narrow));
@@ -393,6 +399,9 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         SPR_HIP(spr::launch_scan_i32(d_bsum, d_bsum, max_bands, ctl_at(H, z.st_wind, z.ctr, 2), s, H->bflag.p + NB));
     }
     stage_mark(H, 8);
+    // narrow frames fuse mark + emit into the band sort: records of band t at [2 * start, ...)
+    const spr::EmitArgs E{H->crec.p, d_brec, H->band_cnt.p, H->band_path.p, H->bucket_base.p, H->rule.p,
+                          nullptr, nullptr, w, h};
     if (N > 0) {
         // band sort fused with the winding numbers (global exclusive scan of the sorted winding
         // changes, scanline_rasterizer.cpp:478-506)
@@ -400,19 +409,26 @@ void run_frame(spr_handle* H, bool record_begin = true) {
                                        debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
                                        H->big_list.p + max_bands, d_bigcount, d_bsum,
                                        ctl_at(H, z.st_wind, z.ctr, 4).counter, H->n_cu, s,
-                                       narrow ? H->band_pr.p : nullptr));
+                                       narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr));
         SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                            H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr,
                                            H->winding.p, s, narrow ? H->band_pr.p : nullptr));
+        if (narrow) {
+            SPR_HIP(spr::launch_emit_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->skey.p, H->winding.p,
+                                               H->band_pr.p, E, s));
+            SPR_HIP(spr::launch_band_totals(H->band_cnt.p, H->bflag.p + NB, d_totals, s));
+        }
     }
     stage_mark(H, 9);
     stage_mark(H, 10);
-    SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
-                                  ctl_at(H, z.st_mark, z.ctr, 3), debug ? H->records.p : nullptr, H->crec.p,
-                                  debug ? H->flags.p : nullptr, d_totals, H->bucket_base.p, H->prow.p, d_brec, s,
-                                  nullptr, nullptr, Np, (int64_t)(2 * n1), d_err));
+    if (!narrow)
+        SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
+                                      ctl_at(H, z.st_mark, z.ctr, 3), debug ? H->records.p : nullptr, H->crec.p,
+                                      debug ? H->flags.p : nullptr, d_totals, H->bucket_base.p, H->prow.p, d_brec,
+                                      s, nullptr, nullptr, Np, (int64_t)(2 * n1), d_err));
     H->last_brec = d_brec;
     H->last_learned = false;
+    H->last_fused = narrow;
     stage_mark(H, 11);
     // ---- composite
     SPR_HIP(spr::launch_composite(H->crec.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
@@ -566,9 +582,14 @@ void run_learned_frame(spr_handle* H) {
     H->bflag.ensure((size_t)nb_budget + 1);
     H->band_start.ensure((size_t)max_bands + 1);
     H->band_pr.ensure((size_t)max_bands + 1);
+    H->band_cnt.ensure((size_t)max_bands + 1);
+    H->band_path.ensure((size_t)max_bands + 1);
     H->big_list.ensure(3 * (size_t)max_bands);  // oversized band ids, their winding prefixes, long bands
     if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); H->records.ensure(nr); }
-    H->crec.ensure(nr);
+    // narrow frames fuse mark + emit into the band sort: the records of band t sit at
+    // [2 * start, ...), so the record buffer covers two per sorted entry
+    const int64_t rec_cap = narrow ? 2 * nl : nr;
+    H->crec.ensure(rec_cap);
     H->image.ensure(H->batch ? H->img_bytes : (size_t)4 * H->w * H->h);
     H->hist_l.ensure((size_t)nb_budget + 1);
     H->fill_l.ensure((size_t)nb_budget + 1);
@@ -626,34 +647,48 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, nb_budget, ctl_at(H, z.st_band, z.counters, 3), s,
                                  d_nb, d_nbands));
     SPR_HIP(spr::launch_band_emit(H->bflag.p, H->hist_l.p, (int32_t)nb_budget, nl, H->band_start.p, s, d_nb,
-                                  d_nsorted, H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr));
+                                  d_nsorted, H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr,
+                                  narrow ? H->band_path.p : nullptr));
     SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, nl, H->bucket_base.p, H->prow.p,
                                        H->bflag.p, H->band_start.p, H->fill_l.p, H->bsum_l.p, H->sbig.p, d_err, s,
                                        d_live, narrow));
     SPR_HIP(spr::launch_scan_i32(H->bsum_l.p, H->bsum_l.p, max_bands, ctl_at(H, z.st_wind, z.counters, 4), s,
                                  d_nbands));
+    SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
+    // documents of a batch scan their winding numbers from their own first entry: with the fused
+    // mark + emit that base comes from the band prefixes, before the sort
+    if (H->batch && narrow)
+        SPR_HIP(spr::launch_doc_winding_base_bands(H->band_start.p, H->bsum_l.p, d_nbands, H->sstart.p,
+                                                   H->doc_first.p, Np, H->wbase.p, s));
     stage_mark(H, 8);
+    const spr::EmitArgs E{H->crec.p, H->brec_l.p, H->band_cnt.p, H->band_path.p, H->bucket_base.p, H->rule.p,
+                          pcv, H->batch ? H->wbase.p : nullptr, w, h};
     SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
                                    debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
                                    H->big_list.p + max_bands, d_bigcount, H->bsum_l.p,
                                    ctl_at(H, z.st_wind, z.counters, 6).counter, H->n_cu, s,
-                                   narrow ? H->band_pr.p : nullptr));
+                                   narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr));
     SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                        H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr, H->winding.p,
                                        s, narrow ? H->band_pr.p : nullptr));
-    SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
+    if (narrow) {
+        SPR_HIP(spr::launch_emit_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->skey.p, H->winding.p,
+                                           H->band_pr.p, E, s));
+        SPR_HIP(spr::launch_band_totals(H->band_cnt.p, d_nbands, d_totals, s));
+    }
     stage_mark(H, 9);
-    if (H->batch)
+    if (H->batch && !narrow)
         SPR_HIP(spr::launch_doc_winding_base(H->winding.p, H->sstart.p, H->doc_first.p, Np, H->wbase.p, s));
     stage_mark(H, 10);
-    SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, nl, w, h,
-                                  ctl_at(H, z.st_mark, z.counters, 5), debug ? H->records.p : nullptr, H->crec.p,
-                                  debug ? H->flags.p : nullptr, d_totals, H->bucket_base.p, H->prow.p, H->brec_l.p,
-                                  s, d_nsorted, H->sstart.p,
-                                  Np, nr, d_err, pcv, H->batch ? H->wbase.p : nullptr));
+    if (!narrow)
+        SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, nl, w, h,
+                                      ctl_at(H, z.st_mark, z.counters, 5), debug ? H->records.p : nullptr,
+                                      H->crec.p, debug ? H->flags.p : nullptr, d_totals, H->bucket_base.p, H->prow.p,
+                                      H->brec_l.p, s, d_nsorted, H->sstart.p, Np, nr, d_err, pcv,
+                                      H->batch ? H->wbase.p : nullptr));
     stage_mark(H, 11);
     SPR_HIP(spr::launch_composite(H->crec.p, H->brec_l.p, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
-                                  (int)H->opts.compose, H->image.p, s, nr, H->batch ? H->docs.p : nullptr,
+                                  (int)H->opts.compose, H->image.p, s, rec_cap, H->batch ? H->docs.p : nullptr,
                                   (int)H->n_docs, H->total_rows));
     stage_mark(H, 12);
     SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
@@ -679,6 +714,7 @@ void learned_finish(spr_handle* H) {
     H->frame_done = true;
     H->last_learned = true;
     H->last_brec = H->brec_l.p;
+    H->last_fused = !H->opts.debug && H->w <= spr::kNarrowMaxW;  // as the frame's narrow words
     if (err & 1) {
         H->h_pinned[8] = 0;
         throw Fail{SPR_ERR_DEVICE, "internal: an entry fell outside its path's row bound"};
@@ -1059,12 +1095,18 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 break;
             }
             case SPR_STAGE_SEG: from_dev(H->seg.p, ((size_t)H->Np + 1) * 4); break;
-            case SPR_STAGE_SKEY: from_dev(H->skey.p, (size_t)N * 4); break;
+            case SPR_STAGE_SKEY:
+                if (H->last_fused) throw Fail{SPR_ERR_STATE, "sorted keys are kept only with the debug option"};
+                from_dev(H->skey.p, (size_t)N * 4);
+                break;
             case SPR_STAGE_SIDX:
                 if (!debug) throw Fail{SPR_ERR_STATE, "sorted indices are kept only with the debug option"};
                 from_dev(H->sidx.p, (size_t)N * 4);
                 break;
-            case SPR_STAGE_WINDING: from_dev(H->winding.p, ((size_t)N + 1) * 4); break;
+            case SPR_STAGE_WINDING:
+                if (H->last_fused) throw Fail{SPR_ERR_STATE, "winding numbers are kept only with the debug option"};
+                from_dev(H->winding.p, ((size_t)N + 1) * 4);
+                break;
             case SPR_STAGE_FLAGS:
                 if (!debug) throw Fail{SPR_ERR_STATE, "flags are kept only with the debug option"};
                 from_dev(H->flags.p, (size_t)N * 8);
