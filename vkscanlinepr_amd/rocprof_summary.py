@@ -26,9 +26,10 @@ STAGE_KERNELS = {
     "piece_fragments": ["k_piece_small", "k_piece_fragments", "k_piece_heavy"],
     "gen_fragment": ["k_gen_fragment"],
     "bucket_scatter": ["k_bucket_scatter", "k_band_flags", "k_band_emit"],
-    "band_sort": ["k_band_classify", "k_sort_bands", "k_sort_big_bands"],
+    "band_sort": ["k_band_classify", "k_sort_bands", "k_sort_big_bands", "k_emit_big_bands", "k_band_totals"],
+    # with mark + emit fused into the band sort (frames that export no stage), the whole sort
     "sort": ["k_bucket_scatter", "k_band_flags", "k_band_emit", "k_band_classify", "k_sort_bands",
-             "k_sort_big_bands"],
+             "k_sort_big_bands", "k_emit_big_bands", "k_band_totals", "k_mark_emit"],
     "mark_emit": ["k_mark_emit"],
     "composite": ["k_composite"],
 }
