@@ -1,5 +1,5 @@
 // k_sort.hip -- sort of the cell keys inside each (path, row) band, fused with the winding
-// scan (gfx950).
+// scan and, in frames that export no stage, with mark + emit (gfx950).
 //
 // Replaces naive_seg_sort_pairs.comp:26-97 (one workgroup per path running an O(n^2) odd-even
 // transposition network), shuffle_fragment.comp:17-26 and the winding-number scan of
@@ -9,24 +9,29 @@
 // whose unsigned order equals the reference's (key, index) order, is unique, and carries the
 // winding change, so the reference's separate gather pass disappears.
 //
-// After the bucket scatter every band (<= 2048 entries, a run of whole buckets of one path) only
+// After the band scatter every band (<= 2048 entries, a run of whole buckets of one path) only
 // needs sorting in place: one workgroup loads it once, sorts it in LDS (8-element register
 // network per thread, then merge-path rounds) and stores it once.  A band's sum of winding
 // changes needs no sorting: the scatter adds the sums up per band and one scan turns them into
-// each band's exclusive prefix before any band is sorted, so the bands are independent and each
-// writes its winding numbers next to its keys.  Bands of up to 512 entries are sorted by
-// one-wave workgroups (so that the many short bands of a page of small paths do not leave most
-// of a large workgroup idle), the longer ones, listed by a classifying pass, by 256-thread
-// workgroups that take them in chunks, the first chunk by workgroup id and the rest from an
-// atomic ticket (their cost varies with the band length, so the work is handed out as it frees
-// up, and a frame without long bands takes no ticket at all).  A band above 2048 entries is a single oversized (path, row)
-// bucket; such bands are queued, and each is then sorted by one workgroup with LDS chunk sorts
-// followed by merge-path passes through global memory, the last pass writing keys and winding
-// numbers.
+// each band's exclusive prefix before any band is sorted, so the bands are independent.  Bands
+// of up to 512 entries are sorted by one-wave workgroups (so that the many short bands of a page
+// of small paths do not leave most of a large workgroup idle), the longer ones, listed by a
+// classifying pass, by 256-thread workgroups that take them in chunks, the first chunk by
+// workgroup id and the rest from an atomic ticket (their cost varies with the band length, so
+// the work is handed out as it frees up, and a frame without long bands takes no ticket at all).
+// A band above 2048 entries is a single oversized (path, row) bucket; such bands are queued, and
+// each is then sorted by one workgroup with LDS chunk sorts followed by merge-path passes
+// through global memory, the last pass writing keys and winding numbers.
 //
 // Frames that export no stage sort the narrow 32-bit words of the scatter instead (bucket inside
 // the path, x, winding change: see k_bucket.hip), with the same kernels; the keys are rebuilt
-// from the band's path rows.
+// from the band's path rows.  These frames also run mark + emit (mark_merged_fragment_and_span
+// .comp, the 2N flag scan and gen_merged_fragment_and_span.comp) inside the band sort, on the
+// sorted words and winding numbers still in LDS: every flag of a band depends on its own entries
+// only (a band starts at a bucket, i.e. a new cell row), so each band writes its records at twice
+// its start, the record range of each of its buckets and its (cells, spans); the oversized bands
+// do the same from their sorted keys (k_emit_big_bands).  Debug frames write keys, indices and
+// winding numbers, and k_mark_emit (k_scan.hip) emits the records in the reference's order.
 #include "device_common.hpp"
 #include "kernels.hpp"
 #include "scan_common.hpp"
