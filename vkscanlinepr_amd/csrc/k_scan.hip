@@ -9,7 +9,9 @@
 // scanline_rasterizer.cpp:544-573 and gen_merged_fragment_and_span.comp:152-222 into one pass:
 // the cell/span flags of a tile are counted, the (cells, spans) pair is scanned with one
 // look-back, and the records are written at their final positions.  R is left on the device,
-// which removes the reference's third host synchronisation.
+// which removes the reference's third host synchronisation.  It runs in debug frames (and on
+// canvases too wide for the narrow sort words); the other frames do mark + emit inside the band
+// sort (k_sort.hip).
 #include "kernels.hpp"
 #include "scan_common.hpp"
 
