@@ -523,24 +523,30 @@ __global__ __launch_bounds__(kBigThreads) void k_emit_big_bands(const int32_t* _
     }
 }
 
-// totals = (cells, spans, records) summed over the bands' counts.
-__global__ __launch_bounds__(1024) void k_band_totals(const int2* __restrict__ band_cnt,
-                                                      const int32_t* __restrict__ n_bands,
-                                                      int32_t* __restrict__ totals) {
-    __shared__ int32_t s_c[16], s_s[16];
+// totals (zeroed) += (cells, spans, records) summed over the bands' counts: a grid-stride
+// loop, one sum per block, three atomics per block.
+constexpr int kTotBlocks = 64, kTotThreads = 256;
+__global__ __launch_bounds__(kTotThreads) void k_band_totals(const int2* __restrict__ band_cnt,
+                                                             const int32_t* __restrict__ n_bands,
+                                                             int32_t* __restrict__ totals) {
+    __shared__ int32_t s_c[kTotThreads / 64], s_s[kTotThreads / 64];
     const int32_t nb = *n_bands;
     int32_t c = 0, sp = 0;
-    for (int32_t t = threadIdx.x; t < nb; t += 1024) { const int2 v = band_cnt[t]; c += v.x; sp += v.y; }
+    for (int32_t t = blockIdx.x * kTotThreads + threadIdx.x; t < nb; t += kTotBlocks * kTotThreads) {
+        const int2 v = band_cnt[t];
+        c += v.x;
+        sp += v.y;
+    }
     c = wave_sum(c);
     sp = wave_sum(sp);
     if ((threadIdx.x & 63) == 0) { s_c[threadIdx.x >> 6] = c; s_s[threadIdx.x >> 6] = sp; }
     __syncthreads();
     if (threadIdx.x == 0) {
         int32_t tc = 0, ts = 0;
-        for (int k = 0; k < 16; ++k) { tc += s_c[k]; ts += s_s[k]; }
-        totals[0] = tc;
-        totals[1] = ts;
-        totals[2] = tc + ts;
+        for (int k = 0; k < kTotThreads / 64; ++k) { tc += s_c[k]; ts += s_s[k]; }
+        if (tc) atomicAdd(&totals[0], tc);
+        if (ts) atomicAdd(&totals[1], ts);
+        if (tc + ts) atomicAdd(&totals[2], tc + ts);
     }
 }
 
@@ -598,7 +604,7 @@ hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_l
 }
 
 hipError_t launch_band_totals(const int2* band_cnt, const int32_t* n_bands, int32_t* totals, hipStream_t s) {
-    k_band_totals<<<1, 1024, 0, s>>>(band_cnt, n_bands, totals);
+    k_band_totals<<<kTotBlocks, kTotThreads, 0, s>>>(band_cnt, n_bands, totals);
     return hipGetLastError();
 }
 

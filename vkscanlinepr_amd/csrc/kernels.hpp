@@ -176,7 +176,7 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
 hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
                                  const int32_t* skey, const int32_t* winding, const int2* band_pr,
                                  const EmitArgs& emit, hipStream_t s);
-// totals[0..2] = cells, spans, records: the sums of band_cnt over the bands.
+// totals[0..2] (zeroed) += cells, spans, records: the sums of band_cnt over the bands.
 hipError_t launch_band_totals(const int2* band_cnt, const int32_t* n_bands, int32_t* totals, hipStream_t s);
 hipError_t launch_sort_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_excl,
                                  const int32_t* big_count, unsigned long long* scatter, unsigned long long* tmp,
