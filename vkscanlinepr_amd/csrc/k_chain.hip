@@ -154,26 +154,33 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
 // about 2^-6 of a cubic one), cubic before line on ties.  Cost key: cubic class c -> 2(c+24)+1,
 // line class 64+l -> 8l; the keys are distinct.  The cubic classes of kLongClass octaves and up
 // come first: the wave-solved chains.
+__device__ inline int class_cost_key(int c) { return c < 64 ? 2 * (c + 24) + 1 : 8 * (c - 64); }
+
+// One thread per class: its offset is the number of chains in the classes of higher cost key
+// (the keys are distinct).  The wave-solved classes and the heavy ones come first, so their
+// totals are also the ends of those groups.
 __global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t* __restrict__ class_off,
                                 int32_t* __restrict__ n_long, int32_t* __restrict__ light_begin) {
     __shared__ int32_t cnt[kClasses];
     for (int c = threadIdx.x; c < kClasses; c += blockDim.x) cnt[c] = class_count[c];  // loads at once
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    int32_t off = 0, nl = 0, lb = 0;
-    for (int key = 2 * (63 + 24) + 1; key >= 0; --key) {
-        int c = -1;
-        if (key & 1) c = (key - 1) / 2 - 24;
-        else if ((key & 7) == 0 && key / 8 <= 15) c = 64 + key / 8;
-        if (c < 0) continue;
+    for (int c = threadIdx.x; c < kClasses; c += blockDim.x) {
+        const int key = class_cost_key(c);
+        int32_t off = 0;
+        for (int d = 0; d < kClasses; ++d) off += class_cost_key(d) > key ? cnt[d] : 0;
         class_off[c] = off;
-        off += cnt[c];
-        if (c < 64 && c >= 4 * kLongClass) nl = off;
-        if (c < 64 && c >= kHeavyClass) lb = off;  // heavy classes come first, light ones after
     }
-    class_off[kClasses] = off;
-    *n_long = nl;
-    *light_begin = lb;
+    if (threadIdx.x == 0) {
+        int32_t total = 0, nl = 0, lb = 0;
+        for (int d = 0; d < kClasses; ++d) {
+            total += cnt[d];
+            if (d < 64 && d >= 4 * kLongClass) nl += cnt[d];
+            if (d < 64 && d >= kHeavyClass) lb += cnt[d];
+        }
+        class_off[kClasses] = total;
+        *n_long = nl;
+        *light_begin = lb;
+    }
 }
 
 // Chains in class order: each workgroup owns a contiguous range of chain slots, counts its

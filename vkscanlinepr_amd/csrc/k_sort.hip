@@ -346,22 +346,34 @@ __device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __re
     __syncthreads();
 }
 
-// Long bands (above one 64-thread tile) are listed in `list`, their number in *count, one
-// wave-aggregated atomic per wave; the order of the list does not matter.
+// Long bands (above one 64-thread tile) are listed in `list`, their number in *count, with one
+// atomic per 1024-thread block (the waves' counts are added up in LDS); the order of the list
+// does not matter.
 constexpr int kShortMax = 64 * kItems;
-__global__ __launch_bounds__(256) void k_band_classify(const int32_t* __restrict__ band_start,
-                                                       const int32_t* __restrict__ n_bands,
-                                                       int32_t* __restrict__ list, uint32_t* __restrict__ count) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+constexpr int kClassifyThreads = 1024;
+__global__ __launch_bounds__(kClassifyThreads) void k_band_classify(const int32_t* __restrict__ band_start,
+                                                                    const int32_t* __restrict__ n_bands,
+                                                                    int32_t* __restrict__ list,
+                                                                    uint32_t* __restrict__ count) {
+    __shared__ uint32_t s_n[kClassifyThreads / 64], s_base;
+    const int64_t t = (int64_t)blockIdx.x * kClassifyThreads + threadIdx.x;
     const int32_t nb = *n_bands;
     const bool is_long = t < nb && band_start[t + 1] - band_start[t] > kShortMax;
     const unsigned long long m = __ballot(is_long);
-    if (!m) return;
-    const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
-    if (lane == __ffsll((long long)m) - 1) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = __shfl(base, __ffsll((long long)m) - 1);
-    if (is_long) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)t;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) s_n[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int k = 0; k < kClassifyThreads / 64; ++k) tot += s_n[k];
+        s_base = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    if (is_long) {
+        uint32_t off = s_base;
+        for (int k = 0; k < wave; ++k) off += s_n[k];
+        list[off + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)t;
+    }
 }
 
 // BIG = false: the bands of up to THREADS * kItems entries, the workgroups striding over all
@@ -568,7 +580,8 @@ hipError_t launch_classes(const int32_t* band_start, const int32_t* n_bands, int
     typedef typename W::T T;
     const EmitArgs E = emit ? *emit : EmitArgs{};
     int32_t* long_list = big_list + 2 * max_bands;
-    k_band_classify<<<(unsigned)((max_bands + 255) / 256), 256, 0, s>>>(band_start, n_bands, long_list, ticket + 1);
+    k_band_classify<<<(unsigned)((max_bands + kClassifyThreads - 1) / kClassifyThreads), kClassifyThreads, 0, s>>>(
+        band_start, n_bands, long_list, ticket + 1);
     k_sort_bands<W, 64, false><<<class_grid<T, 64>(max_bands, n_cu), 64, 0, s>>>(
         band_start, n_bands, src, out, bpre, nullptr, nullptr, nullptr, nullptr, nullptr, band_pr, E, emit != nullptr);
     k_sort_bands<W, 256, true><<<class_grid<T, 256>((max_bands + kChunk - 1) / kChunk, n_cu), 256, 0, s>>>(
