@@ -576,18 +576,31 @@ template <class W>
 hipError_t launch_classes(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                           const typename W::T* src, const Unpacked& out, const int32_t* bpre, uint32_t* ticket,
                           int32_t* big_list, int32_t* big_excl, int32_t* big_count, int n_cu, hipStream_t s,
-                          const int2* band_pr, const EmitArgs* emit) {
+                          const int2* band_pr, const EmitArgs* emit, SideStream side) {
     typedef typename W::T T;
     const EmitArgs E = emit ? *emit : EmitArgs{};
     int32_t* long_list = big_list + 2 * max_bands;
+    // the short bands (no dependence on the long ones) on the second stream when there is one
+    hipStream_t ss = s;
+    if (side.s2) {
+        hipError_t e = hipEventRecord(side.fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(side.s2, side.fork, 0);
+        if (e != hipSuccess) return e;
+        ss = side.s2;
+    }
+    k_sort_bands<W, 64, false><<<class_grid<T, 64>(max_bands, n_cu), 64, 0, ss>>>(
+        band_start, n_bands, src, out, bpre, nullptr, nullptr, nullptr, nullptr, nullptr, band_pr, E, emit != nullptr);
     k_band_classify<<<(unsigned)((max_bands + kClassifyThreads - 1) / kClassifyThreads), kClassifyThreads, 0, s>>>(
         band_start, n_bands, long_list, ticket + 1);
-    k_sort_bands<W, 64, false><<<class_grid<T, 64>(max_bands, n_cu), 64, 0, s>>>(
-        band_start, n_bands, src, out, bpre, nullptr, nullptr, nullptr, nullptr, nullptr, band_pr, E, emit != nullptr);
     k_sort_bands<W, 256, true><<<class_grid<T, 256>((max_bands + kChunk - 1) / kChunk, n_cu), 256, 0, s>>>(
         band_start, n_bands, src, out, bpre, long_list, ticket, big_list, big_excl, big_count, band_pr, E,
         emit != nullptr);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && side.s2) {
+        e = hipEventRecord(side.join, side.s2);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, side.join, 0);
+    }
+    return e;
 }
 
 }  // namespace
@@ -596,17 +609,17 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
                              uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr,
-                             const EmitArgs* emit) {
+                             const EmitArgs* emit, SideStream side) {
     if (max_bands <= 0) return hipSuccess;
     const Unpacked out{skey, sidx, winding};
     if (band_pr) {
         if (sidx) return hipErrorInvalidValue;  // narrow words hold no index
         return launch_classes<Narrow>(band_start, n_bands, max_bands, reinterpret_cast<const uint32_t*>(src), out,
-                                      bpre, ticket, big_list, big_excl, big_count, n_cu, s, band_pr, emit);
+                                      bpre, ticket, big_list, big_excl, big_count, n_cu, s, band_pr, emit, side);
     }
     if (emit) return hipErrorInvalidValue;  // wide words: the debug path keeps the separate mark + emit
     return launch_classes<Wide>(band_start, n_bands, max_bands, src, out, bpre, ticket, big_list, big_excl,
-                                big_count, n_cu, s, nullptr, nullptr);
+                                big_count, n_cu, s, nullptr, nullptr, side);
 }
 
 hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,

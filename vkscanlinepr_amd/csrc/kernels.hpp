@@ -167,11 +167,17 @@ struct EmitArgs {
 // must be null.
 // emit (narrow only, or null): mark + emit fused (EmitArgs); the bands then write no keys or
 // winding numbers, except the oversized ones, which launch_emit_big_bands reads back.
+// side (optional): a second stream and two events; the short bands are then sorted there,
+// beside the long ones, and s waits for them before it returns.
+struct SideStream {
+    hipStream_t s2;
+    hipEvent_t fork, join;
+};
 hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
                              uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr = nullptr,
-                             const EmitArgs* emit = nullptr);
+                             const EmitArgs* emit = nullptr, SideStream side = {nullptr, nullptr, nullptr});
 // Mark + emit of the oversized bands from their sorted keys and winding numbers.
 hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
                                  const int32_t* skey, const int32_t* winding, const int2* band_pr,

@@ -82,6 +82,7 @@ struct spr_handle {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // the light chain solve and the pieces that need only it
     hipEvent_t ev_fork = nullptr, ev_light = nullptr, ev_small = nullptr, ev_join = nullptr;
+    hipEvent_t ev_sfork = nullptr, ev_sjoin = nullptr;  // the short-band sort on the second stream
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::vector<hipEvent_t> ev_stage;
     bool stage_timing = false;
@@ -199,6 +200,15 @@ ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
     z.brec = o; o += (size_t)NB;                // NB int2
     z.total = o;
     return z;
+}
+
+// The short bands are sorted on the second stream, beside the long ones, when the paths hold
+// 2048 or more entries on average (then the long bands are most of the sort and the short ones
+// fit beside them); with small paths the short bands are the whole sort and the fork and join
+// would only add their latency.
+spr::SideStream side_for(spr_handle* H, int64_t n_entries, uint32_t Np) {
+    if (Np == 0 || n_entries / (int64_t)Np < 2048) return {nullptr, nullptr, nullptr};
+    return {H->stream2, H->ev_sfork, H->ev_sjoin};
 }
 
 spr::ScanCtl ctl_at(spr_handle* H, size_t st, size_t ctr_word, int counter) {
@@ -409,7 +419,8 @@ void run_frame(spr_handle* H, bool record_begin = true) {
                                        debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
                                        H->big_list.p + max_bands, d_bigcount, d_bsum,
                                        ctl_at(H, z.st_wind, z.ctr, 4).counter, H->n_cu, s,
-                                       narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr));
+                                       narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr,
+                                       side_for(H, N, Np)));
         SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                            H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr,
                                            H->winding.p, s, narrow ? H->band_pr.p : nullptr));
@@ -667,7 +678,8 @@ void run_learned_frame(spr_handle* H) {
                                    debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
                                    H->big_list.p + max_bands, d_bigcount, H->bsum_l.p,
                                    ctl_at(H, z.st_wind, z.counters, 6).counter, H->n_cu, s,
-                                   narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr));
+                                   narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr,
+                                   side_for(H, nl, Np)));
     SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                        H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr, H->winding.p,
                                        s, narrow ? H->band_pr.p : nullptr));
@@ -908,7 +920,7 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
         (void)hipGetLastError();
         SPR_HIP(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
         SPR_HIP(hipStreamCreateWithFlags(&H->stream2, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&H->ev_fork, &H->ev_light, &H->ev_small, &H->ev_join})
+        for (hipEvent_t* e : {&H->ev_fork, &H->ev_light, &H->ev_small, &H->ev_join, &H->ev_sfork, &H->ev_sjoin})
             SPR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
         SPR_HIP(hipEventCreate(&H->ev_begin));
         SPR_HIP(hipEventCreate(&H->ev_end));
@@ -948,7 +960,7 @@ int spr_destroy(spr_handle* H) {
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
     if (H->ev_end) (void)hipEventDestroy(H->ev_end);
-    for (hipEvent_t e : {H->ev_fork, H->ev_light, H->ev_small, H->ev_join})
+    for (hipEvent_t e : {H->ev_fork, H->ev_light, H->ev_small, H->ev_join, H->ev_sfork, H->ev_sjoin})
         if (e) (void)hipEventDestroy(e);
     if (H->stream2) (void)hipStreamDestroy(H->stream2);
     if (H->stream) (void)hipStreamDestroy(H->stream);
