@@ -289,17 +289,16 @@ __global__ __launch_bounds__(256) void k_learned_place(
     const int lane = threadIdx.x & 63;
     if (p >= Np) return;  // wave-uniform
     const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
-    int32_t carry = 0;
+    // a path of at most 64 curves keeps its scan in registers and writes its offsets once, after
+    // the placement; a longer one writes path-relative offsets and adds the base afterwards
+    const bool one = c1 - c0 <= 64u;
+    int32_t carry = 0, cnt0 = 0, inc0 = 0;
     for (uint32_t cs = c0; cs < c1; cs += 64) {
         const uint32_t c = cs + lane;
         const int32_t cnt = c < c1 ? counts[c] : 0;
-        int32_t inc = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int32_t u = __shfl_up(inc, o);
-            if (lane >= o) inc += u;
-        }
-        if (c < c1) { counts[c] = carry + inc - cnt; ends[c] = carry + inc; }
+        const int32_t inc = wave_inclusive_add(cnt);
+        if (one) { cnt0 = cnt; inc0 = inc; }
+        else if (c < c1) { counts[c] = carry + inc - cnt; ends[c] = carry + inc; }
         carry += __shfl(inc, 63);
     }
     const int32_t n = carry;
@@ -320,11 +319,17 @@ __global__ __launch_bounds__(256) void k_learned_place(
     }
     mode = __shfl(mode, 0);
     base = (int64_t)__shfl((int32_t)base, 0);
-    if (mode != 2)
+    if (one) {
+        // a dropped path keeps its path-relative offsets, as a longer one does
+        const int64_t b = mode != 2 ? base : 0;
+        const uint32_t c = c0 + lane;
+        if (c < c1) { counts[c] = (int32_t)(b + inc0 - cnt0); ends[c] = (int32_t)(b + inc0); }
+    } else if (mode != 2) {
         for (uint32_t c = c0 + lane; c < c1; c += 64) {
             counts[c] = (int32_t)(counts[c] + base);
             ends[c] = (int32_t)(ends[c] + base);
         }
+    }
     // gaps: the unused arena tail (the whole arena after an overflow), and the part of a
     // dropped path's pool block that lies inside the budget
     const int64_t g0 = mode == 0 ? A + n : A, g1 = min(A + cap, budget);
