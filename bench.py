@@ -269,8 +269,9 @@ def main(argv=None):
         print(json.dumps({"copy_bytes": src.numel(), "copies": 5}))
         return 0
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build renders on one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path would shard).", file=sys.stderr)
+        print("bench.py: frames render on one GPU; several GPUs would each render a band of image "
+              "rows (or whole documents) and gather the bands once, which is not implemented.",
+              file=sys.stderr)
         return 2
 
     if a.workload == "documents_256":
