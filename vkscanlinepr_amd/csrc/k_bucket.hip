@@ -141,8 +141,9 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
 // Band starts: a band is a run of consecutive buckets of one path whose entries start in the
 // same 1024-entry window; a bucket of more than 1024 entries is a band of its own.  Every band
 // therefore holds at most 2048 entries (one sort tile) unless it is a single larger bucket.
-// One wave per path: the most recent non-empty bucket before each bucket comes from an inclusive
-// max-scan of (start << 32 | size) over the non-empty buckets (starts only grow along a path).
+// One wave per path: the most recent non-empty bucket before each bucket is the nearest lane
+// below it in the ballot of non-empty buckets (or, for the chunk's first ones, the last
+// non-empty bucket of the earlier chunks), read with one shuffle.
 __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict__ bucket_base, uint32_t Np,
                                                        const int32_t* __restrict__ boff,
                                                        int32_t* __restrict__ flag) {
@@ -150,35 +151,35 @@ __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict
     const int lane = threadIdx.x & 63;
     if (p >= Np) return;
     const int32_t b0 = bucket_base[p], b1 = bucket_base[p + 1];
-    unsigned long long carry = 0ull;  // (start + 1) << 32 | size of the last non-empty bucket; 0 = none
+    bool have = false;         // a non-empty bucket in an earlier chunk
+    int32_t cps = 0, cpz = 0;  // its start and size
     for (int32_t c = b0; c < b1; c += 64) {
         const int32_t b = c + lane;
         int32_t o = 0, sz = 0;
         if (b < b1) { o = boff[b]; sz = boff[b + 1] - o; }
-        const unsigned long long mine = sz > 0 ? (((unsigned long long)(uint32_t)(o + 1) << 32) | (uint32_t)sz) : 0ull;
-        unsigned long long inc = mine;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const unsigned long long u = __shfl_up(inc, d);
-            if (lane >= d && u > inc) inc = u;
-        }
-        unsigned long long prev = __shfl_up(inc, 1);
-        if (lane == 0) prev = 0ull;
-        if (carry > prev) prev = carry;
+        const unsigned long long ne = __ballot(sz > 0);
+        const unsigned long long below = ne & ((1ull << lane) - 1ull);
+        const int src = below ? 63 - __clzll((long long)below) : lane;
+        const int32_t ps_l = __shfl(o, src), pz_l = __shfl(sz, src);
         if (b < b1) {
             int32_t f = 0;
             if (sz > 0) {
-                if (prev == 0ull) {
+                const bool prev = below != 0ull || have;
+                if (!prev) {
                     f = 1;
                 } else {
-                    const int32_t ps = (int32_t)(prev >> 32) - 1, pz = (int32_t)(prev & 0xFFFFFFFFull);
+                    const int32_t ps = below ? ps_l : cps, pz = below ? pz_l : cpz;
                     f = (o >> 10) != (ps >> 10) || pz > 1024 || sz > 1024;
                 }
             }
             flag[b] = f;
         }
-        const unsigned long long last = __shfl(inc, 63);
-        if (last > carry) carry = last;
+        if (ne) {
+            const int last = 63 - __clzll((long long)ne);
+            cps = __shfl(o, last);
+            cpz = __shfl(sz, last);
+            have = true;
+        }
     }
 }
 
