@@ -67,6 +67,16 @@ def stage_bytes(P, C, Np, N, R, w, h):
     }
 
 
+def fold_mark_emit(stages, algo, io, N, R):
+    """Mark + emit fused into the sort: one stage, timed as the sort plus what is left between the
+    mark_emit events, rated with the algorithmic bytes of both, and with the fused kernel's own
+    I/O (key, index and winding change in, 12 B/entry; records out, 16 B/record)."""
+    stages["sort"] += stages["mark_emit"]
+    algo["sort"] += algo.pop("mark_emit")
+    io.pop("mark_emit")
+    io["sort"] = 12 * N + 16 * R
+
+
 def stage_io_bytes(P, C, Np, N, R, w, h):
     """Each kernel's own inputs read once and outputs written once, in the reference's data types
     (no bytes for the intermediate buffers a fused kernel never materialises): piece merge with
@@ -320,11 +330,7 @@ def main(argv=None):
     algo = stage_bytes(P, C, Np, N, R, canvas, canvas)
     io = stage_io_bytes(P, C, Np, N, R, canvas, canvas)
     if FUSED_MARK_EMIT:
-        # one fused stage: key, index and winding change in (12 B/entry), records out
-        stages["sort"] += stages["mark_emit"]
-        algo["sort"] += algo.pop("mark_emit")
-        io.pop("mark_emit")
-        io["sort"] = 12 * N + 16 * R
+        fold_mark_emit(stages, algo, io, N, R)
     table = {}
     for k, v in stages.items():
         gbs = algo[k] / (v * 1e-3) / 1e9 if v > 0 and k in algo else None

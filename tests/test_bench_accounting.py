@@ -1,0 +1,31 @@
+"""bench.py's byte accounting (host logic, no GPU): the algorithmic bytes per stage and the fused
+sort + mark/emit stage of frames that export no stage."""
+import sys
+from pathlib import Path
+
+import pytest
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+bench = pytest.importorskip("bench")
+
+
+def test_stage_bytes_follow_the_reference_types():
+    P, C, Np, N, R, w, h = 100, 30, 5, 1000, 700, 64, 32
+    algo = bench.stage_bytes(P, C, Np, N, R, w, h)
+    assert algo["transform"] == 20 * P + 4 * Np
+    assert algo["count"] == 72 * C
+    assert algo["sort"] == (16 + 12 + 8) * N          # sort, winding gather, winding scan
+    assert algo["mark_emit"] == (20 + 16 + 24) * N + 16 * R
+    assert algo["composite"] == 16 * R + 4 * w * h
+
+
+def test_fused_mark_emit_is_one_sort_stage():
+    N, R = 1000, 700
+    algo = bench.stage_bytes(1, 1, 1, N, R, 8, 8)
+    io = bench.stage_io_bytes(1, 1, 1, N, R, 8, 8)
+    stages = {"sort": 1.25, "mark_emit": 0.005}
+    bench.fold_mark_emit(stages, algo, io, N, R)
+    assert stages["sort"] == pytest.approx(1.255)
+    assert "mark_emit" not in algo and "mark_emit" not in io
+    assert algo["sort"] == 96 * N + 16 * R                # every reference stage the kernel replaces
+    assert io["sort"] == 12 * N + 16 * R
