@@ -94,12 +94,43 @@ def stage_io_bytes(P, C, Np, N, R, w, h):
     }
 
 
-# The intersection solve is bound by FP32 vector issue, not HBM: 24 bisection steps per crossing,
-# each a midpoint (add, mul), a cubic evaluated as six lerps of a sub, a mul and an add, and the
-# sign test (sub): 21 operations, no FMA (bit parity with the reference needs every rounding).
-SOLVE_FLOP_PER_CROSSING = 24 * 21
-# MI355X FP32 vector peak without FMA or packed math: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz.
-VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+# The intersection solve is bound by FP32 vector issue, not HBM: each crossing of a cubic costs 24
+# bisection steps, each a midpoint (add, mul), a cubic evaluated as six lerps of a sub, a mul and
+# an add, and the sign test (sub): 21 operations, no FMA (bit parity with the reference needs
+# every rounding).  Line crossings take one reciprocal product and are not counted.
+SOLVE_FLOP_PER_CUBIC_CROSSING = 24 * 21
+# MI355X published FP32 vector peak (FMA counted as two operations, packed FP32 as two lanes), and
+# the rate this kernel can reach without FMA or packed math (bit parity rules out both):
+# 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz.
+FP32_PEAK_TFLOPS = 157.3
+VALU_NOFMA_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def cubic_crossings(counts, cuts, curve_type):
+    """Crossings solved by bisection: each cubic's entries minus its pieces' start entries (a
+    monotone piece holds its start plus its x and y grid crossings)."""
+    n = np.diff(counts.astype(np.int64))
+    n_cuts = cuts.reshape(-1, 5)[:, 4].view(np.uint32).astype(np.int64)
+    cubic = curve_type == 4
+    return int(np.maximum(n[cubic] - (n_cuts[cubic] + 1), 0).sum())
+
+
+def rate_stages(stages, algo, io, traffic):
+    """Per stage: device time, the SURVEY-rule algorithmic rate (effective for fused kernels), the
+    rate on the kernels' own I/O and the measured HBM traffic rate, each as a fraction of the
+    8 TB/s peak.  Times are those of a frame with every kernel on one stream."""
+    table = {}
+    for k, ms in stages.items():
+        row = {"ms": round(ms, 4)}
+        for label, nbytes in (("algo", algo.get(k)), ("io", io.get(k)), ("traffic", (traffic or {}).get(k))):
+            if nbytes is None or ms <= 0:
+                row[f"{label}_GB/s"] = row[f"{label}_frac"] = None
+            else:
+                gbs = nbytes / (ms * 1e-3) / 1e9
+                row[f"{label}_GB/s"] = round(gbs, 1)
+                row[f"{label}_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+        table[k] = row
+    return table
 
 
 def cpu_baseline(kind, seed, n_paths, canvas, compose, budget_s=20.0):
@@ -244,15 +275,92 @@ def bench_documents(a) -> dict:
     }
 
 
-def measured_traffic(workload, mode, stage):
-    """HBM bytes per launch of a stage's kernel(s), from the committed rocprofv3 FETCH_SIZE +
-    WRITE_SIZE summary of this workload (already scaled by the copy-kernel calibration), or
-    None when no such measurement is committed."""
+def measured_traffic(workload, mode):
+    """HBM bytes per launch of each stage's kernels, from the committed rocprofv3 FETCH_SIZE +
+    WRITE_SIZE summary of this workload (already scaled by the copy-kernel calibration), or an
+    empty dict when no such measurement is committed."""
     f = ROOT / "profiles" / f"traffic_{workload}_{mode}.json"
     if not f.exists():
-        return None
-    d = json.loads(f.read_text()).get("stages", {}).get(stage)
-    return None if d is None else d.get("bytes_per_launch")
+        return {}
+    return {k: d.get("bytes_per_launch") for k, d in json.loads(f.read_text()).get("stages", {}).items()}
+
+
+def time_frames(r, steps, warmup):
+    """Wall-clock ms per frame (render + the frame's counters back on the host) and the mean
+    device time of the frames (HIP events on the handle's stream)."""
+    for _ in range(warmup):
+        r.render(readback=False)
+    r.synchronize()
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.render(readback=False)
+        dev.append(r.stats().frame_ms)  # waits for the frame
+    r.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3, float(np.mean(dev))
+
+
+def stage_table(r, steps, sizes, workload, mode):
+    """Per-stage device times of frames that run every kernel on the main stream (HIP events
+    between the stages), rated by algorithmic bytes, own I/O and measured traffic."""
+    P, C, Np, N, R, w, h = sizes
+    r.set_stage_timing(True)
+    acc = np.zeros(16)
+    for _ in range(steps):
+        r.render(readback=False)
+        acc += np.array(r.stats().stage_ms[:16])
+    serial_ms = float(r.stats().frame_ms)
+    r.set_stage_timing(False)
+    stages = dict(zip(STAGE_NAMES, acc[: len(STAGE_NAMES)] / steps))
+    # the reference's sort (with its winding gather and scan) is the bucket scatter, the band sort
+    # and the winding scan folded into it
+    stages["sort"] = stages.pop("bucket_scatter") + stages.pop("band_sort") + stages.pop("winding_scan")
+    algo = stage_bytes(P, C, Np, N, R, w, h)
+    io = stage_io_bytes(P, C, Np, N, R, w, h)
+    if FUSED_MARK_EMIT:
+        fold_mark_emit(stages, algo, io, N, R)
+        stages.pop("mark_emit")
+    return rate_stages(stages, algo, io, measured_traffic(workload, mode)), serial_ms, algo, io
+
+
+def bench_scan(a) -> dict:
+    """The standalone exclusive scan (the reference's naive_scan.comp) on n int32 in HBM, timed
+    with HIP events on the stream it runs on; 8 algorithmic bytes per element."""
+    from vkscanlinepr_amd import hip as H
+
+    n = a.scan_n
+    x = np.random.default_rng(6).integers(0, 1 << 10, n, dtype=np.int32)
+    xd, yd = H.DeviceBuffer(4 * n), H.DeviceBuffer(4 * (n + 1))
+    xd.upload(x)
+    st = H.Stream()
+    e0, e1 = H.Event(), H.Event()
+    r = V.Rasterizer(64, 64, device=0)
+    for _ in range(a.warmup):
+        r.scan_i32(xd.ptr, yd.ptr, n, st.handle)
+    st.synchronize()
+    t0 = time.perf_counter()
+    e0.record(st)
+    for _ in range(a.steps):
+        r.scan_i32(xd.ptr, yd.ptr, n, st.handle)
+    e1.record(st)
+    st.synchronize()
+    wall_ms = (time.perf_counter() - t0) / a.steps * 1e3
+    ms = e0.elapsed_ms(e1) / a.steps
+    # the timed output against numpy (the sums stay below 2^31 here)
+    ok = bool(np.array_equal(yd.download(np.int32, n + 1),
+                             np.concatenate([[0], np.cumsum(x, dtype=np.int64)]).astype(np.int32)))
+    r.close()
+    gbs = 8 * n / (ms * 1e-3) / 1e9
+    return {
+        "metric": "exclusive scan GB/s (int32, 8 B per element) vs 8 TB/s HBM peak",
+        "value": gbs, "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": wall_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic (seeded uniform 0..1023)",
+        "config": {"workload": "scan", "n": n, "device_ms_per_scan": ms, "matches_numpy_cumsum": ok},
+        "roofline": {"bound": "hbm", "kernel": "k_scan_i32", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": 8 * n, "kernel_ms": ms},
+    }
 
 
 def main(argv=None):
@@ -260,9 +368,14 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="illustration_4096", choices=sorted(WORKLOADS) + ["documents_256"])
+    ap.add_argument("--workload", default="illustration_4096",
+                    choices=sorted(WORKLOADS) + ["documents_256", "scan"])
     ap.add_argument("--autotune", action="store_true", help="documents_256: autotune the first group")
-    ap.add_argument("--mode", default="plain", choices=["plain", "learned"])
+    ap.add_argument("--mode", default="both", choices=["both", "plain", "learned"],
+                    help="both: the plain two-pass frame (value) and the learned single-pass frame")
+    ap.add_argument("--train-paths", type=int, default=50000,
+                    help="learned: paths of the training scene (same generator, seed 4)")
+    ap.add_argument("--scan-n", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stages", action="store_true", help="print per-stage times to stderr")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -287,79 +400,59 @@ def main(argv=None):
     if a.workload == "documents_256":
         print(json.dumps(bench_documents(a)))
         return 0
+    if a.workload == "scan":
+        print(json.dumps(bench_scan(a)))
+        return 0
     kind, seed, n_paths, canvas, compose = WORKLOADS[a.workload]
     scene = V.Scene.generate(kind, seed, n_paths, canvas)
-    mode = V.MODE_LEARNED if a.mode == "learned" else V.MODE_PLAIN
-    r = V.Rasterizer(canvas, canvas, compose=compose, mode=mode, device=0)
+    primary = "learned" if a.mode == "learned" else "plain"
     learned = None
-    if mode == V.MODE_LEARNED:
-        # the predictor is trained here, on the GPU, on 50,000 paths of the same generator
-        # (another seed) counted by the plain method; not part of the timed region
-        train = V.Scene.generate(kind, 4, 50000, canvas)
-        learned = {"train": r.learned_train(train, chunk_paths=5000)}
+    rl = None
+    if a.mode in ("both", "learned"):
+        # the predictor is trained here, on the GPU, on paths of the same generator (another
+        # seed) counted by the plain method; not part of any timed region
+        rl = V.Rasterizer(canvas, canvas, compose=compose, mode=V.MODE_LEARNED, device=0)
+        train = V.Scene.generate(kind, 4, a.train_paths, canvas)
+        learned = {"train": rl.learned_train(train, chunk_paths=5000)}
         del train
-    r.load_scene(scene)
-    for _ in range(a.warmup):
-        r.render(readback=False)
-    r.synchronize()
-
-    # timed frames: no per-stage events (learned frames are then replayed as captured graphs)
-    frame_ms = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        r.render(readback=False)
-        s = r.stats()  # synchronises: the frame's device time from HIP events on the stream
-        frame_ms.append(s.frame_ms)
-    r.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall / a.steps * 1e3
+        rl.load_scene(scene)
+    rp = None
+    if a.mode in ("both", "plain"):
+        rp = V.Rasterizer(canvas, canvas, compose=compose, mode=V.MODE_PLAIN, device=0)
+        rp.load_scene(scene)
+        plain_ms, plain_dev = time_frames(rp, a.steps, a.warmup)
+    r = rp if rp is not None else rl
+    if rl is not None:
+        learned_ms, learned_dev = time_frames(rl, a.steps, a.warmup)
+    ms_per_step, dev_ms = (plain_ms, plain_dev) if primary == "plain" else (learned_ms, learned_dev)
     st = r.stats()
     P, C, Np = st.n_points, st.n_curves, st.n_paths
     N, R = st.n_intersections, st.n_cells + st.n_spans
-    # per-stage device times: the same frames again with an event between stages (eager)
-    r.set_stage_timing(True)
-    stage_acc = np.zeros(16)
-    for _ in range(a.steps):
-        r.render(readback=False)
-        stage_acc += np.array(r.stats().stage_ms[:16])
-    r.set_stage_timing(False)
-    stages = dict(zip(STAGE_NAMES, stage_acc[: len(STAGE_NAMES)] / a.steps))
-    # the sort of the reference (with its winding gather and scan) is the bucket scatter, the
-    # band sort and the winding scan folded into it
-    stages["sort"] = stages["bucket_scatter"] + stages["band_sort"] + stages["winding_scan"]
-    algo = stage_bytes(P, C, Np, N, R, canvas, canvas)
-    io = stage_io_bytes(P, C, Np, N, R, canvas, canvas)
-    if FUSED_MARK_EMIT:
-        fold_mark_emit(stages, algo, io, N, R)
-    table = {}
-    for k, v in stages.items():
-        gbs = algo[k] / (v * 1e-3) / 1e9 if v > 0 and k in algo else None
-        iob = io[k] / (v * 1e-3) / 1e9 if v > 0 and k in io else None
-        table[k] = {"ms": round(v, 4), "GB/s": None if gbs is None else round(gbs, 1),
-                    "frac": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4),
-                    "io_GB/s": None if iob is None else round(iob, 1)}
-        if a.stages:
-            print(f"  {k:16s} {v:8.3f} ms  {gbs or 0:8.1f} GB/s  (own I/O {iob or 0:8.1f} GB/s)", file=sys.stderr)
-    # dominant kernel: the longest stage; the solve is rated against the FP32 vector peak
-    timed = [k for k in stages if k in algo and k != "sort"]
-    dom = max(timed, key=lambda k: stages[k])
-    if dom == "chain_solve":
-        tops = SOLVE_FLOP_PER_CROSSING * N / (stages[dom] * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": dom, "achieved": tops, "peak": VALU_PEAK_TOPS,
-                "unit": "TFLOP/s", "frac": tops / VALU_PEAK_TOPS, "traffic": measured_traffic(a.workload, a.mode, dom),
-                "algorithmic_flop": SOLVE_FLOP_PER_CROSSING * N, "kernel_ms": stages[dom]}
-    else:
-        achieved = algo[dom] / (stages[dom] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": measured_traffic(a.workload, a.mode, dom),
-                "algorithmic_bytes": algo[dom], "kernel_ms": stages[dom]}
-    # the longest HBM-bound stage, rated against HBM
-    hb = max((k for k in algo if k in stages and k != "chain_solve"), key=lambda k: stages[k])
-    hb_gbs = algo[hb] / (stages[hb] * 1e-3) / 1e9
-    roof_hbm = {"bound": "hbm", "kernel": hb, "achieved": hb_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": hb_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(a.workload, a.mode, hb),
-                "algorithmic_bytes": algo[hb], "kernel_ms": stages[hb]}
+    sizes = (P, C, Np, N, R, canvas, canvas)
+    table, serial_ms, algo, io = stage_table(r, a.steps, sizes, a.workload, primary)
+    if a.stages:
+        for k, v in table.items():
+            print(f"  {k:16s} {v['ms']:8.3f} ms  algo {v['algo_GB/s'] or 0:8.1f}  io {v['io_GB/s'] or 0:8.1f}  "
+                  f"traffic {v['traffic_GB/s'] or 0:8.1f} GB/s", file=sys.stderr)
+    traffic = measured_traffic(a.workload, primary)
+    # the intersection solve: FP32 vector work of the cubic crossings
+    xc = cubic_crossings(r.export("counts"), r.export("cuts"), scene.arrays()["curve_type"])
+    solve_ms = table["chain_solve"]["ms"]
+    tflops = SOLVE_FLOP_PER_CUBIC_CROSSING * xc / (solve_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "kernel": "k_chain_solve (both launches)", "achieved": tflops,
+            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / FP32_PEAK_TFLOPS,
+            "peak_no_fma_no_packed": VALU_NOFMA_PEAK_TOPS, "frac_no_fma_no_packed": tflops / VALU_NOFMA_PEAK_TOPS,
+            "traffic": traffic.get("chain_solve"), "algorithmic_flop": SOLVE_FLOP_PER_CUBIC_CROSSING * xc,
+            "cubic_crossings": xc, "kernel_ms": solve_ms}
+    # the longest HBM-bound stage, rated on its own I/O; the SURVEY-rule figure (every reference
+    # stage a fused kernel replaces, buffers it never touches included) is labelled apart
+    hb = max((k for k in table if k in io and k != "chain_solve"), key=lambda k: table[k]["ms"])
+    hb_ms = table[hb]["ms"]
+    roof_hbm = {"bound": "hbm", "kernel": hb, "achieved": io[hb] / (hb_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": io[hb] / (hb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "io_bytes": io[hb],
+                "traffic": traffic.get(hb), "kernel_ms": hb_ms,
+                "algorithmic_equivalent_GB/s": algo[hb] / (hb_ms * 1e-3) / 1e9,
+                "algorithmic_equivalent_bytes": algo[hb]}
     out = {
         "metric": "ms/frame & Mpaths/s at 4096^2 (learned 1-pass vs plain 2-pass); HBM GB/s",
         "value": Np / (ms_per_step * 1e-3) / 1e6,
@@ -368,43 +461,43 @@ def main(argv=None):
         "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+int32", "data": "synthetic (seeded generator)",
-        "config": {"workload": a.workload, "mode": a.mode, "paths": Np, "curves": C,
+        "config": {"workload": a.workload, "mode": primary, "paths": Np, "curves": C,
                    "points": P, "intersections": N, "records": R, "canvas": canvas,
                    "compose": "src_over" if compose else "overwrite",
-                   "device_ms_per_frame": float(np.mean(frame_ms))},
+                   "device_ms_per_frame": dev_ms, "serialized_frame_ms": serial_ms,
+                   "routes": {"wave_chains": st.n_wave_chains, "heavy_pieces": st.n_heavy_pieces,
+                              "long_bands": st.n_long_bands, "big_bands": st.n_big_bands}},
         "roofline": roof,
         "roofline_hbm": roof_hbm,
         "stages": table,
     }
-    if learned is not None:
-        # the plain two-pass form on the same scene, same process, same step count
-        p = V.Rasterizer(canvas, canvas, compose=compose, mode=V.MODE_PLAIN, device=0)
-        p.load_scene(scene)
-        for _ in range(a.warmup):
-            p.render(readback=False)
-        p.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(a.steps):
-            p.render(readback=False)
-            p.stats()
-        p.synchronize()
-        plain_ms = (time.perf_counter() - t1) / a.steps * 1e3
-        same = bool(np.array_equal(p.render(), r.render()))
-        visible = int(np.sum(r.export("features").reshape(-1, 16)[:, 12] > 0))
+    if rl is not None:
+        sl = rl.stats()
+        il = rl.render()
+        visible = int(np.sum(rl.export("features").reshape(-1, 16)[:, 12] > 0))
         learned.update({
-            "plain_ms_per_step": plain_ms, "speedup_vs_plain": plain_ms / ms_per_step,
-            "overflow_paths": int(st.n_overflow_paths), "visible_paths": visible,
-            "overflow_rate": st.n_overflow_paths / max(1, visible),
-            "alloc_intersections": int(st.alloc_intersections),
-            "over_allocation": st.alloc_intersections / max(1, N) - 1.0,
-            "fallback_frames": int(r.stats().learned_fallbacks), "image_identical_to_plain": same,
+            "ms_per_step": learned_ms, "device_ms_per_frame": learned_dev,
+            "Mpaths/s": Np / (learned_ms * 1e-3) / 1e6,
+            "overflow_paths": int(sl.n_overflow_paths), "visible_paths": visible,
+            "overflow_rate": sl.n_overflow_paths / max(1, visible),
+            "alloc_intersections": int(sl.alloc_intersections),
+            "over_allocation": sl.alloc_intersections / max(1, N) - 1.0,
+            "budget_readback_frames": int(sl.learned_fallbacks),
         })
+        if rp is not None:
+            learned["plain_ms_per_step"] = plain_ms
+            learned["speedup_vs_plain"] = plain_ms / learned_ms
+            learned["image_identical_to_plain"] = bool(np.array_equal(rp.render(), il))
+            out["config"]["plain_ms_per_step"] = plain_ms
+            out["config"]["learned_ms_per_step"] = learned_ms
+            out["config"]["learned_speedup_vs_plain"] = plain_ms / learned_ms
         out["learned"] = learned
-        p.close()
+        rl.close()
+    if rp is not None:
+        rp.close()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(kind, seed, n_paths, canvas, compose)
     print(json.dumps(out))
-    r.close()
     return 0
 
 

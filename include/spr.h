@@ -69,6 +69,12 @@ typedef struct spr_stats {
     double stage_ms[16];       /* per-stage device time when timing is enabled */
     int64_t learned_fallbacks; /* LEARNED: frames that read their budget bound back to the host
                                   (the first frame after a load, MVP or model change) */
+    /* Which kernel routes the last frame took (each route has its own kernel or mode): */
+    int64_t n_wave_chains;     /* crossing chains of 1024 or more, each solved by a whole wave */
+    int64_t n_heavy_pieces;    /* monotone pieces merged by a 256-thread workgroup each (more than
+                                  1024 entries, or waiting on a chain of 512 or more crossings) */
+    int64_t n_long_bands;      /* sort bands of more than 512 words (256-thread LDS sorts) */
+    int64_t n_big_bands;       /* sort bands of more than 2048 words (global merge-path sort) */
 } spr_stats;
 
 /* Debug stage identifiers for spr_debug_export / spr_debug_inject. */
@@ -140,6 +146,15 @@ int spr_synchronize(spr_handle* h);
 /* Enables per-stage HIP-event timing (adds event records between stages). */
 int spr_set_stage_timing(spr_handle* h, int enable);
 int spr_get_stats(const spr_handle* h, spr_stats* out);
+
+/* --- the reference's standalone exclusive scan (shaders/common/naive_scan.comp:27-73, called at
+ *     src/core/scanline/scanline_rasterizer.cpp:337-358, :478-506 and :544-573) on device memory:
+ *     d_out[i] = d_in[0] + ... + d_in[i-1] for i <= n, so d_out holds n + 1 int32 and its last
+ *     value is the total (d_out may equal d_in: the scan then runs in place, as the reference's
+ *     does).  Sums wrap modulo 2^32, as the reference's int arithmetic does.  n <= 2^31 - 2.
+ *     Runs on `stream` (a hipStream_t; NULL = the handle's stream) and returns without waiting;
+ *     the handle's scan workspace grows on demand, so calls on one handle must not overlap. */
+int spr_scan_i32(spr_handle* h, const int32_t* d_in, int32_t* d_out, uint64_t n, void* stream);
 
 /* --- debug access to intermediate buffers (the reference's drawDebug dumps,
  *     src/core/scanline/scanline_rasterizer.cpp:698-802).  export copies the last frame's stage

@@ -130,6 +130,36 @@ def test_forced_overflow_and_generous_arenas(spr, trained):
     p.close()
 
 
+def test_margin_change_after_capture(spr, trained):
+    """A new margin after the frame was captured as a graph: the next frames use it (the graph
+    and the budgets are rebuilt), and stay identical to the plain frame."""
+    r, _, path = trained
+    scene = spr.Scene.generate(2, 3, 400, 1024)
+    p = spr.Rasterizer(1024, 1024, compose=spr.COMPOSE_SRC_OVER)
+    p.load_scene(scene)
+    ip = p.render()
+    p.close()
+    q = spr.Rasterizer(1024, 1024, compose=spr.COMPOSE_SRC_OVER, mode=spr.MODE_LEARNED)
+    q.learned_load(path)
+    q.load_scene(scene)
+    for frame in range(3):  # eager, eager + capture, replay
+        assert np.array_equal(q.render(), ip), frame
+    n0 = q.stats().n_overflow_paths
+    visible = int(np.sum(q.export("features").reshape(-1, 16)[:, 12] > 0))
+    assert n0 < visible
+    for margin, slack in ((-20.0, 0), (4.0, 64)):
+        print(f"[learned] margin change to {margin}", flush=True)
+        q.learned_set_margin(margin, slack)
+        for frame in range(3):
+            assert np.array_equal(q.render(), ip), (margin, frame)
+            n = q.stats().n_overflow_paths
+            if margin < -10:
+                assert n >= 0.95 * visible  # every visible path overflows its (empty) capacity
+            else:
+                assert n <= n0
+    q.close()
+
+
 @pytest.mark.skip(reason="this case (a learned frame whose budget was exceeded, so that the device "
                          "dropped paths and the host redid the frame with the plain method) faulted "
                          "the GPU twice; that code path is removed (budgets are now a proven upper "

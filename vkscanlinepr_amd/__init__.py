@@ -52,7 +52,8 @@ class Stats(C.Structure):
                 ("n_intersections", C.c_int64), ("n_cells", C.c_int64), ("n_spans", C.c_int64),
                 ("n_overflow_paths", C.c_int64), ("alloc_intersections", C.c_int64),
                 ("frame_ms", C.c_double), ("stage_ms", C.c_double * 16),
-                ("learned_fallbacks", C.c_int64)]
+                ("learned_fallbacks", C.c_int64), ("n_wave_chains", C.c_int64),
+                ("n_heavy_pieces", C.c_int64), ("n_long_bands", C.c_int64), ("n_big_bands", C.c_int64)]
 
 
 class TrainOpts(C.Structure):
@@ -91,7 +92,7 @@ EXPORTS = [
     "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free", "spr_learned_train",
     "spr_learned_save", "spr_learned_load", "spr_learned_set_margin", "spr_mlp_get_params",
     "spr_mlp_set_params", "spr_mlp_predict", "spr_mlp_gradients", "spr_load_batch", "spr_batch_image",
-    "spr_autotune",
+    "spr_autotune", "spr_scan_i32",
 ]
 
 
@@ -139,6 +140,7 @@ def lib() -> C.CDLL:
     L.spr_load_batch.argtypes = [vp, C.POINTER(vp), u32p, u32p, C.c_uint32]
     L.spr_batch_image.argtypes = [vp, C.c_uint32, C.c_void_p, C.c_size_t]
     L.spr_autotune.argtypes = [vp, C.c_uint32, C.POINTER(TuneReport)]
+    L.spr_scan_i32.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
     _lib = L
     return L
 
@@ -303,6 +305,12 @@ class Rasterizer:
         self._c(lib().spr_debug_export(self._h, sid, out.ctypes.data_as(C.c_void_p), n.value,
                                        C.byref(n)))
         return out
+
+    def scan_i32(self, d_in: int, d_out: int, n: int, stream: int | None = None) -> None:
+        """Exclusive prefix sum of n int32 at device address d_in into d_out (n + 1 values, the
+        total last), enqueued on `stream` (a hipStream_t handle; None = the handle's stream)."""
+        self._c(lib().spr_scan_i32(self._h, C.c_void_p(d_in), C.c_void_p(d_out), int(n),
+                                   C.c_void_p(stream) if stream else None))
 
     def inject_intersections(self, inter) -> None:
         if inter is None:

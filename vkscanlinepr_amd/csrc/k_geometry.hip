@@ -19,6 +19,13 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
         p[i] = v;
 }
 
+// Frame summary: dst[i] = *src.p[i] (0 for a null source), one lane per value, so that one
+// device-to-host copy returns every counter of the frame.
+__global__ void k_gather_summary(SummarySrc src, int32_t* __restrict__ dst) {
+    const int i = threadIdx.x;
+    if (i < src.n) dst[i] = src.p[i] ? *src.p[i] : 0;
+}
+
 // Fill of min(n_max, mult * *n_dev + add) words (a count known only on the device).
 __global__ void k_fill_u32_dev(uint32_t* __restrict__ p, int64_t n_max, const int32_t* __restrict__ n_dev,
                                int64_t mult, int64_t add, uint32_t v) {
@@ -210,6 +217,12 @@ hipError_t launch_fill_u32_dev(uint32_t* p, int64_t n_max, const int32_t* n_dev,
     int64_t g = (n_max + kBlock - 1) / kBlock;
     if (g > 4096) g = 4096;
     k_fill_u32_dev<<<(unsigned)g, kBlock, 0, s>>>(p, n_max, n_dev, mult, add, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_summary(const SummarySrc& src, int32_t* dst, hipStream_t s) {
+    if (src.n <= 0 || src.n > kSummaryMax) return hipErrorInvalidValue;
+    k_gather_summary<<<1, 64, 0, s>>>(src, dst);
     return hipGetLastError();
 }
 

@@ -23,6 +23,14 @@ inline int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_u32_dev(uint32_t* p, int64_t n_max, const int32_t* n_dev, int64_t mult, int64_t add,
                                uint32_t v, hipStream_t s);
+// One device-to-host copy per frame: the frame's counters gathered from wherever the kernels
+// left them into one contiguous int32 array.
+constexpr int kSummaryMax = 24;
+struct SummarySrc {
+    const int32_t* p[kSummaryMax];
+    int n;
+};
+hipError_t launch_gather_summary(const SummarySrc& src, int32_t* dst, hipStream_t s);
 
 // ---- geometry (a3, a4, a7, a8)
 hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,

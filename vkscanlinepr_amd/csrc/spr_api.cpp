@@ -73,6 +73,15 @@ struct DBuf {
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Frame summary slots (int32), gathered on the device at the end of every frame and read back
+// with one copy: record totals, the error flags, the kernel routes taken, and the learned frame's
+// allocation counters.
+enum {
+    kSumCells = 0, kSumSpans, kSumErr, kSumWaveChains, kSumHeavyPieces, kSumLongBands, kSumBigBands,
+    kSumPool, kSumOverflow, kSumLive, kSumArena, kSumSorted, kSumCapR, kSumCount
+};
+static_assert(kSumCount <= spr::kSummaryMax, "summary slots");
+
 }  // namespace
 
 struct spr_handle {
@@ -124,7 +133,11 @@ struct spr_handle {
     DBuf<int2> prow;
     DBuf<uint8_t> image;
     DBuf<unsigned long long> zero;  // per-frame zeroed control region
-    int32_t* h_pinned = nullptr;    // 16 ints of pinned host memory
+    int32_t* h_pinned = nullptr;    // 64 ints of pinned host memory: [0, 16) mid-frame readbacks,
+                                    // [16, 16 + kSummaryMax) the last frame's summary
+    DBuf<int32_t> summary;          // the frame summary on the device, copied to h_pinned + 16
+    DBuf<unsigned long long> scan_ws;  // spr_scan_i32: tile statuses and the tile counter
+    int32_t sum(int k) const { return h_pinned[16 + k]; }
 
     // injection
     std::vector<int32_t> inj_inter;
@@ -206,8 +219,10 @@ ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
 // 2048 or more entries on average (then the long bands are most of the sort and the short ones
 // fit beside them); with small paths the short bands are the whole sort and the fork and join
 // would only add their latency.
+bool serial(const spr_handle* H) { return H->stage_timing && !H->capturing; }
+
 spr::SideStream side_for(spr_handle* H, int64_t n_entries, uint32_t Np) {
-    if (Np == 0 || n_entries / (int64_t)Np < 2048) return {nullptr, nullptr, nullptr};
+    if (serial(H) || Np == 0 || n_entries / (int64_t)Np < 2048) return {nullptr, nullptr, nullptr};
     return {H->stream2, H->ev_sfork, H->ev_sjoin};
 }
 
@@ -217,6 +232,12 @@ spr::ScanCtl ctl_at(spr_handle* H, size_t st, size_t ctr_word, int counter) {
     c.counter = reinterpret_cast<uint32_t*>(H->zero.p + ctr_word) + counter;
     c.epoch = 0;
     return c;
+}
+
+// Gathers the frame's counters and starts their copy to the host (read once the stream is idle).
+void finish_summary(spr_handle* H, const spr::SummarySrc& S, hipStream_t s) {
+    SPR_HIP(spr::launch_gather_summary(S, H->summary.p, s));
+    SPR_HIP(hipMemcpyAsync(H->h_pinned + 16, H->summary.p, (size_t)S.n * 4, hipMemcpyDeviceToHost, s));
 }
 
 void stage_mark(spr_handle* H, int idx) {
@@ -232,7 +253,9 @@ void stage_mark(spr_handle* H, int idx) {
 // and 6 are the caller's; mark 5 is set when both solves are done.
 void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragArgs& A, int64_t n_entries,
                       int n_cu_solve, int n_cu_merge) {
-    hipStream_t s = H->stream, s2 = H->stream2;
+    // with per-stage timing every kernel runs on the main stream, so that the events between
+    // stages bracket exactly the device time of each stage's own kernels
+    hipStream_t s = H->stream, s2 = serial(H) ? H->stream : H->stream2;
     const int32_t* d_coff = d_cc + spr::kCcOff;
     const int32_t* d_nlong = d_cc + spr::kCcNLong;
     const int32_t* d_lb = d_cc + spr::kCcLightBegin;
@@ -243,13 +266,14 @@ void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragA
     SPR_HIP(hipStreamWaitEvent(s2, H->ev_fork, 0));
     SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 0,
                                     d_cc + spr::kCcWorkLight, H->chain_t.p, H->mono.p, n_cu_solve, s2));
+    if (s2 == s) stage_mark(H, 5);  // serial: both solves, then every piece kernel
     SPR_HIP(hipEventRecord(H->ev_light, s2));
     SPR_HIP(spr::launch_piece_small(H->pieces.p, C, H->mono.p, H->chain_t.p, L, A, n_cu_merge, s2));
     SPR_HIP(hipEventRecord(H->ev_small, s2));
     SPR_HIP(spr::launch_piece_lo(H->pieces.p, C, H->mono.p, H->chain_t.p, L.lo, L.lo_cnt, A, s2));
     SPR_HIP(hipEventRecord(H->ev_join, s2));
     SPR_HIP(hipStreamWaitEvent(s, H->ev_light, 0));
-    stage_mark(H, 5);
+    if (s2 != s) stage_mark(H, 5);
     SPR_HIP(hipStreamWaitEvent(s, H->ev_small, 0));
     // pieces longer than the LDS stage merge through the sort buffer, which is free until the scatter
     SPR_HIP(spr::launch_piece_heavy(H->pieces.p, L, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p, n_entries, A,
@@ -445,8 +469,16 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     SPR_HIP(spr::launch_composite(H->crec.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
                                   (int)H->opts.compose, H->image.p, s, (int64_t)(2 * n1)));
     stage_mark(H, 12);
-    SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
-    SPR_HIP(hipMemcpyAsync(H->h_pinned + 8, d_err, 4, hipMemcpyDeviceToHost, s));
+    spr::SummarySrc S{};
+    S.n = kSumCount;
+    S.p[kSumCells] = d_totals;
+    S.p[kSumSpans] = d_totals + 1;
+    S.p[kSumErr] = d_err;
+    S.p[kSumWaveChains] = gen ? d_nlong : nullptr;
+    S.p[kSumHeavyPieces] = gen ? d_cc + spr::kCcHeavyPieces : nullptr;
+    S.p[kSumLongBands] = reinterpret_cast<const int32_t*>(ctl_at(H, z.st_wind, z.ctr, 4).counter + 1);
+    S.p[kSumBigBands] = d_bigcount;
+    finish_summary(H, S, s);
     SPR_HIP(hipEventRecord(H->ev_end, s));
     H->frame_done = true;
 }
@@ -703,12 +735,22 @@ void run_learned_frame(spr_handle* H) {
                                   (int)H->opts.compose, H->image.p, s, rec_cap, H->batch ? H->docs.p : nullptr,
                                   (int)H->n_docs, H->total_rows));
     stage_mark(H, 12);
-    SPR_HIP(hipMemcpyAsync(H->h_pinned + 4, d_totals, 12, hipMemcpyDeviceToHost, s));
-    SPR_HIP(hipMemcpyAsync(H->h_pinned + 8, d_err, 4, hipMemcpyDeviceToHost, s));
-    SPR_HIP(hipMemcpyAsync(H->h_pinned + 9, d_ctl, 12, hipMemcpyDeviceToHost, s));
-    SPR_HIP(hipMemcpyAsync(H->h_pinned + 12, H->arena.p + Np, 4, hipMemcpyDeviceToHost, s));
-    SPR_HIP(hipMemcpyAsync(H->h_pinned + 13, d_nsorted, 4, hipMemcpyDeviceToHost, s));
-    SPR_HIP(hipMemcpyAsync(H->h_pinned + 14, d_capr, 4, hipMemcpyDeviceToHost, s));
+    spr::SummarySrc S{};
+    S.n = kSumCount;
+    S.p[kSumCells] = d_totals;
+    S.p[kSumSpans] = d_totals + 1;
+    S.p[kSumErr] = d_err;
+    S.p[kSumWaveChains] = C ? d_nlong : nullptr;
+    S.p[kSumHeavyPieces] = C ? d_cc + spr::kCcHeavyPieces : nullptr;
+    S.p[kSumLongBands] = reinterpret_cast<const int32_t*>(ctl_at(H, z.st_wind, z.counters, 6).counter + 1);
+    S.p[kSumBigBands] = d_bigcount;
+    S.p[kSumPool] = d_ctl;
+    S.p[kSumOverflow] = d_ctl + 1;
+    S.p[kSumLive] = d_ctl + 2;
+    S.p[kSumArena] = H->arena.p + Np;
+    S.p[kSumSorted] = d_nsorted;
+    S.p[kSumCapR] = d_capr;
+    finish_summary(H, S, s);
     if (H->capturing) return;
     SPR_HIP(hipEventRecord(H->ev_end, s));
     learned_finish(H);
@@ -718,27 +760,27 @@ void run_learned_frame(spr_handle* H) {
 // keep the exact budgets of this scene for the next frames.
 void learned_finish(spr_handle* H) {
     SPR_HIP(hipStreamSynchronize(H->stream));
-    const int32_t err = H->h_pinned[8];
-    const int64_t caps = std::max<int32_t>(H->h_pinned[12], 0), pool = std::max<int32_t>(H->h_pinned[9], 0);
-    const int64_t r_need = (int64_t)H->h_pinned[4] + H->h_pinned[5];
-    H->stats.n_overflow_paths = H->h_pinned[10];
+    const int32_t err = H->sum(kSumErr);
+    const int64_t caps = std::max<int32_t>(H->sum(kSumArena), 0), pool = std::max<int32_t>(H->sum(kSumPool), 0);
+    const int64_t r_need = (int64_t)H->sum(kSumCells) + H->sum(kSumSpans);
+    H->stats.n_overflow_paths = H->sum(kSumOverflow);
     H->stats.alloc_intersections = caps;
     H->frame_done = true;
     H->last_learned = true;
     H->last_brec = H->brec_l.p;
     H->last_fused = !H->opts.debug && H->w <= spr::kNarrowMaxW;  // as the frame's narrow words
     if (err & 1) {
-        H->h_pinned[8] = 0;
+        H->h_pinned[16 + kSumErr] = 0;
         throw Fail{SPR_ERR_DEVICE, "internal: an entry fell outside its path's row bound"};
     }
     if (err & 6) {
         // cannot happen with bounded budgets; reported instead of rendered
-        H->h_pinned[8] = 0;
+        H->h_pinned[16 + kSumErr] = 0;
         H->lb_dirty = true;
         throw Fail{SPR_ERR_DEVICE, "internal: learned frame exceeded its bounded budget"};
     }
-    H->N = H->h_pinned[13];
-    H->n_live = H->h_pinned[11];
+    H->N = H->sum(kSumSorted);
+    H->n_live = H->sum(kSumLive);
     // the same scene, MVP and model give the same counts: later frames need exactly this much
     H->lb_n = caps + pool + std::max<int64_t>((caps + pool) / 64, 1024);
     H->lb_r = r_need + std::max<int64_t>(r_need / 64, 1024);
@@ -819,12 +861,16 @@ void sync_stats(spr_handle* H) {
     H->stats.n_curves = H->C;
     H->stats.n_paths = H->Np;
     H->stats.n_intersections = H->N;
-    if (H->h_pinned[8] != 0) {
-        H->h_pinned[8] = 0;
+    if (H->sum(kSumErr) != 0) {
+        H->h_pinned[16 + kSumErr] = 0;
         throw Fail{SPR_ERR_DEVICE, "internal: an entry fell outside its path's row bound"};
     }
-    H->stats.n_cells = H->h_pinned[4];
-    H->stats.n_spans = H->h_pinned[5];
+    H->stats.n_cells = H->sum(kSumCells);
+    H->stats.n_spans = H->sum(kSumSpans);
+    H->stats.n_wave_chains = H->sum(kSumWaveChains);
+    H->stats.n_heavy_pieces = H->sum(kSumHeavyPieces);
+    H->stats.n_long_bands = H->sum(kSumLongBands);
+    H->stats.n_big_bands = H->sum(kSumBigBands);
     H->stats.learned_fallbacks = H->learned_fallbacks;
     float ms = 0.0f;
     SPR_HIP(hipEventElapsedTime(&ms, H->ev_begin, H->ev_end));
@@ -926,8 +972,9 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
         SPR_HIP(hipEventCreate(&H->ev_end));
         H->ev_stage.resize(16);
         for (auto& e : H->ev_stage) SPR_HIP(hipEventCreate(&e));
-        SPR_HIP(hipHostMalloc((void**)&H->h_pinned, 64, hipHostMallocDefault));
-        std::memset(H->h_pinned, 0, 64);
+        SPR_HIP(hipHostMalloc((void**)&H->h_pinned, 256, hipHostMallocDefault));
+        std::memset(H->h_pinned, 0, 256);
+        H->summary.ensure(spr::kSummaryMax);
         const float id[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
         std::memcpy(H->mvp.m, id, sizeof(id));
         H->image.ensure((size_t)4 * width * height);
@@ -956,6 +1003,8 @@ int spr_destroy(spr_handle* H) {
     H->pcurve.release(); H->feat.release(); H->pred.release(); H->train_feat.release(); H->train_tgt.release();
     H->arena.release(); H->capn.release(); H->nexact.release(); H->pbase.release(); H->sstart.release();
     H->hist_l.release(); H->fill_l.release(); H->bsum_l.release(); H->brec_l.release(); H->cend.release(); H->pbound.release();
+    H->summary.release();
+    H->scan_ws.release();
     if (H->h_pinned) (void)hipHostFree(H->h_pinned);
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
@@ -1054,6 +1103,29 @@ int spr_get_stats(const spr_handle* Hc, spr_stats* out) {
     });
 }
 
+int spr_scan_i32(spr_handle* H, const int32_t* d_in, int32_t* d_out, uint64_t n, void* stream) {
+    if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (!d_out || (n && !d_in)) throw Fail{SPR_ERR_INVALID, "null device pointer"};
+        if (n > (uint64_t)INT32_MAX - 1) throw Fail{SPR_ERR_CAPACITY, "scan length exceeds 2^31 - 2"};
+        SPR_HIP(hipSetDevice(H->device));
+        hipStream_t s = stream ? (hipStream_t)stream : H->stream;
+        const int64_t tiles = spr::scan_tiles(std::max<int64_t>((int64_t)n, 1));
+        if ((size_t)tiles + 1 > H->scan_ws.cap) {
+            // the old workspace may still be in use by a scan queued on another stream
+            SPR_HIP(hipDeviceSynchronize());
+            H->scan_ws.ensure((size_t)tiles + 1);
+        }
+        // tile statuses and the tile counter start at zero (a fill kernel: capturable)
+        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->scan_ws.p, 2 * (tiles + 1), 0u, s));
+        spr::ScanCtl c;
+        c.status = H->scan_ws.p;
+        c.counter = reinterpret_cast<uint32_t*>(H->scan_ws.p + tiles);
+        c.epoch = 0;
+        SPR_HIP(spr::launch_scan_i32(d_in, d_out, (int64_t)n, c, s));
+    });
+}
+
 int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nbytes) {
     if (!H || !nbytes) { g_error = "null argument"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
@@ -1125,7 +1197,7 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 break;
             case SPR_STAGE_RECORDS: {
                 if (!debug) throw Fail{SPR_ERR_STATE, "full records are kept only with the debug option"};
-                int64_t R = (int64_t)H->h_pinned[4] + H->h_pinned[5];
+                int64_t R = (int64_t)H->sum(kSumCells) + H->sum(kSumSpans);
                 from_dev(H->records.p, (size_t)R * 16);
                 break;
             }
@@ -1379,6 +1451,9 @@ int spr_learned_set_margin(spr_handle* H, float log2_margin, int32_t slack) {
     if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
         if (!(log2_margin > -30.0f && log2_margin < 30.0f) || slack < 0) throw Fail{SPR_ERR_INVALID, "bad margin"};
+        // the capacities change: the budgets and any captured frame (whose kernels hold the old
+        // margin as an argument) are stale
+        if (H->model.log_margin != log2_margin || H->model.slack != slack) H->lb_dirty = true;
         H->model.log_margin = log2_margin;
         H->model.slack = slack;
     });
