@@ -482,7 +482,7 @@ def main(argv=None):
             "overflow_rate": sl.n_overflow_paths / max(1, visible),
             "alloc_intersections": int(sl.alloc_intersections),
             "over_allocation": sl.alloc_intersections / max(1, N) - 1.0,
-            "budget_readback_frames": int(sl.learned_fallbacks),
+            "fallback_frames": int(sl.learned_fallbacks),
         })
         if rp is not None:
             learned["plain_ms_per_step"] = plain_ms

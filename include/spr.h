@@ -44,9 +44,11 @@ enum { SPR_FILL_NONZERO = 0, SPR_FILL_EVEN_ODD = 1 };
 enum { SPR_COMPOSE_OVERWRITE = 0, SPR_COMPOSE_SRC_OVER = 1 };
 
 /* Allocation modes.  PLAIN is the reference's count-then-write form (count pass, host readback
- * of the intersection total, write pass; scanline_rasterizer.cpp:317-395).  LEARNED replaces the
- * count pass by per-path capacities predicted by the built-in MLP; paths whose prediction
- * overflows are redone by the exact count pass, so the output is identical. */
+ * of the intersection total, write pass; scanline_rasterizer.cpp:317-395).  LEARNED removes the
+ * host readback: the frame's buffers are sized beforehand from the per-path counts predicted by
+ * the built-in MLP (read back once per scene or model), the exact counts are scanned on the device,
+ * and a frame whose exact total exceeds that allocation is culled on the device and rendered with
+ * the plain method instead, so the output is identical. */
 enum { SPR_MODE_PLAIN = 0, SPR_MODE_LEARNED = 1 };
 
 typedef struct spr_opts {
@@ -63,12 +65,12 @@ typedef struct spr_stats {
     int64_t n_intersections;   /* N: the reference's n_fragments */
     int64_t n_cells;           /* merged cell records */
     int64_t n_spans;           /* span records */
-    int64_t n_overflow_paths;  /* LEARNED: paths that took the exact fallback */
+    int64_t n_overflow_paths;  /* LEARNED: paths whose exact count exceeded their predicted capacity */
     int64_t alloc_intersections; /* LEARNED: predicted capacity summed over paths */
     double frame_ms;           /* device time of the last frame (HIP events) */
     double stage_ms[16];       /* per-stage device time when timing is enabled */
-    int64_t learned_fallbacks; /* LEARNED: frames that read their budget bound back to the host
-                                  (the first frame after a load, MVP or model change) */
+    int64_t learned_fallbacks; /* LEARNED: frames whose exact total exceeded the allocation and were
+                                  rendered with the plain method (the allocation then grows) */
     /* Which kernel routes the last frame took (each route has its own kernel or mode): */
     int64_t n_wave_chains;     /* crossing chains of 1024 or more, each solved by a whole wave */
     int64_t n_heavy_pieces;    /* monotone pieces merged by a 256-thread workgroup each (more than

@@ -137,5 +137,15 @@ def solve_quad(a: float, b: float, c: float):
     return np.float32(r0.value), np.float32(r1.value)
 
 
+def solve_crossing(type_: int, xs, ys, side: int, c: float, t_min: float, t1: float) -> np.float32:
+    """One crossing solve of the write pass (make_intersection_1.comp:377-437): the t where the
+    curve's x (side 0) or y (side 1) crosses the grid line c, searched in [t_min, t1]."""
+    x = np.ascontiguousarray(xs, dtype=np.float32)
+    y = np.ascontiguousarray(ys, dtype=np.float32)
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    return np.float32(lib().oracle_solve_crossing(int(type_), f(x), f(y), int(side), float(c), float(t_min),
+                                                  float(t1)))
+
+
 def path_visible(mask: int) -> bool:
     return bool(lib().oracle_path_visible(int(np.int32(np.uint32(mask & 0xFFFFFFFF)))))

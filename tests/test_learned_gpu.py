@@ -189,6 +189,40 @@ def test_budget_growth_falls_back_to_plain(spr, trained):
     p.close()
 
 
+def test_over_budget_frame_renders_plain(spr, trained):
+    """The allocation is predicted for the scene at a quarter of its scale; the identity MVP then
+    needs about four times the entries.  That frame exceeds the allocation: the device culls it,
+    the host renders it with the plain method (identical output) and grows the allocation, and
+    the next frames are learned frames again (eager, captured, replayed), identical too."""
+    r, _, path = trained
+    scene = spr.Scene.generate(2, 6, 200, 1024)
+    q = spr.Rasterizer(1024, 1024, mode=spr.MODE_LEARNED)
+    q.learned_load(path)
+    q.load_scene(scene)
+    p = spr.Rasterizer(1024, 1024)
+    p.load_scene(scene)
+    small = np.diag([0.25, 0.25, 1, 1]).astype(np.float32)
+    q.set_mvp(small)
+    p.set_mvp(small)
+    assert np.array_equal(q.render(), p.render())
+    n_small = p.stats().n_intersections
+    assert q.stats().learned_fallbacks == 0
+    eye = np.eye(4, dtype=np.float32)
+    q.set_mvp(eye)
+    p.set_mvp(eye)
+    ip = p.render()
+    assert p.stats().n_intersections > 2 * n_small
+    print("[learned] full-scale frame after a quarter-scale prediction", flush=True)
+    assert np.array_equal(q.render(), ip)
+    n = q.stats().learned_fallbacks
+    assert n == 1
+    for frame in range(3):  # eager on the grown allocation, eager + capture, replay
+        assert np.array_equal(q.render(), ip), frame
+        assert q.stats().learned_fallbacks == n
+    q.close()
+    p.close()
+
+
 def test_save_load_bit_exact(spr, trained):
     r, _, path = trained
     q = spr.Rasterizer(64, 64)

@@ -325,6 +325,73 @@ def test_write_cubic_crossings_land_on_grid(orc):
     assert total > 1000 and good / total > 0.995
 
 
+def _monotone_cubics(rng, n, span=64.0):
+    """n cubics whose x runs monotonically over about `span` px (control x non-decreasing)."""
+    out = []
+    for _ in range(n):
+        x0 = rng.uniform(0.0, 200.0)
+        xs = np.sort(x0 + rng.uniform(0.0, span, 4))
+        ys = rng.uniform(0.0, 200.0, 4)
+        out.append((xs.astype(np.float32), ys.astype(np.float32)))
+    return out
+
+
+def test_write_cubic_bisection_runs_24_halvings(orc):
+    """CUBIC_ITERATION_NUMBER = 24 (make_intersection_1.comp:7), pinned by a closed form: on the
+    bracket [0, 1] every midpoint is a dyadic rational float32 holds exactly, and the result is the
+    last midpoint, (t0 + t1) / 2 with t1 - t0 = 2^-(K-1) after K - 1 halvings: t * 2^K is an odd
+    integer.  A 23-step (or 20-step) bisection leaves t * 2^24 even; 25 steps leave it fractional.
+    The float64 root lies within the last bracket, up to the float32 evaluation's rounding."""
+    rng = np.random.default_rng(24)
+    odd = checked = 0
+    worst = 0.0
+    for xs, ys in _monotone_cubics(rng, 400):
+        lo, hi = float(xs[0]), float(xs[3])
+        k0 = int(np.ceil(lo / 2.0)) * 2
+        for c in range(k0, int(hi), 2):
+            if not lo < c < hi:
+                continue
+            t = orc.solve_crossing(4, xs, ys, 0, float(c), 0.0, 1.0)
+            if abs(float(G.bezier(np.stack([xs, ys], 1), [float(t)])[0][0]) - c) > 1.0:
+                continue  # the 1 px fallback (pinned below)
+            m = float(t) * 2.0 ** 24
+            assert m == np.floor(m), (t, c)  # a multiple of 2^-24: at least 24 halvings
+            checked += 1
+            odd += int(m) % 2 == 1
+            # float64 root of x(t) = c on the monotone curve, by bisection to 1e-15
+            a, b = 0.0, 1.0
+            ctrl = np.stack([xs, ys], 1).astype(np.float64)
+            for _ in range(60):
+                mid = 0.5 * (a + b)
+                if G.bezier(ctrl, [mid])[0][0] < c:
+                    a = mid
+                else:
+                    b = mid
+            worst = max(worst, abs(float(t) - a) * 2.0 ** 24)
+    assert checked > 2000
+    assert odd == checked  # exactly 24: a 23-step or 20-step bisection gives even multiples
+    # the last bracket is 2^-24 wide, but the float32 sign test near the root is only as good as
+    # the evaluation's rounding (a few ulp of a ~200 px coordinate) over the local slope, which is
+    # small near a monotone cubic's flat ends: the distance to the float64 root is a sanity bound
+    assert worst < 2.0 ** 8, worst
+
+
+def test_write_cubic_fallback_beyond_one_pixel(orc):
+    """make_intersection_1.comp:430-433: when the last midpoint's value misses the grid line by more
+    than 1 px, the solve returns the bracket start t_min; within 1 px it keeps the midpoint."""
+    xs = np.array([0.0, 3.0, 7.0, 10.0], np.float32)  # x runs 0 -> 10 monotonically
+    ys = np.array([0.0, 1.0, 2.0, 3.0], np.float32)
+    # no crossing in the bracket: the search runs to t1 and ends 40 px away -> t_min exactly
+    assert orc.solve_crossing(4, xs, ys, 0, 50.0, 0.25, 1.0) == np.float32(0.25)
+    # 0.9 px beyond the curve's end: within 1 px, the last midpoint (just below t1) is kept
+    t = orc.solve_crossing(4, xs, ys, 0, 10.9, 0.25, 1.0)
+    assert 1.0 - 1e-6 < t < 1.0
+    # 1.1 px beyond: the fallback
+    assert orc.solve_crossing(4, xs, ys, 0, 11.1, 0.25, 1.0) == np.float32(0.25)
+    # the line at the bracket start itself: t_min without any search
+    assert orc.solve_crossing(4, xs, ys, 0, 0.0, 0.0, 1.0) == np.float32(0.0)
+
+
 def test_write_corner_crossings_are_or_merged(orc):
     """A diagonal through grid corners crosses x = k and y = k at the same t.  The reference
     ORs the side bits of equal-t neighbours into both entries (make_intersection_1.comp:

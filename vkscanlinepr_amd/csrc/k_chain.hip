@@ -76,14 +76,25 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
     const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
     const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
     int h, ChainDesc* __restrict__ chains, PieceDesc* __restrict__ pieces, uint8_t* __restrict__ chain_cls,
-    int32_t* __restrict__ class_count, const int32_t* __restrict__ ends, const int2* __restrict__ pcanvas) {
+    int32_t* __restrict__ class_count, Budget budget, bool budgeted, const int2* __restrict__ pcanvas) {
     __shared__ int32_t hist[kClasses];
     if (threadIdx.x < kClasses) hist[threadIdx.x] = 0;
     __syncthreads();
+    // a frame over its budget (or whose int32 total wrapped) writes nothing past any buffer: every
+    // curve takes the culled route (no chain, no piece)
+    bool skip = false;
+    if (budgeted) {
+        const int32_t total = offsets[C];
+        skip = total < 0 || (int64_t)total > budget.entries;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *budget.live = skip ? 0 : total;
+            *budget.over = skip ? 1 : 0;
+        }
+    }
     // grid-stride over curves; the class histogram is flushed once per workgroup
     for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < C; c += gridDim.x * kBlock) {
         const uint32_t path = cpath[c];
-        const uint32_t nc = visible_mask(visible[path]) ? ncuts[c] : 0xFFFFFFFFu;  // all slots empty
+        const uint32_t nc = !skip && visible_mask(visible[path]) ? ncuts[c] : 0xFFFFFFFFu;  // all slots empty
         int cw = w, ch = h;
         if (pcanvas && nc != 0xFFFFFFFFu) { const int2 cvs = pcanvas[path]; cw = cvs.x; ch = cvs.y; }
         const uint32_t type = ctype[c];
@@ -95,9 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
             const float4 qq = cuts[c];
             q[0] = qq.x; q[1] = qq.y; q[2] = qq.z; q[3] = qq.w;
             base0 = offsets[c];
-            // the curve's range ends at the next curve's offset, or at ends[c] when curves are
-            // placed in per-path arenas
-            limit = ends ? ends[c] : offsets[c + 1];
+            limit = offsets[c + 1];  // the curve's range ends at the next curve's offset
         }
         float t0 = 0.0f, p0x = 0.0f, p0y = 0.0f;
         if (nc != 0xFFFFFFFFu) { p0x = cv.x[0]; p0y = cv.y[0]; }
@@ -849,12 +858,13 @@ hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uin
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
                              ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
-                             int n_cu, hipStream_t s, const int32_t* ends, const int2* pcanvas) {
+                             int n_cu, hipStream_t s, const Budget* budget, const int2* pcanvas) {
     if (C == 0) return hipSuccess;
     unsigned g = grid_for((int64_t)C);
     if (g > (unsigned)n_cu * 8) g = (unsigned)n_cu * 8;
+    const Budget b = budget ? *budget : Budget{0, nullptr, nullptr};
     k_chain_prep<<<g, kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts, ncuts, offsets, C, w, h, chains,
-                                      pieces, chain_cls, class_count, ends, pcanvas);
+                                      pieces, chain_cls, class_count, b, budget != nullptr, pcanvas);
     return hipGetLastError();
 }
 

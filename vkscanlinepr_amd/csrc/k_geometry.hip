@@ -102,11 +102,14 @@ __global__ __launch_bounds__(kBlock) void k_count(const float2* __restrict__ tpo
                                                   const int32_t* __restrict__ visible, uint32_t C,
                                                   int w, int h, float4* __restrict__ cuts,
                                                   uint32_t* __restrict__ ncuts,
-                                                  int32_t* __restrict__ counts) {
+                                                  int32_t* __restrict__ counts,
+                                                  const int2* __restrict__ pcanvas) {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= C) return;
+    const uint32_t path = cpath[c];
+    if (pcanvas) { w = pcanvas[path].x; h = pcanvas[path].y; }  // a batch: the path's document canvas
     const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
-    const bool vis = visible_mask(visible[cpath[c]]);
+    const bool vis = visible_mask(visible[path]);
     float q[4];
     const uint32_t n = monotone_cuts(cv, vis, q);
     cuts[c] = make_float4(q[0], q[1], q[2], q[3]);
@@ -245,9 +248,10 @@ hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_
 
 hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                         const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
-                        float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s) {
+                        float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s, const int2* pcanvas) {
     if (C == 0) return hipSuccess;
-    k_count<<<grid_for(C), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, C, w, h, cuts, ncuts, counts);
+    k_count<<<grid_for(C), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, C, w, h, cuts, ncuts, counts,
+                                           pcanvas);
     return hipGetLastError();
 }
 

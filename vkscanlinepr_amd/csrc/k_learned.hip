@@ -2,18 +2,19 @@
 //
 // The plain frame counts every curve's intersections, scans the counts and waits for the host
 // to read the total before anything is written (make_intersection_0.comp -> naive_scan.comp ->
-// host readback, scanline_rasterizer.cpp:317-395).  The learned frame instead:
-//   1. k_path_features: 16 cheap features per path from its transformed control points;
+// host readback, scanline_rasterizer.cpp:317-395).  The learned frame sizes its buffers before
+// the count instead, from predicted per-path counts, and never waits for the host:
+//   1. k_path_features: 16 cheap features per path from its transformed control points (and a
+//      proven upper bound of the path's intersections);
 //   2. k_mlp_infer: a 16-64-64-2 MLP (BF16 weights, FP32 accumulation on the matrix cores)
 //      predicts log2(1 + intersections) and log2(1 + records) per path;
-//   3. k_caps: predicted capacities (with a safety margin) become per-path arenas after one
-//      device scan;
-//   4. k_learned_count: one workgroup per path counts its curves exactly, places them in the
-//      arena when they fit, or takes an exact-size block of the overflow pool when they do not
-//      (the exact fallback), and marks the unused tail of the arena as a gap.
-// No step waits for the host.  A path that fits neither its arena nor the pool's remaining
-// budget is dropped and flagged; the host then redoes the frame with the plain method, so the
-// output never changes.
+//   3. k_caps: predicted capacities with a safety margin; their sum (one device scan) is the
+//      allocation the host sizes the frame's buffers by;
+//   4. k_path_overflow: the paths whose exact count (the count pass still runs, on the device)
+//      exceeded their capacity.
+// The entries keep the plain frame's layout (exact offsets from the device scan of the counts);
+// a frame whose exact total exceeds its budget is culled on the device and rendered with the plain
+// method, so the output never changes.
 //
 // The training kernels (k_dense_*, k_loss_grad, k_adam) are plain FP32 forward/backward passes
 // of the same network, used to fit it on counts produced by the plain method.
@@ -244,106 +245,13 @@ __global__ void k_caps(const float* __restrict__ pred, const int32_t* __restrict
     if ((threadIdx.x & 63) == 0 && cr) atomicAdd(cap_r_total, cr);
 }
 
-// Exact per-curve counts as k_count (make_intersection_0.comp:226-411), each curve on its own
-// path's canvas.
-__global__ __launch_bounds__(256) void k_learned_curve_counts(
-    const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos, const uint32_t* __restrict__ ctype,
-    const uint32_t* __restrict__ cpath, const int32_t* __restrict__ visible, uint32_t C, int w, int h,
-    float4* __restrict__ cuts, uint32_t* __restrict__ ncuts, int32_t* __restrict__ counts,
-    const int2* __restrict__ pcanvas) {
-    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= C) return;
-    const uint32_t p = cpath[c];
-    if (pcanvas) { w = pcanvas[p].x; h = pcanvas[p].y; }
-    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
-    const bool vis = visible_mask(visible[p]);
-    float q[4];
-    const uint32_t n = monotone_cuts(cv, vis, q);
-    cuts[c] = make_float4(q[0], q[1], q[2], q[3]);
-    ncuts[c] = n;
-    int32_t cnt = 0;
-    if (vis) {
-        float p0x = cv.x[0], p0y = cv.y[0];
-#pragma unroll
-        for (uint32_t k = 0; k < 5; ++k) {
-            if (k > n) break;
-            const float t = k < n ? q[k] : 1.0f;
-            const float p1x = cv.ex(t), p1y = cv.ey(t);
-            cnt += piece_count(p0x, p0y, p1x, p1y, w, h);
-            p0x = p1x; p0y = p1y;
-        }
-    }
-    counts[c] = cnt;
-}
-
-// One wave per path: intra-path offsets of the exact curve counts, placement in the arena or
-// the overflow pool, gap marking.  counts[c] becomes the curve's first position and ends[c] its
-// end.  ctl[0] = pool entries taken, ctl[1] = overflowing paths, ctl[2] = live positions.
-__global__ __launch_bounds__(256) void k_learned_place(
-    int32_t* __restrict__ visible, const uint32_t* __restrict__ pcurve, uint32_t Np,
-    const int32_t* __restrict__ arena, const int32_t* __restrict__ cap_n, int64_t budget,
-    int32_t* __restrict__ counts, int32_t* __restrict__ ends, int32_t* __restrict__ ctl,
-    int32_t* __restrict__ n_exact, int32_t* __restrict__ pbase, int2* __restrict__ inter,
-    int8_t* __restrict__ mark, uint32_t* __restrict__ epath, int32_t* __restrict__ err) {
-    const uint32_t p = blockIdx.x * (256 / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (p >= Np) return;  // wave-uniform
-    const uint32_t c0 = pcurve[p], c1 = pcurve[p + 1];
-    // a path of at most 64 curves keeps its scan in registers and writes its offsets once, after
-    // the placement; a longer one writes path-relative offsets and adds the base afterwards
-    const bool one = c1 - c0 <= 64u;
-    int32_t carry = 0, cnt0 = 0, inc0 = 0;
-    for (uint32_t cs = c0; cs < c1; cs += 64) {
-        const uint32_t c = cs + lane;
-        const int32_t cnt = c < c1 ? counts[c] : 0;
-        const int32_t inc = wave_inclusive_add(cnt);
-        if (one) { cnt0 = cnt; inc0 = inc; }
-        else if (c < c1) { counts[c] = carry + inc - cnt; ends[c] = carry + inc; }
-        carry += __shfl(inc, 63);
-    }
-    const int32_t n = carry;
-    const int64_t A = arena[p], cap = cap_n[p], arena_total = arena[Np];
-    int64_t base = A;
-    int mode = 0;
-    if (lane == 0) {
-        if (n > cap || A + cap > budget) {
-            mode = 1;
-            const int32_t off = n > 0 ? atomicAdd(&ctl[0], n) : 0;
-            base = arena_total + off;
-            atomicAdd(&ctl[1], 1);
-            if (base + n > budget) { mode = 2; atomicOr(err, 4); }
-        }
-        n_exact[p] = mode == 2 ? 0 : n;
-        pbase[p] = (int32_t)base;
-        if (mode == 2) visible[p] = 0;  // no chains: nothing is written for a dropped path
-    }
-    mode = __shfl(mode, 0);
-    base = (int64_t)__shfl((int32_t)base, 0);
-    if (one) {
-        // a dropped path keeps its path-relative offsets, as a longer one does
-        const int64_t b = mode != 2 ? base : 0;
-        const uint32_t c = c0 + lane;
-        if (c < c1) { counts[c] = (int32_t)(b + inc0 - cnt0); ends[c] = (int32_t)(b + inc0); }
-    } else if (mode != 2) {
-        for (uint32_t c = c0 + lane; c < c1; c += 64) {
-            counts[c] = (int32_t)(counts[c] + base);
-            ends[c] = (int32_t)(ends[c] + base);
-        }
-    }
-    // gaps: the unused arena tail (the whole arena after an overflow), and the part of a
-    // dropped path's pool block that lies inside the budget
-    const int64_t g0 = mode == 0 ? A + n : A, g1 = min(A + cap, budget);
-    for (int64_t i = g0 + lane; i < g1; i += 64) { inter[i] = make_int2(-1, 0); mark[i] = 1; epath[i] = kNoPath; }
-    if (mode == 2) {
-        const int64_t e = min(base + n, budget);
-        for (int64_t i = base + lane; i < e; i += 64) { inter[i] = make_int2(-1, 0); mark[i] = 1; epath[i] = kNoPath; }
-    }
-}
-
-__global__ void k_learned_finish(const int32_t* __restrict__ arena, uint32_t Np, int64_t budget,
-                                 int32_t* __restrict__ ctl) {
-    const int64_t live = (int64_t)arena[Np] + ctl[0];
-    ctl[2] = (int32_t)(live < budget ? live : budget);
+// Paths whose exact entry count exceeds their predicted capacity: one atomic per wave.
+__global__ void k_path_overflow(const int32_t* __restrict__ seg, const int32_t* __restrict__ cap_n,
+                                const int32_t* __restrict__ visible, uint32_t Np, int32_t* __restrict__ n_over) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool over = p < Np && visible_mask(visible[p]) && seg[p + 1] - seg[p] > cap_n[p];
+    const unsigned long long m = __ballot(over);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(n_over, (int32_t)__popcll(m));
 }
 
 // sstart[p] = first sorted element of path p = bucket offset of the path's first bucket.
@@ -489,18 +397,10 @@ hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, f
     return hipGetLastError();
 }
 
-hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, const uint32_t* cpath,
-                                int32_t* visible, const uint32_t* pcurve, uint32_t C, uint32_t Np, int w, int h,
-                                const int32_t* arena, const int32_t* cap_n, int64_t budget, float4* cuts,
-                                uint32_t* ncuts, int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact,
-                                int32_t* pbase, int2* inter, int8_t* mark, uint32_t* epath, int32_t* err,
-                                hipStream_t s, const int2* pcanvas) {
+hipError_t launch_path_overflow(const int32_t* seg, const int32_t* cap_n, const int32_t* visible, uint32_t Np,
+                                int32_t* n_over, hipStream_t s) {
     if (Np == 0) return hipSuccess;
-    if (C) k_learned_curve_counts<<<blocks(C), 256, 0, s>>>(tpos, cpos, ctype, cpath, visible, C, w, h, cuts, ncuts,
-                                                            counts, pcanvas);
-    k_learned_place<<<(Np + 3) / 4, 256, 0, s>>>(visible, pcurve, Np, arena, cap_n, budget, counts, ends, ctl,
-                                                 n_exact, pbase, inter, mark, epath, err);
-    k_learned_finish<<<1, 1, 0, s>>>(arena, Np, budget, ctl);
+    k_path_overflow<<<blocks(Np), 256, 0, s>>>(seg, cap_n, visible, Np, n_over);
     return hipGetLastError();
 }
 

@@ -38,7 +38,8 @@ hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_
                             hipStream_t s, const int2* pcanvas = nullptr);
 hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                         const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
-                        float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s);
+                        float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s,
+                        const int2* pcanvas = nullptr);
 hipError_t launch_path_segments(const uint32_t* cpath, uint32_t C, const int32_t* offsets,
                                 uint32_t Np, int32_t* seg, hipStream_t s);
 hipError_t launch_segment_stats(const int32_t* seg, uint32_t Np, int32_t* max_len, hipStream_t s);
@@ -83,11 +84,20 @@ constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 
 // (wave-solved and the longest lane-solved chains), and the pieces of at most kHeavyCrossings
 // entries, which never wait on them, are finished by a concurrent stream meanwhile.
 constexpr int kHeavyCrossings = 512;
+// A frame whose buffers were sized before its counts were known (no host synchronisation): the
+// chain prep compares the exact total offsets[C] with the budget; a frame over budget writes no
+// chain and no piece (as if every path were culled), sets *over and leaves 0 in *live, the number
+// of entry positions the later kernels read; a frame within budget leaves the total in *live.
+struct Budget {
+    int64_t entries;
+    int32_t* live;
+    int32_t* over;
+};
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                              const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
                              ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
-                             int n_cu, hipStream_t s, const int32_t* ends = nullptr, const int2* pcanvas = nullptr);
+                             int n_cu, hipStream_t s, const Budget* budget = nullptr, const int2* pcanvas = nullptr);
 hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_t* class_count,
                               int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* light_begin,
                               int32_t* order, int n_cu, hipStream_t s);
@@ -229,12 +239,10 @@ hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, flo
 hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, float log_margin, int32_t slack,
                        int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s,
                        const int32_t* pbound = nullptr);
-hipError_t launch_learned_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype, const uint32_t* cpath,
-                                int32_t* visible, const uint32_t* pcurve, uint32_t C, uint32_t Np, int w, int h,
-                                const int32_t* arena, const int32_t* cap_n, int64_t budget, float4* cuts,
-                                uint32_t* ncuts, int32_t* counts, int32_t* ends, int32_t* ctl, int32_t* n_exact,
-                                int32_t* pbase, int2* inter, int8_t* mark, uint32_t* epath, int32_t* err,
-                                hipStream_t s, const int2* pcanvas = nullptr);
+// *n_over (zeroed) += visible paths whose exact entry count (seg[p+1] - seg[p]) exceeds their
+// predicted capacity cap_n[p].
+hipError_t launch_path_overflow(const int32_t* seg, const int32_t* cap_n, const int32_t* visible, uint32_t Np,
+                                int32_t* n_over, hipStream_t s);
 hipError_t launch_sorted_starts(const int32_t* boff, const int32_t* bucket_base, uint32_t Np, int32_t* sstart,
                                 hipStream_t s);
 hipError_t launch_path_targets(const int32_t* n_or_seg, int seg_form, const int2* brec, const int32_t* bucket_base,

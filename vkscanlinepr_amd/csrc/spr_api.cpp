@@ -29,7 +29,18 @@ struct spr_scene {
 
 namespace {
 
-thread_local std::string g_error;
+// Message of the last failure of a call without a handle.  A fixed buffer: setting it never
+// allocates, so the entry points that allocate nothing can report through it too.
+struct ErrorText {
+    char buf[512] = {0};
+    ErrorText& operator=(const char* m) {
+        std::snprintf(buf, sizeof buf, "%s", m ? m : "");
+        return *this;
+    }
+    ErrorText& operator=(const std::string& m) { return *this = m.c_str(); }
+    const char* c_str() const { return buf; }
+};
+thread_local ErrorText g_error;
 
 struct Fail {
     int code;
@@ -78,7 +89,7 @@ int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // allocation counters.
 enum {
     kSumCells = 0, kSumSpans, kSumErr, kSumWaveChains, kSumHeavyPieces, kSumLongBands, kSumBigBands,
-    kSumPool, kSumOverflow, kSumLive, kSumArena, kSumSorted, kSumCapR, kSumCount
+    kSumTotal, kSumOver, kSumOverflow, kSumLive, kSumArena, kSumSorted, kSumCapR, kSumCount
 };
 static_assert(kSumCount <= spr::kSummaryMax, "summary slots");
 
@@ -92,6 +103,7 @@ struct spr_handle {
     hipStream_t stream2 = nullptr;  // the light chain solve and the pieces that need only it
     hipEvent_t ev_fork = nullptr, ev_light = nullptr, ev_small = nullptr, ev_join = nullptr;
     hipEvent_t ev_sfork = nullptr, ev_sjoin = nullptr;  // the short-band sort on the second stream
+    hipEvent_t ev_pfork = nullptr, ev_pjoin = nullptr;  // the learned frame's prediction on the second stream
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::vector<hipEvent_t> ev_stage;
     bool stage_timing = false;
@@ -148,13 +160,16 @@ struct spr_handle {
     spr::MlpDevice mlpdev;
     DBuf<uint32_t> pcurve;                          // first curve of each path, Np+1
     DBuf<float> feat, pred, train_feat, train_tgt;
-    DBuf<int32_t> arena, capn, nexact, pbase, sstart, hist_l, fill_l, bsum_l, cend, pbound;
+    DBuf<int32_t> arena, capn, sstart, hist_l, fill_l, bsum_l, pbound;
     DBuf<int2> brec_l;
-    int64_t lb_n = 0, lb_r = 0;                     // learned frame budgets
-    bool lb_dirty = true;                           // scene, MVP or model changed since they were set
-    int64_t n_live = 0;                             // positions of the last learned layout
-    int64_t learned_fallbacks = 0;                  // learned frames that read their budgets back
+    int64_t lb_n = 0, lb_r = 0;                     // learned frame budgets (entries, records)
+    bool need_prediction = true;                    // scene or model changed: read the prediction back
+    bool lb_dirty = true;                           // scene, MVP, model or margin changed: the graph is stale
+    int64_t n_live = 0;                             // entries of the last learned frame
+    int64_t learned_fallbacks = 0;                  // learned frames over budget, rendered with the plain method
+    int64_t predictions = 0;                        // learned frames that read the predicted allocation back
     bool last_learned = false;
+    bool last_fallback = false;
     int2* last_brec = nullptr;                      // per-bucket record ranges of the last frame
     bool last_fused = false;                        // last frame fused mark + emit into the sort
 
@@ -478,6 +493,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     S.p[kSumHeavyPieces] = gen ? d_cc + spr::kCcHeavyPieces : nullptr;
     S.p[kSumLongBands] = reinterpret_cast<const int32_t*>(ctl_at(H, z.st_wind, z.ctr, 4).counter + 1);
     S.p[kSumBigBands] = d_bigcount;
+    S.p[kSumTotal] = H->inj ? nullptr : H->counts.p + C;
     finish_summary(H, S, s);
     SPR_HIP(hipEventRecord(H->ev_end, s));
     H->frame_done = true;
@@ -485,29 +501,39 @@ void run_frame(spr_handle* H, bool record_begin = true) {
 
 // ---- learned single-pass frame
 //
-// Same stages as run_frame, but no host synchronisation between them: per-path capacities
-// predicted by the MLP become arenas, one workgroup per path counts its curves exactly and
-// places them (arena, or an exact block of the overflow pool), and every later kernel reads its
-// element counts from device memory.  Buffers are sized by budgets kept from earlier frames
-// (the first learned frame after a load reads the predicted total once).  If a budget turns out
-// too small the device flags it, and the frame is redone with the plain method: the output is
-// always the plain method's.
+// The stages of run_frame with the same entry layout (exact per-curve counts, scanned on the
+// device), but no host synchronisation: every buffer is sized by a budget the host sets before
+// the frame, and every kernel after the count reads its element counts from device memory.
+//  - The budget comes from the predictor.  The first frame after a load or a model change runs
+//    the 16-64-64-2 MLP on its paths and reads the sum of the predicted per-path capacities back
+//    (for a batch of documents: the proven upper bound, as a batch has no plain frame to fall back
+//    on); every frame then grows the budget to cover its own exact need and its prediction, with
+//    headroom, so that the next frames (after a new MVP too) need no readback.
+//  - The features, the MLP and the capacities run on the second stream, beside the count.
+//  - The chain prep checks the exact total against the budget on the device.  A frame over its
+//    budget is culled there (no chain, no piece: nothing is written past any buffer), and the
+//    host renders that frame with the plain method: the output is always the plain method's.
 struct LearnedLayout {
     size_t total = 0, ctl = 0, misc = 0, capr = 0, counters = 0, totals = 0, slots = 0, bound = 0;
-    size_t st_arena = 0, st_pb = 0, st_hist = 0, st_band = 0, st_wind = 0, st_mark = 0, chainctl = 0;
+    size_t st_count = 0, st_arena = 0, st_pb = 0, st_hist = 0, st_band = 0, st_wind = 0, st_mark = 0, chainctl = 0;
 };
 
-LearnedLayout learned_layout(uint32_t Np, int64_t nb_budget, int64_t n_budget) {
+// Scan tile counters of the learned frame (u32 slots of the counters block).
+enum { kLcArena = 0, kLcBucketBase, kLcHist, kLcBandFlags, kLcBandSums, kLcMark, kLcSortTicket, kLcSortTicket2,
+       kLcCount, kLcWords };
+
+LearnedLayout learned_layout(uint32_t C, uint32_t Np, int64_t nb_budget, int64_t n_budget) {
     LearnedLayout z;
     size_t o = 0;
-    z.counters = o; o += 4;     // eight u32 scan tile counters
-    z.ctl = o; o += 2;          // int32 pool entries, overflow paths, live positions, spare
+    z.counters = o; o += (kLcWords + 1) / 2;  // u32 scan tile counters
+    z.ctl = o; o += 2;          // int32 live entries, over-budget flag, overflowing paths, spare
     z.misc = o; o += 1;         // int32 error flags, big-band count
     z.capr = o; o += 1;         // int32 predicted record total, spare
     z.totals = o; o += 2;       // int32 cells, spans, R, spare
     z.slots = o; o += 2;        // int32 sorted entries, bands, spare x2
     z.bound = o; o += 1;        // u64 upper bound of the intersections
     z.chainctl = o; o += (spr::kCcWords + 1) / 2;
+    z.st_count = o; o += spr::scan_tiles(std::max<uint32_t>(C, 1));
     z.st_arena = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
     z.st_pb = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
@@ -518,18 +544,27 @@ LearnedLayout learned_layout(uint32_t Np, int64_t nb_budget, int64_t n_budget) {
     return z;
 }
 
+// The budget covers `need` entries with an eighth of headroom (and the predicted total with a
+// sixteenth); it only grows, and records take at most two per entry.
+void grow_budget(spr_handle* H, int64_t need, int64_t predicted) {
+    const int64_t want = std::max<int64_t>(need + need / 8, predicted + predicted / 16) + 1024;
+    if (want > H->lb_n) H->lb_n = want;
+    H->lb_r = 2 * H->lb_n;
+}
+
 void learned_finish(spr_handle* H);
 
-// Enqueues one frame without any host synchronisation unless the budgets must be derived
-// (H->lb_dirty).  With H->capturing the work is being recorded into a graph: no events, no
-// waits, and every buffer already has its size.
+// Enqueues one frame without any host synchronisation unless the predicted budget must first be
+// read back (H->need_prediction).  With H->capturing the work is being recorded into a graph: no
+// events, no waits, and every buffer already has its size.
 void run_learned_frame(spr_handle* H) {
     if (!H->loaded) throw Fail{SPR_ERR_STATE, "render before load"};
     const bool use_model = H->model.ready;
     if (!use_model && !H->batch)
         throw Fail{SPR_ERR_STATE, "learned mode needs a trained or loaded model (spr_learned_train or spr_learned_load)"};
-    if (H->capturing && H->lb_dirty) throw Fail{SPR_ERR_STATE, "internal: capture of a frame without budgets"};
+    if (H->capturing && H->need_prediction) throw Fail{SPR_ERR_STATE, "internal: capture of a frame without budgets"};
     hipStream_t s = H->stream;
+    hipStream_t s2 = serial(H) ? s : H->stream2;
     if (use_model && !H->mlpdev.valid()) H->mlpdev.upload(H->model, s);
     const uint32_t P = H->P, C = H->C, Np = H->Np;
     const int w = (int)H->w, h = (int)H->h;
@@ -553,18 +588,16 @@ void run_learned_frame(spr_handle* H) {
     H->cuts.ensure(std::max<uint32_t>(C, 1));
     H->ncuts.ensure(std::max<uint32_t>(C, 1));
     H->counts.ensure((size_t)C + 1);
-    H->cend.ensure((size_t)C + 1);
+    H->seg.ensure((size_t)Np + 1);
     H->feat.ensure((size_t)std::max<uint32_t>(Np, 1) * spr::kMlpIn);
     H->pred.ensure((size_t)std::max<uint32_t>(Np, 1) * spr::kMlpOut);
     H->arena.ensure((size_t)Np + 1);
     H->capn.ensure((size_t)Np + 1);
-    H->nexact.ensure((size_t)Np + 1);
-    H->pbase.ensure((size_t)Np + 1);
     H->sstart.ensure((size_t)Np + 1);
     H->pbound.ensure((size_t)Np + 1);
 
-    // phase 1: transform, features, prediction, arenas (no dependency on any count)
-    LearnedLayout z = learned_layout(Np, nb_budget, std::max<int64_t>(H->lb_n, 1));
+    // phase 1: transform; the prediction on the second stream, the exact count on the first
+    LearnedLayout z = learned_layout(C, Np, nb_budget, std::max<int64_t>(H->lb_n, 1));
     H->zero.ensure(z.total);
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->zero.p, 2 * (int64_t)z.total, 0u, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->visible.p, Np, 0u, s));
@@ -574,44 +607,59 @@ void run_learned_frame(spr_handle* H) {
                                   H->row_lo.p, H->row_hi.p, s, pcv));
     stage_mark(H, 1);
     int32_t* d_capr = reinterpret_cast<int32_t*>(H->zero.p + z.capr);
-    SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p, H->rule.p, Np,
-                                      w, h, H->feat.p, s, H->zero.p + z.bound, pcv, H->pbound.p));
-    if (use_model) {
-        SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s));
-        SPR_HIP(spr::launch_caps(H->pred.p, H->visible.p, Np, H->model.log_margin, H->model.slack, H->arena.p,
-                                 H->capn.p, d_capr, s, H->pbound.p));
-    } else {
-        // no model: every path takes an exact block of the pool
-        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->arena.p, (int64_t)Np + 1, 0u, s));
-        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->capn.p, (int64_t)Np + 1, 0u, s));
+    if (s2 != s) {
+        SPR_HIP(hipEventRecord(H->ev_pfork, s));
+        SPR_HIP(hipStreamWaitEvent(s2, H->ev_pfork, 0));
     }
-    SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, 0), s));
-    SPR_HIP(spr::launch_bucket_counts(H->row_lo.p, H->row_hi.p, Np, rmax, H->bucket_base.p, H->prow.p, s, pcv));
-    SPR_HIP(spr::launch_scan_i32(H->bucket_base.p, H->bucket_base.p, Np, ctl_at(H, z.st_pb, z.counters, 1), s));
+    SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p, H->rule.p, Np,
+                                      w, h, H->feat.p, s2, H->zero.p + z.bound, pcv, H->pbound.p));
+    if (use_model) {
+        SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s2));
+        SPR_HIP(spr::launch_caps(H->pred.p, H->visible.p, Np, H->model.log_margin, H->model.slack, H->arena.p,
+                                 H->capn.p, d_capr, s2, H->pbound.p));
+    } else {
+        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->arena.p, (int64_t)Np + 1, 0u, s2));
+        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->capn.p, (int64_t)Np + 1, 0u, s2));
+    }
+    SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, kLcArena), s2));
+    if (s2 != s) SPR_HIP(hipEventRecord(H->ev_pjoin, s2));
+    SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, C, w, h, H->cuts.p,
+                              H->ncuts.p, H->counts.p, s, pcv));
     stage_mark(H, 2);
-    if (H->lb_dirty) {
-        // first learned frame after a change of scene, MVP or model: size the budgets from the
-        // arena total plus a proven upper bound of all intersections (the overflow pool can
-        // then never run out); records are at most two per sorted entry
+    SPR_HIP(spr::launch_scan_i32(H->counts.p, H->counts.p, C, ctl_at(H, z.st_count, z.counters, kLcCount), s));
+    SPR_HIP(spr::launch_path_segments(H->cpath.p, C, H->counts.p, Np, H->seg.p, s));
+    SPR_HIP(spr::launch_bucket_counts(H->row_lo.p, H->row_hi.p, Np, rmax, H->bucket_base.p, H->prow.p, s, pcv));
+    SPR_HIP(spr::launch_scan_i32(H->bucket_base.p, H->bucket_base.p, Np, ctl_at(H, z.st_pb, z.counters, kLcBucketBase),
+                                 s));
+    bool joined = s2 == s;
+    if (H->need_prediction) {
+        // the first frame after a load or a model change: read the predicted allocation back
+        // once (a batch takes the proven bound: it has no plain frame to fall back on)
+        if (!joined) { SPR_HIP(hipStreamWaitEvent(s, H->ev_pjoin, 0)); joined = true; }
         SPR_HIP(hipMemcpyAsync(H->h_pinned + 12, H->arena.p + Np, 4, hipMemcpyDeviceToHost, s));
         SPR_HIP(hipMemcpyAsync(H->h_pinned + 14, H->zero.p + z.bound, 8, hipMemcpyDeviceToHost, s));
         SPR_HIP(hipStreamSynchronize(s));
         const int64_t caps = std::max<int32_t>(H->h_pinned[12], 0);
         unsigned long long bound = 0;
         std::memcpy(&bound, H->h_pinned + 14, 8);
-        H->lb_n = caps + (int64_t)std::min<unsigned long long>(bound, 1ull << 40) + 1024;
-        H->lb_r = 2 * H->lb_n;
-        ++H->learned_fallbacks;
-        // the scan status arrays of phase 2 are sized by the budget: re-lay the control region
-        // (phase 1 is complete: its counters are no longer needed)
-        z = learned_layout(Np, nb_budget, H->lb_n);
+        const int64_t b = (int64_t)std::min<unsigned long long>(bound, 1ull << 40);
+        if (H->batch) grow_budget(H, b, 0);
+        else grow_budget(H, 0, caps);
+        H->need_prediction = false;
+        ++H->predictions;
+        // the status arrays of phase 2 are sized by the budget: re-lay the control region (phase 1
+        // is complete; of its results only the counts, segments and bucket bases, outside the
+        // region, are needed later)
+        z = learned_layout(C, Np, nb_budget, H->lb_n);
         H->zero.ensure(z.total);
         SPR_HIP(spr::launch_fill_u32((uint32_t*)H->zero.p, 2 * (int64_t)z.total, 0u, s));
+        d_capr = reinterpret_cast<int32_t*>(H->zero.p + z.capr);
     }
+    stage_mark(H, 3);
     int32_t* d_ctl = reinterpret_cast<int32_t*>(H->zero.p + z.ctl);
+    int32_t* d_live = d_ctl, *d_over = d_ctl + 1, *d_novf = d_ctl + 2;
     int32_t* d_err = reinterpret_cast<int32_t*>(H->zero.p + z.misc);
     int32_t* d_bigcount = d_err + 1;
-    d_capr = reinterpret_cast<int32_t*>(H->zero.p + z.capr);
     int32_t* d_totals = reinterpret_cast<int32_t*>(H->zero.p + z.totals);
     int32_t* d_nsorted = reinterpret_cast<int32_t*>(H->zero.p + z.slots);
     int32_t* d_nbands = d_nsorted + 1;
@@ -645,19 +693,11 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->bsum_l.p, max_bands, d_nb, 1, 0, 0u, s));  // bands <= buckets
     SPR_HIP(spr::launch_fill_u32_dev((uint32_t*)H->brec_l.p, 2 * nb_budget, d_nb, 2, 0, 0u, s));
 
-    // phase 2: exact counts placed in the arenas, then the plain stages on device counts
-    SPR_HIP(spr::launch_learned_count(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->pcurve.p, C,
-                                      Np, w, h,
-                                      H->arena.p, H->capn.p, H->lb_n, H->cuts.p, H->ncuts.p, H->counts.p,
-                                      H->cend.p, d_ctl, H->nexact.p, H->pbase.p, H->inter.p, H->wn.p, H->epath.p,
-                                      d_err, s, pcv));
-    stage_mark(H, 3);
+    // phase 2: the plain stages on device counts, within the budget
     int32_t* d_cc = reinterpret_cast<int32_t*>(H->zero.p + z.chainctl);
     int32_t* d_cfill = d_cc + spr::kCcFill;
     int32_t* d_coff = d_cc + spr::kCcOff;
     int32_t* d_nlong = d_cc + spr::kCcNLong;
-    int32_t* d_work = d_cc + spr::kCcWork;
-    const int32_t* d_live = d_ctl + 2;
     if (C) {
         H->chains.ensure((size_t)C * 10);
         H->pieces.ensure((size_t)C * 5);
@@ -667,9 +707,10 @@ void run_learned_frame(spr_handle* H) {
         H->lo_cnt.ensure(((size_t)C * 5 + 63) / 64);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
+        const spr::Budget B{nl, d_live, d_over};
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
                                        H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
-                                       d_cc, H->n_cu, s, H->cend.p, pcv));
+                                       d_cc, H->n_cu, s, &B, pcv));
         SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, d_cc + spr::kCcLightBegin,
                                         H->chain_order.p, H->n_cu, s));
     }
@@ -684,19 +725,19 @@ void run_learned_frame(spr_handle* H) {
     }
     stage_mark(H, 6);
     stage_mark(H, 7);
-    SPR_HIP(spr::launch_scan_i32(H->hist_l.p, H->hist_l.p, nb_budget, ctl_at(H, z.st_hist, z.counters, 2), s,
+    SPR_HIP(spr::launch_scan_i32(H->hist_l.p, H->hist_l.p, nb_budget, ctl_at(H, z.st_hist, z.counters, kLcHist), s,
                                  d_nb, d_nsorted));
     SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, H->hist_l.p, H->bflag.p, s));
-    SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, nb_budget, ctl_at(H, z.st_band, z.counters, 3), s,
-                                 d_nb, d_nbands));
+    SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, nb_budget, ctl_at(H, z.st_band, z.counters, kLcBandFlags),
+                                 s, d_nb, d_nbands));
     SPR_HIP(spr::launch_band_emit(H->bflag.p, H->hist_l.p, (int32_t)nb_budget, nl, H->band_start.p, s, d_nb,
                                   d_nsorted, H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr,
                                   narrow ? H->band_path.p : nullptr));
     SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, nl, H->bucket_base.p, H->prow.p,
                                        H->bflag.p, H->band_start.p, H->fill_l.p, H->bsum_l.p, H->sbig.p, d_err, s,
                                        d_live, narrow));
-    SPR_HIP(spr::launch_scan_i32(H->bsum_l.p, H->bsum_l.p, max_bands, ctl_at(H, z.st_wind, z.counters, 4), s,
-                                 d_nbands));
+    SPR_HIP(spr::launch_scan_i32(H->bsum_l.p, H->bsum_l.p, max_bands, ctl_at(H, z.st_wind, z.counters, kLcBandSums),
+                                 s, d_nbands));
     SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
     // documents of a batch scan their winding numbers from their own first entry: with the fused
     // mark + emit that base comes from the band prefixes, before the sort
@@ -706,10 +747,10 @@ void run_learned_frame(spr_handle* H) {
     stage_mark(H, 8);
     const spr::EmitArgs E{H->crec.p, H->brec_l.p, H->band_cnt.p, H->band_path.p, H->bucket_base.p, H->rule.p,
                           pcv, H->batch ? H->wbase.p : nullptr, w, h};
+    uint32_t* ticket = ctl_at(H, z.st_wind, z.counters, kLcSortTicket).counter;
     SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
                                    debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
-                                   H->big_list.p + max_bands, d_bigcount, H->bsum_l.p,
-                                   ctl_at(H, z.st_wind, z.counters, 6).counter, H->n_cu, s,
+                                   H->big_list.p + max_bands, d_bigcount, H->bsum_l.p, ticket, H->n_cu, s,
                                    narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr,
                                    side_for(H, nl, Np)));
     SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
@@ -726,7 +767,7 @@ void run_learned_frame(spr_handle* H) {
     stage_mark(H, 10);
     if (!narrow)
         SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, nl, w, h,
-                                      ctl_at(H, z.st_mark, z.counters, 5), debug ? H->records.p : nullptr,
+                                      ctl_at(H, z.st_mark, z.counters, kLcMark), debug ? H->records.p : nullptr,
                                       H->crec.p, debug ? H->flags.p : nullptr, d_totals, H->bucket_base.p, H->prow.p,
                                       H->brec_l.p, s, d_nsorted, H->sstart.p, Np, nr, d_err, pcv,
                                       H->batch ? H->wbase.p : nullptr));
@@ -735,6 +776,9 @@ void run_learned_frame(spr_handle* H) {
                                   (int)H->opts.compose, H->image.p, s, rec_cap, H->batch ? H->docs.p : nullptr,
                                   (int)H->n_docs, H->total_rows));
     stage_mark(H, 12);
+    // the prediction's paths that overflowed their capacity (statistics only)
+    if (!joined) SPR_HIP(hipStreamWaitEvent(s, H->ev_pjoin, 0));
+    if (use_model) SPR_HIP(spr::launch_path_overflow(H->seg.p, H->capn.p, H->visible.p, Np, d_novf, s));
     spr::SummarySrc S{};
     S.n = kSumCount;
     S.p[kSumCells] = d_totals;
@@ -742,11 +786,12 @@ void run_learned_frame(spr_handle* H) {
     S.p[kSumErr] = d_err;
     S.p[kSumWaveChains] = C ? d_nlong : nullptr;
     S.p[kSumHeavyPieces] = C ? d_cc + spr::kCcHeavyPieces : nullptr;
-    S.p[kSumLongBands] = reinterpret_cast<const int32_t*>(ctl_at(H, z.st_wind, z.counters, 6).counter + 1);
+    S.p[kSumLongBands] = reinterpret_cast<const int32_t*>(ticket + 1);
     S.p[kSumBigBands] = d_bigcount;
-    S.p[kSumPool] = d_ctl;
-    S.p[kSumOverflow] = d_ctl + 1;
-    S.p[kSumLive] = d_ctl + 2;
+    S.p[kSumTotal] = H->counts.p + C;
+    S.p[kSumOver] = d_over;
+    S.p[kSumOverflow] = d_novf;
+    S.p[kSumLive] = d_live;
     S.p[kSumArena] = H->arena.p + Np;
     S.p[kSumSorted] = d_nsorted;
     S.p[kSumCapR] = d_capr;
@@ -756,35 +801,41 @@ void run_learned_frame(spr_handle* H) {
     learned_finish(H);
 }
 
-// The end of a no-host-sync frame (eager or replayed graph): wait, check the device flags, and
-// keep the exact budgets of this scene for the next frames.
+// The end of a no-host-sync frame (eager or replayed graph): wait, check the device flags, fall
+// back to the plain method when the frame was over its budget, and grow the budget.
 void learned_finish(spr_handle* H) {
     SPR_HIP(hipStreamSynchronize(H->stream));
     const int32_t err = H->sum(kSumErr);
-    const int64_t caps = std::max<int32_t>(H->sum(kSumArena), 0), pool = std::max<int32_t>(H->sum(kSumPool), 0);
-    const int64_t r_need = (int64_t)H->sum(kSumCells) + H->sum(kSumSpans);
-    H->stats.n_overflow_paths = H->sum(kSumOverflow);
-    H->stats.alloc_intersections = caps;
+    const int64_t caps = std::max<int32_t>(H->sum(kSumArena), 0);
+    const int64_t total = H->sum(kSumTotal);
+    const int64_t overflow_paths = H->sum(kSumOverflow);
     H->frame_done = true;
     H->last_learned = true;
+    H->last_fallback = false;
     H->last_brec = H->brec_l.p;
     H->last_fused = !H->opts.debug && H->w <= spr::kNarrowMaxW;  // as the frame's narrow words
     if (err & 1) {
         H->h_pinned[16 + kSumErr] = 0;
         throw Fail{SPR_ERR_DEVICE, "internal: an entry fell outside its path's row bound"};
     }
-    if (err & 6) {
-        // cannot happen with bounded budgets; reported instead of rendered
-        H->h_pinned[16 + kSumErr] = 0;
+    if (H->sum(kSumOver)) {
+        // the exact total did not fit: the device culled the frame.  Render it with the plain
+        // method (a batch cannot be: its budget is the proven bound) and cover it from now on.
+        if (H->batch) throw Fail{SPR_ERR_DEVICE, "internal: a batch frame exceeded its proven bound"};
+        ++H->learned_fallbacks;
+        run_frame(H);
+        grow_budget(H, H->N, caps);
         H->lb_dirty = true;
-        throw Fail{SPR_ERR_DEVICE, "internal: learned frame exceeded its bounded budget"};
+        H->last_fallback = true;
+        H->stats.n_overflow_paths = overflow_paths;
+        H->stats.alloc_intersections = caps;
+        return;
     }
+    H->stats.n_overflow_paths = overflow_paths;
+    H->stats.alloc_intersections = caps;
     H->N = H->sum(kSumSorted);
     H->n_live = H->sum(kSumLive);
-    // the same scene, MVP and model give the same counts: later frames need exactly this much
-    H->lb_n = caps + pool + std::max<int64_t>((caps + pool) / 64, 1024);
-    H->lb_r = r_need + std::max<int64_t>(r_need / 64, 1024);
-    H->lb_dirty = false;
+    grow_budget(H, total, caps);
 }
 
 void drop_graph(spr_handle* H) {
@@ -816,12 +867,12 @@ void capture_frame(spr_handle* H) {
     H->graph_lb_r = H->lb_r;
 }
 
-// Frames without a host synchronisation inside (learned mode, batches).  Once the budgets of
-// the current scene, MVP and model are exact and stable, the frame is captured once and later
-// frames replay the graph: one launch per frame.
+// Frames without a host synchronisation inside (learned mode, batches).  While the scene, MVP,
+// model and budget stay the same, the frame is captured once and later frames replay the graph:
+// one launch per frame.
 void render_nosync(spr_handle* H) {
     hipStream_t s = H->stream;
-    if (!H->lb_dirty && H->gexec && H->graph_lb_n == H->lb_n && H->graph_lb_r == H->lb_r && !H->stage_timing) {
+    if (!H->lb_dirty && !H->need_prediction && H->gexec && H->graph_lb_n == H->lb_n && !H->stage_timing) {
         SPR_HIP(hipEventRecord(H->ev_begin, s));
         SPR_HIP(hipGraphLaunch(H->gexec, s));
         SPR_HIP(hipEventRecord(H->ev_end, s));
@@ -830,10 +881,13 @@ void render_nosync(spr_handle* H) {
         return;
     }
     drop_graph(H);
-    const bool was_dirty = H->lb_dirty;
-    const int64_t n0 = H->lb_n, r0 = H->lb_r;
+    const bool predicting = H->need_prediction;
+    const int64_t n0 = H->lb_n;
     run_learned_frame(H);
-    if (!was_dirty && !H->lb_dirty && H->lb_n == n0 && H->lb_r == r0 && !H->stage_timing) capture_frame(H);
+    if (H->last_fallback) return;  // the plain frame rendered it; the next learned frame runs eagerly
+    H->lb_dirty = false;
+    // capture once a frame ran eagerly on an unchanged budget
+    if (!predicting && H->lb_n == n0 && !H->stage_timing) capture_frame(H);
 }
 
 template <class F>
@@ -931,6 +985,7 @@ void load_arrays(spr_handle* H, const float* pos, const uint32_t* pos_path, uint
     H->inj = false;
     H->inj_inter.clear();
     H->lb_dirty = true;
+    H->need_prediction = true;
     H->batch = false;
     drop_graph(H);
 }
@@ -966,7 +1021,8 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
         (void)hipGetLastError();
         SPR_HIP(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
         SPR_HIP(hipStreamCreateWithFlags(&H->stream2, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&H->ev_fork, &H->ev_light, &H->ev_small, &H->ev_join, &H->ev_sfork, &H->ev_sjoin})
+        for (hipEvent_t* e : {&H->ev_fork, &H->ev_light, &H->ev_small, &H->ev_join, &H->ev_sfork, &H->ev_sjoin,
+                              &H->ev_pfork, &H->ev_pjoin})
             SPR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
         SPR_HIP(hipEventCreate(&H->ev_begin));
         SPR_HIP(hipEventCreate(&H->ev_end));
@@ -1001,15 +1057,16 @@ int spr_destroy(spr_handle* H) {
     H->pcanvas.release(); H->docs.release(); H->doc_first.release(); H->wbase.release();
     H->mlpdev.release();
     H->pcurve.release(); H->feat.release(); H->pred.release(); H->train_feat.release(); H->train_tgt.release();
-    H->arena.release(); H->capn.release(); H->nexact.release(); H->pbase.release(); H->sstart.release();
-    H->hist_l.release(); H->fill_l.release(); H->bsum_l.release(); H->brec_l.release(); H->cend.release(); H->pbound.release();
+    H->arena.release(); H->capn.release(); H->sstart.release();
+    H->hist_l.release(); H->fill_l.release(); H->bsum_l.release(); H->brec_l.release(); H->pbound.release();
     H->summary.release();
     H->scan_ws.release();
     if (H->h_pinned) (void)hipHostFree(H->h_pinned);
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
     if (H->ev_end) (void)hipEventDestroy(H->ev_end);
-    for (hipEvent_t e : {H->ev_fork, H->ev_light, H->ev_small, H->ev_join, H->ev_sfork, H->ev_sjoin})
+    for (hipEvent_t e : {H->ev_fork, H->ev_light, H->ev_small, H->ev_join, H->ev_sfork, H->ev_sjoin, H->ev_pfork,
+                         H->ev_pjoin})
         if (e) (void)hipEventDestroy(e);
     if (H->stream2) (void)hipStreamDestroy(H->stream2);
     if (H->stream) (void)hipStreamDestroy(H->stream);
@@ -1216,7 +1273,7 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 H->train_tgt.ensure((size_t)std::max<uint32_t>(H->Np, 1) * 2);
                 DBuf<int32_t> exact;
                 exact.ensure((size_t)std::max<uint32_t>(H->Np, 1) * 2);
-                SPR_HIP(spr::launch_path_targets(H->last_learned ? H->nexact.p : H->seg.p, H->last_learned ? 0 : 1,
+                SPR_HIP(spr::launch_path_targets(H->seg.p, 1,
                                                  H->last_brec, H->bucket_base.p, H->Np, H->train_tgt.p, exact.p,
                                                  H->stream));
                 SPR_HIP(hipStreamSynchronize(H->stream));  // the copy below runs on another stream
@@ -1311,6 +1368,7 @@ int spr_load_batch(spr_handle* H, const spr_scene* const* docs, const uint32_t* 
         H->max_h = max_h;
         H->img_bytes = img;
         H->lb_dirty = true;
+        H->need_prediction = true;
     });
 }
 
@@ -1418,6 +1476,7 @@ int spr_learned_train(spr_handle* H, const spr_scene* sc, const spr_train_opts* 
         H->frame_done = false;
         H->last_brec = nullptr;
         H->lb_dirty = true;
+        H->need_prediction = true;
         if (out) {
             out->samples = rep.samples;
             out->steps = rep.steps;
@@ -1444,6 +1503,7 @@ int spr_learned_load(spr_handle* H, const char* path) {
         try { spr::mlp_load(H->model, path); } catch (const std::exception& e) { throw Fail{SPR_ERR_IO, e.what()}; }
         H->mlpdev.release();
         H->lb_dirty = true;
+        H->need_prediction = true;
     });
 }
 
@@ -1453,7 +1513,7 @@ int spr_learned_set_margin(spr_handle* H, float log2_margin, int32_t slack) {
         if (!(log2_margin > -30.0f && log2_margin < 30.0f) || slack < 0) throw Fail{SPR_ERR_INVALID, "bad margin"};
         // the capacities change: the budgets and any captured frame (whose kernels hold the old
         // margin as an argument) are stale
-        if (H->model.log_margin != log2_margin || H->model.slack != slack) H->lb_dirty = true;
+        if (H->model.log_margin != log2_margin || H->model.slack != slack) H->lb_dirty = H->need_prediction = true;
         H->model.log_margin = log2_margin;
         H->model.slack = slack;
     });
@@ -1483,6 +1543,7 @@ int spr_mlp_set_params(spr_handle* H, const float* src, size_t n) {
         H->model.ready = true;
         H->mlpdev.release();
         H->lb_dirty = true;
+        H->need_prediction = true;
     });
 }
 
