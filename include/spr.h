@@ -157,6 +157,15 @@ int spr_get_stats(const spr_handle* h, spr_stats* out);
  *     Runs on `stream` (a hipStream_t; NULL = the handle's stream) and returns without waiting;
  *     the handle's scan workspace grows on demand, so calls on one handle must not overlap. */
 int spr_scan_i32(spr_handle* h, const int32_t* d_in, int32_t* d_out, uint64_t n, void* stream);
+/* --- the same scan, allocating nothing.  No counterpart in the reference: it is for callers
+ *     that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ *     forbid hipMalloc).  Its workspace (tile statuses and the tile counter) is a static device
+ *     array of 512 KiB per GPU, held until the process ends; so n <= 2^28 (268,435,456), and
+ *     larger n is rejected with SPR_ERR_CAPACITY.  Runs on the calling thread's current device,
+ *     on `stream` (a hipStream_t; NULL = that device's null stream).  Every call on one GPU
+ *     shares the array: calls that overlap on one GPU (on different streams) race; calls on
+ *     one stream are ordered.  Errors are read with spr_global_error(). */
+int spr_scan_i32_noalloc(const int32_t* d_in, int32_t* d_out, uint64_t n, void* stream);
 
 /* --- debug access to intermediate buffers (the reference's drawDebug dumps,
  *     src/core/scanline/scanline_rasterizer.cpp:698-802).  export copies the last frame's stage

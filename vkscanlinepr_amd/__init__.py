@@ -92,7 +92,7 @@ EXPORTS = [
     "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free", "spr_learned_train",
     "spr_learned_save", "spr_learned_load", "spr_learned_set_margin", "spr_mlp_get_params",
     "spr_mlp_set_params", "spr_mlp_predict", "spr_mlp_gradients", "spr_load_batch", "spr_batch_image",
-    "spr_autotune", "spr_scan_i32",
+    "spr_autotune", "spr_scan_i32", "spr_scan_i32_noalloc",
 ]
 
 
@@ -141,6 +141,7 @@ def lib() -> C.CDLL:
     L.spr_batch_image.argtypes = [vp, C.c_uint32, C.c_void_p, C.c_size_t]
     L.spr_autotune.argtypes = [vp, C.c_uint32, C.POINTER(TuneReport)]
     L.spr_scan_i32.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+    L.spr_scan_i32_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
     _lib = L
     return L
 
@@ -154,6 +155,13 @@ def _check(rc: int, handle=None) -> None:
 
 def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def scan_i32_noalloc(d_in: int, d_out: int, n: int, stream: int | None = None) -> None:
+    """spr_scan_i32_noalloc: the exclusive scan with a static device workspace (no allocation),
+    on the current device and `stream` (a hipStream_t handle; None = the null stream)."""
+    _check(lib().spr_scan_i32_noalloc(C.c_void_p(d_in), C.c_void_p(d_out), int(n),
+                                      C.c_void_p(stream) if stream else None))
 
 
 class Scene:

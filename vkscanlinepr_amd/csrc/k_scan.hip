@@ -12,6 +12,8 @@
 // which removes the reference's third host synchronisation.  It runs in debug frames (and on
 // canvases too wide for the narrow sort words); the other frames do mark + emit inside the band
 // sort (k_sort.hip).
+#include <mutex>
+
 #include "kernels.hpp"
 #include "scan_common.hpp"
 
@@ -308,6 +310,50 @@ __global__ __launch_bounds__(kMarkThreads) void k_mark_emit(
 }
 
 }  // namespace
+
+// ---- the scan without allocation: its tile statuses and tile counter live in a static device
+// array (one copy per GPU, held until the process ends).  Host code reaches that array only through
+// the address hipGetSymbolAddress returns on each device; the array's name is not a device address.
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned long long scratch[kNoAllocScanTiles + 1];
+unsigned long long* g_scratch_ptr[kNoAllocMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// The calling thread's current device's copy of `scratch` (looked up once per device).
+hipError_t scratch_for_current_device(unsigned long long** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<unsigned long long*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
+hipError_t launch_scan_i32_noalloc(const int32_t* in, int32_t* out, int64_t n, hipStream_t s) {
+    if (n < 0 || n > kNoAllocScanMaxN) return hipErrorInvalidValue;
+    unsigned long long* ws = nullptr;
+    hipError_t e = scratch_for_current_device(&ws);
+    if (e != hipSuccess) return e;
+    const int64_t tiles = scan_tiles(n > 0 ? n : 1);
+    // the statuses and the counter must start at zero: a fill kernel on the same stream (the
+    // array keeps the previous call's contents)
+    e = launch_fill_u32(reinterpret_cast<uint32_t*>(ws), 2 * (tiles + 1), 0u, s);
+    if (e != hipSuccess) return e;
+    ScanCtl c;
+    c.status = ws;
+    c.counter = reinterpret_cast<uint32_t*>(ws + tiles);
+    c.epoch = 0;
+    return launch_scan_i32(in, out, n, c, s);
+}
 
 hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s,
                            const int32_t* n_dev, int32_t* total_out) {

@@ -138,6 +138,11 @@ hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpo
 // ---- scans
 hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s,
                            const int32_t* n_dev = nullptr, int32_t* total_out = nullptr);
+// The same scan with its workspace in a static device array instead of caller or handle memory:
+// no allocation at all.  n <= kNoAllocScanMaxN; calls that overlap on one GPU race on the array.
+constexpr int64_t kNoAllocScanMaxN = (int64_t)1 << 28;
+constexpr int64_t kNoAllocScanTiles = (kNoAllocScanMaxN + kScanTile - 1) / kScanTile;
+hipError_t launch_scan_i32_noalloc(const int32_t* in, int32_t* out, int64_t n, hipStream_t s);
 
 // ---- (path, row) buckets and the band sort (a9 + a10): per path segment, ascending
 // (key ^ 0x80000000, index); the winding change rides along in the low bits of the sort word.

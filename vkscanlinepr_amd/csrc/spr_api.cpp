@@ -1183,6 +1183,21 @@ int spr_scan_i32(spr_handle* H, const int32_t* d_in, int32_t* d_out, uint64_t n,
     });
 }
 
+int spr_scan_i32_noalloc(const int32_t* d_in, int32_t* d_out, uint64_t n, void* stream) {
+    // no allocation on any path: errors are string literals copied into a fixed buffer
+    if (!d_out || (n && !d_in)) { g_error = "spr_scan_i32_noalloc: null device pointer"; return SPR_ERR_INVALID; }
+    if (n > (uint64_t)spr::kNoAllocScanMaxN) {
+        g_error = "spr_scan_i32_noalloc: n exceeds 268435456, the size of the static workspace";
+        return SPR_ERR_CAPACITY;
+    }
+    const hipError_t e = spr::launch_scan_i32_noalloc(d_in, d_out, (int64_t)n, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        g_error = hipGetErrorString(e);
+        return SPR_ERR_DEVICE;
+    }
+    return SPR_OK;
+}
+
 int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nbytes) {
     if (!H || !nbytes) { g_error = "null argument"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
