@@ -14,6 +14,7 @@
 // sort (k_sort.hip).
 #include <mutex>
 
+#include "device_common.hpp"
 #include "kernels.hpp"
 #include "scan_common.hpp"
 
@@ -22,64 +23,93 @@ namespace spr {
 namespace {
 
 // n_dev (optional) holds the element count on the device (the launch covers n_max); total_out
-// (optional) also receives the total.  ITEMS consecutive elements per thread: large scans use
-// 32 (8192-element tiles), which halves the tiles that take a ticket from the one counter.
-template <int ITEMS>
-__global__ __launch_bounds__(kScanThreads) void k_scan_i32(const int32_t* in, int32_t* out, int64_t n_max,
-                                                           const int32_t* __restrict__ n_dev,
-                                                           int32_t* __restrict__ total_out,
-                                                           unsigned long long* __restrict__ status,
-                                                           uint32_t* __restrict__ counter) {
+// (optional) also receives the total.  A tile is V x THREADS int4: load k of wave w brings in the 64
+// consecutive int4 of segment (k, w) with one coalesced 1 KiB access; each wave scans its segment in
+// registers (lane sums, DPP wave scan), wave 0 scans the V x W segment totals (up to four per lane)
+// and does the tile's look-back, and the results go back with the same coalesced stores.  No LDS
+// staging: many tiles stay in flight, which hides the look-back's latency across XCDs.
+template <int V, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_scan_i32(const int32_t* in, int32_t* out, int64_t n_max,
+                                                      const int32_t* __restrict__ n_dev,
+                                                      int32_t* __restrict__ total_out,
+                                                      unsigned long long* __restrict__ status,
+                                                      uint32_t* __restrict__ counter) {
+    constexpr int W = THREADS / 64, SEG = V * W, PER = (SEG + 63) / 64;
+    static_assert(SEG <= 256, "wave 0 scans the segment totals, four per lane");
+    constexpr int TILE = 4 * V * THREADS;
+    __shared__ int32_t s_seg[64 * PER];
     __shared__ int32_t s_tile;
-    __shared__ int32_t s_w[kWaves];
-    __shared__ int32_t s_prefix;
     const int64_t n = n_dev ? min((int64_t)*n_dev, n_max) : n_max;
     // a launch sized for a budget: the blocks past the live tiles leave without a ticket, so the
     // live tiles' tickets go to exactly the first blocks (no atomics spent on the surplus)
-    if ((int64_t)blockIdx.x * (ITEMS * kScanThreads) >= (n > 0 ? n : 1)) return;
+    if ((int64_t)blockIdx.x * TILE >= (n > 0 ? n : 1)) return;
     const int64_t tile = ordered_tile_id(counter, &s_tile);
-    const int64_t base = tile * (int64_t)(ITEMS * kScanThreads);
+    const int64_t base = tile * (int64_t)TILE;
     if (base >= n && tile != 0) return;
-    const int64_t i0 = base + (int64_t)threadIdx.x * ITEMS;
-    int32_t v[ITEMS];
-    const bool vec = (i0 + ITEMS <= n) && ((((uintptr_t)(in + i0)) & 15u) == 0) &&
-                     ((((uintptr_t)(out + i0)) & 15u) == 0);
-    if (vec) {
-        const int4* p = reinterpret_cast<const int4*>(in + i0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool vec = base + TILE <= n && ((((uintptr_t)(in + base)) & 15u) == 0) &&
+                     ((((uintptr_t)(out + base)) & 15u) == 0);
+    int4 q[V];
 #pragma unroll
-        for (int k = 0; k < ITEMS / 4; ++k) {
-            int4 q = p[k];
-            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+    for (int k = 0; k < V; ++k) {
+        const int idx = (k * W + wave) * 64 + lane;
+        if (vec) {
+            q[k] = reinterpret_cast<const int4*>(in + base)[idx];
+        } else {
+            const int64_t e = base + 4 * (int64_t)idx;
+            q[k].x = e < n ? in[e] : 0;
+            q[k].y = e + 1 < n ? in[e + 1] : 0;
+            q[k].z = e + 2 < n ? in[e + 2] : 0;
+            q[k].w = e + 3 < n ? in[e + 3] : 0;
         }
-    } else {
-#pragma unroll
-        for (int j = 0; j < ITEMS; ++j) v[j] = (i0 + j < n) ? in[i0 + j] : 0;
     }
-    int32_t run = 0;
+    int32_t lex[V];
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) { int32_t x = v[j]; v[j] = run; run = (int32_t)((uint32_t)run + (uint32_t)x); }
-    int32_t total;
-    const int32_t texcl = block_exclusive_add(run, s_w, total);
-    if (threadIdx.x < 64) {
-        const int32_t pre = lookback<OpI32>(status, tile, total);
-        if (threadIdx.x == 0) s_prefix = pre;
+    for (int k = 0; k < V; ++k) {
+        const int32_t sm = wadd(wadd(q[k].x, q[k].y), wadd(q[k].z, q[k].w));
+        const int32_t inc = wave_inclusive_add(sm);
+        lex[k] = (int32_t)((uint32_t)inc - (uint32_t)sm);
+        if (lane == 63) s_seg[k * W + wave] = inc;
     }
     __syncthreads();
-    const int32_t off = (int32_t)((uint32_t)s_prefix + (uint32_t)texcl);
-    if (vec) {
-        int4* p = reinterpret_cast<int4*>(out + i0);
+    if (wave == 0) {
+        int32_t t[PER], ls = 0;
 #pragma unroll
-        for (int k = 0; k < ITEMS / 4; ++k)
-            p[k] = make_int4(off + v[4 * k], off + v[4 * k + 1], off + v[4 * k + 2], off + v[4 * k + 3]);
-    } else {
+        for (int j = 0; j < PER; ++j) {
+            t[j] = PER * lane + j < SEG ? s_seg[PER * lane + j] : 0;
+            ls = wadd(ls, t[j]);
+        }
+        const int32_t inc = wave_inclusive_add(ls);
+        const int32_t total = __shfl(inc, 63);
+        const int32_t pre = lookback<OpI32>(status, tile, total);
+        int32_t run = wadd(pre, (int32_t)((uint32_t)inc - (uint32_t)ls));
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j)
-            if (i0 + j < n) out[i0 + j] = (int32_t)((uint32_t)off + (uint32_t)v[j]);
+        for (int j = 0; j < PER; ++j) {
+            if (PER * lane + j < SEG) s_seg[PER * lane + j] = run;
+            run = wadd(run, t[j]);
+        }
+        const bool last_tile = (n == 0) ? (tile == 0) : (base <= n - 1 && n - 1 < base + TILE);
+        if (last_tile && lane == 0) {
+            const int32_t tot = wadd(pre, total);
+            out[n] = tot;
+            if (total_out) *total_out = tot;
+        }
     }
-    const bool last_tile = (n == 0) ? (tile == 0) : (base <= n - 1 && n - 1 < base + (ITEMS * kScanThreads));
-    if (last_tile && threadIdx.x == 0) {
-        out[n] = (int32_t)((uint32_t)s_prefix + (uint32_t)total);
-        if (total_out) *total_out = out[n];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const int idx = (k * W + wave) * 64 + lane;
+        const int32_t a = wadd(s_seg[k * W + wave], lex[k]);
+        const int32_t b = wadd(a, q[k].x), c = wadd(b, q[k].y), d = wadd(c, q[k].z);
+        if (vec) {
+            reinterpret_cast<int4*>(out + base)[idx] = make_int4(a, b, c, d);
+        } else {
+            const int64_t e = base + 4 * (int64_t)idx;
+            if (e < n) out[e] = a;
+            if (e + 1 < n) out[e + 1] = b;
+            if (e + 2 < n) out[e + 2] = c;
+            if (e + 3 < n) out[e + 3] = d;
+        }
     }
 }
 
@@ -357,15 +387,15 @@ hipError_t launch_scan_i32_noalloc(const int32_t* in, int32_t* out, int64_t n, h
 
 hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s,
                            const int32_t* n_dev, int32_t* total_out) {
-    // the status array holds scan_tiles(n) (4096-element) entries, enough for either tile size
+    // the status array holds scan_tiles(n) (4096-element) entries, enough for either tile shape:
+    // 32768-element tiles (8 int4 x 1024 threads) for large scans, 4096-element tiles otherwise
     if (n >= ((int64_t)1 << 20)) {
-        const int64_t tiles = (n + 32 * kScanThreads - 1) / (32 * kScanThreads);
-        k_scan_i32<32><<<(unsigned)tiles, kScanThreads, 0, s>>>(in, out, n, n_dev, total_out, ctl.status,
-                                                                ctl.counter);
+        constexpr int64_t kTile = 4 * 8 * 1024;
+        const int64_t tiles = (n + kTile - 1) / kTile;
+        k_scan_i32<8, 1024><<<(unsigned)tiles, 1024, 0, s>>>(in, out, n, n_dev, total_out, ctl.status, ctl.counter);
     } else {
         const int64_t tiles = n > 0 ? scan_tiles(n) : 1;
-        k_scan_i32<16><<<(unsigned)tiles, kScanThreads, 0, s>>>(in, out, n, n_dev, total_out, ctl.status,
-                                                                ctl.counter);
+        k_scan_i32<4, 256><<<(unsigned)tiles, 256, 0, s>>>(in, out, n, n_dev, total_out, ctl.status, ctl.counter);
     }
     return hipGetLastError();
 }

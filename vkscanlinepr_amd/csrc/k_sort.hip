@@ -346,6 +346,174 @@ __device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __re
     __syncthreads();
 }
 
+// ---- production bands: one wave per band, sorted in registers
+//
+// Frames that export no stage sort their narrow words band by band with one wave each.  A band of
+// n <= 2048 words is padded to 64 R words (R = 1, 2, 4, ..., 32 per lane: the smallest that holds
+// it) and sorted by a bitonic network over the element index i = 64-lane-block l R + r: stages
+// whose stride is below R compare two registers of one lane, the others exchange register r with
+// lane l ^ (stride / R) (ds_bpermute).  The sorted words then give the winding numbers (the lane's
+// prefix, a DPP wave scan and the band's prefix) and mark + emit runs from the registers: no LDS,
+// no block barrier, and four bands per 256-thread workgroup.
+constexpr int kWaveBandMax = 64 * 32;
+constexpr int kShortBand = 64 * kItems;  // the route counter lists the bands above it
+
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint32_t (&v)[R]) {
+    const int lane = threadIdx.x & 63;
+    constexpr int P = 64 * R;
+#pragma unroll
+    for (int k = 2; k <= P; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < R) {
+                // ascending where bit k of the element index is 0: below R it is a bit of r,
+                // from R up a bit of the lane
+                const bool asc_lane = ((lane * R) & k) == 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int r2 = r ^ j;
+                    if (r2 <= r) continue;
+                    const bool asc = k < R ? (r & k) == 0 : asc_lane;
+                    const uint32_t a = v[r], b = v[r2];
+                    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+                    v[r] = asc ? lo : hi;
+                    v[r2] = asc ? hi : lo;
+                }
+            } else {
+                const int d = j / R;
+                const bool keep_min = ((lane & d) == 0) == (((lane * R) & k) == 0);
+                const int addr = (lane ^ d) << 2;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t o = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v[r]);
+                    const uint32_t lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
+                    v[r] = keep_min ? lo : hi;
+                }
+            }
+        }
+    }
+}
+
+// One band of n words at src (R per lane): sort, winding numbers from the band's prefix `pre`,
+// mark + emit (mark_merged_fragment_and_span.comp:48-120, gen_merged_fragment_and_span.comp:
+// 152-222: a cell is an in-canvas entry whose key differs from the previous entry's, a span runs
+// from the previous entry of the same cell row to this one when the winding number fills), the
+// record range of each of the band's buckets and its (cells, spans).
+template <int R>
+__device__ __forceinline__ void wave_band(const uint32_t* __restrict__ src, int n, int64_t band, int64_t start,
+                                          int32_t pre, int2 pr, const EmitArgs& EA) {
+    const int lane = threadIdx.x & 63;
+    uint32_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane * R + r;
+        v[r] = i < n ? src[i] : Narrow::kPad;  // pads sort to the end
+    }
+    wave_bitonic<R>(v);
+    Narrow wd;
+    wd.pr = pr;
+    int32_t wl[R], key[R], sum = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        wl[r] = sum;
+        key[r] = wd.key(v[r]);
+        if (lane * R + r < n) sum = wadd(sum, wn_of(v[r]));
+    }
+    const int32_t lane_pre = wadd(pre, (int32_t)((uint32_t)wave_inclusive_add(sum) - (uint32_t)sum));
+    const BandPath b = band_path_of(EA, band, pr, start);
+    const int32_t prev_last = __shfl_up(key[R - 1], 1);
+    uint32_t cbits = 0, sbits = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane * R + r;
+        if (i >= n) continue;
+        const int32_t k1 = key[r], kprev = r ? key[r - 1] : prev_last;
+        const int x1 = key_x(k1), y1 = key_y(k1);
+        const bool in_canvas = !(x1 < 0 || y1 < 0 || x1 >= b.cw || y1 >= b.ch);
+        const bool cell = in_canvas && (i == 0 || kprev != k1);
+        bool span = false;
+        if (i > 0) {
+            const int32_t wn = (int32_t)((uint32_t)wadd(lane_pre, wl[r]) - (uint32_t)b.wb);
+            const bool wf = (b.rule == 0u && wn != 0) || (b.rule == 1u && (wn & 1) != 0);
+            span = key_y(kprev) == y1 && key_x(kprev) + 2 < x1 && wf;
+        }
+        cbits |= (uint32_t)cell << r;
+        sbits |= (uint32_t)span << r;
+    }
+    // record slots: cells in the low half, spans in the high half (at most 2048 of each per band)
+    const int32_t mine = __popc(cbits) | (__popc(sbits) << 16);
+    const int32_t inc = wave_inclusive_add(mine);
+    const int32_t ex = inc - mine;
+    int64_t pos = b.r0 + (ex & 0xFFFF) + (ex >> 16);
+    const int32_t next_first = __shfl_down(key[0], 1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane * R + r;
+        if (i >= n) continue;
+        const int32_t k1 = key[r], kprev = r ? key[r - 1] : prev_last;
+        const int cell = (cbits >> r) & 1, span = (sbits >> r) & 1;
+        const int x1 = key_x(k1), y1 = key_y(k1);
+        if (cell) EA.crec[pos] = (uint32_t)x1 | (2u << 16);
+        if (span) {
+            const int x0 = max(0, key_x(kprev) + 2);
+            EA.crec[pos + cell] = (uint32_t)x0 | ((uint32_t)(x1 - x0) << 16);
+        }
+        // record range of this entry's (path, cell row) bucket, for the composite
+        if ((uint32_t)k1 != kSentinelKey && y1 >= 0 && y1 < b.ch) {
+            const int32_t kn = i + 1 < n ? (r + 1 < R ? key[r + 1 < R ? r + 1 : r] : next_first) : 0;
+            const bool first = i == 0 || (kprev >> 16) != (k1 >> 16);
+            const bool last = i == n - 1 || (kn >> 16) != (k1 >> 16);
+            if (first || last) {
+                const int row = y1 >> 1;
+                const int loc = row == 0 ? b.pr.y + 1 : row - b.pr.x;
+                if (loc >= 0 && loc < b.pr.y + 2) {  // out of range only after a flagged bound error
+                    int2* br = &EA.brec[b.bb + loc];
+                    if (first) br->x = (int32_t)pos;
+                    if (last) br->y = (int32_t)(pos + cell + span);
+                }
+            }
+        }
+        pos += cell + span;
+    }
+    if (lane == 63) EA.band_cnt[band] = make_int2(inc & 0xFFFF, inc >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_sort_bands_wave(const int32_t* __restrict__ band_start,
+                                                         const int32_t* __restrict__ n_bands,
+                                                         const uint32_t* __restrict__ src,
+                                                         const int32_t* __restrict__ bpre,
+                                                         const int2* __restrict__ band_pr,
+                                                         int32_t* __restrict__ big_list,
+                                                         int32_t* __restrict__ big_excl,
+                                                         int32_t* __restrict__ big_count,
+                                                         uint32_t* __restrict__ long_count, EmitArgs EA) {
+    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= *n_bands) return;  // wave-uniform
+    const int64_t s = band_start[t], n = band_start[t + 1] - s;
+    if (n > kWaveBandMax) {
+        // a single oversized (path, row) bucket: sorted and emitted by the global-merge kernels
+        if (lane == 0) {
+            const int32_t q = atomicAdd(big_count, 1);
+            big_list[q] = (int32_t)t;
+            big_excl[q] = bpre[t];
+        }
+        return;
+    }
+    if (n > kShortBand && lane == 0) atomicAdd(long_count, 1u);
+    const uint32_t* p = src + s;
+    const int2 pr = band_pr[t];
+    const int32_t pre = bpre[t];
+    const int m = (int)n;
+    if (m <= 64) wave_band<1>(p, m, t, s, pre, pr, EA);
+    else if (m <= 128) wave_band<2>(p, m, t, s, pre, pr, EA);
+    else if (m <= 256) wave_band<4>(p, m, t, s, pre, pr, EA);
+    else if (m <= 512) wave_band<8>(p, m, t, s, pre, pr, EA);
+    else if (m <= 1024) wave_band<16>(p, m, t, s, pre, pr, EA);
+    else wave_band<32>(p, m, t, s, pre, pr, EA);
+}
+
 // Long bands (above one 64-thread tile) are listed in `list`, their number in *count, with one
 // atomic per 1024-thread block (the waves' counts are added up in LDS); the order of the list
 // does not matter.
@@ -614,6 +782,13 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
     const Unpacked out{skey, sidx, winding};
     if (band_pr) {
         if (sidx) return hipErrorInvalidValue;  // narrow words hold no index
+        if (emit) {
+            // production frames: one wave per band, four bands per workgroup
+            k_sort_bands_wave<<<(unsigned)((max_bands + 3) / 4), 256, 0, s>>>(
+                band_start, n_bands, reinterpret_cast<const uint32_t*>(src), bpre, band_pr, big_list, big_excl,
+                big_count, ticket + 1, *emit);
+            return hipGetLastError();
+        }
         return launch_classes<Narrow>(band_start, n_bands, max_bands, reinterpret_cast<const uint32_t*>(src), out,
                                       bpre, ticket, big_list, big_excl, big_count, n_cu, s, band_pr, emit, side);
     }
