@@ -139,8 +139,9 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
 }
 
 // Band starts: a band is a run of consecutive buckets of one path whose entries start in the
-// same 1024-entry window; a bucket of more than 1024 entries is a band of its own.  Every band
-// therefore holds at most 2048 entries (one sort tile) unless it is a single larger bucket.
+// same 512-entry window (kBandWindowLog); a bucket of more than 512 entries is a band of its own.
+// Every band therefore holds at most 1024 entries (sorted by one wave, 16 per lane) unless it is a
+// single larger bucket.
 // One wave per path: the most recent non-empty bucket before each bucket is the nearest lane
 // below it in the ballot of non-empty buckets (or, for the chunk's first ones, the last
 // non-empty bucket of the earlier chunks), read with one shuffle.
@@ -169,7 +170,8 @@ __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict
                     f = 1;
                 } else {
                     const int32_t ps = below ? ps_l : cps, pz = below ? pz_l : cpz;
-                    f = (o >> 10) != (ps >> 10) || pz > 1024 || sz > 1024;
+                    constexpr int32_t kWin = 1 << kBandWindowLog;
+                    f = (o >> kBandWindowLog) != (ps >> kBandWindowLog) || pz > kWin || sz > kWin;
                 }
             }
             flag[b] = f;

@@ -32,6 +32,8 @@
 // its start, the record range of each of its buckets and its (cells, spans); the oversized bands
 // do the same from their sorted keys (k_emit_big_bands).  Debug frames write keys, indices and
 // winding numbers, and k_mark_emit (k_scan.hip) emits the records in the reference's order.
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 #include "scan_common.hpp"
@@ -346,174 +348,6 @@ __device__ __forceinline__ void tile_sort(const W& wd, const typename W::T* __re
     __syncthreads();
 }
 
-// ---- production bands: one wave per band, sorted in registers
-//
-// Frames that export no stage sort their narrow words band by band with one wave each.  A band of
-// n <= 2048 words is padded to 64 R words (R = 1, 2, 4, ..., 32 per lane: the smallest that holds
-// it) and sorted by a bitonic network over the element index i = 64-lane-block l R + r: stages
-// whose stride is below R compare two registers of one lane, the others exchange register r with
-// lane l ^ (stride / R) (ds_bpermute).  The sorted words then give the winding numbers (the lane's
-// prefix, a DPP wave scan and the band's prefix) and mark + emit runs from the registers: no LDS,
-// no block barrier, and four bands per 256-thread workgroup.
-constexpr int kWaveBandMax = 64 * 32;
-constexpr int kShortBand = 64 * kItems;  // the route counter lists the bands above it
-
-template <int R>
-__device__ __forceinline__ void wave_bitonic(uint32_t (&v)[R]) {
-    const int lane = threadIdx.x & 63;
-    constexpr int P = 64 * R;
-#pragma unroll
-    for (int k = 2; k <= P; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j < R) {
-                // ascending where bit k of the element index is 0: below R it is a bit of r,
-                // from R up a bit of the lane
-                const bool asc_lane = ((lane * R) & k) == 0;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int r2 = r ^ j;
-                    if (r2 <= r) continue;
-                    const bool asc = k < R ? (r & k) == 0 : asc_lane;
-                    const uint32_t a = v[r], b = v[r2];
-                    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-                    v[r] = asc ? lo : hi;
-                    v[r2] = asc ? hi : lo;
-                }
-            } else {
-                const int d = j / R;
-                const bool keep_min = ((lane & d) == 0) == (((lane * R) & k) == 0);
-                const int addr = (lane ^ d) << 2;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const uint32_t o = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)v[r]);
-                    const uint32_t lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
-                    v[r] = keep_min ? lo : hi;
-                }
-            }
-        }
-    }
-}
-
-// One band of n words at src (R per lane): sort, winding numbers from the band's prefix `pre`,
-// mark + emit (mark_merged_fragment_and_span.comp:48-120, gen_merged_fragment_and_span.comp:
-// 152-222: a cell is an in-canvas entry whose key differs from the previous entry's, a span runs
-// from the previous entry of the same cell row to this one when the winding number fills), the
-// record range of each of the band's buckets and its (cells, spans).
-template <int R>
-__device__ __forceinline__ void wave_band(const uint32_t* __restrict__ src, int n, int64_t band, int64_t start,
-                                          int32_t pre, int2 pr, const EmitArgs& EA) {
-    const int lane = threadIdx.x & 63;
-    uint32_t v[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int i = lane * R + r;
-        v[r] = i < n ? src[i] : Narrow::kPad;  // pads sort to the end
-    }
-    wave_bitonic<R>(v);
-    Narrow wd;
-    wd.pr = pr;
-    int32_t wl[R], key[R], sum = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        wl[r] = sum;
-        key[r] = wd.key(v[r]);
-        if (lane * R + r < n) sum = wadd(sum, wn_of(v[r]));
-    }
-    const int32_t lane_pre = wadd(pre, (int32_t)((uint32_t)wave_inclusive_add(sum) - (uint32_t)sum));
-    const BandPath b = band_path_of(EA, band, pr, start);
-    const int32_t prev_last = __shfl_up(key[R - 1], 1);
-    uint32_t cbits = 0, sbits = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int i = lane * R + r;
-        if (i >= n) continue;
-        const int32_t k1 = key[r], kprev = r ? key[r - 1] : prev_last;
-        const int x1 = key_x(k1), y1 = key_y(k1);
-        const bool in_canvas = !(x1 < 0 || y1 < 0 || x1 >= b.cw || y1 >= b.ch);
-        const bool cell = in_canvas && (i == 0 || kprev != k1);
-        bool span = false;
-        if (i > 0) {
-            const int32_t wn = (int32_t)((uint32_t)wadd(lane_pre, wl[r]) - (uint32_t)b.wb);
-            const bool wf = (b.rule == 0u && wn != 0) || (b.rule == 1u && (wn & 1) != 0);
-            span = key_y(kprev) == y1 && key_x(kprev) + 2 < x1 && wf;
-        }
-        cbits |= (uint32_t)cell << r;
-        sbits |= (uint32_t)span << r;
-    }
-    // record slots: cells in the low half, spans in the high half (at most 2048 of each per band)
-    const int32_t mine = __popc(cbits) | (__popc(sbits) << 16);
-    const int32_t inc = wave_inclusive_add(mine);
-    const int32_t ex = inc - mine;
-    int64_t pos = b.r0 + (ex & 0xFFFF) + (ex >> 16);
-    const int32_t next_first = __shfl_down(key[0], 1);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int i = lane * R + r;
-        if (i >= n) continue;
-        const int32_t k1 = key[r], kprev = r ? key[r - 1] : prev_last;
-        const int cell = (cbits >> r) & 1, span = (sbits >> r) & 1;
-        const int x1 = key_x(k1), y1 = key_y(k1);
-        if (cell) EA.crec[pos] = (uint32_t)x1 | (2u << 16);
-        if (span) {
-            const int x0 = max(0, key_x(kprev) + 2);
-            EA.crec[pos + cell] = (uint32_t)x0 | ((uint32_t)(x1 - x0) << 16);
-        }
-        // record range of this entry's (path, cell row) bucket, for the composite
-        if ((uint32_t)k1 != kSentinelKey && y1 >= 0 && y1 < b.ch) {
-            const int32_t kn = i + 1 < n ? (r + 1 < R ? key[r + 1 < R ? r + 1 : r] : next_first) : 0;
-            const bool first = i == 0 || (kprev >> 16) != (k1 >> 16);
-            const bool last = i == n - 1 || (kn >> 16) != (k1 >> 16);
-            if (first || last) {
-                const int row = y1 >> 1;
-                const int loc = row == 0 ? b.pr.y + 1 : row - b.pr.x;
-                if (loc >= 0 && loc < b.pr.y + 2) {  // out of range only after a flagged bound error
-                    int2* br = &EA.brec[b.bb + loc];
-                    if (first) br->x = (int32_t)pos;
-                    if (last) br->y = (int32_t)(pos + cell + span);
-                }
-            }
-        }
-        pos += cell + span;
-    }
-    if (lane == 63) EA.band_cnt[band] = make_int2(inc & 0xFFFF, inc >> 16);
-}
-
-__global__ __launch_bounds__(256) void k_sort_bands_wave(const int32_t* __restrict__ band_start,
-                                                         const int32_t* __restrict__ n_bands,
-                                                         const uint32_t* __restrict__ src,
-                                                         const int32_t* __restrict__ bpre,
-                                                         const int2* __restrict__ band_pr,
-                                                         int32_t* __restrict__ big_list,
-                                                         int32_t* __restrict__ big_excl,
-                                                         int32_t* __restrict__ big_count,
-                                                         uint32_t* __restrict__ long_count, EmitArgs EA) {
-    const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (t >= *n_bands) return;  // wave-uniform
-    const int64_t s = band_start[t], n = band_start[t + 1] - s;
-    if (n > kWaveBandMax) {
-        // a single oversized (path, row) bucket: sorted and emitted by the global-merge kernels
-        if (lane == 0) {
-            const int32_t q = atomicAdd(big_count, 1);
-            big_list[q] = (int32_t)t;
-            big_excl[q] = bpre[t];
-        }
-        return;
-    }
-    if (n > kShortBand && lane == 0) atomicAdd(long_count, 1u);
-    const uint32_t* p = src + s;
-    const int2 pr = band_pr[t];
-    const int32_t pre = bpre[t];
-    const int m = (int)n;
-    if (m <= 64) wave_band<1>(p, m, t, s, pre, pr, EA);
-    else if (m <= 128) wave_band<2>(p, m, t, s, pre, pr, EA);
-    else if (m <= 256) wave_band<4>(p, m, t, s, pre, pr, EA);
-    else if (m <= 512) wave_band<8>(p, m, t, s, pre, pr, EA);
-    else if (m <= 1024) wave_band<16>(p, m, t, s, pre, pr, EA);
-    else wave_band<32>(p, m, t, s, pre, pr, EA);
-}
-
 // Long bands (above one 64-thread tile) are listed in `list`, their number in *count, with one
 // atomic per 1024-thread block (the waves' counts are added up in LDS); the order of the list
 // does not matter.
@@ -541,6 +375,283 @@ __global__ __launch_bounds__(kClassifyThreads) void k_band_classify(const int32_
         uint32_t off = s_base;
         for (int k = 0; k < wave; ++k) off += s_n[k];
         list[off + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)t;
+    }
+}
+
+// ---- production bands: one wave per band, sorted in registers
+//
+// Frames that export no stage sort their narrow words band by band with one wave each.  A band of
+// n <= 2048 words is padded to 64 R words (R = 1, 2, 4, ..., 32 per lane: the smallest that holds
+// it) and sorted by a bitonic network over the element index i = l R + r (lane l, register r):
+// stages whose stride is below R compare two registers of one lane, the others exchange register
+// r with lane l ^ (stride / R) (ds_bpermute).  The words are loaded with coalesced accesses (word
+// l + 64 q) and transposed into registers through a wave-private LDS region padded by one word
+// every R words (no bank conflicts either way).  The sorted words then give the winding numbers
+// (the lane's prefix, a DPP wave scan and the band's prefix) and mark + emit runs from registers.
+// Bands of up to 512 words are sorted by one kernel (8 registers per lane: high occupancy), which
+// lists the longer ones for a second kernel (up to 32 per lane).
+constexpr int kWaveBandMax = 64 * 32;
+constexpr int kMediumBandMax = 64 * 16;
+constexpr int kSmallBandMax = 64 * 8;
+
+// The value of lane l ^ D (D a power of two below 64; low: bit D of the lane is clear).  Inside a
+// quad a DPP quad permutation, inside a 16-lane row a DPP row shift (the lower lane of a pair reads
+// l + D, the upper one l - D): plain VALU operations.  Across rows ds_bpermute.
+template <int D>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, bool low, int lane) {
+    if constexpr (D == 1) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    } else if constexpr (D == 2) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    } else if constexpr (D == 4 || D == 8) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x100 + D, 0xF, 0xF, false);  // row_shl
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x110 + D, 0xF, 0xF, false);  // row_shr
+        return low ? up : dn;
+    } else {
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ D) << 2, (int)v);
+    }
+}
+
+// Stage J (stride) of block K (ascending where bit K of the element index is 0; descending lanes
+// have complemented their words for the blocks from R up).  The network is generated at compile
+// time, one stage at a time.
+template <int R, int K, int J>
+__device__ __forceinline__ void bitonic_stage(uint32_t (&v)[R], int lane) {
+    if constexpr (J > 0) {
+        if constexpr (J < R) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int r2 = r ^ J;
+                if (r2 <= r) continue;
+                const bool asc = K < R ? (r & K) == 0 : true;
+                const uint32_t a = v[r], b = v[r2];
+                const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+                v[r] = asc ? lo : hi;
+                v[r2] = asc ? hi : lo;
+            }
+        } else {
+            constexpr int D = J / R;
+            const bool keep_min = (lane & D) == 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t o = lane_xor<D>(v[r], keep_min, lane);
+                const uint32_t lo = v[r] < o ? v[r] : o, hi = v[r] < o ? o : v[r];
+                v[r] = keep_min ? lo : hi;
+            }
+        }
+        bitonic_stage<R, K, J / 2>(v, lane);
+    }
+}
+
+template <int R, int K>
+__device__ __forceinline__ void bitonic_block(uint32_t (&v)[R], int lane) {
+    if constexpr (K <= 64 * R) {
+        // from R up the direction is a bit of the lane, the same for the whole block: a lane whose
+        // block descends complements its words (reversing their order) and sorts ascending
+        const uint32_t flip = K >= R && ((lane * R) & K) != 0 ? ~0u : 0u;
+        if constexpr (K >= R) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] ^= flip;
+        }
+        bitonic_stage<R, K, K / 2>(v, lane);
+        if constexpr (K >= R) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] ^= flip;
+        }
+        bitonic_block<R, 2 * K>(v, lane);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void wave_bitonic(uint32_t (&v)[R]) {
+    bitonic_block<R, 2>(v, threadIdx.x & 63);
+}
+
+// One band of n words at src (R per lane; lds: the wave's 64 R + 64 words): sort, winding numbers
+// from the band's prefix `pre`, mark + emit (mark_merged_fragment_and_span.comp:48-120,
+// gen_merged_fragment_and_span.comp:152-222: a cell is an in-canvas entry whose key differs from
+// the previous entry's, a span runs from the previous entry of the same cell row to this one when
+// the winding number fills), the record range of each of the band's buckets and its (cells,
+// spans).
+template <int R>
+__device__ __forceinline__ void wave_band(const uint32_t* __restrict__ src, int n, int64_t band, int64_t start,
+                                          int32_t pre, int2 pr, const EmitArgs& EA, uint32_t* lds) {
+    const int lane = threadIdx.x & 63;
+    uint32_t v[R];
+    if (R == 1) {
+        v[0] = lane < n ? src[lane] : Narrow::kPad;
+    } else {
+        // coalesced loads, then the transpose: word i sits at LDS slot i + i / R
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int i = 64 * q + lane;
+            lds[i + i / R] = i < n ? src[i] : Narrow::kPad;  // pads sort to the end
+        }
+        // one wave's LDS accesses run in order; the barrier keeps the compiler's order too
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = lds[lane * (R + 1) + r];
+        __builtin_amdgcn_wave_barrier();
+    }
+    wave_bitonic<R>(v);
+    Narrow wd;
+    wd.pr = pr;
+    // keys and winding numbers are recomputed in each pass from the sorted words (registers
+    // for occupancy): the lane's sum of winding changes first
+    int32_t sum = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (lane * R + r < n) sum = wadd(sum, wn_of(v[r]));
+    const int32_t lane_pre = wadd(pre, (int32_t)((uint32_t)wave_inclusive_add(sum) - (uint32_t)sum));
+    const BandPath b = band_path_of(EA, band, pr, start);
+    const int32_t prev_last = __shfl_up(wd.key(v[R - 1]), 1);
+    uint32_t cbits = 0, sbits = 0;
+    {
+        int32_t kprev = prev_last, wl = lane_pre;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = lane * R + r;
+            if (i >= n) continue;
+            const int32_t k1 = wd.key(v[r]);
+            const int x1 = key_x(k1), y1 = key_y(k1);
+            const bool in_canvas = !(x1 < 0 || y1 < 0 || x1 >= b.cw || y1 >= b.ch);
+            const bool cell = in_canvas && (i == 0 || kprev != k1);
+            bool span = false;
+            if (i > 0) {
+                const int32_t wn = (int32_t)((uint32_t)wl - (uint32_t)b.wb);
+                const bool wf = (b.rule == 0u && wn != 0) || (b.rule == 1u && (wn & 1) != 0);
+                span = key_y(kprev) == y1 && key_x(kprev) + 2 < x1 && wf;
+            }
+            cbits |= (uint32_t)cell << r;
+            sbits |= (uint32_t)span << r;
+            kprev = k1;
+            wl = wadd(wl, wn_of(v[r]));
+        }
+    }
+    // record slots: cells in the low half, spans in the high half (at most 2048 of each per band);
+    // the records are staged in the wave's LDS region (2 per entry at most: 128 R words) at their
+    // slot inside the band and stored with coalesced writes
+    const int32_t mine = __popc(cbits) | (__popc(sbits) << 16);
+    const int32_t inc = wave_inclusive_add(mine);
+    const int32_t ex = inc - mine;
+    int32_t rel = (ex & 0xFFFF) + (ex >> 16);  // slot inside the band's records (the global slot is r0 + rel)
+    const int32_t next_first = __shfl_down(wd.key(v[0]), 1);
+    int32_t kprev = prev_last, k1 = wd.key(v[0]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = lane * R + r;
+        if (i >= n) continue;
+        const int32_t kn = i + 1 < n ? (r + 1 < R ? wd.key(v[r + 1 < R ? r + 1 : r]) : next_first) : 0;
+        const int cell = (cbits >> r) & 1, span = (sbits >> r) & 1;
+        const int x1 = key_x(k1), y1 = key_y(k1);
+        if (cell) lds[rel] = (uint32_t)x1 | (2u << 16);
+        if (span) {
+            const int x0 = max(0, key_x(kprev) + 2);
+            lds[rel + cell] = (uint32_t)x0 | ((uint32_t)(x1 - x0) << 16);
+        }
+        // record range of this entry's (path, cell row) bucket, for the composite
+        if ((uint32_t)k1 != kSentinelKey && y1 >= 0 && y1 < b.ch) {
+            const bool first = i == 0 || (kprev >> 16) != (k1 >> 16);
+            const bool last = i == n - 1 || (kn >> 16) != (k1 >> 16);
+            if (first || last) {
+                const int row = y1 >> 1;
+                const int loc = row == 0 ? b.pr.y + 1 : row - b.pr.x;
+                if (loc >= 0 && loc < b.pr.y + 2) {  // out of range only after a flagged bound error
+                    int2* br = &EA.brec[b.bb + loc];
+                    if (first) br->x = (int32_t)(b.r0 + rel);
+                    if (last) br->y = (int32_t)(b.r0 + rel + cell + span);
+                }
+            }
+        }
+        rel += cell + span;
+        kprev = k1;
+        k1 = kn;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int32_t total = __shfl(inc, 63);
+    const int32_t n_rec = (total & 0xFFFF) + (total >> 16);
+    for (int32_t q = lane; q < n_rec; q += 64) EA.crec[b.r0 + q] = lds[q];
+    if (lane == 63) EA.band_cnt[band] = make_int2(inc & 0xFFFF, inc >> 16);
+}
+
+// Bands above 512 words in two lists: up to 1024 words from the front of `list`, longer ones (single
+// buckets) from the back (list holds `cap` slots); counts in count[0] and count[1].  One atomic per
+// list per 1024-thread block.
+__global__ __launch_bounds__(kClassifyThreads) void k_band_classify2(const int32_t* __restrict__ band_start,
+                                                                     const int32_t* __restrict__ n_bands,
+                                                                     int32_t* __restrict__ list, int64_t cap,
+                                                                     uint32_t* __restrict__ count) {
+    __shared__ uint32_t s_n[2][kClassifyThreads / 64], s_base[2];
+    const int64_t t = (int64_t)blockIdx.x * kClassifyThreads + threadIdx.x;
+    const int32_t nb = *n_bands;
+    const int32_t n = t < nb ? band_start[t + 1] - band_start[t] : 0;
+    const bool med = n > kSmallBandMax && n <= kMediumBandMax, large = n > kMediumBandMax;
+    const unsigned long long mm = __ballot(med), ml = __ballot(large);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { s_n[0][wave] = (uint32_t)__popcll(mm); s_n[1][wave] = (uint32_t)__popcll(ml); }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t tot = 0;
+        for (int k = 0; k < kClassifyThreads / 64; ++k) tot += s_n[threadIdx.x][k];
+        s_base[threadIdx.x] = tot ? atomicAdd(&count[threadIdx.x], tot) : 0u;
+    }
+    __syncthreads();
+    if (med || large) {
+        const int c = large ? 1 : 0;
+        uint32_t off = s_base[c];
+        for (int k = 0; k < wave; ++k) off += s_n[c][k];
+        off += (uint32_t)__popcll((large ? ml : mm) & ((1ull << lane) - 1ull));
+        if (large) list[cap - 1 - off] = (int32_t)t;
+        else list[off] = (int32_t)t;
+    }
+}
+
+// CLS 0: every band of up to 512 words, one wave each (four per workgroup; the others are
+// skipped).  CLS 1: the listed bands of 513..1024 words (16 per lane).  CLS 2: the listed longer
+// bands (32 per lane); those above 2048 words (single oversized buckets) are queued for the
+// global-merge kernels instead.
+template <int CLS>
+__global__ __launch_bounds__(256) void k_sort_bands_wave(const int32_t* __restrict__ band_start,
+                                                         const int32_t* __restrict__ n_bands,
+                                                         const uint32_t* __restrict__ src,
+                                                         const int32_t* __restrict__ bpre,
+                                                         const int2* __restrict__ band_pr,
+                                                         int32_t* __restrict__ big_list,
+                                                         int32_t* __restrict__ big_excl,
+                                                         int32_t* __restrict__ big_count,
+                                                         const int32_t* __restrict__ list, int64_t cap,
+                                                         const uint32_t* __restrict__ count, EmitArgs EA) {
+    constexpr int RMAX = CLS == 0 ? 8 : (CLS == 1 ? 16 : 32);
+    __shared__ uint32_t s_lds[4][128 * RMAX];  // the transposed load (64 R + 64), then the records (<= 128 R)
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= (CLS == 0 ? (int64_t)*n_bands : (int64_t)count[CLS - 1])) return;  // wave-uniform
+    const int64_t t = CLS == 0 ? w : (CLS == 1 ? list[w] : list[cap - 1 - w]);
+    const int64_t s = band_start[t], n = band_start[t + 1] - s;
+    if (CLS == 0 && n > kSmallBandMax) return;
+    if (CLS == 2 && n > kWaveBandMax) {
+        // a single oversized (path, row) bucket: sorted and emitted by the global-merge kernels
+        if (lane == 0) {
+            const int32_t q = atomicAdd(big_count, 1);
+            big_list[q] = (int32_t)t;
+            big_excl[q] = bpre[t];
+        }
+        return;
+    }
+    const uint32_t* p = src + s;
+    const int2 pr = band_pr[t];
+    const int32_t pre = bpre[t];
+    const int m = (int)n;
+    uint32_t* lds = s_lds[threadIdx.x >> 6];
+    if constexpr (CLS == 2) {
+        wave_band<32>(p, m, t, s, pre, pr, EA, lds);
+    } else if constexpr (CLS == 1) {
+        wave_band<16>(p, m, t, s, pre, pr, EA, lds);
+    } else {
+        if (m <= 64) wave_band<1>(p, m, t, s, pre, pr, EA, lds);
+        else if (m <= 128) wave_band<2>(p, m, t, s, pre, pr, EA, lds);
+        else if (m <= 256) wave_band<4>(p, m, t, s, pre, pr, EA, lds);
+        else wave_band<8>(p, m, t, s, pre, pr, EA, lds);
     }
 }
 
@@ -777,16 +888,27 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
                              uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr,
-                             const EmitArgs* emit, SideStream side) {
+                             const EmitArgs* emit, SideStream side, int64_t n_entries) {
     if (max_bands <= 0) return hipSuccess;
     const Unpacked out{skey, sidx, winding};
     if (band_pr) {
         if (sidx) return hipErrorInvalidValue;  // narrow words hold no index
         if (emit) {
-            // production frames: one wave per band, four bands per workgroup
-            k_sort_bands_wave<<<(unsigned)((max_bands + 3) / 4), 256, 0, s>>>(
-                band_start, n_bands, reinterpret_cast<const uint32_t*>(src), bpre, band_pr, big_list, big_excl,
-                big_count, ticket + 1, *emit);
+            // production frames: one wave per band, four bands per workgroup; the bands above 512
+            // words (at most N / 513 of them) in a second launch
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(src);
+            int32_t* lists = big_list + 2 * max_bands;  // 2 * max_bands slots: both lists of long bands
+            const int64_t cap = 2 * max_bands;
+            k_band_classify2<<<(unsigned)((max_bands + kClassifyThreads - 1) / kClassifyThreads), kClassifyThreads, 0,
+                               s>>>(band_start, n_bands, lists, cap, ticket + 1);
+            k_sort_bands_wave<0><<<(unsigned)((max_bands + 3) / 4), 256, 0, s>>>(
+                band_start, n_bands, w, bpre, band_pr, big_list, big_excl, big_count, lists, cap, ticket + 1, *emit);
+            const int64_t max_med = std::min<int64_t>(max_bands, n_entries / (kSmallBandMax + 1) + 1);
+            k_sort_bands_wave<1><<<(unsigned)((max_med + 3) / 4), 256, 0, s>>>(
+                band_start, n_bands, w, bpre, band_pr, big_list, big_excl, big_count, lists, cap, ticket + 1, *emit);
+            const int64_t max_large = std::min<int64_t>(max_bands, n_entries / (kMediumBandMax + 1) + 1);
+            k_sort_bands_wave<2><<<(unsigned)((max_large + 3) / 4), 256, 0, s>>>(
+                band_start, n_bands, w, bpre, band_pr, big_list, big_excl, big_count, lists, cap, ticket + 1, *emit);
             return hipGetLastError();
         }
         return launch_classes<Narrow>(band_start, n_bands, max_bands, reinterpret_cast<const uint32_t*>(src), out,

@@ -147,6 +147,14 @@ hipError_t launch_scan_i32_noalloc(const int32_t* in, int32_t* out, int64_t n, h
 // ---- (path, row) buckets and the band sort (a9 + a10): per path segment, ascending
 // (key ^ 0x80000000, index); the winding change rides along in the low bits of the sort word.
 constexpr int kSortTile = 2048;
+// Bands: runs of consecutive buckets of one path whose entries start in one 2^kBandWindowLog-entry
+// window (a bucket above the window's size is a band of its own), so a band holds at most twice
+// the window unless it is a single larger bucket.  Band starts come from path starts, window
+// changes and the buckets on either side of a large bucket: at most Np + 3 N / window + 2.
+constexpr int kBandWindowLog = 9;
+inline int64_t max_bands_for(uint32_t Np, int64_t n_entries) {
+    return (int64_t)Np + 3 * (n_entries >> kBandWindowLog) + 2;
+}
 hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, uint32_t Np, int rmax,
                                 int32_t* nb, int2* prow, hipStream_t s, const int2* pcanvas = nullptr);
 // narrow: 32-bit sort words (first half of `out`) without the entry index, for frames that
@@ -191,7 +199,8 @@ struct EmitArgs {
 // emit (narrow only, or null): mark + emit fused (EmitArgs); the bands then write no keys or
 // winding numbers, except the oversized ones, which launch_emit_big_bands reads back.
 // side (optional): a second stream and two events; the short bands are then sorted there,
-// beside the long ones, and s waits for them before it returns.
+// beside the long ones, and s waits for them before it returns.  n_entries: an upper bound of the
+// sorted entries (sizes the launch for the bands above 512 words of frames with emit).
 struct SideStream {
     hipStream_t s2;
     hipEvent_t fork, join;
@@ -200,7 +209,8 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
                              uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr = nullptr,
-                             const EmitArgs* emit = nullptr, SideStream side = {nullptr, nullptr, nullptr});
+                             const EmitArgs* emit = nullptr, SideStream side = {nullptr, nullptr, nullptr},
+                             int64_t n_entries = 0);
 // Mark + emit of the oversized bands from their sorted keys and winding numbers.
 hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
                                  const int32_t* skey, const int32_t* winding, const int2* band_pr,

@@ -220,11 +220,11 @@ ZeroLayout zero_layout(uint32_t C, uint32_t Np, int64_t N, int64_t NB) {
     z.chainctl = o; o += (spr::kCcWords + 1) / 2;  // chain classes, offsets, counters (int32)
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
     z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(NB, 1));
-    z.st_wind = o; o += (size_t)Np + 3 * (size_t)(N / 1024) + 3;  // statuses of the band-sum scan
+    z.st_wind = o; o += (size_t)spr::max_bands_for(Np, N) + 1;  // statuses of the band-sum scan
     z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(N, 1));
     z.hist = o; o += (size_t)(NB + 2 + 1) / 2;  // NB+1 int32 (scanned in place)
     z.fill = o; o += (size_t)(NB + 1) / 2;      // NB int32
-    z.bsum = o; o += ((size_t)Np + 3 * (size_t)(N / 1024) + 4) / 2 + 1;  // max_bands + 1 int32
+    z.bsum = o; o += ((size_t)spr::max_bands_for(Np, N) + 2) / 2 + 1;  // max_bands + 1 int32
     z.brec = o; o += (size_t)NB;                // NB int2
     z.total = o;
     return z;
@@ -367,7 +367,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     const ZeroLayout z = zero_layout(C, Np, N, NB);
     H->zero.ensure(z.total);
     const size_t n1 = (size_t)std::max<int64_t>(N, 1), nb1 = (size_t)std::max<int64_t>(NB, 1);
-    const int64_t max_bands = (int64_t)Np + 3 * (N / 1024) + 2;
+    const int64_t max_bands = spr::max_bands_for(Np, N);
     H->inter.ensure(n1);
     H->chain_t.ensure(n1);
     H->key.ensure(n1);
@@ -382,7 +382,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->band_pr.ensure((size_t)max_bands + 1);
     H->band_cnt.ensure((size_t)max_bands + 1);
     H->band_path.ensure((size_t)max_bands + 1);
-    H->big_list.ensure(3 * (size_t)max_bands);  // oversized band ids, their winding prefixes, long bands
+    H->big_list.ensure(4 * (size_t)max_bands);  // oversized band ids, their winding prefixes, two lists of long bands
     if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
     H->crec.ensure(2 * n1);
     H->image.ensure((size_t)4 * H->w * H->h);
@@ -459,7 +459,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
                                        H->big_list.p + max_bands, d_bigcount, d_bsum,
                                        ctl_at(H, z.st_wind, z.ctr, 4).counter, H->n_cu, s,
                                        narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr,
-                                       side_for(H, N, Np)));
+                                       side_for(H, N, Np), N));
         SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                            H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr,
                                            H->winding.p, s, narrow ? H->band_pr.p : nullptr));
@@ -520,7 +520,7 @@ struct LearnedLayout {
 
 // Scan tile counters of the learned frame (u32 slots of the counters block).
 enum { kLcArena = 0, kLcBucketBase, kLcHist, kLcBandFlags, kLcBandSums, kLcMark, kLcSortTicket, kLcSortTicket2,
-       kLcCount, kLcWords };
+       kLcSortTicket3, kLcCount, kLcWords };
 
 LearnedLayout learned_layout(uint32_t C, uint32_t Np, int64_t nb_budget, int64_t n_budget) {
     LearnedLayout z;
@@ -538,7 +538,7 @@ LearnedLayout learned_layout(uint32_t C, uint32_t Np, int64_t nb_budget, int64_t
     z.st_pb = o; o += spr::scan_tiles(std::max<uint32_t>(Np, 1));
     z.st_hist = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
     z.st_band = o; o += spr::scan_tiles(std::max<int64_t>(nb_budget, 1));
-    z.st_wind = o; o += (size_t)Np + 3 * (size_t)(std::max<int64_t>(n_budget, 1) / 1024) + 3;  // band-sum scan
+    z.st_wind = o; o += (size_t)spr::max_bands_for(Np, std::max<int64_t>(n_budget, 1)) + 1;  // band-sum scan
     z.st_mark = o; o += spr::scan_tiles(std::max<int64_t>(n_budget, 1));
     z.total = o;
     return z;
@@ -666,7 +666,7 @@ void run_learned_frame(spr_handle* H) {
     if (H->lb_n > (int64_t)1 << 30 || H->lb_r > (int64_t)1 << 30)
         throw Fail{SPR_ERR_CAPACITY, "intersection or record budget exceeds 2^30"};
     const int64_t nl = std::max<int64_t>(H->lb_n, 1), nr = std::max<int64_t>(H->lb_r, 1);
-    const int64_t max_bands = (int64_t)Np + 3 * (nl / 1024) + 2;
+    const int64_t max_bands = spr::max_bands_for(Np, nl);
     H->inter.ensure(nl); H->chain_t.ensure(nl); H->key.ensure(nl); H->wn.ensure(nl); H->epath.ensure(nl);
     H->sbig.ensure(nl); H->stmp.ensure(nl); H->skey.ensure(nl);
     H->winding.ensure(nl + 1);
@@ -675,7 +675,7 @@ void run_learned_frame(spr_handle* H) {
     H->band_pr.ensure((size_t)max_bands + 1);
     H->band_cnt.ensure((size_t)max_bands + 1);
     H->band_path.ensure((size_t)max_bands + 1);
-    H->big_list.ensure(3 * (size_t)max_bands);  // oversized band ids, their winding prefixes, long bands
+    H->big_list.ensure(4 * (size_t)max_bands);  // oversized band ids, their winding prefixes, two lists of long bands
     if (debug) { H->sidx.ensure(nl); H->flags.ensure(2 * nl); H->records.ensure(nr); }
     // narrow frames fuse mark + emit into the band sort: the records of band t sit at
     // [2 * start, ...), so the record buffer covers two per sorted entry
@@ -752,7 +752,7 @@ void run_learned_frame(spr_handle* H) {
                                    debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
                                    H->big_list.p + max_bands, d_bigcount, H->bsum_l.p, ticket, H->n_cu, s,
                                    narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr,
-                                   side_for(H, nl, Np)));
+                                   side_for(H, nl, Np), nl));
     SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
                                        H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr, H->winding.p,
                                        s, narrow ? H->band_pr.p : nullptr));
