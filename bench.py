@@ -175,71 +175,16 @@ def documents(seed=5, n_docs=256):
     return out
 
 
-def polygon_bound(scene) -> float:
-    """The learned frame's upper bound of a scene's intersections (control-polygon variation)."""
-    a = scene.arrays()
-    pos = a["pos"].astype(np.float64)
-    var = 0.0
-    for k in (2, 4):
-        idx = a["curve_pos"][a["curve_type"] == k][:, None] + np.arange(k)[None, :]
-        var += np.abs(np.diff(pos[idx], axis=1)).sum()
-    return 1.5 * var * 1.001 + 20 * len(a["curve_type"]) + 2 * len(a["rule"])
-
-
 def bench_documents(a) -> dict:
+    """256 documents rendered in groups, each group one batch frame replayed as one captured
+    graph, against the same documents rendered one by one with the plain two-pass method."""
     docs = documents()
-    # documents are grouped so that each group's intersection bound fits one frame's budget
-    # (records take two int4 per entry: the budget stays under 2^29 entries); a document whose
-    # bound alone exceeds it is rendered with the plain method inside the timed loop
-    limit = 4.0e8
-    groups, cur, acc, alone = [], [], 0.0, []
-    for i, (sc, c) in enumerate(docs):
-        b = polygon_bound(sc)
-        if b > limit:
-            alone.append(i)
-            continue
-        if cur and acc + b > limit:
-            groups.append(cur)
-            cur, acc = [], 0.0
-        cur.append(i)
-        acc += b
-    if cur:
-        groups.append(cur)
-    # each group is loaded once, its frame captured after two eager frames, and then replayed;
-    # groups are measured one after another so that only one group's buffers are resident
-    batched_ms = 0.0
-    tune = None
-    for gi, g in enumerate(groups):
-        r = V.Rasterizer(64, 64, device=0)
-        r.load_batch([docs[i][0] for i in g], [(docs[i][1], docs[i][1]) for i in g])
-        for _ in range(max(a.warmup, 2)):  # eager frame (budgets), eager frame + capture
-            r.render(readback=False)
-        if gi == 0 and a.autotune:
-            tune = r.autotune(reps=2)
-            for _ in range(2):
-                r.render(readback=False)
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            r.render(readback=False)  # one graph launch; waits for the group's flags
-        batched_ms += (time.perf_counter() - t0) / a.steps * 1e3
-        r.close()
-    for i in alone:
-        p = V.Rasterizer(docs[i][1], docs[i][1], device=0)
-        p.load_scene(docs[i][0])
-        p.render(readback=False)
-        p.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            p.render(readback=False)
-            p.synchronize()
-        batched_ms += (time.perf_counter() - t0) / a.steps * 1e3
-        p.close()
-    # the same documents rendered one by one with the plain two-pass method (inputs resident)
+    # the sequential plain renders (the baseline, inputs resident) also give each document's
+    # exact intersection count
     plain = {}
     seq_ms = 0.0
-    check = {0, 1, 2, len(docs) - 1}
-    same = True
-    for i, (sc, c) in enumerate(docs):
+    counts = []
+    for sc, c in docs:
         if c not in plain:
             plain[c] = V.Rasterizer(c, c, device=0)
         p = plain[c]
@@ -250,28 +195,60 @@ def bench_documents(a) -> dict:
         p.render(readback=False)
         p.synchronize()
         seq_ms += (time.perf_counter() - t1) * 1e3
-    # spot-check identity of batched images against plain renders
-    hb = V.Rasterizer(64, 64, device=0)
-    sel = sorted(check)
-    hb.load_batch([docs[i][0] for i in sel], [(docs[i][1], docs[i][1]) for i in sel])
-    hb.render(readback=False)
-    for j, i in enumerate(sel):
-        p = plain[docs[i][1]]
-        p.load_scene(docs[i][0])
-        same &= bool(np.array_equal(hb.batch_image(j), p.render()))
-    hb.close()
+        counts.append(int(p.stats().n_intersections))
     for p in plain.values():
         p.close()
+    # groups of consecutive documents whose counts fit one frame's budget (int32 entry offsets,
+    # with the budget's headroom): each group is one batch, one graph launch per frame
+    limit = 9.0e8
+    groups, cur, acc = [], [], 0
+    for i, n in enumerate(counts):
+        if cur and acc + n > limit:
+            groups.append(cur)
+            cur, acc = [], 0
+        cur.append(i)
+        acc += n
+    if cur:
+        groups.append(cur)
+    # identity against plain renders on a spread of documents (every kind, the smallest and the
+    # largest canvas, the most intersections)
+    check = sorted({0, 1, 2, len(docs) - 1, int(np.argmax(counts)), int(np.argmin(counts)),
+                    int(np.argmax([c for _, c in docs])), int(np.argmin([c for _, c in docs]))})
+    batched_ms = 0.0
+    tune = None
+    same = True
+    for gi, g in enumerate(groups):
+        r = V.Rasterizer(64, 64, device=0)
+        r.load_batch([docs[i][0] for i in g], [(docs[i][1], docs[i][1]) for i in g])
+        for _ in range(max(a.warmup, 3)):  # count and render, eager + capture, replay
+            r.render(readback=False)
+        if gi == 0 and a.autotune:
+            tune = r.autotune(reps=2)
+            for _ in range(3):
+                r.render(readback=False)
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            r.render(readback=False)  # one graph launch; waits for the group's counters
+        batched_ms += (time.perf_counter() - t0) / a.steps * 1e3
+        for j, i in enumerate(g):
+            if i in check:
+                sc, c = docs[i]
+                p = V.Rasterizer(c, c, device=0)
+                p.load_scene(sc)
+                same &= bool(np.array_equal(r.batch_image(j), p.render()))
+                p.close()
+        r.close()
     n_paths = int(sum(sc.n_paths for sc, _ in docs))
     return {
         "metric": "documents/s, 256 batched documents (one captured graph per group) vs sequential plain renders",
         "value": len(docs) / (batched_ms * 1e-3), "unit": "documents/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": batched_ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+int32", "data": "synthetic (seeded generators)",
-        "config": {"workload": "documents_256", "documents": len(docs), "paths": n_paths, "groups": len(groups),
-                   "documents_in_groups": sum(len(g) for g in groups), "documents_alone_plain": len(alone),
+        "config": {"workload": "documents_256", "documents": len(docs), "paths": n_paths,
+                   "intersections": int(sum(counts)), "groups": len(groups),
+                   "group_sizes": [len(g) for g in groups],
                    "sequential_plain_ms": seq_ms, "speedup_vs_sequential": seq_ms / batched_ms,
-                   "images_identical_spot_check": same, "autotune": tune},
+                   "images_identical_to_plain": same, "checked_documents": check, "autotune": tune},
     }
 
 

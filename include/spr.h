@@ -246,14 +246,25 @@ int spr_mlp_gradients(spr_handle* h, const float* features, const float* targets
 int spr_load_batch(spr_handle* h, const spr_scene* const* docs, const uint32_t* widths, const uint32_t* heights,
                    uint32_t n_docs);
 int spr_batch_image(spr_handle* h, uint32_t doc, uint8_t* rgba_out, size_t cap);
-/* Project-written autotuner: times the captured frame of the loaded batch (or learned scene)
- * for each candidate of the persistent-kernel occupancies (chain solve and long-piece merge
- * blocks per CU, 4/8/16 each, `reps` replays per candidate) and keeps the fastest. */
+/* Project-written autotuner: times the loaded scene's frame (captured-graph replays for a batch or
+ * a learned scene, eager frames for a plain one; `reps` frames per candidate) and keeps the
+ * fastest launch knobs, tuned one at a time from the defaults (coordinate descent):
+ *  - chain solve and long-piece merge: persistent blocks per CU (8, 4, 16 each);
+ *  - the tile of the large scans (the reference's fixed BATCH_SIZE = 1024 of
+ *    shaders/common/naive_scan.comp:2): 8 int4 x 1024 threads, 8 x 512, 4 x 1024, 16 x 512;
+ *  - the sort's band window, which sets the band (sort tile) size (the reference's fixed
+ *    BATCH_SIZE of shaders/common/naive_seg_sort_pairs.comp:3): 512 or 1024 entries.
+ * The knobs stay with the handle for later frames. */
 typedef struct spr_tune_report {
     int32_t chain_blocks_per_cu, merge_blocks_per_cu, n_tried, reserved;
     double default_ms, best_ms;
+    int32_t scan_shape;        /* 0: 8 x 1024, 1: 8 x 512, 2: 4 x 1024, 3: 16 x 512 */
+    int32_t band_window_log;   /* bands start in 2^band_window_log-entry windows (9 or 10) */
 } spr_tune_report;
 int spr_autotune(spr_handle* h, uint32_t reps, spr_tune_report* out);
+/* Sets the launch knobs of spr_autotune explicitly (a tuned configuration restored; the timing
+ * fields are ignored).  Every value is checked against the ones the autotuner tries. */
+int spr_set_tuning(spr_handle* h, const spr_tune_report* knobs);
 
 #ifdef __cplusplus
 }

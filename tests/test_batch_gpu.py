@@ -61,6 +61,30 @@ def test_learned_mode_graph_replays(spr):
     r.close()
 
 
+@pytest.mark.parametrize("compose", [0, 1])
+def test_batch_documents_match_oracle(spr, orc, compose):
+    """Each document of a batch (square and non-square canvases, one single-path document)
+    against the CPU oracle rendering it alone on its own canvas."""
+    import os
+    threads = max(1, min(16, os.cpu_count() or 1))
+    scenes = [spr.Scene.generate(k, s, n, max(w, h)) for k, s, n, w, h in DOCS]
+    r = spr.Rasterizer(64, 64, compose=compose)
+    r.load_batch(scenes, [(w, h) for _, _, _, w, h in DOCS])
+    for frame in range(3):  # the first frame counts (budget), then eager + capture, then a replay
+        print(f"[batch-oracle] compose={compose} frame={frame}", flush=True)
+        r.render(readback=False)
+    assert r.stats().learned_fallbacks == 1  # the first frame had no budget: counted, rendered again
+    for d, (sc, (_, _, _, w, h)) in enumerate(zip(scenes, DOCS)):
+        f = orc.Frame(sc.arrays(), w, h, compose=compose, threads=threads)
+        img, oi = r.batch_image(d), f.image()
+        if compose == 0:
+            assert np.array_equal(img, oi), d
+        else:
+            diff = np.abs(img.astype(np.int16) - oi.astype(np.int16)).max(-1)
+            assert (diff <= 1).mean() >= 0.999, d
+    r.close()
+
+
 def test_autotune_keeps_output(spr):
     scenes = [spr.Scene.generate(k, s, n, max(w, h)) for k, s, n, w, h in DOCS[:5]]
     ref = [_plain(spr, sc, w, h, 0) for sc, (_, _, _, w, h) in zip(scenes, DOCS[:5])]
@@ -70,8 +94,24 @@ def test_autotune_keeps_output(spr):
     print(f"[autotune] {rep}", flush=True)
     assert rep["n_tried"] == 9 and rep["best_ms"] <= rep["default_ms"] + 1e-9
     assert rep["chain_blocks_per_cu"] in (4, 8, 16) and rep["merge_blocks_per_cu"] in (4, 8, 16)
+    assert rep["scan_shape"] in (0, 1, 2, 3) and rep["band_window_log"] in (9, 10)
     for _ in range(3):
         r.render(readback=False)
         for d in range(5):
             assert np.array_equal(r.batch_image(d), ref[d])
+    r.close()
+
+
+def test_autotune_plain_handle(spr, orc):
+    """The autotuner on a plain (count-then-write) handle times eager frames; the output stays the
+    oracle's."""
+    scene = spr.Scene.generate(2, 3, 200, 2048)
+    a = scene.arrays()
+    r = spr.Rasterizer(2048, 2048, compose=1)
+    r.load_scene(scene)
+    rep = r.autotune(reps=2)
+    print(f"[autotune-plain] {rep}", flush=True)
+    assert rep["n_tried"] == 9
+    f = orc.Frame(a, 2048, 2048, compose=1, threads=8)
+    assert np.array_equal(r.render(), f.image())
     r.close()

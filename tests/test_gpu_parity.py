@@ -193,3 +193,35 @@ def test_golden_vectors(spr):
                 assert np.array_equal(got, g[k]), k
         assert make_golden.digest(img) == str(g[f"{name}/image_sha256"]), name
         r.close()
+
+
+@pytest.mark.parametrize("knobs", [
+    dict(band_window_log=10), dict(scan_shape=1), dict(scan_shape=2), dict(scan_shape=3),
+    dict(chain_blocks_per_cu=4, merge_blocks_per_cu=16), dict(chain_blocks_per_cu=16, merge_blocks_per_cu=4),
+    dict(band_window_log=10, scan_shape=3, chain_blocks_per_cu=16, merge_blocks_per_cu=16),
+])
+def test_tuning_knobs_keep_output(spr, orc, knobs):
+    """Every launch knob the autotuner can pick gives the oracle's frame: 1024-entry band windows
+    (bands up to 2048 words), each large-scan tile shape, other persistent-kernel occupancies."""
+    for name in ("illustration_small", "glyphs_small"):
+        scene, compose = _scene(spr, name)
+        a = scene.arrays()
+        w, h = a["width"], a["height"]
+        print(f"[knobs] {knobs} case={name}", flush=True)
+        f = orc.Frame(a, w, h, compose=compose, threads=8)
+        r = spr.Rasterizer(w, h, compose=compose)
+        r.set_tuning(**knobs)
+        r.load_scene(scene)
+        img = r.render()
+        st = r.stats()
+        assert st.n_intersections == len(f.get("skey"))
+        if compose == 0:
+            assert np.array_equal(img, f.image())
+        else:
+            diff = np.abs(img.astype(np.int16) - f.image().astype(np.int16)).max(-1)
+            assert (diff <= 1).mean() >= 0.999
+        r.close()
+    r = spr.Rasterizer(64, 64)
+    with pytest.raises(spr.SprError):
+        r.set_tuning(band_window_log=11)
+    r.close()

@@ -86,3 +86,21 @@ def test_scan_rejects_bad_arguments(spr, hip):
     with pytest.raises(spr.SprError):
         r.scan_i32(16, 16, 1 << 31)
     r.close()
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+def test_scan_tile_shapes(spr, hip, shape):
+    """Every tile shape of the large scans (the autotuner's knob), around its tile boundaries."""
+    r = spr.Rasterizer(64, 64, device=0)
+    r.set_tuning(scan_shape=shape)
+    st = hip.Stream()
+    for n in ((1 << 20), (1 << 20) + 32767, 3 * 32768 + 1, 7_000_001):
+        x = np.random.default_rng(n + shape).integers(-1000, 1000, n, dtype=np.int32)
+        xd, yd = hip.DeviceBuffer(4 * n), hip.DeviceBuffer(4 * (n + 1))
+        xd.upload(x)
+        print(f"[scan] shape={shape} n={n}", flush=True)
+        r.scan_i32(xd.ptr, yd.ptr, n, st.handle)
+        st.synchronize()
+        assert np.array_equal(yd.download(np.int32, n + 1), _exclusive(x))
+    st.destroy()
+    r.close()

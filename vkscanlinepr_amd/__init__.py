@@ -65,7 +65,7 @@ class TrainOpts(C.Structure):
 class TuneReport(C.Structure):
     _fields_ = [("chain_blocks_per_cu", C.c_int32), ("merge_blocks_per_cu", C.c_int32),
                 ("n_tried", C.c_int32), ("reserved", C.c_int32), ("default_ms", C.c_double),
-                ("best_ms", C.c_double)]
+                ("best_ms", C.c_double), ("scan_shape", C.c_int32), ("band_window_log", C.c_int32)]
 
 
 class TrainReport(C.Structure):
@@ -92,7 +92,7 @@ EXPORTS = [
     "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free", "spr_learned_train",
     "spr_learned_save", "spr_learned_load", "spr_learned_set_margin", "spr_mlp_get_params",
     "spr_mlp_set_params", "spr_mlp_predict", "spr_mlp_gradients", "spr_load_batch", "spr_batch_image",
-    "spr_autotune", "spr_scan_i32", "spr_scan_i32_noalloc",
+    "spr_autotune", "spr_set_tuning", "spr_scan_i32", "spr_scan_i32_noalloc",
 ]
 
 
@@ -140,6 +140,7 @@ def lib() -> C.CDLL:
     L.spr_load_batch.argtypes = [vp, C.POINTER(vp), u32p, u32p, C.c_uint32]
     L.spr_batch_image.argtypes = [vp, C.c_uint32, C.c_void_p, C.c_size_t]
     L.spr_autotune.argtypes = [vp, C.c_uint32, C.POINTER(TuneReport)]
+    L.spr_set_tuning.argtypes = [vp, C.POINTER(TuneReport)]
     L.spr_scan_i32.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
     L.spr_scan_i32_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
     _lib = L
@@ -406,7 +407,12 @@ def _rasterizer_batch_methods():
         self._c(lib().spr_autotune(self._h, int(reps), C.byref(r)))
         return {k: getattr(r, k) for k, _ in TuneReport._fields_ if k != "reserved"}
 
-    for f in (load_batch, batch_image, autotune):
+    def set_tuning(self, chain_blocks_per_cu=8, merge_blocks_per_cu=8, scan_shape=0, band_window_log=9) -> None:
+        r = TuneReport(chain_blocks_per_cu=chain_blocks_per_cu, merge_blocks_per_cu=merge_blocks_per_cu,
+                       scan_shape=scan_shape, band_window_log=band_window_log)
+        self._c(lib().spr_set_tuning(self._h, C.byref(r)))
+
+    for f in (load_batch, batch_image, autotune, set_tuning):
         setattr(Rasterizer, f.__name__, f)
 
 

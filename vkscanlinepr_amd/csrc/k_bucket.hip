@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
 // non-empty bucket of the earlier chunks), read with one shuffle.
 __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict__ bucket_base, uint32_t Np,
                                                        const int32_t* __restrict__ boff,
-                                                       int32_t* __restrict__ flag) {
+                                                       int32_t* __restrict__ flag, int window_log) {
     const uint32_t p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (p >= Np) return;
@@ -170,8 +170,8 @@ __global__ __launch_bounds__(kBlock) void k_band_flags(const int32_t* __restrict
                     f = 1;
                 } else {
                     const int32_t ps = below ? ps_l : cps, pz = below ? pz_l : cpz;
-                    constexpr int32_t kWin = 1 << kBandWindowLog;
-                    f = (o >> kBandWindowLog) != (ps >> kBandWindowLog) || pz > kWin || sz > kWin;
+                    const int32_t win = 1 << window_log;
+                    f = (o >> window_log) != (ps >> window_log) || pz > win || sz > win;
                 }
             }
             flag[b] = f;
@@ -237,9 +237,10 @@ hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uin
 }
 
 hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
-                             hipStream_t s) {
+                             hipStream_t s, int window_log) {
     if (Np == 0) return hipSuccess;
-    k_band_flags<<<(Np + 3) / 4, kBlock, 0, s>>>(bucket_base, Np, boff, flag);
+    if (window_log < kBandWindowLog || window_log > 10) return hipErrorInvalidValue;  // bands <= max_bands_for
+    k_band_flags<<<(Np + 3) / 4, kBlock, 0, s>>>(bucket_base, Np, boff, flag, window_log);
     return hipGetLastError();
 }
 

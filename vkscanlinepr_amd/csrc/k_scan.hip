@@ -390,9 +390,17 @@ hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl c
     // the status array holds scan_tiles(n) (4096-element) entries, enough for either tile shape:
     // 32768-element tiles (8 int4 x 1024 threads) for large scans, 4096-element tiles otherwise
     if (n >= ((int64_t)1 << 20)) {
-        constexpr int64_t kTile = 4 * 8 * 1024;
-        const int64_t tiles = (n + kTile - 1) / kTile;
-        k_scan_i32<8, 1024><<<(unsigned)tiles, 1024, 0, s>>>(in, out, n, n_dev, total_out, ctl.status, ctl.counter);
+        auto go = [&](auto kern, int v, int threads) {
+            const int64_t tile = 4 * (int64_t)v * threads;
+            kern<<<(unsigned)((n + tile - 1) / tile), threads, 0, s>>>(in, out, n, n_dev, total_out, ctl.status,
+                                                                       ctl.counter);
+        };
+        switch (ctl.shape) {
+            case 1: go(k_scan_i32<8, 512>, 8, 512); break;
+            case 2: go(k_scan_i32<4, 1024>, 4, 1024); break;
+            case 3: go(k_scan_i32<16, 512>, 16, 512); break;
+            default: go(k_scan_i32<8, 1024>, 8, 1024); break;
+        }
     } else {
         const int64_t tiles = n > 0 ? scan_tiles(n) : 1;
         k_scan_i32<4, 256><<<(unsigned)tiles, 256, 0, s>>>(in, out, n, n_dev, total_out, ctl.status, ctl.counter);

@@ -493,6 +493,8 @@ __device__ __forceinline__ void wave_band(const uint32_t* __restrict__ src, int 
         for (int r = 0; r < R; ++r) v[r] = lds[lane * (R + 1) + r];
         __builtin_amdgcn_wave_barrier();
     }
+    // the band's path tables (rule, canvas, winding base, first bucket) load while the words sort
+    const BandPath b = band_path_of(EA, band, pr, start);
     wave_bitonic<R>(v);
     Narrow wd;
     wd.pr = pr;
@@ -503,7 +505,6 @@ __device__ __forceinline__ void wave_band(const uint32_t* __restrict__ src, int 
     for (int r = 0; r < R; ++r)
         if (lane * R + r < n) sum = wadd(sum, wn_of(v[r]));
     const int32_t lane_pre = wadd(pre, (int32_t)((uint32_t)wave_inclusive_add(sum) - (uint32_t)sum));
-    const BandPath b = band_path_of(EA, band, pr, start);
     const int32_t prev_last = __shfl_up(wd.key(v[R - 1]), 1);
     uint32_t cbits = 0, sbits = 0;
     {

@@ -15,7 +15,11 @@ struct ScanCtl {
     unsigned long long* status;  // >= tiles entries
     uint32_t* counter;           // one u32, zero before the first use of the frame
     uint32_t epoch;              // distinct per scan invocation inside a frame sequence
+    int shape = 0;               // tile of the scans of 2^20 elements and more (kScanShapes; tuned)
 };
+// Tile shapes (int4 per thread, threads) of the large scans: 8 x 1024 (the default), 8 x 512,
+// 4 x 1024, 16 x 512.  The status array is sized by 4096-element tiles, enough for every shape.
+constexpr int kScanShapes = 4;
 constexpr int kScanTile = 4096;  // elements per scan tile (256 threads x 16)
 inline int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile; }
 
@@ -166,8 +170,9 @@ hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uin
                                  const int32_t* bucket_base, const int2* prow, const int32_t* flag_scan,
                                  const int32_t* band_start, int32_t* fill, int32_t* bsum, unsigned long long* out,
                                  int32_t* err, hipStream_t s, const int32_t* n_dev = nullptr, bool narrow = false);
+// window_log: bands start in 2^window_log-entry windows (9 = kBandWindowLog or 10; tuned).
 hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
-                             hipStream_t s);
+                             hipStream_t s, int window_log = kBandWindowLog);
 // band_pr (or null): each band's path rows (prow of its path), for the narrow sort.
 // band_path (with band_pr): each band's path.
 hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32_t NB, int64_t N,

@@ -189,6 +189,8 @@ struct spr_handle {
     int64_t graph_lb_n = -1, graph_lb_r = -1;
     bool capturing = false;
     int chain_k = 8, merge_k = 8;                   // persistent blocks per CU
+    int scan_shape = 0;                             // tile shape of the large scans (spr::kScanShapes)
+    int band_window = spr::kBandWindowLog;          // log2 of the band window (9 or 10)
 
     // last frame
     int64_t N = 0;
@@ -246,6 +248,7 @@ spr::ScanCtl ctl_at(spr_handle* H, size_t st, size_t ctr_word, int counter) {
     c.status = H->zero.p + st;
     c.counter = reinterpret_cast<uint32_t*>(H->zero.p + ctr_word) + counter;
     c.epoch = 0;
+    c.shape = H->scan_shape;
     return c;
 }
 
@@ -424,7 +427,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         const spr::FragArgs A{H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h, nullptr,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err,
                               debug ? H->inter.p : nullptr};
-        solve_and_pieces(H, C, d_cc, A, (int64_t)n1, H->n_cu, H->n_cu);
+        solve_and_pieces(H, C, d_cc, A, (int64_t)n1, H->n_cu * H->chain_k / 8, H->n_cu * H->merge_k / 8);
     } else {
         stage_mark(H, 5);
     }
@@ -436,7 +439,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     // ---- sort: bucket offsets, bands, scatter into band order, band sort in LDS
     if (N > 0) {
         SPR_HIP(spr::launch_scan_i32(d_hist, d_hist, NB, ctl_at(H, z.st_hist, z.ctr, 0), s));
-        SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, d_hist, H->bflag.p, s));
+        SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, d_hist, H->bflag.p, s, H->band_window));
         SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, NB, ctl_at(H, z.st_band, z.ctr, 1), s));
         SPR_HIP(spr::launch_band_emit(H->bflag.p, d_hist, (int32_t)NB, N, H->band_start.p, s, nullptr, nullptr,
                                       H->bucket_base.p, Np, H->prow.p, narrow ? H->band_pr.p : nullptr,
@@ -632,19 +635,15 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_scan_i32(H->bucket_base.p, H->bucket_base.p, Np, ctl_at(H, z.st_pb, z.counters, kLcBucketBase),
                                  s));
     bool joined = s2 == s;
+    if (H->need_prediction && !use_model) H->need_prediction = false;  // a batch without a model: no prediction
     if (H->need_prediction) {
-        // the first frame after a load or a model change: read the predicted allocation back
-        // once (a batch takes the proven bound: it has no plain frame to fall back on)
+        // the first frame after a load or a model change: read the predicted allocation back once
         if (!joined) { SPR_HIP(hipStreamWaitEvent(s, H->ev_pjoin, 0)); joined = true; }
         SPR_HIP(hipMemcpyAsync(H->h_pinned + 12, H->arena.p + Np, 4, hipMemcpyDeviceToHost, s));
         SPR_HIP(hipMemcpyAsync(H->h_pinned + 14, H->zero.p + z.bound, 8, hipMemcpyDeviceToHost, s));
         SPR_HIP(hipStreamSynchronize(s));
         const int64_t caps = std::max<int32_t>(H->h_pinned[12], 0);
-        unsigned long long bound = 0;
-        std::memcpy(&bound, H->h_pinned + 14, 8);
-        const int64_t b = (int64_t)std::min<unsigned long long>(bound, 1ull << 40);
-        if (H->batch) grow_budget(H, b, 0);
-        else grow_budget(H, 0, caps);
+        grow_budget(H, 0, caps);
         H->need_prediction = false;
         ++H->predictions;
         // the status arrays of phase 2 are sized by the budget: re-lay the control region (phase 1
@@ -663,8 +662,9 @@ void run_learned_frame(spr_handle* H) {
     int32_t* d_totals = reinterpret_cast<int32_t*>(H->zero.p + z.totals);
     int32_t* d_nsorted = reinterpret_cast<int32_t*>(H->zero.p + z.slots);
     int32_t* d_nbands = d_nsorted + 1;
-    if (H->lb_n > (int64_t)1 << 30 || H->lb_r > (int64_t)1 << 30)
-        throw Fail{SPR_ERR_CAPACITY, "intersection or record budget exceeds 2^30"};
+    // entry offsets are int32 and so are record slots, two per entry
+    if (H->lb_n > (int64_t)1 << 30 || H->lb_r > ((int64_t)1 << 31) - 2)
+        throw Fail{SPR_ERR_CAPACITY, "intersection budget exceeds 2^30"};
     const int64_t nl = std::max<int64_t>(H->lb_n, 1), nr = std::max<int64_t>(H->lb_r, 1);
     const int64_t max_bands = spr::max_bands_for(Np, nl);
     H->inter.ensure(nl); H->chain_t.ensure(nl); H->key.ensure(nl); H->wn.ensure(nl); H->epath.ensure(nl);
@@ -727,7 +727,7 @@ void run_learned_frame(spr_handle* H) {
     stage_mark(H, 7);
     SPR_HIP(spr::launch_scan_i32(H->hist_l.p, H->hist_l.p, nb_budget, ctl_at(H, z.st_hist, z.counters, kLcHist), s,
                                  d_nb, d_nsorted));
-    SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, H->hist_l.p, H->bflag.p, s));
+    SPR_HIP(spr::launch_band_flags(H->bucket_base.p, Np, H->hist_l.p, H->bflag.p, s, H->band_window));
     SPR_HIP(spr::launch_scan_i32(H->bflag.p, H->bflag.p, nb_budget, ctl_at(H, z.st_band, z.counters, kLcBandFlags),
                                  s, d_nb, d_nbands));
     SPR_HIP(spr::launch_band_emit(H->bflag.p, H->hist_l.p, (int32_t)nb_budget, nl, H->band_start.p, s, d_nb,
@@ -819,14 +819,22 @@ void learned_finish(spr_handle* H) {
         throw Fail{SPR_ERR_DEVICE, "internal: an entry fell outside its path's row bound"};
     }
     if (H->sum(kSumOver)) {
-        // the exact total did not fit: the device culled the frame.  Render it with the plain
-        // method (a batch cannot be: its budget is the proven bound) and cover it from now on.
-        if (H->batch) throw Fail{SPR_ERR_DEVICE, "internal: a batch frame exceeded its proven bound"};
+        // the exact total did not fit: the device culled the frame.  A single scene is rendered
+        // with the plain method; a batch, which has no plain frame, is rendered again on a budget
+        // grown to its exact total (read here, as the plain method reads its count).  Either way
+        // the budget covers it from now on.
         ++H->learned_fallbacks;
+        H->last_fallback = true;
+        H->lb_dirty = true;
+        if (H->batch) {
+            if (total < 0 || total > ((int64_t)1 << 30) - ((int64_t)1 << 27))
+                throw Fail{SPR_ERR_CAPACITY, "the batch needs more than 2^30 intersections: split it"};
+            grow_budget(H, total, caps);
+            run_learned_frame(H);
+            return;
+        }
         run_frame(H);
         grow_budget(H, H->N, caps);
-        H->lb_dirty = true;
-        H->last_fallback = true;
         H->stats.n_overflow_paths = overflow_paths;
         H->stats.alloc_intersections = caps;
         return;
@@ -835,6 +843,7 @@ void learned_finish(spr_handle* H) {
     H->stats.alloc_intersections = caps;
     H->N = H->sum(kSumSorted);
     H->n_live = H->sum(kSumLive);
+    H->last_fallback = false;
     grow_budget(H, total, caps);
 }
 
@@ -1179,6 +1188,7 @@ int spr_scan_i32(spr_handle* H, const int32_t* d_in, int32_t* d_out, uint64_t n,
         c.status = H->scan_ws.p;
         c.counter = reinterpret_cast<uint32_t*>(H->scan_ws.p + tiles);
         c.epoch = 0;
+        c.shape = H->scan_shape;
         SPR_HIP(spr::launch_scan_i32(d_in, d_out, (int64_t)n, c, s));
     });
 }
@@ -1405,46 +1415,81 @@ int spr_autotune(spr_handle* H, uint32_t reps, spr_tune_report* out) {
     if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
         if (!H->loaded) throw Fail{SPR_ERR_STATE, "autotune before load"};
-        if (!H->batch && !(H->opts.mode == SPR_MODE_LEARNED && H->model.ready))
-            throw Fail{SPR_ERR_STATE, "autotune times captured frames: a batch, or learned mode with a model"};
+        if (H->inj) throw Fail{SPR_ERR_STATE, "autotune with injected intersections"};
         if (reps == 0) reps = 3;
+        const bool nosync = H->batch || (H->opts.mode == SPR_MODE_LEARNED && H->model.ready);
         const bool timing = H->stage_timing;
         H->stage_timing = false;
-        const int ks[3] = {4, 8, 16};
-        double best = 1e30, dflt = 0.0;
-        int best_c = H->chain_k, best_m = H->merge_k, tried = 0;
-        for (int ci = 0; ci < 3; ++ci)
-            for (int mi = 0; mi < 3; ++mi) {
-                H->chain_k = ks[ci];
-                H->merge_k = ks[mi];
+        // device time of the current knobs: captured-graph replays for frames without a host
+        // synchronisation, eager frames otherwise
+        auto measure = [&]() {
+            if (nosync) {
                 drop_graph(H);
-                render_nosync(H);  // eager; sizes the budgets when needed
+                render_nosync(H);  // eager; sizes the budget when needed
                 render_nosync(H);  // eager, then captured
                 if (!H->gexec) render_nosync(H);
-                double acc = 0.0;
-                for (uint32_t r = 0; r < reps; ++r) {
-                    render_nosync(H);
-                    float ms = 0.0f;
-                    SPR_HIP(hipEventElapsedTime(&ms, H->ev_begin, H->ev_end));
-                    acc += ms;
-                }
-                const double avg = acc / reps;
-                if (ks[ci] == 8 && ks[mi] == 8) dflt = avg;
-                if (avg < best) { best = avg; best_c = ks[ci]; best_m = ks[mi]; }
-                ++tried;
+            } else {
+                run_frame(H);
             }
-        H->chain_k = best_c;
-        H->merge_k = best_m;
+            double acc = 0.0;
+            for (uint32_t r = 0; r < reps; ++r) {
+                if (nosync) render_nosync(H); else run_frame(H);
+                SPR_HIP(hipStreamSynchronize(H->stream));
+                float ms = 0.0f;
+                SPR_HIP(hipEventElapsedTime(&ms, H->ev_begin, H->ev_end));
+                acc += ms;
+            }
+            return acc / reps;
+        };
+        // coordinate descent from the defaults: each knob in turn over its values, the others at
+        // their best so far
+        int* knob[4] = {&H->chain_k, &H->merge_k, &H->scan_shape, &H->band_window};
+        const std::vector<int> values[4] = {{8, 4, 16}, {8, 4, 16}, {0, 1, 2, 3}, {spr::kBandWindowLog, 10}};
+        for (int k = 0; k < 4; ++k) *knob[k] = values[k][0];
+        const double dflt = measure();
+        double best = dflt;
+        int tried = 1;
+        for (int k = 0; k < 4; ++k) {
+            const int keep = *knob[k];
+            int best_v = keep;
+            for (size_t i = 1; i < values[k].size(); ++i) {
+                *knob[k] = values[k][i];
+                const double ms = measure();
+                ++tried;
+                if (ms < best) { best = ms; best_v = values[k][i]; }
+            }
+            *knob[k] = best_v;
+        }
         drop_graph(H);
         H->stage_timing = timing;
         if (out) {
-            out->chain_blocks_per_cu = best_c;
-            out->merge_blocks_per_cu = best_m;
+            out->chain_blocks_per_cu = H->chain_k;
+            out->merge_blocks_per_cu = H->merge_k;
             out->n_tried = tried;
             out->reserved = 0;
             out->default_ms = dflt;
             out->best_ms = best;
+            out->scan_shape = H->scan_shape;
+            out->band_window_log = H->band_window;
         }
+    });
+}
+
+int spr_set_tuning(spr_handle* H, const spr_tune_report* k) {
+    if (!H || !k) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        auto one_of = [](int v, std::initializer_list<int> ok) {
+            for (int o : ok) if (v == o) return true;
+            return false;
+        };
+        if (!one_of(k->chain_blocks_per_cu, {4, 8, 16}) || !one_of(k->merge_blocks_per_cu, {4, 8, 16}) ||
+            k->scan_shape < 0 || k->scan_shape >= spr::kScanShapes || !one_of(k->band_window_log, {spr::kBandWindowLog, 10}))
+            throw Fail{SPR_ERR_INVALID, "tuning knob out of range"};
+        H->chain_k = k->chain_blocks_per_cu;
+        H->merge_k = k->merge_blocks_per_cu;
+        H->scan_shape = k->scan_shape;
+        H->band_window = k->band_window_log;
+        drop_graph(H);  // the captured frame holds the old launch shapes
     });
 }
 
