@@ -77,6 +77,9 @@ typedef struct spr_stats {
                                   1024 entries, or waiting on a chain of 512 or more crossings) */
     int64_t n_long_bands;      /* sort bands of more than 512 words (256-thread LDS sorts) */
     int64_t n_big_bands;       /* sort bands of more than 2048 words (global merge-path sort) */
+    int64_t graph_captures;    /* LEARNED and batches: frames captured as a HIP graph so far (a new
+                                  scene, model, margin, tuning or budget needs a new capture; a
+                                  new MVP does not: the captured frame reads it at each replay) */
 } spr_stats;
 
 /* Debug stage identifiers for spr_debug_export / spr_debug_inject. */
@@ -99,7 +102,10 @@ enum {
                                   in no set order, so only each run's first number is the
                                   reference's) */
     SPR_STAGE_FLAGS = 11,      /* int32 x 2N: cell flags, then span flags (debug) */
-    SPR_STAGE_RECORDS = 12,    /* int4 x R: (yx, width, fill, frag_index) output records (debug) */
+    SPR_STAGE_RECORDS = 12,    /* int4 x R: (yx, width, fill, frag_index) output records (debug).
+                                  Injected, the following frames draw these records instead of
+                                  rendering the scene (the reference's MOCK_DATA replay of a
+                                  record file, scanline_rasterizer.cpp:261-281, :1057-1109) */
     SPR_STAGE_IMAGE = 13,      /* uint8 x 4WH: RGBA8 image, top row first */
     SPR_STAGE_FEATURES = 14,   /* float x 16Np: per-path predictor features (last learned frame) */
     SPR_STAGE_PRED = 15,       /* float x 2Np: predicted log2(1 + intersections), log2(1 + records) */
@@ -172,9 +178,18 @@ int spr_scan_i32_noalloc(const int32_t* d_in, int32_t* d_out, uint64_t n, void* 
  *     into dst (cap bytes) and stores the full size in *nbytes; with dst == NULL only the size
  *     is returned.  inject(SPR_STAGE_INTER) makes the following frames use the given
  *     intersections (int2 x N, in curve order) instead of generating them, so the integer
- *     stages can be compared on identical intersection inputs; inject with size 0 clears it. */
+ *     stages can be compared on identical intersection inputs; inject(SPR_STAGE_RECORDS) makes
+ *     them draw the given records (int4 x R, at most 2^22) on the handle's canvas, no scene
+ *     needed; inject with size 0 clears it. */
 int spr_debug_export(spr_handle* h, int stage, void* dst, size_t cap, size_t* nbytes);
 int spr_debug_inject(spr_handle* h, int stage, const void* src, size_t nbytes);
+
+/* --- presentation: the reference presents its frames in a GLFW swapchain
+ *     (src/core/scanline/scanline_rasterizer.cpp:611-666); headless, a frame goes to a PNG file
+ *     (8-bit RGBA, top row first).  spr_write_png writes the handle's last frame (a batch's
+ *     documents are read with spr_batch_image), spr_write_png_rgba any image in host memory. */
+int spr_write_png(spr_handle* h, const char* path);
+int spr_write_png_rgba(const char* path, const uint8_t* rgba, uint32_t width, uint32_t height);
 
 /* --- scenes: the VGContainer of src/core/vg/vg_container.h:21-87 plus the flattening of
  *     scanline_rasterizer.cpp:67-134, and the synthetic workload generators. */

@@ -184,3 +184,31 @@ def test_cpp_adapter_header_compiles_and_links(spr, tmp_path):
     subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}", str(src),
                     f"-L{lib_dir}", "-lspr", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], check=True)
     assert subprocess.run([str(exe)]).returncode == 0
+
+
+def test_png_encoder_round_trip(spr, tmp_path):
+    """spr_write_png_rgba: the PNG decodes with Python's zlib (which also checks the chunk CRCs
+    and the stream's Adler-32) to the same pixels, for sizes that cross the 65535-byte stored
+    deflate blocks."""
+    import zlib
+    rng = np.random.default_rng(3)
+    for w, h in ((1, 1), (7, 3), (300, 200)):
+        img = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+        p = tmp_path / f"img_{w}x{h}.png"
+        spr.write_png(p, img)
+        data = p.read_bytes()
+        assert data[:8] == b"\x89PNG\r\n\x1a\n"
+        pos, idat = 8, b""
+        while pos < len(data):
+            n = int.from_bytes(data[pos:pos + 4], "big")
+            typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+            assert int.from_bytes(data[pos + 8 + n:pos + 12 + n], "big") == zlib.crc32(typ + body)
+            if typ == b"IHDR":
+                assert (int.from_bytes(body[:4], "big"), int.from_bytes(body[4:8], "big")) == (w, h)
+                assert body[8:] == bytes([8, 6, 0, 0, 0])
+            idat += body if typ == b"IDAT" else b""
+            pos += 12 + n
+        raw = np.frombuffer(zlib.decompress(idat), dtype=np.uint8).reshape(h, 4 * w + 1)
+        assert np.all(raw[:, 0] == 0) and np.array_equal(raw[:, 1:].reshape(h, w, 4), img)
+    with pytest.raises(spr.SprError):
+        spr.write_png(tmp_path / "no_dir" / "x.png", np.zeros((2, 2, 4), np.uint8))

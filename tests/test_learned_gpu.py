@@ -223,6 +223,57 @@ def test_over_budget_frame_renders_plain(spr, trained):
     p.close()
 
 
+def test_mvp_changes_replay_the_captured_frame(spr, trained):
+    """Interactive use: a new MVP every frame.  The captured frame reads the MVP at each replay, so
+    pans, a rotation and zooms out are replays of the one graph (no new capture), each identical
+    to the plain frame with that MVP; a zoom in beyond the allocation falls back to the plain
+    method once, and the frames after it are captured and replayed again."""
+    r, _, path = trained
+    scene = spr.Scene.generate(2, 9, 300, 1024)
+    q = spr.Rasterizer(1024, 1024, compose=spr.COMPOSE_SRC_OVER, mode=spr.MODE_LEARNED)
+    q.learned_load(path)
+    q.learned_set_margin(2.0, 0)  # capacities of 4x the prediction: room for the pans and zooms out
+    q.load_scene(scene)
+    p = spr.Rasterizer(1024, 1024, compose=spr.COMPOSE_SRC_OVER)
+    p.load_scene(scene)
+
+    def view(scale, angle, dx, dy):
+        c, s_ = scale * np.cos(angle), scale * np.sin(angle)
+        return np.array([[c, -s_, 0, dx], [s_, c, 0, dy], [0, 0, 1, 0], [0, 0, 0, 1]], np.float32)
+
+    # the scene shrunk to 15%: every view below fits the predicted allocation (4x margin)
+    q.set_mvp(view(0.15, 0.0, 100.0, 100.0))
+    p.set_mvp(view(0.15, 0.0, 100.0, 100.0))
+    for _ in range(3):  # eager with the prediction, eager + capture, replay
+        q.render(readback=False)
+    st = q.stats()
+    assert st.graph_captures == 1
+    mvps = [view(0.15, 0.0, 107.5, 96.75), view(0.15, 0.0, 60.0, 300.0), view(0.15, 0.1, 500.0, 400.0),
+            view(0.13, 0.3, 200.0, 250.0), view(0.1, -0.2, 700.0, 90.0), view(0.15, 0.0, 100.0, 100.0)]
+    for k, m in enumerate(mvps):
+        q.set_mvp(m)
+        p.set_mvp(m)
+        print(f"[mvp] frame {k}", flush=True)
+        ip = p.render()
+        assert np.array_equal(q.render(), ip), k
+        st = q.stats()
+        assert st.learned_fallbacks == 0 and st.graph_captures == 1, k  # a replay of the one graph
+        p.render(readback=False)  # eager work of another handle between replays
+    n_fb = q.stats().learned_fallbacks
+    big = np.eye(4, dtype=np.float32)  # the full-size scene: about 6.7 times the entries
+    q.set_mvp(big)
+    p.set_mvp(big)
+    ip = p.render()
+    assert np.array_equal(q.render(), ip)
+    assert q.stats().learned_fallbacks == n_fb + 1  # over the allocation: the plain method
+    caps = q.stats().graph_captures
+    for k in range(3):
+        assert np.array_equal(q.render(), ip), k
+    assert q.stats().graph_captures == caps + 1 and q.stats().learned_fallbacks == n_fb + 1
+    q.close()
+    p.close()
+
+
 def test_save_load_bit_exact(spr, trained):
     r, _, path = trained
     q = spr.Rasterizer(64, 64)

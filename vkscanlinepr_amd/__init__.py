@@ -53,7 +53,8 @@ class Stats(C.Structure):
                 ("n_overflow_paths", C.c_int64), ("alloc_intersections", C.c_int64),
                 ("frame_ms", C.c_double), ("stage_ms", C.c_double * 16),
                 ("learned_fallbacks", C.c_int64), ("n_wave_chains", C.c_int64),
-                ("n_heavy_pieces", C.c_int64), ("n_long_bands", C.c_int64), ("n_big_bands", C.c_int64)]
+                ("n_heavy_pieces", C.c_int64), ("n_long_bands", C.c_int64), ("n_big_bands", C.c_int64),
+                ("graph_captures", C.c_int64)]
 
 
 class TrainOpts(C.Structure):
@@ -92,7 +93,8 @@ EXPORTS = [
     "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free", "spr_learned_train",
     "spr_learned_save", "spr_learned_load", "spr_learned_set_margin", "spr_mlp_get_params",
     "spr_mlp_set_params", "spr_mlp_predict", "spr_mlp_gradients", "spr_load_batch", "spr_batch_image",
-    "spr_autotune", "spr_set_tuning", "spr_scan_i32", "spr_scan_i32_noalloc",
+    "spr_autotune", "spr_set_tuning", "spr_scan_i32", "spr_scan_i32_noalloc", "spr_write_png",
+    "spr_write_png_rgba",
 ]
 
 
@@ -141,6 +143,8 @@ def lib() -> C.CDLL:
     L.spr_batch_image.argtypes = [vp, C.c_uint32, C.c_void_p, C.c_size_t]
     L.spr_autotune.argtypes = [vp, C.c_uint32, C.POINTER(TuneReport)]
     L.spr_set_tuning.argtypes = [vp, C.POINTER(TuneReport)]
+    L.spr_write_png.argtypes = [vp, C.c_char_p]
+    L.spr_write_png_rgba.argtypes = [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]
     L.spr_scan_i32.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
     L.spr_scan_i32_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
     _lib = L
@@ -156,6 +160,14 @@ def _check(rc: int, handle=None) -> None:
 
 def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def write_png(path, rgba: np.ndarray) -> None:
+    """An (h, w, 4) uint8 image (top row first) to an 8-bit RGBA PNG file."""
+    a = np.ascontiguousarray(rgba, dtype=np.uint8)
+    if a.ndim != 3 or a.shape[2] != 4:
+        raise ValueError("expected an (h, w, 4) uint8 image")
+    _check(lib().spr_write_png_rgba(str(path).encode(), a.ctypes.data_as(C.c_void_p), a.shape[1], a.shape[0]))
 
 
 def scan_i32_noalloc(d_in: int, d_out: int, n: int, stream: int | None = None) -> None:
@@ -320,6 +332,19 @@ class Rasterizer:
         total last), enqueued on `stream` (a hipStream_t handle; None = the handle's stream)."""
         self._c(lib().spr_scan_i32(self._h, C.c_void_p(d_in), C.c_void_p(d_out), int(n),
                                    C.c_void_p(stream) if stream else None))
+
+    def write_png(self, path) -> None:
+        """The last frame as an 8-bit RGBA PNG file."""
+        self._c(lib().spr_write_png(self._h, str(path).encode()))
+
+    def inject_records(self, records) -> None:
+        """Following frames draw these (R, 4) int32 records (yx, width, fill, frag_index) instead
+        of rendering the scene (record replay); None clears them."""
+        if records is None:
+            self._c(lib().spr_debug_inject(self._h, STAGES["records"], None, 0))
+            return
+        a = np.ascontiguousarray(records, dtype=np.int32).reshape(-1)
+        self._c(lib().spr_debug_inject(self._h, STAGES["records"], a.ctypes.data_as(C.c_void_p), a.nbytes))
 
     def inject_intersections(self, inter) -> None:
         if inter is None:

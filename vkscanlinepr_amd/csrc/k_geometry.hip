@@ -46,9 +46,14 @@ __global__ __launch_bounds__(kBlock) void k_transform(const float2* __restrict__
                                                       int32_t* __restrict__ visible,
                                                       int32_t* __restrict__ row_lo,
                                                       int32_t* __restrict__ row_hi,
-                                                      const int2* __restrict__ pcanvas) {
+                                                      const int2* __restrict__ pcanvas,
+                                                      const float* __restrict__ mvp_dev) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const bool valid = i < P;
+    if (mvp_dev) {  // the matrix in device memory (a captured frame that follows setMVP)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) m.m[k] = mvp_dev[k];
+    }
     int32_t bits = 0, lo = 0, hi = 0;
     uint32_t path = 0xFFFFFFFFu;
     if (valid) {
@@ -239,10 +244,10 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
 
 hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
                             int w, int h, float2* tpos, int32_t* visible, int32_t* row_lo, int32_t* row_hi,
-                            hipStream_t s, const int2* pcanvas) {
+                            hipStream_t s, const int2* pcanvas, const float* mvp_dev) {
     if (P == 0) return hipSuccess;
     k_transform<<<grid_for(P), kBlock, 0, s>>>(pos, pos_path, P, m, (float)w, (float)h, (h - 1) / 2, tpos,
-                                               visible, row_lo, row_hi, pcanvas);
+                                               visible, row_lo, row_hi, pcanvas, mvp_dev);
     return hipGetLastError();
 }
 

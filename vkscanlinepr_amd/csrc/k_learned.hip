@@ -309,19 +309,21 @@ __global__ void k_dense_bwd_data(const float* __restrict__ dY, int B, int N, con
     dA[i] = A[i] > 0.0f ? acc : 0.0f;
 }
 
-// dW[k][n] = sum_r X[r][k] dY[r][n] for k < K; row K of the grid gives db[n] = sum_r dY[r][n].
-__global__ void k_dense_bwd_weight(const float* __restrict__ X, const float* __restrict__ dY, int B, int K,
-                                   int N, float* __restrict__ dW, float* __restrict__ db) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (K + 1) * N) return;
-    const int k = i / N, n = i % N;
+// dW[k][n] = sum_r X[r][k] dY[r][n] for k < K; output row K gives db[n] = sum_r dY[r][n].  One
+// wave per output: the lanes take every 64th row of the batch and a wave reduction adds them up.
+__global__ __launch_bounds__(256) void k_dense_bwd_weight(const float* __restrict__ X, const float* __restrict__ dY,
+                                                          int B, int K, int N, float* __restrict__ dW,
+                                                          float* __restrict__ db) {
+    const int o = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (o >= (K + 1) * N) return;  // wave-uniform
+    const int k = o / N, n = o % N;
     float acc = 0.0f;
-    if (k < K) {
-        for (int r = 0; r < B; ++r) acc += X[(int64_t)r * K + k] * dY[(int64_t)r * N + n];
-        dW[k * N + n] = acc;
-    } else {
-        for (int r = 0; r < B; ++r) acc += dY[(int64_t)r * N + n];
-        db[n] = acc;
+    for (int r = lane; r < B; r += 64) acc += (k < K ? X[(int64_t)r * K + k] : 1.0f) * dY[(int64_t)r * N + n];
+    acc = wave_sum_f(acc);
+    if (lane == 0) {
+        if (k < K) dW[k * N + n] = acc;
+        else db[n] = acc;
     }
 }
 
@@ -448,7 +450,7 @@ hipError_t launch_dense_bwd_data(const float* dY, int B, int N, const float* W, 
 }
 hipError_t launch_dense_bwd_weight(const float* X, const float* dY, int B, int K, int N, float* dW, float* db,
                                    hipStream_t s) {
-    k_dense_bwd_weight<<<blocks((int64_t)(K + 1) * N, 64), 64, 0, s>>>(X, dY, B, K, N, dW, db);
+    k_dense_bwd_weight<<<(unsigned)(((int64_t)(K + 1) * N + 3) / 4), 256, 0, s>>>(X, dY, B, K, N, dW, db);
     return hipGetLastError();
 }
 hipError_t launch_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1, float b2, float eps,

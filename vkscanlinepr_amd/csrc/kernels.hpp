@@ -37,9 +37,10 @@ struct SummarySrc {
 hipError_t launch_gather_summary(const SummarySrc& src, int32_t* dst, hipStream_t s);
 
 // ---- geometry (a3, a4, a7, a8)
+// mvp_dev (optional): the matrix read from device memory instead of m.
 hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_t P, const Mvp& m,
                             int w, int h, float2* tpos, int32_t* visible, int32_t* row_lo, int32_t* row_hi,
-                            hipStream_t s, const int2* pcanvas = nullptr);
+                            hipStream_t s, const int2* pcanvas = nullptr, const float* mvp_dev = nullptr);
 hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                         const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
                         float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s,
@@ -294,6 +295,11 @@ hipError_t launch_doc_winding_base_bands(const int32_t* band_start, const int32_
                                          int32_t* wbase, hipStream_t s);
 hipError_t launch_doc_winding_base(const int32_t* winding, const int32_t* sstart, const int32_t* doc_first_path,
                                    uint32_t Np, int32_t* wbase, hipStream_t s);
+
+// ---- record replay: draws a given list of (yx, width, fill, frag_index) records in list order
+// (overwrite or src-over) into a white w x h RGBA8 image, top row first.
+hipError_t launch_draw_records(const int4* records, int32_t R, int w, int h, int compose, uint8_t* image,
+                               hipStream_t s);
 
 // ---- composite (a12)
 hipError_t launch_composite(const uint32_t* records, const int2* brec, const int32_t* bucket_base,

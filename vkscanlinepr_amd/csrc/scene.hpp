@@ -39,4 +39,8 @@ std::string validate(const float* pos, const uint32_t* pos_path, uint32_t P, con
 // opacity, each channel times 255 truncated to a byte, and 0 when the alpha byte is 0.
 uint32_t pack_fill(float r, float g, float b, float a, float opacity);
 
+// PNG (8-bit RGBA, rows top first) of an image in host memory (png.cpp).
+std::vector<uint8_t> png_encode_rgba(const uint8_t* rgba, uint32_t w, uint32_t h);
+void png_write_rgba(const char* path, const uint8_t* rgba, uint32_t w, uint32_t h);
+
 }  // namespace spr

@@ -150,10 +150,15 @@ struct spr_handle {
     DBuf<int32_t> summary;          // the frame summary on the device, copied to h_pinned + 16
     DBuf<unsigned long long> scan_ws;  // spr_scan_i32: tile statuses and the tile counter
     int32_t sum(int k) const { return h_pinned[16 + k]; }
+    float* mvp_pinned() const { return reinterpret_cast<float*>(h_pinned + 48); }  // [48, 64): the MVP rows
+    DBuf<float> mvp_dev;            // the MVP of frames without a host synchronisation, copied from mvp_pinned
 
     // injection
     std::vector<int32_t> inj_inter;
     bool inj = false;
+    DBuf<int4> inj_rec;             // injected records (replay), on the device
+    int64_t inj_rec_n = -1;         // their number; -1: no records injected
+    int64_t inj_rec_cells = 0;
 
     // learned allocation
     spr::MlpModel model;
@@ -168,6 +173,7 @@ struct spr_handle {
     int64_t n_live = 0;                             // entries of the last learned frame
     int64_t learned_fallbacks = 0;                  // learned frames over budget, rendered with the plain method
     int64_t predictions = 0;                        // learned frames that read the predicted allocation back
+    int64_t captures = 0;                           // frames captured as a graph
     bool last_learned = false;
     bool last_fallback = false;
     int2* last_brec = nullptr;                      // per-bucket record ranges of the last frame
@@ -547,11 +553,13 @@ LearnedLayout learned_layout(uint32_t C, uint32_t Np, int64_t nb_budget, int64_t
     return z;
 }
 
-// The budget covers `need` entries with an eighth of headroom (and the predicted total with a
-// sixteenth); it only grows, and records take at most two per entry.
+// The budget only grows, and only when a frame's exact need comes within 1/16 of it or the
+// predicted total exceeds it: then to the need plus an eighth (and the prediction plus a
+// sixteenth), so that frames whose counts drift a little (a new MVP) keep the captured frame.
+// Records take at most two per entry.
 void grow_budget(spr_handle* H, int64_t need, int64_t predicted) {
-    const int64_t want = std::max<int64_t>(need + need / 8, predicted + predicted / 16) + 1024;
-    if (want > H->lb_n) H->lb_n = want;
+    if (need > H->lb_n - H->lb_n / 16 || predicted > H->lb_n)
+        H->lb_n = std::max<int64_t>(std::max<int64_t>(need + need / 8, predicted + predicted / 16) + 1024, H->lb_n);
     H->lb_r = 2 * H->lb_n;
 }
 
@@ -606,8 +614,12 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->visible.p, Np, 0u, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_lo.p, Np, 0x7FFFFFFFu, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_hi.p, Np, 0x80000000u, s));
+    // the MVP goes through a copy from pinned host memory, which a captured frame repeats at every
+    // replay: spr_set_mvp only rewrites that memory, and the graph follows it
+    H->mvp_dev.ensure(16);
+    SPR_HIP(hipMemcpyAsync(H->mvp_dev.p, H->mvp_pinned(), 64, hipMemcpyHostToDevice, s));
     SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p,
-                                  H->row_lo.p, H->row_hi.p, s, pcv));
+                                  H->row_lo.p, H->row_hi.p, s, pcv, H->mvp_dev.p));
     stage_mark(H, 1);
     int32_t* d_capr = reinterpret_cast<int32_t*>(H->zero.p + z.capr);
     if (s2 != s) {
@@ -874,6 +886,7 @@ void capture_frame(spr_handle* H) {
     SPR_HIP(hipGraphInstantiate(&H->gexec, H->graph, nullptr, nullptr, 0));
     H->graph_lb_n = H->lb_n;
     H->graph_lb_r = H->lb_r;
+    ++H->captures;
 }
 
 // Frames without a host synchronisation inside (learned mode, batches).  While the scene, MVP,
@@ -934,6 +947,7 @@ void sync_stats(spr_handle* H) {
     H->stats.n_heavy_pieces = H->sum(kSumHeavyPieces);
     H->stats.n_long_bands = H->sum(kSumLongBands);
     H->stats.n_big_bands = H->sum(kSumBigBands);
+    H->stats.graph_captures = H->captures;
     H->stats.learned_fallbacks = H->learned_fallbacks;
     float ms = 0.0f;
     SPR_HIP(hipEventElapsedTime(&ms, H->ev_begin, H->ev_end));
@@ -1042,6 +1056,7 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
         H->summary.ensure(spr::kSummaryMax);
         const float id[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
         std::memcpy(H->mvp.m, id, sizeof(id));
+        std::memcpy(H->mvp_pinned(), id, sizeof(id));
         H->image.ensure((size_t)4 * width * height);
     });
     if (rc != SPR_OK) { spr_destroy(H); return rc; }
@@ -1070,6 +1085,8 @@ int spr_destroy(spr_handle* H) {
     H->hist_l.release(); H->fill_l.release(); H->bsum_l.release(); H->brec_l.release(); H->pbound.release();
     H->summary.release();
     H->scan_ws.release();
+    H->inj_rec.release();
+    H->mvp_dev.release();
     if (H->h_pinned) (void)hipHostFree(H->h_pinned);
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
@@ -1118,15 +1135,45 @@ int spr_load_rvg(spr_handle* H, const char* path) {
 
 int spr_set_mvp(spr_handle* H, const float m_rows[16]) {
     if (!H || !m_rows) { g_error = "null argument"; return SPR_ERR_INVALID; }
-    if (std::memcmp(H->mvp.m, m_rows, sizeof(H->mvp.m)) != 0) H->lb_dirty = true;
+    // every frame has ended when this returns (frames end with a stream synchronisation), so no
+    // copy is reading the pinned matrix; a captured frame picks the new one up at its next replay
     std::memcpy(H->mvp.m, m_rows, sizeof(H->mvp.m));
+    std::memcpy(H->mvp_pinned(), m_rows, sizeof(H->mvp.m));
     return SPR_OK;
+}
+
+// Record replay: draws the injected records into the handle's image.
+void run_records_frame(spr_handle* H) {
+    hipStream_t s = H->stream;
+    H->image.ensure((size_t)4 * H->w * H->h);
+    SPR_HIP(hipEventRecord(H->ev_begin, s));
+    SPR_HIP(spr::launch_draw_records(H->inj_rec.p, (int32_t)H->inj_rec_n, (int)H->w, (int)H->h,
+                                     (int)H->opts.compose, H->image.p, s));
+    SPR_HIP(hipEventRecord(H->ev_end, s));
+    SPR_HIP(hipStreamSynchronize(s));
+    H->n_marks = 0;
+    H->N = 0;
+    std::memset(H->h_pinned + 16, 0, sizeof(int32_t) * kSumCount);
+    H->h_pinned[16 + kSumCells] = (int32_t)H->inj_rec_cells;
+    H->h_pinned[16 + kSumSpans] = (int32_t)(H->inj_rec_n - H->inj_rec_cells);
+    H->frame_done = true;
+    H->last_learned = false;
+    H->last_brec = nullptr;
 }
 
 int spr_render_frame(spr_handle* H, uint8_t* rgba_out) {
     if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
         SPR_HIP(hipSetDevice(H->device));
+        if (H->inj_rec_n >= 0 && !H->batch) {
+            run_records_frame(H);
+            if (rgba_out) {
+                SPR_HIP(hipMemcpyAsync(rgba_out, H->image.p, (size_t)4 * H->w * H->h, hipMemcpyDeviceToHost,
+                                       H->stream));
+                SPR_HIP(hipStreamSynchronize(H->stream));
+            }
+            return;
+        }
         if (H->batch) {
             if (rgba_out) throw Fail{SPR_ERR_INVALID, "a batch has one image per document: use spr_batch_image"};
             render_nosync(H);
@@ -1319,7 +1366,23 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
 int spr_debug_inject(spr_handle* H, int stage, const void* src, size_t nbytes) {
     if (!H) { g_error = "null handle"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
-        if (stage != SPR_STAGE_INTER) throw Fail{SPR_ERR_INVALID, "only SPR_STAGE_INTER can be injected"};
+        if (stage == SPR_STAGE_RECORDS) {
+            if (nbytes == 0) { H->inj_rec_n = -1; H->inj_rec.release(); return; }
+            if (!src || nbytes % 16) throw Fail{SPR_ERR_INVALID, "records are int4 (yx, width, fill, frag_index)"};
+            const size_t n = nbytes / 16;
+            if (n > ((size_t)1 << 22)) throw Fail{SPR_ERR_CAPACITY, "a replayed record list holds at most 2^22 records"};
+            SPR_HIP(hipSetDevice(H->device));
+            SPR_HIP(hipStreamSynchronize(H->stream));
+            H->inj_rec.ensure(std::max<size_t>(n, 1));
+            if (n) SPR_HIP(hipMemcpy(H->inj_rec.p, src, nbytes, hipMemcpyHostToDevice));
+            const int32_t* v = (const int32_t*)src;
+            int64_t cells = 0;
+            for (size_t i = 0; i < n; ++i) cells += v[4 * i + 3] != 0;  // cells carry their index, spans 0
+            H->inj_rec_n = (int64_t)n;
+            H->inj_rec_cells = cells;
+            return;
+        }
+        if (stage != SPR_STAGE_INTER) throw Fail{SPR_ERR_INVALID, "only SPR_STAGE_INTER and SPR_STAGE_RECORDS can be injected"};
         if (nbytes == 0) { H->inj = false; H->inj_inter.clear(); return; }
         if (!src || nbytes % 8) throw Fail{SPR_ERR_INVALID, "intersections are int2 records"};
         if (!H->loaded) throw Fail{SPR_ERR_STATE, "inject before load"};
@@ -1696,6 +1759,33 @@ int spr_scene_write_rvg(const spr_scene* s, const char* path) {
 }
 
 void spr_scene_free(spr_scene* s) { delete s; }
+
+int spr_write_png_rgba(const char* path, const uint8_t* rgba, uint32_t width, uint32_t height) {
+    if (!path || !rgba) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    try {
+        spr::png_write_rgba(path, rgba, width, height);
+        return SPR_OK;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return SPR_ERR_IO;
+    }
+}
+
+int spr_write_png(spr_handle* H, const char* path) {
+    if (!H || !path) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    return guarded(H, [&] {
+        if (!H->frame_done) throw Fail{SPR_ERR_STATE, "no frame rendered"};
+        if (H->batch) throw Fail{SPR_ERR_STATE, "a batch has one image per document: use spr_batch_image"};
+        std::vector<uint8_t> img((size_t)4 * H->w * H->h);
+        SPR_HIP(hipMemcpyAsync(img.data(), H->image.p, img.size(), hipMemcpyDeviceToHost, H->stream));
+        SPR_HIP(hipStreamSynchronize(H->stream));
+        try {
+            spr::png_write_rgba(path, img.data(), H->w, H->h);
+        } catch (const std::exception& e) {
+            throw Fail{SPR_ERR_IO, e.what()};
+        }
+    });
+}
 
 }  // extern "C"
 
