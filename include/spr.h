@@ -192,17 +192,56 @@ int spr_write_png(spr_handle* h, const char* path);
 int spr_write_png_rgba(const char* path, const uint8_t* rgba, uint32_t width, uint32_t height);
 
 /* --- scenes: the VGContainer of src/core/vg/vg_container.h:21-87 plus the flattening of
- *     scanline_rasterizer.cpp:67-134, and the synthetic workload generators. */
+ *     scanline_rasterizer.cpp:67-134, and the synthetic workload generators.
+ *     Curve types are the reference's CurveType values (vg_container.h:9-15): 2 line (2 points),
+ *     3 quadratic (3 points), 4 cubic (4 points), 0x13 arc (3 points: start, control point,
+ *     end, a rational quadratic whose control point has weight curve_weight[c] > 0).  The
+ *     reference declares QUADRIC and ARC but leaves them TODO in its shaders; here they are
+ *     rasterised natively.  A path's colour is fill_info unless path_paint names a gradient. */
+#define SPR_PAINT_LINEAR 1u
+#define SPR_PAINT_RADIAL 2u
+#define SPR_SPREAD_PAD 0u
+#define SPR_SPREAD_REPEAT 1u
+#define SPR_SPREAD_REFLECT 2u
+#define SPR_SPREAD_TRANSPARENT 3u
+#define SPR_SOLID_PAINT 0xFFFFFFFFu
+/* A gradient.  At a pixel centre p (canvas coordinates), the scene point s = MVP^-1 p, the
+ * gradient point g = xform * s; linear: t = ((g - p1) . d) / (d . d) with d = p2 - p1 (t = 1
+ * when d = 0); radial: t = the largest root of |g - f - t (c - f)| = t r (the focus f strictly
+ * inside the circle).  The spread maps t to [0, 1] (pad clamps, repeat wraps, reflect mirrors,
+ * transparent gives a transparent colour outside), the stops are interpolated linearly in
+ * non-premultiplied RGBA, and the colour is packed like fill_info with alpha times opacity. */
+typedef struct spr_paint {
+    uint32_t kind;                  /* SPR_PAINT_LINEAR or SPR_PAINT_RADIAL */
+    uint32_t spread;                /* SPR_SPREAD_* */
+    uint32_t first_stop, n_stops;   /* stops [first_stop, first_stop + n_stops), n_stops >= 1 */
+    float opacity;
+    float xform[6];                 /* scene -> gradient space: gx = x0 x + x1 y + x2, gy = x3 x + x4 y + x5 */
+    float geom[5];                  /* linear: x1, y1, x2, y2, unused; radial: cx, cy, fx, fy, r */
+} spr_paint;
+typedef struct spr_stop {
+    float offset, r, g, b, a;       /* offsets non-decreasing within a paint */
+} spr_stop;
+
 typedef struct spr_scene_view {
     const float* pos_xy; const uint32_t* pos_path_idx; uint32_t n_points;
     const uint32_t* curve_pos_map; const uint32_t* curve_type; const uint32_t* curve_path_idx;
     uint32_t n_curves;
     const uint32_t* fill_rule; const uint32_t* fill_info; uint32_t n_paths;
     uint32_t width, height;  /* suggested canvas (the RVG viewport, or the generator's) */
+    const float* curve_weight;      /* per curve, read for arcs only; NULL when there is no arc */
+    const uint32_t* path_paint;     /* per path: SPR_SOLID_PAINT or a paint index; NULL = all solid */
+    const spr_paint* paints; uint32_t n_paints;
+    const spr_stop* stops; uint32_t n_stops;
 } spr_scene_view;
 
-/* RVG::load, src/core/vg/rvg.cpp:9-255 (M/L/C/Z, solid rgb/rgba paint, nzfill/ofill). */
+/* RVG::load, src/core/vg/rvg.cpp:9-255 (M/L/C/Z, solid rgb/rgba paint, nzfill/ofill), plus the
+ * format's parts that parser skips: Q and A segments (an A segment of weight below 1/2 is split
+ * into halves, so stored arcs have 1/2 <= w), dyn_affine path, paint and scene transforms, and
+ * linear and radial gradients. */
 int spr_scene_load_rvg(const char* path, spr_scene** out);
+/* A scene from arrays (copied and validated), curve_weight and paints included. */
+int spr_scene_create(const spr_scene_view* view, spr_scene** out);
 /* Synthetic workloads.  kind: 0 = random cubic scribbles (64 paths, 1024^2 by default),
  * 1 = glyph page (quadratic glyphs elevated to cubics, even-odd), 2 = illustration (mixed
  * rules, alpha, per-path affine).  n_paths / canvas = 0 select the defaults. */

@@ -30,7 +30,8 @@ def lib() -> C.CDLL:
         L = C.CDLL(str(LIB_PATH))
         vp, u32p, f32p, i32p = C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int32)
         L.oracle_run.argtypes = [f32p, u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, u32p, u32p,
-                                 C.c_uint32, f32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int64]
+                                 C.c_uint32, f32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int64,
+                                 f32p, u32p, vp, f32p]
         L.oracle_run.restype = vp
         L.oracle_get.argtypes = [vp, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
         L.oracle_free.argtypes = [vp]
@@ -43,6 +44,8 @@ def lib() -> C.CDLL:
         L.oracle_eval.restype = C.c_float
         L.oracle_solve_crossing.argtypes = [C.c_uint32, f32p, f32p, C.c_int, C.c_float, C.c_float, C.c_float]
         L.oracle_solve_crossing.restype = C.c_float
+        L.oracle_paint.argtypes = [vp, f32p, f32p, C.c_int, C.c_int]
+        L.oracle_paint.restype = C.c_uint32
         L.oracle_path_visible.argtypes = [C.c_int32]
         L.oracle_float2int_rd.argtypes = [C.c_float]
         L.oracle_float2int_rd.restype = C.c_int32
@@ -77,10 +80,24 @@ class Frame:
             self._keep.append(ov)
             n_ov = len(ov) // 2
         f = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        # arc weights and gradient paints (absent: None or empty)
+        opt = {}
+        for key, dt in (("curve_weight", np.float32), ("path_paint", np.uint32), ("paints", None),
+                        ("stops", None)):
+            a = arrays.get(key)
+            if a is not None and len(a):
+                a = np.ascontiguousarray(a) if dt is None else np.ascontiguousarray(a, dtype=dt)
+                self._keep.append(a)
+                opt[key] = a
+        cw = f(opt["curve_weight"], C.c_float) if "curve_weight" in opt else None
+        ppt = f(opt["path_paint"], C.c_uint32) if "path_paint" in opt else None
+        pnt = opt["paints"].ctypes.data if "paints" in opt else None
+        stp = opt["stops"].ctypes.data_as(C.POINTER(C.c_float)) if "stops" in opt else None
         self._ctx = L.oracle_run(f(pos, C.c_float), f(pp, C.c_uint32), P, f(cpos, C.c_uint32),
                                  f(ctype, C.c_uint32), f(cpath, C.c_uint32), Cn, f(rule, C.c_uint32),
                                  f(fill, C.c_uint32), Np, f(m, C.c_float), int(width), int(height),
-                                 int(compose), f(ov, C.c_int32) if ov is not None else None, n_ov)
+                                 int(compose), f(ov, C.c_int32) if ov is not None else None, n_ov,
+                                 cw, ppt, pnt, stp)
         self.width, self.height = int(width), int(height)
 
     def get(self, stage: str) -> np.ndarray:
@@ -131,10 +148,26 @@ def curve_cuts(type_: int, xs, ys, visible: bool = True):
     return n, q
 
 
+def eval_axis(type_: int, t: float, c4) -> float:
+    """One axis of a curve at t; an arc's values as stored (x0, x1, x2, w)."""
+    c = np.ascontiguousarray(c4, dtype=np.float32)
+    return float(lib().oracle_eval(int(type_), float(t), c.ctypes.data_as(C.POINTER(C.c_float))))
+
+
 def solve_quad(a: float, b: float, c: float):
     r0, r1 = C.c_float(), C.c_float()
     lib().oracle_solve_quad(a, b, c, C.byref(r0), C.byref(r1))
     return np.float32(r0.value), np.float32(r1.value)
+
+
+def paint(paint_rec: np.ndarray, stops: np.ndarray, x: int, y: int, mvp=None) -> int:
+    """The packed RGBA8 colour of one gradient paint (a record of PAINT_DTYPE, its stops
+    renumbered from 0) at the centre of canvas pixel (x, y)."""
+    p = np.ascontiguousarray(paint_rec).reshape(1)
+    st = np.ascontiguousarray(stops, dtype=np.float32).reshape(-1)
+    m = np.ascontiguousarray(IDENTITY if mvp is None else mvp, dtype=np.float32).reshape(16)
+    return int(lib().oracle_paint(p.ctypes.data, st.ctypes.data_as(C.POINTER(C.c_float)),
+                                  m.ctypes.data_as(C.POINTER(C.c_float)), int(x), int(y)))
 
 
 def solve_crossing(type_: int, xs, ys, side: int, c: float, t_min: float, t1: float) -> np.float32:

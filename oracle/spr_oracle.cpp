@@ -35,6 +35,12 @@
 //   * the draw uses the real canvas size instead of the hard-coded 1200x1024 viewport.
 // The reference quirk of the 4-cut insertion sort (the 4th cut is compared with itself and so
 // never moves, make_intersection_0.comp:338-341) is reproduced on purpose.
+//
+// Beyond the reference, whose shaders leave the QUADRIC and ARC cases TODO and whose draw has
+// solid colours only: quadratic and arc (rational quadratic) curves, evaluated, monotonised and
+// bisected as defined at eval1 / monotone_cuts / solve_crossing, and gradient paints in the
+// draw (paint_rgba).  These follow the definitions documented in include/spr.h; the tests pin
+// them to closed forms (points on circles, analytic extrema and crossings, gradient values).
 
 #include <algorithm>
 #include <cmath>
@@ -51,6 +57,8 @@ namespace {
 constexpr int FRAG = 2;  // FRAG_SIZE, make_intersection_0.comp:3
 constexpr int CUBIC_ITERS = 24;  // CUBIC_ITERATION_NUMBER, make_intersection_1.comp:7
 constexpr uint32_t T_LINE = 0x02, T_CUBIC = 0x04;  // make_intersection_0.comp:22-25
+// QUADRIC and ARC (make_intersection_0.comp:23-25; their shader cases are TODO in the reference)
+constexpr uint32_t T_QUAD = 0x03, T_ARC = 0x13;
 constexpr int32_t KEY_SENTINEL = (int32_t)0xFFFEFFFE;  // gen_fragment.comp:111
 
 inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
@@ -78,8 +86,18 @@ inline bool path_visible(int32_t m) {
 
 // Curve evaluation along one axis: interpolateGeneralCurve, make_intersection_1.comp:117-148
 // (the count pass's variant, make_intersection_0.comp:131-165, agrees for LINE and CUBIC).
+// QUADRIC: de Casteljau on c[0..2].  ARC, a rational quadratic held as c = (x0, w*x1, x2, w):
+// de Casteljau on the homogeneous values and on the weights (1, w, 1), then their quotient.
 inline float eval1(uint32_t type, float t, const float* c) {
     if (type == T_LINE) return lerp(c[0], c[1], t);
+    if (type == T_QUAD || type == T_ARC) {
+        float a = lerp(c[0], c[1], t), b = lerp(c[1], c[2], t);
+        float num = lerp(a, b, t);
+        if (type == T_QUAD) return num;
+        float wa = lerp(1.0f, c[3], t), wb = lerp(c[3], 1.0f, t);
+        float den = lerp(wa, wb, t);
+        return num / den;
+    }
     float q0 = lerp(c[0], c[1], t), q1 = lerp(c[1], c[2], t), q2 = lerp(c[2], c[3], t);
     float l0 = lerp(q0, q1, t), l1 = lerp(q1, q2, t);
     return lerp(l0, l1, t);
@@ -152,7 +170,16 @@ struct Curve {
     float x[4], y[4];
 };
 
-inline Curve load_curve(const float* tpos, const uint32_t* cpos, const uint32_t* ctype, uint32_t c) {
+// An arc's control point is stored in Euclidean form with its weight beside; evaluation takes
+// the homogeneous form (w*x1, w*y1) and the weight in slot 3.
+inline void to_eval_form(Curve& cv, float w) {
+    if (cv.type != T_ARC) return;
+    cv.x[1] = w * cv.x[1]; cv.y[1] = w * cv.y[1];
+    cv.x[3] = w; cv.y[3] = w;
+}
+
+inline Curve load_curve(const float* tpos, const uint32_t* cpos, const uint32_t* ctype, const float* cweight,
+                        uint32_t c) {
     Curve cv;
     cv.type = ctype[c];
     uint32_t np = cv.type & 7;  // make_intersection_0.comp:251-257
@@ -161,6 +188,7 @@ inline Curve load_curve(const float* tpos, const uint32_t* cpos, const uint32_t*
         if (i < np) { cv.x[i] = tpos[2 * (base + i)]; cv.y[i] = tpos[2 * (base + i) + 1]; }
         else { cv.x[i] = 0.0f; cv.y[i] = 0.0f; }
     }
+    to_eval_form(cv, cv.type == T_ARC ? cweight[c] : 1.0f);
     return cv;
 }
 
@@ -180,6 +208,34 @@ inline uint32_t monotone_cuts(const Curve& cv, bool visible, float q[4]) {
                 if (r0 > 0.0f && r0 < 1.0f) t1q[n++] = r0;
                 if (r1 > 0.0f && r1 < 1.0f && r1 != r0) t1q[n++] = r1;
             }
+        } else if (cv.type == T_QUAD || cv.type == T_ARC) {
+            // stationary points per axis.  QUADRIC: B'(t)/2 = (x0 - 2 x1 + x2) t + (x1 - x0).
+            // ARC: with homogeneous values (a0, a1, a2) and weights (1, w, 1), the numerator of
+            // x'(t) is 2 [e0 (1-t)^2 + e1 t (1-t) + e2 t^2], e0 = a1 - a0 w, e1 = a2 - a0,
+            // e2 = a2 w - a1; in powers of t: (e0 - e1 + e2) t^2 + (e1 - 2 e0) t + e0.
+            for (int axis = 0; axis < 2; ++axis) {
+                const float* p = axis == 0 ? cv.x : cv.y;
+                float A, B, Cc;
+                if (cv.type == T_QUAD) {
+                    A = 0.0f;
+                    B = (p[0] - p[1]) + (p[2] - p[1]);
+                    Cc = p[1] - p[0];
+                } else {
+                    float w = p[3];
+                    float e0 = p[1] - p[0] * w, e1 = p[2] - p[0], e2 = p[2] * w - p[1];
+                    A = (e0 - e1) + e2;
+                    B = e1 - 2.0f * e0;
+                    Cc = e0;
+                }
+                float r0 = 0.0f, r1 = 0.0f;
+                solve_quad(A, B, Cc, r0, r1);
+                if (r0 > 0.0f && r0 < 1.0f) t1q[n++] = r0;
+                if (r1 > 0.0f && r1 < 1.0f && r1 != r0) t1q[n++] = r1;
+            }
+            // conic cuts in ascending order (no reference ordering to follow)
+            std::sort(t1q, t1q + n);
+            q[0] = t1q[0]; q[1] = t1q[1]; q[2] = t1q[2]; q[3] = t1q[3];
+            return n;
         }
         q0 = t1q[0]; q1 = t1q[1]; q2 = t1q[2]; q3 = t1q[3];
         if (n >= 2) {  // :315-322
@@ -222,15 +278,16 @@ inline float solve_crossing(const Curve& cv, int side, float c, float t_min, flo
         a = (a != 0.0f ? (1.0f / a) : 0.0f);
         return gl_min(gl_max((c - x0) * a, t_min), t1_ms);
     }
+    // quadratics and arcs: the cubic's bisection on their own evaluation
     float t0 = t_min, t1 = t1_ms;
-    float vt0 = eval1(T_CUBIC, t0, p);
+    float vt0 = eval1(cv.type, t0, p);
     float t_solve = t0;
     if (vt0 != c) {
         float raw_t0 = t0;
         float last_vtm = 0.0f;
         for (int j = 0; j < CUBIC_ITERS; ++j) {
             float tm = (t0 + t1) * 0.5f;
-            float vtm = eval1(T_CUBIC, tm, p);
+            float vtm = eval1(cv.type, tm, p);
             t_solve = tm;
             last_vtm = vtm;
             if ((int32_t)(fbits(vtm - c) ^ fbits(vt0 - c)) >= 0) { t0 = tm; vt0 = vtm; }
@@ -241,12 +298,26 @@ inline float solve_crossing(const Curve& cv, int side, float c, float t_min, flo
     return t_solve;
 }
 
+// A gradient paint, 16 words: kind (1 linear, 2 radial), spread (0 pad, 1 repeat, 2 reflect,
+// 3 transparent), first stop, stop count, then as floats the opacity, the scene-to-gradient
+// affine map (gx = x0 x + x1 y + x2, gy = x3 x + x4 y + x5) and the geometry (linear x1, y1,
+// x2, y2; radial cx, cy, fx, fy, r).  Stops are 5 floats: offset, r, g, b, a.
+struct PaintRec {
+    uint32_t kind, spread, first, count;
+    float opacity, xf[6], geom[5];
+};
+
 struct Frame {
     // inputs (borrowed)
     const float* pos; const uint32_t* pos_path; uint32_t P;
     const uint32_t* cpos; const uint32_t* ctype; const uint32_t* cpath; uint32_t C;
     const uint32_t* rule; const uint32_t* fill; uint32_t Np;
+    const float* cweight = nullptr;          // arc weights (read for arcs only)
+    const uint32_t* path_paint = nullptr;    // per path: 0xFFFFFFFF or a paint index
+    const PaintRec* paints = nullptr;
+    const float* stops = nullptr;
     float m[16]; int w, h; int compose;
+    float inv[9];                             // inverse of the MVP's 2D projective part
     // stages
     std::vector<float> tpos;
     std::vector<int32_t> visible;
@@ -258,6 +329,7 @@ struct Frame {
     std::vector<int32_t> flags;   // 2N: cell flags then span flags
     std::vector<int32_t> flag_scan;  // 2N+1 exclusive
     std::vector<int32_t> records;    // 4R
+    std::vector<int32_t> rec_path;   // R: the path of each record
     int32_t n_cells = 0, n_spans = 0;
     std::vector<uint8_t> image;
     bool injected = false;
@@ -307,7 +379,7 @@ void stage_count(Frame& F) {
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads_setting)
     for (int64_t ci = 0; ci < (int64_t)F.C; ++ci) {
         uint32_t c = (uint32_t)ci;
-        Curve cv = load_curve(F.tpos.data(), F.cpos, F.ctype, c);
+        Curve cv = load_curve(F.tpos.data(), F.cpos, F.ctype, F.cweight, c);
         bool vis = path_visible(F.visible[F.cpath[c]]);
         float q[5];
         uint32_t n = monotone_cuts(cv, vis, q);
@@ -343,7 +415,7 @@ void stage_write(Frame& F) {
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads_setting)
     for (int64_t ci = 0; ci < (int64_t)F.C; ++ci) {
         uint32_t c = (uint32_t)ci;
-        Curve cv = load_curve(F.tpos.data(), F.cpos, F.ctype, c);
+        Curve cv = load_curve(F.tpos.data(), F.cpos, F.ctype, F.cweight, c);
         bool vis = path_visible(F.visible[F.cpath[c]]);
         float q[5];
         for (int i = 0; i < 4; ++i) q[i] = F.cuts5[5 * (size_t)c + i];
@@ -427,7 +499,7 @@ void stage_gen_fragment(Frame& F) {
         if (c0 != c1) t1 = 1.0f;
         int32_t swn = 0;
         if (t0 < t1) {
-            Curve cv = load_curve(F.tpos.data(), F.cpos, F.ctype, (uint32_t)c0);
+            Curve cv = load_curve(F.tpos.data(), F.cpos, F.ctype, F.cweight, (uint32_t)c0);
             float pfx = eval1(cv.type, t0, cv.x), pfy = eval1(cv.type, t0, cv.y);
             float plx = eval1(cv.type, t1, cv.x), ply = eval1(cv.type, t1, cv.y);
             int raw_x = wmul(float2int_rd(((pfx + plx) * 0.5f) / (float)FRAG), FRAG);
@@ -528,6 +600,7 @@ void stage_emit(Frame& F) {
     const int32_t N = (int32_t)F.skey.size();
     const int32_t R = F.n_cells + F.n_spans;
     F.records.assign(4 * (size_t)R, 0);
+    F.rec_path.assign((size_t)R, 0);
     const int32_t* S = F.flag_scan.data();
     for (int32_t f = 0; f < N; ++f) {
         int32_t ff = F.flags[f];
@@ -545,6 +618,7 @@ void stage_emit(Frame& F) {
             r[1] = 2;
             r[2] = (int32_t)F.fill[pidx];
             r[3] = fidx1;
+            F.rec_path[o] = pidx;
         }
         if (sf != 0) {
             int32_t o = nfb + nsb + ff;
@@ -559,8 +633,103 @@ void stage_emit(Frame& F) {
             r[1] = x1 - x0;
             r[2] = (int32_t)F.fill[pidx];
             r[3] = 0;
+            F.rec_path[o] = pidx;
         }
     }
+}
+
+// Inverse of the MVP's 2D projective part: a point (x, y, 0, 1) meets rows 0, 1 and 3, so the
+// map is [[m0, m1, m3], [m4, m5, m7], [m12, m13, m15]]; cofactors and determinant in double,
+// rounded to float; all zeros when singular.
+void mvp_inverse(const float* m, float* out) {
+    double a = m[0], b = m[1], c = m[3], d = m[4], e = m[5], f = m[7], g = m[12], h = m[13], k = m[15];
+    double c00 = e * k - f * h, c01 = -(d * k - f * g), c02 = d * h - e * g;
+    double det = a * c00 + b * c01 + c * c02;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
+        for (int i = 0; i < 9; ++i) out[i] = 0.0f;
+        return;
+    }
+    double adj[9] = {c00, -(b * k - c * h), b * f - c * e, c01, a * k - c * g, -(a * f - c * d), c02,
+                     -(a * h - b * g), a * e - b * d};
+    for (int i = 0; i < 9; ++i) out[i] = (float)(adj[i] / det);
+}
+
+// loadVG's colour packing (scanline_rasterizer.cpp:107-118): alpha times opacity, channels
+// times 255 truncated into bytes, 0 when the alpha byte is 0.
+uint32_t pack_rgba(float r, float g, float b, float a, float opacity) {
+    float ch[4] = {r, g, b, a * opacity};
+    uint32_t out = 0;
+    for (int i = 0; i < 4; ++i) {
+        float v = ch[i] * 255.0f;
+        int iv = (v != v) ? 0 : (v <= 0.0f ? 0 : (v >= 255.0f ? 255 : (int)v));
+        out |= (uint32_t)iv << (8 * i);
+    }
+    return (out & 0xFF000000u) ? out : 0u;
+}
+
+// Colour of gradient paint pi at the centre of canvas pixel (x, y).  The centre goes back to
+// the scene through MVP^-1, then to gradient space through the paint's affine map.  Linear:
+// t = ((g - p1) . d) / (d . d), d = p2 - p1 (equal points: the last stop).  Radial: the t >= 0
+// for which g lies on the circle of centre f + t (c - f) and radius t r, the larger root of
+// (d.d - r^2) t^2 - 2 (q.d) t + q.q = 0 with q = g - f, d = c - f (focus inside: d.d < r^2).
+// NaN t counts as 0.  Spread: pad clamps to [0, 1], repeat keeps t - floor(t), reflect folds
+// t mod 2 back into [0, 1], transparent gives colour 0 outside [0, 1].  Between stops the colour
+// is interpolated per channel (non-premultiplied), then packed.
+uint32_t paint_rgba(const Frame& F, uint32_t pi, int x, int y) {
+    const float* H = F.inv;
+    float px = (float)x + 0.5f, py = (float)y + 0.5f;
+    float u = (H[0] * px + H[1] * py) + H[2];
+    float v = (H[3] * px + H[4] * py) + H[5];
+    float q = (H[6] * px + H[7] * py) + H[8];
+    float sx = u / q, sy = v / q;
+    const PaintRec& P = F.paints[pi];
+    float gx = (P.xf[0] * sx + P.xf[1] * sy) + P.xf[2];
+    float gy = (P.xf[3] * sx + P.xf[4] * sy) + P.xf[5];
+    float t = 0.0f;
+    bool last_stop = false;
+    if (P.kind == 1) {
+        float dx = P.geom[2] - P.geom[0], dy = P.geom[3] - P.geom[1];
+        float dd = dx * dx + dy * dy;
+        if (dd > 0.0f) t = ((gx - P.geom[0]) * dx + (gy - P.geom[1]) * dy) / dd;
+        else { t = 1.0f; last_stop = true; }
+    } else {
+        float qx = gx - P.geom[2], qy = gy - P.geom[3];
+        float dx = P.geom[0] - P.geom[2], dy = P.geom[1] - P.geom[3];
+        float r = P.geom[4];
+        float A = (dx * dx + dy * dy) - r * r;  // < 0
+        float B = qx * dx + qy * dy;             // the root formula's -b/2
+        float Cq = qx * qx + qy * qy;
+        t = (B - std::sqrt(B * B - A * Cq)) / A;
+    }
+    if (t != t) t = 0.0f;
+    if (!last_stop) {
+        switch (P.spread) {
+            case 0: t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t); break;
+            case 1: t = t - std::floor(t); break;
+            case 2: {
+                float m2 = t - 2.0f * std::floor(t * 0.5f);
+                t = m2 > 1.0f ? 2.0f - m2 : m2;
+                break;
+            }
+            default:
+                if (t < 0.0f || t > 1.0f) return 0u;
+        }
+        if (t != t) t = 0.0f;
+    }
+    const float* S = F.stops + 5 * (size_t)P.first;
+    uint32_t n = P.count;
+    float col[4];
+    if (t <= S[0]) {
+        for (int k = 0; k < 4; ++k) col[k] = S[1 + k];
+    } else if (t >= S[5 * (n - 1)]) {
+        for (int k = 0; k < 4; ++k) col[k] = S[5 * (n - 1) + 1 + k];
+    } else {
+        uint32_t i = 0;
+        while (i + 1 < n && !(t < S[5 * (i + 1)])) ++i;
+        float f = (t - S[5 * i]) / (S[5 * (i + 1)] - S[5 * i]);
+        for (int k = 0; k < 4; ++k) col[k] = lerp(S[5 * i + 1 + k], S[5 * (i + 1) + 1 + k], f);
+    }
+    return pack_rgba(col[0], col[1], col[2], col[3], P.opacity);
 }
 
 // Line-list draw: scanlinepr.vert:18-50 + scanlinepr.frag:7-14 with the pipeline state of
@@ -570,20 +739,30 @@ void stage_emit(Frame& F) {
 // maps canvas row y to image row h-1-y (the vertex shader flips y).  Image bytes are R,G,B,A.
 // compose 0 = overwrite (the reference); compose 1 = src-over d = s*a + d*(1-a) per colour
 // channel, alpha a + d_a*(1-a), accumulated in fp32 in record order and rounded once.
+// A record of a path with a gradient paint (no reference counterpart) takes each pixel's
+// colour from the paint instead of its fill.
 void stage_draw(Frame& F) {
     const int w = F.w, h = F.h;
     const int32_t R = F.n_cells + F.n_spans;
     F.image.assign(4 * (size_t)w * h, 255);
+    auto paint_of = [&](int32_t k) -> uint32_t {
+        if (!F.path_paint || F.rec_path.empty()) return 0xFFFFFFFFu;
+        return F.path_paint[F.rec_path[k]];
+    };
     if (F.compose == 0) {
         for (int32_t k = 0; k < R; ++k) {
             const int32_t* r = &F.records[4 * (size_t)k];
             int x = r[0] & 0xFFFF, y = r[0] >> 16;
             int x0 = std::max(x, 0), x1 = std::min(x + r[1], w);
             uint32_t c = (uint32_t)r[2];
+            const uint32_t pi = paint_of(k);
             for (int yy = y; yy < y + 2; ++yy) {
                 if (yy < 0 || yy >= h) continue;
                 uint8_t* row = &F.image[4 * (size_t)(h - 1 - yy) * w];
-                for (int xx = x0; xx < x1; ++xx) std::memcpy(row + 4 * xx, &c, 4);
+                for (int xx = x0; xx < x1; ++xx) {
+                    if (pi != 0xFFFFFFFFu) c = paint_rgba(F, pi, xx, yy);
+                    std::memcpy(row + 4 * xx, &c, 4);
+                }
             }
         }
         return;
@@ -593,14 +772,15 @@ void stage_draw(Frame& F) {
         const int32_t* r = &F.records[4 * (size_t)k];
         int x = r[0] & 0xFFFF, y = r[0] >> 16;
         int x0 = std::max(x, 0), x1 = std::min(x + r[1], w);
-        uint32_t c = (uint32_t)r[2];
-        float s[4] = {(float)(c & 0xFF) / 255.0f, (float)((c >> 8) & 0xFF) / 255.0f,
-                      (float)((c >> 16) & 0xFF) / 255.0f, (float)((c >> 24) & 0xFF) / 255.0f};
-        float a = s[3], ia = 1.0f - a;
+        const uint32_t pi = paint_of(k);
         for (int yy = y; yy < y + 2; ++yy) {
             if (yy < 0 || yy >= h) continue;
             float* row = &acc[4 * (size_t)(h - 1 - yy) * w];
             for (int xx = x0; xx < x1; ++xx) {
+                uint32_t c = pi != 0xFFFFFFFFu ? paint_rgba(F, pi, xx, yy) : (uint32_t)r[2];
+                float s[4] = {(float)(c & 0xFF) / 255.0f, (float)((c >> 8) & 0xFF) / 255.0f,
+                              (float)((c >> 16) & 0xFF) / 255.0f, (float)((c >> 24) & 0xFF) / 255.0f};
+                float a = s[3], ia = 1.0f - a;
                 float* d = row + 4 * xx;
                 d[0] = s[0] * a + d[0] * ia;
                 d[1] = s[1] * a + d[1] * ia;
@@ -639,15 +819,23 @@ int oracle_max_threads(void) {
 
 // Runs the whole frame.  inter_override (optional, 2*n_inter ints) replaces the two-pass
 // intersection generation, so the integer stages can be checked on identical intersections.
+// cweight (arc weights), path_paint, paints (16 words each) and stops (5 floats each) may be
+// null for scenes without arcs or gradients.
 void* oracle_run(const float* pos, const uint32_t* pos_path, uint32_t P, const uint32_t* cpos,
                  const uint32_t* ctype, const uint32_t* cpath, uint32_t C, const uint32_t* rule,
                  const uint32_t* fill, uint32_t Np, const float* mvp_rows, int w, int h,
-                 int compose, const int32_t* inter_override, int64_t n_inter) {
+                 int compose, const int32_t* inter_override, int64_t n_inter, const float* cweight,
+                 const uint32_t* path_paint, const void* paints, const float* stops) {
     Frame* F = new Frame();
     F->pos = pos; F->pos_path = pos_path; F->P = P;
     F->cpos = cpos; F->ctype = ctype; F->cpath = cpath; F->C = C;
     F->rule = rule; F->fill = fill; F->Np = Np;
+    F->cweight = cweight;
+    F->path_paint = path_paint;
+    F->paints = (const PaintRec*)paints;
+    F->stops = stops;
     std::memcpy(F->m, mvp_rows, sizeof(F->m));
+    mvp_inverse(F->m, F->inv);
     F->w = w; F->h = h; F->compose = compose;
     stage_transform(*F);
     stage_count(*F);
@@ -696,19 +884,41 @@ void oracle_free(void* ctx) { delete (Frame*)ctx; }
 // Single-stage entry points used by the known-answer tests.
 void oracle_solve_quad(float a, float b, float c, float* r0, float* r1) { solve_quad(a, b, c, *r0, *r1); }
 
-int oracle_curve_cuts(uint32_t type, const float* xs, const float* ys, int visible, float* q4) {
+// The curve entry points take the control values as stored: for an arc (x0, x1, x2, w) with
+// x1 the Euclidean control point and the weight in slot 3 (ys[3] is ignored).
+Curve stored_curve(uint32_t type, const float* xs, const float* ys) {
     Curve cv; cv.type = type;
     for (int i = 0; i < 4; ++i) { cv.x[i] = xs[i]; cv.y[i] = ys[i]; }
+    to_eval_form(cv, xs[3]);
+    return cv;
+}
+
+int oracle_curve_cuts(uint32_t type, const float* xs, const float* ys, int visible, float* q4) {
+    Curve cv = stored_curve(type, xs, ys);
     return (int)monotone_cuts(cv, visible != 0, q4);
 }
 
-float oracle_eval(uint32_t type, float t, const float* c4) { return eval1(type, t, c4); }
+// One axis at t; for an arc c4 = (x0, x1, x2, w) as stored.
+float oracle_eval(uint32_t type, float t, const float* c4) {
+    float c[4] = {c4[0], c4[1], c4[2], c4[3]};
+    if (type == T_ARC) c[1] = c4[3] * c4[1];
+    return eval1(type, t, c);
+}
 
 float oracle_solve_crossing(uint32_t type, const float* xs, const float* ys, int side, float c,
                             float t_min, float t1) {
-    Curve cv; cv.type = type;
-    for (int i = 0; i < 4; ++i) { cv.x[i] = xs[i]; cv.y[i] = ys[i]; }
+    Curve cv = stored_curve(type, xs, ys);
     return solve_crossing(cv, side, c, t_min, t1);
+}
+
+// One gradient colour at canvas pixel (x, y) under the given MVP: the composite's paint rule.
+uint32_t oracle_paint(const void* paint, const float* stops, const float* mvp_rows, int x, int y) {
+    Frame F;
+    F.paints = (const PaintRec*)paint;
+    F.stops = stops;
+    std::memcpy(F.m, mvp_rows, sizeof(F.m));
+    mvp_inverse(F.m, F.inv);
+    return paint_rgba(F, 0, x, y);
 }
 
 int oracle_path_visible(int32_t mask) { return path_visible(mask) ? 1 : 0; }

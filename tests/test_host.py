@@ -100,8 +100,9 @@ def test_rvg_known_values(spr):
 
 
 @pytest.mark.parametrize("body,msg", [
-    ("M 0,0 A 1,1 0 0 1 5,5 Z", "unsupported path command 'A'"),
-    ("M 0,0 Q 1,1 2,2 Z", "unsupported path command 'Q'"),
+    ("M 0,0 A 1,1 0 0 1 5,5 Z", "bad arc '1,1'"),  # SVG's arc syntax, not RVG's (x1,y1,w1 x2,y2)
+    ("M 0,0 A 1,1,-1 5,5 Z", "arc weight must be finite and greater than -1"),
+    ("M 0,0 q 1,1 2,2 Z", "unsupported path command 'q'"),
     ("M 0,0 l 1,1 Z", "unsupported path command 'l'"),
 ])
 def test_rvg_rejects_unsupported_commands(spr, tmp_path, body, msg):

@@ -81,7 +81,20 @@ class SceneView(C.Structure):
                 ("curve_pos_map", C.POINTER(C.c_uint32)), ("curve_type", C.POINTER(C.c_uint32)),
                 ("curve_path_idx", C.POINTER(C.c_uint32)), ("n_curves", C.c_uint32),
                 ("fill_rule", C.POINTER(C.c_uint32)), ("fill_info", C.POINTER(C.c_uint32)),
-                ("n_paths", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32)]
+                ("n_paths", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("curve_weight", C.POINTER(C.c_float)), ("path_paint", C.POINTER(C.c_uint32)),
+                ("paints", C.c_void_p), ("n_paints", C.c_uint32),
+                ("stops", C.c_void_p), ("n_stops", C.c_uint32)]
+
+
+# spr_paint (64 bytes) and spr_stop (5 floats) as numpy records
+PAINT_DTYPE = np.dtype([("kind", "<u4"), ("spread", "<u4"), ("first_stop", "<u4"), ("n_stops", "<u4"),
+                        ("opacity", "<f4"), ("xform", "<f4", (6,)), ("geom", "<f4", (5,))])
+assert PAINT_DTYPE.itemsize == 64
+PAINT_LINEAR, PAINT_RADIAL = 1, 2
+SPREAD_PAD, SPREAD_REPEAT, SPREAD_REFLECT, SPREAD_TRANSPARENT = 0, 1, 2, 3
+SOLID_PAINT = 0xFFFFFFFF
+CURVE_LINE, CURVE_QUAD, CURVE_CUBIC, CURVE_ARC = 2, 3, 4, 0x13
 
 
 _lib = None
@@ -90,7 +103,7 @@ EXPORTS = [
     "spr_load_rvg", "spr_load_scene", "spr_set_mvp", "spr_render_frame", "spr_image_device_ptr",
     "spr_synchronize", "spr_set_stage_timing", "spr_get_stats", "spr_debug_export",
     "spr_debug_inject", "spr_scene_load_rvg", "spr_scene_generate", "spr_scene_subset",
-    "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free", "spr_learned_train",
+    "spr_scene_view_get", "spr_scene_write_rvg", "spr_scene_free", "spr_scene_create", "spr_learned_train",
     "spr_learned_save", "spr_learned_load", "spr_learned_set_margin", "spr_mlp_get_params",
     "spr_mlp_set_params", "spr_mlp_predict", "spr_mlp_gradients", "spr_load_batch", "spr_batch_image",
     "spr_autotune", "spr_set_tuning", "spr_scan_i32", "spr_scan_i32_noalloc", "spr_write_png",
@@ -130,6 +143,7 @@ def lib() -> C.CDLL:
     L.spr_scene_view_get.argtypes = [vp, C.POINTER(SceneView)]
     L.spr_scene_write_rvg.argtypes = [vp, C.c_char_p]
     L.spr_scene_free.argtypes = [vp]
+    L.spr_scene_create.argtypes = [C.POINTER(SceneView), C.POINTER(vp)]
     L.spr_scene_free.restype = None
     L.spr_learned_train.argtypes = [vp, vp, C.POINTER(TrainOpts), C.POINTER(TrainReport)]
     L.spr_learned_save.argtypes = [vp, C.c_char_p]
@@ -224,8 +238,49 @@ class Scene:
                 rule=cp(v.fill_rule, v.n_paths, np.uint32),
                 fill=cp(v.fill_info, v.n_paths, np.uint32),
                 width=int(v.width), height=int(v.height),
+                curve_weight=cp(v.curve_weight, v.n_curves, np.float32) if v.curve_weight else None,
+                path_paint=cp(v.path_paint, v.n_paths, np.uint32) if v.path_paint else None,
+                paints=(np.frombuffer(C.string_at(v.paints, 64 * v.n_paints), dtype=PAINT_DTYPE).copy()
+                        if v.n_paints else np.zeros(0, dtype=PAINT_DTYPE)),
+                stops=(np.frombuffer(C.string_at(v.stops, 20 * v.n_stops), dtype=np.float32).reshape(-1, 5).copy()
+                       if v.n_stops else np.zeros((0, 5), dtype=np.float32)),
             )
         return self._arrays
+
+    @classmethod
+    def create(cls, pos, pos_path, curve_pos, curve_type, curve_path, rule, fill, width: int = 0,
+               height: int = 0, curve_weight=None, path_paint=None, paints=None, stops=None) -> "Scene":
+        """A scene from arrays (spr_scene_create): copied and validated by the library."""
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a
+
+        v = SceneView()
+        pos = arr(pos, np.float32).reshape(-1)
+        pp, cpos, ctype, cpath = (arr(pos_path, np.uint32), arr(curve_pos, np.uint32),
+                                  arr(curve_type, np.uint32), arr(curve_path, np.uint32))
+        rl, fl = arr(rule, np.uint32), arr(fill, np.uint32)
+        u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+        v.pos_xy = pos.ctypes.data_as(C.POINTER(C.c_float))
+        v.pos_path_idx, v.n_points = u32(pp), len(pp)
+        v.curve_pos_map, v.curve_type, v.curve_path_idx, v.n_curves = u32(cpos), u32(ctype), u32(cpath), len(ctype)
+        v.fill_rule, v.fill_info, v.n_paths = u32(rl), u32(fl), len(rl)
+        v.width, v.height = int(width), int(height)
+        if curve_weight is not None:
+            cw = arr(curve_weight, np.float32)
+            v.curve_weight = cw.ctypes.data_as(C.POINTER(C.c_float))
+        if path_paint is not None:
+            v.path_paint = u32(arr(path_paint, np.uint32))
+            g = arr(paints, PAINT_DTYPE)
+            st = arr(stops, np.float32).reshape(-1, 5)
+            v.paints, v.n_paints = g.ctypes.data, len(g)
+            v.stops, v.n_stops = st.ctypes.data, len(st)
+        out = C.c_void_p()
+        _check(lib().spr_scene_create(C.byref(v), C.byref(out)))
+        return cls(out.value)
 
     @property
     def n_paths(self) -> int:

@@ -14,7 +14,9 @@ namespace spr {
 
 constexpr int kFrag = 2;           // FRAG_SIZE (make_intersection_0.comp:3)
 constexpr int kCubicIters = 24;    // CUBIC_ITERATION_NUMBER (make_intersection_1.comp:7)
-constexpr uint32_t kLine = 2u, kCubic = 4u;
+// CurveType (src/core/vg/vg_container.h:9-15).  A curve owns type & 7 consecutive points
+// (make_intersection_0.comp:251-257): line 2, quadratic 3, cubic 4, arc 3.
+constexpr uint32_t kLine = 2u, kQuad = 3u, kCubic = 4u, kArc = 0x13u;
 constexpr uint32_t kSentinelKey = 0xFFFEFFFEu;  // off-canvas cell key (gen_fragment.comp:111)
 constexpr uint32_t kNoPath = 0xFFFFFFFFu;      // entry path of a gap in a predicted-allocation layout
 constexpr uint32_t kTwoBits = 0x40000000u;      // bit pattern of 2.0f: an exhausted chain
@@ -60,33 +62,55 @@ __host__ __device__ inline int32_t outcode_bits(float x, float y, float fw, floa
     }
 }
 
-// One axis of a line or cubic at t (interpolateGeneralCurve, make_intersection_1.comp:117-148).
-__host__ __device__ inline float eval_axis(bool cubic, float t, float c0, float c1, float c2, float c3) {
-    if (!cubic) return lerp(c0, c1, t);
-    float q0 = lerp(c0, c1, t), q1 = lerp(c1, c2, t), q2 = lerp(c2, c3, t);
-    float l0 = lerp(q0, q1, t), l1 = lerp(q1, q2, t);
-    return lerp(l0, l1, t);
+// One axis of a curve at t.  Line and cubic: interpolateGeneralCurve
+// (make_intersection_1.comp:117-148).  The reference declares QUADRIC and ARC (a rational
+// quadratic) but leaves both cases TODO there; here a quadratic is de Casteljau on its three
+// control values, and an arc de Casteljau on its homogeneous values (c0, w*x1, c2; c1 holds the
+// product) and on its weights (1, w, 1; c3 holds w), then one division.
+__host__ __device__ inline float eval_axis(uint32_t type, float t, float c0, float c1, float c2, float c3) {
+    if (type == kCubic) {
+        float q0 = lerp(c0, c1, t), q1 = lerp(c1, c2, t), q2 = lerp(c2, c3, t);
+        float l0 = lerp(q0, q1, t), l1 = lerp(q1, q2, t);
+        return lerp(l0, l1, t);
+    }
+    if (type == kLine) return lerp(c0, c1, t);
+    const float n0 = lerp(c0, c1, t), n1 = lerp(c1, c2, t);
+    const float n = lerp(n0, n1, t);
+    if (type == kQuad) return n;
+    const float d0 = lerp(1.0f, c3, t), d1 = lerp(c3, 1.0f, t);
+    return n / lerp(d0, d1, t);
 }
 
+// Control values of a curve in evaluation form: an arc's x[1], y[1] hold w times its control
+// point and x[3] = y[3] = w; unused slots are 0.
 struct CurvePts {
     float x[4], y[4];
-    bool cubic;
-    __host__ __device__ inline float ex(float t) const { return eval_axis(cubic, t, x[0], x[1], x[2], x[3]); }
-    __host__ __device__ inline float ey(float t) const { return eval_axis(cubic, t, y[0], y[1], y[2], y[3]); }
+    uint32_t type;
+    __host__ __device__ inline float ex(float t) const { return eval_axis(type, t, x[0], x[1], x[2], x[3]); }
+    __host__ __device__ inline float ey(float t) const { return eval_axis(type, t, y[0], y[1], y[2], y[3]); }
 };
 
-__device__ inline CurvePts load_curve(const float2* __restrict__ tpos, uint32_t base, uint32_t type) {
-    CurvePts c;
-    c.cubic = (type == kCubic);
-    float2 p0 = tpos[base], p1 = tpos[base + 1];
-    c.x[0] = p0.x; c.y[0] = p0.y; c.x[1] = p1.x; c.y[1] = p1.y;
-    if (c.cubic) {
-        float2 p2 = tpos[base + 2], p3 = tpos[base + 3];
-        c.x[2] = p2.x; c.y[2] = p2.y; c.x[3] = p3.x; c.y[3] = p3.y;
-    } else {
-        c.x[2] = c.x[3] = c.y[2] = c.y[3] = 0.0f;
+// Curve c from the transformed points; an arc's weight (cweight[c], w > 0) is read only for arcs.
+__device__ inline CurvePts load_curve(const float2* __restrict__ tpos, uint32_t base, uint32_t type,
+                                      const float* __restrict__ cweight, uint32_t c) {
+    CurvePts cv;
+    cv.type = type;
+    const float2 p0 = tpos[base], p1 = tpos[base + 1];
+    cv.x[0] = p0.x; cv.y[0] = p0.y; cv.x[1] = p1.x; cv.y[1] = p1.y;
+    cv.x[2] = cv.y[2] = cv.x[3] = cv.y[3] = 0.0f;
+    if (type == kCubic) {
+        const float2 p2 = tpos[base + 2], p3 = tpos[base + 3];
+        cv.x[2] = p2.x; cv.y[2] = p2.y; cv.x[3] = p3.x; cv.y[3] = p3.y;
+    } else if (type != kLine) {
+        const float2 p2 = tpos[base + 2];
+        cv.x[2] = p2.x; cv.y[2] = p2.y;
+        if (type == kArc) {
+            const float w = cweight[c];
+            cv.x[1] = w * p1.x; cv.y[1] = w * p1.y;
+            cv.x[3] = w; cv.y[3] = w;
+        }
     }
-    return c;
+    return cv;
 }
 
 // Roots of a*t^2 + b*t + c in citardauq form (solveQuadEquation, make_intersection_0.comp:76-129).
@@ -112,21 +136,50 @@ __host__ __device__ inline void solve_quad(float a, float b, float c, float& r0,
     }
 }
 
-// Interior monotone cuts of a curve, ordered as the reference orders them (including its
-// 4-cut insertion that never moves the fourth cut, make_intersection_0.comp:338-341).  Slots
+// Interior monotone cuts of a curve: the roots in (0, 1) of each axis's derivative, x roots
+// first.  A cubic's are ordered as the reference orders them (including its 4-cut insertion
+// that never moves the fourth cut, make_intersection_0.comp:338-341).  A quadratic's derivative
+// is linear; an arc's derivative numerator N'D - ND' is, per axis with homogeneous values
+// (a0, a1, a2) and weights (1, w, 1), 2[e0 (1-t)^2 + e1 t(1-t) + e2 t^2] with e0 = a1 - a0 w,
+// e1 = a2 - a0, e2 = a2 w - a1, solved in power form; conic cuts are sorted ascending.  Slots
 // at or beyond the returned count are 0.
 __host__ __device__ inline uint32_t monotone_cuts(const CurvePts& cv, bool visible, float q[4]) {
     q[0] = q[1] = q[2] = q[3] = 0.0f;
-    if (!visible || !cv.cubic) return 0;
+    if (!visible || cv.type == kLine) return 0;
     uint32_t n = 0;
     float t[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int axis = 0; axis < 2; ++axis) {
         const float* p = axis == 0 ? cv.x : cv.y;
+        float a, b, c;
+        if (cv.type == kCubic) {
+            a = 3.0f * (p[1] - p[2]) + (p[3] - p[0]);
+            b = 2.0f * ((p[0] - p[1]) + (p[2] - p[1]));
+            c = p[1] - p[0];
+        } else if (cv.type == kQuad) {
+            a = 0.0f;
+            b = (p[0] - p[1]) + (p[2] - p[1]);
+            c = p[1] - p[0];
+        } else {
+            const float w = p[3];
+            const float e0 = p[1] - p[0] * w, e1 = p[2] - p[0], e2 = p[2] * w - p[1];
+            a = (e0 - e1) + e2;
+            b = e1 - 2.0f * e0;
+            c = e0;
+        }
         float r0, r1;
-        solve_quad(3.0f * (p[1] - p[2]) + (p[3] - p[0]), 2.0f * ((p[0] - p[1]) + (p[2] - p[1])), p[1] - p[0], r0, r1);
+        solve_quad(a, b, c, r0, r1);
         if (r0 > 0.0f && r0 < 1.0f) t[n++] = r0;
         if (r1 > 0.0f && r1 < 1.0f && r1 != r0) t[n++] = r1;
+    }
+    if (cv.type != kCubic) {
+#pragma unroll
+        for (int i = 1; i < 4; ++i)
+#pragma unroll
+            for (int j = i; j > 0; --j)
+                if (i < (int)n && t[j] < t[j - 1]) { const float s = t[j]; t[j] = t[j - 1]; t[j - 1] = s; }
+        q[0] = t[0]; q[1] = t[1]; q[2] = t[2]; q[3] = t[3];
+        return n;
     }
     float q0 = t[0], q1 = t[1], q2 = t[2], q3 = t[3];
     if (n >= 2 && q1 < q0) { float s = q0; q0 = q1; q1 = s; }
@@ -192,23 +245,20 @@ __host__ __device__ inline float flag_piece_end(float t_raw, float p1x) {
     return (floorf(p1x) == p1x) ? bitsf(b | 2u) : bitsf(b | 3u);
 }
 
-// Control values of one axis of a curve.
+// Control values of one axis of a curve, in the evaluation form of CurvePts.
 struct CurveAxis {
     float c[4];
 };
 
 // One grid-crossing solve (make_intersection_1.comp:377-437): a line is solved by its
 // reciprocal slope, a cubic by 24 bisection steps on [t_min, t1] keyed on the sign of B(t)-c.
-__device__ inline float solve_axis(bool cubic, const CurveAxis& ax, float c, float t_min, float t1) {
-    const float* p = ax.c;
-    if (!cubic) {
-        float a = p[1] - p[0];
-        a = (a != 0.0f ? (1.0f / a) : 0.0f);
-        return glmin(glmax((c - p[0]) * a, t_min), t1);
-    }
-    const float c0 = p[0], c1 = p[1], c2 = p[2], c3 = p[3];
+// Quadratics and arcs (TODO in the reference) take the cubic's bisection on their own
+// evaluation.
+template <uint32_t T>
+__device__ inline float bisect_axis(const CurveAxis& ax, float c, float t_min, float t1) {
+    const float c0 = ax.c[0], c1 = ax.c[1], c2 = ax.c[2], c3 = ax.c[3];
     float t0 = t_min;
-    float vt0 = eval_axis(true, t0, c0, c1, c2, c3);
+    float vt0 = eval_axis(T, t0, c0, c1, c2, c3);
     if (vt0 == c) return t0;
     // The sign of vt0-c never changes (vt0 is only replaced by a value of equal sign), so the
     // comparison is against the initial sign bit.
@@ -217,10 +267,23 @@ __device__ inline float solve_axis(bool cubic, const CurveAxis& ax, float c, flo
 #pragma unroll 4
     for (int j = 0; j < kCubicIters; ++j) {
         tm = (t0 + t1) * 0.5f;
-        vtm = eval_axis(true, tm, c0, c1, c2, c3);
+        vtm = eval_axis(T, tm, c0, c1, c2, c3);
         if ((fbits(vtm - c) & 0x80000000u) == s0) t0 = tm; else t1 = tm;
     }
     return fabsf(vtm - c) > 1.0f ? t_min : tm;
+}
+
+__device__ inline float solve_line(const CurveAxis& ax, float c, float t_min, float t1) {
+    float a = ax.c[1] - ax.c[0];
+    a = (a != 0.0f ? (1.0f / a) : 0.0f);
+    return glmin(glmax((c - ax.c[0]) * a, t_min), t1);
+}
+
+__device__ inline float solve_axis(uint32_t type, const CurveAxis& ax, float c, float t_min, float t1) {
+    if (type == kLine) return solve_line(ax, c, t_min, t1);
+    if (type == kCubic) return bisect_axis<kCubic>(ax, c, t_min, t1);
+    if (type == kQuad) return bisect_axis<kQuad>(ax, c, t_min, t1);
+    return bisect_axis<kArc>(ax, c, t_min, t1);
 }
 
 // Bucket of an entry inside its path's (path, cell row) bucket range.  pr = (first row >= 1,

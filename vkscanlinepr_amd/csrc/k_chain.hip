@@ -52,11 +52,12 @@ inline unsigned grid_for(int64_t n) {
     return (unsigned)(g < 1 ? 1 : g);
 }
 
-// Length class of a chain of n >= 1 crossings: cubic chains by octave and by the two bits after
-// the leading one (quarter octaves), line chains by octave.
-__device__ inline int chain_class(int n, bool cubic) {
+// Length class of a chain of n >= 1 crossings: bisected chains (cubics, quadratics and arcs,
+// called cubic chains below) by octave and by the two bits after the leading one (quarter
+// octaves), line chains by octave.
+__device__ inline int chain_class(int n, bool bisected) {
     const int l = 31 - __clz(n);
-    if (!cubic) return 64 + (l > 15 ? 15 : l);
+    if (!bisected) return 64 + (l > 15 ? 15 : l);
     if (l > 15) return 63;
     const int sub = l >= 2 ? (n >> (l - 2)) & 3 : (l == 1 ? (n & 1) << 1 : 0);
     return 4 * l + sub;
@@ -72,7 +73,7 @@ __device__ inline int class_cost_octave(int cls) { return cls < 64 ? cls >> 2 : 
 // class byte (0xFF: no chain).
 __global__ __launch_bounds__(kBlock) void k_chain_prep(
     const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
-    const uint32_t* __restrict__ ctype, const uint32_t* __restrict__ cpath,
+    const uint32_t* __restrict__ ctype, const float* __restrict__ cweight, const uint32_t* __restrict__ cpath,
     const int32_t* __restrict__ visible, const float4* __restrict__ cuts,
     const uint32_t* __restrict__ ncuts, const int32_t* __restrict__ offsets, uint32_t C, int w,
     int h, ChainDesc* __restrict__ chains, PieceDesc* __restrict__ pieces, uint8_t* __restrict__ chain_cls,
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
         float q[4] = {1.0f, 1.0f, 1.0f, 1.0f};
         int base0 = 0, limit = 0;
         if (nc != 0xFFFFFFFFu) {
-            cv = load_curve(tpos, cpos[c], type);
+            cv = load_curve(tpos, cpos[c], type, cweight, c);
             const float4 qq = cuts[c];
             q[0] = qq.x; q[1] = qq.y; q[2] = qq.z; q[3] = qq.w;
             base0 = offsets[c];
@@ -141,9 +142,9 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
                         d.step = side ? G.dy : G.dx;
                         d.t0 = t0;
                         d.t1 = t1;
-                        d.cubic = type == kCubic;
+                        d.type = (int16_t)type;
                         chains[2 * (5 * (int64_t)c + k) + side] = d;
-                        const uint32_t kc = (uint32_t)chain_class(n, d.cubic != 0);
+                        const uint32_t kc = (uint32_t)chain_class(n, type != kLine);
                         atomicAdd(&hist[kc], 1);
                         if (side) k1 = kc; else k0 = kc;
                     }
@@ -220,26 +221,34 @@ __global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restric
     }
 }
 
+// The chain's axis of its curve, in evaluation form (an arc's c1 = w * x1 and c3 = w).
 __device__ inline CurveAxis load_axis(const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
-                                      const ChainDesc& d) {
+                                      const float* __restrict__ cweight, const ChainDesc& d) {
     const float2* p = tpos + cpos[d.curve];
+    const uint32_t type = (uint32_t)d.type;
     CurveAxis a;
-    const int npts = d.cubic ? 4 : 2;
+    const int npts = (int)(type & 7u);
     a.c[0] = d.side ? p[0].y : p[0].x;
     a.c[1] = d.side ? p[1].y : p[1].x;
-    a.c[2] = npts == 4 ? (d.side ? p[2].y : p[2].x) : 0.0f;
+    a.c[2] = npts >= 3 ? (d.side ? p[2].y : p[2].x) : 0.0f;
     a.c[3] = npts == 4 ? (d.side ? p[3].y : p[3].x) : 0.0f;
+    if (type == kArc) {
+        const float w = cweight[d.curve];
+        a.c[1] = w * a.c[1];
+        a.c[3] = w;
+    }
     return a;
 }
 
 // Sequential chain (one lane): the reference's solve per crossing.
-__device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
-                                        uint8_t* __restrict__ mono) {
+template <uint32_t T>
+__device__ inline void solve_chain_lane_t(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
+                                          uint8_t* __restrict__ mono) {
     float tprev = d.t0, line = d.line0;
     bool up = true, down = true;
     const uint32_t side = (uint32_t)d.side;
     for (int k = 0; k < d.n; ++k) {
-        const float ts = solve_axis(d.cubic, a, line, tprev, d.t1);
+        const float ts = T == kLine ? solve_line(a, line, tprev, d.t1) : bisect_axis<T>(a, line, tprev, d.t1);
         const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | side);
         out[k] = v;
         // the y chain is preceded by the piece start in the merge, the x chain is not
@@ -250,6 +259,15 @@ __device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, 
     *mono = (up ? 1 : 0) | (down ? 2 : 0);
 }
 
+__device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
+                                        uint8_t* __restrict__ mono) {
+    const uint32_t type = (uint32_t)d.type;
+    if (type == kCubic) solve_chain_lane_t<kCubic>(d, a, out, mono);
+    else if (type == kLine) solve_chain_lane_t<kLine>(d, a, out, mono);
+    else if (type == kQuad) solve_chain_lane_t<kQuad>(d, a, out, mono);
+    else solve_chain_lane_t<kArc>(d, a, out, mono);
+}
+
 // Cubic chain solved by one wave: 4 rounds of a 6-level speculative bisection per crossing.
 // Lane l (1..63) is node l of the round's decision tree: its depth is floor(log2 l) and the bits
 // of l below the leading one are its path (1 = right).  Each lane halves the round's interval
@@ -258,8 +276,9 @@ __device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, 
 // depth-5 node (lanes 32..63) lies on that path when D holds, at its five ancestors, the turns
 // that lead to it: one masked compare per lane and a second ballot name the node, whose
 // interval, midpoint and value are then read from its lane.
-__device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
-                                        uint8_t* __restrict__ mono) {
+template <uint32_t T>
+__device__ inline void solve_chain_wave_t(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
+                                          uint8_t* __restrict__ mono) {
     const int lane = threadIdx.x & 63;
     const int depth = lane ? 31 - __clz(lane) : 0;
     bool on[kLevels - 1], go_right[kLevels - 1];
@@ -284,7 +303,7 @@ __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, 
     for (int k = 0; k < d.n; ++k) {
         const float c = line;
         const float t_min = tprev;
-        const float vt0 = eval_axis(true, t_min, a.c[0], a.c[1], a.c[2], a.c[3]);
+        const float vt0 = eval_axis(T, t_min, a.c[0], a.c[1], a.c[2], a.c[3]);
         float ts = t_min;
         if (vt0 != c) {
             const uint32_t s0 = fbits(vt0 - c) & 0x80000000u;
@@ -300,7 +319,7 @@ __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, 
                     lhi = (on[lev] && !go_right[lev]) ? mid : lhi;
                 }
                 const float mym = (llo + lhi) * 0.5f;
-                const float myv = eval_axis(true, mym, a.c[0], a.c[1], a.c[2], a.c[3]);
+                const float myv = eval_axis(T, mym, a.c[0], a.c[1], a.c[2], a.c[3]);
                 // lane 0 is no node: its bit is never read
                 const bool right = (fbits(myv - c) & 0x80000000u) == s0;
                 // the interval this node leaves for the next round
@@ -326,8 +345,18 @@ __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, 
     if (lane == 0) *mono = (up ? 1 : 0) | (down ? 2 : 0);
 }
 
+// Wave-solved chains are never lines (line chains take the lane classes).
+__device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
+                                        uint8_t* __restrict__ mono) {
+    const uint32_t type = (uint32_t)d.type;
+    if (type == kCubic) solve_chain_wave_t<kCubic>(d, a, out, mono);
+    else if (type == kQuad) solve_chain_wave_t<kQuad>(d, a, out, mono);
+    else solve_chain_wave_t<kArc>(d, a, out, mono);
+}
+
 __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict__ tpos,
                                                         const uint32_t* __restrict__ cpos,
+                                                        const float* __restrict__ cweight,
                                                         const ChainDesc* __restrict__ chains,
                                                         const int32_t* __restrict__ order,
                                                         const int32_t* __restrict__ class_off,
@@ -351,7 +380,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
             __builtin_amdgcn_s_setprio(3);
             const int32_t ci = order[item];
             const ChainDesc d = chains[ci];
-            const CurveAxis a = load_axis(tpos, cpos, d);
+            const CurveAxis a = load_axis(tpos, cpos, cweight, d);
             solve_chain_wave(d, a, chain_t + d.out, mono + ci);
             __builtin_amdgcn_s_setprio(0);
         } else {
@@ -359,14 +388,14 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
             const int32_t k = k0 + lane;
             // issue priority follows the cost of the wave's chains (the longest run first and
             // must not be starved by the short ones)
-            const int oct = class_cost_octave(chain_class(chains[order[k0]].n, chains[order[k0]].cubic != 0));
+            const int oct = class_cost_octave(chain_class(chains[order[k0]].n, chains[order[k0]].type != (int16_t)kLine));
             if (oct >= 8) __builtin_amdgcn_s_setprio(3);
             else if (oct >= 6) __builtin_amdgcn_s_setprio(2);
             else if (oct >= 4) __builtin_amdgcn_s_setprio(1);
             if (k < lane1) {
                 const int32_t ci = order[k];
                 const ChainDesc d = chains[ci];
-                const CurveAxis a = load_axis(tpos, cpos, d);
+                const CurveAxis a = load_axis(tpos, cpos, cweight, d);
                 solve_chain_lane(d, a, chain_t + d.out, mono + ci);
             }
             __builtin_amdgcn_s_setprio(0);
@@ -521,7 +550,7 @@ __device__ inline PieceMeta gather_meta(const PieceDesc* __restrict__ pieces, in
     m.path = A.cpath[c];
     m.w = A.w; m.h = A.h;
     if (A.pcanvas) { const int2 cw = A.pcanvas[m.path]; m.w = cw.x; m.h = cw.y; }
-    m.cv = load_curve(A.tpos, A.cpos[c], A.ctype[c]);
+    m.cv = load_curve(A.tpos, A.cpos[c], A.ctype[c], A.cweight, c);
     m.bb = (uint32_t)A.bucket_base[m.path];
     m.pr = A.prow[m.path];
     return m;
@@ -541,7 +570,7 @@ __device__ inline PieceMeta meta_of_lane(const PieceMeta& m, int k) {
     r.pr = make_int2(rl_i(m.pr.x, k), rl_i(m.pr.y, k));
 #pragma unroll
     for (int i = 0; i < 4; ++i) { r.cv.x[i] = rl_f(m.cv.x[i], k); r.cv.y[i] = rl_f(m.cv.y[i], k); }
-    r.cv.cubic = rl_i(m.cv.cubic ? 1 : 0, k) != 0;
+    r.cv.type = (uint32_t)rl_i((int32_t)m.cv.type, k);
     return r;
 }
 
@@ -642,7 +671,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         CurvePts cv;
 #pragma unroll
         for (int k = 0; k < 4; ++k) cv.x[k] = cv.y[k] = 0.0f;
-        cv.cubic = false;
+        cv.type = kLine;
         // chain values: when the wave's pieces lie in one window of the LDS size (consecutive
         // pieces are neighbours), the wave loads the window coalesced and value i of a piece is
         // at (base + 1 - lo) + i; otherwise each lane loads its own at i * 64 + lane
@@ -661,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
             const uint32_t c = (uint32_t)pd.curve;
             path = A.cpath[c];
             if (A.pcanvas) { const int2 cw = A.pcanvas[path]; w = cw.x; h = cw.y; }
-            cv = load_curve(A.tpos, A.cpos[c], A.ctype[c]);
+            cv = load_curve(A.tpos, A.cpos[c], A.ctype[c], A.cweight, c);
             bb = (uint32_t)A.bucket_base[path];
             pr = A.prow[path];
         }
@@ -855,7 +884,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_heavy(const PieceDesc* __restr
 }  // namespace
 
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
-                             const uint32_t* cpath, const int32_t* visible, const float4* cuts,
+                             const float* cweight, const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
                              ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
                              int n_cu, hipStream_t s, const Budget* budget, const int2* pcanvas) {
@@ -863,7 +892,7 @@ hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uin
     unsigned g = grid_for((int64_t)C);
     if (g > (unsigned)n_cu * 8) g = (unsigned)n_cu * 8;
     const Budget b = budget ? *budget : Budget{0, nullptr, nullptr};
-    k_chain_prep<<<g, kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, cuts, ncuts, offsets, C, w, h, chains,
+    k_chain_prep<<<g, kBlock, 0, s>>>(tpos, cpos, ctype, cweight, cpath, visible, cuts, ncuts, offsets, C, w, h, chains,
                                       pieces, chain_cls, class_count, b, budget != nullptr, pcanvas);
     return hipGetLastError();
 }
@@ -879,13 +908,14 @@ hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_
     return hipGetLastError();
 }
 
-hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
+hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const float* cweight,
+                              const ChainDesc* chains,
                               const int32_t* order, const int32_t* class_off, const int32_t* n_long,
                               const int32_t* light_begin, int heavy, int32_t* work, float* chain_t, uint8_t* mono,
                               int n_cu, hipStream_t s) {
     // the heavy launch holds a few hundred long items: one wave per SIMD at most
     const unsigned grid = heavy ? (unsigned)n_cu : (unsigned)(n_cu * 8);
-    k_chain_solve<<<grid, kBlock, 0, s>>>(tpos, cpos, chains, order, class_off, n_long, light_begin, heavy, work,
+    k_chain_solve<<<grid, kBlock, 0, s>>>(tpos, cpos, cweight, chains, order, class_off, n_long, light_begin, heavy, work,
                                           chain_t, mono);
     return hipGetLastError();
 }

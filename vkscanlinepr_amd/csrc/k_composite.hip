@@ -18,6 +18,12 @@
 // thread then folds the set bits of its pixels in path order (overwrite: the last one wins;
 // src-over: d = s*a + d*(1-a)).  A record costs two LDS operations whatever its width, and each
 // image row is written once.
+//
+// Gradient paints (no reference counterpart: the reference draws solid colours only): a path
+// with a paint takes, at each covered pixel, the paint's colour at that pixel's centre, packed
+// like a solid fill, so the two rows of a cell row get their own colours; the variant that
+// evaluates them keeps a second row of pixels per thread.
+#include "device_common.hpp"
 #include "kernels.hpp"
 
 namespace spr {
@@ -60,7 +66,103 @@ __device__ inline u64 block_exclusive_xor(u64 v, u64* lds) {
     return off ^ inc ^ v;
 }
 
-template <int COMPOSE>
+// A gradient's colour at the centre of canvas pixel (x, y), packed like a solid fill (channel
+// times 255 truncated, alpha times opacity, 0 when the alpha byte is 0).  Every float operation
+// is written out in order (the library is built without contraction).
+__device__ __forceinline__ uint32_t paint_color(const PaintArgs& pa, uint32_t pi, int x, int y) {
+    const float* h = pa.inv;
+    const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+    const float u = (h[0] * px + h[1] * py) + h[2];
+    const float v = (h[3] * px + h[4] * py) + h[5];
+    const float q = (h[6] * px + h[7] * py) + h[8];
+    const float sx = u / q, sy = v / q;
+    const Paint& g = pa.paints[pi];
+    const float gx = (g.xform[0] * sx + g.xform[1] * sy) + g.xform[2];
+    const float gy = (g.xform[3] * sx + g.xform[4] * sy) + g.xform[5];
+    float t;
+    bool degenerate = false;
+    if (g.kind == kPaintLinear) {
+        const float dx = g.geom[2] - g.geom[0], dy = g.geom[3] - g.geom[1];
+        const float den = dx * dx + dy * dy;
+        if (den > 0.0f) {
+            t = ((gx - g.geom[0]) * dx + (gy - g.geom[1]) * dy) / den;
+        } else {
+            t = 1.0f;  // both points equal: the last stop's colour
+            degenerate = true;
+        }
+    } else {
+        // |g - f - t (c - f)| = t r: (d.d - r^2) t^2 - 2 (q.d) t + q.q = 0, a < 0, largest root
+        const float qx = gx - g.geom[2], qy = gy - g.geom[3];
+        const float dx = g.geom[0] - g.geom[2], dy = g.geom[1] - g.geom[3];
+        const float r = g.geom[4];
+        const float a = (dx * dx + dy * dy) - r * r;
+        const float b = qx * dx + qy * dy;
+        const float c = qx * qx + qy * qy;
+        t = (b - sqrtf(b * b - a * c)) / a;
+    }
+    if (!(t == t)) t = 0.0f;
+    if (!degenerate) {
+        if (g.spread == kSpreadPad) {
+            t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+        } else if (g.spread == kSpreadRepeat) {
+            t = t - floorf(t);
+        } else if (g.spread == kSpreadReflect) {
+            const float m = t - 2.0f * floorf(t * 0.5f);
+            t = m > 1.0f ? 2.0f - m : m;
+        } else if (t < 0.0f || t > 1.0f) {
+            return 0u;  // transparent outside [0, 1]
+        }
+        if (!(t == t)) t = 0.0f;
+    }
+    const float* st = pa.stops + 5 * (size_t)g.first_stop;
+    const uint32_t n = g.n_stops;
+    float c[4];
+    if (t <= st[0]) {
+        c[0] = st[1]; c[1] = st[2]; c[2] = st[3]; c[3] = st[4];
+    } else if (t >= st[5 * (n - 1)]) {
+        const float* e = st + 5 * (n - 1);
+        c[0] = e[1]; c[1] = e[2]; c[2] = e[3]; c[3] = e[4];
+    } else {
+        uint32_t i = 0;
+        while (i + 1 < n && st[5 * (i + 1)] <= t) ++i;  // off[i] <= t < off[i + 1]
+        const float* e0 = st + 5 * i;
+        const float* e1 = e0 + 5;
+        const float f = (t - e0[0]) / (e1[0] - e0[0]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c[k] = lerp(e0[1 + k], e1[1 + k], f);
+    }
+    c[3] = c[3] * g.opacity;
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float vv = c[k] * 255.0f;
+        const int iv = (vv != vv) ? 0 : (vv <= 0.0f ? 0 : (vv >= 255.0f ? 255 : (int)vv));
+        out |= (uint32_t)iv << (8 * k);
+    }
+    return (out & 0xFF000000u) ? out : 0u;
+}
+
+// The colours of pixels (x, y) and (x, y + 1), the two rows of a cell row.
+__device__ __noinline__ uint2 paint_pair(const PaintArgs pa, uint32_t pi, int x, int y) {
+    return make_uint2(paint_color(pa, pi, x, y), paint_color(pa, pi, x, y + 1));
+}
+
+// (r*a, g*a, b*a, a) of a packed colour, as the blend uses it.
+__device__ inline float4 premul(uint32_t col) {
+    const float a = (float)((col >> 24) & 0xFFu) / 255.0f;
+    return make_float4(((float)(col & 0xFFu) / 255.0f) * a, ((float)((col >> 8) & 0xFFu) / 255.0f) * a,
+                       ((float)((col >> 16) & 0xFFu) / 255.0f) * a, a);
+}
+
+__device__ inline void blend(float* d, const float4 sa) {
+    const float ia = 1.0f - sa.w;
+    d[0] = sa.x + d[0] * ia;
+    d[1] = sa.y + d[1] * ia;
+    d[2] = sa.z + d[2] * ia;
+    d[3] = sa.w + d[3] * ia;
+}
+
+template <int COMPOSE, int GRAD>
 __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict__ records,
                                                       const int2* __restrict__ brec,
                                                       const int32_t* __restrict__ bucket_base,
@@ -68,10 +170,13 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ fill, uint32_t Np,
                                                       int w, int h, int64_t rec_cap,
                                                       uint8_t* __restrict__ image,
-                                                      const DocDev* __restrict__ docs, int n_docs) {
+                                                      const DocDev* __restrict__ docs, int n_docs,
+                                                      const PaintArgs pa) {
     __shared__ u64 ev[kChunk];
     __shared__ int2 list[kList];
     __shared__ uint32_t lcol[kList];
+    __shared__ uint32_t lpaint[GRAD ? kList : 1];  // each listed path's paint (kSolidPaint: its colour)
+    __shared__ u64 lgrad;                          // the batch's paths that have a paint
     __shared__ float4 lsa[kBatch];   // (r*a, g*a, b*a, a) of each path of the batch, as the blend uses them
     __shared__ int32_t wcount[kBlock / 64];
     __shared__ int32_t bpre[kBatch + 1];
@@ -96,12 +201,14 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
     const int xb = tid * kPix;  // first column of this thread inside the chunk
     for (int c0 = 0; c0 < w; c0 += kChunk) {
         const int cw = w - c0 < kChunk ? w - c0 : kChunk;
-        uint32_t pix[kPix];
-        float acc[COMPOSE ? kPix : 1][4];
+        uint32_t pix[kPix], pix1[GRAD ? kPix : 1];  // rows y and, with paints, y + 1
+        float acc[COMPOSE ? kPix : 1][4], acc1[(COMPOSE && GRAD) ? kPix : 1][4];
 #pragma unroll
         for (int j = 0; j < kPix; ++j) {
             pix[j] = 0xFFFFFFFFu;
+            if constexpr (GRAD != 0) pix1[j] = 0xFFFFFFFFu;
             if constexpr (COMPOSE != 0) { acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 1.0f; }
+            if constexpr (COMPOSE != 0 && GRAD != 0) { acc1[j][0] = acc1[j][1] = acc1[j][2] = acc1[j][3] = 1.0f; }
         }
         uint32_t p0 = pbeg;
         while (p0 < pend) {
@@ -134,6 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                     const int slot = off + (int)__popcll(m & ((1ull << lane) - 1ull));
                     list[slot] = rng;
                     lcol[slot] = fill[p];
+                    if constexpr (GRAD != 0) lpaint[slot] = pa.path_paint[p];
                 }
                 __syncthreads();
                 n += add;
@@ -146,13 +254,11 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 for (int j = 0; j < kPix; ++j) ev[j * kBlock + tid] = 0ull;
                 if (wave == 0) {
                     if constexpr (COMPOSE != 0) {
-                        if (lane < qn) {
-                            const uint32_t col = lcol[q0 + lane];
-                            const float a = (float)((col >> 24) & 0xFFu) / 255.0f;
-                            lsa[lane] = make_float4(((float)(col & 0xFFu) / 255.0f) * a,
-                                                    ((float)((col >> 8) & 0xFFu) / 255.0f) * a,
-                                                    ((float)((col >> 16) & 0xFFu) / 255.0f) * a, a);
-                        }
+                        if (lane < qn) lsa[lane] = premul(lcol[q0 + lane]);
+                    }
+                    if constexpr (GRAD != 0) {
+                        const u64 gm = __ballot(lane < qn && lpaint[q0 + lane] != kSolidPaint);
+                        if (lane == 0) lgrad = gm;
                     }
                     const int2 rg = lane < qn ? list[q0 + lane] : make_int2(0, 0);
                     int inc = rg.y - rg.x;
@@ -203,13 +309,23 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
 #pragma unroll
                 for (int j = 0; j < kPix; ++j) run ^= ev[j * kBlock + tid];
                 u64 cov = block_exclusive_xor(run, xlds);
+                const u64 gmask = GRAD ? lgrad : 0ull;
 #pragma unroll
                 for (int j = 0; j < kPix; ++j) {
                     cov ^= ev[j * kBlock + tid];
                     u64 mm = cov;
                     if (mm == 0ull) continue;
+                    const int x = c0 + xb + j;  // the pixel's canvas column
                     if constexpr (COMPOSE == 0) {
-                        pix[j] = lcol[q0 + 63 - __clzll((long long)mm)];
+                        const int q = 63 - __clzll((long long)mm);
+                        if (GRAD && ((gmask >> q) & 1ull)) {
+                            const uint2 pc = paint_pair(pa, lpaint[q0 + q], x, y);
+                            pix[j] = pc.x;
+                            pix1[j] = pc.y;
+                        } else {
+                            pix[j] = lcol[q0 + q];
+                            if constexpr (GRAD != 0) pix1[j] = pix[j];
+                        }
                     } else {
                         // src-over, d = s*a + d*(1-a) per channel, the paths in order
                         // the mask's two words in turn: 32-bit bit scans and clears
@@ -220,12 +336,18 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                             while (m32) {
                                 const int q = __builtin_ctz(m32);
                                 m32 &= m32 - 1u;
-                                const float4 sa = L[q];
-                                const float ia = 1.0f - sa.w;
-                                acc[j][0] = sa.x + acc[j][0] * ia;
-                                acc[j][1] = sa.y + acc[j][1] * ia;
-                                acc[j][2] = sa.z + acc[j][2] * ia;
-                                acc[j][3] = sa.w + acc[j][3] * ia;
+                                if constexpr (GRAD != 0) {
+                                    if ((gmask >> (32 * hw + q)) & 1ull) {
+                                        const uint2 pc = paint_pair(pa, lpaint[q0 + 32 * hw + q], x, y);
+                                        blend(acc[j], premul(pc.x));
+                                        blend(acc1[j], premul(pc.y));
+                                    } else {
+                                        blend(acc[j], L[q]);
+                                        blend(acc1[j], L[q]);
+                                    }
+                                } else {
+                                    blend(acc[j], L[q]);
+                                }
                             }
                         }
                     }
@@ -233,30 +355,40 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 __syncthreads();
             }
         }
+        auto round8 = [](const float* a) -> uint32_t {
+            uint32_t o = 0;
+#pragma unroll
+            for (int ch = 0; ch < 4; ++ch) {
+                float q = a[ch] * 255.0f + 0.5f;
+                q = q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q);
+                o |= (uint32_t)q << (8 * ch);
+            }
+            return o;
+        };
         if constexpr (COMPOSE != 0) {
 #pragma unroll
             for (int j = 0; j < kPix; ++j) {
-                uint32_t o = 0;
-#pragma unroll
-                for (int ch = 0; ch < 4; ++ch) {
-                    float q = acc[j][ch] * 255.0f + 0.5f;
-                    q = q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q);
-                    o |= (uint32_t)q << (8 * ch);
-                }
-                pix[j] = o;
+                pix[j] = round8(acc[j]);
+                if constexpr (GRAD != 0) pix1[j] = round8(acc1[j]);
             }
         }
         for (int yy = y; yy < y + 2 && yy < h; ++yy) {
             uint32_t* row = reinterpret_cast<uint32_t*>(image) + (int64_t)(h - 1 - yy) * w + c0;
+            uint32_t out[kPix];
+#pragma unroll
+            for (int j = 0; j < kPix; ++j) {
+                if constexpr (GRAD != 0) out[j] = yy == y ? pix[j] : pix1[j];
+                else out[j] = pix[j];
+            }
             if (xb + kPix <= cw && (((uintptr_t)(row + xb)) & 15u) == 0) {
                 uint4* q = reinterpret_cast<uint4*>(row + xb);
 #pragma unroll
                 for (int k = 0; k < kPix / 4; ++k)
-                    q[k] = make_uint4(pix[4 * k], pix[4 * k + 1], pix[4 * k + 2], pix[4 * k + 3]);
+                    q[k] = make_uint4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
             } else {
 #pragma unroll
                 for (int j = 0; j < kPix; ++j)
-                    if (xb + j < cw) row[xb + j] = pix[j];
+                    if (xb + j < cw) row[xb + j] = out[j];
             }
         }
         __syncthreads();
@@ -268,15 +400,19 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
 hipError_t launch_composite(const uint32_t* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
                             uint8_t* image, hipStream_t s, int64_t rec_cap, const DocDev* docs, int n_docs,
-                            int total_rows) {
+                            int total_rows, const PaintArgs* paint) {
     const int rows = docs ? total_rows : (h + 1) / 2;
     if (rows == 0 || (!docs && w == 0)) return hipSuccess;
-    if (compose)
-        k_composite<1><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image,
-                                               docs, n_docs);
-    else
-        k_composite<0><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image,
-                                               docs, n_docs);
+    const PaintArgs pa = paint ? *paint : PaintArgs{nullptr, nullptr, nullptr, nullptr};
+#define SPR_COMPOSITE(C, G)                                                                                 \
+    k_composite<C, G><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image, \
+                                              docs, n_docs, pa)
+    if (compose) {
+        if (paint) SPR_COMPOSITE(1, 1); else SPR_COMPOSITE(1, 0);
+    } else {
+        if (paint) SPR_COMPOSITE(0, 1); else SPR_COMPOSITE(0, 0);
+    }
+#undef SPR_COMPOSITE
     return hipGetLastError();
 }
 

@@ -103,6 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_transform(const float2* __restrict__
 __global__ __launch_bounds__(kBlock) void k_count(const float2* __restrict__ tpos,
                                                   const uint32_t* __restrict__ cpos,
                                                   const uint32_t* __restrict__ ctype,
+                                                  const float* __restrict__ cweight,
                                                   const uint32_t* __restrict__ cpath,
                                                   const int32_t* __restrict__ visible, uint32_t C,
                                                   int w, int h, float4* __restrict__ cuts,
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const float2* __restrict__ tpo
     if (c >= C) return;
     const uint32_t path = cpath[c];
     if (pcanvas) { w = pcanvas[path].x; h = pcanvas[path].y; }  // a batch: the path's document canvas
-    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c]);
+    const CurvePts cv = load_curve(tpos, cpos[c], ctype[c], cweight, c);
     const bool vis = visible_mask(visible[path]);
     float q[4];
     const uint32_t n = monotone_cuts(cv, vis, q);
@@ -163,7 +164,8 @@ __global__ void k_segment_stats(const int32_t* __restrict__ seg, uint32_t Np, in
 // curve, and its winding change across the cell's centre row.  Returns the entry's bucket.
 __device__ inline uint32_t gen_one(const int2* __restrict__ inter, int64_t N, int64_t i,
                                    const uint32_t* __restrict__ cpos, const uint32_t* __restrict__ ctype,
-                                   const uint32_t* __restrict__ cpath, const float2* __restrict__ tpos, int w,
+                                   const float* __restrict__ cweight, const uint32_t* __restrict__ cpath,
+                                   const float2* __restrict__ tpos, int w,
                                    int h, int32_t* __restrict__ key, int8_t* __restrict__ wn,
                                    uint32_t* __restrict__ epath, const int32_t* __restrict__ bucket_base,
                                    const int2* __restrict__ prow, int32_t* __restrict__ err) {
@@ -176,7 +178,7 @@ __device__ inline uint32_t gen_one(const int2* __restrict__ inter, int64_t N, in
     if (a.x != b.x) t1 = 1.0f;
     uint32_t yx;
     int swn;
-    if (t0 < t1) segment_cell(load_curve(tpos, cpos[c], ctype[c]), t0, t1, w, h, yx, swn);
+    if (t0 < t1) segment_cell(load_curve(tpos, cpos[c], ctype[c], cweight, c), t0, t1, w, h, yx, swn);
     else { yx = kSentinelKey; swn = 0; }
     key[i] = (int32_t)yx;
     wn[i] = (int8_t)swn;
@@ -189,6 +191,7 @@ __device__ inline uint32_t gen_one(const int2* __restrict__ inter, int64_t N, in
 __global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict__ inter, int64_t N,
                                                          const uint32_t* __restrict__ cpos,
                                                          const uint32_t* __restrict__ ctype,
+                                                         const float* __restrict__ cweight,
                                                          const uint32_t* __restrict__ cpath,
                                                          const float2* __restrict__ tpos, int w,
                                                          int h, int32_t* __restrict__ key,
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict_
     if (valid) {
         int cw = w, ch = h;
         if (pcanvas) { const int2 cv = pcanvas[cpath[inter[i].x]]; cw = cv.x; ch = cv.y; }
-        bucket = gen_one(inter, n, i, cpos, ctype, cpath, tpos, cw, ch, key, wn, epath, bucket_base, prow, err);
+        bucket = gen_one(inter, n, i, cpos, ctype, cweight, cpath, tpos, cw, ch, key, wn, epath, bucket_base, prow, err);
     }
     wave_count(hist, bucket, valid);
 }
@@ -252,10 +255,10 @@ hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_
 }
 
 hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
-                        const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
+                        const float* cweight, const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
                         float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s, const int2* pcanvas) {
     if (C == 0) return hipSuccess;
-    k_count<<<grid_for(C), kBlock, 0, s>>>(tpos, cpos, ctype, cpath, visible, C, w, h, cuts, ncuts, counts,
+    k_count<<<grid_for(C), kBlock, 0, s>>>(tpos, cpos, ctype, cweight, cpath, visible, C, w, h, cuts, ncuts, counts,
                                            pcanvas);
     return hipGetLastError();
 }
@@ -274,12 +277,12 @@ hipError_t launch_segment_stats(const int32_t* seg, uint32_t Np, int32_t* max_le
 }
 
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
-                               const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
+                               const uint32_t* ctype, const float* cweight, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
                                const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
                                hipStream_t s, const int32_t* n_dev, const int2* pcanvas) {
     if (N == 0) return hipSuccess;
-    k_gen_fragment<<<grid_for(N), kBlock, 0, s>>>(inter, N, cpos, ctype, cpath, tpos, w, h, key, wn, epath,
+    k_gen_fragment<<<grid_for(N), kBlock, 0, s>>>(inter, N, cpos, ctype, cweight, cpath, tpos, w, h, key, wn, epath,
                                                   bucket_base, prow, hist, err, n_dev, pcanvas);
     return hipGetLastError();
 }

@@ -75,12 +75,12 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
     float n_cub = 0, xmin = 3.4e38f, xmax = -3.4e38f, ymin = 3.4e38f, ymax = -3.4e38f;
     float plen = 0, sdx = 0, sdy = 0, cdx = 0, cdy = 0, turn = 0, inside = 0, npts = 0;
     for (uint32_t c = c0 + lane; c < c1; c += 64) {
-        const uint32_t k = ctype[c] == 4u ? 4u : 2u;
+        const uint32_t k = ctype[c] & 7u;  // points: line 2, quadratic and arc 3, cubic 4
         const uint32_t b = cpos[c];
         float2 q[4];  // static indices only (the line's points repeat its end point)
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) q[i] = tpos[b + (i < k ? i : k - 1)];
-        n_cub += k == 4u ? 1.0f : 0.0f;
+        n_cub += k > 2u ? 1.0f : 0.0f;
         float pdx = 0, pdy = 0;
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
                 pdx = dx; pdy = dy;
             }
         }
-        const float2 qe = k == 4u ? q[3] : q[1];
+        const float2 qe = k == 4u ? q[3] : (k == 3u ? q[2] : q[1]);
         cdx += fabsf(qe.x - q[0].x);
         cdy += fabsf(qe.y - q[0].y);
     }
@@ -126,9 +126,10 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
         f[13] = npts > 0 ? inside / npts : 0.0f;
         f[14] = (float)rule[p];
         f[15] = log2f(1.0f + 0.25f * cw * ch);
-        // Upper bound of the path's intersections.  A Bezier curve varies no more than its
-        // control polygon; its <= 5 pieces cover [0, 1] at most three times over (the 4-cut
-        // order quirk runs one piece backwards); a piece from a to b gives at most
+        // Upper bound of the path's intersections.  A Bezier curve (also a rational one with
+        // positive weights: x(t) = c where the Bernstein coefficients w_i (x_i - c) change sign)
+        // varies no more than its control polygon; its <= 5 pieces cover [0, 1] at most three
+        // times over (the 4-cut order quirk runs one piece backwards); a piece from a to b gives at most
         // 1 + (|dx|/2 + 1) + (|dy|/2 + 1) entries.  The float terms carry a 0.1% margin.
         double b = 0.0;
         if (visible_mask(visible[p])) b = 1.5 * ((double)sdx + (double)sdy) * 1.001 + 20.0 * (double)(c1 - c0) + 2.0;

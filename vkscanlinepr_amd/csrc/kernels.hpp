@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "paint.hpp"
+
 namespace spr {
 
 struct Mvp { float m[16]; };
@@ -42,7 +44,7 @@ hipError_t launch_transform(const float2* pos, const uint32_t* pos_path, uint32_
                             int w, int h, float2* tpos, int32_t* visible, int32_t* row_lo, int32_t* row_hi,
                             hipStream_t s, const int2* pcanvas = nullptr, const float* mvp_dev = nullptr);
 hipError_t launch_count(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
-                        const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
+                        const float* cweight, const uint32_t* cpath, const int32_t* visible, uint32_t C, int w, int h,
                         float4* cuts, uint32_t* ncuts, int32_t* counts, hipStream_t s,
                         const int2* pcanvas = nullptr);
 hipError_t launch_path_segments(const uint32_t* cpath, uint32_t C, const int32_t* offsets,
@@ -51,7 +53,7 @@ hipError_t launch_segment_stats(const int32_t* seg, uint32_t Np, int32_t* max_le
 // One (curve, piece, axis) chain of dependent crossing solves, and one monotone piece.
 struct ChainDesc {
     int32_t curve, n, out;  // curve, number of crossings, first output slot
-    int16_t side, cubic;
+    int16_t side, type;     // type: the curve's (kLine, kQuad, kCubic, kArc)
     float line0, step;      // first grid line and its step (+-2)
     float t0, t1;           // flagged piece start and end parameters
 };
@@ -64,6 +66,7 @@ struct PieceDesc {
 struct FragArgs {
     const uint32_t* cpos;
     const uint32_t* ctype;
+    const float* cweight;       // arc weights (read for arcs only; null when there is none)
     const uint32_t* cpath;
     const float2* tpos;
     int w, h;
@@ -99,7 +102,7 @@ struct Budget {
     int32_t* over;
 };
 hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
-                             const uint32_t* cpath, const int32_t* visible, const float4* cuts,
+                             const float* cweight, const uint32_t* cpath, const int32_t* visible, const float4* cuts,
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
                              ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
                              int n_cu, hipStream_t s, const Budget* budget = nullptr, const int2* pcanvas = nullptr);
@@ -108,7 +111,8 @@ hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_
                               int32_t* order, int n_cu, hipStream_t s);
 // heavy = 1: the wave-solved chains and the lane-solved chains of kHeavyCrossings and more;
 // heavy = 0: the rest.  The two launches are independent.
-hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const ChainDesc* chains,
+hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const float* cweight,
+                              const ChainDesc* chains,
                               const int32_t* order, const int32_t* class_off, const int32_t* n_long,
                               const int32_t* light_begin, int heavy, int32_t* work, float* chain_t, uint8_t* mono,
                               int n_cu, hipStream_t s);
@@ -135,7 +139,7 @@ hipError_t launch_piece_heavy(const PieceDesc* pieces, const PieceLists& L, cons
                               const float* chain_t, uint32_t* scratch, int64_t n_entries, const FragArgs& A, int n_cu,
                               hipStream_t s);
 hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpos,
-                               const uint32_t* ctype, const uint32_t* cpath, const float2* tpos,
+                               const uint32_t* ctype, const float* cweight, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
                                const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
                                hipStream_t s, const int32_t* n_dev = nullptr, const int2* pcanvas = nullptr);
@@ -301,10 +305,18 @@ hipError_t launch_doc_winding_base(const int32_t* winding, const int32_t* sstart
 hipError_t launch_draw_records(const int4* records, int32_t R, int w, int h, int compose, uint8_t* image,
                                hipStream_t s);
 
-// ---- composite (a12)
+// ---- gradient paints (paint.hpp); stops are read as 5 floats (offset, r, g, b, a).
+struct PaintArgs {
+    const uint32_t* path_paint;  // per path: kSolidPaint or an index into paints
+    const Paint* paints;
+    const float* stops;
+    const float* inv;            // 9 floats: inverse of the MVP's 2D projective part, rows
+};
+
+// ---- composite (a12).  paint (optional): per-pixel gradient colours for the paths that have one.
 hipError_t launch_composite(const uint32_t* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
                             uint8_t* image, hipStream_t s, int64_t rec_cap, const DocDev* docs = nullptr,
-                            int n_docs = 0, int total_rows = 0);
+                            int n_docs = 0, int total_rows = 0, const PaintArgs* paint = nullptr);
 
 }  // namespace spr

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -22,6 +23,9 @@
 #include "kernels.hpp"
 #include "learned.hpp"
 #include "scene.hpp"
+
+static_assert(sizeof(spr_paint) == sizeof(spr::Paint) && sizeof(spr_stop) == sizeof(spr::Stop),
+              "spr_paint and spr_stop share the layout of the internal paint records");
 
 struct spr_scene {
     spr::Scene s;
@@ -84,6 +88,25 @@ struct DBuf {
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// The MVP rows and their inverse, copied to the device as one block.
+constexpr size_t kMvpBytes = 25 * sizeof(float);
+
+// Inverse of the MVP's 2D projective part (a point (x, y, 0, 1) goes to rows 0, 1 and 3:
+// [[m0, m1, m3], [m4, m5, m7], [m12, m13, m15]]), by cofactors in double, rounded to float;
+// zeros when it is singular.  The composite maps pixel centres back to the scene with it.
+void mvp_inverse(const float m[16], float out[9]) {
+    const double a = m[0], b = m[1], c = m[3], d = m[4], e = m[5], f = m[7], g = m[12], h = m[13], k = m[15];
+    const double A = e * k - f * h, B = -(d * k - f * g), Cc = d * h - e * g;
+    const double det = a * A + b * B + c * Cc;
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
+        for (int i = 0; i < 9; ++i) out[i] = 0.0f;
+        return;
+    }
+    const double inv[9] = {A, -(b * k - c * h), b * f - c * e, B, a * k - c * g, -(a * f - c * d), Cc,
+                           -(a * h - b * g), a * e - b * d};
+    for (int i = 0; i < 9; ++i) out[i] = (float)(inv[i] / det);
+}
+
 // Frame summary slots (int32), gathered on the device at the end of every frame and read back
 // with one copy: record totals, the error flags, the kernel routes taken, and the learned frame's
 // allocation counters.
@@ -117,6 +140,13 @@ struct spr_handle {
     DBuf<uint32_t> pos_path, cpos, ctype, cpath, rule, fill;
     std::vector<uint32_t> h_cpath;
     spr::Mvp mvp{};
+    DBuf<float> cweight;            // arc weights, uploaded when the scene has an arc
+    bool has_arcs = false;
+    const float* arcw() const { return has_arcs ? cweight.p : nullptr; }
+    DBuf<uint32_t> path_paint;      // gradient paints, uploaded when a path has one
+    DBuf<spr::Paint> paints;
+    DBuf<float> stops;              // 5 floats per stop
+    bool has_paint = false;
 
     // frame buffers
     DBuf<float2> tpos;
@@ -145,13 +175,16 @@ struct spr_handle {
     DBuf<int2> prow;
     DBuf<uint8_t> image;
     DBuf<unsigned long long> zero;  // per-frame zeroed control region
-    int32_t* h_pinned = nullptr;    // 64 ints of pinned host memory: [0, 16) mid-frame readbacks,
-                                    // [16, 16 + kSummaryMax) the last frame's summary
+    int32_t* h_pinned = nullptr;    // 128 words of pinned host memory: [0, 16) mid-frame readbacks,
+                                    // [16, 16 + kSummaryMax) the last frame's summary, [48, 73) the
+                                    // MVP rows and the inverse of its 2D projective part
     DBuf<int32_t> summary;          // the frame summary on the device, copied to h_pinned + 16
     DBuf<unsigned long long> scan_ws;  // spr_scan_i32: tile statuses and the tile counter
     int32_t sum(int k) const { return h_pinned[16 + k]; }
     float* mvp_pinned() const { return reinterpret_cast<float*>(h_pinned + 48); }  // [48, 64): the MVP rows
-    DBuf<float> mvp_dev;            // the MVP of frames without a host synchronisation, copied from mvp_pinned
+    float* inv_pinned() const { return reinterpret_cast<float*>(h_pinned + 64); }  // [64, 73): its inverse
+    DBuf<float> mvp_dev;            // [0, 16) the MVP, [16, 25) its inverse: copied from mvp_pinned in
+                                    // frames without a host synchronisation and for paints
 
     // injection
     std::vector<int32_t> inj_inter;
@@ -203,6 +236,17 @@ struct spr_handle {
     spr_stats stats{};
     bool frame_done = false;
 };
+
+namespace {
+
+// The composite's paint arguments (null without paints); the inverse MVP is read on the device.
+const spr::PaintArgs* paint_args(const spr_handle* H, spr::PaintArgs& pa) {
+    if (!H->has_paint) return nullptr;
+    pa = spr::PaintArgs{H->path_paint.p, H->paints.p, H->stops.p, H->mvp_dev.p + 16};
+    return &pa;
+}
+
+}  // namespace
 
 namespace {
 
@@ -285,10 +329,10 @@ void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragA
     const int32_t* d_lb = d_cc + spr::kCcLightBegin;
     const spr::PieceLists L{H->lo_list.p, H->lo_cnt.p, H->heavy_list.p, d_cc + spr::kCcHeavyPieces};
     SPR_HIP(hipEventRecord(H->ev_fork, s));
-    SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 1,
+    SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->arcw(), H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 1,
                                     d_cc + spr::kCcWork, H->chain_t.p, H->mono.p, n_cu_solve, s));
     SPR_HIP(hipStreamWaitEvent(s2, H->ev_fork, 0));
-    SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 0,
+    SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->arcw(), H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 0,
                                     d_cc + spr::kCcWorkLight, H->chain_t.p, H->mono.p, n_cu_solve, s2));
     if (s2 == s) stage_mark(H, 5);  // serial: both solves, then every piece kernel
     SPR_HIP(hipEventRecord(H->ev_light, s2));
@@ -334,10 +378,11 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->visible.p, Np, 0u, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_lo.p, Np, 0x7FFFFFFFu, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_hi.p, Np, 0x80000000u, s));
+    if (H->has_paint) SPR_HIP(hipMemcpyAsync(H->mvp_dev.p, H->mvp_pinned(), kMvpBytes, hipMemcpyHostToDevice, s));
     SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p,
                                   H->row_lo.p, H->row_hi.p, s));
     stage_mark(H, 1);
-    SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, C, w, h,
+    SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->visible.p, C, w, h,
                               H->cuts.p, H->ncuts.p, H->counts.p, s));
     stage_mark(H, 2);
     SPR_HIP(spr::launch_scan_i32(H->counts.p, H->counts.p, C, ctl_at(H, za.st_count, za.ctr_a, 0), s));
@@ -421,7 +466,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         H->lo_cnt.ensure(((size_t)C * 5 + 63) / 64);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
-        SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
+        SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->visible.p, H->cuts.p,
                                        H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
                                        d_cc, H->n_cu, s));
         SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, d_cc + spr::kCcLightBegin,
@@ -430,7 +475,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     stage_mark(H, 4);
     if (gen) {
         // chain solve, then piece merge and fragment keys in one pass
-        const spr::FragArgs A{H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h, nullptr,
+        const spr::FragArgs A{H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->tpos.p, w, h, nullptr,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err,
                               debug ? H->inter.p : nullptr};
         solve_and_pieces(H, C, d_cc, A, (int64_t)n1, H->n_cu * H->chain_k / 8, H->n_cu * H->merge_k / 8);
@@ -439,7 +484,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     }
     stage_mark(H, 6);
     if (!gen)  // injected entries (or none)
-        SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h,
+        SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->tpos.p, w, h,
                                          H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err, s));
     stage_mark(H, 7);
     // ---- sort: bucket offsets, bands, scatter into band order, band sort in LDS
@@ -490,8 +535,10 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->last_fused = narrow;
     stage_mark(H, 11);
     // ---- composite
+    spr::PaintArgs pa;
     SPR_HIP(spr::launch_composite(H->crec.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
-                                  (int)H->opts.compose, H->image.p, s, (int64_t)(2 * n1)));
+                                  (int)H->opts.compose, H->image.p, s, (int64_t)(2 * n1), nullptr, 0, 0,
+                                  paint_args(H, pa)));
     stage_mark(H, 12);
     spr::SummarySrc S{};
     S.n = kSumCount;
@@ -616,8 +663,7 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_hi.p, Np, 0x80000000u, s));
     // the MVP goes through a copy from pinned host memory, which a captured frame repeats at every
     // replay: spr_set_mvp only rewrites that memory, and the graph follows it
-    H->mvp_dev.ensure(16);
-    SPR_HIP(hipMemcpyAsync(H->mvp_dev.p, H->mvp_pinned(), 64, hipMemcpyHostToDevice, s));
+    SPR_HIP(hipMemcpyAsync(H->mvp_dev.p, H->mvp_pinned(), kMvpBytes, hipMemcpyHostToDevice, s));
     SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p,
                                   H->row_lo.p, H->row_hi.p, s, pcv, H->mvp_dev.p));
     stage_mark(H, 1);
@@ -638,7 +684,7 @@ void run_learned_frame(spr_handle* H) {
     }
     SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, kLcArena), s2));
     if (s2 != s) SPR_HIP(hipEventRecord(H->ev_pjoin, s2));
-    SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, C, w, h, H->cuts.p,
+    SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->visible.p, C, w, h, H->cuts.p,
                               H->ncuts.p, H->counts.p, s, pcv));
     stage_mark(H, 2);
     SPR_HIP(spr::launch_scan_i32(H->counts.p, H->counts.p, C, ctl_at(H, z.st_count, z.counters, kLcCount), s));
@@ -720,7 +766,7 @@ void run_learned_frame(spr_handle* H) {
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         const spr::Budget B{nl, d_live, d_over};
-        SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->cpath.p, H->visible.p, H->cuts.p,
+        SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->visible.p, H->cuts.p,
                                        H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
                                        d_cc, H->n_cu, s, &B, pcv));
         SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, d_cc + spr::kCcLightBegin,
@@ -728,7 +774,7 @@ void run_learned_frame(spr_handle* H) {
     }
     stage_mark(H, 4);
     if (C) {
-        const spr::FragArgs A{H->cpos.p, H->ctype.p, H->cpath.p, H->tpos.p, w, h, pcv,
+        const spr::FragArgs A{H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->tpos.p, w, h, pcv,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, H->hist_l.p, d_err,
                               debug ? H->inter.p : nullptr};
         solve_and_pieces(H, C, d_cc, A, nl, H->n_cu * H->chain_k / 8, H->n_cu * H->merge_k / 8);
@@ -784,9 +830,10 @@ void run_learned_frame(spr_handle* H) {
                                       H->brec_l.p, s, d_nsorted, H->sstart.p, Np, nr, d_err, pcv,
                                       H->batch ? H->wbase.p : nullptr));
     stage_mark(H, 11);
+    spr::PaintArgs pa;
     SPR_HIP(spr::launch_composite(H->crec.p, H->brec_l.p, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
                                   (int)H->opts.compose, H->image.p, s, rec_cap, H->batch ? H->docs.p : nullptr,
-                                  (int)H->n_docs, H->total_rows));
+                                  (int)H->n_docs, H->total_rows, paint_args(H, pa)));
     stage_mark(H, 12);
     // the prediction's paths that overflowed their capacity (statistics only)
     if (!joined) SPR_HIP(hipStreamWaitEvent(s, H->ev_pjoin, 0));
@@ -964,11 +1011,37 @@ void sync_stats(spr_handle* H) {
 
 void drop_graph(spr_handle* H);
 
+// Arc weights and gradient paints of a scene (each optional).
+struct Extras {
+    const float* cweight = nullptr;
+    const uint32_t* path_paint = nullptr;
+    const spr::Paint* paints = nullptr;
+    uint32_t n_paints = 0;
+    const spr::Stop* stops = nullptr;
+    uint32_t n_stops = 0;
+};
+
+Extras extras_of(const spr::Scene& s) {
+    Extras e;
+    e.cweight = s.curve_weight.empty() ? nullptr : s.curve_weight.data();
+    e.path_paint = s.path_paint.empty() ? nullptr : s.path_paint.data();
+    e.paints = s.paints.data(); e.n_paints = (uint32_t)s.paints.size();
+    e.stops = s.stops.data(); e.n_stops = (uint32_t)s.stops.size();
+    return e;
+}
+
 void load_arrays(spr_handle* H, const float* pos, const uint32_t* pos_path, uint32_t P, const uint32_t* cpos,
                  const uint32_t* ctype, const uint32_t* cpath, uint32_t C, const uint32_t* rule,
-                 const uint32_t* fill, uint32_t Np) {
+                 const uint32_t* fill, uint32_t Np, const Extras& ex = Extras{}) {
     std::string bad = spr::validate(pos, pos_path, P, cpos, ctype, cpath, C, rule, Np);
     if (!bad.empty()) throw Fail{SPR_ERR_INVALID, bad};
+    bad = spr::validate_extras(ctype, C, ex.cweight, Np, ex.path_paint, ex.paints, ex.n_paints, ex.stops,
+                               ex.n_stops);
+    if (!bad.empty()) throw Fail{SPR_ERR_INVALID, bad};
+    bool arcs = false, paint = false;
+    for (uint32_t c = 0; c < C && !arcs; ++c) arcs = ctype[c] == 0x13u;  // an arc
+    if (ex.path_paint)
+        for (uint32_t p = 0; p < Np && !paint; ++p) paint = ex.path_paint[p] != spr::kSolidPaint;
     if (Np && !fill) throw Fail{SPR_ERR_INVALID, "null fill_info"};
     if (Np > (1u << 30) - 1) throw Fail{SPR_ERR_CAPACITY, "too many paths"};
     SPR_HIP(hipSetDevice(H->device));
@@ -1000,7 +1073,23 @@ void load_arrays(spr_handle* H, const float* pos, const uint32_t* pos_path, uint
     }
     H->pcurve.ensure((size_t)Np + 1);
     SPR_HIP(hipMemcpyAsync(H->pcurve.p, pc.data(), pc.size() * 4, hipMemcpyHostToDevice, s));
+    if (arcs) {
+        H->cweight.ensure(C);
+        SPR_HIP(hipMemcpyAsync(H->cweight.p, ex.cweight, (size_t)C * 4, hipMemcpyHostToDevice, s));
+    }
+    if (paint) {
+        H->path_paint.ensure(Np);
+        H->paints.ensure(std::max<uint32_t>(ex.n_paints, 1));
+        H->stops.ensure(5 * (size_t)std::max<uint32_t>(ex.n_stops, 1));
+        SPR_HIP(hipMemcpyAsync(H->path_paint.p, ex.path_paint, (size_t)Np * 4, hipMemcpyHostToDevice, s));
+        SPR_HIP(hipMemcpyAsync(H->paints.p, ex.paints, (size_t)ex.n_paints * sizeof(spr::Paint),
+                               hipMemcpyHostToDevice, s));
+        SPR_HIP(hipMemcpyAsync(H->stops.p, ex.stops, (size_t)ex.n_stops * sizeof(spr::Stop), hipMemcpyHostToDevice,
+                               s));
+    }
     SPR_HIP(hipStreamSynchronize(s));
+    H->has_arcs = arcs;
+    H->has_paint = paint;
     H->h_cpath.assign(cpath, cpath + C);
     H->P = P; H->C = C; H->Np = Np;
     H->loaded = true;
@@ -1051,12 +1140,14 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
         SPR_HIP(hipEventCreate(&H->ev_end));
         H->ev_stage.resize(16);
         for (auto& e : H->ev_stage) SPR_HIP(hipEventCreate(&e));
-        SPR_HIP(hipHostMalloc((void**)&H->h_pinned, 256, hipHostMallocDefault));
-        std::memset(H->h_pinned, 0, 256);
+        SPR_HIP(hipHostMalloc((void**)&H->h_pinned, 512, hipHostMallocDefault));
+        std::memset(H->h_pinned, 0, 512);
         H->summary.ensure(spr::kSummaryMax);
+        H->mvp_dev.ensure(32);  // before any capture: a frame never allocates it
         const float id[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
         std::memcpy(H->mvp.m, id, sizeof(id));
         std::memcpy(H->mvp_pinned(), id, sizeof(id));
+        mvp_inverse(id, H->inv_pinned());
         H->image.ensure((size_t)4 * width * height);
     });
     if (rc != SPR_OK) { spr_destroy(H); return rc; }
@@ -1087,6 +1178,7 @@ int spr_destroy(spr_handle* H) {
     H->scan_ws.release();
     H->inj_rec.release();
     H->mvp_dev.release();
+    H->cweight.release(); H->path_paint.release(); H->paints.release(); H->stops.release();
     if (H->h_pinned) (void)hipHostFree(H->h_pinned);
     for (auto& e : H->ev_stage) if (e) (void)hipEventDestroy(e);
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
@@ -1116,7 +1208,7 @@ int spr_load_scene(spr_handle* H, const spr_scene* sc) {
     const spr::Scene& s = sc->s;
     return guarded(H, [&] {
         load_arrays(H, s.pos.data(), s.pos_path.data(), s.n_points(), s.curve_pos.data(), s.curve_type.data(),
-                    s.curve_path.data(), s.n_curves(), s.rule.data(), s.fill.data(), s.n_paths());
+                    s.curve_path.data(), s.n_curves(), s.rule.data(), s.fill.data(), s.n_paths(), extras_of(s));
     });
 }
 
@@ -1129,7 +1221,7 @@ int spr_load_rvg(spr_handle* H, const char* path) {
     if (rc != SPR_OK) return rc;
     return guarded(H, [&] {
         load_arrays(H, s.pos.data(), s.pos_path.data(), s.n_points(), s.curve_pos.data(), s.curve_type.data(),
-                    s.curve_path.data(), s.n_curves(), s.rule.data(), s.fill.data(), s.n_paths());
+                    s.curve_path.data(), s.n_curves(), s.rule.data(), s.fill.data(), s.n_paths(), extras_of(s));
     });
 }
 
@@ -1139,6 +1231,7 @@ int spr_set_mvp(spr_handle* H, const float m_rows[16]) {
     // copy is reading the pinned matrix; a captured frame picks the new one up at its next replay
     std::memcpy(H->mvp.m, m_rows, sizeof(H->mvp.m));
     std::memcpy(H->mvp_pinned(), m_rows, sizeof(H->mvp.m));
+    mvp_inverse(m_rows, H->inv_pinned());
     return SPR_OK;
 }
 
@@ -1403,8 +1496,16 @@ int spr_load_batch(spr_handle* H, const spr_scene* const* docs, const uint32_t* 
     if (!H || !docs || !widths || !heights) { g_error = "null argument"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
         if (n_docs == 0) throw Fail{SPR_ERR_INVALID, "empty batch"};
-        std::vector<float> pos;
-        std::vector<uint32_t> pos_path, cpos, ctype, cpath, rule, fill;
+        std::vector<float> pos, cweight;
+        std::vector<uint32_t> pos_path, cpos, ctype, cpath, rule, fill, path_paint;
+        std::vector<spr::Paint> paints;
+        std::vector<spr::Stop> stops;
+        bool any_arcs = false, any_paint = false;
+        for (uint32_t d = 0; d < n_docs; ++d) {
+            if (!docs[d]) continue;
+            any_arcs = any_arcs || !docs[d]->s.curve_weight.empty();
+            any_paint = any_paint || !docs[d]->s.path_paint.empty();
+        }
         std::vector<spr::DocDev> dd(n_docs);
         std::vector<int2> pcv;
         std::vector<int32_t> first;
@@ -1423,6 +1524,20 @@ int spr_load_batch(spr_handle* H, const spr_scene* const* docs, const uint32_t* 
             for (uint32_t v : S.curve_path) cpath.push_back(v + p_off);
             rule.insert(rule.end(), S.rule.begin(), S.rule.end());
             fill.insert(fill.end(), S.fill.begin(), S.fill.end());
+            if (any_arcs) {
+                if (S.curve_weight.empty()) cweight.insert(cweight.end(), S.n_curves(), 1.0f);
+                else cweight.insert(cweight.end(), S.curve_weight.begin(), S.curve_weight.end());
+            }
+            if (any_paint) {
+                // the document's paints and stops after those of the earlier documents
+                const uint32_t g_off = (uint32_t)paints.size(), s_off = (uint32_t)stops.size();
+                for (uint32_t p = 0; p < S.n_paths(); ++p) {
+                    const uint32_t q = S.path_paint.empty() ? spr::kSolidPaint : S.path_paint[p];
+                    path_paint.push_back(q == spr::kSolidPaint ? q : q + g_off);
+                }
+                for (spr::Paint g : S.paints) { g.first_stop += s_off; paints.push_back(g); }
+                stops.insert(stops.end(), S.stops.begin(), S.stops.end());
+            }
             dd[d].p0 = (int32_t)p_off;
             dd[d].p1 = (int32_t)rule.size();
             dd[d].w = (int32_t)w;
@@ -1438,7 +1553,9 @@ int spr_load_batch(spr_handle* H, const spr_scene* const* docs, const uint32_t* 
         if (rule.size() > (1u << 30) || pos_path.size() > (1u << 31)) throw Fail{SPR_ERR_CAPACITY, "batch too large"};
         const uint32_t Np = (uint32_t)rule.size();
         load_arrays(H, pos.data(), pos_path.data(), (uint32_t)pos_path.size(), cpos.data(), ctype.data(), cpath.data(),
-                    (uint32_t)ctype.size(), rule.data(), fill.data(), Np);
+                    (uint32_t)ctype.size(), rule.data(), fill.data(), Np,
+                    Extras{any_arcs ? cweight.data() : nullptr, any_paint ? path_paint.data() : nullptr, paints.data(),
+                           (uint32_t)paints.size(), stops.data(), (uint32_t)stops.size()});
         first.push_back(0);
         pcv.push_back(make_int2(1, 1));
         H->pcanvas.ensure(pcv.size());
@@ -1573,7 +1690,7 @@ int spr_learned_train(spr_handle* H, const spr_scene* sc, const spr_train_opts* 
             const spr::Scene sub = spr::subset(S, first, cnt);
             load_arrays(H, sub.pos.data(), sub.pos_path.data(), sub.n_points(), sub.curve_pos.data(),
                         sub.curve_type.data(), sub.curve_path.data(), sub.n_curves(), sub.rule.data(),
-                        sub.fill.data(), sub.n_paths());
+                        sub.fill.data(), sub.n_paths(), extras_of(sub));
             run_frame(H);
             SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p,
                                               H->rule.p, cnt, (int)H->w, (int)H->h,
@@ -1749,7 +1866,50 @@ int spr_scene_view_get(const spr_scene* sc, spr_scene_view* v) {
     v->curve_path_idx = s.curve_path.data(); v->n_curves = s.n_curves();
     v->fill_rule = s.rule.data(); v->fill_info = s.fill.data(); v->n_paths = s.n_paths();
     v->width = s.width; v->height = s.height;
+    v->curve_weight = s.curve_weight.empty() ? nullptr : s.curve_weight.data();
+    v->path_paint = s.path_paint.empty() ? nullptr : s.path_paint.data();
+    v->paints = reinterpret_cast<const spr_paint*>(s.paints.data());
+    v->n_paints = (uint32_t)s.paints.size();
+    v->stops = reinterpret_cast<const spr_stop*>(s.stops.data());
+    v->n_stops = (uint32_t)s.stops.size();
     return SPR_OK;
+}
+
+int spr_scene_create(const spr_scene_view* v, spr_scene** out) {
+    if (!v || !out) { g_error = "null argument"; return SPR_ERR_INVALID; }
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        const uint32_t P = v->n_points, C = v->n_curves, Np = v->n_paths;
+        std::string bad = spr::validate(v->pos_xy, v->pos_path_idx, P, v->curve_pos_map, v->curve_type,
+                                        v->curve_path_idx, C, v->fill_rule, Np);
+        if (bad.empty() && Np && !v->fill_info) bad = "null fill_info";
+        if (bad.empty())
+            bad = spr::validate_extras(v->curve_type, C, v->curve_weight, Np, v->path_paint,
+                                       reinterpret_cast<const spr::Paint*>(v->paints), v->n_paints,
+                                       reinterpret_cast<const spr::Stop*>(v->stops), v->n_stops);
+        if (!bad.empty()) throw Fail{SPR_ERR_INVALID, bad};
+        spr_scene* sc = new spr_scene();
+        spr::Scene& s = sc->s;
+        s.pos.assign(v->pos_xy, v->pos_xy + 2 * (size_t)P);
+        s.pos_path.assign(v->pos_path_idx, v->pos_path_idx + P);
+        s.curve_pos.assign(v->curve_pos_map, v->curve_pos_map + C);
+        s.curve_type.assign(v->curve_type, v->curve_type + C);
+        s.curve_path.assign(v->curve_path_idx, v->curve_path_idx + C);
+        s.rule.assign(v->fill_rule, v->fill_rule + Np);
+        s.fill.assign(v->fill_info, v->fill_info + Np);
+        bool arcs = false;
+        for (uint32_t c = 0; c < C; ++c) arcs = arcs || v->curve_type[c] == 0x13u;
+        if (arcs) s.curve_weight.assign(v->curve_weight, v->curve_weight + C);
+        if (v->path_paint) {
+            s.path_paint.assign(v->path_paint, v->path_paint + Np);
+            const spr::Paint* g = reinterpret_cast<const spr::Paint*>(v->paints);
+            const spr::Stop* t = reinterpret_cast<const spr::Stop*>(v->stops);
+            s.paints.assign(g, g + v->n_paints);
+            s.stops.assign(t, t + v->n_stops);
+        }
+        s.width = v->width; s.height = v->height;
+        *out = sc;
+    });
 }
 
 int spr_scene_write_rvg(const spr_scene* s, const char* path) {
