@@ -56,7 +56,13 @@ typedef struct spr_opts {
     uint32_t mode;      /* SPR_MODE_* */
     int32_t device;     /* HIP device ordinal, -1 = current */
     uint32_t debug;     /* 1 = keep every intermediate for spr_debug_export (slower) */
-    uint32_t reserved[4];
+    uint32_t aa;        /* 0 = the reference's binary coverage of the 2x2 cells; 16 = anti-aliased:
+                         * every boundary cell gets a 16-samples-per-pixel coverage mask (the
+                         * "comb-like multisampling" stencil mask the reference leaves TODO,
+                         * gen_fragment.comp:202, scanlinepr.vert:48-50) and each path's colour is
+                         * weighted by its pixel coverage.  Needs SPR_COMPOSE_SRC_OVER and
+                         * SPR_MODE_PLAIN. */
+    uint32_t reserved[3];
 } spr_opts;
 
 /* Per-frame counters and stage times, filled by spr_get_stats. */
@@ -110,7 +116,10 @@ enum {
     SPR_STAGE_FEATURES = 14,   /* float x 16Np: per-path predictor features (last learned frame) */
     SPR_STAGE_PRED = 15,       /* float x 2Np: predicted log2(1 + intersections), log2(1 + records) */
     SPR_STAGE_PATH_COUNTS = 16,/* int32 x 2Np: exact (intersections, records) per path, last frame */
-    SPR_STAGE_COUNT_ = 17
+    SPR_STAGE_COVERAGE = 17,   /* uint64 x R: per record, the 64-sample coverage mask (aa frames;
+                                  bit (2 py + px) * 16 + j = sample j of pixel (px, py) of the 2x2 cell,
+                                  sample j at ((7j mod 16) + 1/2)/16, (j + 1/2)/16; spans all ones) */
+    SPR_STAGE_COUNT_ = 18
 };
 
 /* --- lifecycle: VGRasterizer::initialize(void* window, uint32_t w, uint32_t h),

@@ -17,6 +17,7 @@ LIB_PATH = Path(os.environ.get("SPR_ORACLE_LIB", Path(__file__).resolve().parent
 STAGES = {
     "tpos": 0, "visible": 1, "cuts": 2, "counts": 3, "inter": 4, "key": 5, "wn": 6, "seg": 7,
     "skey": 8, "sidx": 9, "winding": 10, "flags": 11, "records": 12, "image": 13, "path": 14,
+    "cmask": 15,
 }
 IDENTITY = np.eye(4, dtype=np.float32)
 _lib = None
@@ -31,7 +32,7 @@ def lib() -> C.CDLL:
         vp, u32p, f32p, i32p = C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_float), C.POINTER(C.c_int32)
         L.oracle_run.argtypes = [f32p, u32p, C.c_uint32, u32p, u32p, u32p, C.c_uint32, u32p, u32p,
                                  C.c_uint32, f32p, C.c_int, C.c_int, C.c_int, i32p, C.c_int64,
-                                 f32p, u32p, vp, f32p]
+                                 f32p, u32p, vp, f32p, C.c_int]
         L.oracle_run.restype = vp
         L.oracle_get.argtypes = [vp, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
         L.oracle_free.argtypes = [vp]
@@ -63,7 +64,7 @@ class Frame:
     """One oracle frame: every stage's result as numpy arrays."""
 
     def __init__(self, arrays: dict, width: int, height: int, mvp=None, compose: int = 0,
-                 inter_override=None, threads: int = 1):
+                 inter_override=None, threads: int = 1, aa: int = 0):
         L = lib()
         L.oracle_set_threads(int(threads))
         self._keep = [np.ascontiguousarray(arrays["pos"], dtype=np.float32).reshape(-1),
@@ -97,7 +98,7 @@ class Frame:
                                  f(ctype, C.c_uint32), f(cpath, C.c_uint32), Cn, f(rule, C.c_uint32),
                                  f(fill, C.c_uint32), Np, f(m, C.c_float), int(width), int(height),
                                  int(compose), f(ov, C.c_int32) if ov is not None else None, n_ov,
-                                 cw, ppt, pnt, stp)
+                                 cw, ppt, pnt, stp, int(aa))
         self.width, self.height = int(width), int(height)
 
     def get(self, stage: str) -> np.ndarray:
@@ -106,7 +107,8 @@ class Frame:
         rc = lib().oracle_get(self._ctx, STAGES[stage], C.byref(p), C.byref(n))
         if rc != 0:
             raise KeyError(stage)
-        dt = np.uint8 if stage == "image" else (np.float32 if stage in ("tpos", "cuts") else np.int32)
+        dt = np.uint8 if stage == "image" else (np.float32 if stage in ("tpos", "cuts") else
+                                                (np.uint64 if stage == "cmask" else np.int32))
         cnt = n.value // np.dtype(dt).itemsize
         if cnt == 0:
             return np.zeros(0, dtype=dt)

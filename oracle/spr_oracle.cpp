@@ -317,6 +317,7 @@ struct Frame {
     const PaintRec* paints = nullptr;
     const float* stops = nullptr;
     float m[16]; int w, h; int compose;
+    int aa = 0;                               // 16: anti-aliased coverage of boundary cells
     float inv[9];                             // inverse of the MVP's 2D projective part
     // stages
     std::vector<float> tpos;
@@ -330,6 +331,7 @@ struct Frame {
     std::vector<int32_t> flag_scan;  // 2N+1 exclusive
     std::vector<int32_t> records;    // 4R
     std::vector<int32_t> rec_path;   // R: the path of each record
+    std::vector<uint64_t> cmask;     // R: sample coverage of each record (anti-aliased mode)
     int32_t n_cells = 0, n_spans = 0;
     std::vector<uint8_t> image;
     bool injected = false;
@@ -594,6 +596,73 @@ void stage_mark(Frame& F) {
     F.n_spans = F.flag_scan[2 * (size_t)N] - F.n_cells;
 }
 
+// Anti-aliased coverage of a boundary cell (no reference counterpart: the reference leaves the
+// cells' "comb-like multisampling" stencil mask TODO, gen_fragment.comp:202, scanlinepr.vert:48-50).
+// 16 samples per pixel, sample j of a pixel at ((7j mod 16) + 1/2)/16 across and (j + 1/2)/16 up
+// (one per sample row and column); bit (2 py + px) * 16 + j of the cell's 64-bit mask is sample j
+// of its pixel (px, py).  The path is its chord polygon, the chords being the segments between
+// consecutive intersections that gen_fragment forms (pf -> pl).  The winding number at a sample
+// is the cell's centre-row winding number at its left edge (the exclusive scan at the cell's
+// first sorted entry) plus the changes met on the way from there to the sample: along the
+// centre row to the sample's column, the winding change of each chord crossing the row (the
+// half-open rule of gen_fragment.comp:189-198), then along that column to the sample's height,
+// +1 for each chord going right and -1 for each chord going left when moving up (the reverse
+// when moving down).  Both legs stay inside the cell, so only the cell's own chords are met.
+uint64_t cell_mask(const Frame& F, int32_t f) {
+    const int32_t N = (int32_t)F.skey.size();
+    const int32_t NI = (int32_t)(F.inter.size() / 2);
+    const int32_t key = F.skey[f];
+    const int32_t p = F.path[f] & 0x3FFFFFFF;
+    const int rx = (key & 0xFFFF) - 0x7FFF, ry = ((key >> 16) & 0xFFFF) - 0x7FFF;
+    const float yc = (float)(ry + 1);
+    struct Chord { float ax, ay, bx, by; };
+    std::vector<Chord> ch;
+    for (int32_t g = f; g < N && F.skey[g] == key && (F.path[g] & 0x3FFFFFFF) == p; ++g) {
+        const int32_t i = F.sidx[g];
+        int32_t c0 = F.inter[2 * (size_t)i], b0 = F.inter[2 * (size_t)i + 1];
+        int32_t c1 = -1, b1 = 0x3f800000;
+        if (i + 1 < NI) { c1 = F.inter[2 * (size_t)i + 2]; b1 = F.inter[2 * (size_t)i + 3]; }
+        float t0 = gl_max(bitsf((uint32_t)b0 & 0xFFFFFFFCu), 0.0f);
+        float t1 = gl_max(bitsf((uint32_t)b1 & 0xFFFFFFFCu), 0.0f);
+        if (c0 != c1) t1 = 1.0f;
+        Curve cv = load_curve(F.tpos.data(), F.cpos, F.ctype, F.cweight, (uint32_t)c0);
+        ch.push_back({eval1(cv.type, t0, cv.x), eval1(cv.type, t0, cv.y), eval1(cv.type, t1, cv.x),
+                      eval1(cv.type, t1, cv.y)});
+    }
+    const int32_t W0 = F.winding[f];
+    const uint32_t rule = F.rule[p];
+    uint64_t m = 0;
+    for (int py = 0; py < 2; ++py) {
+        for (int px = 0; px < 2; ++px) {
+            for (int j = 0; j < 16; ++j) {
+                float xs = (float)(rx + px) + ((float)((7 * j) & 15) + 0.5f) / 16.0f;
+                float ys = (float)(ry + py) + ((float)j + 0.5f) / 16.0f;
+                int32_t W = W0;
+                for (const Chord& c : ch) {
+                    int sr = 0;
+                    if (c.ay < yc && yc <= c.by) sr = -1;
+                    else if (c.by < yc && yc <= c.ay) sr = 1;
+                    if (sr != 0) {
+                        float xc = c.ax + (yc - c.ay) * ((c.bx - c.ax) / (c.by - c.ay));
+                        if (xc < xs) W += sr;
+                    }
+                    int dc = 0;
+                    if (c.ax < xs && xs <= c.bx) dc = 1;
+                    else if (c.bx < xs && xs <= c.ax) dc = -1;
+                    if (dc != 0) {
+                        float yx = c.ay + (xs - c.ax) * ((c.by - c.ay) / (c.bx - c.ax));
+                        if (ys > yc) { if (yc <= yx && yx < ys) W += dc; }
+                        else if (ys < yx && yx <= yc) W -= dc;
+                    }
+                }
+                bool in = rule == 0 ? W != 0 : (W & 1) != 0;
+                if (in) m |= 1ull << ((py * 2 + px) * 16 + j);
+            }
+        }
+    }
+    return m;
+}
+
 // gen_merged_fragment_and_span.comp:152-222 (the host reads n_output_fragments and n_spans back,
 // scanline_rasterizer.cpp:572-580)
 void stage_emit(Frame& F) {
@@ -601,6 +670,7 @@ void stage_emit(Frame& F) {
     const int32_t R = F.n_cells + F.n_spans;
     F.records.assign(4 * (size_t)R, 0);
     F.rec_path.assign((size_t)R, 0);
+    F.cmask.assign(F.aa ? (size_t)R : 0, ~0ull);
     const int32_t* S = F.flag_scan.data();
     for (int32_t f = 0; f < N; ++f) {
         int32_t ff = F.flags[f];
@@ -619,6 +689,7 @@ void stage_emit(Frame& F) {
             r[2] = (int32_t)F.fill[pidx];
             r[3] = fidx1;
             F.rec_path[o] = pidx;
+            if (F.aa) F.cmask[o] = cell_mask(F, f);
         }
         if (sf != 0) {
             int32_t o = nfb + nsb + ff;
@@ -773,6 +844,7 @@ void stage_draw(Frame& F) {
         int x = r[0] & 0xFFFF, y = r[0] >> 16;
         int x0 = std::max(x, 0), x1 = std::min(x + r[1], w);
         const uint32_t pi = paint_of(k);
+        const uint64_t cm = F.aa ? F.cmask[k] : ~0ull;
         for (int yy = y; yy < y + 2; ++yy) {
             if (yy < 0 || yy >= h) continue;
             float* row = &acc[4 * (size_t)(h - 1 - yy) * w];
@@ -782,6 +854,19 @@ void stage_draw(Frame& F) {
                               (float)((c >> 16) & 0xFF) / 255.0f, (float)((c >> 24) & 0xFF) / 255.0f};
                 float a = s[3], ia = 1.0f - a;
                 float* d = row + 4 * xx;
+                if (cm != ~0ull) {
+                    // a boundary cell of the anti-aliased mode: the premultiplied colour times the
+                    // pixel's share of covered samples
+                    int bit = ((yy - y) * 2 + (xx - x)) * 16;
+                    float cov = (float)__builtin_popcountll((cm >> bit) & 0xFFFFull) / 16.0f;
+                    float s0 = (s[0] * a) * cov, s1 = (s[1] * a) * cov, s2 = (s[2] * a) * cov, sw = a * cov;
+                    float iw = 1.0f - sw;
+                    d[0] = s0 + d[0] * iw;
+                    d[1] = s1 + d[1] * iw;
+                    d[2] = s2 + d[2] * iw;
+                    d[3] = sw + d[3] * iw;
+                    continue;
+                }
                 d[0] = s[0] * a + d[0] * ia;
                 d[1] = s[1] * a + d[1] * ia;
                 d[2] = s[2] * a + d[2] * ia;
@@ -804,7 +889,7 @@ extern "C" {
 enum {
     ORC_TPOS = 0, ORC_VISIBLE = 1, ORC_CUTS = 2, ORC_COUNTS = 3, ORC_INTER = 4, ORC_KEY = 5,
     ORC_WN = 6, ORC_SEG = 7, ORC_SKEY = 8, ORC_SIDX = 9, ORC_WINDING = 10, ORC_FLAGS = 11,
-    ORC_RECORDS = 12, ORC_IMAGE = 13, ORC_PATH = 14
+    ORC_RECORDS = 12, ORC_IMAGE = 13, ORC_PATH = 14, ORC_CMASK = 15
 };
 
 void oracle_set_threads(int n) { nthreads_setting = n < 1 ? 1 : n; }
@@ -825,7 +910,7 @@ void* oracle_run(const float* pos, const uint32_t* pos_path, uint32_t P, const u
                  const uint32_t* ctype, const uint32_t* cpath, uint32_t C, const uint32_t* rule,
                  const uint32_t* fill, uint32_t Np, const float* mvp_rows, int w, int h,
                  int compose, const int32_t* inter_override, int64_t n_inter, const float* cweight,
-                 const uint32_t* path_paint, const void* paints, const float* stops) {
+                 const uint32_t* path_paint, const void* paints, const float* stops, int aa) {
     Frame* F = new Frame();
     F->pos = pos; F->pos_path = pos_path; F->P = P;
     F->cpos = cpos; F->ctype = ctype; F->cpath = cpath; F->C = C;
@@ -837,6 +922,7 @@ void* oracle_run(const float* pos, const uint32_t* pos_path, uint32_t P, const u
     std::memcpy(F->m, mvp_rows, sizeof(F->m));
     mvp_inverse(F->m, F->inv);
     F->w = w; F->h = h; F->compose = compose;
+    F->aa = aa;
     stage_transform(*F);
     stage_count(*F);
     exclusive_scan_inplace(F->counts, C);
@@ -875,6 +961,7 @@ int oracle_get(void* ctx, int stage, const void** ptr, int64_t* nbytes) {
         case ORC_RECORDS: return put(F->records.data(), F->records.size() * 4);
         case ORC_IMAGE: return put(F->image.data(), F->image.size());
         case ORC_PATH: return put(F->path.data(), F->path.size() * 4);
+        case ORC_CMASK: return put(F->cmask.data(), F->cmask.size() * 8);
         default: return -1;
     }
 }

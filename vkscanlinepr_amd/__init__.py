@@ -24,14 +24,16 @@ MODE_PLAIN, MODE_LEARNED = 0, 1
 STAGES = {
     "tpos": 0, "visible": 1, "cuts": 2, "counts": 3, "inter": 4, "key": 5, "wn": 6, "seg": 7,
     "skey": 8, "sidx": 9, "winding": 10, "flags": 11, "records": 12, "image": 13, "features": 14,
-    "pred": 15, "path_counts": 16,
+    "pred": 15, "path_counts": 16, "coverage": 17,
 }
 _STAGE_DTYPE = {
     "tpos": np.float32, "visible": np.int32, "cuts": np.float32, "counts": np.int32,
     "inter": np.int32, "key": np.int32, "wn": np.int32, "seg": np.int32, "skey": np.int32,
     "sidx": np.int32, "winding": np.int32, "flags": np.int32, "records": np.int32,
     "image": np.uint8, "features": np.float32, "pred": np.float32, "path_counts": np.int32,
+    "coverage": np.uint64,
 }
+AA_SAMPLES = 16  # samples per pixel of the anti-aliased mode
 MLP_PARAMS = 16 * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2  # W1, b1, W2, b2, W3, b3
 MLP_FLOATS = MLP_PARAMS + 32                          # + feature mean[16], 1/std[16]
 
@@ -44,7 +46,7 @@ class SprError(RuntimeError):
 
 class Opts(C.Structure):
     _fields_ = [("compose", C.c_uint32), ("mode", C.c_uint32), ("device", C.c_int32),
-                ("debug", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("debug", C.c_uint32), ("aa", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class Stats(C.Structure):
@@ -307,9 +309,9 @@ class Rasterizer:
     """One rasteriser handle: a w x h RGBA8 target on one GPU, with its own HIP stream."""
 
     def __init__(self, width: int, height: int, compose: int = COMPOSE_OVERWRITE,
-                 mode: int = MODE_PLAIN, device: int = -1, debug: bool = False):
+                 mode: int = MODE_PLAIN, device: int = -1, debug: bool = False, aa: int = 0):
         self.width, self.height = int(width), int(height)
-        o = Opts(compose=compose, mode=mode, device=device, debug=1 if debug else 0)
+        o = Opts(compose=compose, mode=mode, device=device, debug=1 if debug else 0, aa=int(aa))
         out = C.c_void_p()
         _check(lib().spr_create(self.width, self.height, C.byref(o), C.byref(out)))
         self._h = out

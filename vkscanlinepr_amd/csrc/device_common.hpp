@@ -256,8 +256,8 @@ struct CurveAxis {
 // evaluation.
 template <uint32_t T>
 __device__ inline float bisect_axis(const CurveAxis& ax, float c, float t_min, float t1) {
-    const float c0 = ax.c[0], c1 = ax.c[1], c2 = ax.c[2], c3 = ax.c[3];
     float t0 = t_min;
+    const float c0 = ax.c[0], c1 = ax.c[1], c2 = ax.c[2], c3 = ax.c[3];
     float vt0 = eval_axis(T, t0, c0, c1, c2, c3);
     if (vt0 == c) return t0;
     // The sign of vt0-c never changes (vt0 is only replaced by a value of equal sign), so the

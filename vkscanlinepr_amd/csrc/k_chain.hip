@@ -112,7 +112,6 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
         float t0 = 0.0f, p0x = 0.0f, p0y = 0.0f;
         if (nc != 0xFFFFFFFFu) { p0x = cv.x[0]; p0y = cv.y[0]; }
         int off = 0;
-        uint32_t cls = 0;  // the slots' class bytes, two per slot
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             PieceDesc pd;
@@ -241,16 +240,29 @@ __device__ inline CurveAxis load_axis(const float2* __restrict__ tpos, const uin
 }
 
 // Sequential chain (one lane): the reference's solve per crossing.
+// The chain's values go to out (= chain_t + d.out) through a 16-float LDS slot of the lane
+// (value k at stage[(k % 16) * kBlock]): every 16 crossings, at the same step in all lanes of
+// the wave, each lane stores its 16 values back to back, so each cache line of a chain is
+// written whole while it is resident instead of one value per solve (a lane's stores are
+// otherwise hundreds of instructions apart, and the 64 lanes write 64 different chains).
 template <uint32_t T>
 __device__ inline void solve_chain_lane_t(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
-                                          uint8_t* __restrict__ mono) {
+                                          uint8_t* __restrict__ mono, float* __restrict__ stage) {
     float tprev = d.t0, line = d.line0;
     bool up = true, down = true;
     const uint32_t side = (uint32_t)d.side;
     for (int k = 0; k < d.n; ++k) {
         const float ts = T == kLine ? solve_line(a, line, tprev, d.t1) : bisect_axis<T>(a, line, tprev, d.t1);
         const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | side);
-        out[k] = v;
+        stage[(k & 15) * kBlock] = v;
+        if ((k & 15) == 15) {
+            float* dst = out + (k - 15);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) dst[j] = stage[j * kBlock];
+        } else if (k == d.n - 1) {
+            float* dst = out + (k & ~15);
+            for (int j = 0; j <= (k & 15); ++j) dst[j] = stage[j * kBlock];
+        }
         // the y chain is preceded by the piece start in the merge, the x chain is not
         if (k > 0 || side) { up = up && v >= tprev; down = down && v <= tprev; }
         tprev = v;
@@ -260,12 +272,12 @@ __device__ inline void solve_chain_lane_t(const ChainDesc& d, const CurveAxis& a
 }
 
 __device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
-                                        uint8_t* __restrict__ mono) {
+                                        uint8_t* __restrict__ mono, float* __restrict__ stage) {
     const uint32_t type = (uint32_t)d.type;
-    if (type == kCubic) solve_chain_lane_t<kCubic>(d, a, out, mono);
-    else if (type == kLine) solve_chain_lane_t<kLine>(d, a, out, mono);
-    else if (type == kQuad) solve_chain_lane_t<kQuad>(d, a, out, mono);
-    else solve_chain_lane_t<kArc>(d, a, out, mono);
+    if (type == kCubic) solve_chain_lane_t<kCubic>(d, a, out, mono, stage);
+    else if (type == kLine) solve_chain_lane_t<kLine>(d, a, out, mono, stage);
+    else if (type == kQuad) solve_chain_lane_t<kQuad>(d, a, out, mono, stage);
+    else solve_chain_lane_t<kArc>(d, a, out, mono, stage);
 }
 
 // Cubic chain solved by one wave: 4 rounds of a 6-level speculative bisection per crossing.
@@ -300,6 +312,7 @@ __device__ inline void solve_chain_wave_t(const ChainDesc& d, const CurveAxis& a
     const uint32_t side = (uint32_t)d.side;
     float tprev = d.t0, line = d.line0;
     bool up = true, down = true;
+    float kept = 0.0f;
     for (int k = 0; k < d.n; ++k) {
         const float c = line;
         const float t_min = tprev;
@@ -337,7 +350,11 @@ __device__ inline void solve_chain_wave_t(const ChainDesc& d, const CurveAxis& a
             ts = fabsf(vtm - c) > 1.0f ? t_min : tm;
         }
         const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | side);
-        if (lane == 0) out[k] = v;
+        // lane k % 64 keeps value k; 64 values are stored at once (coalesced)
+        if (lane == (k & 63)) kept = v;
+        if ((k & 63) == 63 || k == d.n - 1) {
+            if (lane <= (k & 63)) out[k - (k & 63) + lane] = kept;
+        }
         if (k > 0 || side) { up = up && v >= tprev; down = down && v <= tprev; }
         tprev = v;
         line += d.step;
@@ -365,6 +382,8 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
                                                         int32_t* __restrict__ work,
                                                         float* __restrict__ chain_t,
                                                         uint8_t* __restrict__ mono) {
+    __shared__ float s_stage[16 * kBlock];
+    float* stage = s_stage + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const int32_t total = class_off[kClasses];
     const int32_t lb = *light_begin;
@@ -396,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
                 const int32_t ci = order[k];
                 const ChainDesc d = chains[ci];
                 const CurveAxis a = load_axis(tpos, cpos, cweight, d);
-                solve_chain_lane(d, a, chain_t + d.out, mono + ci);
+                solve_chain_lane(d, a, chain_t + d.out, mono + ci, stage);
             }
             __builtin_amdgcn_s_setprio(0);
         }

@@ -23,6 +23,11 @@
 // with a paint takes, at each covered pixel, the paint's colour at that pixel's centre, packed
 // like a solid fill, so the two rows of a cell row get their own colours; the variant that
 // evaluates them keeps a second row of pixels per thread.
+//
+// Anti-aliased coverage (src-over): a boundary cell record whose 64-sample mask (k_aa.hip) is not
+// full writes each of its pixels' covered-sample counts (both rows) into an LDS table indexed by
+// (path of the batch, column); the fold scales such a path's premultiplied colour by count/16.
+// Batches are then 8 paths (the table is 8 x 4096 x 2 bytes) and both rows are kept.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -34,6 +39,7 @@ constexpr int kBlock = 256;
 constexpr int kPix = 16;                   // consecutive pixels per thread
 constexpr int kChunk = kBlock * kPix;      // columns per pass
 constexpr int kBatch = 64;                 // paths per coverage mask
+constexpr int kAaBatch = 8;                // paths per batch with anti-aliased coverage
 constexpr int kList = 2 * kBlock;          // paths listed before their batches are folded
 constexpr int kUnroll = 4;                 // records loaded per thread before applying
 typedef unsigned long long u64;
@@ -154,6 +160,9 @@ __device__ inline float4 premul(uint32_t col) {
                        ((float)((col >> 16) & 0xFFu) / 255.0f) * a, a);
 }
 
+// A premultiplied colour times a coverage share.
+__device__ inline float4 scale(const float4 s, float c) { return make_float4(s.x * c, s.y * c, s.z * c, s.w * c); }
+
 __device__ inline void blend(float* d, const float4 sa) {
     const float ia = 1.0f - sa.w;
     d[0] = sa.x + d[0] * ia;
@@ -162,7 +171,7 @@ __device__ inline void blend(float* d, const float4 sa) {
     d[3] = sa.w + d[3] * ia;
 }
 
-template <int COMPOSE, int GRAD>
+template <int COMPOSE, int GRAD, int AA>
 __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict__ records,
                                                       const int2* __restrict__ brec,
                                                       const int32_t* __restrict__ bucket_base,
@@ -171,7 +180,11 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                                       int w, int h, int64_t rec_cap,
                                                       uint8_t* __restrict__ image,
                                                       const DocDev* __restrict__ docs, int n_docs,
-                                                      const PaintArgs pa) {
+                                                      const PaintArgs pa,
+                                                      const unsigned long long* __restrict__ cmask) {
+    constexpr int TWO = (GRAD || AA) ? 1 : 0;            // the two rows of a cell row differ
+    constexpr int BATCH = AA ? kAaBatch : kBatch;
+    __shared__ uint16_t lcov[AA ? kAaBatch * kChunk : 1];  // 0x8000 | row y count | row y+1 count << 5
     __shared__ u64 ev[kChunk];
     __shared__ int2 list[kList];
     __shared__ uint32_t lcol[kList];
@@ -201,14 +214,14 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
     const int xb = tid * kPix;  // first column of this thread inside the chunk
     for (int c0 = 0; c0 < w; c0 += kChunk) {
         const int cw = w - c0 < kChunk ? w - c0 : kChunk;
-        uint32_t pix[kPix], pix1[GRAD ? kPix : 1];  // rows y and, with paints, y + 1
-        float acc[COMPOSE ? kPix : 1][4], acc1[(COMPOSE && GRAD) ? kPix : 1][4];
+        uint32_t pix[kPix], pix1[TWO ? kPix : 1];  // rows y and, with paints or coverage, y + 1
+        float acc[COMPOSE ? kPix : 1][4], acc1[(COMPOSE && TWO) ? kPix : 1][4];
 #pragma unroll
         for (int j = 0; j < kPix; ++j) {
             pix[j] = 0xFFFFFFFFu;
-            if constexpr (GRAD != 0) pix1[j] = 0xFFFFFFFFu;
+            if constexpr (TWO != 0) pix1[j] = 0xFFFFFFFFu;
             if constexpr (COMPOSE != 0) { acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 1.0f; }
-            if constexpr (COMPOSE != 0 && GRAD != 0) { acc1[j][0] = acc1[j][1] = acc1[j][2] = acc1[j][3] = 1.0f; }
+            if constexpr (COMPOSE != 0 && TWO != 0) { acc1[j][0] = acc1[j][1] = acc1[j][2] = acc1[j][3] = 1.0f; }
         }
         uint32_t p0 = pbeg;
         while (p0 < pend) {
@@ -247,8 +260,12 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 n += add;
                 p0 += kBlock;
             }
-            for (int q0 = 0; q0 < n; q0 += kBatch) {
-                const int qn = n - q0 < kBatch ? n - q0 : kBatch;
+            for (int q0 = 0; q0 < n; q0 += BATCH) {
+                const int qn = n - q0 < BATCH ? n - q0 : BATCH;
+                if constexpr (AA != 0) {
+                    uint4* lc = reinterpret_cast<uint4*>(lcov);
+                    for (int k = tid; k < kAaBatch * kChunk / 8; k += kBlock) lc[k] = make_uint4(0, 0, 0, 0);
+                }
                 // clear the event words; the batch's colours; its records as one flat index space
 #pragma unroll
                 for (int j = 0; j < kPix; ++j) ev[j * kBlock + tid] = 0ull;
@@ -275,6 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 for (int f0 = tid; f0 < total; f0 += kBlock * kUnroll) {
                     uint32_t rec[kUnroll];
                     int qi[kUnroll];
+                    int64_t ri[kUnroll];
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u) {
                         const int f = f0 + u * kBlock;
@@ -286,7 +304,8 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                 if (bpre[mid] <= f) lo = mid; else hi = mid;
                             }
                             qi[u] = lo;
-                            rec[u] = records[list[q0 + lo].x + (f - bpre[lo])];
+                            ri[u] = (int64_t)list[q0 + lo].x + (f - bpre[lo]);
+                            rec[u] = records[ri[u]];
                         }
                     }
 #pragma unroll
@@ -300,6 +319,20 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                         if (xs < xe) {
                             atomicXor(&ev[ev_slot(xs)], bit);
                             if (xe < cw) atomicXor(&ev[ev_slot(xe)], bit);  // ends past the chunk change nothing
+                        }
+                        if constexpr (AA != 0) {
+                            const unsigned long long cm = cmask[ri[u]];
+                            if (cm != ~0ull) {  // a boundary cell (2 wide): its pixels' sample counts
+                                const int x0 = (int)(rec[u] & 0xFFFFu) - c0;
+#pragma unroll
+                                for (int px = 0; px < 2; ++px) {
+                                    const int col = x0 + px;
+                                    if (col < 0 || col >= cw) continue;
+                                    const uint32_t n0 = __popcll((cm >> (16 * px)) & 0xFFFFull);
+                                    const uint32_t n1 = __popcll((cm >> (16 * (2 + px))) & 0xFFFFull);
+                                    lcov[qi[u] * kChunk + ev_slot(col)] = (uint16_t)(0x8000u | n0 | (n1 << 5));
+                                }
+                            }
                         }
                     }
                 }
@@ -336,15 +369,22 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                             while (m32) {
                                 const int q = __builtin_ctz(m32);
                                 m32 &= m32 - 1u;
-                                if constexpr (GRAD != 0) {
-                                    if ((gmask >> (32 * hw + q)) & 1ull) {
+                                if constexpr (TWO != 0) {
+                                    float4 s0 = L[q], s1 = L[q];
+                                    if (GRAD && ((gmask >> (32 * hw + q)) & 1ull)) {
                                         const uint2 pc = paint_pair(pa, lpaint[q0 + 32 * hw + q], x, y);
-                                        blend(acc[j], premul(pc.x));
-                                        blend(acc1[j], premul(pc.y));
-                                    } else {
-                                        blend(acc[j], L[q]);
-                                        blend(acc1[j], L[q]);
+                                        s0 = premul(pc.x);
+                                        s1 = premul(pc.y);
                                     }
+                                    if constexpr (AA != 0) {
+                                        const uint32_t e = lcov[(32 * hw + q) * kChunk + j * kBlock + tid];
+                                        if (e & 0x8000u) {
+                                            s0 = scale(s0, (float)(e & 31u) / 16.0f);
+                                            s1 = scale(s1, (float)((e >> 5) & 31u) / 16.0f);
+                                        }
+                                    }
+                                    blend(acc[j], s0);
+                                    blend(acc1[j], s1);
                                 } else {
                                     blend(acc[j], L[q]);
                                 }
@@ -369,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
 #pragma unroll
             for (int j = 0; j < kPix; ++j) {
                 pix[j] = round8(acc[j]);
-                if constexpr (GRAD != 0) pix1[j] = round8(acc1[j]);
+                if constexpr (TWO != 0) pix1[j] = round8(acc1[j]);
             }
         }
         for (int yy = y; yy < y + 2 && yy < h; ++yy) {
@@ -377,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
             uint32_t out[kPix];
 #pragma unroll
             for (int j = 0; j < kPix; ++j) {
-                if constexpr (GRAD != 0) out[j] = yy == y ? pix[j] : pix1[j];
+                if constexpr (TWO != 0) out[j] = yy == y ? pix[j] : pix1[j];
                 else out[j] = pix[j];
             }
             if (xb + kPix <= cw && (((uintptr_t)(row + xb)) & 15u) == 0) {
@@ -400,17 +440,20 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
 hipError_t launch_composite(const uint32_t* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
                             uint8_t* image, hipStream_t s, int64_t rec_cap, const DocDev* docs, int n_docs,
-                            int total_rows, const PaintArgs* paint) {
+                            int total_rows, const PaintArgs* paint, const unsigned long long* cmask) {
     const int rows = docs ? total_rows : (h + 1) / 2;
     if (rows == 0 || (!docs && w == 0)) return hipSuccess;
+    if (cmask && !compose) return hipErrorInvalidValue;  // coverage needs the src-over blend
     const PaintArgs pa = paint ? *paint : PaintArgs{nullptr, nullptr, nullptr, nullptr};
-#define SPR_COMPOSITE(C, G)                                                                                 \
-    k_composite<C, G><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image, \
-                                              docs, n_docs, pa)
-    if (compose) {
-        if (paint) SPR_COMPOSITE(1, 1); else SPR_COMPOSITE(1, 0);
+#define SPR_COMPOSITE(C, G, A)                                                                              \
+    k_composite<C, G, A><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap,     \
+                                                 image, docs, n_docs, pa, cmask)
+    if (cmask) {
+        if (paint) SPR_COMPOSITE(1, 1, 1); else SPR_COMPOSITE(1, 0, 1);
+    } else if (compose) {
+        if (paint) SPR_COMPOSITE(1, 1, 0); else SPR_COMPOSITE(1, 0, 0);
     } else {
-        if (paint) SPR_COMPOSITE(0, 1); else SPR_COMPOSITE(0, 0);
+        if (paint) SPR_COMPOSITE(0, 1, 0); else SPR_COMPOSITE(0, 0, 0);
     }
 #undef SPR_COMPOSITE
     return hipGetLastError();

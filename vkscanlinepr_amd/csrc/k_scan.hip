@@ -166,7 +166,8 @@ __global__ __launch_bounds__(kMarkThreads) void k_mark_emit(
     int4* __restrict__ records, uint32_t* __restrict__ crec, int32_t* __restrict__ flags,
     int32_t* __restrict__ totals,
     const int32_t* __restrict__ bucket_base, const int2* __restrict__ prow, int2* __restrict__ brec,
-    int32_t* __restrict__ err, const int2* __restrict__ pcanvas, const int32_t* __restrict__ wbase) {
+    int32_t* __restrict__ err, const int2* __restrict__ pcanvas, const int32_t* __restrict__ wbase,
+    int32_t* __restrict__ cell_first) {
     __shared__ int32_t s_tile;
     __shared__ unsigned long long s_w[kMarkThreads / 64];
     __shared__ unsigned long long s_prefix;
@@ -284,6 +285,11 @@ __global__ __launch_bounds__(kMarkThreads) void k_mark_emit(
                 const int loc = (int)(o - tile_rec0);
                 // compact record for the composite: x | width << 16 (row and colour are implied by
                 // the (path, row) record range); the full int4 record only for debug export
+                // anti-aliased frames: each record's first sorted entry (-1 for a span)
+                if (cell_first) {
+                    if (cell) cell_first[o] = (int32_t)f;
+                    if (span) cell_first[o + cell] = -1;
+                }
                 if (cell) {
                     const int x1 = key_x(k1);
                     s_rec[loc] = (uint32_t)x1 | (2u << 16);
@@ -413,12 +419,13 @@ hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const u
                             ScanCtl ctl, int4* records, uint32_t* crec, int32_t* flags, int32_t* totals,
                             const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s,
                             const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
-                            int32_t* err, const int2* pcanvas, const int32_t* wbase) {
+                            int32_t* err, const int2* pcanvas, const int32_t* wbase, int32_t* cell_first) {
     if (N == 0) return hipSuccess;
     k_mark_emit<<<(unsigned)((N + kMarkTile - 1) / kMarkTile), kMarkThreads, 0, s>>>(skey, winding, epath, rule, fill, N, n_dev,
                                                                  sstart, Np, rec_cap, w, h, ctl.status,
                                                                  ctl.counter, records, crec, flags, totals,
-                                                                 bucket_base, prow, brec, err, pcanvas, wbase);
+                                                                 bucket_base, prow, brec, err, pcanvas, wbase,
+                                                                 cell_first);
     return hipGetLastError();
 }
 

@@ -238,7 +238,18 @@ hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const u
                             ScanCtl ctl, int4* records, uint32_t* crec, int32_t* flags, int32_t* totals,
                             const int32_t* bucket_base, const int2* prow, int2* brec, hipStream_t s,
                             const int32_t* n_dev, const int32_t* sstart, uint32_t Np, int64_t rec_cap,
-                            int32_t* err, const int2* pcanvas = nullptr, const int32_t* wbase = nullptr);
+                            int32_t* err, const int2* pcanvas = nullptr, const int32_t* wbase = nullptr,
+                            int32_t* cell_first = nullptr);
+
+// ---- anti-aliased coverage (k_aa.hip): the 64-sample mask of every boundary cell record (2x2
+// pixels x 16 samples, bit (2 py + px) * 16 + j), ~0 for spans.  cell_first[o] = the record's
+// first sorted entry (-1: a span); the record count is totals[0] + totals[1] on the device.
+constexpr int kAaSamples = 16;
+hipError_t launch_cell_cover(const int32_t* cell_first, const int32_t* totals, int64_t rec_cap,
+                             const int32_t* skey, const int32_t* sidx, const int32_t* winding,
+                             const uint32_t* epath, const int2* inter, int64_t N, const float2* tpos,
+                             const uint32_t* cpos, const uint32_t* ctype, const float* cweight,
+                             const uint32_t* rule, unsigned long long* cmask, hipStream_t s);
 
 // ---- learned single-pass allocation (k_learned.hip)
 constexpr int kMlpIn = 16, kMlpHidden = 64, kMlpOut = 2;
@@ -314,9 +325,11 @@ struct PaintArgs {
 };
 
 // ---- composite (a12).  paint (optional): per-pixel gradient colours for the paths that have one.
+// cmask (optional, src-over only): the records' sample coverage (launch_cell_cover).
 hipError_t launch_composite(const uint32_t* records, const int2* brec, const int32_t* bucket_base,
                             const int2* prow, const uint32_t* fill, uint32_t Np, int w, int h, int compose,
                             uint8_t* image, hipStream_t s, int64_t rec_cap, const DocDev* docs = nullptr,
-                            int n_docs = 0, int total_rows = 0, const PaintArgs* paint = nullptr);
+                            int n_docs = 0, int total_rows = 0, const PaintArgs* paint = nullptr,
+                            const unsigned long long* cmask = nullptr);
 
 }  // namespace spr

@@ -167,6 +167,8 @@ struct spr_handle {
     DBuf<unsigned long long> sorted, sbig, stmp;
     DBuf<int32_t> skey, sidx, winding, flags;
     DBuf<int4> records;     // full records, kept only with the debug option
+    DBuf<int32_t> cell_first;            // anti-aliased frames: each record's first sorted entry
+    DBuf<unsigned long long> cmask;      // anti-aliased frames: each record's sample coverage
     DBuf<uint32_t> crec;    // compact records (x | width << 16) read by the composite
     DBuf<int32_t> row_lo, row_hi, bucket_base, bflag, band_start, big_list;
     DBuf<int2> band_pr;
@@ -355,8 +357,11 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     const uint32_t P = H->P, C = H->C, Np = H->Np;
     const int w = (int)H->w, h = (int)H->h;
     const bool debug = H->opts.debug != 0;
-    // 32-bit sort words unless the sorted indices are exported
-    const bool narrow = !debug && w <= spr::kNarrowMaxW;
+    // anti-aliased coverage reads each cell's sorted entries and their intersections
+    const bool aa = H->opts.aa != 0;
+    const bool keep = debug || aa;
+    // 32-bit sort words unless the sorted indices are exported or read
+    const bool narrow = !keep && w <= spr::kNarrowMaxW;
 
     if (record_begin) SPR_HIP(hipEventRecord(H->ev_begin, s));
     H->n_marks = 0;
@@ -437,7 +442,9 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->band_cnt.ensure((size_t)max_bands + 1);
     H->band_path.ensure((size_t)max_bands + 1);
     H->big_list.ensure(4 * (size_t)max_bands);  // oversized band ids, their winding prefixes, two lists of long bands
-    if (debug) { H->sidx.ensure(n1); H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
+    if (debug) { H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
+    if (keep) H->sidx.ensure(n1);
+    if (aa) { H->cell_first.ensure(2 * n1); H->cmask.ensure(2 * n1); }
     H->crec.ensure(2 * n1);
     H->image.ensure((size_t)4 * H->w * H->h);
     SPR_HIP(spr::launch_fill_u32((uint32_t*)(H->zero.p + z.a_words), 2 * (int64_t)(z.total - z.a_words), 0u, s));
@@ -477,7 +484,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         // chain solve, then piece merge and fragment keys in one pass
         const spr::FragArgs A{H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->tpos.p, w, h, nullptr,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err,
-                              debug ? H->inter.p : nullptr};
+                              keep ? H->inter.p : nullptr};
         solve_and_pieces(H, C, d_cc, A, (int64_t)n1, H->n_cu * H->chain_k / 8, H->n_cu * H->merge_k / 8);
     } else {
         stage_mark(H, 5);
@@ -509,13 +516,13 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         // band sort fused with the winding numbers (global exclusive scan of the sorted winding
         // changes, scanline_rasterizer.cpp:478-506)
         SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
-                                       debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
+                                       keep ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
                                        H->big_list.p + max_bands, d_bigcount, d_bsum,
                                        ctl_at(H, z.st_wind, z.ctr, 4).counter, H->n_cu, s,
                                        narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr,
                                        side_for(H, N, Np), N));
         SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
-                                           H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr,
+                                           H->sbig.p, H->stmp.p, H->skey.p, keep ? H->sidx.p : nullptr,
                                            H->winding.p, s, narrow ? H->band_pr.p : nullptr));
         if (narrow) {
             SPR_HIP(spr::launch_emit_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->skey.p, H->winding.p,
@@ -529,7 +536,12 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         SPR_HIP(spr::launch_mark_emit(H->skey.p, H->winding.p, H->epath.p, H->rule.p, H->fill.p, N, w, h,
                                       ctl_at(H, z.st_mark, z.ctr, 3), debug ? H->records.p : nullptr, H->crec.p,
                                       debug ? H->flags.p : nullptr, d_totals, H->bucket_base.p, H->prow.p, d_brec,
-                                      s, nullptr, nullptr, Np, (int64_t)(2 * n1), d_err));
+                                      s, nullptr, nullptr, Np, (int64_t)(2 * n1), d_err, nullptr, nullptr,
+                                      aa ? H->cell_first.p : nullptr));
+    if (aa)
+        SPR_HIP(spr::launch_cell_cover(H->cell_first.p, d_totals, (int64_t)(2 * n1), H->skey.p, H->sidx.p,
+                                       H->winding.p, H->epath.p, H->inter.p, N, H->tpos.p, H->cpos.p, H->ctype.p,
+                                       H->arcw(), H->rule.p, H->cmask.p, s));
     H->last_brec = d_brec;
     H->last_learned = false;
     H->last_fused = narrow;
@@ -538,7 +550,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     spr::PaintArgs pa;
     SPR_HIP(spr::launch_composite(H->crec.p, d_brec, H->bucket_base.p, H->prow.p, H->fill.p, Np, w, h,
                                   (int)H->opts.compose, H->image.p, s, (int64_t)(2 * n1), nullptr, 0, 0,
-                                  paint_args(H, pa)));
+                                  paint_args(H, pa), aa ? H->cmask.p : nullptr));
     stage_mark(H, 12);
     spr::SummarySrc S{};
     S.n = kSumCount;
@@ -1123,6 +1135,10 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
         H->w = width; H->h = height;
         if (opts) H->opts = *opts;
         if (H->opts.compose > 1 || H->opts.mode > 1) throw Fail{SPR_ERR_INVALID, "bad compose or mode"};
+        if (H->opts.aa != 0 && H->opts.aa != (uint32_t)spr::kAaSamples)
+            throw Fail{SPR_ERR_INVALID, "aa must be 0 or 16 (samples per pixel)"};
+        if (H->opts.aa && (H->opts.compose != SPR_COMPOSE_SRC_OVER || H->opts.mode != SPR_MODE_PLAIN))
+            throw Fail{SPR_ERR_INVALID, "anti-aliased coverage needs compose src-over and the plain mode"};
         int dev = 0;
         if (H->opts.device >= 0) dev = H->opts.device; else SPR_HIP(hipGetDevice(&dev));
         H->device = dev;
@@ -1166,6 +1182,7 @@ int spr_destroy(spr_handle* H) {
     H->heavy_list.release(); H->lo_list.release(); H->lo_cnt.release(); H->chain_cls.release(); H->mono.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
+    H->cell_first.release(); H->cmask.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
     H->band_start.release(); H->band_pr.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
     drop_graph(H);
@@ -1424,6 +1441,13 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 break;
             }
             case SPR_STAGE_IMAGE: from_dev(H->image.p, (size_t)4 * H->w * H->h); break;
+            case SPR_STAGE_COVERAGE: {
+                if (!H->opts.aa || H->last_learned || H->inj_rec_n >= 0)
+                    throw Fail{SPR_ERR_STATE, "sample coverage is computed by anti-aliased frames"};
+                int64_t R = (int64_t)H->sum(kSumCells) + H->sum(kSumSpans);
+                from_dev(H->cmask.p, (size_t)R * 8);
+                break;
+            }
             case SPR_STAGE_FEATURES:
                 if (!H->last_learned) throw Fail{SPR_ERR_STATE, "features are computed by learned frames"};
                 from_dev(H->feat.p, (size_t)H->Np * spr::kMlpIn * 4);
@@ -1496,6 +1520,7 @@ int spr_load_batch(spr_handle* H, const spr_scene* const* docs, const uint32_t* 
     if (!H || !docs || !widths || !heights) { g_error = "null argument"; return SPR_ERR_INVALID; }
     return guarded(H, [&] {
         if (n_docs == 0) throw Fail{SPR_ERR_INVALID, "empty batch"};
+        if (H->opts.aa) throw Fail{SPR_ERR_INVALID, "batches render without anti-aliased coverage"};
         std::vector<float> pos, cweight;
         std::vector<uint32_t> pos_path, cpos, ctype, cpath, rule, fill, path_paint;
         std::vector<spr::Paint> paints;
