@@ -354,6 +354,7 @@ def main(argv=None):
                     help="learned: paths of the training scene (same generator, seed 4)")
     ap.add_argument("--scan-n", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-aa", action="store_true", help="skip the anti-aliased frames of the same scene")
     ap.add_argument("--stages", action="store_true", help="print per-stage times to stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device-to-device copies of 1 GiB (the reference kernel that "
@@ -448,6 +449,17 @@ def main(argv=None):
         "roofline_hbm": roof_hbm,
         "stages": table,
     }
+    if rp is not None and compose == V.COMPOSE_SRC_OVER and not a.no_aa:
+        # the same frame with anti-aliased coverage (16 samples per pixel in the boundary cells)
+        ra = V.Rasterizer(canvas, canvas, compose=compose, mode=V.MODE_PLAIN, device=0, aa=V.AA_SAMPLES)
+        ra.load_scene(scene)
+        aa_ms, aa_dev = time_frames(ra, a.steps, a.warmup)
+        cov = ra.export("coverage")
+        out["aa"] = {"samples_per_pixel": V.AA_SAMPLES, "ms_per_step": aa_ms, "device_ms_per_frame": aa_dev,
+                     "Mpaths/s": Np / (aa_ms * 1e-3) / 1e6,
+                     "partial_cells": int((cov != np.uint64(0xFFFFFFFFFFFFFFFF)).sum()), "records": int(len(cov))}
+        ra.close()
+        del cov
     if rl is not None:
         sl = rl.stats()
         il = rl.render()
@@ -468,6 +480,25 @@ def main(argv=None):
             out["config"]["plain_ms_per_step"] = plain_ms
             out["config"]["learned_ms_per_step"] = learned_ms
             out["config"]["learned_speedup_vs_plain"] = plain_ms / learned_ms
+        # interactive view: a new MVP every frame (a 2-px pan, so the scene's counts stay put and the
+        # captured frame is replayed with the matrix copied from pinned memory inside the graph)
+        pan = np.eye(4, dtype=np.float32)
+        for k in range(a.warmup):
+            pan[0, 3] = 2.0 * (k + 1)
+            rl.set_mvp(pan)
+            rl.render(readback=False)
+        rl.synchronize()
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            pan[0, 3] = 2.0 * ((k + a.warmup) % 16 + 1)
+            rl.set_mvp(pan)
+            rl.render(readback=False)
+        rl.synchronize()
+        pan_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+        ls = rl.stats()
+        learned["interactive_pan"] = {"ms_per_step": pan_ms, "graph_captures": int(ls.graph_captures),
+                                      "fallback_frames": int(ls.learned_fallbacks)}
+        rl.set_mvp(np.eye(4, dtype=np.float32))
         out["learned"] = learned
         rl.close()
     if rp is not None:

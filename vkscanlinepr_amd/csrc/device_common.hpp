@@ -344,11 +344,10 @@ __device__ inline void wave_count(int32_t* __restrict__ cnt, uint32_t key, bool 
     if ((heads >> lane) & 1ull) atomicAdd(&cnt[key], run_length(lane, heads, vmask));
 }
 
-// gen_fragment.comp:90-224: cell key of the curve segment [t0, t1] (the key of its midpoint's
-// cell, or the off-canvas sentinel when t0 >= t1 or the cell row is off the canvas) and its
-// winding change across the cell's centre row.
-// The same from the segment's end points (pf at t0, pl at t1), for callers that share the
-// evaluation of a point between two consecutive segments.
+// gen_fragment.comp:90-224: cell key of the curve segment [t0, t1] from its end points (pf at t0,
+// pl at t1; callers share the evaluation of a point between two consecutive segments): the key
+// of its midpoint's cell, or the off-canvas sentinel when t0 >= t1 or the cell row is off the
+// canvas, and its winding change across the cell's centre row.
 __device__ inline void segment_cell_pts(bool forward, float pfx, float pfy, float plx, float ply, int w, int h,
                                         uint32_t& yx, int& swn) {
     yx = kSentinelKey;
@@ -365,11 +364,6 @@ __device__ inline void segment_cell_pts(bool forward, float pfx, float pfy, floa
     if (pfy == ply) swn = 0;
     else if (pfy < fwy && fwy <= ply) swn = -1;
     else if (ply < fwy && fwy <= pfy) swn = 1;
-}
-
-__device__ inline void segment_cell(const CurvePts& cv, float t0, float t1, int w, int h, uint32_t& yx, int& swn) {
-    if (!(t0 < t1)) { yx = kSentinelKey; swn = 0; return; }
-    segment_cell_pts(true, cv.ex(t0), cv.ey(t0), cv.ex(t1), cv.ey(t1), w, h, yx, swn);
 }
 
 }  // namespace spr

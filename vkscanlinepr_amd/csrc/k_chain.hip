@@ -620,6 +620,7 @@ __device__ inline void piece_fragments(const PieceMeta& m, const uint32_t* fin, 
             A.wn[i] = (int8_t)swn;
             A.epath[i] = m.path;
             if (A.inter) A.inter[i] = make_int2(m.pd.curve, (int32_t)f);
+            if (A.chord) A.chord[i] = make_float4(px, py, qx, qy);
             bucket = m.bb + bucket_local((int32_t)yx, m.pr, A.err);
         }
         wave_count(A.hist, bucket, valid);
@@ -777,6 +778,7 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
                 A.wn[i] = (int8_t)swn;
                 A.epath[i] = path;
                 if (A.inter) A.inter[i] = make_int2(pd.curve, (int32_t)fin0);
+                if (A.chord) A.chord[i] = make_float4(px, py, qx, qy);
                 const uint32_t bucket = bb + bucket_local((int32_t)yx, pr, A.err);
                 if (run_n > 0 && bucket != run_b) {
                     atomicAdd(&A.hist[run_b], run_n);

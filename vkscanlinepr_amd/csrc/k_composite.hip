@@ -25,9 +25,10 @@
 // evaluates them keeps a second row of pixels per thread.
 //
 // Anti-aliased coverage (src-over): a boundary cell record whose 64-sample mask (k_aa.hip) is not
-// full writes each of its pixels' covered-sample counts (both rows) into an LDS table indexed by
-// (path of the batch, column); the fold scales such a path's premultiplied colour by count/16.
-// Batches are then 8 paths (the table is 8 x 4096 x 2 bytes) and both rows are kept.
+// full appends, at each of its two columns, an entry (path of the batch, covered samples of row y,
+// of row y + 1) to that column's list in LDS (at most kAaSlots per column); the fold scales such
+// a path's premultiplied colour by count/16 in each row.  A batch whose lists would overflow is
+// redone with half as many paths (kAaSlots paths can never overflow).
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -39,7 +40,7 @@ constexpr int kBlock = 256;
 constexpr int kPix = 16;                   // consecutive pixels per thread
 constexpr int kChunk = kBlock * kPix;      // columns per pass
 constexpr int kBatch = 64;                 // paths per coverage mask
-constexpr int kAaBatch = 8;                // paths per batch with anti-aliased coverage
+constexpr int kAaSlots = 4;                // partial-coverage entries per column and batch
 constexpr int kList = 2 * kBlock;          // paths listed before their batches are folded
 constexpr int kUnroll = 4;                 // records loaded per thread before applying
 typedef unsigned long long u64;
@@ -183,8 +184,11 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                                       const PaintArgs pa,
                                                       const unsigned long long* __restrict__ cmask) {
     constexpr int TWO = (GRAD || AA) ? 1 : 0;            // the two rows of a cell row differ
-    constexpr int BATCH = AA ? kAaBatch : kBatch;
-    __shared__ uint16_t lcov[AA ? kAaBatch * kChunk : 1];  // 0x8000 | row y count | row y+1 count << 5
+    // partial coverage: entry k of column c at lslot[k * kChunk + ev_slot(c)], q | n0 << 6 | n1 << 11;
+    // byte c of lcnt counts the column's entries (a thread's 16 columns in one 16-byte word)
+    __shared__ uint16_t lslot[AA ? kAaSlots * kChunk : 1];
+    __shared__ uint32_t lcnt[AA ? kChunk / 4 : 1];
+    __shared__ int32_t lovf;
     __shared__ u64 ev[kChunk];
     __shared__ int2 list[kList];
     __shared__ uint32_t lcol[kList];
@@ -260,11 +264,13 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 n += add;
                 p0 += kBlock;
             }
-            for (int q0 = 0; q0 < n; q0 += BATCH) {
-                const int qn = n - q0 < BATCH ? n - q0 : BATCH;
+            int qn = 0;
+            for (int q0 = 0; q0 < n; q0 += qn) {
+                qn = n - q0 < kBatch ? n - q0 : kBatch;
+            batch:
                 if constexpr (AA != 0) {
-                    uint4* lc = reinterpret_cast<uint4*>(lcov);
-                    for (int k = tid; k < kAaBatch * kChunk / 8; k += kBlock) lc[k] = make_uint4(0, 0, 0, 0);
+                    for (int k = tid; k < kChunk / 4; k += kBlock) lcnt[k] = 0u;
+                    if (tid == 0) lovf = 0;
                 }
                 // clear the event words; the batch's colours; its records as one flat index space
 #pragma unroll
@@ -330,25 +336,62 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                     if (col < 0 || col >= cw) continue;
                                     const uint32_t n0 = __popcll((cm >> (16 * px)) & 0xFFFFull);
                                     const uint32_t n1 = __popcll((cm >> (16 * (2 + px))) & 0xFFFFull);
-                                    lcov[qi[u] * kChunk + ev_slot(col)] = (uint16_t)(0x8000u | n0 | (n1 << 5));
+                                    const int sl = ev_slot(col);
+                                    const uint32_t k = (atomicAdd(&lcnt[col >> 2], 1u << (8 * (col & 3))) >>
+                                                        (8 * (col & 3))) & 0xFFu;
+                                    if (k < (uint32_t)kAaSlots)
+                                        lslot[k * kChunk + sl] = (uint16_t)((uint32_t)qi[u] | (n0 << 6) | (n1 << 11));
+                                    else
+                                        lovf = 1;
                                 }
                             }
                         }
                     }
                 }
                 __syncthreads();
+                if constexpr (AA != 0) {
+                    if (lovf) {  // some column has more partial cells than slots: fewer paths
+                        qn = qn / 2;
+                        __syncthreads();
+                        goto batch;
+                    }
+                }
                 // coverage = prefix XOR of the events, then the fold in path order
                 u64 run = 0;
 #pragma unroll
                 for (int j = 0; j < kPix; ++j) run ^= ev[j * kBlock + tid];
                 u64 cov = block_exclusive_xor(run, xlds);
                 const u64 gmask = GRAD ? lgrad : 0ull;
+                uint32_t cnt4[4] = {0u, 0u, 0u, 0u};  // AA: the partial-entry counts of the 16 columns
+                if constexpr (AA != 0) {
+                    const uint4 c4 = reinterpret_cast<const uint4*>(lcnt)[tid];
+                    cnt4[0] = c4.x; cnt4[1] = c4.y; cnt4[2] = c4.z; cnt4[3] = c4.w;
+                }
 #pragma unroll
                 for (int j = 0; j < kPix; ++j) {
                     cov ^= ev[j * kBlock + tid];
                     u64 mm = cov;
                     if (mm == 0ull) continue;
                     const int x = c0 + xb + j;  // the pixel's canvas column
+                    // this pixel's partial-coverage entries (AA), and the paths they belong to
+                    uint32_t pe[AA ? kAaSlots : 1];
+                    uint32_t np = 0;
+                    u64 pm = 0ull;
+                    if constexpr (AA != 0) {
+                        np = (cnt4[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+#pragma unroll
+                        for (int k = 0; k < kAaSlots; ++k) pe[k] = 0u;
+                        if (np) {
+                            const int sl = j * kBlock + tid;
+#pragma unroll
+                            for (int k = 0; k < kAaSlots; ++k) {
+                                if ((uint32_t)k < np) {
+                                    pe[k] = (uint32_t)lslot[k * kChunk + sl];
+                                    pm |= 1ull << (pe[k] & 63u);
+                                }
+                            }
+                        }
+                    }
                     if constexpr (COMPOSE == 0) {
                         const int q = 63 - __clzll((long long)mm);
                         if (GRAD && ((gmask >> q) & 1ull)) {
@@ -377,10 +420,14 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                         s1 = premul(pc.y);
                                     }
                                     if constexpr (AA != 0) {
-                                        const uint32_t e = lcov[(32 * hw + q) * kChunk + j * kBlock + tid];
-                                        if (e & 0x8000u) {
-                                            s0 = scale(s0, (float)(e & 31u) / 16.0f);
-                                            s1 = scale(s1, (float)((e >> 5) & 31u) / 16.0f);
+                                        const uint32_t qq = (uint32_t)(32 * hw + q);
+                                        if ((pm >> qq) & 1ull) {
+                                            uint32_t e = 0;
+#pragma unroll
+                                            for (int k = 0; k < kAaSlots; ++k)
+                                                if ((uint32_t)k < np && (pe[k] & 63u) == qq) e = pe[k];
+                                            s0 = scale(s0, (float)((e >> 6) & 31u) / 16.0f);
+                                            s1 = scale(s1, (float)((e >> 11) & 31u) / 16.0f);
                                         }
                                     }
                                     blend(acc[j], s0);

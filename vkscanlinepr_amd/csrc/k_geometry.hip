@@ -168,7 +168,8 @@ __device__ inline uint32_t gen_one(const int2* __restrict__ inter, int64_t N, in
                                    const float2* __restrict__ tpos, int w,
                                    int h, int32_t* __restrict__ key, int8_t* __restrict__ wn,
                                    uint32_t* __restrict__ epath, const int32_t* __restrict__ bucket_base,
-                                   const int2* __restrict__ prow, int32_t* __restrict__ err) {
+                                   const int2* __restrict__ prow, int32_t* __restrict__ err,
+                                   float4* __restrict__ chord) {
     const int2 a = inter[i];
     const int2 b = (i + 1 != N) ? inter[i + 1] : make_int2(-1, 0x3f800000);
     float t0 = glmax(bitsf((uint32_t)a.y & 0xFFFFFFFCu), 0.0f);
@@ -178,8 +179,15 @@ __device__ inline uint32_t gen_one(const int2* __restrict__ inter, int64_t N, in
     if (a.x != b.x) t1 = 1.0f;
     uint32_t yx;
     int swn;
-    if (t0 < t1) segment_cell(load_curve(tpos, cpos[c], ctype[c], cweight, c), t0, t1, w, h, yx, swn);
-    else { yx = kSentinelKey; swn = 0; }
+    if (t0 < t1) {
+        const CurvePts cv = load_curve(tpos, cpos[c], ctype[c], cweight, c);
+        const float pfx = cv.ex(t0), pfy = cv.ey(t0), plx = cv.ex(t1), ply = cv.ey(t1);
+        segment_cell_pts(true, pfx, pfy, plx, ply, w, h, yx, swn);
+        if (chord) chord[i] = make_float4(pfx, pfy, plx, ply);
+    } else {
+        yx = kSentinelKey;
+        swn = 0;
+    }
     key[i] = (int32_t)yx;
     wn[i] = (int8_t)swn;
     epath[i] = path;
@@ -202,7 +210,8 @@ __global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict_
                                                          int32_t* __restrict__ hist,
                                                          int32_t* __restrict__ err,
                                                          const int32_t* __restrict__ n_dev,
-                                                         const int2* __restrict__ pcanvas) {
+                                                         const int2* __restrict__ pcanvas,
+                                                         float4* __restrict__ chord) {
     const int64_t n = n_dev ? min((int64_t)*n_dev, N) : N;
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     bool valid = i < n;
@@ -215,7 +224,8 @@ __global__ __launch_bounds__(kBlock) void k_gen_fragment(const int2* __restrict_
     if (valid) {
         int cw = w, ch = h;
         if (pcanvas) { const int2 cv = pcanvas[cpath[inter[i].x]]; cw = cv.x; ch = cv.y; }
-        bucket = gen_one(inter, n, i, cpos, ctype, cweight, cpath, tpos, cw, ch, key, wn, epath, bucket_base, prow, err);
+        bucket = gen_one(inter, n, i, cpos, ctype, cweight, cpath, tpos, cw, ch, key, wn, epath, bucket_base, prow, err,
+                         chord);
     }
     wave_count(hist, bucket, valid);
 }
@@ -280,10 +290,10 @@ hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpo
                                const uint32_t* ctype, const float* cweight, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
                                const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
-                               hipStream_t s, const int32_t* n_dev, const int2* pcanvas) {
+                               hipStream_t s, const int32_t* n_dev, const int2* pcanvas, float4* chord) {
     if (N == 0) return hipSuccess;
     k_gen_fragment<<<grid_for(N), kBlock, 0, s>>>(inter, N, cpos, ctype, cweight, cpath, tpos, w, h, key, wn, epath,
-                                                  bucket_base, prow, hist, err, n_dev, pcanvas);
+                                                  bucket_base, prow, hist, err, n_dev, pcanvas, chord);
     return hipGetLastError();
 }
 

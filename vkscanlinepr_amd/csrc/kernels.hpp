@@ -79,6 +79,7 @@ struct FragArgs {
     int32_t* hist;              // per-bucket entry counts
     int32_t* err;
     int2* inter;                // (curve, t bits) entries for export, or null
+    float4* chord;              // each entry's chord (pf.x, pf.y, pl.x, pl.y) for coverage, or null
 };
 // Chain length classes: cubic chains by quarter octave of their length (16 octaves x 4), then
 // line chains by octave.  The chain control block holds, as int32: class counts, class fill
@@ -142,7 +143,8 @@ hipError_t launch_gen_fragment(const int2* inter, int64_t N, const uint32_t* cpo
                                const uint32_t* ctype, const float* cweight, const uint32_t* cpath, const float2* tpos,
                                int w, int h, int32_t* key, int8_t* wn, uint32_t* epath,
                                const int32_t* bucket_base, const int2* prow, int32_t* hist, int32_t* err,
-                               hipStream_t s, const int32_t* n_dev = nullptr, const int2* pcanvas = nullptr);
+                               hipStream_t s, const int32_t* n_dev = nullptr, const int2* pcanvas = nullptr,
+                               float4* chord = nullptr);
 
 // ---- scans
 hipError_t launch_scan_i32(const int32_t* in, int32_t* out, int64_t n, ScanCtl ctl, hipStream_t s,
@@ -245,11 +247,11 @@ hipError_t launch_mark_emit(const int32_t* skey, const int32_t* winding, const u
 // pixels x 16 samples, bit (2 py + px) * 16 + j), ~0 for spans.  cell_first[o] = the record's
 // first sorted entry (-1: a span); the record count is totals[0] + totals[1] on the device.
 constexpr int kAaSamples = 16;
+// chord[i]: entry i's chord as the fragment stage formed it (FragArgs::chord).
 hipError_t launch_cell_cover(const int32_t* cell_first, const int32_t* totals, int64_t rec_cap,
                              const int32_t* skey, const int32_t* sidx, const int32_t* winding,
-                             const uint32_t* epath, const int2* inter, int64_t N, const float2* tpos,
-                             const uint32_t* cpos, const uint32_t* ctype, const float* cweight,
-                             const uint32_t* rule, unsigned long long* cmask, hipStream_t s);
+                             const uint32_t* epath, const float4* chord, int64_t N, const uint32_t* rule,
+                             unsigned long long* cmask, hipStream_t s);
 
 // ---- learned single-pass allocation (k_learned.hip)
 constexpr int kMlpIn = 16, kMlpHidden = 64, kMlpOut = 2;

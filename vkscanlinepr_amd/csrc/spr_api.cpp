@@ -169,6 +169,7 @@ struct spr_handle {
     DBuf<int4> records;     // full records, kept only with the debug option
     DBuf<int32_t> cell_first;            // anti-aliased frames: each record's first sorted entry
     DBuf<unsigned long long> cmask;      // anti-aliased frames: each record's sample coverage
+    DBuf<float4> chord;                  // anti-aliased frames: each entry's chord
     DBuf<uint32_t> crec;    // compact records (x | width << 16) read by the composite
     DBuf<int32_t> row_lo, row_hi, bucket_base, bflag, band_start, big_list;
     DBuf<int2> band_pr;
@@ -444,7 +445,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->big_list.ensure(4 * (size_t)max_bands);  // oversized band ids, their winding prefixes, two lists of long bands
     if (debug) { H->flags.ensure(2 * n1); H->records.ensure(2 * n1); }
     if (keep) H->sidx.ensure(n1);
-    if (aa) { H->cell_first.ensure(2 * n1); H->cmask.ensure(2 * n1); }
+    if (aa) { H->cell_first.ensure(2 * n1); H->cmask.ensure(2 * n1); H->chord.ensure(n1); }
     H->crec.ensure(2 * n1);
     H->image.ensure((size_t)4 * H->w * H->h);
     SPR_HIP(spr::launch_fill_u32((uint32_t*)(H->zero.p + z.a_words), 2 * (int64_t)(z.total - z.a_words), 0u, s));
@@ -484,7 +485,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         // chain solve, then piece merge and fragment keys in one pass
         const spr::FragArgs A{H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->tpos.p, w, h, nullptr,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err,
-                              keep ? H->inter.p : nullptr};
+                              keep ? H->inter.p : nullptr, aa ? H->chord.p : nullptr};
         solve_and_pieces(H, C, d_cc, A, (int64_t)n1, H->n_cu * H->chain_k / 8, H->n_cu * H->merge_k / 8);
     } else {
         stage_mark(H, 5);
@@ -492,7 +493,8 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     stage_mark(H, 6);
     if (!gen)  // injected entries (or none)
         SPR_HIP(spr::launch_gen_fragment(H->inter.p, N, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->tpos.p, w, h,
-                                         H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err, s));
+                                         H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err, s,
+                                         nullptr, nullptr, aa ? H->chord.p : nullptr));
     stage_mark(H, 7);
     // ---- sort: bucket offsets, bands, scatter into band order, band sort in LDS
     if (N > 0) {
@@ -540,8 +542,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
                                       aa ? H->cell_first.p : nullptr));
     if (aa)
         SPR_HIP(spr::launch_cell_cover(H->cell_first.p, d_totals, (int64_t)(2 * n1), H->skey.p, H->sidx.p,
-                                       H->winding.p, H->epath.p, H->inter.p, N, H->tpos.p, H->cpos.p, H->ctype.p,
-                                       H->arcw(), H->rule.p, H->cmask.p, s));
+                                       H->winding.p, H->epath.p, H->chord.p, N, H->rule.p, H->cmask.p, s));
     H->last_brec = d_brec;
     H->last_learned = false;
     H->last_fused = narrow;
@@ -1182,7 +1183,7 @@ int spr_destroy(spr_handle* H) {
     H->heavy_list.release(); H->lo_list.release(); H->lo_cnt.release(); H->chain_cls.release(); H->mono.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
-    H->cell_first.release(); H->cmask.release();
+    H->cell_first.release(); H->cmask.release(); H->chord.release();
     H->row_lo.release(); H->row_hi.release(); H->bucket_base.release(); H->bflag.release();
     H->band_start.release(); H->band_pr.release(); H->big_list.release(); H->prow.release(); H->image.release(); H->zero.release();
     drop_graph(H);

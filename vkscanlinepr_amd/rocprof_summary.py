@@ -26,21 +26,28 @@ STAGE_KERNELS = {
     "piece_fragments": ["k_piece_small", "k_piece_fragments", "k_piece_heavy"],
     "gen_fragment": ["k_gen_fragment"],
     "bucket_scatter": ["k_bucket_scatter", "k_band_flags", "k_band_emit"],
-    "band_sort": ["k_band_classify", "k_sort_bands", "k_sort_big_bands", "k_emit_big_bands", "k_band_totals"],
+    "band_sort": ["k_band_classify2", "k_sort_bands_wave", "k_sort_big_bands", "k_emit_big_bands",
+                  "k_band_totals"],
     # with mark + emit fused into the band sort (frames that export no stage), the whole sort
-    "sort": ["k_bucket_scatter", "k_band_flags", "k_band_emit", "k_band_classify", "k_sort_bands",
+    "sort": ["k_bucket_scatter", "k_band_flags", "k_band_emit", "k_band_classify2", "k_sort_bands_wave",
              "k_sort_big_bands", "k_emit_big_bands", "k_band_totals", "k_mark_emit"],
     "mark_emit": ["k_mark_emit"],
+    "cell_cover": ["k_cell_cover"],
     "composite": ["k_composite"],
+    "scan": ["k_scan_i32"],
 }
 
 
-def kernel_name(full: str) -> str:
+def kernel_name(full: str, templated: bool = False) -> str:
     s = re.sub(r"\(anonymous namespace\)::", "", full)
     s = re.sub(r"^void ", "", s)
     s = s.split("(")[0]
-    s = s.split("<")[0]
-    return s.split("::")[-1]
+    args = ""
+    if "<" in s:
+        args = "<" + s.split("<", 1)[1]
+        s = s.split("<")[0]
+    s = s.split("::")[-1]
+    return s + re.sub(r"spr::", "", args) if templated else s
 
 
 def _per_kernel(dirpath: Path, counter: str):
@@ -91,7 +98,7 @@ def kernels(stats_csv, title, command):
            "rocprofv3 kernel-trace summary (per-kernel device time):", "",
            "| kernel | calls | avg us | min us | max us | % |", "|---|---:|---:|---:|---:|---:|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
-        out.append(f"| {kernel_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+        out.append(f"| {kernel_name(r['Name'], True)} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                    f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
                    f"{100 * float(r['TotalDurationNs']) / total:.1f} |")
     return "\n".join(out) + "\n"
