@@ -81,7 +81,8 @@ typedef struct spr_stats {
     int64_t n_wave_chains;     /* crossing chains of 1024 or more, each solved by a whole wave */
     int64_t n_heavy_pieces;    /* monotone pieces merged by a 256-thread workgroup each (more than
                                   1024 entries, or waiting on a chain of 512 or more crossings) */
-    int64_t n_long_bands;      /* sort bands of more than 512 words (256-thread LDS sorts) */
+    int64_t n_long_bands;      /* sort bands of more than 512 words (one wave at 16 or 32 words per
+                                  lane; 256-thread LDS sorts in frames that export a stage) */
     int64_t n_big_bands;       /* sort bands of more than 2048 words (global merge-path sort) */
     int64_t graph_captures;    /* LEARNED and batches: frames captured as a HIP graph so far (a new
                                   scene, model, margin, tuning or budget needs a new capture; a

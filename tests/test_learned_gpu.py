@@ -162,8 +162,9 @@ def test_margin_change_after_capture(spr, trained):
 
 @pytest.mark.skip(reason="this case (a learned frame whose budget was exceeded, so that the device "
                          "dropped paths and the host redid the frame with the plain method) faulted "
-                         "the GPU twice; that code path is removed (budgets are now a proven upper "
-                         "bound, an exceeded budget is an error) and the case stays off the GPU")
+                         "the GPU twice; that code path (the per-path drop and the overflow pool) is "
+                         "removed, a frame over budget is now culled before any entry is written "
+                         "(test_over_budget_frame_renders_plain), and this case stays off the GPU")
 def test_budget_growth_falls_back_to_plain(spr, trained):
     r, _, path = trained
     scene = spr.Scene.generate(2, 6, 200, 1024)

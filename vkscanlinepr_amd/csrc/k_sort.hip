@@ -24,10 +24,13 @@
 // through global memory, the last pass writing keys and winding numbers.
 //
 // Frames that export no stage sort the narrow 32-bit words of the scatter instead (bucket inside
-// the path, x, winding change: see k_bucket.hip), with the same kernels; the keys are rebuilt
-// from the band's path rows.  These frames also run mark + emit (mark_merged_fragment_and_span
+// the path, x, winding change: see k_bucket.hip); the keys are rebuilt from the band's path rows.
+// Each band of up to 2048 narrow words is sorted by one wave in registers (k_sort_bands_wave: a
+// bitonic network generated at compile time, in-lane for short strides, DPP and ds_bpermute
+// across lanes), in three size classes of 8, 16 and 32 words per lane; k_band_classify2 lists the
+// bands of the two larger classes.  These frames also run mark + emit (mark_merged_fragment_and_span
 // .comp, the 2N flag scan and gen_merged_fragment_and_span.comp) inside the band sort, on the
-// sorted words and winding numbers still in LDS: every flag of a band depends on its own entries
+// sorted words and winding numbers still in registers: every flag of a band depends on its own entries
 // only (a band starts at a bucket, i.e. a new cell row), so each band writes its records at twice
 // its start, the record range of each of its buckets and its (cells, spans); the oversized bands
 // do the same from their sorted keys (k_emit_big_bands).  Debug frames write keys, indices and
