@@ -255,7 +255,7 @@ struct CurveAxis {
 // Quadratics and arcs (TODO in the reference) take the cubic's bisection on their own
 // evaluation.
 template <uint32_t T>
-__device__ inline float bisect_axis(const CurveAxis& ax, float c, float t_min, float t1) {
+__host__ __device__ inline float bisect_axis(const CurveAxis& ax, float c, float t_min, float t1) {
     float t0 = t_min;
     const float c0 = ax.c[0], c1 = ax.c[1], c2 = ax.c[2], c3 = ax.c[3];
     float vt0 = eval_axis(T, t0, c0, c1, c2, c3);
@@ -273,7 +273,77 @@ __device__ inline float bisect_axis(const CurveAxis& ax, float c, float t_min, f
     return fabsf(vtm - c) > 1.0f ? t_min : tm;
 }
 
-__device__ inline float solve_line(const CurveAxis& ax, float c, float t_min, float t1) {
+// The chain solves take a shorter, bit-identical form of bisect_axis for cubics and quadratics.
+//   (1) Every control value is finite with |c_i| < 2^17 (fast_axis).  Values are then finite,
+//       and the sign bit of fl(v - c) is set exactly when v < c (fl(v - v) = +0).  So with the
+//       start below the line a node turns right where v < c, and with the start above it where
+//       v >= c (a value on the line counts as above).
+//   (2) Evaluation commutes exactly with negation (lerp(-a, -b, t) = -lerp(a, b, t): every
+//       rounding is symmetric), so a lane-solved chain stepping down runs on the negated axis,
+//       where "v >= c" is "v' <= c'", that is v' < next_up(c'): the same single compare as a
+//       chain stepping up (bisect_below).
+//   (3) Within a chain, the start side is known after a crossing that did not fall back.  The
+//       value at that crossing is within 1 of its grid line c - step.  The next bracket starts
+//       at that parameter with its two low bits replaced (at most 3 * 2^-23 away; fast_bracket
+//       keeps brackets in [0, 1 + 2^-20]).  The float evaluation differs from the exact
+//       polynomial by at most 15.01 * 2^-24 * M (M = max |c_i|: three levels of lerps, 5 u M
+//       each), and |B'| <= 6M.  So the start value lies within 1 + 66.01 * 2^-24 * M < 1.52 of
+//       c - step, |step| = 2: on the side the chain comes from, never on the line.
+constexpr float kFastMaxCoord = 131072.0f;               // 2^17
+constexpr float kFastMaxT = 1.00000095367431640625f;     // 1 + 2^-20
+
+__host__ __device__ inline bool fast_axis(uint32_t type, const CurveAxis& ax) {
+    return (type == kCubic || type == kQuad) && fabsf(ax.c[0]) < kFastMaxCoord &&
+           fabsf(ax.c[1]) < kFastMaxCoord && fabsf(ax.c[2]) < kFastMaxCoord && fabsf(ax.c[3]) < kFastMaxCoord;
+}
+
+// t in [+0, 1 + 2^-20] (negative values, -0 and NaN fail): one unsigned compare.
+__host__ __device__ inline bool fast_t(float t) { return fbits(t) <= fbits(kFastMaxT); }
+
+__host__ __device__ inline bool fast_bracket(float t_min, float t1) { return fast_t(t_min) && fast_t(t1); }
+
+__host__ __device__ inline CurveAxis negated(const CurveAxis& a) {
+    CurveAxis n;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) n.c[i] = -a.c[i];
+    return n;
+}
+
+// The least float above a finite x (above -0 and +0: the least positive denormal; the kernels
+// keep fp32 denormals).
+__host__ __device__ inline float next_up(float x) {
+    const uint32_t b = fbits(x);
+    if (b == 0x80000000u) return bitsf(1u);
+    return bitsf((int32_t)b >= 0 ? b + 1u : b - 1u);
+}
+
+// bisect_axis under (1), with every decision "v < cut" (cut = c for a start below the line,
+// next_up(c) on the negated axis); c itself decides the fallback to t_min (|B(tm) - c| > 1),
+// reported in *ok.
+// FULL unrolls the 24 steps (the straight-line form of the common path).
+template <uint32_t T, bool FULL = false>
+__host__ __device__ inline float bisect_below(const CurveAxis& ax, float cut, float c, float t_min, float t1, bool* ok) {
+    const float c0 = ax.c[0], c1 = ax.c[1], c2 = ax.c[2], c3 = ax.c[3];
+    float t0 = t_min, tm = t_min, vtm = 0.0f;
+    auto step = [&]() {
+        tm = (t0 + t1) * 0.5f;
+        vtm = eval_axis(T, tm, c0, c1, c2, c3);
+        const bool right = vtm < cut;
+        t0 = right ? tm : t0;
+        t1 = right ? t1 : tm;
+    };
+    if constexpr (FULL) {
+#pragma unroll
+        for (int j = 0; j < kCubicIters; ++j) step();
+    } else {
+#pragma unroll 4
+        for (int j = 0; j < kCubicIters; ++j) step();
+    }
+    *ok = !(fabsf(vtm - c) > 1.0f);
+    return *ok ? tm : t_min;
+}
+
+__host__ __device__ inline float solve_line(const CurveAxis& ax, float c, float t_min, float t1) {
     float a = ax.c[1] - ax.c[0];
     a = (a != 0.0f ? (1.0f / a) : 0.0f);
     return glmin(glmax((c - ax.c[0]) * a, t_min), t1);
@@ -285,6 +355,55 @@ __device__ inline float solve_axis(uint32_t type, const CurveAxis& ax, float c, 
     if (type == kQuad) return bisect_axis<kQuad>(ax, c, t_min, t1);
     return bisect_axis<kArc>(ax, c, t_min, t1);
 }
+
+// One lane's chain of crossings on one axis (make_intersection_1.comp:377-437 per crossing), in
+// order: next() returns the raw parameter of the crossing with the given line, bracketed by the
+// previous crossing (flagged) and the piece end.  Cubics and quadratics that satisfy (1) take
+// bisect_below, on the negated axis when the chain steps down, and skip the start evaluation
+// when (3) gives the start side; everything else takes solve_line or bisect_axis.  common()
+// names the usual case (such a chain, start side known, t_min in range), for which
+// next_common() is the same computation in straight-line form: a wave whose lanes are all in
+// that case runs it without divergent branches.
+template <uint32_t T>
+struct ChainSolver {
+    CurveAxis a, an;
+    bool fast, neg, known;
+    __host__ __device__ ChainSolver(const CurveAxis& ax, float step, float t1) : a(ax) {
+        fast = fast_axis(T, ax) && fast_t(t1);
+        neg = fast && step < 0.0f;
+        an = neg ? negated(ax) : ax;
+        known = false;  // the next crossing starts on the side the chain comes from
+    }
+    __host__ __device__ bool common(float t_min) const { return T != kLine && fast && known && fast_t(t_min); }
+    __host__ __device__ float next_common(float line, float t_min, float t1) {
+        const float cn = neg ? -line : line;
+        return bisect_below<T, true>(an, neg ? next_up(cn) : cn, cn, t_min, t1, &known);
+    }
+    __host__ __device__ float next(float line, float t_min, float t1) {
+        if (T == kLine) return solve_line(a, line, t_min, t1);
+        if (!fast) return bisect_axis<T>(a, line, t_min, t1);
+        bool run = true, from = true;
+        const bool fb = fast_t(t_min);  // t1 was checked with the axis
+        if (!(fb && known)) {
+            const float vt0 = eval_axis(T, t_min, a.c[0], a.c[1], a.c[2], a.c[3]);
+            run = vt0 != line;
+            from = neg ? vt0 > line : vt0 < line;
+        }
+        float ts = t_min;
+        known = fb;  // a start on the line itself is a crossing with no fallback
+        if (run) {
+            if (from) {
+                const float cn = neg ? -line : line;
+                ts = bisect_below<T>(an, neg ? next_up(cn) : cn, cn, t_min, t1, &known);
+                known = known && fb;
+            } else {
+                ts = bisect_axis<T>(a, line, t_min, t1);
+                known = false;
+            }
+        }
+        return ts;
+    }
+};
 
 // Bucket of an entry inside its path's (path, cell row) bucket range.  pr = (first row >= 1,
 // number of rows): rows first..first+nrows-1 come first, then the off-canvas bucket, then row 0,

@@ -249,10 +249,13 @@ template <uint32_t T>
 __device__ inline void solve_chain_lane_t(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
                                           uint8_t* __restrict__ mono, float* __restrict__ stage) {
     float tprev = d.t0, line = d.line0;
-    bool up = true, down = true;
+    uint32_t rises = 0, falls = 0;  // the chain's values go up / down somewhere
     const uint32_t side = (uint32_t)d.side;
+    ChainSolver<T> solver(a, d.step, d.t1);
     for (int k = 0; k < d.n; ++k) {
-        const float ts = T == kLine ? solve_line(a, line, tprev, d.t1) : bisect_axis<T>(a, line, tprev, d.t1);
+        // the usual case in every lane: the straight-line form, no divergent branches
+        const float ts = __all(solver.common(tprev)) ? solver.next_common(line, tprev, d.t1)
+                                                     : solver.next(line, tprev, d.t1);
         const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | side);
         stage[(k & 15) * kBlock] = v;
         if ((k & 15) == 15) {
@@ -264,11 +267,13 @@ __device__ inline void solve_chain_lane_t(const ChainDesc& d, const CurveAxis& a
             for (int j = 0; j <= (k & 15); ++j) dst[j] = stage[j * kBlock];
         }
         // the y chain is preceded by the piece start in the merge, the x chain is not
-        if (k > 0 || side) { up = up && v >= tprev; down = down && v <= tprev; }
+        const float prev = k > 0 || side ? tprev : v;
+        falls |= !(v >= prev) ? 1u : 0u;  // as the flags of the reference order: NaN breaks both
+        rises |= !(v <= prev) ? 1u : 0u;
         tprev = v;
         line += d.step;
     }
-    *mono = (up ? 1 : 0) | (down ? 2 : 0);
+    *mono = (falls ? 0 : 1) | (rises ? 0 : 2);
 }
 
 __device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
@@ -288,66 +293,103 @@ __device__ inline void solve_chain_lane(const ChainDesc& d, const CurveAxis& a, 
 // depth-5 node (lanes 32..63) lies on that path when D holds, at its five ancestors, the turns
 // that lead to it: one masked compare per lane and a second ballot name the node, whose
 // interval, midpoint and value are then read from its lane.
-template <uint32_t T>
-__device__ inline void solve_chain_wave_t(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
-                                          uint8_t* __restrict__ mono) {
-    const int lane = threadIdx.x & 63;
-    const int depth = lane ? 31 - __clz(lane) : 0;
+struct WaveTree {
     bool on[kLevels - 1], go_right[kLevels - 1];
-    unsigned long long anc_mask = 0ull, anc_turn = 1ull;  // inner nodes never match
+    unsigned long long anc_mask, anc_turn;
+};
+
+__device__ inline WaveTree wave_tree(int lane) {
+    WaveTree W;
+    const int depth = lane ? 31 - __clz(lane) : 0;
+    W.anc_mask = 0ull;
+    W.anc_turn = 1ull;  // inner nodes never match
 #pragma unroll
     for (int lev = 0; lev < kLevels - 1; ++lev) {
-        on[lev] = lev < depth;
-        go_right[lev] = on[lev] && ((lane >> (depth - 1 - lev)) & 1);
+        W.on[lev] = lev < depth;
+        W.go_right[lev] = W.on[lev] && ((lane >> (depth - 1 - lev)) & 1);
     }
     if (depth == kLevels - 1) {
-        anc_turn = 0ull;
+        W.anc_turn = 0ull;
 #pragma unroll
         for (int lev = 0; lev < kLevels - 1; ++lev) {
             const int anc = lane >> (kLevels - 1 - lev);  // ancestor at depth lev
-            anc_mask |= 1ull << anc;
-            anc_turn |= (unsigned long long)((lane >> (kLevels - 2 - lev)) & 1) << anc;
+            W.anc_mask |= 1ull << anc;
+            W.anc_turn |= (unsigned long long)((lane >> (kLevels - 2 - lev)) & 1) << anc;
         }
     }
+    return W;
+}
+
+// The 24 levels of one crossing's bisection on [lo, hi]: tm and vtm of the last level.  A node
+// turns right where the sign of v - c is s0 (MODE 0), or, with every value finite, where v < c
+// for a start below the line (MODE 1) or v >= c for a start above it (MODE 2).
+template <uint32_t T, int MODE>
+__device__ __forceinline__ void wave_bisect(const CurveAxis& a, float c, float lo, float hi, uint32_t s0,
+                                            const WaveTree& W, float& tm, float& vtm) {
+#pragma unroll
+    for (int round = 0; round < kRounds; ++round) {
+        float llo = lo, lhi = hi;
+#pragma unroll
+        for (int lev = 0; lev < kLevels - 1; ++lev) {
+            const float mid = (llo + lhi) * 0.5f;
+            llo = W.go_right[lev] ? mid : llo;
+            lhi = (W.on[lev] && !W.go_right[lev]) ? mid : lhi;
+        }
+        const float mym = (llo + lhi) * 0.5f;
+        const float myv = eval_axis(T, mym, a.c[0], a.c[1], a.c[2], a.c[3]);
+        // lane 0 is no node: its bit is never read
+        const bool right = MODE == 1 ? myv < c : (MODE == 2 ? !(myv < c) : (fbits(myv - c) & 0x80000000u) == s0);
+        // the interval this node leaves for the next round
+        const float nlo = right ? mym : llo, nhi = right ? lhi : mym;
+        const unsigned long long D = __ballot(right);
+        const unsigned long long P = __ballot((D & W.anc_mask) == W.anc_turn);
+        const int node = __builtin_amdgcn_readfirstlane(__builtin_ctzll(P));
+        lo = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(nlo), node));
+        hi = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(nhi), node));
+        if (round == kRounds - 1) {
+            tm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(mym), node));
+            vtm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(myv), node));
+        }
+    }
+}
+
+// DIRECT (every control value finite and below 2^17 in magnitude): a crossing whose start lies
+// on the side the chain comes from takes the single-compare rounds, and a crossing after one
+// that did not fall back starts there without evaluating at t_min (see bisect_below).
+template <uint32_t T, bool DIRECT>
+__device__ inline void solve_chain_wave_t(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
+                                          uint8_t* __restrict__ mono) {
+    const int lane = threadIdx.x & 63;
+    const WaveTree W = wave_tree(lane);
     const uint32_t side = (uint32_t)d.side;
+    const bool neg = d.step < 0.0f;
     float tprev = d.t0, line = d.line0;
     bool up = true, down = true;
     float kept = 0.0f;
+    bool known = false;  // this crossing starts on the side the chain comes from
     for (int k = 0; k < d.n; ++k) {
         const float c = line;
         const float t_min = tprev;
-        const float vt0 = eval_axis(T, t_min, a.c[0], a.c[1], a.c[2], a.c[3]);
+        const bool fb = DIRECT && fast_bracket(t_min, d.t1);
+        bool run = true, from = true;
+        uint32_t s0 = neg ? 0u : 0x80000000u;
+        if (!(fb && known)) {
+            // the same value in every lane: uniform branches below
+            const float vt0 = eval_axis(T, t_min, a.c[0], a.c[1], a.c[2], a.c[3]);
+            run = __builtin_amdgcn_readfirstlane(vt0 != c ? 1 : 0) != 0;
+            from = __builtin_amdgcn_readfirstlane((neg ? vt0 > c : vt0 < c) ? 1 : 0) != 0;
+            s0 = fbits(vt0 - c) & 0x80000000u;
+        }
         float ts = t_min;
-        if (vt0 != c) {
-            const uint32_t s0 = fbits(vt0 - c) & 0x80000000u;
-            float lo = t_min, hi = d.t1;
-            float tm = t_min, vtm = vt0;
-#pragma unroll
-            for (int round = 0; round < kRounds; ++round) {
-                float llo = lo, lhi = hi;
-#pragma unroll
-                for (int lev = 0; lev < kLevels - 1; ++lev) {
-                    const float mid = (llo + lhi) * 0.5f;
-                    llo = go_right[lev] ? mid : llo;
-                    lhi = (on[lev] && !go_right[lev]) ? mid : lhi;
-                }
-                const float mym = (llo + lhi) * 0.5f;
-                const float myv = eval_axis(T, mym, a.c[0], a.c[1], a.c[2], a.c[3]);
-                // lane 0 is no node: its bit is never read
-                const bool right = (fbits(myv - c) & 0x80000000u) == s0;
-                // the interval this node leaves for the next round
-                const float nlo = right ? mym : llo, nhi = right ? lhi : mym;
-                const unsigned long long D = __ballot(right);
-                const unsigned long long P = __ballot((D & anc_mask) == anc_turn);
-                const int node = __builtin_amdgcn_readfirstlane(__builtin_ctzll(P));
-                lo = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(nlo), node));
-                hi = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(nhi), node));
-                if (round == kRounds - 1) {
-                    tm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(mym), node));
-                    vtm = bitsf((uint32_t)__builtin_amdgcn_readlane((int)fbits(myv), node));
-                }
-            }
-            ts = fabsf(vtm - c) > 1.0f ? t_min : tm;
+        known = fb;  // a start on the line itself is a crossing with no fallback
+        if (run) {
+            float tm = t_min, vtm = 0.0f;
+            if (DIRECT && from && !neg) wave_bisect<T, DIRECT ? 1 : 0>(a, c, t_min, d.t1, s0, W, tm, vtm);
+            else if (DIRECT && from) wave_bisect<T, DIRECT ? 2 : 0>(a, c, t_min, d.t1, s0, W, tm, vtm);
+            else wave_bisect<T, 0>(a, c, t_min, d.t1, s0, W, tm, vtm);
+            const bool fell = fabsf(vtm - c) > 1.0f;
+            ts = fell ? t_min : tm;
+            known = fb && from && !fell;
         }
         const float v = bitsf((fbits(ts) & 0xFFFFFFFCu) | side);
         // lane k % 64 keeps value k; 64 values are stored at once (coalesced)
@@ -366,9 +408,16 @@ __device__ inline void solve_chain_wave_t(const ChainDesc& d, const CurveAxis& a
 __device__ inline void solve_chain_wave(const ChainDesc& d, const CurveAxis& a, float* __restrict__ out,
                                         uint8_t* __restrict__ mono) {
     const uint32_t type = (uint32_t)d.type;
-    if (type == kCubic) solve_chain_wave_t<kCubic>(d, a, out, mono);
-    else if (type == kQuad) solve_chain_wave_t<kQuad>(d, a, out, mono);
-    else solve_chain_wave_t<kArc>(d, a, out, mono);
+    const bool direct = fast_axis(type, a);
+    if (type == kCubic) {
+        if (direct) solve_chain_wave_t<kCubic, true>(d, a, out, mono);
+        else solve_chain_wave_t<kCubic, false>(d, a, out, mono);
+    } else if (type == kQuad) {
+        if (direct) solve_chain_wave_t<kQuad, true>(d, a, out, mono);
+        else solve_chain_wave_t<kQuad, false>(d, a, out, mono);
+    } else {
+        solve_chain_wave_t<kArc, false>(d, a, out, mono);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict__ tpos,
