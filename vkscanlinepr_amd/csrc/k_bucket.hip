@@ -194,21 +194,22 @@ __global__ void k_band_emit(const int32_t* __restrict__ flag_scan, const int32_t
                             int2* __restrict__ band_pr, int32_t* __restrict__ band_path) {
     const int32_t NB = nb_dev ? min(*nb_dev, NB_max) : NB_max;
     const int64_t N = n_dev ? min((int64_t)*n_dev, N_max) : N_max;
-    const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (b >= NB) return;
-    if (flag_scan[b + 1] != flag_scan[b]) {
-        band_start[flag_scan[b]] = boff[b];
-        if (band_pr) {
-            uint32_t lo = 0, hi = Np;  // bucket_base[lo] <= b < bucket_base[hi]
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (bucket_base[mid] <= b) lo = mid; else hi = mid;
+    // grid-stride over the live buckets (a launch for a bucket budget is capped)
+    for (int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x; b < NB; b += (int64_t)gridDim.x * kBlock) {
+        if (flag_scan[b + 1] != flag_scan[b]) {
+            band_start[flag_scan[b]] = boff[b];
+            if (band_pr) {
+                uint32_t lo = 0, hi = Np;  // bucket_base[lo] <= b < bucket_base[hi]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (bucket_base[mid] <= b) lo = mid; else hi = mid;
+                }
+                band_pr[flag_scan[b]] = prow[lo];
+                if (band_path) band_path[flag_scan[b]] = (int32_t)lo;
             }
-            band_pr[flag_scan[b]] = prow[lo];
-            if (band_path) band_path[flag_scan[b]] = (int32_t)lo;
         }
+        if (b == NB - 1) band_start[flag_scan[NB]] = (int32_t)N;
     }
-    if (b == NB - 1) band_start[flag_scan[NB]] = (int32_t)N;
 }
 
 }  // namespace
@@ -249,8 +250,10 @@ hipError_t launch_band_emit(const int32_t* flag_scan, const int32_t* boff, int32
                             const int32_t* bucket_base, uint32_t Np, const int2* prow, int2* band_pr,
                             int32_t* band_path) {
     if (NB == 0) return hipSuccess;
-    k_band_emit<<<grid_for(NB), kBlock, 0, s>>>(flag_scan, boff, NB, N, band_start, nb_dev, n_dev, bucket_base, Np,
-                                                prow, band_pr, band_path);
+    unsigned g = grid_for(NB);
+    if (g > 2048u) g = 2048u;  // the kernel strides over the live buckets
+    k_band_emit<<<g, kBlock, 0, s>>>(flag_scan, boff, NB, N, band_start, nb_dev, n_dev, bucket_base, Np, prow,
+                                     band_pr, band_path);
     return hipGetLastError();
 }
 

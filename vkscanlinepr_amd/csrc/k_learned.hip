@@ -60,7 +60,9 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
                                                        int w, int h, float* __restrict__ feat,
                                                        unsigned long long* __restrict__ bound_total,
                                                        const int2* __restrict__ pcanvas,
-                                                       int32_t* __restrict__ pbound) {
+                                                       int32_t* __restrict__ pbound,
+                                                       const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;  // the prediction of an earlier frame still holds
     __shared__ unsigned long long s_bound[4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int w0 = w, h0 = h;
@@ -150,7 +152,8 @@ __global__ __launch_bounds__(256) void k_path_features(const float2* __restrict_
 // Operand layout of the 16x16x16 BF16 MFMA: lane l holds A[l%16][4*(l/16)+q], B[4*(l/16)+q][l%16]
 // (q = 0..3) and receives D[4*(l/16)+q][l%16].
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ feat, uint32_t Np, MlpDev m,
-                                                   float* __restrict__ pred) {
+                                                   float* __restrict__ pred, const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ float act[4][16][kH + 1];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
@@ -223,7 +226,9 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ fea
 // to `arena` (scanned afterwards into arena bases); the record capacities are summed.
 __global__ void k_caps(const float* __restrict__ pred, const int32_t* __restrict__ visible, uint32_t Np,
                        float log_margin, int32_t slack, int32_t* __restrict__ arena, int32_t* __restrict__ cap_n,
-                       int32_t* __restrict__ cap_r_total, const int32_t* __restrict__ pbound) {
+                       int32_t* __restrict__ cap_r_total, const int32_t* __restrict__ pbound,
+                       const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t cr = 0;
     if (p < Np) {
@@ -248,7 +253,9 @@ __global__ void k_caps(const float* __restrict__ pred, const int32_t* __restrict
 
 // Paths whose exact entry count exceeds their predicted capacity: one atomic per wave.
 __global__ void k_path_overflow(const int32_t* __restrict__ seg, const int32_t* __restrict__ cap_n,
-                                const int32_t* __restrict__ visible, uint32_t Np, int32_t* __restrict__ n_over) {
+                                const int32_t* __restrict__ visible, uint32_t Np, int32_t* __restrict__ n_over,
+                                const int32_t* __restrict__ skip) {
+    if (skip && *skip) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     const bool over = p < Np && visible_mask(visible[p]) && seg[p + 1] - seg[p] > cap_n[p];
     const unsigned long long m = __ballot(over);
@@ -380,30 +387,33 @@ inline unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) /
 hipError_t launch_path_features(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                                 const uint32_t* pcurve, const int32_t* visible, const uint32_t* rule,
                                 uint32_t Np, int w, int h, float* feat, hipStream_t s,
-                                unsigned long long* bound_total, const int2* pcanvas, int32_t* pbound) {
+                                unsigned long long* bound_total, const int2* pcanvas, int32_t* pbound,
+                                const int32_t* skip) {
     if (Np == 0) return hipSuccess;
     k_path_features<<<(Np + 15) / 16, 256, 0, s>>>(tpos, cpos, ctype, pcurve, visible, rule, Np, w, h, feat,
-                                                   bound_total, pcanvas, pbound);
+                                                   bound_total, pcanvas, pbound, skip);
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, float* pred, hipStream_t s) {
+hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, float* pred, hipStream_t s,
+                            const int32_t* skip) {
     if (Np == 0) return hipSuccess;
-    k_mlp_infer<<<(Np + 63) / 64, 256, 0, s>>>(feat, Np, m, pred);
+    k_mlp_infer<<<(Np + 63) / 64, 256, 0, s>>>(feat, Np, m, pred, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, float log_margin, int32_t slack,
-                       int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s, const int32_t* pbound) {
+                       int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s, const int32_t* pbound,
+                       const int32_t* skip) {
     if (Np == 0) return hipSuccess;
-    k_caps<<<blocks(Np), 256, 0, s>>>(pred, visible, Np, log_margin, slack, arena, cap_n, cap_r_total, pbound);
+    k_caps<<<blocks(Np), 256, 0, s>>>(pred, visible, Np, log_margin, slack, arena, cap_n, cap_r_total, pbound, skip);
     return hipGetLastError();
 }
 
 hipError_t launch_path_overflow(const int32_t* seg, const int32_t* cap_n, const int32_t* visible, uint32_t Np,
-                                int32_t* n_over, hipStream_t s) {
+                                int32_t* n_over, hipStream_t s, const int32_t* skip) {
     if (Np == 0) return hipSuccess;
-    k_path_overflow<<<blocks(Np), 256, 0, s>>>(seg, cap_n, visible, Np, n_over);
+    k_path_overflow<<<blocks(Np), 256, 0, s>>>(seg, cap_n, visible, Np, n_over, skip);
     return hipGetLastError();
 }
 

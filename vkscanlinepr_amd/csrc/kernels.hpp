@@ -268,19 +268,23 @@ struct MlpDev {
     const float* mean;      // [16] feature normalisation
     const float* inv_std;   // [16]
 };
+// The prediction's kernels take an optional device flag `skip`: when *skip != 0 they return at
+// once and their outputs keep the values of the last frame that ran them (a captured frame holds
+// them and the flag decides at each replay).
 hipError_t launch_path_features(const float2* tpos, const uint32_t* cpos, const uint32_t* ctype,
                                 const uint32_t* pcurve, const int32_t* visible, const uint32_t* rule,
                                 uint32_t Np, int w, int h, float* feat, hipStream_t s,
                                 unsigned long long* bound_total = nullptr, const int2* pcanvas = nullptr,
-                                int32_t* pbound = nullptr);
-hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, float* pred, hipStream_t s);
+                                int32_t* pbound = nullptr, const int32_t* skip = nullptr);
+hipError_t launch_mlp_infer(const float* feat, uint32_t Np, const MlpDev& m, float* pred, hipStream_t s,
+                            const int32_t* skip = nullptr);
 hipError_t launch_caps(const float* pred, const int32_t* visible, uint32_t Np, float log_margin, int32_t slack,
                        int32_t* arena, int32_t* cap_n, int32_t* cap_r_total, hipStream_t s,
-                       const int32_t* pbound = nullptr);
+                       const int32_t* pbound = nullptr, const int32_t* skip = nullptr);
 // *n_over (zeroed) += visible paths whose exact entry count (seg[p+1] - seg[p]) exceeds their
 // predicted capacity cap_n[p].
 hipError_t launch_path_overflow(const int32_t* seg, const int32_t* cap_n, const int32_t* visible, uint32_t Np,
-                                int32_t* n_over, hipStream_t s);
+                                int32_t* n_over, hipStream_t s, const int32_t* skip = nullptr);
 hipError_t launch_sorted_starts(const int32_t* boff, const int32_t* bucket_base, uint32_t Np, int32_t* sstart,
                                 hipStream_t s);
 hipError_t launch_path_targets(const int32_t* n_or_seg, int seg_form, const int2* brec, const int32_t* bucket_base,

@@ -89,7 +89,9 @@ struct DBuf {
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // The MVP rows and their inverse, copied to the device as one block.
-constexpr size_t kMvpBytes = 25 * sizeof(float);
+// the MVP rows (16), the inverse of its 2D projective part (9), and two words for the learned
+// frame's prediction: skip it (the last one still holds), the element count of its arena scan
+constexpr size_t kMvpBytes = 27 * sizeof(float);
 
 // Inverse of the MVP's 2D projective part (a point (x, y, 0, 1) goes to rows 0, 1 and 3:
 // [[m0, m1, m3], [m4, m5, m7], [m12, m13, m15]]), by cofactors in double, rounded to float;
@@ -127,6 +129,7 @@ struct spr_handle {
     hipEvent_t ev_fork = nullptr, ev_light = nullptr, ev_small = nullptr, ev_join = nullptr;
     hipEvent_t ev_sfork = nullptr, ev_sjoin = nullptr;  // the short-band sort on the second stream
     hipEvent_t ev_pfork = nullptr, ev_pjoin = nullptr;  // the learned frame's prediction on the second stream
+    hipEvent_t ev_pseg = nullptr;  // its per-path counts, read by the overflow statistics there
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::vector<hipEvent_t> ev_stage;
     bool stage_timing = false;
@@ -180,14 +183,19 @@ struct spr_handle {
     DBuf<unsigned long long> zero;  // per-frame zeroed control region
     int32_t* h_pinned = nullptr;    // 128 words of pinned host memory: [0, 16) mid-frame readbacks,
                                     // [16, 16 + kSummaryMax) the last frame's summary, [48, 73) the
-                                    // MVP rows and the inverse of its 2D projective part
+                                    // MVP rows and the inverse of its 2D projective part, [73, 75)
+                                    // the prediction's skip flag and arena count
     DBuf<int32_t> summary;          // the frame summary on the device, copied to h_pinned + 16
     DBuf<unsigned long long> scan_ws;  // spr_scan_i32: tile statuses and the tile counter
     int32_t sum(int k) const { return h_pinned[16 + k]; }
     float* mvp_pinned() const { return reinterpret_cast<float*>(h_pinned + 48); }  // [48, 64): the MVP rows
     float* inv_pinned() const { return reinterpret_cast<float*>(h_pinned + 64); }  // [64, 73): its inverse
-    DBuf<float> mvp_dev;            // [0, 16) the MVP, [16, 25) its inverse: copied from mvp_pinned in
-                                    // frames without a host synchronisation and for paints
+    DBuf<float> mvp_dev;            // [0, 16) the MVP, [16, 25) its inverse, [25, 27) the prediction's
+                                    // flag and count: copied from mvp_pinned in frames without a host
+                                    // synchronisation and for paints
+    int32_t* pred_pinned() const { return h_pinned + 73; }  // [73, 75): skip flag, arena scan count
+    bool pred_stale = true;         // the MVP changed since the learned frame last ran its prediction
+    bool last_predicted = false;    // the last learned frame ran the prediction
 
     // injection
     std::vector<int32_t> inj_inter;
@@ -625,6 +633,18 @@ void grow_budget(spr_handle* H, int64_t need, int64_t predicted) {
 
 void learned_finish(spr_handle* H);
 
+// Whether the next learned frame runs the prediction (after a load, a model or margin change, or a
+// new MVP; always without a model, where only the features run), published in the pinned words
+// that the frame copies to the device with the MVP.  Every earlier frame has ended (frames end with
+// a stream synchronisation), so no copy is reading them.
+void publish_prediction(spr_handle* H) {
+    const bool predict = !H->model.ready || H->need_prediction || H->pred_stale;
+    H->pred_pinned()[0] = predict ? 0 : 1;
+    H->pred_pinned()[1] = predict ? (int32_t)H->Np : 0;
+    H->last_predicted = predict;
+    if (predict && !H->capturing) H->pred_stale = false;
+}
+
 // Enqueues one frame without any host synchronisation unless the predicted budget must first be
 // read back (H->need_prediction).  With H->capturing the work is being recorded into a graph: no
 // events, no waits, and every buffer already has its size.
@@ -674,9 +694,14 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->visible.p, Np, 0u, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_lo.p, Np, 0x7FFFFFFFu, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_hi.p, Np, 0x80000000u, s));
+    // the prediction is a function of the scene, the MVP and the model: it runs after a change of
+    // one of them and is otherwise skipped on the device (its outputs still hold)
+    publish_prediction(H);
     // the MVP goes through a copy from pinned host memory, which a captured frame repeats at every
     // replay: spr_set_mvp only rewrites that memory, and the graph follows it
     SPR_HIP(hipMemcpyAsync(H->mvp_dev.p, H->mvp_pinned(), kMvpBytes, hipMemcpyHostToDevice, s));
+    const int32_t* d_skip = reinterpret_cast<const int32_t*>(H->mvp_dev.p + 25);
+    const int32_t* d_arena_n = reinterpret_cast<const int32_t*>(H->mvp_dev.p + 26);
     SPR_HIP(spr::launch_transform(H->pos.p, H->pos_path.p, P, H->mvp, w, h, H->tpos.p, H->visible.p,
                                   H->row_lo.p, H->row_hi.p, s, pcv, H->mvp_dev.p));
     stage_mark(H, 1);
@@ -686,22 +711,33 @@ void run_learned_frame(spr_handle* H) {
         SPR_HIP(hipStreamWaitEvent(s2, H->ev_pfork, 0));
     }
     SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p, H->rule.p, Np,
-                                      w, h, H->feat.p, s2, H->zero.p + z.bound, pcv, H->pbound.p));
+                                      w, h, H->feat.p, s2, H->zero.p + z.bound, pcv, H->pbound.p, d_skip));
     if (use_model) {
-        SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s2));
+        SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s2, d_skip));
         SPR_HIP(spr::launch_caps(H->pred.p, H->visible.p, Np, H->model.log_margin, H->model.slack, H->arena.p,
-                                 H->capn.p, d_capr, s2, H->pbound.p));
+                                 H->capn.p, d_capr, s2, H->pbound.p, d_skip));
     } else {
         SPR_HIP(spr::launch_fill_u32((uint32_t*)H->arena.p, (int64_t)Np + 1, 0u, s2));
         SPR_HIP(spr::launch_fill_u32((uint32_t*)H->capn.p, (int64_t)Np + 1, 0u, s2));
     }
-    SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, kLcArena), s2));
-    if (s2 != s) SPR_HIP(hipEventRecord(H->ev_pjoin, s2));
+    SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, kLcArena), s2,
+                                 d_arena_n));
     SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->visible.p, C, w, h, H->cuts.p,
                               H->ncuts.p, H->counts.p, s, pcv));
     stage_mark(H, 2);
     SPR_HIP(spr::launch_scan_i32(H->counts.p, H->counts.p, C, ctl_at(H, z.st_count, z.counters, kLcCount), s));
     SPR_HIP(spr::launch_path_segments(H->cpath.p, C, H->counts.p, Np, H->seg.p, s));
+    // the prediction's paths that overflowed their capacity (statistics only): on the second stream
+    // once the per-path counts exist, unless this frame reads the prediction back (and re-lays the
+    // control region), in which case they are counted at its end
+    const bool ovf_side = use_model && s2 != s && !H->need_prediction;
+    if (ovf_side) {
+        SPR_HIP(hipEventRecord(H->ev_pseg, s));
+        SPR_HIP(hipStreamWaitEvent(s2, H->ev_pseg, 0));
+        SPR_HIP(spr::launch_path_overflow(H->seg.p, H->capn.p, H->visible.p, Np,
+                                          reinterpret_cast<int32_t*>(H->zero.p + z.ctl) + 2, s2, d_skip));
+    }
+    if (s2 != s) SPR_HIP(hipEventRecord(H->ev_pjoin, s2));
     SPR_HIP(spr::launch_bucket_counts(H->row_lo.p, H->row_hi.p, Np, rmax, H->bucket_base.p, H->prow.p, s, pcv));
     SPR_HIP(spr::launch_scan_i32(H->bucket_base.p, H->bucket_base.p, Np, ctl_at(H, z.st_pb, z.counters, kLcBucketBase),
                                  s));
@@ -809,7 +845,8 @@ void run_learned_frame(spr_handle* H) {
                                        d_live, narrow));
     SPR_HIP(spr::launch_scan_i32(H->bsum_l.p, H->bsum_l.p, max_bands, ctl_at(H, z.st_wind, z.counters, kLcBandSums),
                                  s, d_nbands));
-    SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
+    // the sorted start of each path: read by the documents' winding bases and the separate mark + emit
+    if (H->batch || !narrow) SPR_HIP(spr::launch_sorted_starts(H->hist_l.p, H->bucket_base.p, Np, H->sstart.p, s));
     // documents of a batch scan their winding numbers from their own first entry: with the fused
     // mark + emit that base comes from the band prefixes, before the sort
     if (H->batch && narrow)
@@ -848,9 +885,9 @@ void run_learned_frame(spr_handle* H) {
                                   (int)H->opts.compose, H->image.p, s, rec_cap, H->batch ? H->docs.p : nullptr,
                                   (int)H->n_docs, H->total_rows, paint_args(H, pa)));
     stage_mark(H, 12);
-    // the prediction's paths that overflowed their capacity (statistics only)
     if (!joined) SPR_HIP(hipStreamWaitEvent(s, H->ev_pjoin, 0));
-    if (use_model) SPR_HIP(spr::launch_path_overflow(H->seg.p, H->capn.p, H->visible.p, Np, d_novf, s));
+    if (use_model && !ovf_side)
+        SPR_HIP(spr::launch_path_overflow(H->seg.p, H->capn.p, H->visible.p, Np, d_novf, s, d_skip));
     spr::SummarySrc S{};
     S.n = kSumCount;
     S.p[kSumCells] = d_totals;
@@ -907,11 +944,11 @@ void learned_finish(spr_handle* H) {
         }
         run_frame(H);
         grow_budget(H, H->N, caps);
-        H->stats.n_overflow_paths = overflow_paths;
+        if (H->last_predicted) H->stats.n_overflow_paths = overflow_paths;
         H->stats.alloc_intersections = caps;
         return;
     }
-    H->stats.n_overflow_paths = overflow_paths;
+    if (H->last_predicted) H->stats.n_overflow_paths = overflow_paths;
     H->stats.alloc_intersections = caps;
     H->N = H->sum(kSumSorted);
     H->n_live = H->sum(kSumLive);
@@ -955,6 +992,7 @@ void capture_frame(spr_handle* H) {
 void render_nosync(spr_handle* H) {
     hipStream_t s = H->stream;
     if (!H->lb_dirty && !H->need_prediction && H->gexec && H->graph_lb_n == H->lb_n && !H->stage_timing) {
+        publish_prediction(H);  // read by the graph's copy of the MVP block
         SPR_HIP(hipEventRecord(H->ev_begin, s));
         SPR_HIP(hipGraphLaunch(H->gexec, s));
         SPR_HIP(hipEventRecord(H->ev_end, s));
@@ -1151,7 +1189,7 @@ int spr_create(uint32_t width, uint32_t height, const spr_opts* opts, spr_handle
         SPR_HIP(hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking));
         SPR_HIP(hipStreamCreateWithFlags(&H->stream2, hipStreamNonBlocking));
         for (hipEvent_t* e : {&H->ev_fork, &H->ev_light, &H->ev_small, &H->ev_join, &H->ev_sfork, &H->ev_sjoin,
-                              &H->ev_pfork, &H->ev_pjoin})
+                              &H->ev_pfork, &H->ev_pjoin, &H->ev_pseg})
             SPR_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
         SPR_HIP(hipEventCreate(&H->ev_begin));
         SPR_HIP(hipEventCreate(&H->ev_end));
@@ -1202,7 +1240,7 @@ int spr_destroy(spr_handle* H) {
     if (H->ev_begin) (void)hipEventDestroy(H->ev_begin);
     if (H->ev_end) (void)hipEventDestroy(H->ev_end);
     for (hipEvent_t e : {H->ev_fork, H->ev_light, H->ev_small, H->ev_join, H->ev_sfork, H->ev_sjoin, H->ev_pfork,
-                         H->ev_pjoin})
+                         H->ev_pjoin, H->ev_pseg})
         if (e) (void)hipEventDestroy(e);
     if (H->stream2) (void)hipStreamDestroy(H->stream2);
     if (H->stream) (void)hipStreamDestroy(H->stream);
@@ -1249,6 +1287,7 @@ int spr_set_mvp(spr_handle* H, const float m_rows[16]) {
     // copy is reading the pinned matrix; a captured frame picks the new one up at its next replay
     std::memcpy(H->mvp.m, m_rows, sizeof(H->mvp.m));
     std::memcpy(H->mvp_pinned(), m_rows, sizeof(H->mvp.m));
+    H->pred_stale = true;  // the prediction depends on the transformed points
     mvp_inverse(m_rows, H->inv_pinned());
     return SPR_OK;
 }
