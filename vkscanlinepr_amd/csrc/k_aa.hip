@@ -31,7 +31,28 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// Lane t covers pixel t % 4 of record t / 4.
+// The first sample column c (0..15) of a pixel whose x, x0 + (c + 1/2)/16, lies above v; 16 when
+// none does (NaN included).  x0 is an integer and every sample x is exact, so for v within a few
+// columns of x0, u = 16 (v - x0) - 1/2 is exact and the columns above v are those with c > u; the
+// two boundary compares keep the answer exact wherever u is not.
+__device__ inline int first_above(float v, float x0) {
+    if (!(v == v)) return 16;
+    const float u = fminf(fmaxf((v - x0) * 16.0f - 0.5f, -2.0f), 17.0f);
+    int k = (int)floorf(u) + 1;
+    k = k < 0 ? 0 : (k > 16 ? 16 : k);
+    if (k < 16 && !(x0 + ((float)k + 0.5f) / 16.0f > v)) ++k;
+    if (k > 0 && x0 + ((float)(k - 1) + 0.5f) / 16.0f > v) --k;
+    return k;
+}
+
+// Lane t covers pixel t % 4 of record t / 4.  Its 16 winding numbers are kept by sample column c
+// (sample j = 7c mod 16 sits in column c, (7 * 7) mod 16 = 1), so every chord is one
+// straight-line pass over the columns with fixed register indices:
+//  - the centre-row crossing adds its change to the columns above its x (a suffix c >= kr);
+//  - a chord spanning column c (ax < xs <= bx or bx < xs <= ax: c in [ka, kb)) crosses it at
+//    height yx = ay + (xs - ax) * slope, which counts in the upper pixel when yc <= yx < ys and
+//    in the lower one (with the opposite sign) when ys < yx <= yc; both tests are one form on
+//    values multiplied by +1 or -1 (exact).
 __device__ inline void cover_one(int64_t t, const int32_t* __restrict__ cell_first, const int32_t* __restrict__ skey,
                                  const int32_t* __restrict__ sidx, const int32_t* __restrict__ winding,
                                  const uint32_t* __restrict__ epath, const float4* __restrict__ chord, int64_t N,
@@ -49,38 +70,32 @@ __device__ inline void cover_one(int64_t t, const int32_t* __restrict__ cell_fir
     const int px = pix & 1, py = pix >> 1;
     const float yc = (float)(ry + 1);
     const float x0 = (float)(rx + px), y0 = (float)(ry + py);
-    int32_t W[kAaSamples];
+    const float sy = py ? 1.0f : -1.0f;  // the upper pixel tests [yc, ys), the lower one (ys, yc]
+    const float syc = yc * sy;
+    int32_t W[kAaSamples];  // by sample column
     const int32_t w0 = winding[f];
 #pragma unroll
-    for (int k = 0; k < kAaSamples; ++k) W[k] = w0;
-    // one chord's winding changes at the 16 samples
+    for (int c = 0; c < kAaSamples; ++c) W[c] = w0;
     auto add_chord = [&](const float4 ch) {
         const float ax = ch.x, ay = ch.y, bx = ch.z, by = ch.w;
-        // the centre-row crossing (sample independent)
-        int sr = 0;
-        if (ay < yc && yc <= by) sr = -1;
-        else if (by < yc && yc <= ay) sr = 1;
-        const float xc = sr != 0 ? ax + (yc - ay) * ((bx - ax) / (by - ay)) : 0.0f;
-        // the column crossings' slope, once per chord (read only when the chord spans a column)
+        // the centre-row crossing: its change at the columns right of its x
+        const int sr = (ay < yc && yc <= by) ? -1 : ((by < yc && yc <= ay) ? 1 : 0);
+        const float xc = ax + (yc - ay) * ((bx - ax) / (by - ay));
+        const int kr = sr != 0 ? first_above(xc, x0) : 16;
+        // the spanned columns and the sign of their crossings in this pixel
+        const bool rightward = ax < bx;
+        const float lo = rightward ? ax : bx, hi = rightward ? bx : ax;
+        const int ka = first_above(lo, x0), kb = hi == hi ? first_above(hi, x0) : 0;
+        const int sc = (rightward ? 1 : -1) * (py ? 1 : -1);
         const float sl = (by - ay) / (bx - ax);
 #pragma unroll
-        for (int j = 0; j < kAaSamples; ++j) {
-            const float xs = x0 + ((float)((7 * j) & 15) + 0.5f) / 16.0f;
-            const float ys = y0 + ((float)j + 0.5f) / 16.0f;
-            int32_t d = 0;
-            if (sr != 0 && xc < xs) d += sr;
-            int dc = 0;
-            if (ax < xs && xs <= bx) dc = 1;
-            else if (bx < xs && xs <= ax) dc = -1;
-            if (dc != 0) {
-                const float yx = ay + (xs - ax) * sl;
-                if (ys > yc) {
-                    if (yc <= yx && yx < ys) d += dc;
-                } else if (ys < yx && yx <= yc) {
-                    d -= dc;
-                }
-            }
-            W[j] += d;
+        for (int c = 0; c < kAaSamples; ++c) {
+            const float xs = x0 + ((float)c + 0.5f) / 16.0f;
+            const float ys = y0 + ((float)((7 * c) & 15) + 0.5f) / 16.0f;
+            const float yx = ay + (xs - ax) * sl;
+            const float v = yx * sy;
+            const bool hit = (c >= ka) & (c < kb) & (syc <= v) & (v < ys * sy);  // no short circuit: no branches
+            W[c] += (c >= kr ? sr : 0) + (hit ? sc : 0);
         }
     };
     // the cell's first kPre entries fetched together (keys, paths and indices, then chords): a
@@ -91,17 +106,16 @@ __device__ inline void cover_one(int64_t t, const int32_t* __restrict__ cell_fir
     bool more = true;
 #pragma unroll
     for (int u = 0; u < kPre; ++u) {
-        const int64_t g = f + u;
-        const bool inb = g < N;
-        const int32_t kg = inb ? skey[g] : ~key;
-        const uint32_t pg = inb ? (epath[g] & 0x3FFFFFFFu) : ~path;
-        ii[u] = inb ? sidx[g] : 0;
-        more = more && kg == key && pg == path;
+        const int64_t g = f + u < N ? f + u : N - 1;  // loads in range, the run test below stops at N
+        const int32_t kg = skey[g];
+        const uint32_t pg = epath[g] & 0x3FFFFFFFu;
+        ii[u] = sidx[g];
+        more = more && f + u < N && kg == key && pg == path;
         nv += more ? 1 : 0;
     }
     float4 chs[kPre];
 #pragma unroll
-    for (int u = 0; u < kPre; ++u) chs[u] = u < nv ? chord[ii[u]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int u = 0; u < kPre; ++u) chs[u] = chord[u < nv ? ii[u] : ii[0]];
 #pragma unroll
     for (int u = 0; u < kPre; ++u)
         if (u < nv) add_chord(chs[u]);
@@ -111,9 +125,9 @@ __device__ inline void cover_one(int64_t t, const int32_t* __restrict__ cell_fir
     const bool even_odd = rule[path] == 1u;
     uint32_t m = 0;
 #pragma unroll
-    for (int k = 0; k < kAaSamples; ++k) {
-        const bool in = even_odd ? (W[k] & 1) != 0 : W[k] != 0;
-        m |= (uint32_t)in << k;
+    for (int c = 0; c < kAaSamples; ++c) {
+        const bool in = even_odd ? (W[c] & 1) != 0 : W[c] != 0;
+        m |= (uint32_t)in << ((7 * c) & 15);  // column c holds sample 7c mod 16
     }
     cmask16[t] = (uint16_t)m;  // bits (2 py + px) * 16 .. + 15 of the record's 64-bit mask
 }
