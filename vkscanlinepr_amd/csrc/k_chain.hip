@@ -42,9 +42,9 @@ constexpr int kLongClass = 10;     // cubic chains of 2^10 = 1024+ crossings get
 constexpr int kStage = 4096;       // piece values staged in LDS by the merging wave
 constexpr int kClasses = kChainClasses;  // 0..63: cubic chains (4 per octave), 64..79: line chains
 constexpr int kRounds = 4, kLevels = 6;  // wave solve: 4 rounds of 6 bisection levels
-// first heavy cubic class: a quarter-octave class 4*l + sub holds lengths >= 2^l (l = 9: 512)
-constexpr int kHeavyClass = 4 * 9;
-static_assert(kHeavyCrossings == 1 << 9, "heavy class boundary");
+// first heavy cubic class: a quarter-octave class 4*l + sub holds lengths >= 2^l (l = 8: 256)
+constexpr int kHeavyClass = 4 * 8;
+static_assert(kHeavyCrossings == 1 << 8, "heavy class boundary");
 static_assert(kRounds * kLevels == kCubicIters, "wave solve must run the reference's iterations");
 
 inline unsigned grid_for(int64_t n) {

@@ -92,7 +92,7 @@ constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 
 // Chains of at least kHeavyCrossings crossings are "heavy": they are solved by their own launch
 // (wave-solved and the longest lane-solved chains), and the pieces of at most kHeavyCrossings
 // entries, which never wait on them, are finished by a concurrent stream meanwhile.
-constexpr int kHeavyCrossings = 512;
+constexpr int kHeavyCrossings = 256;
 // A frame whose buffers were sized before its counts were known (no host synchronisation): the
 // chain prep compares the exact total offsets[C] with the budget; a frame over budget writes no
 // chain and no piece (as if every path were culled), sets *over and leaves 0 in *live, the number
