@@ -19,9 +19,11 @@
 // going left when moving up (the reverse when moving down).  Both legs stay inside the cell, so
 // only the cell's own chords (one sorted run of entries) are met.
 //
-// Four lanes per record, one per pixel: each walks the cell's chords (the lanes of a record load
-// the same lines) with its 16 winding numbers in registers and stores its 16 mask bits.  Every
-// float expression has a fixed operation order (no contraction).
+// k_cell_cover gives spans their full masks, lists each workgroup's cells by number of chords,
+// longest first, so that the cells of one wave have (nearly) the same number, and then runs four
+// lanes per listed cell, one per pixel: each walks the cell's chords (the lanes of a cell load the
+// same lines) with its 16 winding numbers in registers and stores its 16 mask bits.  Every float
+// expression has a fixed operation order (no contraction).
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -30,6 +32,10 @@ namespace spr {
 namespace {
 
 constexpr int kBlock = 256;
+// Cell records by number of chords (the run of sorted entries with the cell's key and path):
+// class k < kCoverClasses - 1 holds the cells of k chords, the last class those of more.
+constexpr int kCoverClasses = 32;
+constexpr int kChunk = 4 * kBlock;  // records per workgroup pass of the cover kernel
 
 // The first sample column c (0..15) of a pixel whose x, x0 + (c + 1/2)/16, lies above v; 16 when
 // none does (NaN included).  x0 is an integer and every sample x is exact, so for v within a few
@@ -53,17 +59,12 @@ __device__ inline int first_above(float v, float x0) {
 //    height yx = ay + (xs - ax) * slope, which counts in the upper pixel when yc <= yx < ys and
 //    in the lower one (with the opposite sign) when ys < yx <= yc; both tests are one form on
 //    values multiplied by +1 or -1 (exact).
-__device__ inline void cover_one(int64_t t, const int32_t* __restrict__ cell_first, const int32_t* __restrict__ skey,
+__device__ inline void cover_one(int64_t o, int32_t f, int cls, int pix, const int32_t* __restrict__ skey,
                                  const int32_t* __restrict__ sidx, const int32_t* __restrict__ winding,
                                  const uint32_t* __restrict__ epath, const float4* __restrict__ chord, int64_t N,
                                  const uint32_t* __restrict__ rule, uint16_t* __restrict__ cmask16) {
-    const int64_t o = t >> 2;            // the record
-    const int pix = (int)(t & 3);        // its pixel: px = pix & 1, py = pix >> 1
-    const int32_t f = cell_first[o];
-    if (f < 0) {
-        cmask16[t] = 0xFFFFu;  // a span: every sample
-        return;
-    }
+    // o: a cell record, f: its first sorted entry, cls: its chord class (run_class), pix: its
+    // pixel, px = pix & 1, py = pix >> 1
     const int32_t key = skey[f];
     const uint32_t path = epath[f] & 0x3FFFFFFFu;
     const int rx = (key & 0xFFFF) - 0x7FFF, ry = ((key >> 16) & 0xFFFF) - 0x7FFF;
@@ -98,29 +99,21 @@ __device__ inline void cover_one(int64_t t, const int32_t* __restrict__ cell_fir
             W[c] += (c >= kr ? sr : 0) + (hit ? sc : 0);
         }
     };
-    // the cell's first kPre entries fetched together (keys, paths and indices, then chords): a
-    // cell rarely has more, which the loop after them walks one at a time
+    // the cell's first kPre chords fetched together (indices, then chords), the rest one at a time
     constexpr int kPre = 4;
+    const int nv = cls;
     int32_t ii[kPre];
-    int nv = 0;
-    bool more = true;
 #pragma unroll
-    for (int u = 0; u < kPre; ++u) {
-        const int64_t g = f + u < N ? f + u : N - 1;  // loads in range, the run test below stops at N
-        const int32_t kg = skey[g];
-        const uint32_t pg = epath[g] & 0x3FFFFFFFu;
-        ii[u] = sidx[g];
-        more = more && f + u < N && kg == key && pg == path;
-        nv += more ? 1 : 0;
-    }
+    for (int u = 0; u < kPre; ++u) ii[u] = sidx[f + (u < nv ? u : nv - 1)];  // loads within the run
     float4 chs[kPre];
 #pragma unroll
-    for (int u = 0; u < kPre; ++u) chs[u] = chord[u < nv ? ii[u] : ii[0]];
+    for (int u = 0; u < kPre; ++u) chs[u] = chord[ii[u]];
 #pragma unroll
     for (int u = 0; u < kPre; ++u)
         if (u < nv) add_chord(chs[u]);
-    if (nv == kPre)
-        for (int64_t g = f + kPre; g < N && skey[g] == key && (epath[g] & 0x3FFFFFFFu) == path; ++g)
+    for (int64_t g = f + kPre; g < f + nv; ++g) add_chord(chord[sidx[g]]);
+    if (nv == kCoverClasses - 1)  // the open class: the rest of the run
+        for (int64_t g = f + nv; g < N && skey[g] == key && (epath[g] & 0x3FFFFFFFu) == path; ++g)
             add_chord(chord[sidx[g]]);
     const bool even_odd = rule[path] == 1u;
     uint32_t m = 0;
@@ -129,9 +122,39 @@ __device__ inline void cover_one(int64_t t, const int32_t* __restrict__ cell_fir
         const bool in = even_odd ? (W[c] & 1) != 0 : W[c] != 0;
         m |= (uint32_t)in << ((7 * c) & 15);  // column c holds sample 7c mod 16
     }
-    cmask16[t] = (uint16_t)m;  // bits (2 py + px) * 16 .. + 15 of the record's 64-bit mask
+    cmask16[4 * o + pix] = (uint16_t)m;  // bits (2 py + px) * 16 .. + 15 of the record's 64-bit mask
 }
 
+// The chord class of the cell whose run starts at entry f: the run's length, or
+// kCoverClasses - 1 for a run at least that long.  Four entries are loaded at a time.
+__device__ inline int run_class(int32_t f, const int32_t* __restrict__ skey, const uint32_t* __restrict__ epath,
+                                int64_t N) {
+    const int32_t key = skey[f];
+    const uint32_t path = epath[f] & 0x3FFFFFFFu;
+    int k = 1;
+    bool more = true;
+    while (more) {
+        int32_t kg[4];
+        uint32_t pg[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t g = f + k + u < N ? f + k + u : N - 1;  // loads in range, the test stops at N
+            kg[u] = skey[g];
+            pg[u] = epath[g] & 0x3FFFFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            more = more && k < kCoverClasses - 1 && f + k < N && kg[u] == key && pg[u] == path;
+            k += more ? 1 : 0;
+        }
+    }
+    return k;
+}
+
+// A workgroup takes kChunk records at a time: spans get every sample at once; the cells are
+// counted by chord class in LDS and listed there longest first, and the four waves then cover
+// the listed cells four lanes per cell, so that the 16 cells of a wave have (nearly) the same
+// number of chords: a wave runs as many chord passes as its longest cell needs.
 __global__ __launch_bounds__(kBlock) void k_cell_cover(const int32_t* __restrict__ cell_first,
                                                        const int32_t* __restrict__ totals, int64_t rec_cap,
                                                        const int32_t* __restrict__ skey,
@@ -141,10 +164,57 @@ __global__ __launch_bounds__(kBlock) void k_cell_cover(const int32_t* __restrict
                                                        const float4* __restrict__ chord, int64_t N,
                                                        const uint32_t* __restrict__ rule,
                                                        uint16_t* __restrict__ cmask16) {
-    // grid-stride over the records actually emitted (the launch cannot know their number)
+    __shared__ int32_t cnt[kCoverClasses];  // per class: the count, then the next list place
+    __shared__ int32_t list_f[kChunk];      // listed cells: first entry
+    __shared__ uint16_t list_oc[kChunk];    // listed cells: record within the chunk << 5 | class
     const int64_t R = min((int64_t)totals[0] + (int64_t)totals[1], rec_cap);
-    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < 4 * R; t += (int64_t)gridDim.x * kBlock)
-        cover_one(t, cell_first, skey, sidx, winding, epath, chord, N, rule, cmask16);
+    for (int64_t c0 = (int64_t)blockIdx.x * kChunk; c0 < R; c0 += (int64_t)gridDim.x * kChunk) {
+        if (threadIdx.x < kCoverClasses) cnt[threadIdx.x] = 0;
+        __syncthreads();
+        int32_t fr[4];
+        int kr[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t o = c0 + u * kBlock + threadIdx.x;
+            fr[u] = o < R ? cell_first[o] : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t o = c0 + u * kBlock + threadIdx.x;
+            kr[u] = 0;
+            if (o < R) {
+                if (fr[u] < 0) {
+                    reinterpret_cast<unsigned long long*>(cmask16)[o] = ~0ull;  // a span: every sample
+                } else {
+                    kr[u] = run_class(fr[u], skey, epath, N);
+                    atomicAdd(&cnt[kr[u]], 1);
+                }
+            }
+        }
+        __syncthreads();
+        // list places, longest class first
+        int32_t off = 0;
+        if (threadIdx.x < kCoverClasses)
+            for (int d = threadIdx.x + 1; d < kCoverClasses; ++d) off += cnt[d];
+        __syncthreads();
+        if (threadIdx.x < kCoverClasses) cnt[threadIdx.x] = off;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (kr[u]) {
+                const int at = atomicAdd(&cnt[kr[u]], 1);
+                list_f[at] = fr[u];
+                list_oc[at] = (uint16_t)(((u * kBlock + threadIdx.x) << 5) | kr[u]);
+            }
+        __syncthreads();
+        const int cells = cnt[1];  // class 1 is listed last: its end is the list's length
+        for (int t = threadIdx.x; t < 4 * cells; t += kBlock) {
+            const int oc = list_oc[t >> 2];
+            cover_one(c0 + (oc >> 5), list_f[t >> 2], oc & 31, t & 3, skey, sidx, winding, epath, chord, N, rule,
+                      cmask16);
+        }
+        __syncthreads();  // the next chunk reuses the list
+    }
 }
 
 }  // namespace
@@ -154,8 +224,8 @@ hipError_t launch_cell_cover(const int32_t* cell_first, const int32_t* totals, i
                              const uint32_t* epath, const float4* chord, int64_t N, const uint32_t* rule,
                              unsigned long long* cmask, hipStream_t s) {
     if (rec_cap <= 0 || N <= 0) return hipSuccess;
-    int64_t g = (4 * rec_cap + kBlock - 1) / kBlock;
-    g = g < 8192 ? g : 8192;  // 32 waves per CU; the blocks stride over the live records
+    int64_t g = (rec_cap + kChunk - 1) / kChunk;
+    g = g < 4096 ? g : 4096;  // the workgroups stride over the live records
     k_cell_cover<<<(unsigned)g, kBlock, 0, s>>>(cell_first, totals, rec_cap, skey, sidx, winding, epath, chord, N,
                                                 rule, reinterpret_cast<uint16_t*>(cmask));
     return hipGetLastError();
