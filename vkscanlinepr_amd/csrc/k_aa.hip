@@ -37,28 +37,14 @@ constexpr int kBlock = 256;
 constexpr int kCoverClasses = 32;
 constexpr int kChunk = 4 * kBlock;  // records per workgroup pass of the cover kernel
 
-// The first sample column c (0..15) of a pixel whose x, x0 + (c + 1/2)/16, lies above v; 16 when
-// none does (NaN included).  x0 is an integer and every sample x is exact, so for v within a few
-// columns of x0, u = 16 (v - x0) - 1/2 is exact and the columns above v are those with c > u; the
-// two boundary compares keep the answer exact wherever u is not.
-__device__ inline int first_above(float v, float x0) {
-    if (!(v == v)) return 16;
-    const float u = fminf(fmaxf((v - x0) * 16.0f - 0.5f, -2.0f), 17.0f);
-    int k = (int)floorf(u) + 1;
-    k = k < 0 ? 0 : (k > 16 ? 16 : k);
-    if (k < 16 && !(x0 + ((float)k + 0.5f) / 16.0f > v)) ++k;
-    if (k > 0 && x0 + ((float)(k - 1) + 0.5f) / 16.0f > v) --k;
-    return k;
-}
-
-// Lane t covers pixel t % 4 of record t / 4.  Its 16 winding numbers are kept by sample column c
-// (sample j = 7c mod 16 sits in column c, (7 * 7) mod 16 = 1), so every chord is one
-// straight-line pass over the columns with fixed register indices:
-//  - the centre-row crossing adds its change to the columns above its x (a suffix c >= kr);
-//  - a chord spanning column c (ax < xs <= bx or bx < xs <= ax: c in [ka, kb)) crosses it at
-//    height yx = ay + (xs - ax) * slope, which counts in the upper pixel when yc <= yx < ys and
-//    in the lower one (with the opposite sign) when ys < yx <= yc; both tests are one form on
-//    values multiplied by +1 or -1 (exact).
+// A lane covers one pixel of a cell.  Its 16 winding numbers are kept by sample column c (sample
+// j = 7c mod 16 sits in column c, (7 * 7) mod 16 = 1), so every chord is one straight-line pass
+// over the columns with fixed register indices, each column tested on its sample x, xs:
+//  - the centre-row crossing adds its change to the columns whose xs lies above its x;
+//  - a chord spanning column c (ax < xs <= bx or bx < xs <= ax) crosses it at height
+//    yx = ay + (xs - ax) * slope, which counts in the upper pixel when yc <= yx < ys and in the
+//    lower one (with the opposite sign) when ys < yx <= yc; both tests are one form on values
+//    multiplied by +1 or -1 (exact).
 __device__ inline void cover_one(int64_t o, int32_t f, int cls, int pix, const int32_t* __restrict__ skey,
                                  const int32_t* __restrict__ sidx, const int32_t* __restrict__ winding,
                                  const uint32_t* __restrict__ epath, const float4* __restrict__ chord, int64_t N,
@@ -79,24 +65,25 @@ __device__ inline void cover_one(int64_t o, int32_t f, int cls, int pix, const i
     for (int c = 0; c < kAaSamples; ++c) W[c] = w0;
     auto add_chord = [&](const float4 ch) {
         const float ax = ch.x, ay = ch.y, bx = ch.z, by = ch.w;
-        // the centre-row crossing: its change at the columns right of its x
+        // the centre-row crossing: its change sr at the columns right of its x, xc (sr = 0 for a
+        // chord that does not cross the row, whose xc is never used)
         const int sr = (ay < yc && yc <= by) ? -1 : ((by < yc && yc <= ay) ? 1 : 0);
         const float xc = ax + (yc - ay) * ((bx - ax) / (by - ay));
-        const int kr = sr != 0 ? first_above(xc, x0) : 16;
-        // the spanned columns and the sign of their crossings in this pixel
+        // the spanned columns (lo < xs <= hi) and the sign of their crossings in this pixel
         const bool rightward = ax < bx;
         const float lo = rightward ? ax : bx, hi = rightward ? bx : ax;
-        const int ka = first_above(lo, x0), kb = hi == hi ? first_above(hi, x0) : 0;
         const int sc = (rightward ? 1 : -1) * (py ? 1 : -1);
         const float sl = (by - ay) / (bx - ax);
+        // yx * sy = ay * sy + (xs - ax) * (sl * sy) up to the sign of a zero (negation is exact)
+        const float ays = ay * sy, sls = sl * sy;
 #pragma unroll
         for (int c = 0; c < kAaSamples; ++c) {
             const float xs = x0 + ((float)c + 0.5f) / 16.0f;
             const float ys = y0 + ((float)((7 * c) & 15) + 0.5f) / 16.0f;
-            const float yx = ay + (xs - ax) * sl;
-            const float v = yx * sy;
-            const bool hit = (c >= ka) & (c < kb) & (syc <= v) & (v < ys * sy);  // no short circuit: no branches
-            W[c] += (c >= kr ? sr : 0) + (hit ? sc : 0);
+            const float v = ays + (xs - ax) * sls;
+            // no short circuit: no branches
+            const bool hit = (xs > lo) & (xs <= hi) & (syc <= v) & (v < ys * sy);
+            W[c] += ((xs > xc) ? sr : 0) + (hit ? sc : 0);
         }
     };
     // the cell's first kPre chords fetched together (indices, then chords), the rest one at a time
@@ -155,15 +142,14 @@ __device__ inline int run_class(int32_t f, const int32_t* __restrict__ skey, con
 // counted by chord class in LDS and listed there longest first, and the four waves then cover
 // the listed cells four lanes per cell, so that the 16 cells of a wave have (nearly) the same
 // number of chords: a wave runs as many chord passes as its longest cell needs.
-__global__ __launch_bounds__(kBlock) void k_cell_cover(const int32_t* __restrict__ cell_first,
-                                                       const int32_t* __restrict__ totals, int64_t rec_cap,
-                                                       const int32_t* __restrict__ skey,
-                                                       const int32_t* __restrict__ sidx,
-                                                       const int32_t* __restrict__ winding,
-                                                       const uint32_t* __restrict__ epath,
-                                                       const float4* __restrict__ chord, int64_t N,
-                                                       const uint32_t* __restrict__ rule,
-                                                       uint16_t* __restrict__ cmask16) {
+// Five waves per SIMD (at most 96 VGPRs, the sample x and y values rematerialised): faster than
+// four waves holding them all in registers.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void k_cell_cover(const int32_t* __restrict__ cell_first, const int32_t* __restrict__ totals, int64_t rec_cap,
+                  const int32_t* __restrict__ skey, const int32_t* __restrict__ sidx,
+                  const int32_t* __restrict__ winding, const uint32_t* __restrict__ epath,
+                  const float4* __restrict__ chord, int64_t N, const uint32_t* __restrict__ rule,
+                  uint16_t* __restrict__ cmask16) {
     __shared__ int32_t cnt[kCoverClasses];  // per class: the count, then the next list place
     __shared__ int32_t list_f[kChunk];      // listed cells: first entry
     __shared__ uint16_t list_oc[kChunk];    // listed cells: record within the chunk << 5 | class

@@ -6,10 +6,11 @@
 // wins), and canvas row y lands on image row h-1-y.  All records sit on even rows, so the image
 // is a stack of independent two-pixel "cell rows".
 //
-// One workgroup owns one cell row, in column chunks of 4096; each thread keeps 16 consecutive
-// pixels of the chunk in registers.  The per-column event words live in LDS column-interleaved
-// (column c at (c % 16) * 256 + c / 16), so the 64 lanes of a wave touching "their" j-th column
-// hit consecutive words instead of one bank.  The (path, row) bucket table gives each path's records on
+// One workgroup owns one cell row, in column chunks of 4096; each thread keeps PIX consecutive
+// pixels of the chunk in registers (16; 8 in the anti-aliased variants, which keep both rows of a
+// cell row, in workgroups of 512 threads, so that they run as many waves per SIMD).  The per-column event words
+// live in LDS column-interleaved (column c at (c % PIX) * (4096 / PIX) + c / PIX), so the 64 lanes
+// of a wave touching "their" j-th column hit consecutive words instead of one bank.  The (path, row) bucket table gives each path's records on
 // this row as one contiguous range of non-overlapping compact records (x | width << 16) with one
 // colour.  Paths are taken in order, 64 at a time: a record toggles its path's bit at its first
 // column and at its end column in a 64-bit event word per column (LDS), and a prefix XOR over
@@ -36,25 +37,27 @@ namespace spr {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kPix = 16;                   // consecutive pixels per thread
-constexpr int kChunk = kBlock * kPix;      // columns per pass
+constexpr int kChunk = 4096;               // columns per pass (a workgroup of kChunk / PIX threads)
+constexpr int kPixOne = 16;                // pixels per thread of the variants that keep one row
 constexpr int kBatch = 64;                 // paths per coverage mask
 constexpr int kAaSlots = 4;                // partial-coverage entries per column and batch
-constexpr int kList = 2 * kBlock;          // paths listed before their batches are folded
+constexpr int kList = 512;                 // paths listed before their batches are folded
+constexpr int kListStep = 256;             // candidate paths examined per listing step
 constexpr int kUnroll = 4;                 // records loaded per thread before applying
 typedef unsigned long long u64;
 
-// LDS slot of chunk column c (0 <= c < kChunk): thread t's column t * kPix + j sits at
-// j * kBlock + t.
-__device__ inline int ev_slot(int c) { return (c % kPix) * kBlock + c / kPix; }
+// LDS slot of chunk column c (0 <= c < kChunk) with PIX consecutive pixels per thread: thread t's
+// column t * PIX + j sits at j * (kChunk / PIX) + t.
+template <int PIX>
+__device__ inline int ev_slot(int c) { return (c % PIX) * (kChunk / PIX) + c / PIX; }
 
 __device__ inline u64 shfl_up_u64(u64 v, int d) {
     const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
     return (u64)(uint32_t)__shfl_up((int)lo, d) | ((u64)(uint32_t)__shfl_up((int)hi, d) << 32);
 }
 
-// Exclusive prefix XOR of one value per thread over the workgroup.
+// Exclusive prefix XOR of one value per thread over a workgroup of BLOCK threads.
+template <int BLOCK>
 __device__ inline u64 block_exclusive_xor(u64 v, u64* lds) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     u64 inc = v;
@@ -67,7 +70,7 @@ __device__ inline u64 block_exclusive_xor(u64 v, u64* lds) {
     __syncthreads();
     u64 off = 0;
 #pragma unroll
-    for (int k = 0; k < kBlock / 64; ++k)
+    for (int k = 0; k < BLOCK / 64; ++k)
         if (k < wave) off ^= lds[k];
     __syncthreads();
     return off ^ inc ^ v;
@@ -172,8 +175,8 @@ __device__ inline void blend(float* d, const float4 sa) {
     d[3] = sa.w + d[3] * ia;
 }
 
-template <int COMPOSE, int GRAD, int AA>
-__global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict__ records,
+template <int COMPOSE, int GRAD, int AA, int PIX>
+__global__ __launch_bounds__(kChunk / PIX) __attribute__((amdgpu_waves_per_eu((AA && !GRAD) ? 4 : 1))) void k_composite(const uint32_t* __restrict__ records,
                                                       const int2* __restrict__ brec,
                                                       const int32_t* __restrict__ bucket_base,
                                                       const int2* __restrict__ prow,
@@ -184,7 +187,9 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                                       const PaintArgs pa,
                                                       const unsigned long long* __restrict__ cmask) {
     constexpr int TWO = (GRAD || AA) ? 1 : 0;            // the two rows of a cell row differ
-    // partial coverage: entry k of column c at lslot[k * kChunk + ev_slot(c)], q | n0 << 6 | n1 << 11;
+    constexpr int BLOCK = kChunk / PIX;                  // threads; each keeps PIX consecutive pixels
+    static_assert(PIX % 4 == 0 && BLOCK % 64 == 0 && BLOCK >= kListStep, "composite shape");
+    // partial coverage: entry k of column c at lslot[k * kChunk + ev_slot<PIX>(c)], q | n0 << 6 | n1 << 11;
     // byte c of lcnt counts the column's entries (a thread's 16 columns in one 16-byte word)
     __shared__ uint16_t lslot[AA ? kAaSlots * kChunk : 1];
     __shared__ uint32_t lcnt[AA ? kChunk / 4 : 1];
@@ -195,9 +200,9 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
     __shared__ uint32_t lpaint[GRAD ? kList : 1];  // each listed path's paint (kSolidPaint: its colour)
     __shared__ u64 lgrad;                          // the batch's paths that have a paint
     __shared__ float4 lsa[kBatch];   // (r*a, g*a, b*a, a) of each path of the batch, as the blend uses them
-    __shared__ int32_t wcount[kBlock / 64];
+    __shared__ int32_t wcount[BLOCK / 64];
     __shared__ int32_t bpre[kBatch + 1];
-    __shared__ u64 xlds[kBlock / 64];
+    __shared__ u64 xlds[BLOCK / 64];
     int r = blockIdx.x;
     uint32_t pbeg = 0, pend = Np;
     if (docs) {
@@ -215,13 +220,13 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
     }
     const int y = 2 * r;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int xb = tid * kPix;  // first column of this thread inside the chunk
+    const int xb = tid * PIX;  // first column of this thread inside the chunk
     for (int c0 = 0; c0 < w; c0 += kChunk) {
         const int cw = w - c0 < kChunk ? w - c0 : kChunk;
-        uint32_t pix[kPix], pix1[TWO ? kPix : 1];  // rows y and, with paints or coverage, y + 1
-        float acc[COMPOSE ? kPix : 1][4], acc1[(COMPOSE && TWO) ? kPix : 1][4];
+        uint32_t pix[PIX], pix1[TWO ? PIX : 1];  // rows y and, with paints or coverage, y + 1
+        float acc[COMPOSE ? PIX : 1][4], acc1[(COMPOSE && TWO) ? PIX : 1][4];
 #pragma unroll
-        for (int j = 0; j < kPix; ++j) {
+        for (int j = 0; j < PIX; ++j) {
             pix[j] = 0xFFFFFFFFu;
             if constexpr (TWO != 0) pix1[j] = 0xFFFFFFFFu;
             if constexpr (COMPOSE != 0) { acc[j][0] = acc[j][1] = acc[j][2] = acc[j][3] = 1.0f; }
@@ -232,10 +237,10 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
             // ordered list of the paths with records on this cell row, 256 candidates at a time
             // until the list could overflow: batches are then full whatever the path numbering
             int n = 0;
-            while (p0 < pend && n + kBlock <= kList) {
+            while (p0 < pend && n + kListStep <= kList) {
                 const uint32_t p = p0 + tid;
                 int2 rng = make_int2(0, 0);
-                if (p < pend) {
+                if (tid < kListStep && p < pend) {
                     const int2 pr = prow[p];
                     const int loc = r == 0 ? pr.y + 1 : ((r >= pr.x && r < pr.x + pr.y) ? r - pr.x : -1);
                     if (loc >= 0) rng = brec[bucket_base[p] + loc];
@@ -250,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 __syncthreads();
                 int off = n, add = 0;
 #pragma unroll
-                for (int k = 0; k < kBlock / 64; ++k) {
+                for (int k = 0; k < BLOCK / 64; ++k) {
                     if (k < wave) off += wcount[k];
                     add += wcount[k];
                 }
@@ -262,19 +267,19 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 }
                 __syncthreads();
                 n += add;
-                p0 += kBlock;
+                p0 += kListStep;
             }
             int qn = 0;
             for (int q0 = 0; q0 < n; q0 += qn) {
                 qn = n - q0 < kBatch ? n - q0 : kBatch;
             batch:
                 if constexpr (AA != 0) {
-                    for (int k = tid; k < kChunk / 4; k += kBlock) lcnt[k] = 0u;
+                    for (int k = tid; k < kChunk / 4; k += BLOCK) lcnt[k] = 0u;
                     if (tid == 0) lovf = 0;
                 }
                 // clear the event words; the batch's colours; its records as one flat index space
 #pragma unroll
-                for (int j = 0; j < kPix; ++j) ev[j * kBlock + tid] = 0ull;
+                for (int j = 0; j < PIX; ++j) ev[j * BLOCK + tid] = 0ull;
                 if (wave == 0) {
                     if constexpr (COMPOSE != 0) {
                         if (lane < qn) lsa[lane] = premul(lcol[q0 + lane]);
@@ -295,13 +300,13 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 }
                 __syncthreads();
                 const int total = bpre[qn];
-                for (int f0 = tid; f0 < total; f0 += kBlock * kUnroll) {
+                for (int f0 = tid; f0 < total; f0 += BLOCK * kUnroll) {
                     uint32_t rec[kUnroll];
                     int qi[kUnroll];
                     int64_t ri[kUnroll];
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u) {
-                        const int f = f0 + u * kBlock;
+                        const int f = f0 + u * BLOCK;
                         qi[u] = -1;
                         if (f < total) {
                             int lo = 0, hi = qn;
@@ -323,8 +328,8 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                         xs = xs < 0 ? 0 : xs;
                         xe = xe > cw ? cw : xe;
                         if (xs < xe) {
-                            atomicXor(&ev[ev_slot(xs)], bit);
-                            if (xe < cw) atomicXor(&ev[ev_slot(xe)], bit);  // ends past the chunk change nothing
+                            atomicXor(&ev[ev_slot<PIX>(xs)], bit);
+                            if (xe < cw) atomicXor(&ev[ev_slot<PIX>(xe)], bit);  // ends past the chunk change nothing
                         }
                         if constexpr (AA != 0) {
                             const unsigned long long cm = cmask[ri[u]];
@@ -336,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                                     if (col < 0 || col >= cw) continue;
                                     const uint32_t n0 = __popcll((cm >> (16 * px)) & 0xFFFFull);
                                     const uint32_t n1 = __popcll((cm >> (16 * (2 + px))) & 0xFFFFull);
-                                    const int sl = ev_slot(col);
+                                    const int sl = ev_slot<PIX>(col);
                                     const uint32_t k = (atomicAdd(&lcnt[col >> 2], 1u << (8 * (col & 3))) >>
                                                         (8 * (col & 3))) & 0xFFu;
                                     if (k < (uint32_t)kAaSlots)
@@ -359,17 +364,15 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
                 // coverage = prefix XOR of the events, then the fold in path order
                 u64 run = 0;
 #pragma unroll
-                for (int j = 0; j < kPix; ++j) run ^= ev[j * kBlock + tid];
-                u64 cov = block_exclusive_xor(run, xlds);
+                for (int j = 0; j < PIX; ++j) run ^= ev[j * BLOCK + tid];
+                u64 cov = block_exclusive_xor<BLOCK>(run, xlds);
                 const u64 gmask = GRAD ? lgrad : 0ull;
-                uint32_t cnt4[4] = {0u, 0u, 0u, 0u};  // AA: the partial-entry counts of the 16 columns
-                if constexpr (AA != 0) {
-                    const uint4 c4 = reinterpret_cast<const uint4*>(lcnt)[tid];
-                    cnt4[0] = c4.x; cnt4[1] = c4.y; cnt4[2] = c4.z; cnt4[3] = c4.w;
-                }
+                uint32_t cnt4[PIX / 4];  // AA: the partial-entry counts of the thread's columns, bytes
 #pragma unroll
-                for (int j = 0; j < kPix; ++j) {
-                    cov ^= ev[j * kBlock + tid];
+                for (int k = 0; k < PIX / 4; ++k) cnt4[k] = AA ? lcnt[tid * (PIX / 4) + k] : 0u;
+#pragma unroll
+                for (int j = 0; j < PIX; ++j) {
+                    cov ^= ev[j * BLOCK + tid];
                     u64 mm = cov;
                     if (mm == 0ull) continue;
                     const int x = c0 + xb + j;  // the pixel's canvas column
@@ -382,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
 #pragma unroll
                         for (int k = 0; k < kAaSlots; ++k) pe[k] = 0u;
                         if (np) {
-                            const int sl = j * kBlock + tid;
+                            const int sl = j * BLOCK + tid;
 #pragma unroll
                             for (int k = 0; k < kAaSlots; ++k) {
                                 if ((uint32_t)k < np) {
@@ -454,27 +457,27 @@ __global__ __launch_bounds__(kBlock) void k_composite(const uint32_t* __restrict
         };
         if constexpr (COMPOSE != 0) {
 #pragma unroll
-            for (int j = 0; j < kPix; ++j) {
+            for (int j = 0; j < PIX; ++j) {
                 pix[j] = round8(acc[j]);
                 if constexpr (TWO != 0) pix1[j] = round8(acc1[j]);
             }
         }
         for (int yy = y; yy < y + 2 && yy < h; ++yy) {
             uint32_t* row = reinterpret_cast<uint32_t*>(image) + (int64_t)(h - 1 - yy) * w + c0;
-            uint32_t out[kPix];
+            uint32_t out[PIX];
 #pragma unroll
-            for (int j = 0; j < kPix; ++j) {
+            for (int j = 0; j < PIX; ++j) {
                 if constexpr (TWO != 0) out[j] = yy == y ? pix[j] : pix1[j];
                 else out[j] = pix[j];
             }
-            if (xb + kPix <= cw && (((uintptr_t)(row + xb)) & 15u) == 0) {
+            if (xb + PIX <= cw && (((uintptr_t)(row + xb)) & 15u) == 0) {
                 uint4* q = reinterpret_cast<uint4*>(row + xb);
 #pragma unroll
-                for (int k = 0; k < kPix / 4; ++k)
+                for (int k = 0; k < PIX / 4; ++k)
                     q[k] = make_uint4(out[4 * k], out[4 * k + 1], out[4 * k + 2], out[4 * k + 3]);
             } else {
 #pragma unroll
-                for (int j = 0; j < kPix; ++j)
+                for (int j = 0; j < PIX; ++j)
                     if (xb + j < cw) row[xb + j] = out[j];
             }
         }
@@ -492,9 +495,11 @@ hipError_t launch_composite(const uint32_t* records, const int2* brec, const int
     if (rows == 0 || (!docs && w == 0)) return hipSuccess;
     if (cmask && !compose) return hipErrorInvalidValue;  // coverage needs the src-over blend
     const PaintArgs pa = paint ? *paint : PaintArgs{nullptr, nullptr, nullptr, nullptr};
+// pixels per thread: the coverage variants keep two rows of accumulators, so they run 8 per
+// thread in workgroups of 512 threads (without paints: at most 128 VGPRs, 4 waves per SIMD)
 #define SPR_COMPOSITE(C, G, A)                                                                              \
-    k_composite<C, G, A><<<rows, kBlock, 0, s>>>(records, brec, bucket_base, prow, fill, Np, w, h, rec_cap,     \
-                                                 image, docs, n_docs, pa, cmask)
+    k_composite<C, G, A, (A) ? 8 : kPixOne><<<rows, kChunk / ((A) ? 8 : kPixOne), 0, s>>>(                    \
+        records, brec, bucket_base, prow, fill, Np, w, h, rec_cap, image, docs, n_docs, pa, cmask)
     if (cmask) {
         if (paint) SPR_COMPOSITE(1, 1, 1); else SPR_COMPOSITE(1, 0, 1);
     } else if (compose) {
