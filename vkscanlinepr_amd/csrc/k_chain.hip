@@ -27,9 +27,10 @@
 //  5. k_piece_fragments: one wave per listed piece stages its two chains in LDS, merges them with
 //     its start in the reference's order (merge-path when both are non-decreasing, a
 //     concatenation when both are non-increasing, the sequential walk otherwise), applies the
-//     equal-t OR-merge there and produces the same fragment outputs.  Pieces that wait on heavy
-//     chains or hold more than 1024 entries go to k_piece_heavy, which does the same with a
-//     whole workgroup per piece and more LDS.
+//     equal-t OR-merge there and produces the same fragment outputs: the light list beside the
+//     heavy solve, the list of pieces that wait on heavy chains after it.  Pieces of more than
+//     1024 entries (and short ones whose next piece waits on a heavy chain) go to k_piece_heavy,
+//     which does the same with a whole workgroup per piece and more LDS.
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -694,10 +695,11 @@ __device__ inline void queue_append(int32_t* __restrict__ list, int32_t* __restr
 // LDS (entry i of lane l at i * 64 + l), walks the reference's sequential merge
 // (make_intersection_1.comp:310-359, ties to x), OR-merges runs of equal masked t on the fly
 // (:361-375: an entry is the OR of its run up to itself and the next entry when that is in the
-// run) and emits each entry's fragment once the next entry is known.  Longer pieces go to the
-// lo lists when their chains are light and they hold at most kWaveStage entries (one list of up
-// to 64 slots per group of 64 slots, no atomics; one wave per piece later), else to the heavy
-// queue (one workgroup per piece once every chain is solved).
+// run) and emits each entry's fragment once the next entry is known.  Longer pieces of at most
+// kWaveStage entries are finished one wave per piece later: listed when their chains are light
+// (one lo list of up to 64 slots per group of 64 slots, no atomics), queued when they wait on
+// heavy chains.  The rest go to the heavy queue (one workgroup per piece once every chain is
+// solved).
 __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restrict__ pieces, int64_t n_slots,
                                                         const uint8_t* __restrict__ mono,
                                                         const float* __restrict__ chain_t,
@@ -725,13 +727,17 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
         const bool light = nx < kHeavyCrossings && ny < kHeavyCrossings && (pd.tend >= 0.0f || next_ok);
         const bool mine = n > 0 && n <= kLaneMax && light;
         const bool to_lo = n > kLaneMax && n <= kWaveStage && light;
-        const bool to_heavy = n > 0 && !mine && !to_lo;
+        const bool to_lo_h = n > kLaneMax && n <= kWaveStage && !light;
+        const bool to_heavy = n > 0 && !mine && !to_lo && !to_lo_h;
         {
             const int64_t group = (slot - lane) / 64;  // this wave's 64 slots
             const unsigned long long ml = __ballot(to_lo);
             if (to_lo) L.lo[group * 64 + __popcll(ml & ((1ull << lane) - 1ull))] = (int32_t)slot;
             if (lane == 0 && group * 64 < n_slots) L.lo_cnt[group] = (int32_t)__popcll(ml);
         }
+        // the pieces that wait on heavy chains come from a few long curves, in consecutive slots:
+        // queued, so that the waves that finish them stride over them
+        queue_append(L.lo_h, L.lo_h_cnt, (int32_t)slot, to_lo_h);
         queue_append(L.heavy, L.heavy_count, (int32_t)slot, to_heavy);
         float tend = 1.0f;
         uint32_t path = 0, bb = 0;
@@ -844,30 +850,18 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
     }
 }
 
-// One wave (a workgroup of its own) per quarter of a lo list (the lo pieces of 64 consecutive
-// slots): wave w of list g takes the list's pieces w, w + 4, ...; lane k gathers the meta of the
-// wave's k-th piece up front, so the wave pays that chain of loads once.  The chains of the next
-// piece are loaded into registers while the current one is merged, OR-merged and turned into
-// fragments in the wave's LDS.
-constexpr int kLoQuarters = 4;
-__global__ __launch_bounds__(64) void k_piece_fragments(const PieceDesc* __restrict__ pieces,
-                                                        const int32_t* __restrict__ lo_list,
-                                                        const int32_t* __restrict__ lo_cnt, int64_t n_groups,
-                                                        const uint8_t* __restrict__ mono,
-                                                        const float* __restrict__ chain_t, const FragArgs A) {
-    __shared__ float xin[kWaveStage];
-    __shared__ uint32_t vout[kWaveStage];
+// One wave finishes `mine` pieces, slots list[k * stride] (k < mine <= 64): lane k gathers the
+// meta of the k-th piece up front, so the wave pays that chain of loads once.  The chains of the
+// next piece are loaded into registers while the current one is merged, OR-merged and turned
+// into fragments in the wave's LDS (xin, vout: kWaveStage values each).
+__device__ inline void wave_pieces(const int32_t* __restrict__ list, int stride, int mine,
+                                   const PieceDesc* __restrict__ pieces, const uint8_t* __restrict__ mono,
+                                   const float* __restrict__ chain_t, const FragArgs& A, float* xin, uint32_t* vout) {
     constexpr int kPre = kWaveStage / 64;
     uint32_t* fin = reinterpret_cast<uint32_t*>(xin);  // the staged chains are dead after the merge
-    const int lane = threadIdx.x;
-    const int64_t g = blockIdx.x / kLoQuarters;
-    const int q = (int)(blockIdx.x % kLoQuarters);
-    if (g >= n_groups) return;
-    const int32_t cnt = lo_cnt[g];
-    const int32_t mine = (cnt - q + kLoQuarters - 1) / kLoQuarters;
-    if (mine <= 0) return;
+    const int lane = threadIdx.x & 63;
     PieceMeta mm;
-    if (lane < mine) mm = gather_meta(pieces, lo_list[g * 64 + q + kLoQuarters * lane], mono, chain_t, A);
+    if (lane < mine) mm = gather_meta(pieces, list[(int64_t)stride * lane], mono, chain_t, A);
     // the chains of piece k: values base + 1 .. base + nv, lane + 64 j held in pre[j]
     float pre[kPre];
     auto fetch = [&](int k) {
@@ -898,8 +892,44 @@ __global__ __launch_bounds__(64) void k_piece_fragments(const PieceDesc* __restr
     }
 }
 
-// One workgroup per piece of the heavy queue (pieces that wait on heavy chains, and pieces of
-// more than kWaveStage entries), its four waves together.  Every wave gathers the meta of the
+// One wave (a workgroup of its own) per quarter of a lo list (the lo pieces of 64 consecutive
+// slots): wave w of list g takes the list's pieces w, w + 4, ...
+constexpr int kLoQuarters = 4;
+__global__ __launch_bounds__(64) void k_piece_fragments(const PieceDesc* __restrict__ pieces,
+                                                        const int32_t* __restrict__ lo_list,
+                                                        const int32_t* __restrict__ lo_cnt, int64_t n_groups,
+                                                        const uint8_t* __restrict__ mono,
+                                                        const float* __restrict__ chain_t, const FragArgs A) {
+    __shared__ float xin[kWaveStage];
+    __shared__ uint32_t vout[kWaveStage];
+    const int64_t g = blockIdx.x / kLoQuarters;
+    const int q = (int)(blockIdx.x % kLoQuarters);
+    if (g >= n_groups) return;
+    const int32_t cnt = lo_cnt[g];
+    const int32_t mine = (cnt - q + kLoQuarters - 1) / kLoQuarters;
+    if (mine <= 0) return;
+    wave_pieces(lo_list + g * 64 + q, kLoQuarters, mine, pieces, mono, chain_t, A, xin, vout);
+}
+
+// One wave (a workgroup of its own) per gridDim.x-th of a queue: wave b takes the queue's pieces
+// b, b + gridDim.x, ..., 64 at a time.
+__global__ __launch_bounds__(64) void k_piece_waves(const PieceDesc* __restrict__ pieces,
+                                                    const int32_t* __restrict__ queue,
+                                                    const int32_t* __restrict__ queue_count,
+                                                    const uint8_t* __restrict__ mono,
+                                                    const float* __restrict__ chain_t, const FragArgs A) {
+    __shared__ float xin[kWaveStage];
+    __shared__ uint32_t vout[kWaveStage];
+    const int32_t cnt = *queue_count;
+    const int32_t W = (int32_t)gridDim.x;
+    for (int32_t q0 = (int32_t)blockIdx.x; q0 < cnt; q0 += 64 * W) {
+        const int32_t mine = min(64, (cnt - q0 + W - 1) / W);
+        wave_pieces(queue + q0, W, mine, pieces, mono, chain_t, A, xin, vout);
+    }
+}
+
+// One workgroup per piece of the heavy queue (pieces of more than kWaveStage entries, and short
+// pieces whose last segment waits on a heavy chain of the next piece), its four waves together.  Every wave gathers the meta of the
 // workgroup's pieces (lane k: piece blockIdx.x + k * gridDim.x of the queue).  Pieces up to
 // kStage entries are staged in LDS; longer ones are merged into scratch[base..] in global memory
 // and OR-merged into scratch[n_entries + base..].
@@ -1005,6 +1035,13 @@ hipError_t launch_piece_lo(const PieceDesc* pieces, uint32_t C, const uint8_t* m
     if (C == 0) return hipSuccess;
     const int64_t groups = ((int64_t)C * 5 + 63) / 64;
     k_piece_fragments<<<(unsigned)(groups * kLoQuarters), 64, 0, s>>>(pieces, list, cnt, groups, mono, chain_t, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_piece_waves(const PieceDesc* pieces, const uint8_t* mono, const float* chain_t,
+                              const int32_t* queue, const int32_t* queue_count, const FragArgs& A, int n_cu,
+                              hipStream_t s) {
+    k_piece_waves<<<(unsigned)(n_cu * 16), 64, 0, s>>>(pieces, queue, queue_count, mono, chain_t, A);
     return hipGetLastError();
 }
 

@@ -84,11 +84,12 @@ struct FragArgs {
 // Chain length classes: cubic chains by quarter octave of their length (16 octaves x 4), then
 // line chains by octave.  The chain control block holds, as int32: class counts, class fill
 // counters, class offsets [kChainClasses + 1], the number of wave-solved chains, the work counter
-// of the persistent solve and the count of queued heavy pieces.
+// of the persistent solve, the count of queued heavy pieces, the first light class's place, the
+// light solve's work counter and the count of queued wave pieces that wait on heavy chains.
 constexpr int kChainClasses = 80;
 constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 * kChainClasses + 1,
               kCcWork = kCcNLong + 1, kCcHeavyPieces = kCcWork + 1, kCcLightBegin = kCcHeavyPieces + 1,
-              kCcWorkLight = kCcLightBegin + 1, kCcWords = kCcWorkLight + 1;
+              kCcWorkLight = kCcLightBegin + 1, kCcWavePieces = kCcWorkLight + 1, kCcWords = kCcWavePieces + 1;
 // Chains of at least kHeavyCrossings crossings are "heavy": they are solved by their own launch
 // (wave-solved and the longest lane-solved chains), and the pieces of at most kHeavyCrossings
 // entries, which never wait on them, are finished by a concurrent stream meanwhile.
@@ -127,8 +128,10 @@ hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const fl
 //  launch_piece_heavy: one workgroup per piece of the heavy queue, after every chain is solved;
 //    scratch holds 2 * n_entries words.
 struct PieceLists {
-    int32_t* lo;       // [groups * 64] slots
+    int32_t* lo;       // [groups * 64] slots: pieces whose chains are all light
     int32_t* lo_cnt;   // [groups]
+    int32_t* lo_h;     // [slots] queue: pieces of up to kWaveStage entries that wait on a heavy chain
+    int32_t* lo_h_cnt; // its length
     int32_t* heavy;    // [slots]
     int32_t* heavy_count;
 };
@@ -136,6 +139,10 @@ hipError_t launch_piece_small(const PieceDesc* pieces, uint32_t C, const uint8_t
                               const PieceLists& L, const FragArgs& A, int n_cu, hipStream_t s);
 hipError_t launch_piece_lo(const PieceDesc* pieces, uint32_t C, const uint8_t* mono, const float* chain_t,
                            const int32_t* list, const int32_t* cnt, const FragArgs& A, hipStream_t s);
+// One wave per piece of a queue (the pieces that wait on heavy chains), the waves striding over it.
+hipError_t launch_piece_waves(const PieceDesc* pieces, const uint8_t* mono, const float* chain_t,
+                              const int32_t* queue, const int32_t* queue_count, const FragArgs& A, int n_cu,
+                              hipStream_t s);
 hipError_t launch_piece_heavy(const PieceDesc* pieces, const PieceLists& L, const uint8_t* mono,
                               const float* chain_t, uint32_t* scratch, int64_t n_entries, const FragArgs& A, int n_cu,
                               hipStream_t s);

@@ -161,7 +161,7 @@ struct spr_handle {
     DBuf<float> chain_t;
     DBuf<spr::ChainDesc> chains;
     DBuf<spr::PieceDesc> pieces;
-    DBuf<int32_t> chain_order, heavy_list, lo_list, lo_cnt;
+    DBuf<int32_t> chain_order, heavy_list, lo_list, lo_cnt, wave_list;
     DBuf<uint8_t> mono, chain_cls;
     int n_cu = 256;
     DBuf<int32_t> key;
@@ -338,7 +338,8 @@ void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragA
     const int32_t* d_coff = d_cc + spr::kCcOff;
     const int32_t* d_nlong = d_cc + spr::kCcNLong;
     const int32_t* d_lb = d_cc + spr::kCcLightBegin;
-    const spr::PieceLists L{H->lo_list.p, H->lo_cnt.p, H->heavy_list.p, d_cc + spr::kCcHeavyPieces};
+    const spr::PieceLists L{H->lo_list.p, H->lo_cnt.p, H->wave_list.p, d_cc + spr::kCcWavePieces, H->heavy_list.p,
+                            d_cc + spr::kCcHeavyPieces};
     SPR_HIP(hipEventRecord(H->ev_fork, s));
     SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->arcw(), H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 1,
                                     d_cc + spr::kCcWork, H->chain_t.p, H->mono.p, n_cu_solve, s));
@@ -354,6 +355,8 @@ void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragA
     SPR_HIP(hipStreamWaitEvent(s, H->ev_light, 0));
     if (s2 != s) stage_mark(H, 5);
     SPR_HIP(hipStreamWaitEvent(s, H->ev_small, 0));
+    // the pieces of up to kWaveStage entries that wait on heavy chains: one wave each
+    SPR_HIP(spr::launch_piece_waves(H->pieces.p, H->mono.p, H->chain_t.p, L.lo_h, L.lo_h_cnt, A, n_cu_merge, s));
     // pieces longer than the LDS stage merge through the sort buffer, which is free until the scatter
     SPR_HIP(spr::launch_piece_heavy(H->pieces.p, L, H->mono.p, H->chain_t.p, (uint32_t*)H->sbig.p, n_entries, A,
                                     n_cu_merge, s));
@@ -480,6 +483,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         H->heavy_list.ensure((size_t)C * 5);
         H->lo_list.ensure((size_t)C * 5 + 64);
         H->lo_cnt.ensure(((size_t)C * 5 + 63) / 64);
+        H->wave_list.ensure((size_t)C * 5);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         SPR_HIP(spr::launch_chain_prep(H->tpos.p, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->visible.p, H->cuts.p,
@@ -812,6 +816,7 @@ void run_learned_frame(spr_handle* H) {
         H->heavy_list.ensure((size_t)C * 5);
         H->lo_list.ensure((size_t)C * 5 + 64);
         H->lo_cnt.ensure(((size_t)C * 5 + 63) / 64);
+        H->wave_list.ensure((size_t)C * 5);
         H->chain_cls.ensure((size_t)C * 10);
         H->mono.ensure((size_t)C * 10);
         const spr::Budget B{nl, d_live, d_over};
@@ -1218,7 +1223,7 @@ int spr_destroy(spr_handle* H) {
     H->rule.release(); H->fill.release(); H->tpos.release(); H->visible.release(); H->cuts.release();
     H->ncuts.release(); H->counts.release(); H->seg.release(); H->inter.release();
     H->chain_t.release(); H->chains.release(); H->pieces.release(); H->chain_order.release();
-    H->heavy_list.release(); H->lo_list.release(); H->lo_cnt.release(); H->chain_cls.release(); H->mono.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
+    H->heavy_list.release(); H->lo_list.release(); H->lo_cnt.release(); H->wave_list.release(); H->chain_cls.release(); H->mono.release(); H->key.release(); H->wn.release(); H->epath.release(); H->sorted.release();
     H->sbig.release(); H->stmp.release(); H->skey.release(); H->sidx.release();
     H->winding.release(); H->flags.release(); H->records.release(); H->crec.release();
     H->cell_first.release(); H->cmask.release(); H->chord.release();
