@@ -225,8 +225,9 @@ def test_over_budget_frame_renders_plain(spr, trained):
 
 
 def test_mvp_changes_replay_the_captured_frame(spr, trained):
-    """Interactive use: a new MVP every frame.  The captured frame reads the MVP at each replay, so
-    pans, a rotation and zooms out are replays of the one graph (no new capture), each identical
+    """Interactive use: a new MVP every frame.  Still frames replay a graph without the prediction;
+    the first new MVP captures the graph with it, which reads the MVP at each replay, so the pans,
+    a rotation and zooms out after it are replays of that graph (no new capture), each identical
     to the plain frame with that MVP; a zoom in beyond the allocation falls back to the plain
     method once, and the frames after it are captured and replayed again."""
     r, _, path = trained
@@ -245,7 +246,7 @@ def test_mvp_changes_replay_the_captured_frame(spr, trained):
     # the scene shrunk to 15%: every view below fits the predicted allocation (4x margin)
     q.set_mvp(view(0.15, 0.0, 100.0, 100.0))
     p.set_mvp(view(0.15, 0.0, 100.0, 100.0))
-    for _ in range(3):  # eager with the prediction, eager + capture, replay
+    for _ in range(3):  # eager with the prediction, eager + capture (no prediction), replay
         q.render(readback=False)
     st = q.stats()
     assert st.graph_captures == 1
@@ -258,7 +259,8 @@ def test_mvp_changes_replay_the_captured_frame(spr, trained):
         ip = p.render()
         assert np.array_equal(q.render(), ip), k
         st = q.stats()
-        assert st.learned_fallbacks == 0 and st.graph_captures == 1, k  # a replay of the one graph
+        # the first new MVP: eager + capture of the frame with the prediction; then its replays
+        assert st.learned_fallbacks == 0 and st.graph_captures == 2, k
         p.render(readback=False)  # eager work of another handle between replays
     n_fb = q.stats().learned_fallbacks
     big = np.eye(4, dtype=np.float32)  # the full-size scene: about 6.7 times the entries

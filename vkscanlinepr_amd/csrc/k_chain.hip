@@ -166,43 +166,41 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
 // come first: the wave-solved chains.
 __device__ inline int class_cost_key(int c) { return c < 64 ? 2 * (c + 24) + 1 : 8 * (c - 64); }
 
-// One thread per class: its offset is the number of chains in the classes of higher cost key
-// (the keys are distinct).  The wave-solved classes and the heavy ones come first, so their
-// totals are also the ends of those groups.
-__global__ void k_chain_offsets(const int32_t* __restrict__ class_count, int32_t* __restrict__ class_off,
-                                int32_t* __restrict__ n_long, int32_t* __restrict__ light_begin) {
-    __shared__ int32_t cnt[kClasses];
-    for (int c = threadIdx.x; c < kClasses; c += blockDim.x) cnt[c] = class_count[c];  // loads at once
-    __syncthreads();
-    for (int c = threadIdx.x; c < kClasses; c += blockDim.x) {
-        const int key = class_cost_key(c);
-        int32_t off = 0;
-        for (int d = 0; d < kClasses; ++d) off += class_cost_key(d) > key ? cnt[d] : 0;
-        class_off[c] = off;
+// Chains in class order: each workgroup computes the class offsets (a class's offset is the
+// number of chains in the classes of higher cost key; the keys are distinct, and the wave-solved
+// classes and the heavy ones come first, so their totals are also the ends of those groups;
+// workgroup 0 stores them for the solve), owns a contiguous range of chain slots, counts its
+// chains per class (from the one-byte classes written by k_chain_prep), takes one block of places
+// per class with a single atomic, then writes the chain indices.
+__global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restrict__ chain_cls, int64_t n_slots,
+                                                        const int32_t* __restrict__ class_count,
+                                                        int32_t* __restrict__ class_fill,
+                                                        int32_t* __restrict__ class_off, int32_t* __restrict__ n_long,
+                                                        int32_t* __restrict__ light_begin,
+                                                        int32_t* __restrict__ order) {
+    __shared__ int32_t cnt[kClasses], base[kClasses], off[kClasses], total[kClasses];
+    if (threadIdx.x < kClasses) {
+        total[threadIdx.x] = class_count[threadIdx.x];
+        cnt[threadIdx.x] = 0;
     }
-    if (threadIdx.x == 0) {
-        int32_t total = 0, nl = 0, lb = 0;
+    __syncthreads();
+    if (threadIdx.x < kClasses) {
+        const int key = class_cost_key(threadIdx.x);
+        int32_t o = 0;
+        for (int d = 0; d < kClasses; ++d) o += class_cost_key(d) > key ? total[d] : 0;
+        off[threadIdx.x] = o;
+        if (blockIdx.x == 0) class_off[threadIdx.x] = o;
+    } else if (blockIdx.x == 0 && threadIdx.x == kClasses) {
+        int32_t all = 0, nl = 0, lb = 0;
         for (int d = 0; d < kClasses; ++d) {
-            total += cnt[d];
-            if (d < 64 && d >= 4 * kLongClass) nl += cnt[d];
-            if (d < 64 && d >= kHeavyClass) lb += cnt[d];
+            all += total[d];
+            if (d < 64 && d >= 4 * kLongClass) nl += total[d];
+            if (d < 64 && d >= kHeavyClass) lb += total[d];
         }
-        class_off[kClasses] = total;
+        class_off[kClasses] = all;
         *n_long = nl;
         *light_begin = lb;
     }
-}
-
-// Chains in class order: each workgroup owns a contiguous range of chain slots, counts its
-// chains per class (from the one-byte classes written by k_chain_prep), takes one block of
-// places per class with a single atomic, then writes the chain indices.
-__global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restrict__ chain_cls, int64_t n_slots,
-                                                        int32_t* __restrict__ class_fill,
-                                                        const int32_t* __restrict__ class_off,
-                                                        int32_t* __restrict__ order) {
-    __shared__ int32_t cnt[kClasses], base[kClasses];
-    if (threadIdx.x < kClasses) cnt[threadIdx.x] = 0;
-    __syncthreads();
     const int64_t per = ((n_slots + gridDim.x - 1) / gridDim.x + kBlock - 1) / kBlock * kBlock;
     const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < n_slots ? b0 + per : n_slots;
     for (int64_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
@@ -217,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restric
     __syncthreads();
     for (int64_t i = b0 + threadIdx.x; i < b1; i += kBlock) {
         const uint32_t k = chain_cls[i];
-        if (k != 0xFFu) order[class_off[k] + base[k] + atomicAdd(&cnt[k], 1)] = (int32_t)i;
+        if (k != 0xFFu) order[off[k] + base[k] + atomicAdd(&cnt[k], 1)] = (int32_t)i;
     }
 }
 
@@ -1001,10 +999,10 @@ hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_
                               int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* light_begin,
                               int32_t* order, int n_cu, hipStream_t s) {
     if (C == 0) return hipSuccess;
-    k_chain_offsets<<<1, 64, 0, s>>>(class_count, class_off, n_long, light_begin);
     unsigned g = grid_for((int64_t)C * 10);
     if (g > (unsigned)n_cu * 4) g = (unsigned)n_cu * 4;
-    k_chain_order<<<g, kBlock, 0, s>>>(chain_cls, (int64_t)C * 10, class_fill, class_off, order);
+    k_chain_order<<<g, kBlock, 0, s>>>(chain_cls, (int64_t)C * 10, class_count, class_fill, class_off, n_long,
+                                       light_begin, order);
     return hipGetLastError();
 }
 

@@ -233,11 +233,13 @@ struct spr_handle {
     DBuf<spr::DocDev> docs;
     DBuf<int32_t> doc_first, wbase;
 
-    // captured frame (no-host-sync frames only) and launch knobs picked by spr_autotune
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int64_t graph_lb_n = -1, graph_lb_r = -1;
+    // captured frames (no-host-sync frames only), one without and one with the prediction, and
+    // launch knobs picked by spr_autotune
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int64_t graph_lb_n[2] = {-1, -1}, graph_lb_r[2] = {-1, -1};
     bool capturing = false;
+    bool cap_predict = false;                       // the frame being captured runs the prediction
     int chain_k = 8, merge_k = 8;                   // persistent blocks per CU
     int scan_shape = 0;                             // tile shape of the large scans (spr::kScanShapes)
     int band_window = spr::kBandWindowLog;          // log2 of the band window (9 or 10)
@@ -641,12 +643,15 @@ void learned_finish(spr_handle* H);
 // new MVP; always without a model, where only the features run), published in the pinned words
 // that the frame copies to the device with the MVP.  Every earlier frame has ended (frames end with
 // a stream synchronisation), so no copy is reading them.
-void publish_prediction(spr_handle* H) {
-    const bool predict = !H->model.ready || H->need_prediction || H->pred_stale;
+bool prediction_due(const spr_handle* H) { return !H->model.ready || H->need_prediction || H->pred_stale; }
+
+bool publish_prediction(spr_handle* H) {
+    const bool predict = H->capturing ? H->cap_predict : prediction_due(H);
     H->pred_pinned()[0] = predict ? 0 : 1;
     H->pred_pinned()[1] = predict ? (int32_t)H->Np : 0;
     H->last_predicted = predict;
     if (predict && !H->capturing) H->pred_stale = false;
+    return predict;
 }
 
 // Enqueues one frame without any host synchronisation unless the predicted budget must first be
@@ -699,8 +704,9 @@ void run_learned_frame(spr_handle* H) {
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_lo.p, Np, 0x7FFFFFFFu, s));
     SPR_HIP(spr::launch_fill_u32((uint32_t*)H->row_hi.p, Np, 0x80000000u, s));
     // the prediction is a function of the scene, the MVP and the model: it runs after a change of
-    // one of them and is otherwise skipped on the device (its outputs still hold)
-    publish_prediction(H);
+    // one of them and is otherwise left out (its outputs still hold); a captured frame with it
+    // reads the same decision from the pinned words at every replay
+    const bool predict = publish_prediction(H);
     // the MVP goes through a copy from pinned host memory, which a captured frame repeats at every
     // replay: spr_set_mvp only rewrites that memory, and the graph follows it
     SPR_HIP(hipMemcpyAsync(H->mvp_dev.p, H->mvp_pinned(), kMvpBytes, hipMemcpyHostToDevice, s));
@@ -710,22 +716,25 @@ void run_learned_frame(spr_handle* H) {
                                   H->row_lo.p, H->row_hi.p, s, pcv, H->mvp_dev.p));
     stage_mark(H, 1);
     int32_t* d_capr = reinterpret_cast<int32_t*>(H->zero.p + z.capr);
+    if (!predict) s2 = s;  // nothing to run beside the count
     if (s2 != s) {
         SPR_HIP(hipEventRecord(H->ev_pfork, s));
         SPR_HIP(hipStreamWaitEvent(s2, H->ev_pfork, 0));
     }
-    SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p, H->rule.p, Np,
-                                      w, h, H->feat.p, s2, H->zero.p + z.bound, pcv, H->pbound.p, d_skip));
-    if (use_model) {
-        SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s2, d_skip));
-        SPR_HIP(spr::launch_caps(H->pred.p, H->visible.p, Np, H->model.log_margin, H->model.slack, H->arena.p,
-                                 H->capn.p, d_capr, s2, H->pbound.p, d_skip));
-    } else {
-        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->arena.p, (int64_t)Np + 1, 0u, s2));
-        SPR_HIP(spr::launch_fill_u32((uint32_t*)H->capn.p, (int64_t)Np + 1, 0u, s2));
+    if (predict) {
+        SPR_HIP(spr::launch_path_features(H->tpos.p, H->cpos.p, H->ctype.p, H->pcurve.p, H->visible.p, H->rule.p,
+                                          Np, w, h, H->feat.p, s2, H->zero.p + z.bound, pcv, H->pbound.p, d_skip));
+        if (use_model) {
+            SPR_HIP(spr::launch_mlp_infer(H->feat.p, Np, H->mlpdev.dev(), H->pred.p, s2, d_skip));
+            SPR_HIP(spr::launch_caps(H->pred.p, H->visible.p, Np, H->model.log_margin, H->model.slack, H->arena.p,
+                                     H->capn.p, d_capr, s2, H->pbound.p, d_skip));
+        } else {
+            SPR_HIP(spr::launch_fill_u32((uint32_t*)H->arena.p, (int64_t)Np + 1, 0u, s2));
+            SPR_HIP(spr::launch_fill_u32((uint32_t*)H->capn.p, (int64_t)Np + 1, 0u, s2));
+        }
+        SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, kLcArena), s2,
+                                     d_arena_n));
     }
-    SPR_HIP(spr::launch_scan_i32(H->arena.p, H->arena.p, Np, ctl_at(H, z.st_arena, z.counters, kLcArena), s2,
-                                 d_arena_n));
     SPR_HIP(spr::launch_count(H->tpos.p, H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->visible.p, C, w, h, H->cuts.p,
                               H->ncuts.p, H->counts.p, s, pcv));
     stage_mark(H, 2);
@@ -734,7 +743,7 @@ void run_learned_frame(spr_handle* H) {
     // the prediction's paths that overflowed their capacity (statistics only): on the second stream
     // once the per-path counts exist, unless this frame reads the prediction back (and re-lays the
     // control region), in which case they are counted at its end
-    const bool ovf_side = use_model && s2 != s && !H->need_prediction;
+    const bool ovf_side = use_model && predict && s2 != s && !H->need_prediction;
     if (ovf_side) {
         SPR_HIP(hipEventRecord(H->ev_pseg, s));
         SPR_HIP(hipStreamWaitEvent(s2, H->ev_pseg, 0));
@@ -891,7 +900,7 @@ void run_learned_frame(spr_handle* H) {
                                   (int)H->n_docs, H->total_rows, paint_args(H, pa)));
     stage_mark(H, 12);
     if (!joined) SPR_HIP(hipStreamWaitEvent(s, H->ev_pjoin, 0));
-    if (use_model && !ovf_side)
+    if (use_model && predict && !ovf_side)
         SPR_HIP(spr::launch_path_overflow(H->seg.p, H->capn.p, H->visible.p, Np, d_novf, s, d_skip));
     spr::SummarySrc S{};
     S.n = kSumCount;
@@ -962,17 +971,23 @@ void learned_finish(spr_handle* H) {
 }
 
 void drop_graph(spr_handle* H) {
-    if (H->gexec) { (void)hipGraphExecDestroy(H->gexec); H->gexec = nullptr; }
-    if (H->graph) { (void)hipGraphDestroy(H->graph); H->graph = nullptr; }
-    H->graph_lb_n = H->graph_lb_r = -1;
+    for (int k = 0; k < 2; ++k) {
+        if (H->gexec[k]) { (void)hipGraphExecDestroy(H->gexec[k]); H->gexec[k] = nullptr; }
+        if (H->graph[k]) { (void)hipGraphDestroy(H->graph[k]); H->graph[k] = nullptr; }
+        H->graph_lb_n[k] = H->graph_lb_r[k] = -1;
+    }
 }
 
-// Records the frame's device work (kernels and the final device-to-host copies) into a graph.
-void capture_frame(spr_handle* H) {
+// Records the frame's device work (kernels and the final device-to-host copies) into graph k:
+// 1 with the prediction, 0 without.
+void capture_frame(spr_handle* H, int k) {
     hipStream_t s = H->stream;
-    drop_graph(H);
+    if (H->gexec[k]) { (void)hipGraphExecDestroy(H->gexec[k]); H->gexec[k] = nullptr; }
+    if (H->graph[k]) { (void)hipGraphDestroy(H->graph[k]); H->graph[k] = nullptr; }
+    H->graph_lb_n[k] = H->graph_lb_r[k] = -1;
     SPR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     H->capturing = true;
+    H->cap_predict = k == 1;
     try {
         run_learned_frame(H);
     } catch (...) {
@@ -984,22 +999,28 @@ void capture_frame(spr_handle* H) {
         throw;
     }
     H->capturing = false;
-    SPR_HIP(hipStreamEndCapture(s, &H->graph));
-    SPR_HIP(hipGraphInstantiate(&H->gexec, H->graph, nullptr, nullptr, 0));
-    H->graph_lb_n = H->lb_n;
-    H->graph_lb_r = H->lb_r;
+    SPR_HIP(hipStreamEndCapture(s, &H->graph[k]));
+    SPR_HIP(hipGraphInstantiate(&H->gexec[k], H->graph[k], nullptr, nullptr, 0));
+    H->graph_lb_n[k] = H->lb_n;
+    H->graph_lb_r[k] = H->lb_r;
     ++H->captures;
 }
 
-// Frames without a host synchronisation inside (learned mode, batches).  While the scene, MVP,
-// model and budget stay the same, the frame is captured once and later frames replay the graph:
-// one launch per frame.
+// The captured frame that the next learned frame would replay: with the prediction when it is
+// due, else without.
+int graph_slot(const spr_handle* H) { return prediction_due(H) ? 1 : 0; }
+
+// Frames without a host synchronisation inside (learned mode, batches).  While the scene, model
+// and budget stay the same, the frame is captured once and later frames replay the graph: one
+// launch per frame.  Frames that run the prediction (after a new MVP: the graph copies it from
+// pinned memory) and frames that do not have a graph each.
 void render_nosync(spr_handle* H) {
     hipStream_t s = H->stream;
-    if (!H->lb_dirty && !H->need_prediction && H->gexec && H->graph_lb_n == H->lb_n && !H->stage_timing) {
+    const int k = graph_slot(H);
+    if (!H->lb_dirty && !H->need_prediction && H->gexec[k] && H->graph_lb_n[k] == H->lb_n && !H->stage_timing) {
         publish_prediction(H);  // read by the graph's copy of the MVP block
         SPR_HIP(hipEventRecord(H->ev_begin, s));
-        SPR_HIP(hipGraphLaunch(H->gexec, s));
+        SPR_HIP(hipGraphLaunch(H->gexec[k], s));
         SPR_HIP(hipEventRecord(H->ev_end, s));
         H->n_marks = 0;
         learned_finish(H);
@@ -1012,7 +1033,7 @@ void render_nosync(spr_handle* H) {
     if (H->last_fallback) return;  // the plain frame rendered it; the next learned frame runs eagerly
     H->lb_dirty = false;
     // capture once a frame ran eagerly on an unchanged budget
-    if (!predicting && H->lb_n == n0 && !H->stage_timing) capture_frame(H);
+    if (!predicting && H->lb_n == n0 && !H->stage_timing) capture_frame(H, k);
 }
 
 template <class F>
@@ -1677,7 +1698,7 @@ int spr_autotune(spr_handle* H, uint32_t reps, spr_tune_report* out) {
                 drop_graph(H);
                 render_nosync(H);  // eager; sizes the budget when needed
                 render_nosync(H);  // eager, then captured
-                if (!H->gexec) render_nosync(H);
+                if (!H->gexec[graph_slot(H)]) render_nosync(H);
             } else {
                 run_frame(H);
             }

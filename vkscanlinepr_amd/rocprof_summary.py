@@ -21,9 +21,9 @@ from pathlib import Path
 STAGE_KERNELS = {
     "transform": ["k_transform"],
     "count": ["k_count"],
-    "chain_prep": ["k_chain_prep", "k_chain_offsets", "k_chain_order"],
+    "chain_prep": ["k_chain_prep", "k_chain_order"],
     "chain_solve": ["k_chain_solve"],
-    "piece_fragments": ["k_piece_small", "k_piece_fragments", "k_piece_heavy"],
+    "piece_fragments": ["k_piece_small", "k_piece_fragments", "k_piece_waves", "k_piece_heavy"],
     "gen_fragment": ["k_gen_fragment"],
     "bucket_scatter": ["k_bucket_scatter", "k_band_flags", "k_band_emit"],
     "band_sort": ["k_band_classify2", "k_sort_bands_wave", "k_sort_big_bands", "k_emit_big_bands",
