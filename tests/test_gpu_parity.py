@@ -86,7 +86,9 @@ def test_production_frames_match_oracle(spr, orc, name):
     r.load_scene(scene)
     img = r.render()
     N = len(f.get("skey"))
-    for stage in ("skey", "winding"):
+    # production frames keep neither the fragment keys nor the sorted ones: the piece kernels store
+    # each entry's bucket and narrow sort word instead
+    for stage in ("key", "wn", "skey", "winding"):
         with pytest.raises(RuntimeError):
             r.export(stage)
     st = r.stats()

@@ -54,9 +54,11 @@ __global__ void k_bucket_counts(const int32_t* __restrict__ row_lo, const int32_
 // slot atomic (one per run of equal bands, on the band's fill counter), of all chunks before it
 // waits on any of them.  The head of a run also adds the run's winding changes to the band's sum
 // (bsum), from which one scan gives every band its exclusive prefix before the band sort.
+// PACKED (narrow only): the piece kernels already stored each entry's global bucket (in key) and
+// its narrow word (in epath), so the path-table gathers are skipped.
 constexpr int kRounds = 4;
 
-template <bool NARROW>
+template <bool NARROW, bool PACKED>
 __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __restrict__ key,
                                                            const int8_t* __restrict__ wn,
                                                            const uint32_t* __restrict__ epath, int64_t N,
@@ -76,24 +78,35 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
     uint32_t pth[kRounds];
     int32_t k[kRounds], wv[kRounds];
     bool valid[kRounds];
+    uint32_t b[kRounds], loc[kRounds];
+    bool sent[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; ++r) {
         const int64_t i = g0 + r * 64 + lane;
         valid[r] = i < n;
         pth[r] = kNoPath; k[r] = 0; wv[r] = 0;
-        if (valid[r]) { pth[r] = epath[i]; k[r] = key[i]; wv[r] = wn[i]; }
-        if (pth[r] == kNoPath) { valid[r] = false; wv[r] = 0; }  // gap of a predicted-allocation layout
-    }
-    uint32_t b[kRounds], loc[kRounds];
-    bool sent[kRounds];
-#pragma unroll
-    for (int r = 0; r < kRounds; ++r) {
         b[r] = 0; loc[r] = 0; sent[r] = false;
-        if (valid[r]) {
-            const int2 pr = prow[pth[r]];
-            loc[r] = bucket_local(k[r], pr, err);
-            sent[r] = loc[r] == (uint32_t)pr.y;
-            b[r] = (uint32_t)bucket_base[pth[r]] + loc[r];
+        if constexpr (PACKED) {
+            // k = the narrow word, b = the global bucket
+            if (valid[r]) {
+                b[r] = (uint32_t)key[i];
+                k[r] = (int32_t)epath[i];
+                wv[r] = (int32_t)((uint32_t)k[r] & 3u) - 1;
+            }
+        } else {
+            if (valid[r]) { pth[r] = epath[i]; k[r] = key[i]; wv[r] = wn[i]; }
+            if (pth[r] == kNoPath) { valid[r] = false; wv[r] = 0; }  // an entry without a fragment
+        }
+    }
+    if constexpr (!PACKED) {
+#pragma unroll
+        for (int r = 0; r < kRounds; ++r) {
+            if (valid[r]) {
+                const int2 pr = prow[pth[r]];
+                loc[r] = bucket_local(k[r], pr, err);
+                sent[r] = loc[r] == (uint32_t)pr.y;
+                b[r] = (uint32_t)bucket_base[pth[r]] + loc[r];
+            }
         }
     }
 #pragma unroll
@@ -125,7 +138,9 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scatter(const int32_t* __rest
         const int32_t slot = __shfl(base[r], h) + (lane - h);
         if (valid[r]) {
             const int64_t i = g0 + r * 64 + lane;
-            if constexpr (NARROW) {
+            if constexpr (PACKED) {
+                out32[slot] = (uint32_t)k[r];
+            } else if constexpr (NARROW) {
                 // the off-canvas bucket holds one key: its x is left at 0
                 const uint32_t xl = sent[r] ? 0u : ((uint32_t)k[r] & 0xFFFFu) - kNarrowX0;
                 out32[slot] = (loc[r] << 17) | (xl << 2) | (uint32_t)(wv[r] + 1);
@@ -224,16 +239,21 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
                                  const int32_t* bucket_base, const int2* prow, const int32_t* flag_scan,
                                  const int32_t* band_start, int32_t* fill, int32_t* bsum, unsigned long long* out,
-                                 int32_t* err, hipStream_t s, const int32_t* n_dev, bool narrow) {
+                                 int32_t* err, hipStream_t s, const int32_t* n_dev, bool narrow, bool packed) {
     if (N == 0) return hipSuccess;
+    if (packed && !narrow) return hipErrorInvalidValue;
     const unsigned g = grid_for((N + kRounds - 1) / kRounds);
-    if (narrow)
-        k_bucket_scatter<true><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan, band_start,
-                                                    fill, bsum, nullptr, reinterpret_cast<uint32_t*>(out), err,
-                                                    n_dev);
+    if (packed)
+        k_bucket_scatter<true, true><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan,
+                                                          band_start, fill, bsum, nullptr,
+                                                          reinterpret_cast<uint32_t*>(out), err, n_dev);
+    else if (narrow)
+        k_bucket_scatter<true, false><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan,
+                                                           band_start, fill, bsum, nullptr,
+                                                           reinterpret_cast<uint32_t*>(out), err, n_dev);
     else
-        k_bucket_scatter<false><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan, band_start,
-                                                     fill, bsum, out, nullptr, err, n_dev);
+        k_bucket_scatter<false, false><<<g, kBlock, 0, s>>>(key, wn, epath, N, bucket_base, prow, flag_scan,
+                                                            band_start, fill, bsum, out, nullptr, err, n_dev);
     return hipGetLastError();
 }
 

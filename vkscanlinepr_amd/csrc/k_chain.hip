@@ -641,6 +641,26 @@ __device__ inline PieceMeta meta_of_lane(const PieceMeta& m, int k) {
     return r;
 }
 
+// An entry's fragment outputs; returns its global bucket.  Frames with narrow sort words store the
+// bucket (in key) and the narrow word (bucket inside the path, x + 2 or 0 in the off-canvas
+// bucket, winding change + 1), which is all their scatter needs; the others store the key, the
+// winding change and the path.
+__device__ inline uint32_t store_fragment(const FragArgs& A, int64_t i, uint32_t yx, int swn, uint32_t path,
+                                          uint32_t bb, int2 pr) {
+    const uint32_t loc = bucket_local((int32_t)yx, pr, A.err);
+    const uint32_t bucket = bb + loc;
+    if (A.nword) {
+        const uint32_t xl = loc == (uint32_t)pr.y ? 0u : (yx & 0xFFFFu) - kNarrowX0;
+        A.key[i] = (int32_t)bucket;
+        A.nword[i] = (loc << 17) | (xl << 2) | (uint32_t)(swn + 1);
+    } else {
+        A.key[i] = (int32_t)yx;
+        A.wn[i] = (int8_t)swn;
+        A.epath[i] = path;
+    }
+    return bucket;
+}
+
 // gen_fragment for the entries of one piece, fin[0..n) = their final t bits: entry o's segment
 // runs to entry o+1, the last one's to m.t_end.  Keys, winding changes and paths are written
 // coalesced; bucket counts are aggregated over runs of the wave.  Every lane of the n_waves
@@ -664,12 +684,9 @@ __device__ inline void piece_fragments(const PieceMeta& m, const uint32_t* fin, 
             int swn;
             segment_cell_pts(t < t1, px, py, qx, qy, m.w, m.h, yx, swn);
             const int64_t i = (int64_t)m.pd.base + o;
-            A.key[i] = (int32_t)yx;
-            A.wn[i] = (int8_t)swn;
-            A.epath[i] = m.path;
+            bucket = store_fragment(A, i, yx, swn, m.path, m.bb, m.pr);
             if (A.inter) A.inter[i] = make_int2(m.pd.curve, (int32_t)f);
             if (A.chord) A.chord[i] = make_float4(px, py, qx, qy);
-            bucket = m.bb + bucket_local((int32_t)yx, m.pr, A.err);
         }
         wave_count(A.hist, bucket, valid);
     }
@@ -827,12 +844,9 @@ __global__ __launch_bounds__(kBlock) void k_piece_small(const PieceDesc* __restr
                 int swn;
                 segment_cell_pts(t0 < t1, px, py, qx, qy, w, h, yx, swn);
                 const int64_t i = (int64_t)pd.base + o;
-                A.key[i] = (int32_t)yx;
-                A.wn[i] = (int8_t)swn;
-                A.epath[i] = path;
+                const uint32_t bucket = store_fragment(A, i, yx, swn, path, bb, pr);
                 if (A.inter) A.inter[i] = make_int2(pd.curve, (int32_t)fin0);
                 if (A.chord) A.chord[i] = make_float4(px, py, qx, qy);
-                const uint32_t bucket = bb + bucket_local((int32_t)yx, pr, A.err);
                 if (run_n > 0 && bucket != run_b) {
                     atomicAdd(&A.hist[run_b], run_n);
                     run_n = 0;

@@ -80,6 +80,8 @@ struct FragArgs {
     int32_t* err;
     int2* inter;                // (curve, t bits) entries for export, or null
     float4* chord;              // each entry's chord (pf.x, pf.y, pl.x, pl.y) for coverage, or null
+    uint32_t* nword;            // frames with narrow sort words: each entry's word, with key[i] = its
+                                // global bucket (wn and epath are then not written), or null
 };
 // Chain length classes: cubic chains by quarter octave of their length (16 octaves x 4), then
 // line chains by octave.  The chain control block holds, as int32: class counts, class fill
@@ -180,10 +182,13 @@ hipError_t launch_bucket_counts(const int32_t* row_lo, const int32_t* row_hi, ui
 constexpr uint32_t kNarrowX0 = 0x7FFDu;   // x field of the leftmost key (x = -2)
 constexpr int kNarrowMaxW = (1 << 15) - 4;
 // bsum (zeroed, one int32 per band) receives each band's sum of winding changes.
+// packed (narrow only): the entries are (key = global bucket, epath = narrow word), as the piece
+// kernels write them with FragArgs::nword; otherwise (key, wn, epath).
 hipError_t launch_bucket_scatter(const int32_t* key, const int8_t* wn, const uint32_t* epath, int64_t N,
                                  const int32_t* bucket_base, const int2* prow, const int32_t* flag_scan,
                                  const int32_t* band_start, int32_t* fill, int32_t* bsum, unsigned long long* out,
-                                 int32_t* err, hipStream_t s, const int32_t* n_dev = nullptr, bool narrow = false);
+                                 int32_t* err, hipStream_t s, const int32_t* n_dev = nullptr, bool narrow = false,
+                                 bool packed = false);
 // window_log: bands start in 2^window_log-entry windows (9 = kBandWindowLog or 10; tuned).
 hipError_t launch_band_flags(const int32_t* bucket_base, uint32_t Np, const int32_t* boff, int32_t* flag,
                              hipStream_t s, int window_log = kBandWindowLog);

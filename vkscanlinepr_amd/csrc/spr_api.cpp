@@ -222,6 +222,7 @@ struct spr_handle {
     bool last_fallback = false;
     int2* last_brec = nullptr;                      // per-bucket record ranges of the last frame
     bool last_fused = false;                        // last frame fused mark + emit into the sort
+    bool last_packed = false;                       // last frame stored buckets and sort words, not keys
 
     // batch of documents (spr_load_batch): per-path canvas, per-document tables
     bool batch = false;
@@ -497,9 +498,11 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     stage_mark(H, 4);
     if (gen) {
         // chain solve, then piece merge and fragment keys in one pass
+        // narrow frames: the piece kernels store each entry's bucket (key) and sort word (epath)
         const spr::FragArgs A{H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->tpos.p, w, h, nullptr,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, d_hist, d_err,
-                              keep ? H->inter.p : nullptr, aa ? H->chord.p : nullptr};
+                              keep ? H->inter.p : nullptr, aa ? H->chord.p : nullptr,
+                              narrow ? H->epath.p : nullptr};
         solve_and_pieces(H, C, d_cc, A, (int64_t)n1, H->n_cu * H->chain_k / 8, H->n_cu * H->merge_k / 8);
     } else {
         stage_mark(H, 5);
@@ -520,7 +523,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
                                       narrow ? H->band_path.p : nullptr));
         SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, N, H->bucket_base.p, H->prow.p,
                                            H->bflag.p, H->band_start.p, d_fill, d_bsum, H->sbig.p, d_err, s, nullptr,
-                                           narrow));
+                                           narrow, narrow && gen));
         // each band's exclusive prefix of the winding changes (total at [n_bands])
         SPR_HIP(spr::launch_scan_i32(d_bsum, d_bsum, max_bands, ctl_at(H, z.st_wind, z.ctr, 2), s, H->bflag.p + NB));
     }
@@ -560,6 +563,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     H->last_brec = d_brec;
     H->last_learned = false;
     H->last_fused = narrow;
+    H->last_packed = narrow && gen;
     stage_mark(H, 11);
     // ---- composite
     spr::PaintArgs pa;
@@ -839,7 +843,7 @@ void run_learned_frame(spr_handle* H) {
     if (C) {
         const spr::FragArgs A{H->cpos.p, H->ctype.p, H->arcw(), H->cpath.p, H->tpos.p, w, h, pcv,
                               H->key.p, H->wn.p, H->epath.p, H->bucket_base.p, H->prow.p, H->hist_l.p, d_err,
-                              debug ? H->inter.p : nullptr};
+                              debug ? H->inter.p : nullptr, nullptr, narrow ? H->epath.p : nullptr};
         solve_and_pieces(H, C, d_cc, A, nl, H->n_cu * H->chain_k / 8, H->n_cu * H->merge_k / 8);
     } else {
         stage_mark(H, 5);
@@ -856,7 +860,7 @@ void run_learned_frame(spr_handle* H) {
                                   narrow ? H->band_path.p : nullptr));
     SPR_HIP(spr::launch_bucket_scatter(H->key.p, H->wn.p, H->epath.p, nl, H->bucket_base.p, H->prow.p,
                                        H->bflag.p, H->band_start.p, H->fill_l.p, H->bsum_l.p, H->sbig.p, d_err, s,
-                                       d_live, narrow));
+                                       d_live, narrow, narrow));
     SPR_HIP(spr::launch_scan_i32(H->bsum_l.p, H->bsum_l.p, max_bands, ctl_at(H, z.st_wind, z.counters, kLcBandSums),
                                  s, d_nbands));
     // the sorted start of each path: read by the documents' winding bases and the separate mark + emit
@@ -937,6 +941,7 @@ void learned_finish(spr_handle* H) {
     H->last_fallback = false;
     H->last_brec = H->brec_l.p;
     H->last_fused = !H->opts.debug && H->w <= spr::kNarrowMaxW;  // as the frame's narrow words
+    H->last_packed = H->last_fused;
     if (err & 1) {
         H->h_pinned[16 + kSumErr] = 0;
         throw Fail{SPR_ERR_DEVICE, "internal: an entry fell outside its path's row bound"};
@@ -1474,8 +1479,12 @@ int spr_debug_export(spr_handle* H, int stage, void* dst, size_t cap, size_t* nb
                 if (!debug && !H->inj) throw Fail{SPR_ERR_STATE, "intersections are kept only with the debug option"};
                 from_dev(H->inter.p, (size_t)NP * 8);
                 break;
-            case SPR_STAGE_KEY: from_dev(H->key.p, (size_t)NP * 4); break;
+            case SPR_STAGE_KEY:
+                if (H->last_packed) throw Fail{SPR_ERR_STATE, "fragment keys are kept only with the debug option"};
+                from_dev(H->key.p, (size_t)NP * 4);
+                break;
             case SPR_STAGE_WN: {
+                if (H->last_packed) throw Fail{SPR_ERR_STATE, "winding changes are kept only with the debug option"};
                 std::vector<int8_t> v((size_t)NP);
                 if (NP) SPR_HIP(hipMemcpy(v.data(), H->wn.p, (size_t)NP, hipMemcpyDeviceToHost));
                 tmp.resize((size_t)NP * 4);
