@@ -39,7 +39,9 @@ namespace spr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kLongClass = 10;     // cubic chains of 2^10 = 1024+ crossings get a whole wave
+// cubic chains of class 4 * 9 + 3 and up (896+ crossings) get a whole wave: a lane-solved chain of
+// 900-1023 crossings outlasted the longest wave-solved chains
+constexpr int kWaveClass = 4 * 9 + 3;
 constexpr int kStage = 4096;       // piece values staged in LDS by the merging wave
 constexpr int kClasses = kChainClasses;  // 0..63: cubic chains (4 per octave), 64..79: line chains
 constexpr int kRounds = 4, kLevels = 6;  // wave solve: 4 rounds of 6 bisection levels
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
 
 // Class order: by estimated cost in quarter octaves, costliest first (a line crossing costs
 // about 2^-6 of a cubic one), cubic before line on ties.  Cost key: cubic class c -> 2(c+24)+1,
-// line class 64+l -> 8l; the keys are distinct.  The cubic classes of kLongClass octaves and up
+// line class 64+l -> 8l; the keys are distinct.  The cubic classes from kWaveClass up
 // come first: the wave-solved chains.
 __device__ inline int class_cost_key(int c) { return c < 64 ? 2 * (c + 24) + 1 : 8 * (c - 64); }
 
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restric
         int32_t all = 0, nl = 0, lb = 0;
         for (int d = 0; d < kClasses; ++d) {
             all += total[d];
-            if (d < 64 && d >= 4 * kLongClass) nl += total[d];
+            if (d < 64 && d >= kWaveClass) nl += total[d];
             if (d < 64 && d >= kHeavyClass) lb += total[d];
         }
         class_off[kClasses] = all;
