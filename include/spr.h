@@ -96,8 +96,9 @@ enum {
     SPR_STAGE_CUTS = 2,        /* float x 5C: cut cache (4 cuts + n_cuts bits) */
     SPR_STAGE_COUNTS = 3,      /* int32 x (C+1): exclusive-scanned per-curve counts */
     SPR_STAGE_INTER = 4,       /* int2 x N: (curve, t bits) intersections (debug) */
-    SPR_STAGE_KEY = 5,         /* int32 x N: cell keys in intersection order */
-    SPR_STAGE_WN = 6,          /* int32 x N: winding change in intersection order */
+    SPR_STAGE_KEY = 5,         /* int32 x N: cell keys in intersection order (debug, anti-aliased
+                                  or injected frames, or canvases wider than 32,764) */
+    SPR_STAGE_WN = 6,          /* int32 x N: winding change in intersection order (as KEY) */
     SPR_STAGE_SEG = 7,         /* int32 x (Np+1): per-path segment starts */
     SPR_STAGE_SKEY = 8,        /* int32 x N: keys sorted inside each path segment (debug; without
                                   it, canvases up to 32764 wide fuse mark + emit into the sort and
