@@ -42,6 +42,12 @@ constexpr int kBlock = 256;
 // cubic chains of class 4 * 9 + 3 and up (896+ crossings) get a whole wave: a lane-solved chain of
 // 900-1023 crossings outlasted the longest wave-solved chains
 constexpr int kWaveClass = 4 * 9 + 3;
+// below that, each launch also wave-solves its costliest cubic classes while they hold at most
+// kWaveBudget chains in all and at least 2^5 crossings each: in a scene with few long chains
+// (a 64-path scene, a glyph page) those chains alone set the launch's span, and a wave takes
+// 0.64 us per crossing against about 1.2 us for a lane
+constexpr int kWaveBudget = 512;
+constexpr int kWaveMinClass = 4 * 5;
 constexpr int kStage = 4096;       // piece values staged in LDS by the merging wave
 constexpr int kClasses = kChainClasses;  // 0..63: cubic chains (4 per octave), 64..79: line chains
 constexpr int kRounds = 4, kLevels = 6;  // wave solve: 4 rounds of 6 bisection levels
@@ -179,6 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restric
                                                         int32_t* __restrict__ class_fill,
                                                         int32_t* __restrict__ class_off, int32_t* __restrict__ n_long,
                                                         int32_t* __restrict__ light_begin,
+                                                        int32_t* __restrict__ n_long_light,
                                                         int32_t* __restrict__ order) {
     __shared__ int32_t cnt[kClasses], base[kClasses], off[kClasses], total[kClasses];
     if (threadIdx.x < kClasses) {
@@ -199,9 +206,20 @@ __global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restric
             if (d < 64 && d >= kWaveClass) nl += total[d];
             if (d < 64 && d >= kHeavyClass) lb += total[d];
         }
+        // the heavy launch's range holds cubic classes only (every line key is below theirs)
+        for (int c = kWaveClass - 1; c >= kHeavyClass && nl + total[c] <= kWaveBudget; --c) nl += total[c];
+        // the light launch's range in key order: cubic class c comes right after line class
+        // (c + 25) / 4 when (c + 25) % 4 == 0 (key 8l = 2c + 50); wave items stop at a line class
+        int32_t nll = 0;
+        for (int c = kHeavyClass - 1; c >= kWaveMinClass; --c) {
+            if ((c + 25) % 4 == 0 && total[64 + (c + 25) / 4] > 0) break;
+            if (nll + total[c] > kWaveBudget) break;
+            nll += total[c];
+        }
         class_off[kClasses] = all;
         *n_long = nl;
         *light_begin = lb;
+        *n_long_light = nll;
     }
     const int64_t per = ((n_slots + gridDim.x - 1) / gridDim.x + kBlock - 1) / kBlock * kBlock;
     const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < n_slots ? b0 + per : n_slots;
@@ -437,41 +455,51 @@ __global__ __launch_bounds__(kBlock) void k_chain_solve(const float2* __restrict
     const int lane = threadIdx.x & 63;
     const int32_t total = class_off[kClasses];
     const int32_t lb = *light_begin;
-    // heavy launch: wave items on chains [0, nl), lane items on [nl, lb); light: lane items on
-    // [lb, total)
-    const int32_t nl = heavy ? *n_long : 0;
-    const int32_t lane0 = heavy ? nl : lb, lane1 = heavy ? lb : total;
+    // the launch's chains [r0, r1) in class order (heavy: [0, lb), light: [lb, total)): wave items
+    // on the first nl, lane items on the rest
+    const int32_t r0 = heavy ? 0 : lb, r1 = heavy ? lb : total;
+    const int32_t nl = *n_long;
+    const int32_t lane0 = r0 + nl, lane1 = r1;
     const int32_t items = nl + (lane1 - lane0 + 63) / 64;
     const int32_t n_waves = (int32_t)gridDim.x * (kBlock / 64);
     int32_t item = (int32_t)blockIdx.x * (kBlock / 64) + (int32_t)(threadIdx.x >> 6);
+    // the wave's next item is taken one item ahead, so that the atomic's round trip overlaps the
+    // loads of the current item instead of following its solve
+    int32_t next = 0;
+    if (lane == 0 && item < items) next = n_waves + atomicAdd(work, 1);
     while (item < items) {
         if (item < nl) {
             __builtin_amdgcn_s_setprio(3);
-            const int32_t ci = order[item];
+            const int32_t ci = order[r0 + item];
             const ChainDesc d = chains[ci];
             const CurveAxis a = load_axis(tpos, cpos, cweight, d);
             solve_chain_wave(d, a, chain_t + d.out, mono + ci);
             __builtin_amdgcn_s_setprio(0);
         } else {
-            const int32_t k0 = lane0 + (item - nl) * 64;
-            const int32_t k = k0 + lane;
-            // issue priority follows the cost of the wave's chains (the longest run first and
-            // must not be starved by the short ones)
-            const int oct = class_cost_octave(chain_class(chains[order[k0]].n, chains[order[k0]].type != (int16_t)kLine));
+            const int32_t k = lane0 + (item - nl) * 64 + lane;
+            ChainDesc d;
+            d.n = 0; d.type = (int16_t)kLine;
+            int32_t ci = 0;
+            if (k < lane1) {
+                ci = order[k];
+                d = chains[ci];
+            }
+            // issue priority follows the cost of the wave's chains, read from lane 0's (the item's
+            // first chain: the longest run first and must not be starved by the short ones)
+            const int n0 = __builtin_amdgcn_readfirstlane(d.n);
+            const int b0 = __builtin_amdgcn_readfirstlane(d.type != (int16_t)kLine ? 1 : 0);
+            const int oct = class_cost_octave(chain_class(n0, b0 != 0));
             if (oct >= 8) __builtin_amdgcn_s_setprio(3);
             else if (oct >= 6) __builtin_amdgcn_s_setprio(2);
             else if (oct >= 4) __builtin_amdgcn_s_setprio(1);
             if (k < lane1) {
-                const int32_t ci = order[k];
-                const ChainDesc d = chains[ci];
                 const CurveAxis a = load_axis(tpos, cpos, cweight, d);
                 solve_chain_lane(d, a, chain_t + d.out, mono + ci, stage);
             }
             __builtin_amdgcn_s_setprio(0);
         }
-        int32_t next = 0;
-        if (lane == 0) next = n_waves + atomicAdd(work, 1);
         item = __shfl(next, 0);
+        if (lane == 0 && item < items) next = n_waves + atomicAdd(work, 1);
     }
 }
 
@@ -1013,12 +1041,12 @@ hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uin
 
 hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_t* class_count,
                               int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* light_begin,
-                              int32_t* order, int n_cu, hipStream_t s) {
+                              int32_t* n_long_light, int32_t* order, int n_cu, hipStream_t s) {
     if (C == 0) return hipSuccess;
     unsigned g = grid_for((int64_t)C * 10);
     if (g > (unsigned)n_cu * 4) g = (unsigned)n_cu * 4;
     k_chain_order<<<g, kBlock, 0, s>>>(chain_cls, (int64_t)C * 10, class_count, class_fill, class_off, n_long,
-                                       light_begin, order);
+                                       light_begin, n_long_light, order);
     return hipGetLastError();
 }
 

@@ -87,11 +87,13 @@ struct FragArgs {
 // line chains by octave.  The chain control block holds, as int32: class counts, class fill
 // counters, class offsets [kChainClasses + 1], the number of wave-solved chains, the work counter
 // of the persistent solve, the count of queued heavy pieces, the first light class's place, the
-// light solve's work counter and the count of queued wave pieces that wait on heavy chains.
+// light solve's work counter, the count of queued wave pieces that wait on heavy chains and the
+// number of wave-solved chains at the head of the light launch.
 constexpr int kChainClasses = 80;
 constexpr int kCcFill = kChainClasses, kCcOff = 2 * kChainClasses, kCcNLong = 3 * kChainClasses + 1,
               kCcWork = kCcNLong + 1, kCcHeavyPieces = kCcWork + 1, kCcLightBegin = kCcHeavyPieces + 1,
-              kCcWorkLight = kCcLightBegin + 1, kCcWavePieces = kCcWorkLight + 1, kCcWords = kCcWavePieces + 1;
+              kCcWorkLight = kCcLightBegin + 1, kCcWavePieces = kCcWorkLight + 1, kCcNLongLight = kCcWavePieces + 1,
+              kCcWords = kCcNLongLight + 1;
 // Chains of at least kHeavyCrossings crossings are "heavy": they are solved by their own launch
 // (wave-solved and the longest lane-solved chains), and the pieces of at most kHeavyCrossings
 // entries, which never wait on them, are finished by a concurrent stream meanwhile.
@@ -110,11 +112,13 @@ hipError_t launch_chain_prep(const float2* tpos, const uint32_t* cpos, const uin
                              const uint32_t* ncuts, const int32_t* offsets, uint32_t C, int w, int h,
                              ChainDesc* chains, PieceDesc* pieces, uint8_t* chain_cls, int32_t* class_count,
                              int n_cu, hipStream_t s, const Budget* budget = nullptr, const int2* pcanvas = nullptr);
+// n_long / n_long_light: the wave-solved chains at the head of the heavy / light launch's range.
 hipError_t launch_chain_order(const uint8_t* chain_cls, uint32_t C, const int32_t* class_count,
                               int32_t* class_off, int32_t* class_fill, int32_t* n_long, int32_t* light_begin,
-                              int32_t* order, int n_cu, hipStream_t s);
-// heavy = 1: the wave-solved chains and the lane-solved chains of kHeavyCrossings and more;
-// heavy = 0: the rest.  The two launches are independent.
+                              int32_t* n_long_light, int32_t* order, int n_cu, hipStream_t s);
+// heavy = 1: the chains of kHeavyCrossings and more; heavy = 0: the rest.  Each launch
+// wave-solves the first *n_long chains of its range and lane-solves the others.  The two
+// launches are independent.
 hipError_t launch_chain_solve(const float2* tpos, const uint32_t* cpos, const float* cweight,
                               const ChainDesc* chains,
                               const int32_t* order, const int32_t* class_off, const int32_t* n_long,

@@ -347,7 +347,8 @@ void solve_and_pieces(spr_handle* H, uint32_t C, int32_t* d_cc, const spr::FragA
     SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->arcw(), H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 1,
                                     d_cc + spr::kCcWork, H->chain_t.p, H->mono.p, n_cu_solve, s));
     SPR_HIP(hipStreamWaitEvent(s2, H->ev_fork, 0));
-    SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->arcw(), H->chains.p, H->chain_order.p, d_coff, d_nlong, d_lb, 0,
+    SPR_HIP(spr::launch_chain_solve(H->tpos.p, H->cpos.p, H->arcw(), H->chains.p, H->chain_order.p, d_coff,
+                                    d_cc + spr::kCcNLongLight, d_lb, 0,
                                     d_cc + spr::kCcWorkLight, H->chain_t.p, H->mono.p, n_cu_solve, s2));
     if (s2 == s) stage_mark(H, 5);  // serial: both solves, then every piece kernel
     SPR_HIP(hipEventRecord(H->ev_light, s2));
@@ -493,6 +494,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
                                        H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
                                        d_cc, H->n_cu, s));
         SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, d_cc + spr::kCcLightBegin,
+                                        d_cc + spr::kCcNLongLight,
                                         H->chain_order.p, H->n_cu, s));
     }
     stage_mark(H, 4);
@@ -837,6 +839,7 @@ void run_learned_frame(spr_handle* H) {
                                        H->ncuts.p, H->counts.p, C, w, h, H->chains.p, H->pieces.p, H->chain_cls.p,
                                        d_cc, H->n_cu, s, &B, pcv));
         SPR_HIP(spr::launch_chain_order(H->chain_cls.p, C, d_cc, d_coff, d_cfill, d_nlong, d_cc + spr::kCcLightBegin,
+                                        d_cc + spr::kCcNLongLight,
                                         H->chain_order.p, H->n_cu, s));
     }
     stage_mark(H, 4);
