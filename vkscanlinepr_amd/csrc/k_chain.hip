@@ -54,6 +54,7 @@ constexpr int kRounds = 4, kLevels = 6;  // wave solve: 4 rounds of 6 bisection 
 // first heavy cubic class: a quarter-octave class 4*l + sub holds lengths >= 2^l (l = 8: 256)
 constexpr int kHeavyClass = 4 * 8;
 static_assert(kHeavyCrossings == 1 << 8, "heavy class boundary");
+static_assert(kFrag == 2, "chain descriptors keep only the sign of the grid step");
 static_assert(kRounds * kLevels == kCubicIters, "wave solve must run the reference's iterations");
 
 inline unsigned grid_for(int64_t n) {
@@ -143,11 +144,11 @@ __global__ __launch_bounds__(kBlock) void k_chain_prep(
                         if (n == 0) continue;
                         ChainDesc d;
                         d.n = n;
-                        d.curve = (int32_t)c;
-                        d.side = side;
+                        d.pos = (int32_t)cpos[c];
+                        d.flags = (int16_t)(side | ((side ? G.dy : G.dx) < 0.0f ? 2 : 0));
                         d.out = base + 1 + (side ? G.nx : 0);
                         d.line0 = side ? G.y0 : G.x0;
-                        d.step = side ? G.dy : G.dx;
+                        d.w = type == kArc ? cweight[c] : 0.0f;
                         d.t0 = t0;
                         d.t1 = t1;
                         d.type = (int16_t)type;
@@ -242,18 +243,18 @@ __global__ __launch_bounds__(kBlock) void k_chain_order(const uint8_t* __restric
 // The chain's axis of its curve, in evaluation form (an arc's c1 = w * x1 and c3 = w).
 __device__ inline CurveAxis load_axis(const float2* __restrict__ tpos, const uint32_t* __restrict__ cpos,
                                       const float* __restrict__ cweight, const ChainDesc& d) {
-    const float2* p = tpos + cpos[d.curve];
+    const float2* p = tpos + d.pos;
     const uint32_t type = (uint32_t)d.type;
+    const bool ys = chain_side(d) != 0;
     CurveAxis a;
     const int npts = (int)(type & 7u);
-    a.c[0] = d.side ? p[0].y : p[0].x;
-    a.c[1] = d.side ? p[1].y : p[1].x;
-    a.c[2] = npts >= 3 ? (d.side ? p[2].y : p[2].x) : 0.0f;
-    a.c[3] = npts == 4 ? (d.side ? p[3].y : p[3].x) : 0.0f;
+    a.c[0] = ys ? p[0].y : p[0].x;
+    a.c[1] = ys ? p[1].y : p[1].x;
+    a.c[2] = npts >= 3 ? (ys ? p[2].y : p[2].x) : 0.0f;
+    a.c[3] = npts == 4 ? (ys ? p[3].y : p[3].x) : 0.0f;
     if (type == kArc) {
-        const float w = cweight[d.curve];
-        a.c[1] = w * a.c[1];
-        a.c[3] = w;
+        a.c[1] = d.w * a.c[1];
+        a.c[3] = d.w;
     }
     return a;
 }
@@ -269,8 +270,8 @@ __device__ inline void solve_chain_lane_t(const ChainDesc& d, const CurveAxis& a
                                           uint8_t* __restrict__ mono, float* __restrict__ stage) {
     float tprev = d.t0, line = d.line0;
     uint32_t rises = 0, falls = 0;  // the chain's values go up / down somewhere
-    const uint32_t side = (uint32_t)d.side;
-    ChainSolver<T> solver(a, d.step, d.t1);
+    const uint32_t side = (uint32_t)chain_side(d);
+    ChainSolver<T> solver(a, chain_step(d), d.t1);
     for (int k = 0; k < d.n; ++k) {
         // the usual case in every lane: the straight-line form, no divergent branches
         const float ts = __all(solver.common(tprev)) ? solver.next_common(line, tprev, d.t1)
@@ -290,7 +291,7 @@ __device__ inline void solve_chain_lane_t(const ChainDesc& d, const CurveAxis& a
         falls |= !(v >= prev) ? 1u : 0u;  // as the flags of the reference order: NaN breaks both
         rises |= !(v <= prev) ? 1u : 0u;
         tprev = v;
-        line += d.step;
+        line += chain_step(d);
     }
     *mono = (falls ? 0 : 1) | (rises ? 0 : 2);
 }
@@ -380,8 +381,8 @@ __device__ inline void solve_chain_wave_t(const ChainDesc& d, const CurveAxis& a
                                           uint8_t* __restrict__ mono) {
     const int lane = threadIdx.x & 63;
     const WaveTree W = wave_tree(lane);
-    const uint32_t side = (uint32_t)d.side;
-    const bool neg = d.step < 0.0f;
+    const uint32_t side = (uint32_t)chain_side(d);
+    const bool neg = chain_step(d) < 0.0f;
     float tprev = d.t0, line = d.line0;
     bool up = true, down = true;
     float kept = 0.0f;
@@ -418,7 +419,7 @@ __device__ inline void solve_chain_wave_t(const ChainDesc& d, const CurveAxis& a
         }
         if (k > 0 || side) { up = up && v >= tprev; down = down && v <= tprev; }
         tprev = v;
-        line += d.step;
+        line += chain_step(d);
     }
     if (lane == 0) *mono = (up ? 1 : 0) | (down ? 2 : 0);
 }

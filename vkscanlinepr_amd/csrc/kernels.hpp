@@ -52,11 +52,14 @@ hipError_t launch_path_segments(const uint32_t* cpath, uint32_t C, const int32_t
 hipError_t launch_segment_stats(const int32_t* seg, uint32_t Np, int32_t* max_len, hipStream_t s);
 // One (curve, piece, axis) chain of dependent crossing solves, and one monotone piece.
 struct ChainDesc {
-    int32_t curve, n, out;  // curve, number of crossings, first output slot
-    int16_t side, type;     // type: the curve's (kLine, kQuad, kCubic, kArc)
-    float line0, step;      // first grid line and its step (+-2)
+    int32_t pos, n, out;    // the curve's first transformed point, number of crossings, first output slot
+    int16_t flags, type;    // flags: bit 0 the axis (y), bit 1 the grid step is -2 (else +2);
+                            // type: the curve's (kLine, kQuad, kCubic, kArc)
+    float line0, w;         // first grid line; an arc's weight (0 for other curves)
     float t0, t1;           // flagged piece start and end parameters
 };
+__host__ __device__ inline int chain_side(const ChainDesc& d) { return d.flags & 1; }
+__host__ __device__ inline float chain_step(const ChainDesc& d) { return (d.flags & 2) ? -2.0f : 2.0f; }
 struct PieceDesc {
     int32_t base, nx, ny, curve;  // base = -1: no piece in this slot
     float t0;                     // flagged start parameter
