@@ -78,7 +78,8 @@ typedef struct spr_stats {
     int64_t learned_fallbacks; /* LEARNED: frames whose exact total exceeded the allocation and were
                                   rendered with the plain method (the allocation then grows) */
     /* Which kernel routes the last frame took (each route has its own kernel or mode): */
-    int64_t n_wave_chains;     /* crossing chains of 1024 or more, each solved by a whole wave */
+    int64_t n_wave_chains;     /* crossing chains solved by a whole wave each (896 or more crossings,
+                                  and each launch's costliest classes while they hold at most 512) */
     int64_t n_heavy_pieces;    /* monotone pieces merged by a 256-thread workgroup each (more than
                                   1024 entries, or waiting on a chain of 512 or more crossings) */
     int64_t n_long_bands;      /* sort bands of more than 512 words (one wave at 16 or 32 words per

@@ -114,7 +114,7 @@ void mvp_inverse(const float m[16], float out[9]) {
 // allocation counters.
 enum {
     kSumCells = 0, kSumSpans, kSumErr, kSumWaveChains, kSumHeavyPieces, kSumLongBands, kSumBigBands,
-    kSumTotal, kSumOver, kSumOverflow, kSumLive, kSumArena, kSumSorted, kSumCapR, kSumCount
+    kSumTotal, kSumOver, kSumOverflow, kSumLive, kSumArena, kSumSorted, kSumCapR, kSumWaveChainsLight, kSumCount
 };
 static_assert(kSumCount <= spr::kSummaryMax, "summary slots");
 
@@ -579,6 +579,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     S.p[kSumSpans] = d_totals + 1;
     S.p[kSumErr] = d_err;
     S.p[kSumWaveChains] = gen ? d_nlong : nullptr;
+    S.p[kSumWaveChainsLight] = gen ? d_cc + spr::kCcNLongLight : nullptr;
     S.p[kSumHeavyPieces] = gen ? d_cc + spr::kCcHeavyPieces : nullptr;
     S.p[kSumLongBands] = reinterpret_cast<const int32_t*>(ctl_at(H, z.st_wind, z.ctr, 4).counter + 1);
     S.p[kSumBigBands] = d_bigcount;
@@ -915,6 +916,7 @@ void run_learned_frame(spr_handle* H) {
     S.p[kSumSpans] = d_totals + 1;
     S.p[kSumErr] = d_err;
     S.p[kSumWaveChains] = C ? d_nlong : nullptr;
+    S.p[kSumWaveChainsLight] = C ? d_cc + spr::kCcNLongLight : nullptr;
     S.p[kSumHeavyPieces] = C ? d_cc + spr::kCcHeavyPieces : nullptr;
     S.p[kSumLongBands] = reinterpret_cast<const int32_t*>(ticket + 1);
     S.p[kSumBigBands] = d_bigcount;
@@ -1075,7 +1077,7 @@ void sync_stats(spr_handle* H) {
     }
     H->stats.n_cells = H->sum(kSumCells);
     H->stats.n_spans = H->sum(kSumSpans);
-    H->stats.n_wave_chains = H->sum(kSumWaveChains);
+    H->stats.n_wave_chains = H->sum(kSumWaveChains) + H->sum(kSumWaveChainsLight);
     H->stats.n_heavy_pieces = H->sum(kSumHeavyPieces);
     H->stats.n_long_bands = H->sum(kSumLongBands);
     H->stats.n_big_bands = H->sum(kSumBigBands);
