@@ -81,13 +81,15 @@ typedef struct spr_stats {
     int64_t n_wave_chains;     /* crossing chains solved by a whole wave each (896 or more crossings,
                                   and each launch's costliest classes while they hold at most 512) */
     int64_t n_heavy_pieces;    /* monotone pieces merged by a 256-thread workgroup each (more than
-                                  1024 entries, or waiting on a chain of 512 or more crossings) */
+                                  1024 entries, or whose last segment waits on a chain of 256 or
+                                  more crossings of the next piece) */
     int64_t n_long_bands;      /* sort bands of more than 512 words (one wave at 16 or 32 words per
                                   lane; 256-thread LDS sorts in frames that export a stage) */
     int64_t n_big_bands;       /* sort bands of more than 2048 words (global merge-path sort) */
     int64_t graph_captures;    /* LEARNED and batches: frames captured as a HIP graph so far (a new
                                   scene, model, margin, tuning or budget needs a new capture; a
-                                  new MVP does not: the captured frame reads it at each replay) */
+                                  still frame and a frame after a new MVP have a graph each, and
+                                  the latter reads the MVP at each replay) */
 } spr_stats;
 
 /* Debug stage identifiers for spr_debug_export / spr_debug_inject. */
