@@ -134,6 +134,31 @@ def test_injected_intersections_integer_stages(spr, orc):
     r.inject_intersections(None)
 
 
+def test_injected_intersections_production_frame(spr, orc):
+    """Injected intersections in a frame without debug (narrow sort words, mark + emit fused into
+    the band sort): the fragment stage then stores keys rather than the packed bucket and sort
+    word of generated entries, and the image, counts and keys are the oracle's."""
+    scene, _ = _scene(spr, "scribbles_A")
+    a = scene.arrays()
+    f = orc.Frame(a, a["width"], a["height"], threads=8)
+    r = spr.Rasterizer(a["width"], a["height"])
+    r.load_scene(scene)
+    r.inject_intersections(f.get("inter").copy())
+    img = r.render()
+    assert np.array_equal(img, f.image())
+    assert np.array_equal(r.export("key"), f.get("key"))
+    assert np.array_equal(r.export("wn"), f.get("wn"))
+    st = r.stats()
+    rec = f.get("records").reshape(-1, 4)
+    assert st.n_cells == int((rec[:, 3] != 0).sum())
+    assert st.n_spans == int((rec[:, 3] == 0).sum())
+    r.inject_intersections(None)
+    assert np.array_equal(r.render(), img)  # generated again: packed entries, same image
+    with pytest.raises(RuntimeError):
+        r.export("key")
+    r.close()
+
+
 def test_repeat_frames_identical(spr):
     scene, _ = _scene(spr, "illustration_overwrite")
     a = scene.arrays()
