@@ -92,8 +92,10 @@ def test_illustration_4096_production_frame(spr, illustration):
     routes = _routes(st)
     print(f"[scale] illustration_4096 production: N={st.n_intersections} routes={routes}", flush=True)
     assert st.n_intersections > 40_000_000
-    # the frame exercises the wave-solved chains, the workgroup-merged pieces and the long bands
+    # the frame exercises the wave-solved chains, the workgroup-merged pieces, the long bands and
+    # the oversized ones (sorted and emitted on the second stream beside the wave sorts)
     assert routes["wave_chains"] > 0 and routes["heavy_pieces"] > 0 and routes["long_bands"] > 0
+    assert routes["big_bands"] > 0
     # src-over accumulates in fp32 in path order on both sides: identical in practice
     assert np.array_equal(img, f.image())
 

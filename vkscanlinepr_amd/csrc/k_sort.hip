@@ -584,12 +584,24 @@ __device__ __forceinline__ void wave_band(const uint32_t* __restrict__ src, int 
 __global__ __launch_bounds__(kClassifyThreads) void k_band_classify2(const int32_t* __restrict__ band_start,
                                                                      const int32_t* __restrict__ n_bands,
                                                                      int32_t* __restrict__ list, int64_t cap,
-                                                                     uint32_t* __restrict__ count) {
+                                                                     uint32_t* __restrict__ count,
+                                                                     const int32_t* __restrict__ bpre,
+                                                                     int32_t* __restrict__ big_list,
+                                                                     int32_t* __restrict__ big_excl,
+                                                                     int32_t* __restrict__ big_count) {
     __shared__ uint32_t s_n[2][kClassifyThreads / 64], s_base[2];
     const int64_t t = (int64_t)blockIdx.x * kClassifyThreads + threadIdx.x;
     const int32_t nb = *n_bands;
     const int32_t n = t < nb ? band_start[t + 1] - band_start[t] : 0;
-    const bool med = n > kSmallBandMax && n <= kMediumBandMax, large = n > kMediumBandMax;
+    const bool med = n > kSmallBandMax && n <= kMediumBandMax;
+    const bool large = n > kMediumBandMax && n <= kWaveBandMax;
+    if (n > kWaveBandMax) {
+        // a single oversized (path, row) bucket: queued with its exclusive prefix for the
+        // global-merge kernels, which may run beside the wave sorts
+        const int32_t q = atomicAdd(big_count, 1);
+        big_list[q] = (int32_t)t;
+        big_excl[q] = bpre[t];
+    }
     const unsigned long long mm = __ballot(med), ml = __ballot(large);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) { s_n[0][wave] = (uint32_t)__popcll(mm); s_n[1][wave] = (uint32_t)__popcll(ml); }
@@ -611,9 +623,9 @@ __global__ __launch_bounds__(kClassifyThreads) void k_band_classify2(const int32
 }
 
 // CLS 0: every band of up to 512 words, one wave each (four per workgroup; the others are
-// skipped).  CLS 1: the listed bands of 513..1024 words (16 per lane).  CLS 2: the listed longer
-// bands (32 per lane); those above 2048 words (single oversized buckets) are queued for the
-// global-merge kernels instead.
+// skipped).  CLS 1: the listed bands of 513..1024 words (16 per lane).  CLS 2: the listed bands of
+// 1025..2048 words (32 per lane); k_band_classify2 queues the longer ones (single oversized
+// buckets) for the global-merge kernels.
 template <int CLS>
 __global__ __launch_bounds__(256) void k_sort_bands_wave(const int32_t* __restrict__ band_start,
                                                          const int32_t* __restrict__ n_bands,
@@ -633,15 +645,6 @@ __global__ __launch_bounds__(256) void k_sort_bands_wave(const int32_t* __restri
     const int64_t t = CLS == 0 ? w : (CLS == 1 ? list[w] : list[cap - 1 - w]);
     const int64_t s = band_start[t], n = band_start[t + 1] - s;
     if (CLS == 0 && n > kSmallBandMax) return;
-    if (CLS == 2 && n > kWaveBandMax) {
-        // a single oversized (path, row) bucket: sorted and emitted by the global-merge kernels
-        if (lane == 0) {
-            const int32_t q = atomicAdd(big_count, 1);
-            big_list[q] = (int32_t)t;
-            big_excl[q] = bpre[t];
-        }
-        return;
-    }
     const uint32_t* p = src + s;
     const int2 pr = band_pr[t];
     const int32_t pre = bpre[t];
@@ -898,13 +901,11 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
     if (band_pr) {
         if (sidx) return hipErrorInvalidValue;  // narrow words hold no index
         if (emit) {
-            // production frames: one wave per band, four bands per workgroup; the bands above 512
-            // words (at most N / 513 of them) in a second launch
+            // production frames (after launch_band_classify): one wave per band, four bands per
+            // workgroup; the bands above 512 words (at most N / 513 of them) in later launches
             const uint32_t* w = reinterpret_cast<const uint32_t*>(src);
             int32_t* lists = big_list + 2 * max_bands;  // 2 * max_bands slots: both lists of long bands
             const int64_t cap = 2 * max_bands;
-            k_band_classify2<<<(unsigned)((max_bands + kClassifyThreads - 1) / kClassifyThreads), kClassifyThreads, 0,
-                               s>>>(band_start, n_bands, lists, cap, ticket + 1);
             k_sort_bands_wave<0><<<(unsigned)((max_bands + 3) / 4), 256, 0, s>>>(
                 band_start, n_bands, w, bpre, band_pr, big_list, big_excl, big_count, lists, cap, ticket + 1, *emit);
             const int64_t max_med = std::min<int64_t>(max_bands, n_entries / (kSmallBandMax + 1) + 1);
@@ -921,6 +922,16 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
     if (emit) return hipErrorInvalidValue;  // wide words: the debug path keeps the separate mark + emit
     return launch_classes<Wide>(band_start, n_bands, max_bands, src, out, bpre, ticket, big_list, big_excl,
                                 big_count, n_cu, s, nullptr, nullptr, side);
+}
+
+hipError_t launch_band_classify(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
+                                const int32_t* bpre, int32_t* big_list, int32_t* big_excl, int32_t* big_count,
+                                uint32_t* ticket, hipStream_t s) {
+    if (max_bands <= 0) return hipSuccess;
+    int32_t* lists = big_list + 2 * max_bands;
+    k_band_classify2<<<(unsigned)((max_bands + kClassifyThreads - 1) / kClassifyThreads), kClassifyThreads, 0, s>>>(
+        band_start, n_bands, lists, 2 * max_bands, ticket + 1, bpre, big_list, big_excl, big_count);
+    return hipGetLastError();
 }
 
 hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,

@@ -228,7 +228,10 @@ struct EmitArgs {
 // zeroed u32 counters.  narrow (band_pr non-null): 32-bit words from the narrow scatter; sidx
 // must be null.
 // emit (narrow only, or null): mark + emit fused (EmitArgs); the bands then write no keys or
-// winding numbers, except the oversized ones, which launch_emit_big_bands reads back.
+// winding numbers, except the oversized ones, which launch_emit_big_bands reads back.  With
+// emit, launch_band_classify must run first: it lists the long bands for the wave sorts and
+// queues the oversized ones, so that launch_sort_big_bands and launch_emit_big_bands may run on
+// another stream beside the wave sorts.
 // side (optional): a second stream and two events; the short bands are then sorted there,
 // beside the long ones, and s waits for them before it returns.  n_entries: an upper bound of the
 // sorted entries (sizes the launch for the bands above 512 words of frames with emit).
@@ -242,6 +245,12 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
                              uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr = nullptr,
                              const EmitArgs* emit = nullptr, SideStream side = {nullptr, nullptr, nullptr},
                              int64_t n_entries = 0);
+// Lists the bands of 513..1024 and 1025..2048 words (ticket[1..2] count them) and queues the
+// longer ones with their exclusive prefixes (big_count zeroed): the first launch of a sort with
+// emit.
+hipError_t launch_band_classify(const int32_t* band_start, const int32_t* n_bands, int64_t max_bands,
+                                const int32_t* bpre, int32_t* big_list, int32_t* big_excl, int32_t* big_count,
+                                uint32_t* ticket, hipStream_t s);
 // Mark + emit of the oversized bands from their sorted keys and winding numbers.
 hipError_t launch_emit_big_bands(const int32_t* band_start, const int32_t* big_list, const int32_t* big_count,
                                  const int32_t* skey, const int32_t* winding, const int2* band_pr,

@@ -307,6 +307,14 @@ spr::SideStream side_for(spr_handle* H, int64_t n_entries, uint32_t Np) {
     return {H->stream2, H->ev_sfork, H->ev_sjoin};
 }
 
+// The stream of the oversized bands' sort and emit in frames with emit: the second stream, beside
+// the wave sorts, in frames large enough to have such bands (paths averaging 2048+ entries, 8 M
+// entries in all); otherwise the main stream, as the fork and join would cost more than they hide.
+hipStream_t big_band_stream(const spr_handle* H, int64_t n_entries, uint32_t Np) {
+    if (serial(H) || Np == 0 || n_entries < ((int64_t)1 << 23) || n_entries / (int64_t)Np < 2048) return H->stream;
+    return H->stream2;
+}
+
 spr::ScanCtl ctl_at(spr_handle* H, size_t st, size_t ctr_word, int counter) {
     spr::ScanCtl c;
     c.status = H->zero.p + st;
@@ -536,19 +544,37 @@ void run_frame(spr_handle* H, bool record_begin = true) {
     if (N > 0) {
         // band sort fused with the winding numbers (global exclusive scan of the sorted winding
         // changes, scanline_rasterizer.cpp:478-506)
-        SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
-                                       keep ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
-                                       H->big_list.p + max_bands, d_bigcount, d_bsum,
-                                       ctl_at(H, z.st_wind, z.ctr, 4).counter, H->n_cu, s,
-                                       narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr,
-                                       side_for(H, N, Np), N));
-        SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
-                                           H->sbig.p, H->stmp.p, H->skey.p, keep ? H->sidx.p : nullptr,
-                                           H->winding.p, s, narrow ? H->band_pr.p : nullptr));
+        uint32_t* ticket = ctl_at(H, z.st_wind, z.ctr, 4).counter;
         if (narrow) {
+            // the oversized bands (queued by the classify pass) are sorted and emitted on the second
+            // stream beside the wave sorts
+            hipStream_t s2 = big_band_stream(H, N, Np);
+            SPR_HIP(spr::launch_band_classify(H->band_start.p, H->bflag.p + NB, max_bands, d_bsum, H->big_list.p,
+                                              H->big_list.p + max_bands, d_bigcount, ticket, s));
+            if (s2 != s) {
+                SPR_HIP(hipEventRecord(H->ev_sfork, s));
+                SPR_HIP(hipStreamWaitEvent(s2, H->ev_sfork, 0));
+            }
+            SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
+                                               H->sbig.p, H->stmp.p, H->skey.p, nullptr, H->winding.p, s2,
+                                               H->band_pr.p));
             SPR_HIP(spr::launch_emit_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->skey.p, H->winding.p,
-                                               H->band_pr.p, E, s));
+                                               H->band_pr.p, E, s2));
+            if (s2 != s) SPR_HIP(hipEventRecord(H->ev_sjoin, s2));
+            SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
+                                           nullptr, H->winding.p, H->big_list.p, H->big_list.p + max_bands,
+                                           d_bigcount, d_bsum, ticket, H->n_cu, s, H->band_pr.p, &E,
+                                           side_for(H, N, Np), N));
+            if (s2 != s) SPR_HIP(hipStreamWaitEvent(s, H->ev_sjoin, 0));
             SPR_HIP(spr::launch_band_totals(H->band_cnt.p, H->bflag.p + NB, d_totals, s));
+        } else {
+            SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
+                                           keep ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
+                                           H->big_list.p + max_bands, d_bigcount, d_bsum, ticket, H->n_cu, s,
+                                           nullptr, nullptr, side_for(H, N, Np), N));
+            SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
+                                               H->sbig.p, H->stmp.p, H->skey.p, keep ? H->sidx.p : nullptr,
+                                               H->winding.p, s, nullptr));
         }
     }
     stage_mark(H, 9);
@@ -878,18 +904,35 @@ void run_learned_frame(spr_handle* H) {
     const spr::EmitArgs E{H->crec.p, H->brec_l.p, H->band_cnt.p, H->band_path.p, H->bucket_base.p, H->rule.p,
                           pcv, H->batch ? H->wbase.p : nullptr, w, h};
     uint32_t* ticket = ctl_at(H, z.st_wind, z.counters, kLcSortTicket).counter;
-    SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
-                                   debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
-                                   H->big_list.p + max_bands, d_bigcount, H->bsum_l.p, ticket, H->n_cu, s,
-                                   narrow ? H->band_pr.p : nullptr, narrow ? &E : nullptr,
-                                   side_for(H, nl, Np), nl));
-    SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
-                                       H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr, H->winding.p,
-                                       s, narrow ? H->band_pr.p : nullptr));
     if (narrow) {
+        // the oversized bands (queued by the classify pass) are sorted and emitted on the second
+        // stream beside the wave sorts
+        hipStream_t sb = big_band_stream(H, nl, Np);
+        SPR_HIP(spr::launch_band_classify(H->band_start.p, d_nbands, max_bands, H->bsum_l.p, H->big_list.p,
+                                          H->big_list.p + max_bands, d_bigcount, ticket, s));
+        if (sb != s) {
+            SPR_HIP(hipEventRecord(H->ev_sfork, s));
+            SPR_HIP(hipStreamWaitEvent(sb, H->ev_sfork, 0));
+        }
+        SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
+                                           H->sbig.p, H->stmp.p, H->skey.p, nullptr, H->winding.p, sb,
+                                           H->band_pr.p));
         SPR_HIP(spr::launch_emit_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->skey.p, H->winding.p,
-                                           H->band_pr.p, E, s));
+                                           H->band_pr.p, E, sb));
+        if (sb != s) SPR_HIP(hipEventRecord(H->ev_sjoin, sb));
+        SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p, nullptr,
+                                       H->winding.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
+                                       H->bsum_l.p, ticket, H->n_cu, s, H->band_pr.p, &E, side_for(H, nl, Np), nl));
+        if (sb != s) SPR_HIP(hipStreamWaitEvent(s, H->ev_sjoin, 0));
         SPR_HIP(spr::launch_band_totals(H->band_cnt.p, d_nbands, d_totals, s));
+    } else {
+        SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
+                                       debug ? H->sidx.p : nullptr, H->winding.p, H->big_list.p,
+                                       H->big_list.p + max_bands, d_bigcount, H->bsum_l.p, ticket, H->n_cu, s,
+                                       nullptr, nullptr, side_for(H, nl, Np), nl));
+        SPR_HIP(spr::launch_sort_big_bands(H->band_start.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
+                                           H->sbig.p, H->stmp.p, H->skey.p, debug ? H->sidx.p : nullptr,
+                                           H->winding.p, s, nullptr));
     }
     stage_mark(H, 9);
     if (H->batch && !narrow)
