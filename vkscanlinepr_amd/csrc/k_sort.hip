@@ -895,7 +895,7 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
                              const unsigned long long* src, int32_t* skey, int32_t* sidx, int32_t* winding,
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
                              uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr,
-                             const EmitArgs* emit, SideStream side, int64_t n_entries) {
+                             const EmitArgs* emit, SideStream side, int64_t n_entries, hipStream_t s_small) {
     if (max_bands <= 0) return hipSuccess;
     const Unpacked out{skey, sidx, winding};
     if (band_pr) {
@@ -906,7 +906,7 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
             const uint32_t* w = reinterpret_cast<const uint32_t*>(src);
             int32_t* lists = big_list + 2 * max_bands;  // 2 * max_bands slots: both lists of long bands
             const int64_t cap = 2 * max_bands;
-            k_sort_bands_wave<0><<<(unsigned)((max_bands + 3) / 4), 256, 0, s>>>(
+            k_sort_bands_wave<0><<<(unsigned)((max_bands + 3) / 4), 256, 0, s_small ? s_small : s>>>(
                 band_start, n_bands, w, bpre, band_pr, big_list, big_excl, big_count, lists, cap, ticket + 1, *emit);
             const int64_t max_med = std::min<int64_t>(max_bands, n_entries / (kSmallBandMax + 1) + 1);
             k_sort_bands_wave<1><<<(unsigned)((max_med + 3) / 4), 256, 0, s>>>(

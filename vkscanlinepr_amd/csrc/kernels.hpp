@@ -234,7 +234,9 @@ struct EmitArgs {
 // another stream beside the wave sorts.
 // side (optional): a second stream and two events; the short bands are then sorted there,
 // beside the long ones, and s waits for them before it returns.  n_entries: an upper bound of the
-// sorted entries (sizes the launch for the bands above 512 words of frames with emit).
+// sorted entries (sizes the launch for the bands above 512 words of frames with emit).  s_small
+// (emit only, optional): the stream of the sort of the bands of up to 512 words, which the caller
+// forks from s after launch_band_classify and joins.
 struct SideStream {
     hipStream_t s2;
     hipEvent_t fork, join;
@@ -244,7 +246,7 @@ hipError_t launch_sort_bands(const int32_t* band_start, const int32_t* n_bands, 
                              int32_t* big_list, int32_t* big_excl, int32_t* big_count, const int32_t* bpre,
                              uint32_t* ticket, int n_cu, hipStream_t s, const int2* band_pr = nullptr,
                              const EmitArgs* emit = nullptr, SideStream side = {nullptr, nullptr, nullptr},
-                             int64_t n_entries = 0);
+                             int64_t n_entries = 0, hipStream_t s_small = nullptr);
 // Lists the bands of 513..1024 and 1025..2048 words (ticket[1..2] count them) and queues the
 // longer ones with their exclusive prefixes (big_count zeroed): the first launch of a sort with
 // emit.

@@ -547,7 +547,7 @@ void run_frame(spr_handle* H, bool record_begin = true) {
         uint32_t* ticket = ctl_at(H, z.st_wind, z.ctr, 4).counter;
         if (narrow) {
             // the oversized bands (queued by the classify pass) are sorted and emitted on the second
-            // stream beside the wave sorts
+            // stream, followed there by the bands of up to 512 words, beside the longer bands
             hipStream_t s2 = big_band_stream(H, N, Np);
             SPR_HIP(spr::launch_band_classify(H->band_start.p, H->bflag.p + NB, max_bands, d_bsum, H->big_list.p,
                                               H->big_list.p + max_bands, d_bigcount, ticket, s));
@@ -560,12 +560,14 @@ void run_frame(spr_handle* H, bool record_begin = true) {
                                                H->band_pr.p));
             SPR_HIP(spr::launch_emit_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->skey.p, H->winding.p,
                                                H->band_pr.p, E, s2));
-            if (s2 != s) SPR_HIP(hipEventRecord(H->ev_sjoin, s2));
             SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
                                            nullptr, H->winding.p, H->big_list.p, H->big_list.p + max_bands,
                                            d_bigcount, d_bsum, ticket, H->n_cu, s, H->band_pr.p, &E,
-                                           side_for(H, N, Np), N));
-            if (s2 != s) SPR_HIP(hipStreamWaitEvent(s, H->ev_sjoin, 0));
+                                           side_for(H, N, Np), N, s2));
+            if (s2 != s) {
+                SPR_HIP(hipEventRecord(H->ev_sjoin, s2));  // after the short bands' sort on s2
+                SPR_HIP(hipStreamWaitEvent(s, H->ev_sjoin, 0));
+            }
             SPR_HIP(spr::launch_band_totals(H->band_cnt.p, H->bflag.p + NB, d_totals, s));
         } else {
             SPR_HIP(spr::launch_sort_bands(H->band_start.p, H->bflag.p + NB, max_bands, H->sbig.p, H->skey.p,
@@ -906,7 +908,7 @@ void run_learned_frame(spr_handle* H) {
     uint32_t* ticket = ctl_at(H, z.st_wind, z.counters, kLcSortTicket).counter;
     if (narrow) {
         // the oversized bands (queued by the classify pass) are sorted and emitted on the second
-        // stream beside the wave sorts
+        // stream, followed there by the bands of up to 512 words, beside the longer bands
         hipStream_t sb = big_band_stream(H, nl, Np);
         SPR_HIP(spr::launch_band_classify(H->band_start.p, d_nbands, max_bands, H->bsum_l.p, H->big_list.p,
                                           H->big_list.p + max_bands, d_bigcount, ticket, s));
@@ -919,11 +921,14 @@ void run_learned_frame(spr_handle* H) {
                                            H->band_pr.p));
         SPR_HIP(spr::launch_emit_big_bands(H->band_start.p, H->big_list.p, d_bigcount, H->skey.p, H->winding.p,
                                            H->band_pr.p, E, sb));
-        if (sb != s) SPR_HIP(hipEventRecord(H->ev_sjoin, sb));
         SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p, nullptr,
                                        H->winding.p, H->big_list.p, H->big_list.p + max_bands, d_bigcount,
-                                       H->bsum_l.p, ticket, H->n_cu, s, H->band_pr.p, &E, side_for(H, nl, Np), nl));
-        if (sb != s) SPR_HIP(hipStreamWaitEvent(s, H->ev_sjoin, 0));
+                                       H->bsum_l.p, ticket, H->n_cu, s, H->band_pr.p, &E, side_for(H, nl, Np), nl,
+                                       sb));
+        if (sb != s) {
+            SPR_HIP(hipEventRecord(H->ev_sjoin, sb));  // after the short bands' sort on sb
+            SPR_HIP(hipStreamWaitEvent(s, H->ev_sjoin, 0));
+        }
         SPR_HIP(spr::launch_band_totals(H->band_cnt.p, d_nbands, d_totals, s));
     } else {
         SPR_HIP(spr::launch_sort_bands(H->band_start.p, d_nbands, max_bands, H->sbig.p, H->skey.p,
