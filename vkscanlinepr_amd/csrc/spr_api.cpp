@@ -674,12 +674,16 @@ void grow_budget(spr_handle* H, int64_t need, int64_t predicted) {
 
 void learned_finish(spr_handle* H);
 
-// Whether the next learned frame runs the prediction (after a load, a model or margin change, or a
-// new MVP; always without a model, where only the features run), published in the pinned words
-// that the frame copies to the device with the MVP.  Every earlier frame has ended (frames end with
-// a stream synchronisation), so no copy is reading them.
-bool prediction_due(const spr_handle* H) { return !H->model.ready || H->need_prediction || H->pred_stale; }
+// The prediction (features, MLP, capacities) is due after a load, a model or margin change or a
+// new MVP.  A batch without a model runs only the features, for the proven bound that sizes its
+// first budget: later frames of the same batch and view need none of it.
+bool prediction_due(const spr_handle* H) {
+    return (!H->model.ready && !H->batch) || H->need_prediction || H->pred_stale;
+}
 
+// Whether the next learned frame runs the prediction, published in the pinned words that the
+// frame copies to the device with the MVP.  Every earlier frame has ended (frames end with a stream
+// synchronisation), so no copy is reading them.
 bool publish_prediction(spr_handle* H) {
     const bool predict = H->capturing ? H->cap_predict : prediction_due(H);
     H->pred_pinned()[0] = predict ? 0 : 1;
