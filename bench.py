@@ -116,7 +116,8 @@ def cubic_crossings(counts, cuts, curve_type):
 
 
 def rate_stages(stages, algo, io, traffic):
-    """Per stage: device time, the SURVEY-rule algorithmic rate (effective for fused kernels), the
+    """Per stage: device time, the algorithmic rate (every reference stage a kernel replaces read and
+    written once in the reference's types; effective for fused kernels), the
     rate on the kernels' own I/O and the measured HBM traffic rate, each as a fraction of the
     8 TB/s peak.  Times are those of a frame with every kernel on one stream."""
     table = {}
@@ -422,7 +423,7 @@ def main(argv=None):
             "peak_no_fma_no_packed": VALU_NOFMA_PEAK_TOPS, "frac_no_fma_no_packed": tflops / VALU_NOFMA_PEAK_TOPS,
             "traffic": traffic.get("chain_solve"), "algorithmic_flop": SOLVE_FLOP_PER_CUBIC_CROSSING * xc,
             "cubic_crossings": xc, "kernel_ms": solve_ms}
-    # the longest HBM-bound stage, rated on its own I/O; the SURVEY-rule figure (every reference
+    # the longest HBM-bound stage, rated on its own I/O; the algorithmic figure (every reference
     # stage a fused kernel replaces, buffers it never touches included) is labelled apart
     hb = max((k for k in table if k in io and k != "chain_solve"), key=lambda k: table[k]["ms"])
     hb_ms = table[hb]["ms"]
