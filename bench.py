@@ -6,7 +6,7 @@ region (image readback excluded).  Default workload: the illustration scene (5,0
 4-400 cubic segments, log-normal sizes, per-path affine, mixed fill rules, alpha 0.3-1.0,
 src-over composite into 4096x4096 RGBA8), synthetic from a seeded generator.
 
-Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: this build runs on one GPU.
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: the library runs on one GPU.
 """
 from __future__ import annotations
 

@@ -31,7 +31,7 @@ typedef enum spr_status {
     SPR_ERR_DEVICE = -2,   /* HIP runtime error */
     SPR_ERR_STATE = -3,    /* call out of order (e.g. render before load) */
     SPR_ERR_IO = -4,       /* file could not be read or parsed */
-    SPR_ERR_CAPACITY = -5  /* a size exceeds a fixed limit of this build */
+    SPR_ERR_CAPACITY = -5  /* a size exceeds a fixed limit of the library */
 } spr_status;
 
 /* Curve types and fill rules, src/core/vg/vg_container.h:9-19. */
