@@ -55,10 +55,17 @@ def stage_bytes(P, C, Np, N, R, w, h):
     fused with the keys (the rest: 72 B/curve + 28 B/intersection); sort 16 B/entry + the
     winding gather 12 B/entry + the winding scan 8 B/entry (bucket scatter and band sort
     together); mark 20 B/entry + flag scan 2 x 8 B/entry + compaction 24 B/entry + 16 B/record
-    (one fused kernel); composite 16 B/record + 4 B/pixel."""
+    (one fused kernel); composite 16 B/record + 4 B/pixel.  The scans before the host sync count
+    the reference's scan of the per-curve counts (4 B in, 4 B out per curve).  The chain prep and
+    order, which have no reference stage, count their per-curve I/O: the curve data the count pass
+    reads (72 B) and its offset (4 B) in, five 24 B piece descriptors and ten class bytes out, the
+    class bytes read twice by the order (226 B/curve; the 32 B descriptor and 4 B order slot of
+    each non-empty chain are left out, so the rate is a lower bound)."""
     return {
         "transform": 20 * P + 4 * Np,
         "count": 72 * C,
+        "scans+sync": 8 * (C + 1),
+        "chain_prep": 226 * C,
         "chain_solve": 48 * C + 4 * N,
         "piece_fragments": 72 * C + 28 * N,
         "sort": 36 * N,
@@ -82,10 +89,13 @@ def stage_io_bytes(P, C, Np, N, R, w, h):
     (no bytes for the intermediate buffers a fused kernel never materialises): piece merge with
     keys 72 B/curve + t in, key, winding change and path out (16 B/intersection); sort with the
     winding gather and scan: key, index, winding change in and key, index, winding number out
-    (24 B/entry); mark and emit: key, winding number, path in (12 B/entry) and 16 B/record out."""
+    (24 B/entry); mark and emit: key, winding number, path in (12 B/entry) and 16 B/record out;
+    the scans before the host sync: per-curve counts and per-path bucket counts, 8 B each."""
     return {
         "transform": 20 * P + 4 * Np,
         "count": 72 * C,
+        "scans+sync": 8 * (C + 1) + 8 * (Np + 1),
+        "chain_prep": 226 * C,
         "chain_solve": 48 * C + 4 * N,
         "piece_fragments": 72 * C + 16 * N,
         "sort": 24 * N,
@@ -293,6 +303,9 @@ def stage_table(r, steps, sizes, workload, mode):
     # the reference's sort (with its winding gather and scan) is the bucket scatter, the band sort
     # and the winding scan folded into it
     stages["sort"] = stages.pop("bucket_scatter") + stages.pop("band_sort") + stages.pop("winding_scan")
+    # the separate fragment-key kernel runs only on injected intersections: generated ones get
+    # their keys inside the piece kernels, so this row would only time two adjacent events
+    stages.pop("gen_fragment")
     algo = stage_bytes(P, C, Np, N, R, w, h)
     io = stage_io_bytes(P, C, Np, N, R, w, h)
     if FUSED_MARK_EMIT:

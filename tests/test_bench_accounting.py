@@ -17,6 +17,11 @@ def test_stage_bytes_follow_the_reference_types():
     assert algo["sort"] == (16 + 12 + 8) * N          # sort, winding gather, winding scan
     assert algo["mark_emit"] == (20 + 16 + 24) * N + 16 * R
     assert algo["composite"] == 16 * R + 4 * w * h
+    assert algo["scans+sync"] == 8 * (C + 1)            # the scan of the per-curve counts
+    assert algo["chain_prep"] == (72 + 4 + 5 * 24 + 10 + 2 * 10) * C
+    io = bench.stage_io_bytes(P, C, Np, N, R, w, h)
+    assert io["scans+sync"] == 8 * (C + 1) + 8 * (Np + 1)
+    assert set(algo) == set(io)                         # every stage rated both ways
 
 
 def test_fused_mark_emit_is_one_sort_stage():
