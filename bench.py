@@ -291,7 +291,6 @@ def time_frames(r, steps, warmup):
 def stage_table(r, steps, sizes, workload, mode):
     """Per-stage device times of frames that run every kernel on the main stream (HIP events
     between the stages), rated by algorithmic bytes, own I/O and measured traffic."""
-    P, C, Np, N, R, w, h = sizes
     r.set_stage_timing(True)
     acc = np.zeros(16)
     for _ in range(steps):
@@ -299,7 +298,15 @@ def stage_table(r, steps, sizes, workload, mode):
         acc += np.array(r.stats().stage_ms[:16])
     serial_ms = float(r.stats().frame_ms)
     r.set_stage_timing(False)
-    stages = dict(zip(STAGE_NAMES, acc[: len(STAGE_NAMES)] / steps))
+    stages, algo, io = fold_stages(acc[: len(STAGE_NAMES)] / steps, sizes)
+    return rate_stages(stages, algo, io, measured_traffic(workload, mode)), serial_ms, algo, io
+
+
+def fold_stages(stage_ms, sizes):
+    """The per-event stage times (STAGE_NAMES order) folded into the rows of the table, with the
+    algorithmic and own-I/O bytes of each row."""
+    P, C, Np, N, R, w, h = sizes
+    stages = dict(zip(STAGE_NAMES, (float(x) for x in stage_ms)))
     # the reference's sort (with its winding gather and scan) is the bucket scatter, the band sort
     # and the winding scan folded into it
     stages["sort"] = stages.pop("bucket_scatter") + stages.pop("band_sort") + stages.pop("winding_scan")
@@ -311,7 +318,7 @@ def stage_table(r, steps, sizes, workload, mode):
     if FUSED_MARK_EMIT:
         fold_mark_emit(stages, algo, io, N, R)
         stages.pop("mark_emit")
-    return rate_stages(stages, algo, io, measured_traffic(workload, mode)), serial_ms, algo, io
+    return stages, algo, io
 
 
 def bench_scan(a) -> dict:

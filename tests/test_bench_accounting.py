@@ -34,3 +34,18 @@ def test_fused_mark_emit_is_one_sort_stage():
     assert "mark_emit" not in algo and "mark_emit" not in io
     assert algo["sort"] == 96 * N + 16 * R                # every reference stage the kernel replaces
     assert io["sort"] == 12 * N + 16 * R
+
+
+def test_every_table_row_has_algorithmic_and_own_bytes():
+    """The table bench.py prints: every row that a production frame times is rated on its
+    algorithmic bytes and on its own I/O (no null rates), and the stage times add up."""
+    sizes = (1_000, 250, 40, 20_000, 15_000, 256, 256)
+    ms = [0.01 * (k + 1) for k in range(len(bench.STAGE_NAMES))]
+    stages, algo, io = bench.fold_stages(ms, sizes)
+    assert set(stages) == set(algo) == set(io)
+    assert "gen_fragment" not in stages and "bucket_scatter" not in stages
+    dropped = bench.STAGE_NAMES.index("gen_fragment")
+    assert sum(stages.values()) == pytest.approx(sum(ms) - ms[dropped])
+    table = bench.rate_stages(stages, algo, io, None)
+    for name, row in table.items():
+        assert row["algo_GB/s"] is not None and row["io_GB/s"] is not None, name
